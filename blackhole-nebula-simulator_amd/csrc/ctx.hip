@@ -1,0 +1,67 @@
+// Context lifetime, error strings and the scratch arena of the C ABI.
+#include "nb_common.h"
+#include "nb_ctx.h"
+#include "nebula_hip.h"
+
+#include <new>
+
+int32_t ctx_fail_code(nb_ctx* ctx, int32_t code, const char* msg)
+{
+    if (ctx) ctx->err = msg ? msg : "unknown error";
+    return code;
+}
+
+int32_t ctx_fail(nb_ctx* ctx, const char* msg) { return ctx_fail_code(ctx, NB_ERR_ARG, msg); }
+
+int ctx_reserve(nb_ctx* ctx, void** p, size_t* have, size_t bytes)
+{
+    if (bytes == 0) bytes = 1;
+    if (*p && *have >= bytes) return 0;
+    if (*p) {
+        hipFree(*p);
+        *p = nullptr;
+        *have = 0;
+    }
+    size_t want = bytes + bytes / 8;  // headroom: sizes drift from step to step
+    if (hipMalloc(p, want) != hipSuccess) {
+        *p = nullptr;
+        ctx_fail_code(ctx, NB_ERR_HIP, "hipMalloc of library scratch failed");
+        return 1;
+    }
+    *have = want;
+    return 0;
+}
+
+extern "C" nb_ctx* nb_ctx_create(int32_t device)
+{
+    if (hipSetDevice(device) != hipSuccess) return nullptr;
+    nb_ctx* ctx = new (std::nothrow) nb_ctx();
+    if (ctx) ctx->device = device;
+    return ctx;
+}
+
+extern "C" void nb_ctx_destroy(nb_ctx* ctx)
+{
+    if (!ctx) return;
+    void* bufs[] = {ctx->cell_id, ctx->keys_out, ctx->iota, ctx->order, ctx->cell_count, ctx->cell_start,
+                    ctx->gbuf, ctx->rowcnt, ctx->ecount, ctx->eoff, ctx->efill, ctx->ebuf, ctx->tot,
+                    ctx->offs, ctx->tcount, ctx->dscratch, ctx->fscratch, ctx->red, ctx->scal, ctx->cub_tmp};
+    for (void* b : bufs)
+        if (b) hipFree(b);
+    delete ctx;
+}
+
+extern "C" const char* nb_last_error(const nb_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+extern "C" int32_t nb_abi_version(void) { return 1; }
+
+extern "C" int64_t nb_ctx_scratch_bytes(const nb_ctx* ctx)
+{
+    if (!ctx) return 0;
+    size_t s = ctx->cell_id_bytes + ctx->keys_out_bytes + ctx->iota_bytes + ctx->order_bytes +
+               ctx->cell_count_bytes + ctx->cell_start_bytes + ctx->gbuf_bytes + ctx->rowcnt_bytes +
+               ctx->ecount_bytes + ctx->eoff_bytes + ctx->efill_bytes + ctx->ebuf_bytes + ctx->tot_bytes +
+               ctx->offs_bytes + ctx->tcount_bytes + ctx->dscratch_bytes + ctx->fscratch_bytes +
+               ctx->red_bytes + ctx->scal_bytes + ctx->cub_tmp_bytes;
+    return (int64_t)s;
+}

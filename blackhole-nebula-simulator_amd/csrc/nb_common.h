@@ -1,0 +1,84 @@
+// Shared device helpers for the SPH kernels (gfx950).
+//
+// FP64 arithmetic in this library is written so that each product and sum is rounded on
+// its own (the library is compiled with -ffp-contract=off): products are evaluated left to
+// right and integer powers as x**3 = x*(x*x), x**4 = (x*x)*(x*x), x**5 = x*((x*x)*(x*x)),
+// which is the order the reference's numba kernels use. Division and square root are the
+// IEEE correctly rounded operations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define NB_R_KERN 2.0
+
+namespace nb {
+
+__device__ __forceinline__ double ipow3(double x) { return x * (x * x); }
+__device__ __forceinline__ double ipow4(double x) { double x2 = x * x; return x2 * x2; }
+__device__ __forceinline__ double ipow5(double x) { double x2 = x * x; return x * (x2 * x2); }
+
+// Python's builtin min/max of two floats (second argument only if strictly smaller/larger)
+__device__ __forceinline__ double pymin(double a, double b) { return b < a ? b : a; }
+__device__ __forceinline__ double pymax(double a, double b) { return b > a ? b : a; }
+
+// kernels_nb.py:16-28 (kid 0 cubic spline, kid 1 Wendland C4)
+__device__ __forceinline__ double kernel_f(double q, int kid)
+{
+    if (q >= 2.0) return 0.0;
+    if (kid == 0) {
+        if (q < 1.0) return 1.0 - 1.5 * q * q + 0.75 * q * q * q;
+        double t = 2.0 - q;
+        return 0.25 * t * t * t;
+    }
+    double x = 1.0 - 0.5 * q;
+    double x2 = x * x;
+    return x2 * x2 * x2 * (35.0 / 12.0 * q * q + 3.0 * q + 1.0);
+}
+
+// kernels_nb.py:31-43
+__device__ __forceinline__ double kernel_df(double q, int kid)
+{
+    if (q >= 2.0) return 0.0;
+    if (kid == 0) {
+        if (q < 1.0) return -3.0 * q + 2.25 * q * q;
+        double t = 2.0 - q;
+        return -0.75 * t * t;
+    }
+    double x = 1.0 - 0.5 * q;
+    double x2 = x * x;
+    return -7.0 / 3.0 * q * x2 * x2 * x * (5.0 * q + 2.0);
+}
+
+// sph.py:519-522: dW/dr = C / h^4 f'(r/h)
+__device__ __forceinline__ double dwdr(double r, double h, int kid, double cnorm)
+{
+    return cnorm / ipow4(h) * kernel_df(r / h, kid);
+}
+
+// neighbours.py:39-43: minimum image of one separation component (box_c <= 0: open)
+__device__ __forceinline__ double min_image(double d, double box_c)
+{
+    if (box_c > 0.0) d -= box_c * rint(d / box_c);
+    return d;
+}
+
+struct Box {
+    double l[3];  // periodic lengths, 0 = open axis
+};
+
+// Particle rows of a pass: rows[q] if a row list is given, else q.
+__device__ __forceinline__ int64_t row_of(const int64_t* rows, int64_t q)
+{
+    return rows ? rows[q] : q;
+}
+
+}  // namespace nb
+
+// status codes of the C ABI (include/nebula_hip.h)
+#define NB_OK 0
+#define NB_RETRY_CAPACITY 1
+#define NB_ERR_ARG -1
+#define NB_ERR_HIP -2
+#define NB_ERR_BOX -3
+#define NB_ERR_NOT_BUILT -4

@@ -1,0 +1,50 @@
+// Library context: the workspace arena behind the C ABI (include/nebula_hip.h).
+//
+// Device memory for inputs and outputs belongs to the caller; the context owns only
+// scratch space (grid, padded gather rows, scan temporaries, reduction partials), which
+// grows on demand and is reused between calls. One context per stream of work: calls that
+// share a context must be issued in order (they share the scratch).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+struct nb_ctx {
+    int device = 0;
+    std::string err;
+    // neighbour grid
+    uint32_t* cell_id = nullptr; size_t cell_id_bytes = 0;
+    uint32_t* keys_out = nullptr; size_t keys_out_bytes = 0;
+    uint32_t* iota = nullptr; size_t iota_bytes = 0;
+    uint32_t* order = nullptr; size_t order_bytes = 0;
+    uint32_t* cell_count = nullptr; size_t cell_count_bytes = 0;
+    uint32_t* cell_start = nullptr; size_t cell_start_bytes = 0;
+    // gather rows and the transposed extras
+    uint32_t* gbuf = nullptr; size_t gbuf_bytes = 0;
+    int32_t* rowcnt = nullptr; size_t rowcnt_bytes = 0;
+    unsigned long long* ecount = nullptr; size_t ecount_bytes = 0;
+    unsigned long long* eoff = nullptr; size_t eoff_bytes = 0;
+    unsigned int* efill = nullptr; size_t efill_bytes = 0;
+    uint32_t* ebuf = nullptr; size_t ebuf_bytes = 0;
+    unsigned long long* tot = nullptr; size_t tot_bytes = 0;
+    unsigned long long* offs = nullptr; size_t offs_bytes = 0;
+    unsigned long long* tcount = nullptr; size_t tcount_bytes = 0;
+    // density distance cache (one double per search-list entry)
+    double* dscratch = nullptr; size_t dscratch_bytes = 0;
+    float* fscratch = nullptr; size_t fscratch_bytes = 0;
+    // reductions
+    double* red = nullptr; size_t red_bytes = 0;
+    int64_t* scal = nullptr; size_t scal_bytes = 0;
+    void* cub_tmp = nullptr; size_t cub_tmp_bytes = 0;
+    int32_t row_cap = 0;
+    int64_t lists_n = 0, lists_total = 0;
+    int lists_ready = 0;
+    int64_t last_grid_cells = 0;
+    double last_cell = 0.0;
+};
+
+int32_t ctx_fail(nb_ctx* ctx, const char* msg);
+int32_t ctx_fail_code(nb_ctx* ctx, int32_t code, const char* msg);
+// grow *p to at least `bytes` (contents are not kept); nonzero on failure
+int ctx_reserve(nb_ctx* ctx, void** p, size_t* have, size_t bytes);

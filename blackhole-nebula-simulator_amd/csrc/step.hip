@@ -1,0 +1,391 @@
+// Per-particle work of the step: GN generalized leapfrog (integrator.py:106-198), v = dH/dp
+// (fastphys.py:24-42), half kicks, equation of state, cooling, and the step's reductions
+// (time-step minima, energy / angular-momentum ledger). All FP64, one work-item per particle.
+#include "nb_common.h"
+#include "nb_ctx.h"
+#include "nebula_hip.h"
+
+namespace nb {
+
+static constexpr int kBlockS = 256;
+
+// integrator.py:106-120
+__device__ __forceinline__ void dh_dx(double x0, double x1, double x2, double p0, double p1, double p2,
+                                      double& g0, double& g1, double& g2)
+{
+    double r = sqrt(x0 * x0 + x1 * x1 + x2 * x2);
+    double c = -2.0 * (r - 2.0) / ipow4(r);
+    double da = 2.0 / (r * r);
+    double dc = (6.0 * r - 16.0) / ipow5(r);
+    double px = p0 * x0 + p1 * x1 + p2 * x2;
+    double pp = p0 * p0 + p1 * p1 + p2 * p2;
+    double radial = (0.5 * da * pp + 0.5 * dc * px * px) / r + 1.0 / ipow3(r);
+    g0 = radial * x0 + c * px * p0;
+    g1 = radial * x1 + c * px * p1;
+    g2 = radial * x2 + c * px * p2;
+}
+
+// integrator.py:123-130
+__device__ __forceinline__ void dh_dp(double x0, double x1, double x2, double p0, double p1, double p2,
+                                      double& v0, double& v1, double& v2)
+{
+    double r = sqrt(x0 * x0 + x1 * x1 + x2 * x2);
+    double a = (r - 2.0) / r;
+    double c = -2.0 * (r - 2.0) / ipow4(r);
+    double cpx = c * (p0 * x0 + p1 * x1 + p2 * x2);
+    v0 = a * p0 + cpx * x0;
+    v1 = a * p1 + cpx * x1;
+    v2 = a * p2 + cpx * x2;
+}
+
+// integrator.py:133-136
+__device__ __forceinline__ bool converged(double n0, double n1, double n2, double o0, double o1, double o2)
+{
+    double e0 = n0 - o0, e1 = n1 - o1, e2 = n2 - o2;
+    double d = e0 * e0 + e1 * e1 + e2 * e2;
+    return d <= 1e-15 * 1e-15 * (n0 * n0 + n1 * n1 + n2 * n2);
+}
+
+// integrator.py:139-185. dt_arr (per particle) or dt_scalar; iters: count, or -1 / -2 / -3
+// for stage-1 / stage-2 non-convergence / r <= 2.5 (then x, p are returned unchanged).
+__global__ void __launch_bounds__(kBlockS)
+k_glf64(const double* __restrict__ x, const double* __restrict__ p, const double* __restrict__ dt_arr,
+        double dt_scalar, int64_t n, double* __restrict__ x_out, double* __restrict__ p_out,
+        int32_t* __restrict__ iters, int32_t* __restrict__ nfail)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x0 = x[3 * i], x1 = x[3 * i + 1], x2 = x[3 * i + 2];
+    const double p0 = p[3 * i], p1 = p[3 * i + 1], p2 = p[3 * i + 2];
+    const double d = dt_arr ? dt_arr[i] : dt_scalar;
+    double o[6] = {x0, x1, x2, p0, p1, p2};
+    int32_t code;
+    if (x0 * x0 + x1 * x1 + x2 * x2 <= 2.5 * 2.5) {
+        code = -3;
+    } else {
+        const double half = 0.5 * d;
+        int32_t it = 0;
+        double h0 = p0, h1 = p1, h2 = p2;
+        bool ok = false;
+        for (int k = 0; k < 10; k++) {
+            double g0, g1, g2;
+            dh_dx(x0, x1, x2, h0, h1, h2, g0, g1, g2);
+            double n0 = p0 - half * g0, n1 = p1 - half * g1, n2 = p2 - half * g2;
+            it += 1;
+            ok = converged(n0, n1, n2, h0, h1, h2);
+            h0 = n0; h1 = n1; h2 = n2;
+            if (ok) break;
+        }
+        if (!ok) {
+            code = -1;
+        } else {
+            double v0, v1, v2;
+            dh_dp(x0, x1, x2, h0, h1, h2, v0, v1, v2);
+            double y0 = x0 + d * v0, y1 = x1 + d * v1, y2 = x2 + d * v2;
+            ok = false;
+            for (int k = 0; k < 10; k++) {
+                double w0, w1, w2;
+                dh_dp(y0, y1, y2, h0, h1, h2, w0, w1, w2);
+                double n0 = x0 + half * (v0 + w0), n1 = x1 + half * (v1 + w1), n2 = x2 + half * (v2 + w2);
+                it += 1;
+                ok = converged(n0, n1, n2, y0, y1, y2);
+                y0 = n0; y1 = n1; y2 = n2;
+                if (ok) break;
+            }
+            if (!ok) {
+                code = -2;
+            } else {
+                double g0, g1, g2;
+                dh_dx(y0, y1, y2, h0, h1, h2, g0, g1, g2);
+                o[0] = y0; o[1] = y1; o[2] = y2;
+                o[3] = h0 - half * g0; o[4] = h1 - half * g1; o[5] = h2 - half * g2;
+                code = it;
+            }
+        }
+    }
+    for (int c = 0; c < 3; c++) { x_out[3 * i + c] = o[c]; p_out[3 * i + c] = o[3 + c]; }
+    iters[i] = code;
+    if (code < 0 && nfail) atomicAdd(nfail, 1);
+}
+
+// fastphys.py:24-32
+__global__ void k_v_from_p64(const double* __restrict__ x, const double* __restrict__ p, int64_t n,
+                             double* __restrict__ out)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double x0 = x[3 * i], x1 = x[3 * i + 1], x2 = x[3 * i + 2];
+    double p0 = p[3 * i], p1 = p[3 * i + 1], p2 = p[3 * i + 2];
+    double r = sqrt(x0 * x0 + x1 * x1 + x2 * x2);
+    double a = (r - 2.0) / r;
+    double c = -2.0 * (r - 2.0) / ipow4(r);
+    double px = p0 * x0 + p1 * x1 + p2 * x2;
+    out[3 * i] = a * p0 + c * px * x0;
+    out[3 * i + 1] = a * p1 + c * px * x1;
+    out[3 * i + 2] = a * p2 + c * px * x2;
+}
+
+// out = y + s * a over m doubles (coupled.py:229-235 kicks: p + (dt/2) a; u + (dt/2) du/dt)
+__global__ void k_axpy(const double* __restrict__ y, const double* __restrict__ a, double s, int64_t m,
+                       double* __restrict__ out)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < m) out[i] = y[i] + s * a[i];
+}
+
+// eos.py:21-28 with hydro.py:130-131: P = (gamma-1) rho u, c_s = sqrt(gamma (gamma-1) u);
+// counts particles with u <= 0.
+__global__ void k_eos_ideal(const double* __restrict__ rho, const double* __restrict__ u, double gm1,
+                            double ggm1, int64_t n, double* __restrict__ pressure, double* __restrict__ cs,
+                            int32_t* __restrict__ nbad)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double ui = u[i];
+    if (ui <= 0.0) atomicAdd(nbad, 1);
+    pressure[i] = gm1 * rho[i] * ui;
+    cs[i] = sqrt(ggm1 * ui);
+}
+
+// cooling.py:33-40 with coupled.py:244-251: u_floor + (u - u_floor) exp(-dt / t_cool),
+// t_cool = beta r_mid^1.5, r_mid = (|x_old| + |x_new|)/2. e_part[i] = m (u - u_new).
+__global__ void k_cool_beta(const double* __restrict__ u, const double* __restrict__ x_old,
+                            const double* __restrict__ x_new, const double* __restrict__ mass, double u_floor,
+                            double dt, double beta, int64_t n, double* __restrict__ u_out,
+                            double* __restrict__ e_part)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double r0 = sqrt(x_old[3 * i] * x_old[3 * i] + x_old[3 * i + 1] * x_old[3 * i + 1] + x_old[3 * i + 2] * x_old[3 * i + 2]);
+    double r1 = sqrt(x_new[3 * i] * x_new[3 * i] + x_new[3 * i + 1] * x_new[3 * i + 1] + x_new[3 * i + 2] * x_new[3 * i + 2]);
+    double r_mid = 0.5 * (r0 + r1);
+    double t_cool = beta / pow(r_mid, -1.5);
+    double decay = exp(-dt / t_cool);
+    double un = u_floor + (u[i] - u_floor) * decay;
+    e_part[i] = mass[i] * (u[i] - un);
+    u_out[i] = un;
+}
+
+// Optically thin power-law cooling du/dt = -k rho sqrt(u) above u_floor, integrated exactly at
+// fixed rho: sqrt(u) falls linearly by k rho dt / 2, stopping at the floor.
+__global__ void k_cool_powerlaw(const double* __restrict__ u, const double* __restrict__ rho,
+                                const double* __restrict__ mass, double u_floor, double k, double dt,
+                                int64_t n, double* __restrict__ u_out, double* __restrict__ e_part)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double ui = u[i];
+    double un = ui;
+    if (ui > u_floor) {
+        double s = sqrt(ui) - 0.5 * k * rho[i] * dt;
+        double sf = sqrt(u_floor);
+        un = s > sf ? s * s : u_floor;
+    }
+    e_part[i] = mass[i] * (ui - un);
+    u_out[i] = un;
+}
+
+// ----------------------------------------------------------------- reductions
+// Time-step minima (coupled.py:213-226): out[0] = min h/v_sig, out[1] = min sqrt(h/|a|) over
+// |a| > 0, out[2] = min viscous dt C h^2/nu over nu > 0 (if nu given), out[3] = min |x|.
+// Minima are exact whatever the reduction order.
+__global__ void k_dt_partial(const double* __restrict__ h, const double* __restrict__ vsig,
+                             const double* __restrict__ acc, const double* __restrict__ nu, double c_visc,
+                             const double* __restrict__ x, int64_t n, double* __restrict__ part)
+{
+    double v[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double hi = h[i];
+        v[0] = fmin(v[0], hi / vsig[i]);
+        double a0 = acc[3 * i], a1 = acc[3 * i + 1], a2 = acc[3 * i + 2];
+        double am = sqrt(a0 * a0 + a1 * a1 + a2 * a2);
+        if (am > 0.0) v[1] = fmin(v[1], sqrt(hi / am));
+        if (nu && nu[i] > 0.0) v[2] = fmin(v[2], c_visc * (hi * hi) / nu[i]);
+        double x0 = x[3 * i], x1 = x[3 * i + 1], x2 = x[3 * i + 2];
+        v[3] = fmin(v[3], sqrt(x0 * x0 + x1 * x1 + x2 * x2));
+    }
+    __shared__ double sh[4][kBlockS];
+    for (int k = 0; k < 4; k++) sh[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int k = 0; k < 4; k++) sh[k][threadIdx.x] = fmin(sh[k][threadIdx.x], sh[k][threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; k++) part[4 * blockIdx.x + k] = sh[k][0];
+}
+
+// Ledger sums (coupled.py:261-270): out = [sum m (H + u), sum m |H|, sum m (x cross p)_xyz,
+// sum m (u - u_cool) extra term (e_part, may be null)]. H is the GN Hamiltonian
+// (gravity.py:92-102) or |p|^2/2 without the hole.
+__device__ __forceinline__ double gn_h(double x0, double x1, double x2, double p0, double p1, double p2)
+{
+    double r = sqrt(x0 * x0 + x1 * x1 + x2 * x2);
+    double a = (r - 2.0) / r;
+    double c = -2.0 * (r - 2.0) / ipow4(r);
+    double pp = p0 * p0 + p1 * p1 + p2 * p2;
+    double px = p0 * x0 + p1 * x1 + p2 * x2;
+    return 0.5 * a * pp + 0.5 * c * (px * px) - 1.0 / r;
+}
+
+__global__ void k_ledger_partial(const double* __restrict__ x, const double* __restrict__ p,
+                                 const double* __restrict__ u, const double* __restrict__ mass,
+                                 const double* __restrict__ e_part, int gn, int64_t n, double* __restrict__ part)
+{
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double x0 = x[3 * i], x1 = x[3 * i + 1], x2 = x[3 * i + 2];
+        double p0 = p[3 * i], p1 = p[3 * i + 1], p2 = p[3 * i + 2];
+        double hh = gn ? gn_h(x0, x1, x2, p0, p1, p2) : 0.5 * (p0 * p0 + p1 * p1 + p2 * p2);
+        double m = mass[i];
+        v[0] += m * (hh + (u ? u[i] : 0.0));
+        v[1] += m * fabs(hh);
+        v[2] += m * (x1 * p2 - x2 * p1);
+        v[3] += m * (x2 * p0 - x0 * p2);
+        v[4] += m * (x0 * p1 - x1 * p0);
+        if (e_part) v[5] += e_part[i];
+    }
+    __shared__ double sh[6][kBlockS];
+    for (int k = 0; k < 6; k++) sh[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int k = 0; k < 6; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 6; k++) part[6 * blockIdx.x + k] = sh[k][0];
+}
+
+// Final pass of a reduction over `nparts` partial rows of width w: op 0 = min, 1 = sum.
+// Sums use compensated (Kahan-Babuska) accumulation over the partials.
+__global__ void k_reduce_final(const double* __restrict__ part, int nparts, int w, int op, double* __restrict__ out)
+{
+    int k = threadIdx.x;
+    if (k >= w || blockIdx.x != 0) return;
+    if (op == 0) {
+        double v = INFINITY;
+        for (int b = 0; b < nparts; b++) v = fmin(v, part[w * b + k]);
+        out[k] = v;
+    } else {
+        double s = 0.0, c = 0.0;
+        for (int b = 0; b < nparts; b++) {
+            double y = part[w * b + k];
+            double t = s + y;
+            if (fabs(s) >= fabs(y)) c += (s - t) + y;
+            else c += (y - t) + s;
+            s = t;
+        }
+        out[k] = s + c;
+    }
+}
+
+static inline unsigned grid_s(int64_t n) { return (unsigned)((n + kBlockS - 1) / kBlockS); }
+
+}  // namespace nb
+
+using namespace nb;
+
+static int32_t launch_check_s(nb_ctx* ctx, const char* what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        std::string m = std::string(what) + ": " + hipGetErrorString(e);
+        return ctx_fail(ctx, m.c_str());
+    }
+    return NB_OK;
+}
+
+extern "C" int32_t nb_glf_step(nb_ctx* ctx, void* stream, const double* x, const double* p, const double* dt_arr,
+                               double dt_scalar, int64_t n, double* x_out, double* p_out, int32_t* iters,
+                               int32_t* nfail_dev)
+{
+    if (!ctx || n < 0) return ctx_fail(ctx, "nb_glf_step: bad arguments");
+    if (n == 0) return NB_OK;
+    hipLaunchKernelGGL(k_glf64, dim3(grid_s(n)), dim3(kBlockS), 0, (hipStream_t)stream, x, p, dt_arr, dt_scalar, n,
+                       x_out, p_out, iters, nfail_dev);
+    return launch_check_s(ctx, "nb_glf_step");
+}
+
+extern "C" int32_t nb_gn_v_from_p(nb_ctx* ctx, void* stream, const double* x, const double* p, int64_t n,
+                                  double* out)
+{
+    if (!ctx || n < 0) return ctx_fail(ctx, "nb_gn_v_from_p: bad arguments");
+    if (n == 0) return NB_OK;
+    hipLaunchKernelGGL(k_v_from_p64, dim3(grid_s(n)), dim3(kBlockS), 0, (hipStream_t)stream, x, p, n, out);
+    return launch_check_s(ctx, "nb_gn_v_from_p");
+}
+
+extern "C" int32_t nb_axpy(nb_ctx* ctx, void* stream, const double* y, const double* a, double s, int64_t m,
+                           double* out)
+{
+    if (!ctx || m < 0) return ctx_fail(ctx, "nb_axpy: bad arguments");
+    if (m == 0) return NB_OK;
+    hipLaunchKernelGGL(k_axpy, dim3(grid_s(m)), dim3(kBlockS), 0, (hipStream_t)stream, y, a, s, m, out);
+    return launch_check_s(ctx, "nb_axpy");
+}
+
+extern "C" int32_t nb_eos_ideal(nb_ctx* ctx, void* stream, const double* rho, const double* u, double gamma,
+                                int64_t n, double* pressure, double* cs, int32_t* nbad_dev)
+{
+    if (!ctx || n < 0 || !nbad_dev) return ctx_fail(ctx, "nb_eos_ideal: bad arguments");
+    if (n == 0) return NB_OK;
+    hipLaunchKernelGGL(k_eos_ideal, dim3(grid_s(n)), dim3(kBlockS), 0, (hipStream_t)stream, rho, u, gamma - 1.0,
+                       gamma * (gamma - 1.0), n, pressure, cs, nbad_dev);
+    return launch_check_s(ctx, "nb_eos_ideal");
+}
+
+extern "C" int32_t nb_cool_beta(nb_ctx* ctx, void* stream, const double* u, const double* x_old,
+                                const double* x_new, const double* mass, double u_floor, double dt, double beta,
+                                int64_t n, double* u_out, double* e_part)
+{
+    if (!ctx || n < 0) return ctx_fail(ctx, "nb_cool_beta: bad arguments");
+    if (n == 0) return NB_OK;
+    hipLaunchKernelGGL(k_cool_beta, dim3(grid_s(n)), dim3(kBlockS), 0, (hipStream_t)stream, u, x_old, x_new, mass,
+                       u_floor, dt, beta, n, u_out, e_part);
+    return launch_check_s(ctx, "nb_cool_beta");
+}
+
+extern "C" int32_t nb_cool_powerlaw(nb_ctx* ctx, void* stream, const double* u, const double* rho,
+                                    const double* mass, double u_floor, double k, double dt, int64_t n,
+                                    double* u_out, double* e_part)
+{
+    if (!ctx || n < 0) return ctx_fail(ctx, "nb_cool_powerlaw: bad arguments");
+    if (n == 0) return NB_OK;
+    hipLaunchKernelGGL(k_cool_powerlaw, dim3(grid_s(n)), dim3(kBlockS), 0, (hipStream_t)stream, u, rho, mass,
+                       u_floor, k, dt, n, u_out, e_part);
+    return launch_check_s(ctx, "nb_cool_powerlaw");
+}
+
+static int32_t reduce_two_level(nb_ctx* ctx, hipStream_t s, int w, int op, int nparts, double* host_out)
+{
+    double* fin = ctx->red + (size_t)w * nparts;
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, s, ctx->red, nparts, w, op, fin);
+    if (hipMemcpyAsync(host_out, fin, sizeof(double) * w, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return ctx_fail(ctx, "reduction copy failed");
+    return NB_OK;
+}
+
+extern "C" int32_t nb_dt_minima(nb_ctx* ctx, void* stream, const double* h, const double* vsig, const double* acc,
+                                const double* nu, double c_visc, const double* x, int64_t n, double* out4)
+{
+    if (!ctx || n <= 0 || !out4) return ctx_fail(ctx, "nb_dt_minima: bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    int nparts = (int)std::min<int64_t>(1024, (n + kBlockS - 1) / kBlockS);
+    if (ctx_reserve(ctx, (void**)&ctx->red, &ctx->red_bytes, sizeof(double) * 4 * (nparts + 1))) return NB_ERR_HIP;
+    hipLaunchKernelGGL(k_dt_partial, dim3(nparts), dim3(kBlockS), 0, s, h, vsig, acc, nu, c_visc, x, n, ctx->red);
+    return reduce_two_level(ctx, s, 4, 0, nparts, out4);
+}
+
+extern "C" int32_t nb_ledger(nb_ctx* ctx, void* stream, const double* x, const double* p, const double* u,
+                             const double* mass, const double* e_part, int32_t gn, int64_t n, double* out6)
+{
+    if (!ctx || n <= 0 || !out6) return ctx_fail(ctx, "nb_ledger: bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    int nparts = (int)std::min<int64_t>(1024, (n + kBlockS - 1) / kBlockS);
+    if (ctx_reserve(ctx, (void**)&ctx->red, &ctx->red_bytes, sizeof(double) * 6 * (nparts + 1))) return NB_ERR_HIP;
+    hipLaunchKernelGGL(k_ledger_partial, dim3(nparts), dim3(kBlockS), 0, s, x, p, u, mass, e_part, gn, n, ctx->red);
+    return reduce_two_level(ctx, s, 6, 1, nparts, out6);
+}

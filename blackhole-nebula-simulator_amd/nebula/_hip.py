@@ -1,0 +1,168 @@
+"""ctypes binding of libnebula_hip.so (the C ABI declared in include/nebula_hip.h).
+
+The library is the only compute path: if it is missing or no GPU is visible, every entry
+point raises. PyTorch owns the device memory (tensors are passed by ``data_ptr``) and the
+stream (torch's current stream), nothing else.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libnebula_hip.so"
+
+NB_OK = 0
+NB_RETRY_CAPACITY = 1
+NB_ERR_BOX = -3
+
+_vp = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_f64 = ctypes.c_double
+_P64 = ctypes.POINTER(ctypes.c_int64)
+_PD = ctypes.POINTER(ctypes.c_double)
+
+
+class ForceParams(ctypes.Structure):
+    """nb_force_params (include/nebula_hip.h)."""
+
+    _fields_ = [
+        ("kid", _i32),
+        ("use_visc", _i32),
+        ("use_cond", _i32),
+        ("cond_signal", _i32),
+        ("use_recon", _i32),
+        ("closeness", _i32),
+        ("gate", _i32),
+        ("precision", _i32),
+        ("cnorm", _f64),
+        ("beta", _f64),
+        ("alpha_u", _f64),
+        ("eta_crit", _f64),
+        ("eta_fold", _f64),
+    ]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "nb_ctx_create": (_vp, [_i32]),
+    "nb_ctx_destroy": (None, [_vp]),
+    "nb_last_error": (ctypes.c_char_p, [_vp]),
+    "nb_abi_version": (_i32, []),
+    "nb_ctx_scratch_bytes": (_i64, [_vp]),
+    "nb_lists_build": (_i32, [_vp, _vp, _vp, _vp, _i64, _PD, _vp, _vp, _i64, _P64]),
+    "nb_lists_emit": (_i32, [_vp, _vp, _vp, _i64]),
+    "nb_tighten": (_i32, [_vp, _vp, _vp, _vp, _i64, _PD, _vp, _vp, _vp, _i64, _vp, _vp, _i64, _P64]),
+    "nb_density": (_i32, [_vp, _vp, _vp, _vp, _f64, _vp, _vp, _vp, _vp, _i64, _PD, _vp, _i64, _i32,
+                          _f64, _f64, _f64, _f64, _vp, _vp, _vp, _vp, _vp]),
+    "nb_forces": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                         _PD, _vp, _i64, ctypes.POINTER(ForceParams), _vp, _vp, _vp, _vp, _vp]),
+    "nb_gradients": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _PD, _vp, _i64, _i32, _f64,
+                            _vp, _vp, _vp]),
+    "nb_velocity_gradients": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _PD, _vp, _i64, _i32,
+                                     _f64, _vp]),
+    "nb_ns_gradients": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _PD, _vp, _i64,
+                               _i32, _f64, _vp]),
+    "nb_ns_forces": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _PD, _vp, _i64, _i32,
+                            _f64, _vp]),
+    "nb_disc_forces": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f64, _vp, _vp, _PD, _vp,
+                              _i64, _i32, _f64, _vp, _vp]),
+    "nb_glf_step": (_i32, [_vp, _vp, _vp, _vp, _vp, _f64, _i64, _vp, _vp, _vp, _vp]),
+    "nb_gn_v_from_p": (_i32, [_vp, _vp, _vp, _vp, _i64, _vp]),
+    "nb_axpy": (_i32, [_vp, _vp, _vp, _vp, _f64, _i64, _vp]),
+    "nb_eos_ideal": (_i32, [_vp, _vp, _vp, _vp, _f64, _i64, _vp, _vp, _vp]),
+    "nb_cool_beta": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _f64, _f64, _f64, _i64, _vp, _vp]),
+    "nb_cool_powerlaw": (_i32, [_vp, _vp, _vp, _vp, _vp, _f64, _f64, _f64, _i64, _vp, _vp]),
+    "nb_dt_minima": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _f64, _vp, _i64, _PD]),
+    "nb_ledger": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _PD]),
+}
+
+
+def load_library() -> ctypes.CDLL:
+    """Load libnebula_hip.so and declare every entry point (no GPU needed)."""
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python blackhole-nebula-simulator_amd/build.py` "
+            "(or __graft_entry__.build()); there is no CPU fallback"
+        )
+    lib = ctypes.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_state = threading.local()
+_lock = threading.Lock()
+_LIB = None
+_CTX = None
+
+
+def lib() -> ctypes.CDLL:
+    global _LIB
+    with _lock:
+        if _LIB is None:
+            _LIB = load_library()
+        return _LIB
+
+
+def torch_device():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("no GPU visible: the SPH step runs only through its HIP kernels")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def ctx():
+    """The process-wide library context on the current device."""
+    global _CTX
+    L = lib()
+    dev = torch_device()
+    with _lock:
+        if _CTX is None:
+            c = L.nb_ctx_create(dev.index)
+            if not c:
+                raise RuntimeError("nb_ctx_create failed (no usable HIP device)")
+            _CTX = c
+        return _CTX
+
+
+def stream():
+    import torch
+
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def check(rc: int, what: str = "") -> int:
+    """Raise on a negative status code (message from nb_last_error)."""
+    if rc < 0:
+        msg = lib().nb_last_error(_CTX).decode(errors="replace") if _CTX else "library error"
+        if rc == NB_ERR_BOX:
+            raise ValueError(msg)
+        raise RuntimeError(f"{what}: {msg} (status {rc})")
+    return rc
+
+
+def ptr(t):
+    """Device pointer of a torch tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def box_arg(box):
+    """Host double[3] for a periodic box, or None."""
+    if box is None:
+        return None
+    b = np.broadcast_to(np.asarray(box, np.float64), (3,)).copy()
+    return (ctypes.c_double * 3)(*b)
+
+
+def call(name: str, *args) -> int:
+    return check(getattr(lib(), name)(ctx(), stream(), *args), name)

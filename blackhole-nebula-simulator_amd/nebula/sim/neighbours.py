@@ -1,0 +1,140 @@
+"""Neighbour lists on the GPU, behind the reference's call surface (neighbours.py).
+
+``find_neighbours_numba(pos, h, box)`` and ``tighten(nb, pos, h, rows)`` return the same
+symmetric, sorted CSR lists as the reference (neighbours.py:240-323): row i lists every j with
+|x_i - x_j| < 2 max(h_i, h_j), itself included. The lists are built by the HIP cell-grid
+search (csrc/lists.hip) and stay on the device; ``offset`` / ``index`` give NumPy int64 copies
+on first access, as the reference's dataclass fields are.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .. import _dev, _hip
+
+R_KERN = 2.0
+
+
+class Neighbours:
+    """CSR neighbour lists (neighbours.py:22-36): the neighbours of i are
+    index[offset[i]:offset[i + 1]] (sorted, including i). ``box``: periodic lengths or None.
+    Device copies: ``d_offset`` (int64) and ``d_index`` (int32)."""
+
+    def __init__(self, offset=None, index=None, box=None, *, d_offset=None, d_index=None):
+        self._offset = None if offset is None else np.asarray(offset, np.int64)
+        self._index = None if index is None else np.asarray(index, np.int64)
+        self.box = None if box is None else np.broadcast_to(np.asarray(box, np.float64), (3,)).copy()
+        self._d_offset = d_offset
+        self._d_index = d_index
+
+    @classmethod
+    def from_device(cls, d_offset: torch.Tensor, d_index: torch.Tensor, box) -> "Neighbours":
+        return cls(None, None, box, d_offset=d_offset, d_index=d_index)
+
+    @property
+    def offset(self) -> np.ndarray:
+        if self._offset is None:
+            self._offset = _dev.host(self._d_offset)
+        return self._offset
+
+    @property
+    def index(self) -> np.ndarray:
+        if self._index is None:
+            self._index = _dev.host(self._d_index)
+        return self._index
+
+    @property
+    def d_offset(self) -> torch.Tensor:
+        if self._d_offset is None:
+            self._d_offset = _dev.i64(self._offset)
+        return self._d_offset
+
+    @property
+    def d_index(self) -> torch.Tensor:
+        if self._d_index is None:
+            idx = self._index
+            if idx.size and (idx.max() >= 2**31 - 1 or idx.min() < 0):
+                raise ValueError("neighbour indices out of range")
+            self._d_index = _dev.i32(idx)
+        return self._d_index
+
+    @property
+    def n_entries(self) -> int:
+        if self._d_offset is not None:
+            return int(self._d_offset[-1])
+        return int(self._offset[-1])
+
+    def of(self, i: int) -> np.ndarray:
+        return self.index[self.offset[i] : self.offset[i + 1]]
+
+    @property
+    def counts(self) -> np.ndarray:
+        return np.diff(self.offset)
+
+
+def minimum_image(dx: np.ndarray, box) -> np.ndarray:
+    """Separation vectors folded into (-L/2, L/2] (neighbours.py:39-43)."""
+    if box is None:
+        return dx
+    return dx - box * np.round(dx / box)
+
+
+def wrap(pos: np.ndarray, box) -> np.ndarray:
+    """Positions folded into [0, L) (neighbours.py:46-50)."""
+    if box is None:
+        return pos
+    return np.mod(pos, box)
+
+
+def _box3(box):
+    return None if box is None else np.broadcast_to(np.asarray(box, np.float64), (3,)).copy()
+
+
+def build_lists(pos_d: torch.Tensor, h_d: torch.Tensor, box_arr, capacity_hint: int = 0) -> Neighbours:
+    """Symmetric lists of device positions (N,3) at search lengths h (N,), kept on device."""
+    n = int(pos_d.shape[0])
+    off = _dev.empty(n + 1, torch.int64)
+    total = np.zeros(1, np.int64)
+    cap = max(int(capacity_hint), 1)
+    idx = _dev.empty(cap, torch.int32)
+    rc = _hip.call("nb_lists_build", _hip.ptr(pos_d), _hip.ptr(h_d), n, _hip.box_arg(box_arr),
+                   _hip.ptr(off), _hip.ptr(idx), cap, total.ctypes.data_as(_hip._P64))  # fmt: skip
+    if rc == _hip.NB_RETRY_CAPACITY:
+        cap = int(total[0])
+        idx = _dev.empty(max(cap, 1), torch.int32)
+        _hip.call("nb_lists_emit", _hip.ptr(idx), max(cap, 1))
+    m = int(total[0])
+    return Neighbours.from_device(off, idx[:m], box_arr)
+
+
+def find_neighbours_numba(pos, h, box=None) -> Neighbours:
+    """Symmetric neighbour lists for positions (N, 3) and smoothing lengths (N,)
+    (neighbours.py:240-281). Raises ValueError if 2 max(h) reaches half a periodic box."""
+    box_arr = _box3(box)
+    h_np = h.detach().cpu().numpy() if isinstance(h, torch.Tensor) else np.asarray(h, np.float64)
+    if box_arr is not None and len(h_np) and np.any(R_KERN * h_np.max() >= 0.5 * box_arr):
+        raise ValueError("kernel support exceeds half the periodic box: neighbours ambiguous")
+    return build_lists(_dev.f64(pos), _dev.f64(h), box_arr)
+
+
+# the reference's cKDTree twin gives identical lists (neighbours.py:53-71)
+find_neighbours = find_neighbours_numba
+
+
+def tighten(nb: Neighbours, pos, h, rows=None) -> Neighbours:
+    """Keep only the pairs with r < 2 max(h_i, h_j) (neighbours.py:306-323). ``rows``: lists
+    only for these particles, the others' left empty."""
+    pos_d = _dev.f64(pos)
+    h_d = _dev.f64(h)
+    n = int(pos_d.shape[0])
+    rows_d, nrows = _dev.rows_dev(rows, n)
+    cap = max(nb.n_entries, 1)
+    off = _dev.empty(n + 1, torch.int64)
+    idx = _dev.empty(cap, torch.int32)
+    total = np.zeros(1, np.int64)
+    _hip.call("nb_tighten", _hip.ptr(pos_d), _hip.ptr(h_d), n, _hip.box_arg(nb.box), _hip.ptr(nb.d_offset),
+              _hip.ptr(nb.d_index), _hip.ptr(rows_d), nrows, _hip.ptr(off), _hip.ptr(idx), cap,
+              total.ctypes.data_as(_hip._P64))  # fmt: skip
+    return Neighbours.from_device(off, idx[: int(total[0])], nb.box)

@@ -1,0 +1,362 @@
+"""SPH passes on the GPU behind the reference's call surface (sph.py).
+
+Density and smoothing length (grad-h SPH, Newton-Raphson with a bisection bracket), the
+force pass (pressure, grad-h, Monaghan viscosity, conductivity, P dV heating), velocity
+gradients, Navier-Stokes and disc viscosity: each function keeps the reference's name,
+arguments, flags and result types (sph.py:36-1137) and runs one HIP kernel
+(csrc/pairs_fp64.hip). NumPy inputs give NumPy results; torch GPU tensors give GPU tensors.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from .. import _dev, _hip
+from ..physics.kernels import C_NORM, DEFAULT_ETA, DEFAULT_KERNEL, KERNEL_IDS
+from .neighbours import Neighbours, build_lists
+
+FLAG_OK = 0
+FLAG_AT_H_MAX = 1
+FLAG_SEARCH_TOO_SMALL = 2
+FLAG_NO_CONVERGENCE = 3
+MAX_ITER = 50
+
+COND_PRESSURE = 0
+COND_VELOCITY = 1
+GATE_BOTH, GATE_PUBLISHED, GATE_MINMOD = 0, 1, 2
+
+PRECISIONS = {"fp64": 0, "mixed": 1}
+
+
+def _out(t, as_dev: bool):
+    return t if as_dev else _dev.host(t)
+
+
+@dataclass
+class DensityResult:
+    h: object
+    rho: object
+    omega: object
+    iterations: object
+    flags: object
+    neighbours: Neighbours
+    cache: object = None
+    builds: int = 1
+
+
+@dataclass
+class NeighbourCache:
+    """Accepted for signature compatibility (sph.py:62-82). Lists are rebuilt at every density
+    pass on the GPU (a build costs about one pair pass), so no cache is kept."""
+
+    neighbours: Neighbours
+    x_ref: np.ndarray
+    h_search: np.ndarray
+    builds: int = 1
+    reuses: int = 0
+
+
+def _box3(box):
+    return None if box is None else np.broadcast_to(np.asarray(box, np.float64), (3,)).copy()
+
+
+def density_device(pos_d, mass_d, h_start_d, nb: Neighbours, h_search_d, kernel_name, eta, tol, h_max,
+                   rows_d=None, nrows=None, m_uniform=None):  # fmt: skip
+    """Device form of density_numba: returns (h, rho, omega, iters, flags) tensors."""
+    n = int(pos_d.shape[0])
+    nrows = n if nrows is None else nrows
+    kid = KERNEL_IDS[kernel_name]
+    if m_uniform is None:
+        m_uniform = _dev.uniform_mass(mass_d)
+    h_out, rho, omega = (_dev.empty(nrows) for _ in range(3))
+    iters = _dev.empty(nrows, torch.int32)
+    flags = _dev.empty(nrows, torch.int32)
+    _hip.call("nb_density", _hip.ptr(pos_d), _hip.ptr(mass_d), float(m_uniform), _hip.ptr(h_start_d),
+              _hip.ptr(h_search_d), _hip.ptr(nb.d_offset), _hip.ptr(nb.d_index), nb.n_entries,
+              _hip.box_arg(nb.box), _hip.ptr(rows_d), nrows, kid, float(C_NORM[kid]), float(eta),
+              float(tol), float(h_max), _hip.ptr(h_out), _hip.ptr(rho), _hip.ptr(omega),
+              _hip.ptr(iters), _hip.ptr(flags))  # fmt: skip
+    return h_out, rho, omega, iters, flags
+
+
+def density_numba(pos, mass, h_start, nb: Neighbours, h_search, kernel_name=DEFAULT_KERNEL, eta=DEFAULT_ETA,
+                  tol=1e-4, h_max=np.inf, rows=None) -> DensityResult:  # fmt: skip
+    """The density pass on given neighbour lists (sph.py:300-326). ``rows``: solve only these
+    particles (results in that order); default all."""
+    as_dev = _dev.is_dev(pos)
+    pos_d = _dev.f64(pos)
+    rows_d, nrows = _dev.rows_dev(rows, int(pos_d.shape[0]))
+    h, rho, omega, it, fl = density_device(pos_d, _dev.f64(mass), _dev.f64(h_start), nb, _dev.f64(h_search),
+                                           kernel_name, eta, tol, h_max, rows_d, nrows)  # fmt: skip
+    return DensityResult(*(_out(t, as_dev) for t in (h, rho, omega, it, fl)), nb)
+
+
+def compute_density(pos, mass, h_guess, box=None, kernel_name=DEFAULT_KERNEL, eta=DEFAULT_ETA, tol=1e-4,
+                    h_max=np.inf, skin=1.4, cache=None, rows=None) -> DensityResult:  # fmt: skip
+    """Density pass with neighbour lists built at search radius 2 min(h_guess, h_max) x skin
+    (sph.py:329-387); particles whose h outgrows the search are redone with rebuilt lists,
+    doubling their search (up to 12 rounds). ``cache`` is accepted and ignored: lists are
+    rebuilt at every call."""
+    as_dev = _dev.is_dev(pos)
+    pos_d = _dev.f64(pos)
+    mass_d = _dev.f64(mass)
+    res = compute_density_device(pos_d, mass_d, _dev.f64(h_guess, copy=True), _box3(box), kernel_name, eta,
+                                 tol, h_max, skin, rows)  # fmt: skip
+    if not as_dev:
+        for k in ("h", "rho", "omega", "iterations", "flags"):
+            setattr(res, k, _dev.host(getattr(res, k)))
+    return res
+
+
+def compute_density_device(pos_d, mass_d, h_guess_d, box_arr, kernel_name, eta, tol, h_max, skin, rows=None,
+                           m_uniform=None, capacity_hint=0) -> DensityResult:  # fmt: skip
+    """compute_density on device tensors; h_guess_d may be modified."""
+    n = int(pos_d.shape[0])
+    rows_d, nrows = _dev.rows_dev(rows, n)
+    box_cap = np.inf if box_arr is None else 0.2499 * float(np.min(box_arr))
+    h_search = torch.clamp(torch.clamp(h_guess_d, max=h_max) * skin, max=box_cap) if np.isfinite(
+        h_max) else torch.clamp(h_guess_d * skin, max=box_cap)  # fmt: skip
+    if m_uniform is None:
+        m_uniform = _dev.uniform_mass(mass_d)
+    builds = 0
+    for _ in range(12):
+        nb = build_lists(pos_d, h_search, box_arr, capacity_hint)
+        builds += 1
+        h, rho, omega, it, fl = density_device(pos_d, mass_d, h_guess_d, nb, h_search, kernel_name, eta, tol,
+                                               h_max, rows_d, nrows, m_uniform)  # fmt: skip
+        grow = fl == FLAG_SEARCH_TOO_SMALL
+        if not bool(torch.any(grow)):
+            return DensityResult(h, rho, omega, it, fl, nb, None, builds)
+        idx = rows_d[grow] if rows_d is not None else torch.nonzero(grow).flatten()
+        h_search = h_search.clone()
+        h_search[idx] = torch.clamp(h_search[idx] * 2.0, max=box_cap)
+        if rows_d is not None:
+            h_guess_d[rows_d] = h
+        else:
+            h_guess_d = h.clone()
+    raise RuntimeError("smoothing lengths kept outgrowing the neighbour search")
+
+
+# --------------------------------------------------------------------------- forces
+
+
+@dataclass(frozen=True)
+class ShockCapturing:
+    """Shock-capturing settings (sph.py:409-429)."""
+
+    beta: float = 2.0
+    alpha_u: float = 1.0
+    conductivity_signal: int = COND_PRESSURE
+    reconstruction: bool = False
+    closeness: bool = True
+    eta_crit: float = 1.0 / DEFAULT_ETA
+    eta_fold: float = 0.2
+    recon_gate: int = GATE_MINMOD
+
+
+DEFAULT_SHOCK = ShockCapturing()
+
+
+@dataclass
+class ForceResult:
+    acc: object
+    dudt: object
+    vsig_max: object
+    dudt_visc: object = None
+    dudt_cond: object = None
+
+
+def force_params(kernel_name, use_visc, use_cond, shock: ShockCapturing, precision="fp64") -> _hip.ForceParams:
+    kid = KERNEL_IDS[kernel_name]
+    p = _hip.ForceParams()
+    p.kid = kid
+    p.use_visc = int(use_visc)
+    p.use_cond = int(use_cond)
+    p.cond_signal = int(shock.conductivity_signal)
+    p.use_recon = int(use_visc and shock.reconstruction)
+    p.closeness = int(shock.closeness)
+    p.gate = int(shock.recon_gate)
+    p.precision = PRECISIONS[precision]
+    p.cnorm = float(C_NORM[kid])
+    p.beta = float(shock.beta)
+    p.alpha_u = float(shock.alpha_u)
+    p.eta_crit = float(shock.eta_crit)
+    p.eta_fold = float(shock.eta_fold)
+    return p
+
+
+def forces_device(pos_d, vel_d, mass_d, h_d, rho_d, omega_d, p_d, nb, kernel_name, u_d, cs_d, alpha_d, shock,
+                  rows_d=None, nrows=None, gv_d=None, precision="fp64"):  # fmt: skip
+    n = int(pos_d.shape[0])
+    nrows = n if nrows is None else nrows
+    use_visc = alpha_d is not None and cs_d is not None
+    use_cond = use_visc and u_d is not None
+    prm = force_params(kernel_name, use_visc, use_cond, shock, precision)
+    acc = _dev.empty((nrows, 3))
+    dudt, vsig, vheat, cheat = (_dev.empty(nrows) for _ in range(4))
+    _hip.call("nb_forces", _hip.ptr(pos_d), _hip.ptr(vel_d), _hip.ptr(mass_d), _hip.ptr(h_d), _hip.ptr(rho_d),
+              _hip.ptr(omega_d), _hip.ptr(p_d), _hip.ptr(u_d if use_cond else None),
+              _hip.ptr(cs_d if use_visc else None), _hip.ptr(alpha_d if use_visc else None),
+              _hip.ptr(gv_d if prm.use_recon else None), _hip.ptr(nb.d_offset), _hip.ptr(nb.d_index),
+              _hip.box_arg(nb.box), _hip.ptr(rows_d), nrows, prm, _hip.ptr(acc), _hip.ptr(dudt),
+              _hip.ptr(vsig), _hip.ptr(vheat), _hip.ptr(cheat))  # fmt: skip
+    return ForceResult(acc, dudt, vsig, vheat, cheat)
+
+
+def forces_numba(pos, vel, mass, h, rho, omega, pressure, nb: Neighbours, kernel_name=DEFAULT_KERNEL, u=None,
+                 c_s=None, alpha=None, shock: ShockCapturing = DEFAULT_SHOCK, rows=None, grad_v=None,
+                 precision="fp64") -> ForceResult:  # fmt: skip
+    """Force pass (sph.py:655-690). Viscosity is on when ``alpha`` and ``c_s`` are given;
+    conductivity when ``u`` is given as well; ``grad_v`` (N, 3, 3) with reconstruction."""
+    as_dev = _dev.is_dev(pos)
+    n = len(pos)
+    use_visc = alpha is not None and c_s is not None
+    if use_visc and shock.reconstruction and (grad_v is None or tuple(np.shape(grad_v)) != (n, 3, 3)):
+        raise ValueError("velocity reconstruction needs grad_v of shape (N, 3, 3)")
+    pos_d = _dev.f64(pos)
+    rows_d, nrows = _dev.rows_dev(rows, n)
+
+    def opt(a):
+        return None if a is None else _dev.f64(a)
+
+    f = forces_device(pos_d, _dev.f64(vel), _dev.f64(mass), _dev.f64(h), _dev.f64(rho), _dev.f64(omega),
+                      _dev.f64(pressure), nb, kernel_name, opt(u), opt(c_s), opt(alpha), shock, rows_d, nrows,
+                      opt(grad_v) if (use_visc and shock.reconstruction) else None, precision)  # fmt: skip
+    if as_dev:
+        return f
+    return ForceResult(*(_dev.host(t) for t in (f.acc, f.dudt, f.vsig_max, f.dudt_visc, f.dudt_cond)))
+
+
+# --------------------------------------------------------------------------- gradients
+
+
+@dataclass
+class Gradients:
+    div_v: object
+    curl_v: object
+    ddivv_dt: object
+
+
+SINGULAR_TOL = 1e-6
+
+
+def gradients_device(pos_d, vel_d, acc_d, mass_d, h_d, nb, kernel_name, rows_d=None, nrows=None) -> Gradients:
+    n = int(pos_d.shape[0])
+    nrows = n if nrows is None else nrows
+    kid = KERNEL_IDS[kernel_name]
+    div, curl, ddiv = (_dev.empty(nrows) for _ in range(3))
+    _hip.call("nb_gradients", _hip.ptr(pos_d), _hip.ptr(vel_d), _hip.ptr(acc_d), _hip.ptr(mass_d), _hip.ptr(h_d),
+              _hip.ptr(nb.d_offset), _hip.ptr(nb.d_index), _hip.box_arg(nb.box), _hip.ptr(rows_d), nrows, kid,
+              float(C_NORM[kid]), _hip.ptr(div), _hip.ptr(curl), _hip.ptr(ddiv))  # fmt: skip
+    return Gradients(div, curl, ddiv)
+
+
+def gradients_numba(pos, vel, acc, mass, h, nb: Neighbours, kernel_name=DEFAULT_KERNEL, rows=None) -> Gradients:
+    """Corrected velocity / acceleration gradients (sph.py:887-902): div v, |curl v|,
+    d(div v)/dt. ``acc`` None means zero accelerations."""
+    as_dev = _dev.is_dev(pos)
+    pos_d = _dev.f64(pos)
+    rows_d, nrows = _dev.rows_dev(rows, int(pos_d.shape[0]))
+    g = gradients_device(pos_d, _dev.f64(vel), None if acc is None else _dev.f64(acc), _dev.f64(mass),
+                         _dev.f64(h), nb, kernel_name, rows_d, nrows)  # fmt: skip
+    return g if as_dev else Gradients(_dev.host(g.div_v), _dev.host(g.curl_v), _dev.host(g.ddivv_dt))
+
+
+def velocity_gradients_numba(pos, vel, mass, h, nb: Neighbours, kernel_name=DEFAULT_KERNEL, rows=None, out=None):
+    """Corrected velocity gradient G_ij = dv^i/dx^j for ``rows`` (sph.py:818-834); fills
+    ``out`` (N, 3, 3; zeros if not given) and returns it."""
+    as_dev = _dev.is_dev(pos)
+    pos_d, vel_d, mass_d, h_d = _dev.f64(pos), _dev.f64(vel), _dev.f64(mass), _dev.f64(h)
+    n = int(pos_d.shape[0])
+    rows_d, nrows = _dev.rows_dev(rows, n)
+    out_d = torch.zeros((n, 3, 3), dtype=torch.float64, device=pos_d.device) if out is None else _dev.f64(out)
+    velocity_gradients_device(pos_d, vel_d, mass_d, h_d, nb, kernel_name, out_d, rows_d, nrows)
+    if as_dev:
+        return out_d
+    res = _dev.host(out_d)
+    if out is not None:
+        out[...] = res
+        return out
+    return res
+
+
+def velocity_gradients_device(pos_d, vel_d, mass_d, h_d, nb, kernel_name, out_d, rows_d=None, nrows=None):
+    nrows = int(pos_d.shape[0]) if nrows is None else nrows
+    kid = KERNEL_IDS[kernel_name]
+    _hip.call("nb_velocity_gradients", _hip.ptr(pos_d), _hip.ptr(vel_d), _hip.ptr(mass_d), _hip.ptr(h_d),
+              _hip.ptr(nb.d_offset), _hip.ptr(nb.d_index), _hip.box_arg(nb.box), _hip.ptr(rows_d), nrows, kid,
+              float(C_NORM[kid]), _hip.ptr(out_d))  # fmt: skip
+    return out_d
+
+
+def ns_gradients_numba(pos, vel, mass, h, rho, omega, nb: Neighbours, kernel_name=DEFAULT_KERNEL, rows=None,
+                       out=None):  # fmt: skip
+    """Velocity gradient with the SPH difference operator (sph.py:984-999), (N, 3, 3)."""
+    as_dev = _dev.is_dev(pos)
+    pos_d, vel_d, mass_d, h_d, rho_d, om_d = (_dev.f64(a) for a in (pos, vel, mass, h, rho, omega))
+    n = int(pos_d.shape[0])
+    rows_d, nrows = _dev.rows_dev(rows, n)
+    out_d = torch.zeros((n, 3, 3), dtype=torch.float64, device=pos_d.device) if out is None else _dev.f64(out)
+    ns_gradients_device(pos_d, vel_d, mass_d, h_d, rho_d, om_d, nb, kernel_name, out_d, rows_d, nrows)
+    if as_dev:
+        return out_d
+    res = _dev.host(out_d)
+    if out is not None:
+        out[...] = res
+        return out
+    return res
+
+
+def ns_gradients_device(pos_d, vel_d, mass_d, h_d, rho_d, om_d, nb, kernel_name, out_d, rows_d=None, nrows=None):
+    nrows = int(pos_d.shape[0]) if nrows is None else nrows
+    kid = KERNEL_IDS[kernel_name]
+    _hip.call("nb_ns_gradients", _hip.ptr(pos_d), _hip.ptr(vel_d), _hip.ptr(mass_d), _hip.ptr(h_d),
+              _hip.ptr(rho_d), _hip.ptr(om_d), _hip.ptr(nb.d_offset), _hip.ptr(nb.d_index), _hip.box_arg(nb.box),
+              _hip.ptr(rows_d), nrows, kid, float(C_NORM[kid]), _hip.ptr(out_d))  # fmt: skip
+    return out_d
+
+
+def ns_forces_numba(pos, mass, h, rho, omega, stress, nb: Neighbours, kernel_name=DEFAULT_KERNEL, rows=None):
+    """Viscous acceleration of the Navier-Stokes stress (sph.py:1030-1045), (len(rows), 3)."""
+    as_dev = _dev.is_dev(pos)
+    pos_d, mass_d, h_d, rho_d, om_d, st_d = (_dev.f64(a) for a in (pos, mass, h, rho, omega, stress))
+    rows_d, nrows = _dev.rows_dev(rows, int(pos_d.shape[0]))
+    acc = ns_forces_device(pos_d, mass_d, h_d, rho_d, om_d, st_d, nb, kernel_name, rows_d, nrows)
+    return acc if as_dev else _dev.host(acc)
+
+
+def ns_forces_device(pos_d, mass_d, h_d, rho_d, om_d, st_d, nb, kernel_name, rows_d=None, nrows=None):
+    nrows = int(pos_d.shape[0]) if nrows is None else nrows
+    acc = _dev.empty((nrows, 3))
+    kid = KERNEL_IDS[kernel_name]
+    _hip.call("nb_ns_forces", _hip.ptr(pos_d), _hip.ptr(mass_d), _hip.ptr(h_d), _hip.ptr(rho_d), _hip.ptr(om_d),
+              _hip.ptr(st_d), _hip.ptr(nb.d_offset), _hip.ptr(nb.d_index), _hip.box_arg(nb.box), _hip.ptr(rows_d),
+              nrows, kid, float(C_NORM[kid]), _hip.ptr(acc))  # fmt: skip
+    return acc
+
+
+def disc_forces_numba(pos, vel, mass, h, rho, omega, c_s, alpha_av, nb: Neighbours, kernel_name=DEFAULT_KERNEL,
+                      rows=None):  # fmt: skip
+    """Disc viscosity through the AV form on every pair (sph.py:1120-1137): (acc, du/dt)."""
+    as_dev = _dev.is_dev(pos)
+    pos_d, vel_d, mass_d, h_d, rho_d, om_d, cs_d = (_dev.f64(a) for a in (pos, vel, mass, h, rho, omega, c_s))
+    rows_d, nrows = _dev.rows_dev(rows, int(pos_d.shape[0]))
+    acc, dudt = disc_forces_device(pos_d, vel_d, mass_d, h_d, rho_d, om_d, cs_d, alpha_av, nb, kernel_name,
+                                   rows_d, nrows)  # fmt: skip
+    return (acc, dudt) if as_dev else (_dev.host(acc), _dev.host(dudt))
+
+
+def disc_forces_device(pos_d, vel_d, mass_d, h_d, rho_d, om_d, cs_d, alpha_av, nb, kernel_name, rows_d=None,
+                       nrows=None):  # fmt: skip
+    nrows = int(pos_d.shape[0]) if nrows is None else nrows
+    acc = _dev.empty((nrows, 3))
+    dudt = _dev.empty(nrows)
+    kid = KERNEL_IDS[kernel_name]
+    _hip.call("nb_disc_forces", _hip.ptr(pos_d), _hip.ptr(vel_d), _hip.ptr(mass_d), _hip.ptr(h_d),
+              _hip.ptr(rho_d), _hip.ptr(om_d), _hip.ptr(cs_d), float(alpha_av), _hip.ptr(nb.d_offset),
+              _hip.ptr(nb.d_index), _hip.box_arg(nb.box), _hip.ptr(rows_d), nrows, kid, float(C_NORM[kid]),
+              _hip.ptr(acc), _hip.ptr(dudt))  # fmt: skip
+    return acc, dudt

@@ -1,0 +1,173 @@
+/*
+ * nebula_hip.h -- C ABI of libnebula_hip.so, the MI355X (gfx950) SPH step.
+ *
+ * Every entry point is stream-ordered: device pointers in, device pointers out, on the
+ * stream given (a hipStream_t passed as void*; NULL = the default stream). Pointers to
+ * device memory are owned by the caller; the library owns only the scratch space inside an
+ * nb_ctx. Entry points that must report a size to the host (list totals, reductions)
+ * synchronize the stream before returning.
+ *
+ * Return codes: 0 = ok; 1 = NB_RETRY_CAPACITY (output buffer too small, required size
+ * reported, state kept for a follow-up call); negative = error, message from
+ * nb_last_error(ctx): -1 bad argument / failure, -2 HIP allocation, -3 kernel support larger
+ * than half a periodic box (the reference's ValueError), -4 nothing built.
+ *
+ * Layouts: positions, velocities, accelerations (N,3) float64 row-major; per-particle
+ * scalars (N,) float64; CSR neighbour lists with int64 offsets (N+1) and int32 indices,
+ * rows sorted ascending and including the particle itself; row subsets ``rows`` (R,) int64
+ * or NULL for all particles (results then come back in rows order, as in the reference).
+ *
+ * Each entry point names the reference function it replaces (paths in the reference
+ * repository: src/nebula/sim/...). The Python binding is blackhole-nebula-simulator_amd/
+ * nebula/_hip.py; the reference-named wrappers are nebula/sim/{neighbours,sph,integrator,
+ * fastphys,coupled}.py.
+ */
+#ifndef NEBULA_HIP_H
+#define NEBULA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nb_ctx nb_ctx;
+
+/* Shock-capturing and kernel settings of the force pass (sph.py:409-432 ShockCapturing,
+ * sph.py:655-690 argument meaning). */
+typedef struct nb_force_params {
+    int32_t kid;          /* 0 cubic spline, 1 Wendland C4 (kernels_nb.py:12) */
+    int32_t use_visc;     /* alpha and c_s given */
+    int32_t use_cond;     /* u given as well */
+    int32_t cond_signal;  /* 0 pressure, 1 velocity (sph.py:404-405) */
+    int32_t use_recon;    /* velocity reconstruction (needs grad_v) */
+    int32_t closeness;
+    int32_t gate;         /* 0 both, 1 published, 2 min-mod (sph.py:406) */
+    int32_t precision;    /* 0 FP64; 1 mixed (FP32 pair sums, compensated) */
+    double cnorm;
+    double beta;
+    double alpha_u;
+    double eta_crit;
+    double eta_fold;
+} nb_force_params;
+
+/* ------------------------------------------------------------------ context */
+nb_ctx *nb_ctx_create(int32_t device);
+void nb_ctx_destroy(nb_ctx *ctx);
+const char *nb_last_error(const nb_ctx *ctx);
+int32_t nb_abi_version(void);
+/* bytes of scratch the context currently holds */
+int64_t nb_ctx_scratch_bytes(const nb_ctx *ctx);
+
+/* ------------------------------------------------------------------ neighbour lists */
+/* find_neighbours_numba (neighbours.py:240-281): symmetric lists {j : |x_i - x_j| <
+ * 2 max(h_i, h_j)} at search lengths h. Writes out_offset (N+1) always; writes out_index when
+ * its capacity holds the total, else returns NB_RETRY_CAPACITY with *total_out set and keeps
+ * the lists for nb_lists_emit. box3: periodic lengths (host array of 3, 0 = open) or NULL. */
+int32_t nb_lists_build(nb_ctx *ctx, void *stream, const double *pos, const double *h, int64_t n,
+                       const double *box3, int64_t *out_offset, int32_t *out_index, int64_t capacity,
+                       int64_t *total_out);
+int32_t nb_lists_emit(nb_ctx *ctx, void *stream, int32_t *out_index, int64_t capacity);
+
+/* tighten (neighbours.py:306-323): keep pairs with |x_i - x_j| < 2 max(h_i, h_j); rows not in
+ * ``rows`` are left empty. Same capacity protocol (the input list total always suffices). */
+int32_t nb_tighten(nb_ctx *ctx, void *stream, const double *pos, const double *h, int64_t n,
+                   const double *box3, const int64_t *offset, const int32_t *index, const int64_t *rows,
+                   int64_t nrows, int64_t *out_offset, int32_t *out_index, int64_t capacity,
+                   int64_t *total_out);
+
+/* ------------------------------------------------------------------ pair passes */
+/* density_numba (sph.py:300-326 -> _density_kernel sph.py:251-297): Newton-bisection h-rho
+ * solve per row. m_uniform > 0 declares that every mass equals it (masses then not cached).
+ * n_entries = offset[N]. precision 0 = FP64 (bit-identical to the reference's arithmetic),
+ * 1 = mixed. flags: 0 ok, 1 at h_max, 2 search too small, 3 no convergence. */
+int32_t nb_density(nb_ctx *ctx, void *stream, const double *pos, const double *mass, double m_uniform,
+                   const double *h_start, const double *h_search, const int64_t *offset,
+                   const int32_t *index, int64_t n_entries, const double *box3, const int64_t *rows,
+                   int64_t nrows, int32_t kid, double cnorm, double eta, double tol, double h_max,
+                   double *h_out, double *rho, double *omega, int32_t *iters, int32_t *flags);
+
+/* forces_numba (sph.py:655-690 -> _forces_kernel sph.py:525-652). u, c_s, alpha, grad_v may be
+ * NULL when the corresponding term is off. vheat / cheat may be NULL. */
+int32_t nb_forces(nb_ctx *ctx, void *stream, const double *pos, const double *vel, const double *mass,
+                  const double *h, const double *rho, const double *omega, const double *pressure,
+                  const double *u, const double *c_s, const double *alpha, const double *grad_v,
+                  const int64_t *offset, const int32_t *index, const double *box3, const int64_t *rows,
+                  int64_t nrows, const nb_force_params *params, double *acc, double *dudt,
+                  double *vsig_max, double *vheat, double *cheat);
+
+/* gradients_numba (sph.py:887-902 -> _gradients_kernel sph.py:837-884); acc NULL = zeros. */
+int32_t nb_gradients(nb_ctx *ctx, void *stream, const double *pos, const double *vel, const double *acc,
+                     const double *mass, const double *h, const int64_t *offset, const int32_t *index,
+                     const double *box3, const int64_t *rows, int64_t nrows, int32_t kid, double cnorm,
+                     double *div, double *curl, double *ddiv);
+
+/* velocity_gradients_numba (sph.py:818-834 -> sph.py:785-815): out (N,3,3), rows written. */
+int32_t nb_velocity_gradients(nb_ctx *ctx, void *stream, const double *pos, const double *vel,
+                              const double *mass, const double *h, const int64_t *offset,
+                              const int32_t *index, const double *box3, const int64_t *rows, int64_t nrows,
+                              int32_t kid, double cnorm, double *out);
+
+/* ns_gradients_numba (sph.py:984-999 -> sph.py:957-981): out (N,3,3), rows written. */
+int32_t nb_ns_gradients(nb_ctx *ctx, void *stream, const double *pos, const double *vel, const double *mass,
+                        const double *h, const double *rho, const double *omega, const int64_t *offset,
+                        const int32_t *index, const double *box3, const int64_t *rows, int64_t nrows,
+                        int32_t kid, double cnorm, double *out);
+
+/* ns_forces_numba (sph.py:1030-1045 -> sph.py:1002-1027): acc (R,3). */
+int32_t nb_ns_forces(nb_ctx *ctx, void *stream, const double *pos, const double *mass, const double *h,
+                     const double *rho, const double *omega, const double *stress, const int64_t *offset,
+                     const int32_t *index, const double *box3, const int64_t *rows, int64_t nrows,
+                     int32_t kid, double cnorm, double *acc);
+
+/* disc_forces_numba (sph.py:1120-1137 -> _disc_kernel sph.py:1084-1117). */
+int32_t nb_disc_forces(nb_ctx *ctx, void *stream, const double *pos, const double *vel, const double *mass,
+                       const double *h, const double *rho, const double *omega, const double *c_s,
+                       double alpha_av, const int64_t *offset, const int32_t *index, const double *box3,
+                       const int64_t *rows, int64_t nrows, int32_t kid, double cnorm, double *acc,
+                       double *dudt);
+
+/* ------------------------------------------------------------------ GN gravity and the step */
+/* glf_step_many (integrator.py:188-198 -> _glf_one integrator.py:139-176): dt_arr (N) or NULL
+ * with dt_scalar. iters: count or -1 / -2 / -3 (integrator.py:38-41); nfail_dev (device int32,
+ * optional) is incremented per failed particle. */
+int32_t nb_glf_step(nb_ctx *ctx, void *stream, const double *x, const double *p, const double *dt_arr,
+                    double dt_scalar, int64_t n, double *x_out, double *p_out, int32_t *iters,
+                    int32_t *nfail_dev);
+
+/* gn_v_from_p_nb (fastphys.py:35-42). */
+int32_t nb_gn_v_from_p(nb_ctx *ctx, void *stream, const double *x, const double *p, int64_t n, double *out);
+
+/* out = y + s a over m doubles (the half kicks of coupled.py:229-258). */
+int32_t nb_axpy(nb_ctx *ctx, void *stream, const double *y, const double *a, double s, int64_t m,
+                double *out);
+
+/* eos_state, ideal branch (hydro.py:121-139; eos.py:21-28); nbad_dev counts u <= 0. */
+int32_t nb_eos_ideal(nb_ctx *ctx, void *stream, const double *rho, const double *u, double gamma, int64_t n,
+                     double *pressure, double *cs, int32_t *nbad_dev);
+
+/* beta cooling inside the step (coupled.py:244-251; cooling.py:28-40); e_part = m (u - u_new). */
+int32_t nb_cool_beta(nb_ctx *ctx, void *stream, const double *u, const double *x_old, const double *x_new,
+                     const double *mass, double u_floor, double dt, double beta, int64_t n, double *u_out,
+                     double *e_part);
+
+/* optically thin power-law cooling du/dt = -k rho sqrt(u) above u_floor, exact at fixed rho
+ * (the reference has no such law; same place in the split as beta cooling). */
+int32_t nb_cool_powerlaw(nb_ctx *ctx, void *stream, const double *u, const double *rho, const double *mass,
+                         double u_floor, double k, double dt, int64_t n, double *u_out, double *e_part);
+
+/* stable_dt minima (coupled.py:213-226): out4 (host) = [min h/v_sig, min sqrt(h/|a|) over |a|>0,
+ * min c_visc h^2/nu over nu>0 (nu may be NULL), min |x|]. */
+int32_t nb_dt_minima(nb_ctx *ctx, void *stream, const double *h, const double *vsig, const double *acc,
+                     const double *nu, double c_visc, const double *x, int64_t n, double *out4);
+
+/* ledger sums (coupled.py:261-270): out6 (host) = [sum m (H + u), sum m |H|, sum m (x cross p)
+ * (3), sum e_part]. gn = 1: GN Hamiltonian (gravity.py:92-102); 0: |p|^2/2. */
+int32_t nb_ledger(nb_ctx *ctx, void *stream, const double *x, const double *p, const double *u,
+                  const double *mass, const double *e_part, int32_t gn, int64_t n, double *out6);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NEBULA_HIP_H */

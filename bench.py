@@ -1,0 +1,159 @@
+"""Throughput of the SPH step on one MI355X: particle-steps/s of coupled.step at 1M particles.
+
+Workload (the 1M-particle configuration of the metric): two seeded thin cylindrical streams of
+radius 20 and length 2000 (0.05 M_sun each, 1% Gaussian mass noise, 524,288 particles each)
+meeting at r = 1664 M with a 160 degree opening angle at 0.056 c each, T = 1e4 K, cubic spline
+with eta = 1.2, Monaghan viscosity alpha = 1, beta = 2 with a Balsara limiter, power-law cooling
+on. A step is one full coupled.step (GN flow, cooling, neighbour lists, density / h solve,
+equation of state, gradients for the limiter, forces, kicks) plus its stable_dt; the state is
+resident in HBM before the timed region starts. value = N x steps / wall time.
+
+Prints one JSON line. ``roofline`` covers the dominant pair kernel, timed with HIP events on
+the stream the kernels run on (torch's current stream); ``cpu_baseline`` times the FP64 CPU
+oracle (OpenMP, this host's cores) on the same workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from dataclasses import replace
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+for _p in (str(REPO), str(REPO / "blackhole-nebula-simulator_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+PEAK = {"fp64": 78.6, "fp32": 157.3}  # TFLOP/s, MI355X vector (dense) published peaks
+HBM_PEAK = 8000.0  # GB/s
+
+# Algorithmic FLOPs, counted from the reference's source operations (FMA = 2, div = sqrt = 1):
+FLOPS_FORCE_PAIR = 83  # force pair with Monaghan viscosity, no conductivity (sph.py:542-645)
+FLOPS_GRAD_PAIR = 45  # gradient pair (sph.py:848-865)
+FLOPS_SEARCH_ENTRY = 10  # distance pass per search-list entry (sph.py:231-248)
+FLOPS_NEWTON_ENTRY = 20  # one kernel evaluation in the Newton sums (sph.py:215-228)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1_048_576)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    return ap.parse_args()
+
+
+def workload(n):
+    from nebula.ics import stream_crash
+    from nebula.sim import coupled
+
+    ic = stream_crash(n, seed=20261014)
+    return ic
+
+
+def config(precision):
+    from nebula.sim import coupled
+
+    return replace(coupled.baseline_config(balsara=True, precision=precision), cooling_powerlaw=True)
+
+
+def cpu_baseline(ic, cfg, steps):
+    from oracle import coupled as oc
+    from oracle import lib as olib
+
+    cores = len(os.sched_getaffinity(0))
+    olib.set_threads(cores)
+    st = oc.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg)
+    st = oc.step(st, cfg, oc.stable_dt(st, cfg))  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = oc.step(st, cfg, oc.stable_dt(st, cfg))
+    wall = time.perf_counter() - t0
+    n = len(ic.x)
+    return {"value": n * steps / wall, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+            "sample": f"the same {n}-particle stream crash, FP64 C/OpenMP restatement of the reference's numba "
+                      f"kernels, 1 warm-up + {steps} timed coupled steps"}  # fmt: skip
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
+                                   "described in DESIGN.md, not implemented)"}))  # fmt: skip
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from nebula import _hip
+    from nebula.sim import coupled
+
+    dev = _hip.torch_device()
+    ic = workload(args.n)
+    cfg = config(args.precision)
+    print(f"bench: {args.n} particles, {args.precision}, warmup {args.warmup}, steps {args.steps}", file=sys.stderr,
+          flush=True)  # fmt: skip
+    st = coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg)
+    for _ in range(args.warmup):
+        st = coupled.step(st, cfg, coupled.stable_dt(st, cfg))
+    torch.cuda.synchronize()
+    _hip.start_timing()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = coupled.step(st, cfg, coupled.stable_dt(st, cfg))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern = _hip.stop_timing()  # name -> (launches, total ms)
+    n = args.n
+    value = n * args.steps / wall
+    # algorithmic work of the pair kernels, from this run's own lists and counters
+    n_search = st.n_search_entries / n
+    n_pairs = (st.n_tight_entries - n) / n
+    k_it = float(st.iterations.double().mean().item())
+    flops = {
+        "nb_forces": FLOPS_FORCE_PAIR * n_pairs * n,
+        "nb_gradients": FLOPS_GRAD_PAIR * n_pairs * n,
+        "nb_density": (FLOPS_SEARCH_ENTRY * n_search + FLOPS_NEWTON_ENTRY * (k_it + 1) * n_search) * n,
+    }
+    pair = {k: v for k, v in kern.items() if k in flops}
+    dom = max(pair, key=lambda k: pair[k][1] / pair[k][0])
+    launches, total_ms = pair[dom]
+    avg_s = total_ms / launches / 1e3
+    achieved = flops[dom] / avg_s / 1e12
+    peak = PEAK["fp64"] if args.precision == "fp64" else PEAK["fp32"]
+    out = {
+        "metric": "SPH particle-steps/s at 1M particles (plain FP64 vs mixed+learned-h; % FP32 peak)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic (seeded stream-crash initial conditions)",
+        "config": {"workload": "stream crash, 2 x 524,288-particle streams (1,048,576)" if n == 1_048_576 else
+                   f"stream crash, {n} particles", "precision": args.precision, "kernel": "cubic, eta 1.2",
+                   "viscosity": "Monaghan alpha 1 beta 2 + Balsara", "cooling": "power law rho^2 T^0.5"},
+        "roofline": {"bound": "valu", "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None,
+                     "work": f"{flops[dom] / n:.0f} FLOP/particle-step (n_s {n_search:.1f}, pairs {n_pairs:.1f}, "
+                             f"Newton iterations {k_it:.2f})"},
+        "kernels_ms_per_step": {k: v[1] / args.steps for k, v in sorted(kern.items(), key=lambda kv: -kv[1][1])},
+    }
+    if not args.no_cpu:
+        print("bench: CPU baseline (FP64 oracle, OpenMP)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(ic, cfg, args.cpu_steps)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

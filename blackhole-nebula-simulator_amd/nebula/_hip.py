@@ -79,6 +79,15 @@ SIGNATURES = {
     "nb_cool_powerlaw": (_i32, [_vp, _vp, _vp, _vp, _vp, _f64, _f64, _f64, _i64, _vp, _vp]),
     "nb_dt_minima": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _f64, _vp, _i64, _PD]),
     "nb_ledger": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _PD]),
+    "nb_search_lengths": (_i32, [_vp, _vp, _vp, _f64, _f64, _f64, _i64, _vp]),
+    "nb_grow_search": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _f64, _vp, _vp, _P64]),
+    "nb_fill": (_i32, [_vp, _vp, _vp, _f64, _i64]),
+    "nb_alpha_balsara": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _f64, _f64, _i64, _vp]),
+    "nb_alpha_switch": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f64, _f64, _f64, _i64, _vp]),
+    "nb_ns_stress": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _f64, _f64, _i64, _vp, _vp, _vp]),
+    "nb_nu_av": (_i32, [_vp, _vp, _vp, _vp, _f64, _i64, _vp]),
+    "nb_add": (_i32, [_vp, _vp, _vp, _vp, _i64, _vp]),
+    "nb_sum_into": (_i32, [_vp, _vp, _vp, _i64, _vp]),
 }
 
 
@@ -164,5 +173,36 @@ def box_arg(box):
     return (ctypes.c_double * 3)(*b)
 
 
+_TIMING = None  # name -> list of (start, end) events while timing is on
+
+
+def start_timing() -> None:
+    """Record a HIP event pair around every entry point called from now on (on the stream the
+    kernels run on: torch's current stream)."""
+    global _TIMING
+    _TIMING = {}
+
+
+def stop_timing() -> dict:
+    """Stop timing; returns name -> (calls, total milliseconds)."""
+    global _TIMING
+    import torch
+
+    torch.cuda.synchronize()
+    out = {k: (len(v), sum(a.elapsed_time(b) for a, b in v)) for k, v in (_TIMING or {}).items()}
+    _TIMING = None
+    return out
+
+
 def call(name: str, *args) -> int:
-    return check(getattr(lib(), name)(ctx(), stream(), *args), name)
+    if _TIMING is None:
+        return check(getattr(lib(), name)(ctx(), stream(), *args), name)
+    import torch
+
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    rc = check(getattr(lib(), name)(ctx(), stream(), *args), name)
+    b.record()
+    _TIMING.setdefault(name, []).append((a, b))
+    return rc

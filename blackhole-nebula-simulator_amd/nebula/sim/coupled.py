@@ -1,0 +1,310 @@
+"""Gas in the hole's gravity on the GPU: SPH coupled to the GN Hamiltonian through canonical
+momenta (coupled.py of the reference).
+
+The call surface is the reference's (coupled.py:62-270): ``CoupledConfig``, ``GasState``,
+``initialise``, ``step``, ``stable_dt``, ``ledger``. The state lives in GPU memory for the whole
+run (torch tensors; ``GasState.to_numpy()`` copies it out), and every operation of the step is
+a HIP kernel: half kicks, GN generalized-leapfrog flow, cooling, neighbour lists, density,
+equation of state, tightening, viscosity switch, forces, time-step minima and the ledger.
+The host only sequences the launches and reads back a few scalars (list sizes, rebuild
+counts, the step's minima).
+
+Each step (coupled.py:238-258): hydro half-kick; GN flow over dt; cooling in the middle of
+the split; hydro forces at the new positions with the predicted velocity; second half-kick.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field, replace
+
+import numpy as np
+import torch
+
+from .. import _dev, _hip
+from ..physics import units
+from .fastphys import gn_v_from_p_device
+from .hydro import HydroConfig
+from .integrator import glf_device
+from .neighbours import tighten
+from .sph import (
+    COND_VELOCITY,
+    compute_density_device,
+    density_device,
+    disc_forces_device,
+    forces_device,
+    gradients_device,
+    ns_forces_device,
+    ns_gradients_device,
+    velocity_gradients_device,
+)
+
+H_MAX_HOLE = 200.0
+BALSARA_EPS = 1e-4
+
+
+@dataclass(frozen=True)
+class CoupledConfig:
+    hydro: HydroConfig = HydroConfig(conductivity_signal=COND_VELOCITY, alpha_u=0.05, h_max=H_MAX_HOLE)
+    hole: bool = True
+    eta_orb: float = 0.00125
+    wrong_coupling: bool = False
+    gravity: str = "gn"
+    cooling_beta: float = math.inf
+    t_floor: float = units.T_FLOOR
+    # addition of this implementation: optically thin power-law cooling (rho^2 T^1/2)
+    cooling_powerlaw: bool = False
+
+
+def baseline_config(**hydro_overrides) -> CoupledConfig:
+    """The plain method of the cloud workloads: cubic spline, eta 1.2, Monaghan viscosity with
+    alpha = 1 and beta = 2 (no switch, no reconstruction), no conductivity, h_max 200."""
+    hc = HydroConfig(kernel="cubic", eta=1.2, h_max=H_MAX_HOLE, switch=False, alpha_const=1.0, beta=2.0,
+                     reconstruction=False, conductivity=False)  # fmt: skip
+    if hydro_overrides:
+        hc = replace(hc, **hydro_overrides)
+    return CoupledConfig(hydro=hc)
+
+
+@dataclass
+class GasState:
+    t: float
+    x: torch.Tensor
+    p: torch.Tensor
+    u: torch.Tensor
+    mass: torch.Tensor
+    h: torch.Tensor
+    alpha: torch.Tensor
+    a_hydro: torch.Tensor = field(default=None)
+    dudt: torch.Tensor = field(default=None)
+    rho: torch.Tensor = field(default=None)
+    c_s: torch.Tensor = field(default=None)
+    vsig_max: torch.Tensor = field(default=None)
+    neighbour_cache: object = field(default=None)
+    e_radiated_dev: torch.Tensor = field(default=None)  # (1,) device accumulator
+    visc_heating: torch.Tensor = field(default=None)
+    nu: torch.Tensor = field(default=None)
+    omega: torch.Tensor = field(default=None)
+    iterations: torch.Tensor = field(default=None)
+    flags: torch.Tensor = field(default=None)
+    m_uniform: float = 0.0
+    list_capacity: int = 0
+    builds: int = 0
+    n_search_entries: int = 0  # search-list entries of the last density pass (incl. self)
+    n_tight_entries: int = 0  # tightened-list entries of the last force pass (incl. self)
+
+    @property
+    def e_radiated(self) -> float:
+        return 0.0 if self.e_radiated_dev is None else float(self.e_radiated_dev.item())
+
+    def to_numpy(self) -> dict:
+        out = {}
+        for k in ("x", "p", "u", "mass", "h", "alpha", "a_hydro", "dudt", "rho", "c_s", "vsig_max", "visc_heating",
+                  "nu", "omega", "iterations", "flags"):  # fmt: skip
+            v = getattr(self, k)
+            out[k] = None if v is None else _dev.host(v)
+        out["t"] = self.t
+        out["e_radiated"] = self.e_radiated
+        return out
+
+
+def _gn(cfg: CoupledConfig) -> bool:
+    if cfg.gravity not in ("gn",):
+        raise ValueError(f"unsupported gravity {cfg.gravity!r} (the GPU step implements the GN potential)")
+    if cfg.wrong_coupling:
+        raise ValueError("the wrong-coupling comparison is not implemented on the GPU")
+    return cfg.hole and cfg.gravity == "gn"
+
+
+def _axpy(y, a, s, out=None):
+    out = torch.empty_like(y) if out is None else out
+    _hip.call("nb_axpy", _hip.ptr(y), _hip.ptr(a), float(s), y.numel(), _hip.ptr(out))
+    return out
+
+
+def _full(n, v):
+    t = _dev.empty(n)
+    _hip.call("nb_fill", _hip.ptr(t), float(v), n)
+    return t
+
+
+def velocity(cfg: CoupledConfig, x, p):
+    """v = dH/dp (coupled.py:104-109)."""
+    if not _gn(cfg):
+        return p
+    return gn_v_from_p_device(x, p)
+
+
+def p_from_v(cfg: CoupledConfig, x: np.ndarray, v: np.ndarray) -> np.ndarray:
+    """Canonical momentum from velocity (gravity.py:66-77), on the host at set-up."""
+    if not _gn(cfg):
+        return np.array(v, float)
+    r = np.sqrt(np.sum(x * x, axis=-1, keepdims=True))
+    f = r / (r - 2.0)
+    g = 2.0 * r / (r - 2.0) ** 2
+    rdot = np.sum(x * v, axis=-1, keepdims=True) / r
+    return f * v + g * rdot * x / r
+
+
+def gravity_flow(cfg: CoupledConfig, x, p, dt):
+    """coupled.py:131-142 (GN generalized leapfrog, or a straight drift without the hole)."""
+    if _gn(cfg):
+        x1, p1, _ = glf_device(x, p, float(dt))
+    else:
+        x1 = _axpy(x, p, dt)
+        p1 = p.clone()
+    if cfg.hydro.box is not None:
+        raise ValueError("periodic boxes are not supported by the coupled GPU step")
+    return x1, p1
+
+
+def _hydro(st: GasState, cfg: CoupledConfig, v, dt_alpha: float, a_prev) -> GasState:
+    """coupled.py:145-195 on device tensors."""
+    hc = cfg.hydro
+    if hc.isothermal_cs is not None or hc.eos != "ideal":
+        raise ValueError("the GPU step implements the ideal-gas equation of state")
+    n = int(st.x.shape[0])
+    d = compute_density_device(st.x, st.mass, st.h.clone(), None, hc.kernel, hc.eta, 1e-4, hc.h_max, hc.skin,
+                               m_uniform=st.m_uniform, capacity_hint=st.list_capacity)  # fmt: skip
+    p_gas = _dev.empty(n)
+    cs = _dev.empty(n)
+    nbad = torch.zeros(1, dtype=torch.int32, device=st.x.device)
+    _hip.call("nb_eos_ideal", _hip.ptr(d.rho), _hip.ptr(st.u), float(hc.gamma), n, _hip.ptr(p_gas), _hip.ptr(cs),
+              _hip.ptr(nbad))  # fmt: skip
+    bad = int(nbad.item())
+    if bad:
+        raise FloatingPointError(f"non-positive internal energy at {bad} particles")
+    nb = tighten(d.neighbours, st.x, d.h)
+    alpha = st.alpha
+    if hc.viscosity and hc.switch and dt_alpha > 0:
+        g = gradients_device(st.x, v, a_prev, st.mass, d.h, nb, hc.kernel)
+        alpha = _dev.empty(n)
+        _hip.call("nb_alpha_switch", _hip.ptr(g.div_v), _hip.ptr(g.curl_v), _hip.ptr(g.ddivv_dt), _hip.ptr(cs),
+                  _hip.ptr(d.h), _hip.ptr(st.alpha), float(dt_alpha), 1.0, 0.1, n, _hip.ptr(alpha))  # fmt: skip
+    elif hc.viscosity and not hc.switch:
+        if hc.balsara:
+            g = gradients_device(st.x, v, None, st.mass, d.h, nb, hc.kernel)
+            alpha = _dev.empty(n)
+            _hip.call("nb_alpha_balsara", _hip.ptr(g.div_v), _hip.ptr(g.curl_v), _hip.ptr(cs), _hip.ptr(d.h),
+                      float(hc.alpha_const), BALSARA_EPS, n, _hip.ptr(alpha))  # fmt: skip
+        else:
+            alpha = _full(n, hc.alpha_const)
+    shock = hc.shock()
+    gv = None
+    if shock.reconstruction:
+        gv = torch.zeros((n, 3, 3), dtype=torch.float64, device=st.x.device)
+        velocity_gradients_device(st.x, v, st.mass, d.h, nb, hc.kernel, gv)
+    f = forces_device(st.x, v, st.mass, d.h, d.rho, d.omega, p_gas, nb, hc.kernel,
+                      st.u if hc.conductivity else None, cs if hc.viscosity else None,
+                      alpha if hc.viscosity else None, shock, gv_d=gv, precision=hc.precision)  # fmt: skip
+    acc, dudt = f.acc, f.dudt
+    nu = None
+    if (hc.ns_nu > 0) + (hc.ns_alpha > 0) + (hc.disc_alpha_av > 0) > 1:
+        raise ValueError("set only one of ns_nu, ns_alpha, disc_alpha_av")
+    if hc.ns_nu > 0 or hc.ns_alpha > 0:
+        g = torch.zeros((n, 3, 3), dtype=torch.float64, device=st.x.device)
+        ns_gradients_device(st.x, v, st.mass, d.h, d.rho, d.omega, nb, hc.kernel, g)
+        nu = _dev.empty(n)
+        stress = _dev.empty((n, 3, 3))
+        heat = _dev.empty(n)
+        _hip.call("nb_ns_stress", _hip.ptr(g), _hip.ptr(d.rho), _hip.ptr(cs), _hip.ptr(st.x), float(hc.ns_alpha),
+                  float(hc.ns_nu), n, _hip.ptr(nu), _hip.ptr(stress), _hip.ptr(heat))  # fmt: skip
+        a_ns = ns_forces_device(st.x, st.mass, d.h, d.rho, d.omega, stress, nb, hc.kernel)
+        _hip.call("nb_add", _hip.ptr(acc), _hip.ptr(a_ns), 3 * n, _hip.ptr(acc))
+        _hip.call("nb_add", _hip.ptr(dudt), _hip.ptr(heat), n, _hip.ptr(dudt))
+    if hc.disc_alpha_av > 0:
+        a_disc, u_disc = disc_forces_device(st.x, v, st.mass, d.h, d.rho, d.omega, cs, hc.disc_alpha_av, nb,
+                                            hc.kernel)  # fmt: skip
+        _hip.call("nb_add", _hip.ptr(acc), _hip.ptr(a_disc), 3 * n, _hip.ptr(acc))
+        _hip.call("nb_add", _hip.ptr(dudt), _hip.ptr(u_disc), n, _hip.ptr(dudt))
+        nu = _dev.empty(n)
+        _hip.call("nb_nu_av", _hip.ptr(cs), _hip.ptr(d.h), float(hc.disc_alpha_av), n, _hip.ptr(nu))
+    return replace(
+        st, h=d.h, alpha=alpha, a_hydro=acc, dudt=dudt, rho=d.rho, c_s=cs,
+        vsig_max=f.vsig_max if hc.viscosity else cs, visc_heating=f.dudt_visc, nu=nu, omega=d.omega,
+        iterations=d.iterations, flags=d.flags, builds=d.builds,
+        list_capacity=max(st.list_capacity, d.neighbours.n_entries), n_search_entries=d.neighbours.n_entries,
+        n_tight_entries=nb.n_entries,
+    )  # fmt: skip
+
+
+def initialise(x, v, u, mass, h_guess, cfg: CoupledConfig, t: float = 0.0) -> GasState:
+    """Start from positions and velocities, p computed from v (coupled.py:198-210)."""
+    x = np.array(x, float)
+    n = len(x)
+    v = np.array(v, float)
+    p = p_from_v(cfg, x, v)
+    hc = cfg.hydro
+    mass_h = np.array(mass, float) * np.ones(n)
+    m_uniform = float(mass_h[0]) if n and np.all(mass_h == mass_h[0]) and mass_h[0] > 0 else 0.0
+    st = GasState(
+        t, _dev.f64(x), _dev.f64(p), _dev.f64(np.array(u, float) * np.ones(n)), _dev.f64(mass_h),
+        _dev.f64(np.array(h_guess, float) * np.ones(n)),
+        _dev.f64(np.full(n, hc.alpha_init if hc.switch else hc.alpha_const)),
+        e_radiated_dev=torch.zeros(1, dtype=torch.float64, device=_hip.torch_device()), m_uniform=m_uniform,
+    )  # fmt: skip
+    x_d, p_d = st.x, st.p
+    return _hydro(st, cfg, velocity(cfg, x_d, p_d), 0.0, None)
+
+
+def stable_dt(st: GasState, cfg: CoupledConfig) -> float:
+    """Shared step: min over particles of the Courant, force, viscous and orbit criteria
+    (coupled.py:213-226). The minima are taken on the device (exact in any order); r^1.5 of the
+    smallest radius equals the smallest r^1.5 (pow is monotone)."""
+    hc = cfg.hydro
+    n = int(st.x.shape[0])
+    out = np.zeros(4)
+    _hip.call("nb_dt_minima", _hip.ptr(st.h), _hip.ptr(st.vsig_max), _hip.ptr(st.a_hydro), _hip.ptr(st.nu), 0.25,
+              _hip.ptr(st.x), n, out.ctypes.data_as(_hip._PD))  # fmt: skip
+    dt = hc.courant * out[0]
+    if np.isfinite(out[1]):
+        dt = min(dt, hc.c_force * out[1])
+    if st.nu is not None:
+        dt = min(dt, float(out[2]))
+    if cfg.hole:
+        dt = min(dt, cfg.eta_orb * out[3] ** 1.5)
+    return float(dt)
+
+
+def step(st: GasState, cfg: CoupledConfig, dt: float) -> GasState:
+    """One Strang-split step (coupled.py:238-258), with optional power-law cooling in the same
+    place of the split as the reference's beta cooling."""
+    hc = cfg.hydro
+    n = int(st.x.shape[0])
+    half = 0.5 * dt
+    p_half = _axpy(st.p, st.a_hydro, half)
+    u_half = _axpy(st.u, st.dudt, half)
+    x1, p_flow = gravity_flow(cfg, st.x, p_half, dt)
+    e_rad = st.e_radiated_dev.clone()
+    if math.isfinite(cfg.cooling_beta) or cfg.cooling_powerlaw:
+        u_floor = units.ideal_u_floor(cfg.t_floor, hc.gamma)
+        e_part = _dev.empty(n)
+        if math.isfinite(cfg.cooling_beta):
+            u_cool = _dev.empty(n)
+            _hip.call("nb_cool_beta", _hip.ptr(u_half), _hip.ptr(st.x), _hip.ptr(x1), _hip.ptr(st.mass),
+                      float(u_floor), float(dt), float(cfg.cooling_beta), n, _hip.ptr(u_cool),
+                      _hip.ptr(e_part))  # fmt: skip
+            _hip.call("nb_sum_into", _hip.ptr(e_part), n, _hip.ptr(e_rad))
+            u_half = u_cool
+        if cfg.cooling_powerlaw:
+            u_cool = _dev.empty(n)
+            _hip.call("nb_cool_powerlaw", _hip.ptr(u_half), _hip.ptr(st.rho), _hip.ptr(st.mass), float(u_floor),
+                      float(units.powerlaw_cooling_coefficient(hc.gamma)), float(dt), n, _hip.ptr(u_cool),
+                      _hip.ptr(e_part))  # fmt: skip
+            _hip.call("nb_sum_into", _hip.ptr(e_part), n, _hip.ptr(e_rad))
+            u_half = u_cool
+    p_pred = _axpy(p_flow, st.a_hydro, half)
+    pred = replace(st, t=st.t + dt, x=x1, p=p_flow, u=_axpy(u_half, st.dudt, half))
+    new = _hydro(pred, cfg, velocity(cfg, x1, p_pred), dt, st.a_hydro)
+    return replace(new, p=_axpy(p_flow, new.a_hydro, half), u=_axpy(u_half, new.dudt, half), e_radiated_dev=e_rad)
+
+
+def ledger(st: GasState, cfg: CoupledConfig) -> dict:
+    """Conserved totals (coupled.py:261-270): E = sum m (H + u) + E_radiated, sum m |H|, and the
+    angular momentum sum m (x cross p). Device sums, compensated over the block partials."""
+    n = int(st.x.shape[0])
+    out = np.zeros(6)
+    _hip.call("nb_ledger", _hip.ptr(st.x), _hip.ptr(st.p), _hip.ptr(st.u), _hip.ptr(st.mass), None, int(_gn(cfg)), n,
+              out.ctypes.data_as(_hip._PD))  # fmt: skip
+    e_rad = st.e_radiated
+    return {"energy": float(out[0]) + e_rad, "e_radiated": e_rad, "abs_h": float(out[1]), "angmom": out[2:5].copy()}

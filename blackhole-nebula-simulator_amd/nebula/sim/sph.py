@@ -113,30 +113,27 @@ def compute_density(pos, mass, h_guess, box=None, kernel_name=DEFAULT_KERNEL, et
 
 def compute_density_device(pos_d, mass_d, h_guess_d, box_arr, kernel_name, eta, tol, h_max, skin, rows=None,
                            m_uniform=None, capacity_hint=0) -> DensityResult:  # fmt: skip
-    """compute_density on device tensors; h_guess_d may be modified."""
+    """compute_density on device tensors (h_guess_d is overwritten by the rebuild rounds)."""
     n = int(pos_d.shape[0])
     rows_d, nrows = _dev.rows_dev(rows, n)
     box_cap = np.inf if box_arr is None else 0.2499 * float(np.min(box_arr))
-    h_search = torch.clamp(torch.clamp(h_guess_d, max=h_max) * skin, max=box_cap) if np.isfinite(
-        h_max) else torch.clamp(h_guess_d * skin, max=box_cap)  # fmt: skip
+    h_search = _dev.empty(n)
+    _hip.call("nb_search_lengths", _hip.ptr(h_guess_d), float(h_max), float(skin), float(box_cap), n,
+              _hip.ptr(h_search))  # fmt: skip
     if m_uniform is None:
         m_uniform = _dev.uniform_mass(mass_d)
     builds = 0
+    grown = np.zeros(1, np.int64)
     for _ in range(12):
         nb = build_lists(pos_d, h_search, box_arr, capacity_hint)
         builds += 1
         h, rho, omega, it, fl = density_device(pos_d, mass_d, h_guess_d, nb, h_search, kernel_name, eta, tol,
                                                h_max, rows_d, nrows, m_uniform)  # fmt: skip
-        grow = fl == FLAG_SEARCH_TOO_SMALL
-        if not bool(torch.any(grow)):
+        _hip.call("nb_grow_search", _hip.ptr(fl), _hip.ptr(h), _hip.ptr(rows_d), nrows, float(box_cap),
+                  _hip.ptr(h_search), _hip.ptr(h_guess_d), grown.ctypes.data_as(_hip._P64))  # fmt: skip
+        if grown[0] == 0:
             return DensityResult(h, rho, omega, it, fl, nb, None, builds)
-        idx = rows_d[grow] if rows_d is not None else torch.nonzero(grow).flatten()
-        h_search = h_search.clone()
-        h_search[idx] = torch.clamp(h_search[idx] * 2.0, max=box_cap)
-        if rows_d is not None:
-            h_guess_d[rows_d] = h
-        else:
-            h_guess_d = h.clone()
+        capacity_hint = max(capacity_hint, nb.n_entries)
     raise RuntimeError("smoothing lengths kept outgrowing the neighbour search")
 
 

@@ -166,6 +166,35 @@ int32_t nb_dt_minima(nb_ctx *ctx, void *stream, const double *h, const double *v
 int32_t nb_ledger(nb_ctx *ctx, void *stream, const double *x, const double *p, const double *u,
                   const double *mass, const double *e_part, int32_t gn, int64_t n, double *out6);
 
+/* ------------------------------------------------------------------ element-wise step glue */
+/* search lengths min(min(h, h_max) * skin, box_cap) (sph.py:371). */
+int32_t nb_search_lengths(nb_ctx *ctx, void *stream, const double *h, double h_max, double skin, double box_cap,
+                          int64_t n, double *out);
+/* the rebuild round of compute_density (sph.py:382-386): flagged rows (flag 2) double their search
+ * (capped), every row's next starting h is its result; *grown_out = number of flagged rows. */
+int32_t nb_grow_search(nb_ctx *ctx, void *stream, const int32_t *flags, const double *h_res, const int64_t *rows,
+                       int64_t nrows, double box_cap, double *h_search, double *h_guess, int64_t *grown_out);
+int32_t nb_fill(nb_ctx *ctx, void *stream, double *p, double v, int64_t n);
+/* Balsara limiter on the viscosity's sound-speed part (no counterpart in the reference, which
+ * has only the Cullen-Dehnen switch): alpha = alpha_const |div v| / (|div v| + |curl v| + eps c_s/h). */
+int32_t nb_alpha_balsara(nb_ctx *ctx, void *stream, const double *div, const double *curl, const double *cs,
+                         const double *h, double alpha_const, double eps, int64_t n, double *alpha);
+/* Cullen-Dehnen switch update (viscosity.py:82-106 as used at coupled.py:159-162). */
+int32_t nb_alpha_switch(nb_ctx *ctx, void *stream, const double *div, const double *curl, const double *ddiv,
+                        const double *cs, const double *h, const double *alpha_in, double dt, double alpha_max,
+                        double sigma, int64_t n, double *alpha_out);
+/* Navier-Stokes glue (navier_stokes.py:29-50 and hydro.py:221-230): nu, viscous stress (N,3,3),
+ * viscous heating (N). alpha_ss > 0: nu = alpha_ss c_s^2 |x|^1.5, else nu = nu_const. */
+int32_t nb_ns_stress(nb_ctx *ctx, void *stream, const double *g, const double *rho, const double *cs,
+                     const double *x, double alpha_ss, double nu_const, int64_t n, double *nu, double *stress,
+                     double *heat);
+/* nu_AV = alpha_AV c_s h / 10 (disc_viscosity.py:46-48). */
+int32_t nb_nu_av(nb_ctx *ctx, void *stream, const double *cs, const double *h, double alpha_av, int64_t n,
+                 double *nu);
+int32_t nb_add(nb_ctx *ctx, void *stream, const double *a, const double *b, int64_t m, double *out);
+/* acc_dev[0] += sum(x) on the device (no host synchronization). */
+int32_t nb_sum_into(nb_ctx *ctx, void *stream, const double *x, int64_t n, double *acc_dev);
+
 #ifdef __cplusplus
 }
 #endif

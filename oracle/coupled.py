@@ -180,7 +180,7 @@ def stable_dt(st: State, cfg) -> float:
         dt = min(dt, float(np.min(ph.viscous_dt(st.h, st.nu))))
     if cfg.hole:
         r = np.linalg.norm(st.x, axis=1)
-        dt = min(dt, cfg.eta_orb * np.min(r**1.5))
+        dt = min(dt, cfg.eta_orb * float(np.min(r)) ** 1.5)  # = min(r**1.5): pow is monotone
     return float(dt)
 
 

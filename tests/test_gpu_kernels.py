@@ -244,7 +244,7 @@ def test_glf_and_v_from_p_bit_identical(rng):
     v = rng.normal(0, 1, (n, 3)) * np.sqrt(1.0 / r)[:, None] * 0.5
     p = oph.gn_p_from_v(x, v)
     dt = 0.005 * r**1.5
-    for dtv in (dt, float(np.median(dt))):
+    for dtv in (dt, float(np.min(dt))):
         x1r, p1r, itr = oi.glf_step_many(x, p, dtv)
         x1, p1, it = glf_step_many(x, p, dtv)
         bit_equal("GLF x1", x1r, x1)

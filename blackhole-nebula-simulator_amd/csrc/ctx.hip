@@ -44,7 +44,7 @@ extern "C" void nb_ctx_destroy(nb_ctx* ctx)
 {
     if (!ctx) return;
     void* bufs[] = {ctx->cell_id, ctx->keys_out, ctx->iota, ctx->order, ctx->cell_count, ctx->cell_start,
-                    ctx->gbuf, ctx->rowcnt, ctx->ecount, ctx->eoff, ctx->efill, ctx->ebuf, ctx->tot,
+                    ctx->gbuf, ctx->rowcnt, ctx->unsorted, ctx->pending, ctx->edbuf, ctx->ecount, ctx->eoff, ctx->efill, ctx->ebuf, ctx->tot,
                     ctx->offs, ctx->tcount, ctx->dscratch, ctx->fscratch, ctx->red, ctx->scal, ctx->cub_tmp};
     for (void* b : bufs)
         if (b) hipFree(b);
@@ -59,7 +59,7 @@ extern "C" int64_t nb_ctx_scratch_bytes(const nb_ctx* ctx)
 {
     if (!ctx) return 0;
     size_t s = ctx->cell_id_bytes + ctx->keys_out_bytes + ctx->iota_bytes + ctx->order_bytes +
-               ctx->cell_count_bytes + ctx->cell_start_bytes + ctx->gbuf_bytes + ctx->rowcnt_bytes +
+               ctx->cell_count_bytes + ctx->cell_start_bytes + ctx->gbuf_bytes + ctx->rowcnt_bytes + ctx->unsorted_bytes + ctx->pending_bytes + ctx->edbuf_bytes +
                ctx->ecount_bytes + ctx->eoff_bytes + ctx->efill_bytes + ctx->ebuf_bytes + ctx->tot_bytes +
                ctx->offs_bytes + ctx->tcount_bytes + ctx->dscratch_bytes + ctx->fscratch_bytes +
                ctx->red_bytes + ctx->scal_bytes + ctx->cub_tmp_bytes;

@@ -1,4 +1,4 @@
-// Neighbour lists on the GPU: cell grid, gather, symmetric union, tighten.
+// Neighbour lists on the GPU: cell grid, gather, symmetric union, tighten, grid ordering.
 //
 // The lists are those of the reference's find_neighbours_numba (neighbours.py:240-281):
 // row i holds every j with d2 < (2 max(h_i, h_j))^2, where d2 = (dx*dx + dy*dy) + dz*dz in
@@ -8,20 +8,28 @@
 // tested, and its cell ranges are conservative (a candidate passing the predicate always lies
 // in the range, by monotonicity of the rounded floor((x -+ r - lo) / cell)).
 //
-// Pipeline (all on one stream):
-//   k_bbox/k_bbox_final    bounding box and the largest / mean search length
-//   k_cell_id              dense raster cell id per particle
-//   radix sort (hipCUB)    particles ordered by cell, ascending index within a cell
-//   k_count_cells + scan   cell start offsets
-//   k_gather               one work-item per particle: candidates from the cell range, the
-//                          predicate with r = 2 h_i, written to a padded row; pairs that j's own
-//                          gather will not see (d2 >= (2 h_j)^2) are flagged and counted for j
-//   k_sort_rows            each padded row sorted ascending (insertion sort: rows arrive
-//                          almost sorted when particles are stored in cell order)
-//   k_extras_fill          the flagged pairs transposed into per-row extra lists
-//   k_merge_emit           sorted gather row merged with its sorted extras -> CSR
+// Row layout. Gathered rows, and the lists handed to the pair passes, are stored in 64-row
+// blocks: entry k of row i sits at ((i / 64) * cap + k) * 64 + i % 64, so when the 64 lanes of
+// a wavefront walk their own rows in step, every load and store of entry k is one contiguous
+// 256-byte transaction. The CSR form (int64 offsets) is produced for callers that want the
+// reference's Neighbours arrays.
+//
+// Grid. Cells are cubes of about the mean search length (at least 1/2048 of the widest span,
+// at most 2^27 cells), numbered in raster order (x slowest). The mean is an order-independent
+// fixed-point sum (each h quantised to h / h_max * 2^32), so the grid, and the particle order it
+// induces, do not depend on the reduction order. Candidates are visited cell by cell in raster
+// order and, inside a cell, in ascending index: when the particles are stored in this grid's
+// order (nb_grid_order), every gathered row comes out sorted and the sort pass does no work.
+//
+// Pipeline (one stream): k_bbox / k_bbox_final / k_hsum (grid shape), k_cell_id, radix sort
+// (hipCUB: by cell, ascending index inside a cell), k_count via atomics + scan (cell starts),
+// k_gather (one work-item per particle; pairs that j's own gather will not see, d2 >= (2 h_j)^2,
+// are flagged and counted for j), k_sort_rows (only rows that came out unsorted), k_extras_fill
+// (the flagged pairs transposed into per-row extra lists), k_merge_* (sorted gather row merged
+// with its sorted extras, into CSR or blocked rows).
 #include "nb_common.h"
 #include "nb_ctx.h"
+#include "nebula_hip.h"
 
 #include <hipcub/hipcub.hpp>
 
@@ -34,6 +42,8 @@ namespace nb {
 static constexpr int kBlock = 256;
 static constexpr uint32_t kFlag = 0x80000000u;
 static constexpr uint32_t kMask = 0x7fffffffu;
+
+__device__ __forceinline__ int64_t blk(int64_t i, int64_t k, int64_t cap) { return ((i >> 6) * cap + k) * 64 + (i & 63); }
 
 // numpy's np.mod for floats (npy_divmod): result has the sign of the divisor
 __device__ __forceinline__ double np_mod(double a, double b)
@@ -53,50 +63,80 @@ __device__ __forceinline__ double wrapped(const double* pos, int64_t i, int c, c
     return box.l[c] > 0.0 ? np_mod(x, box.l[c]) : x;
 }
 
-// ----------------------------------------------------------------- bounding box
-// out per block: minx, miny, minz, maxx, maxy, maxz, maxh, sumh
-__global__ void k_bbox(const double* __restrict__ pos, const double* __restrict__ h, int64_t n,
-                       Box box, double* __restrict__ part)
+// ----------------------------------------------------------------- grid shape
+// partial rows per block: minx, miny, minz, maxx, maxy, maxz, maxh, (unused)
+__global__ void k_bbox(const double* __restrict__ pos, const double* __restrict__ h, int64_t n, Box box,
+                       double* __restrict__ part)
 {
-    double v[8] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0, 0.0};
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
+    double v[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0};
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         for (int c = 0; c < 3; c++) {
             double x = wrapped(pos, i, c, box);
             v[c] = fmin(v[c], x);
             v[3 + c] = fmax(v[3 + c], x);
         }
         v[6] = fmax(v[6], h[i]);
-        v[7] += h[i];
     }
-    __shared__ double sh[8][kBlock];
-    for (int k = 0; k < 8; k++) sh[k][threadIdx.x] = v[k];
+    __shared__ double sh[7][kBlock];
+    for (int k = 0; k < 7; k++) sh[k][threadIdx.x] = v[k];
     __syncthreads();
     for (int s = blockDim.x / 2; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) {
             for (int k = 0; k < 3; k++) sh[k][threadIdx.x] = fmin(sh[k][threadIdx.x], sh[k][threadIdx.x + s]);
             for (int k = 3; k < 7; k++) sh[k][threadIdx.x] = fmax(sh[k][threadIdx.x], sh[k][threadIdx.x + s]);
-            sh[7][threadIdx.x] += sh[7][threadIdx.x + s];
         }
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        for (int k = 0; k < 8; k++) part[8 * blockIdx.x + k] = sh[k][0];
+        for (int k = 0; k < 7; k++) part[8 * blockIdx.x + k] = sh[k][0];
 }
 
-__global__ void k_bbox_final(const double* __restrict__ part, int nparts, double* __restrict__ out)
+// out[0..6] = bbox + max h; out[7] = 0 (the fixed-point sum lands in the u64 after it).
+// One block; min / max are exact in any order.
+__global__ void k_bbox_final(const double* __restrict__ part, int nparts, double* __restrict__ out,
+                             unsigned long long* __restrict__ hsum)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double v[8] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0, 0.0};
-    for (int b = 0; b < nparts; b++) {
+    double v[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0};
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
         for (int k = 0; k < 3; k++) v[k] = fmin(v[k], part[8 * b + k]);
         for (int k = 3; k < 7; k++) v[k] = fmax(v[k], part[8 * b + k]);
-        v[7] += part[8 * b + 7];
     }
-    for (int k = 0; k < 8; k++) out[k] = v[k];
+    __shared__ double sh[7][kBlock];
+    for (int k = 0; k < 7; k++) sh[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            for (int k = 0; k < 3; k++) sh[k][threadIdx.x] = fmin(sh[k][threadIdx.x], sh[k][threadIdx.x + s]);
+            for (int k = 3; k < 7; k++) sh[k][threadIdx.x] = fmax(sh[k][threadIdx.x], sh[k][threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 7; k++) out[k] = sh[k][0];
+        out[7] = 0.0;
+        *hsum = 0ull;
+    }
 }
 
-// ----------------------------------------------------------------- grid
+// order-independent sum of floor(h_i * (2^32 / h_max)) (integer additions commute exactly)
+__global__ void k_hsum(const double* __restrict__ h, int64_t n, const double* __restrict__ bb,
+                       unsigned long long* __restrict__ hsum)
+{
+    const double hmax = bb[6];
+    const double scale = hmax > 0.0 ? 4294967296.0 / hmax : 0.0;
+    unsigned long long s = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        s += (unsigned long long)(h[i] * scale);
+    __shared__ unsigned long long sh[kBlock];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(hsum, sh[0]);
+}
+
 struct Grid {
     double lo[3];
     double inv[3];
@@ -106,15 +146,13 @@ struct Grid {
 __device__ __forceinline__ int32_t cell_of(double x, const Grid& g, int c)
 {
     double f = floor((x - g.lo[c]) * g.inv[c]);
-    int64_t k = (int64_t)f;
-    if (f < 0.0) k = 0;  // also catches values far below the int range
-    if (k >= g.nc[c]) k = g.nc[c] - 1;
-    return (int32_t)k;
+    if (!(f >= 0.0)) return 0;
+    if (f >= (double)g.nc[c]) return g.nc[c] - 1;
+    return (int32_t)f;
 }
 
-__global__ void k_cell_id(const double* __restrict__ pos, int64_t n, Box box, Grid g,
-                          uint32_t* __restrict__ cell_id, uint32_t* __restrict__ iota,
-                          uint32_t* __restrict__ cell_count)
+__global__ void k_cell_id(const double* __restrict__ pos, int64_t n, Box box, Grid g, uint32_t* __restrict__ cell_id,
+                          uint32_t* __restrict__ iota, uint32_t* __restrict__ cell_count)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -129,14 +167,18 @@ __global__ void k_cell_id(const double* __restrict__ pos, int64_t n, Box box, Gr
 
 // neighbours.py:129-145: first cell and count along one axis; periodic searches that span the
 // whole axis visit every cell once.
-__device__ __forceinline__ void cell_range(double xc, double r, const Grid& g, int c, bool periodic,
-                                           int64_t& a_out, int64_t& cnt_out)
+__device__ __forceinline__ void cell_range(double xc, double r, const Grid& g, int c, bool periodic, int64_t& a_out,
+                                           int64_t& cnt_out)
 {
     double fa = floor((xc - r - g.lo[c]) * g.inv[c]);
     double fb = floor((xc + r - g.lo[c]) * g.inv[c]);
     if (periodic) {
         double count = fb - fa + 1.0;
-        if (count >= (double)g.nc[c]) { a_out = 0; cnt_out = g.nc[c]; return; }
+        if (count >= (double)g.nc[c]) {
+            a_out = 0;
+            cnt_out = g.nc[c];
+            return;
+        }
         a_out = (int64_t)fa;
         cnt_out = (int64_t)count;
         return;
@@ -153,15 +195,16 @@ __device__ __forceinline__ int32_t wrap_cell(int64_t raw, int32_t nc)
     return (int32_t)(m < 0 ? m + nc : m);
 }
 
-// One work-item per particle. Row i of the padded buffer holds the gather set
-// {j : d2 < (2 h_i)^2}; entries carry kFlag when d2 >= (2 h_j)^2 (pair unseen from j's side),
-// and ecount[j] counts those. rowcnt[i] is the true count even when it exceeds cap (then the
-// row is truncated and the caller rebuilds with a larger cap; max_count records the largest).
+// ----------------------------------------------------------------- gather
+// One work-item per particle; row i (blocked, capacity cap) holds {j : d2 < (2 h_i)^2} in
+// traversal order; entries carry kFlag when d2 >= (2 h_j)^2 (pair unseen from j's side) and
+// ecount[j] counts those. rowcnt[i] is the true count even past cap (then the caller rebuilds
+// with a larger cap; scal[0] holds the largest). Rows that come out unsorted are marked.
 __global__ void __launch_bounds__(kBlock)
 k_gather(const double* __restrict__ pos, const double* __restrict__ h, int64_t n, Box box, Grid g,
-         const uint32_t* __restrict__ start, const uint32_t* __restrict__ order,
-         uint32_t* __restrict__ gbuf, int32_t cap, int32_t* __restrict__ rowcnt,
-         unsigned long long* __restrict__ ecount, int32_t* __restrict__ max_count)
+         const uint32_t* __restrict__ start, const uint32_t* __restrict__ order, uint32_t* __restrict__ gbuf,
+         int64_t cap, int32_t* __restrict__ rowcnt, uint8_t* __restrict__ unsorted,
+         unsigned long long* __restrict__ ecount, int32_t* __restrict__ scal)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -173,30 +216,33 @@ k_gather(const double* __restrict__ pos, const double* __restrict__ h, int64_t n
     cell_range(yi, r, g, 1, box.l[1] > 0.0, ay, ny);
     cell_range(zi, r, g, 2, box.l[2] > 0.0, az, nz);
     int32_t m = 0;
-    uint32_t* row = gbuf + (int64_t)i * cap;
+    int64_t last = -1;
+    bool sorted = true;
+    uint32_t* row = gbuf + blk(i, 0, cap);
     for (int64_t di = 0; di < nx; di++) {
-        int32_t cx = wrap_cell(ax + di, g.nc[0]);
+        const int32_t cx = wrap_cell(ax + di, g.nc[0]);
         for (int64_t dj = 0; dj < ny; dj++) {
-            int32_t cy = wrap_cell(ay + dj, g.nc[1]);
-            uint32_t base = ((uint32_t)cx * (uint32_t)g.nc[1] + (uint32_t)cy) * (uint32_t)g.nc[2];
+            const int32_t cy = wrap_cell(ay + dj, g.nc[1]);
+            const uint32_t base = ((uint32_t)cx * (uint32_t)g.nc[1] + (uint32_t)cy) * (uint32_t)g.nc[2];
             for (int64_t dk = 0; dk < nz; dk++) {
-                int32_t cz = wrap_cell(az + dk, g.nc[2]);
-                uint32_t c = base + (uint32_t)cz;
-                uint32_t k0 = start[c], k1 = start[c + 1];
-                for (uint32_t k = k0; k < k1; k++) {
-                    uint32_t j = order[k];
-                    double dx = min_image(wrapped(pos, j, 0, box) - xi, box.l[0]);
-                    double dy = min_image(wrapped(pos, j, 1, box) - yi, box.l[1]);
-                    double dz = min_image(wrapped(pos, j, 2, box) - zi, box.l[2]);
-                    double d2 = dx * dx + dy * dy + dz * dz;
+                const uint32_t c = base + (uint32_t)wrap_cell(az + dk, g.nc[2]);
+                const uint32_t k1 = start[c + 1];
+                for (uint32_t k = start[c]; k < k1; k++) {
+                    const uint32_t j = order[k];
+                    const double dx = min_image(wrapped(pos, j, 0, box) - xi, box.l[0]);
+                    const double dy = min_image(wrapped(pos, j, 1, box) - yi, box.l[1]);
+                    const double dz = min_image(wrapped(pos, j, 2, box) - zi, box.l[2]);
+                    const double d2 = dx * dx + dy * dy + dz * dz;
                     if (d2 < r2) {
-                        double rj = NB_R_KERN * h[j];
+                        const double rj = NB_R_KERN * h[j];
                         uint32_t e = j;
                         if (!(d2 < rj * rj)) {
                             e |= kFlag;
                             atomicAdd(&ecount[j], 1ull);
                         }
-                        if (m < cap) row[m] = e;
+                        if ((int64_t)j < last) sorted = false;
+                        last = j;
+                        if (m < cap) row[(int64_t)m * 64] = e;
                         m++;
                     }
                 }
@@ -204,70 +250,78 @@ k_gather(const double* __restrict__ pos, const double* __restrict__ h, int64_t n
         }
     }
     rowcnt[i] = m;
-    atomicMax(max_count, m);
+    unsorted[i] = sorted ? 0 : 1;
+    if (!sorted) atomicAdd(&scal[1], 1);
+    atomicMax(&scal[0], m);
 }
 
-// Insertion sort of each padded row by index (flag bit ignored). Rows arrive nearly sorted
-// when the particles are stored in cell order, which makes this linear in practice.
-__global__ void k_sort_rows(uint32_t* __restrict__ gbuf, int32_t cap, const int32_t* __restrict__ rowcnt,
-                            int64_t n)
+// Insertion sort of the rows marked unsorted (flag bit ignored in the key).
+__global__ void k_sort_rows(uint32_t* __restrict__ gbuf, int64_t cap, const int32_t* __restrict__ rowcnt,
+                            const uint8_t* __restrict__ unsorted, int64_t n)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t* row = gbuf + (int64_t)i * cap;
-    int32_t m = rowcnt[i];
+    if (i >= n || !unsorted[i]) return;
+    uint32_t* row = gbuf + blk(i, 0, cap);
+    const int32_t m = rowcnt[i];
     for (int32_t a = 1; a < m; a++) {
-        uint32_t v = row[a];
-        uint32_t key = v & kMask;
+        const uint32_t v = row[(int64_t)a * 64];
+        const uint32_t key = v & kMask;
         int32_t b = a - 1;
-        while (b >= 0 && (row[b] & kMask) > key) {
-            row[b + 1] = row[b];
+        while (b >= 0 && (row[(int64_t)b * 64] & kMask) > key) {
+            row[(int64_t)(b + 1) * 64] = row[(int64_t)b * 64];
             b--;
         }
-        row[b + 1] = v;
+        row[(int64_t)(b + 1) * 64] = v;
     }
 }
 
-__global__ void k_extras_fill(const uint32_t* __restrict__ gbuf, int32_t cap,
-                              const int32_t* __restrict__ rowcnt,
-                              const unsigned long long* __restrict__ eoff,
-                              unsigned int* __restrict__ efill, uint32_t* __restrict__ ebuf, int64_t n)
+__global__ void k_extras_fill(const uint32_t* __restrict__ gbuf, int64_t cap, const int32_t* __restrict__ rowcnt,
+                              const unsigned long long* __restrict__ eoff, unsigned int* __restrict__ efill,
+                              uint32_t* __restrict__ ebuf, int64_t n)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint32_t* row = gbuf + (int64_t)i * cap;
-    int32_t m = rowcnt[i];
+    const uint32_t* row = gbuf + blk(i, 0, cap);
+    const int32_t m = rowcnt[i];
     for (int32_t a = 0; a < m; a++) {
-        uint32_t v = row[a];
+        const uint32_t v = row[(int64_t)a * 64];
         if (v & kFlag) {
-            uint32_t j = v & kMask;
-            unsigned int s = atomicAdd(&efill[j], 1u);
+            const uint32_t j = v & kMask;
+            const unsigned int s = atomicAdd(&efill[j], 1u);
             ebuf[eoff[j] + s] = (uint32_t)i;
         }
     }
 }
 
+// final row lengths (gather + extras), their largest value (scal[2]) and, for the CSR scan,
+// tot[n] = 0.
 __global__ void k_row_total(const int32_t* __restrict__ rowcnt, const unsigned long long* __restrict__ ecount,
-                            unsigned long long* __restrict__ tot, int64_t n)
+                            unsigned long long* __restrict__ tot, int32_t* __restrict__ scal, int64_t n)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i < n) tot[i] = (unsigned long long)rowcnt[i] + ecount[i];
+    if (i < n) {
+        unsigned long long t = (unsigned long long)rowcnt[i] + ecount[i];
+        tot[i] = t;
+        atomicMax(&scal[2], (int32_t)t);
+    }
     if (i == n) tot[i] = 0;
 }
 
-// The extras of a row are sorted in place, then merged with the sorted gather row. The two
-// sets are disjoint by construction (an extra j of row i has d2 >= (2 h_i)^2).
-__global__ void k_merge_emit(const uint32_t* __restrict__ gbuf, int32_t cap,
-                             const int32_t* __restrict__ rowcnt, const unsigned long long* __restrict__ eoff,
-                             uint32_t* __restrict__ ebuf, const unsigned long long* __restrict__ off,
-                             int32_t* __restrict__ out_idx, int64_t out_cap, int64_t n)
+// Sorted extras of row i (sorted here, in place), merged with the sorted gather row. The sets are
+// disjoint by construction (an extra j of row i has d2 >= (2 h_i)^2). kBlocked: output rows in
+// the blocked layout (out_len, capacity cap_out); else CSR at off[i].
+template <bool kBlocked>
+__global__ void k_merge(const uint32_t* __restrict__ gbuf, int64_t cap, const int32_t* __restrict__ rowcnt,
+                        const unsigned long long* __restrict__ eoff, uint32_t* __restrict__ ebuf,
+                        const unsigned long long* __restrict__ off, int32_t* __restrict__ out_idx,
+                        int32_t* __restrict__ out_len, int64_t cap_out, int64_t n)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t* e = ebuf + eoff[i];
-    int64_t ne = (int64_t)(eoff[i + 1] - eoff[i]);
+    const int64_t ne = (int64_t)(eoff[i + 1] - eoff[i]);
     for (int64_t a = 1; a < ne; a++) {
-        uint32_t v = e[a];
+        const uint32_t v = e[a];
         int64_t b = a - 1;
         while (b >= 0 && e[b] > v) {
             e[b + 1] = e[b];
@@ -275,46 +329,49 @@ __global__ void k_merge_emit(const uint32_t* __restrict__ gbuf, int32_t cap,
         }
         e[b + 1] = v;
     }
-    const uint32_t* g = gbuf + (int64_t)i * cap;
-    int32_t ng = rowcnt[i];
-    int64_t o = (int64_t)off[i];
-    if (o + ng + ne > out_cap) return;
+    const uint32_t* g = gbuf + blk(i, 0, cap);
+    const int32_t ng = rowcnt[i];
+    if (ng + ne > cap_out) return;  // the caller asked with too small a capacity
+    int32_t* out = kBlocked ? out_idx + blk(i, 0, cap_out) : out_idx + (int64_t)off[i];
+    const int64_t stride = kBlocked ? 64 : 1;
     int32_t p = 0;
-    int64_t q = 0;
+    int64_t q = 0, o = 0;
     while (p < ng || q < ne) {
         uint32_t v;
-        if (q >= ne || (p < ng && (g[p] & kMask) < e[q])) v = g[p++] & kMask;
+        if (q >= ne || (p < ng && (g[(int64_t)p * 64] & kMask) < e[q])) v = g[(int64_t)(p++) * 64] & kMask;
         else v = e[q++];
-        out_idx[o++] = (int32_t)v;
+        out[o * stride] = (int32_t)v;
+        o++;
     }
+    if (kBlocked) out_len[i] = (int32_t)o;
 }
 
 // ----------------------------------------------------------------- tighten
-// neighbours.py:284-303: keep j with d2 < (2 max(h_i, h_j))^2 (pos as given, minimum image).
+// neighbours.py:284-303: keep j with d2 < (2 max(h_i, h_j))^2 (positions as given, minimum image).
+// CSR in -> CSR out (count pass, then fill pass).
 template <bool kFill>
-__global__ void k_tighten(const double* __restrict__ pos, const double* __restrict__ h, Box box,
-                          const int64_t* __restrict__ offset, const int32_t* __restrict__ index,
-                          const int64_t* __restrict__ rows, int64_t nrows,
-                          unsigned long long* __restrict__ counts, const unsigned long long* __restrict__ out_off,
-                          int32_t* __restrict__ out_idx)
+__global__ void k_tighten_csr(const double* __restrict__ pos, const double* __restrict__ h, Box box,
+                              const int64_t* __restrict__ offset, const int32_t* __restrict__ index,
+                              const int64_t* __restrict__ rows, int64_t nrows, unsigned long long* __restrict__ counts,
+                              const unsigned long long* __restrict__ out_off, int32_t* __restrict__ out_idx)
 {
     int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (q >= nrows) return;
-    int64_t i = row_of(rows, q);
+    const int64_t i = row_of(rows, q);
     const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
     const double hi = h[i];
     int64_t m = 0;
-    int64_t o = kFill ? (int64_t)out_off[i] : 0;
+    const int64_t o = kFill ? (int64_t)out_off[i] : 0;
     for (int64_t k = offset[i]; k < offset[i + 1]; k++) {
-        int32_t j = index[k];
-        double dx = min_image(pos[3 * j] - xi, box.l[0]);
-        double dy = min_image(pos[3 * j + 1] - yi, box.l[1]);
-        double dz = min_image(pos[3 * j + 2] - zi, box.l[2]);
+        const int32_t j = index[k];
+        const double dx = min_image(pos[3 * j] - xi, box.l[0]);
+        const double dy = min_image(pos[3 * j + 1] - yi, box.l[1]);
+        const double dz = min_image(pos[3 * j + 2] - zi, box.l[2]);
         double d2 = 0.0;
         d2 += dx * dx;
         d2 += dy * dy;
         d2 += dz * dz;
-        double rr = NB_R_KERN * pymax(hi, h[j]);
+        const double rr = NB_R_KERN * pymax(hi, h[j]);
         if (d2 < rr * rr) {
             if (kFill) out_idx[o + m] = j;
             m++;
@@ -323,10 +380,405 @@ __global__ void k_tighten(const double* __restrict__ pos, const double* __restri
     if (!kFill) counts[i] = (unsigned long long)m;
 }
 
+// Blocked in -> blocked out, one pass (the tightened row never outgrows the search row).
+// total (device) accumulates the entries kept.
+__global__ void __launch_bounds__(kBlock)
+k_tighten_blk(const double* __restrict__ pos, const double* __restrict__ h, Box box, const int32_t* __restrict__ in_len,
+              const int32_t* __restrict__ in_idx, int64_t cap, int64_t n, int32_t* __restrict__ out_len,
+              int32_t* __restrict__ out_idx, unsigned long long* __restrict__ total)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int32_t m = 0;
+    if (i < n) {
+        const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+        const double hi = h[i];
+        const int32_t* src = in_idx + blk(i, 0, cap);
+        int32_t* dst = out_idx + blk(i, 0, cap);
+        const int32_t len = in_len[i];
+        for (int32_t k = 0; k < len; k++) {
+            const int32_t j = src[(int64_t)k * 64];
+            const double dx = min_image(pos[3 * j] - xi, box.l[0]);
+            const double dy = min_image(pos[3 * j + 1] - yi, box.l[1]);
+            const double dz = min_image(pos[3 * j + 2] - zi, box.l[2]);
+            double d2 = 0.0;
+            d2 += dx * dx;
+            d2 += dy * dy;
+            d2 += dz * dz;
+            const double rr = NB_R_KERN * pymax(hi, h[j]);
+            if (d2 < rr * rr) {
+                dst[(int64_t)m * 64] = j;
+                m++;
+            }
+        }
+        out_len[i] = m;
+    }
+    __shared__ unsigned long long sh[kBlock];
+    sh[threadIdx.x] = (unsigned long long)m;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(total, sh[0]);
+}
+
 __global__ void k_fill_u64(unsigned long long* p, int64_t n, unsigned long long v)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// dst row i = src row order[i] (rows of w doubles)
+__global__ void k_permute(const double* __restrict__ src, const int32_t* __restrict__ order, int64_t n, int w,
+                          double* __restrict__ dst)
+{
+    int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= n * w) return;
+    int64_t i = t / w;
+    int c = (int)(t - i * w);
+    dst[t] = src[(int64_t)order[i] * w + c];
+}
+
+__global__ void k_permute_i64(const int64_t* __restrict__ src, const int32_t* __restrict__ order, int64_t n,
+                              int64_t* __restrict__ dst)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[order[i]];
+}
+
+// ----------------------------------------------------------------- fused step passes
+// The coupled step needs, per density pass, the search lists only for the density solve and,
+// afterwards, only the tightened lists. So the gather also keeps each accepted pair's d2 (and
+// the neighbour's mass when masses differ) next to the index, and the same work-item solves its
+// row's h-rho right after gathering it: the solve sums over the gathered row filtered by
+// d2 < (2 min(h_search, h_max))^2, which is exactly the reference's distance pass over the
+// final (sorted) row, because extras always have d2 >= (2 h_search_i)^2 and a gather row in
+// grid order is already sorted. Rows that came out unsorted (or overflowed) are solved after
+// the sort pass. The tightened rows are then emitted straight from the gather rows and the
+// extras (which carry their d2 too), with the final smoothing lengths.
+struct DensArgs {
+    const double* mass;
+    double m_uniform;  // > 0: every mass equals it
+    const double* h_start;
+    double h_max;
+    int kid;
+    double cnorm, eta, tol;
+    double* h_out;
+    double* rho;
+    double* omega;
+    int32_t* iters;
+    int32_t* flags;
+    uint8_t* pending;  // 1: row not solved yet
+    int32_t* scal;     // scal[3] counts rows left pending
+};
+
+// sph.py:215-228 on the cached d2 of a blocked row: FP64 (exact order) or FP32 with
+// compensated (Kahan) accumulation (mixed precision).
+template <bool kMixed>
+__device__ __forceinline__ void row_sums(const double* __restrict__ d2row, const double* __restrict__ mrow,
+                                         double m_uniform, int32_t m, double r_max2, double h, int kid, double cnorm,
+                                         double& rs_out, double& dws_out)
+{
+    if (!kMixed) {
+        double rs = 0.0, dws = 0.0;
+        const double inv_h = 1.0 / h;
+        const double ih3 = ipow3(inv_h);
+        const double ncih4 = -cnorm * ipow4(inv_h);
+        for (int32_t k = 0; k < m; k++) {
+            const double d2 = d2row[(int64_t)k * 64];
+            if (d2 < r_max2) {
+                const double q = sqrt(d2) * inv_h;
+                if (q < 2.0) {
+                    const double mk = mrow ? mrow[(int64_t)k * 64] : m_uniform;
+                    const double fq = kernel_f(q, kid);
+                    rs += mk * cnorm * ih3 * fq;
+                    dws += mk * (ncih4 * (3.0 * fq + q * kernel_df(q, kid)));
+                }
+            }
+        }
+        rs_out = rs;
+        dws_out = dws;
+    } else {
+        // FP32 terms relative to the row's own scale: sum of m_k f(q_k) and m_k (3 f + q f')
+        float s1 = 0.f, c1 = 0.f, s2 = 0.f, c2 = 0.f;
+        const float inv_hf = (float)(1.0 / h);
+        const float rmax2f = (float)r_max2;
+        const float mu = (float)(m_uniform > 0.0 ? m_uniform : 1.0);
+        for (int32_t k = 0; k < m; k++) {
+            const float d2 = (float)d2row[(int64_t)k * 64];
+            if (d2 < rmax2f) {
+                const float q = __fsqrt_rn(d2) * inv_hf;
+                if (q < 2.f) {
+                    const float mk = mrow ? (float)mrow[(int64_t)k * 64] : mu;
+                    float fq, dfq;
+                    if (kid == 0) {
+                        if (q < 1.f) {
+                            fq = 1.f - 1.5f * q * q + 0.75f * q * q * q;
+                            dfq = -3.f * q + 2.25f * q * q;
+                        } else {
+                            const float t = 2.f - q;
+                            fq = 0.25f * t * t * t;
+                            dfq = -0.75f * t * t;
+                        }
+                    } else {
+                        const float x = 1.f - 0.5f * q, x2 = x * x;
+                        fq = x2 * x2 * x2 * (35.f / 12.f * q * q + 3.f * q + 1.f);
+                        dfq = -7.f / 3.f * q * x2 * x2 * x * (5.f * q + 2.f);
+                    }
+                    const float t1 = mk * fq - c1;
+                    const float u1 = s1 + t1;
+                    c1 = (u1 - s1) - t1;
+                    s1 = u1;
+                    const float t2 = mk * (3.f * fq + q * dfq) - c2;
+                    const float u2 = s2 + t2;
+                    c2 = (u2 - s2) - t2;
+                    s2 = u2;
+                }
+            }
+        }
+        const double inv_h = 1.0 / h;
+        const double scale_m = mrow ? 1.0 : (m_uniform > 0.0 ? m_uniform / (double)mu : 1.0);
+        rs_out = (double)s1 * scale_m * cnorm * ipow3(inv_h);
+        dws_out = -(double)s2 * scale_m * cnorm * ipow4(inv_h);
+    }
+}
+
+// sph.py:251-297 for one row whose gathered d2 (and masses) sit in blocked buffers.
+template <bool kMixed>
+__device__ void solve_row(int64_t i, int32_t m, const double* __restrict__ dbuf, const double* __restrict__ mbuf,
+                          int64_t cap, double h_search_i, const DensArgs& A)
+{
+    const double m_i = A.mass[i];
+    const double r_max = 2.0 * pymin(h_search_i, A.h_max);
+    const double r_max2 = r_max * r_max;
+    const double* d2row = dbuf + ((i >> 6) * cap) * 64 + (i & 63);
+    const double* mrow = mbuf ? mbuf + ((i >> 6) * cap) * 64 + (i & 63) : nullptr;
+    double hi = pymin(h_search_i, A.h_max);
+    double lo = 0.0;
+    double h = pymin(A.h_start[i], hi);
+    const double h0 = h;
+    int flag = 3;
+    int it = 0;
+    const double eta3 = ipow3(A.eta);
+    double rs, dws;
+    while (it < 50) {
+        it += 1;
+        row_sums<kMixed>(d2row, mrow, A.m_uniform, m, r_max2, h, A.kid, A.cnorm, rs, dws);
+        const double f = rs - m_i * ipow3(A.eta / h);
+        const double df = dws + 3.0 * m_i * eta3 / ipow4(h);
+        if (f > 0) hi = h;
+        else lo = h;
+        double h_new = df > 0 ? h - f / df : 0.5 * (lo + hi);
+        if (!(lo <= h_new && h_new <= hi)) h_new = 0.5 * (lo + hi);
+        if (fabs(h_new - h) < A.tol * h0) {
+            h = h_new;
+            flag = 0;
+            break;
+        }
+        h = h_new;
+    }
+    const double h_lim = pymin(h_search_i, A.h_max);
+    if (h >= h_lim * (1 - 1e-3)) {
+        double rs_lim, dum;
+        row_sums<kMixed>(d2row, mrow, A.m_uniform, m, r_max2, h_lim, A.kid, A.cnorm, rs_lim, dum);
+        if (rs_lim - m_i * ipow3(A.eta / h_lim) < 0) {
+            h = h_lim;
+            flag = h_lim >= A.h_max ? 1 : 2;
+        }
+    }
+    row_sums<kMixed>(d2row, mrow, A.m_uniform, m, r_max2, h, A.kid, A.cnorm, rs, dws);
+    A.h_out[i] = h;
+    A.rho[i] = rs;
+    A.omega[i] = flag == 1 ? 1.0 : 1.0 + h / (3.0 * rs) * dws;
+    A.iters[i] = it;
+    A.flags[i] = flag;
+}
+
+// Gather (as k_gather, open boxes) keeping d2 (and masses), then the row's density solve.
+template <bool kMixed>
+__global__ void __launch_bounds__(kBlock)
+k_gather_solve(const double* __restrict__ pos, const double* __restrict__ h, int64_t n, Grid g,
+               const uint32_t* __restrict__ start, const uint32_t* __restrict__ order, uint32_t* __restrict__ gbuf,
+               double* __restrict__ dbuf, double* __restrict__ mbuf, int64_t cap, int32_t* __restrict__ rowcnt,
+               uint8_t* __restrict__ unsorted, unsigned long long* __restrict__ ecount, DensArgs A)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double hs = h[i];
+    const double r = NB_R_KERN * hs;
+    const double r2 = r * r;
+    const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+    int64_t ax, nx, ay, ny, az, nz;
+    cell_range(xi, r, g, 0, false, ax, nx);
+    cell_range(yi, r, g, 1, false, ay, ny);
+    cell_range(zi, r, g, 2, false, az, nz);
+    int32_t m = 0;
+    int64_t last = -1;
+    bool sorted = true;
+    const int64_t base = ((i >> 6) * cap) * 64 + (i & 63);
+    for (int64_t di = 0; di < nx; di++) {
+        const uint32_t cx = (uint32_t)(ax + di);
+        for (int64_t dj = 0; dj < ny; dj++) {
+            const uint32_t cy = (uint32_t)(ay + dj);
+            const uint32_t c0 = (cx * (uint32_t)g.nc[1] + cy) * (uint32_t)g.nc[2] + (uint32_t)az;
+            // the z-run of cells is one contiguous range of the cell-sorted particles
+            const uint32_t k0 = start[c0], k1 = start[c0 + (uint32_t)nz];
+            for (uint32_t k = k0; k < k1; k++) {
+                const uint32_t j = order[k];
+                const double dx = pos[3 * j] - xi;
+                const double dy = pos[3 * j + 1] - yi;
+                const double dz = pos[3 * j + 2] - zi;
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                if (d2 < r2) {
+                    const double rj = NB_R_KERN * h[j];
+                    uint32_t e = j;
+                    if (!(d2 < rj * rj)) {
+                        e |= kFlag;
+                        atomicAdd(&ecount[j], 1ull);
+                    }
+                    if ((int64_t)j < last) sorted = false;
+                    last = j;
+                    if (m < cap) {
+                        gbuf[base + (int64_t)m * 64] = e;
+                        dbuf[base + (int64_t)m * 64] = d2;
+                        if (mbuf) mbuf[base + (int64_t)m * 64] = A.mass[j];
+                    }
+                    m++;
+                }
+            }
+        }
+    }
+    rowcnt[i] = m;
+    unsorted[i] = sorted ? 0 : 1;
+    if (!sorted) atomicAdd(&A.scal[1], 1);
+    atomicMax(&A.scal[0], m);
+    if (sorted && m <= cap) {
+        A.pending[i] = 0;
+        solve_row<kMixed>(i, m, dbuf, mbuf, cap, hs, A);
+    } else {
+        A.pending[i] = 1;
+        atomicAdd(&A.scal[3], 1);
+    }
+}
+
+// Insertion sort of unsorted rows, carrying d2 and mass along.
+__global__ void k_sort_rows_payload(uint32_t* __restrict__ gbuf, double* __restrict__ dbuf, double* __restrict__ mbuf,
+                                    int64_t cap, const int32_t* __restrict__ rowcnt,
+                                    const uint8_t* __restrict__ unsorted, int64_t n)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n || !unsorted[i]) return;
+    const int64_t base = ((i >> 6) * cap) * 64 + (i & 63);
+    const int32_t m = rowcnt[i];
+    for (int32_t a = 1; a < m; a++) {
+        const uint32_t v = gbuf[base + (int64_t)a * 64];
+        const double dv = dbuf[base + (int64_t)a * 64];
+        const double mv = mbuf ? mbuf[base + (int64_t)a * 64] : 0.0;
+        const uint32_t key = v & kMask;
+        int32_t b = a - 1;
+        while (b >= 0 && (gbuf[base + (int64_t)b * 64] & kMask) > key) {
+            gbuf[base + (int64_t)(b + 1) * 64] = gbuf[base + (int64_t)b * 64];
+            dbuf[base + (int64_t)(b + 1) * 64] = dbuf[base + (int64_t)b * 64];
+            if (mbuf) mbuf[base + (int64_t)(b + 1) * 64] = mbuf[base + (int64_t)b * 64];
+            b--;
+        }
+        gbuf[base + (int64_t)(b + 1) * 64] = v;
+        dbuf[base + (int64_t)(b + 1) * 64] = dv;
+        if (mbuf) mbuf[base + (int64_t)(b + 1) * 64] = mv;
+    }
+}
+
+template <bool kMixed>
+__global__ void k_solve_pending(const double* __restrict__ h, int64_t n, const double* __restrict__ dbuf,
+                                const double* __restrict__ mbuf, int64_t cap, const int32_t* __restrict__ rowcnt,
+                                DensArgs A)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n || !A.pending[i]) return;
+    solve_row<kMixed>(i, rowcnt[i], dbuf, mbuf, cap, h[i], A);
+}
+
+// extras with their d2
+__global__ void k_extras_fill_d2(const uint32_t* __restrict__ gbuf, const double* __restrict__ dbuf, int64_t cap,
+                                 const int32_t* __restrict__ rowcnt, const unsigned long long* __restrict__ eoff,
+                                 unsigned int* __restrict__ efill, uint32_t* __restrict__ ebuf,
+                                 double* __restrict__ edbuf, int64_t n)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t base = ((i >> 6) * cap) * 64 + (i & 63);
+    const int32_t m = rowcnt[i];
+    for (int32_t a = 0; a < m; a++) {
+        const uint32_t v = gbuf[base + (int64_t)a * 64];
+        if (v & kFlag) {
+            const uint32_t j = v & kMask;
+            const unsigned int s = atomicAdd(&efill[j], 1u);
+            ebuf[eoff[j] + s] = (uint32_t)i;
+            edbuf[eoff[j] + s] = dbuf[base + (int64_t)a * 64];
+        }
+    }
+}
+
+// neighbours.py:284-303 on the merged row (gather row + sorted extras, both with d2), with the
+// final smoothing lengths hf: keep d2 < (2 max(hf_i, hf_j))^2, into blocked rows (cap_out).
+__global__ void __launch_bounds__(kBlock)
+k_tight_emit(const uint32_t* __restrict__ gbuf, const double* __restrict__ dbuf, int64_t cap,
+             const int32_t* __restrict__ rowcnt, const unsigned long long* __restrict__ eoff, uint32_t* __restrict__ ebuf,
+             double* __restrict__ edbuf, const double* __restrict__ hf, int64_t n, int32_t* __restrict__ out_len,
+             int32_t* __restrict__ out_idx, int64_t cap_out, unsigned long long* __restrict__ total)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int32_t o = 0;
+    if (i < n) {
+        uint32_t* e = ebuf + eoff[i];
+        double* ed = edbuf + eoff[i];
+        const int64_t ne = (int64_t)(eoff[i + 1] - eoff[i]);
+        for (int64_t a = 1; a < ne; a++) {
+            const uint32_t v = e[a];
+            const double dv = ed[a];
+            int64_t b = a - 1;
+            while (b >= 0 && e[b] > v) {
+                e[b + 1] = e[b];
+                ed[b + 1] = ed[b];
+                b--;
+            }
+            e[b + 1] = v;
+            ed[b + 1] = dv;
+        }
+        const int64_t base = ((i >> 6) * cap) * 64 + (i & 63);
+        const int64_t obase = ((i >> 6) * cap_out) * 64 + (i & 63);
+        const int32_t ng = rowcnt[i];
+        const double hi = hf[i];
+        int32_t p = 0;
+        int64_t q = 0;
+        while (p < ng || q < ne) {
+            uint32_t j;
+            double d2;
+            if (q >= ne || (p < ng && (gbuf[base + (int64_t)p * 64] & kMask) < e[q])) {
+                j = gbuf[base + (int64_t)p * 64] & kMask;
+                d2 = dbuf[base + (int64_t)p * 64];
+                p++;
+            } else {
+                j = e[q];
+                d2 = ed[q];
+                q++;
+            }
+            const double rr = NB_R_KERN * pymax(hi, hf[j]);
+            if (d2 < rr * rr && o < cap_out) {
+                out_idx[obase + (int64_t)o * 64] = (int32_t)j;
+                o++;
+            }
+        }
+        out_len[i] = o;
+    }
+    __shared__ unsigned long long sh[kBlock];
+    sh[threadIdx.x] = (unsigned long long)o;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(total, sh[0]);
 }
 
 static inline unsigned grid_of(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
@@ -338,9 +790,7 @@ using namespace nb;
 
 namespace {
 
-// exclusive scan of n+1 values (the last input is 0), into out (n+1); returns out[n]
-int scan_u64(nb_ctx* ctx, hipStream_t s, const unsigned long long* in, unsigned long long* out,
-             int64_t count)
+int scan_u64(nb_ctx* ctx, hipStream_t s, const unsigned long long* in, unsigned long long* out, int64_t count)
 {
     size_t bytes = 0;
     if (hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)count, s) != hipSuccess)
@@ -351,54 +801,56 @@ int scan_u64(nb_ctx* ctx, hipStream_t s, const unsigned long long* in, unsigned 
     return NB_OK;
 }
 
-}  // namespace
-
-static const int64_t kMaxAxisCells = 2048;
-static const double kMaxTotalCells = 134217728.0;  // 2^27 cells (512 MiB of start offsets)
-
-extern "C" int32_t nb_lists_build(nb_ctx* ctx, void* stream, const double* pos, const double* h,
-                                  int64_t n, const double* box3, int64_t* out_offset,
-                                  int32_t* out_index, int64_t capacity, int64_t* total_out)
+int sync_copy(nb_ctx* ctx, hipStream_t s, void* host, const void* dev, size_t bytes, const char* what)
 {
-    hipStream_t s = (hipStream_t)stream;
-    if (!ctx || n < 0 || !total_out) return ctx_fail(ctx, "nb_lists_build: bad arguments");
-    ctx->lists_ready = 0;
-    if (n >= (int64_t)kMask) return ctx_fail(ctx, "nb_lists_build: too many particles");
-    Box box{{0.0, 0.0, 0.0}};
-    if (box3) for (int c = 0; c < 3; c++) box.l[c] = box3[c];
-    if (n == 0) {
-        if (hipMemsetAsync(out_offset, 0, sizeof(int64_t), s) != hipSuccess) return ctx_fail(ctx, "memset");
-        *total_out = 0;
-        ctx->lists_n = 0;
-        ctx->lists_total = 0;
-        ctx->lists_ready = 1;
-        return NB_OK;
-    }
-    // bounding box (host needs it for the grid shape)
+    if (hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+        return ctx_fail(ctx, what);
+    return NB_OK;
+}
+
+const int64_t kMaxAxisCells = 2048;
+const double kMaxTotalCells = 134217728.0;  // 2^27
+
+Box make_box(const double* box3)
+{
+    Box b{{0.0, 0.0, 0.0}};
+    if (box3)
+        for (int c = 0; c < 3; c++) b.l[c] = box3[c];
+    return b;
+}
+
+// Grid of the given positions and search lengths, particles sorted by cell into ctx->order and
+// cell starts into ctx->cell_start.
+int build_grid(nb_ctx* ctx, hipStream_t s, const double* pos, const double* h, int64_t n, const Box& box, Grid& g,
+               int64_t& ncells)
+{
     int nparts = (int)std::min<int64_t>(1024, (n + kBlock - 1) / kBlock);
-    if (ctx_reserve(ctx, (void**)&ctx->red, &ctx->red_bytes, sizeof(double) * 8 * (nparts + 1)))
-        return NB_ERR_HIP;
+    if (ctx_reserve(ctx, (void**)&ctx->red, &ctx->red_bytes, sizeof(double) * 8 * (nparts + 2))) return NB_ERR_HIP;
+    double* fin = ctx->red + 8 * nparts;
+    unsigned long long* hsum = (unsigned long long*)(fin + 8);
     hipLaunchKernelGGL(k_bbox, dim3(nparts), dim3(kBlock), 0, s, pos, h, n, box, ctx->red);
-    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(1), 0, s, ctx->red, nparts, ctx->red + 8 * nparts);
-    double bb[8];
-    if (hipMemcpyAsync(bb, ctx->red + 8 * nparts, sizeof(bb), hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess)
-        return ctx_fail(ctx, "nb_lists_build: bounding box copy failed");
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlock), 0, s, ctx->red, nparts, fin, hsum);
+    hipLaunchKernelGGL(k_hsum, dim3(nparts), dim3(kBlock), 0, s, h, n, fin, hsum);
+    double bb[9];
+    int rc = sync_copy(ctx, s, bb, fin, sizeof(double) * 9, "grid: bounding box copy failed");
+    if (rc) return rc;
+    unsigned long long hs;
+    std::memcpy(&hs, &bb[8], sizeof(hs));
+    const double hmax = bb[6];
     for (int c = 0; c < 3; c++)
-        if (box.l[c] > 0.0 && NB_R_KERN * bb[6] >= 0.5 * box.l[c])
-            return ctx_fail_code(ctx, NB_ERR_BOX,
-                                 "kernel support exceeds half the periodic box: neighbours ambiguous");
-    if (!(bb[6] == bb[6]) || !std::isfinite(bb[0] + bb[1] + bb[2] + bb[3] + bb[4] + bb[5]))
-        return ctx_fail(ctx, "nb_lists_build: non-finite positions or smoothing lengths");
-    // grid: cell about the mean search length, bounded in count
-    Grid g;
+        if (box.l[c] > 0.0 && NB_R_KERN * hmax >= 0.5 * box.l[c])
+            return ctx_fail_code(ctx, NB_ERR_BOX, "kernel support exceeds half the periodic box: neighbours ambiguous");
+    if (!std::isfinite(bb[0] + bb[1] + bb[2] + bb[3] + bb[4] + bb[5]) || !std::isfinite(hmax))
+        return ctx_fail(ctx, "grid: non-finite positions or smoothing lengths");
     double span[3];
     for (int c = 0; c < 3; c++) {
         g.lo[c] = box.l[c] > 0.0 ? 0.0 : bb[c];
         span[c] = box.l[c] > 0.0 ? box.l[c] : bb[3 + c] - bb[c];
     }
-    double maxspan = std::max(span[0], std::max(span[1], span[2]));
-    double cell = std::max(std::max(bb[7] / (double)n, maxspan / (double)kMaxAxisCells), 1e-300);
+    const double scale = hmax > 0.0 ? 4294967296.0 / hmax : 0.0;
+    const double hmean = scale > 0.0 ? ((double)hs / scale) / (double)n : 0.0;
+    const double maxspan = std::max(span[0], std::max(span[1], span[2]));
+    double cell = std::max(std::max(hmean, maxspan / (double)kMaxAxisCells), 1e-300);
     for (;;) {
         double total = 1.0;
         for (int c = 0; c < 3; c++) {
@@ -415,8 +867,9 @@ extern "C" int32_t nb_lists_build(nb_ctx* ctx, void* stream, const double* pos, 
         if (total <= kMaxTotalCells) break;
         cell *= 1.25;
     }
-    const int64_t ncells = (int64_t)g.nc[0] * g.nc[1] * g.nc[2];
-    // buffers
+    ncells = (int64_t)g.nc[0] * g.nc[1] * g.nc[2];
+    ctx->last_grid_cells = ncells;
+    ctx->last_cell = cell;
     if (ctx_reserve(ctx, (void**)&ctx->cell_id, &ctx->cell_id_bytes, sizeof(uint32_t) * n) ||
         ctx_reserve(ctx, (void**)&ctx->order, &ctx->order_bytes, sizeof(uint32_t) * n) ||
         ctx_reserve(ctx, (void**)&ctx->keys_out, &ctx->keys_out_bytes, sizeof(uint32_t) * n) ||
@@ -426,28 +879,38 @@ extern "C" int32_t nb_lists_build(nb_ctx* ctx, void* stream, const double* pos, 
         return NB_ERR_HIP;
     if (hipMemsetAsync(ctx->cell_count, 0, sizeof(uint32_t) * (ncells + 1), s) != hipSuccess)
         return ctx_fail(ctx, "memset cell_count");
-    hipLaunchKernelGGL(k_cell_id, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, n, box, g,
-                       ctx->cell_id, ctx->iota, ctx->cell_count);
+    hipLaunchKernelGGL(k_cell_id, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, n, box, g, ctx->cell_id, ctx->iota,
+                       ctx->cell_count);
     int end_bit = 1;
     while ((int64_t(1) << end_bit) < ncells && end_bit < 32) end_bit++;
     size_t bytes = 0;
-    if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, ctx->cell_id, ctx->keys_out, ctx->iota,
-                                           ctx->order, (int)n, 0, end_bit, s) != hipSuccess)
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, ctx->cell_id, ctx->keys_out, ctx->iota, ctx->order, (int)n,
+                                           0, end_bit, s) != hipSuccess)
         return ctx_fail(ctx, "hipcub sort sizing failed");
     if (ctx_reserve(ctx, (void**)&ctx->cub_tmp, &ctx->cub_tmp_bytes, bytes)) return NB_ERR_HIP;
-    if (hipcub::DeviceRadixSort::SortPairs(ctx->cub_tmp, bytes, ctx->cell_id, ctx->keys_out, ctx->iota,
-                                           ctx->order, (int)n, 0, end_bit, s) != hipSuccess)
+    if (hipcub::DeviceRadixSort::SortPairs(ctx->cub_tmp, bytes, ctx->cell_id, ctx->keys_out, ctx->iota, ctx->order,
+                                           (int)n, 0, end_bit, s) != hipSuccess)
         return ctx_fail(ctx, "hipcub sort failed");
     bytes = 0;
-    if (hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, ctx->cell_count, ctx->cell_start,
-                                         (int)(ncells + 1), s) != hipSuccess)
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, ctx->cell_count, ctx->cell_start, (int)(ncells + 1), s) !=
+        hipSuccess)
         return ctx_fail(ctx, "hipcub scan sizing failed");
     if (ctx_reserve(ctx, (void**)&ctx->cub_tmp, &ctx->cub_tmp_bytes, bytes)) return NB_ERR_HIP;
-    if (hipcub::DeviceScan::ExclusiveSum(ctx->cub_tmp, bytes, ctx->cell_count, ctx->cell_start,
-                                         (int)(ncells + 1), s) != hipSuccess)
+    if (hipcub::DeviceScan::ExclusiveSum(ctx->cub_tmp, bytes, ctx->cell_count, ctx->cell_start, (int)(ncells + 1), s) !=
+        hipSuccess)
         return ctx_fail(ctx, "hipcub scan failed");
-    // gather into padded rows; grow the row capacity until every row fits
+    return NB_OK;
+}
+
+// Grid, gather, row sort, extras and final row lengths: everything up to the emission.
+int gather_stage(nb_ctx* ctx, hipStream_t s, const double* pos, const double* h, int64_t n, const Box& box)
+{
+    Grid g;
+    int64_t ncells = 0;
+    int rc = build_grid(ctx, s, pos, h, n, box, g, ncells);
+    if (rc) return rc;
     if (ctx_reserve(ctx, (void**)&ctx->rowcnt, &ctx->rowcnt_bytes, sizeof(int32_t) * n) ||
+        ctx_reserve(ctx, (void**)&ctx->unsorted, &ctx->unsorted_bytes, n) ||
         ctx_reserve(ctx, (void**)&ctx->ecount, &ctx->ecount_bytes, sizeof(unsigned long long) * (n + 1)) ||
         ctx_reserve(ctx, (void**)&ctx->eoff, &ctx->eoff_bytes, sizeof(unsigned long long) * (n + 1)) ||
         ctx_reserve(ctx, (void**)&ctx->tot, &ctx->tot_bytes, sizeof(unsigned long long) * (n + 1)) ||
@@ -455,59 +918,93 @@ extern "C" int32_t nb_lists_build(nb_ctx* ctx, void* stream, const double* pos, 
         ctx_reserve(ctx, (void**)&ctx->offs, &ctx->offs_bytes, sizeof(unsigned long long) * (n + 1)) ||
         ctx_reserve(ctx, (void**)&ctx->scal, &ctx->scal_bytes, sizeof(int64_t) * 4))
         return NB_ERR_HIP;
-    int32_t cap = ctx->row_cap > 0 ? ctx->row_cap : 256;
+    const int64_t nblk = (n + 63) / 64;
+    int64_t cap = ctx->row_cap > 0 ? ctx->row_cap : 192;
+    int32_t sc[4] = {0, 0, 0, 0};
     for (int attempt = 0; attempt < 8; attempt++) {
-        if (ctx_reserve(ctx, (void**)&ctx->gbuf, &ctx->gbuf_bytes, sizeof(uint32_t) * (size_t)n * cap))
+        if (ctx_reserve(ctx, (void**)&ctx->gbuf, &ctx->gbuf_bytes, sizeof(uint32_t) * (size_t)(nblk * cap * 64)))
             return NB_ERR_HIP;
         if (hipMemsetAsync(ctx->ecount, 0, sizeof(unsigned long long) * (n + 1), s) != hipSuccess ||
             hipMemsetAsync(ctx->scal, 0, sizeof(int64_t) * 4, s) != hipSuccess)
             return ctx_fail(ctx, "memset");
-        hipLaunchKernelGGL(k_gather, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, h, n, box, g,
-                           ctx->cell_start, ctx->order, ctx->gbuf, cap, ctx->rowcnt, ctx->ecount,
-                           (int32_t*)ctx->scal);
-        int32_t maxc = 0;
-        if (hipMemcpyAsync(&maxc, ctx->scal, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
-            hipStreamSynchronize(s) != hipSuccess)
-            return ctx_fail(ctx, "nb_lists_build: gather failed");
-        if (maxc <= cap) break;
-        cap = ((maxc + maxc / 4 + 63) / 64) * 64;
-        if (attempt == 7) return ctx_fail(ctx, "nb_lists_build: row capacity did not settle");
+        hipLaunchKernelGGL(k_gather, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, h, n, box, g, ctx->cell_start, ctx->order,
+                           ctx->gbuf, cap, ctx->rowcnt, ctx->unsorted, ctx->ecount, (int32_t*)ctx->scal);
+        rc = sync_copy(ctx, s, sc, ctx->scal, sizeof(sc), "gather failed");
+        if (rc) return rc;
+        if (sc[0] <= cap) break;
+        cap = ((sc[0] + sc[0] / 8 + 15) / 16) * 16;
+        if (attempt == 7) return ctx_fail(ctx, "row capacity did not settle");
     }
     ctx->row_cap = cap;
-    hipLaunchKernelGGL(k_sort_rows, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->gbuf, cap, ctx->rowcnt, n);
-    int rc = scan_u64(ctx, s, ctx->ecount, ctx->eoff, n + 1);
+    ctx->last_unsorted = sc[1];
+    if (sc[1] > 0)
+        hipLaunchKernelGGL(k_sort_rows, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->gbuf, cap, ctx->rowcnt, ctx->unsorted, n);
+    rc = scan_u64(ctx, s, ctx->ecount, ctx->eoff, n + 1);
     if (rc) return rc;
     unsigned long long ne_total = 0;
-    if (hipMemcpyAsync(&ne_total, ctx->eoff + n, sizeof(ne_total), hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess)
-        return ctx_fail(ctx, "nb_lists_build: extras count copy failed");
-    if (ctx_reserve(ctx, (void**)&ctx->ebuf, &ctx->ebuf_bytes, sizeof(uint32_t) * (ne_total + 1)))
-        return NB_ERR_HIP;
+    rc = sync_copy(ctx, s, &ne_total, ctx->eoff + n, sizeof(ne_total), "extras count copy failed");
+    if (rc) return rc;
+    if (ctx_reserve(ctx, (void**)&ctx->ebuf, &ctx->ebuf_bytes, sizeof(uint32_t) * (ne_total + 1))) return NB_ERR_HIP;
     if (hipMemsetAsync(ctx->efill, 0, sizeof(unsigned int) * n, s) != hipSuccess) return ctx_fail(ctx, "memset");
-    hipLaunchKernelGGL(k_extras_fill, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->gbuf, cap, ctx->rowcnt,
-                       ctx->eoff, ctx->efill, ctx->ebuf, n);
-    hipLaunchKernelGGL(k_row_total, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, ctx->rowcnt, ctx->ecount,
-                       ctx->tot, n);
+    hipLaunchKernelGGL(k_extras_fill, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->gbuf, cap, ctx->rowcnt, ctx->eoff,
+                       ctx->efill, ctx->ebuf, n);
+    hipLaunchKernelGGL(k_row_total, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, ctx->rowcnt, ctx->ecount, ctx->tot,
+                       (int32_t*)ctx->scal, n);
     rc = scan_u64(ctx, s, ctx->tot, ctx->offs, n + 1);
     if (rc) return rc;
     unsigned long long total = 0;
-    if (hipMemcpyAsync(&total, ctx->offs + n, sizeof(total), hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess)
-        return ctx_fail(ctx, "nb_lists_build: total copy failed");
-    if (hipMemcpyAsync(out_offset, ctx->offs, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToDevice, s) != hipSuccess)
-        return ctx_fail(ctx, "offset copy failed");
-    *total_out = (int64_t)total;
+    rc = sync_copy(ctx, s, &total, ctx->offs + n, sizeof(total), "total copy failed");
+    if (rc) return rc;
+    rc = sync_copy(ctx, s, sc, ctx->scal, sizeof(sc), "max row copy failed");
+    if (rc) return rc;
     ctx->lists_n = n;
     ctx->lists_total = (int64_t)total;
+    ctx->lists_maxrow = sc[2];
     ctx->lists_ready = 1;
-    ctx->last_grid_cells = ncells;
-    ctx->last_cell = cell;
-    if ((int64_t)total > capacity || !out_index) return NB_RETRY_CAPACITY;
-    hipLaunchKernelGGL(k_merge_emit, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->gbuf, cap, ctx->rowcnt,
-                       ctx->eoff, ctx->ebuf, ctx->offs, out_index, capacity, n);
-    if (hipGetLastError() != hipSuccess) return ctx_fail(ctx, "nb_lists_build: kernel launch failed");
-    ctx->lists_ready = 2;
     return NB_OK;
+}
+
+}  // namespace
+
+extern "C" int32_t nb_grid_order(nb_ctx* ctx, void* stream, const double* pos, const double* h, int64_t n,
+                                 const double* box3, int32_t* order_out)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (!ctx || n < 0 || !order_out) return ctx_fail(ctx, "nb_grid_order: bad arguments");
+    if (n == 0) return NB_OK;
+    if (n >= (int64_t)kMask) return ctx_fail(ctx, "nb_grid_order: too many particles");
+    Grid g;
+    int64_t ncells;
+    int rc = build_grid(ctx, s, pos, h, n, make_box(box3), g, ncells);
+    if (rc) return rc;
+    if (hipMemcpyAsync(order_out, ctx->order, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return ctx_fail(ctx, "nb_grid_order: copy failed");
+    return NB_OK;
+}
+
+extern "C" int32_t nb_lists_build(nb_ctx* ctx, void* stream, const double* pos, const double* h, int64_t n,
+                                  const double* box3, int64_t* out_offset, int32_t* out_index, int64_t capacity,
+                                  int64_t* total_out)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (!ctx || n < 0 || !total_out || !out_offset) return ctx_fail(ctx, "nb_lists_build: bad arguments");
+    ctx->lists_ready = 0;
+    if (n >= (int64_t)kMask) return ctx_fail(ctx, "nb_lists_build: too many particles");
+    if (n == 0) {
+        if (hipMemsetAsync(out_offset, 0, sizeof(int64_t), s) != hipSuccess) return ctx_fail(ctx, "memset");
+        *total_out = 0;
+        ctx->lists_n = 0;
+        ctx->lists_total = 0;
+        ctx->lists_ready = 1;
+        return NB_OK;
+    }
+    int rc = gather_stage(ctx, s, pos, h, n, make_box(box3));
+    if (rc) return rc;
+    if (hipMemcpyAsync(out_offset, ctx->offs, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return ctx_fail(ctx, "offset copy failed");
+    *total_out = ctx->lists_total;
+    if (ctx->lists_total > capacity || !out_index) return NB_RETRY_CAPACITY;
+    return nb_lists_emit(ctx, stream, out_index, capacity);
 }
 
 extern "C" int32_t nb_lists_emit(nb_ctx* ctx, void* stream, int32_t* out_index, int64_t capacity)
@@ -515,41 +1012,251 @@ extern "C" int32_t nb_lists_emit(nb_ctx* ctx, void* stream, int32_t* out_index, 
     hipStream_t s = (hipStream_t)stream;
     if (!ctx || !ctx->lists_ready) return ctx_fail_code(ctx, NB_ERR_NOT_BUILT, "nb_lists_emit: no lists built");
     if (ctx->lists_total > capacity) return ctx_fail(ctx, "nb_lists_emit: capacity too small");
-    int64_t n = ctx->lists_n;
+    const int64_t n = ctx->lists_n;
     if (n == 0) return NB_OK;
-    hipLaunchKernelGGL(k_merge_emit, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->gbuf, ctx->row_cap,
-                       ctx->rowcnt, ctx->eoff, ctx->ebuf, ctx->offs, out_index, capacity, n);
+    hipLaunchKernelGGL(k_merge<false>, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->gbuf, (int64_t)ctx->row_cap,
+                       ctx->rowcnt, ctx->eoff, ctx->ebuf, ctx->offs, out_index, (int32_t*)nullptr, (int64_t)1 << 40, n);
     if (hipGetLastError() != hipSuccess) return ctx_fail(ctx, "nb_lists_emit: kernel launch failed");
+    ctx->lists_ready = 2;
+    return NB_OK;
+}
+
+extern "C" int32_t nb_lists_build_blocked(nb_ctx* ctx, void* stream, const double* pos, const double* h, int64_t n,
+                                          const double* box3, int32_t* out_len, int32_t* out_index, int64_t row_cap,
+                                          int64_t* need_cap_out, int64_t* total_out)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (!ctx || n < 0 || !need_cap_out || !total_out) return ctx_fail(ctx, "nb_lists_build_blocked: bad arguments");
+    ctx->lists_ready = 0;
+    if (n >= (int64_t)kMask) return ctx_fail(ctx, "nb_lists_build_blocked: too many particles");
+    *need_cap_out = 0;
+    *total_out = 0;
+    if (n == 0) {
+        ctx->lists_n = 0;
+        ctx->lists_total = 0;
+        ctx->lists_ready = 1;
+        return NB_OK;
+    }
+    int rc = gather_stage(ctx, s, pos, h, n, make_box(box3));
+    if (rc) return rc;
+    *need_cap_out = ctx->lists_maxrow;
+    *total_out = ctx->lists_total;
+    if (ctx->lists_maxrow > row_cap || !out_index || !out_len) return NB_RETRY_CAPACITY;
+    return nb_lists_emit_blocked(ctx, stream, out_len, out_index, row_cap);
+}
+
+extern "C" int32_t nb_lists_emit_blocked(nb_ctx* ctx, void* stream, int32_t* out_len, int32_t* out_index,
+                                         int64_t row_cap)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (!ctx || !ctx->lists_ready) return ctx_fail_code(ctx, NB_ERR_NOT_BUILT, "nb_lists_emit_blocked: no lists built");
+    if (ctx->lists_maxrow > row_cap) return ctx_fail(ctx, "nb_lists_emit_blocked: row capacity too small");
+    const int64_t n = ctx->lists_n;
+    if (n == 0) return NB_OK;
+    hipLaunchKernelGGL(k_merge<true>, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->gbuf, (int64_t)ctx->row_cap,
+                       ctx->rowcnt, ctx->eoff, ctx->ebuf, ctx->offs, out_index, out_len, row_cap, n);
+    if (hipGetLastError() != hipSuccess) return ctx_fail(ctx, "nb_lists_emit_blocked: kernel launch failed");
+    ctx->lists_ready = 2;
+    return NB_OK;
+}
+
+extern "C" int32_t nb_lists_stats(const nb_ctx* ctx, int64_t* out4)
+{
+    if (!ctx || !out4) return NB_ERR_ARG;
+    out4[0] = ctx->lists_total;
+    out4[1] = ctx->lists_maxrow;
+    out4[2] = ctx->last_unsorted;
+    out4[3] = ctx->last_grid_cells;
     return NB_OK;
 }
 
 extern "C" int32_t nb_tighten(nb_ctx* ctx, void* stream, const double* pos, const double* h, int64_t n,
-                              const double* box3, const int64_t* offset, const int32_t* index,
-                              const int64_t* rows, int64_t nrows, int64_t* out_offset, int32_t* out_index,
-                              int64_t capacity, int64_t* total_out)
+                              const double* box3, const int64_t* offset, const int32_t* index, const int64_t* rows,
+                              int64_t nrows, int64_t* out_offset, int32_t* out_index, int64_t capacity,
+                              int64_t* total_out)
 {
     hipStream_t s = (hipStream_t)stream;
     if (!ctx || n < 0 || nrows < 0 || !total_out) return ctx_fail(ctx, "nb_tighten: bad arguments");
-    Box box{{0.0, 0.0, 0.0}};
-    if (box3) for (int c = 0; c < 3; c++) box.l[c] = box3[c];
+    const Box box = make_box(box3);
     if (ctx_reserve(ctx, (void**)&ctx->tcount, &ctx->tcount_bytes, sizeof(unsigned long long) * (n + 1)))
         return NB_ERR_HIP;
-    // rows not in ``rows`` stay empty
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, ctx->tcount, n + 1, 0ull);
     if (nrows > 0)
-        hipLaunchKernelGGL((k_tighten<false>), dim3(grid_of(nrows)), dim3(kBlock), 0, s, pos, h, box, offset,
-                           index, rows, nrows, ctx->tcount, nullptr, nullptr);
+        hipLaunchKernelGGL(k_tighten_csr<false>, dim3(grid_of(nrows)), dim3(kBlock), 0, s, pos, h, box, offset, index,
+                           rows, nrows, ctx->tcount, nullptr, nullptr);
     int rc = scan_u64(ctx, s, ctx->tcount, (unsigned long long*)out_offset, n + 1);
     if (rc) return rc;
     int64_t total = 0;
-    if (hipMemcpyAsync(&total, out_offset + n, sizeof(total), hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess)
-        return ctx_fail(ctx, "nb_tighten: total copy failed");
+    rc = sync_copy(ctx, s, &total, out_offset + n, sizeof(total), "nb_tighten: total copy failed");
+    if (rc) return rc;
     *total_out = total;
     if (total > capacity) return NB_RETRY_CAPACITY;
     if (nrows > 0)
-        hipLaunchKernelGGL((k_tighten<true>), dim3(grid_of(nrows)), dim3(kBlock), 0, s, pos, h, box, offset,
-                           index, rows, nrows, nullptr, (const unsigned long long*)out_offset, out_index);
+        hipLaunchKernelGGL(k_tighten_csr<true>, dim3(grid_of(nrows)), dim3(kBlock), 0, s, pos, h, box, offset, index,
+                           rows, nrows, nullptr, (const unsigned long long*)out_offset, out_index);
     if (hipGetLastError() != hipSuccess) return ctx_fail(ctx, "nb_tighten: kernel launch failed");
+    return NB_OK;
+}
+
+extern "C" int32_t nb_tighten_blocked(nb_ctx* ctx, void* stream, const double* pos, const double* h, int64_t n,
+                                      const double* box3, const int32_t* in_len, const int32_t* in_index,
+                                      int64_t row_cap, int32_t* out_len, int32_t* out_index, int64_t* total_out)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (!ctx || n < 0 || !total_out) return ctx_fail(ctx, "nb_tighten_blocked: bad arguments");
+    *total_out = 0;
+    if (n == 0) return NB_OK;
+    if (ctx_reserve(ctx, (void**)&ctx->scal, &ctx->scal_bytes, sizeof(int64_t) * 4)) return NB_ERR_HIP;
+    unsigned long long* tot = (unsigned long long*)ctx->scal + 3;
+    if (hipMemsetAsync(tot, 0, sizeof(unsigned long long), s) != hipSuccess) return ctx_fail(ctx, "memset");
+    hipLaunchKernelGGL(k_tighten_blk, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, h, make_box(box3), in_len, in_index,
+                       row_cap, n, out_len, out_index, tot);
+    unsigned long long t = 0;
+    int rc = sync_copy(ctx, s, &t, tot, sizeof(t), "nb_tighten_blocked: total copy failed");
+    if (rc) return rc;
+    *total_out = (int64_t)t;
+    return NB_OK;
+}
+
+extern "C" int32_t nb_permute(nb_ctx* ctx, void* stream, const double* src, const int32_t* order, int64_t n,
+                              int32_t width, double* dst)
+{
+    if (!ctx || n < 0 || width <= 0 || src == dst) return ctx_fail(ctx, "nb_permute: bad arguments");
+    if (n == 0) return NB_OK;
+    hipLaunchKernelGGL(k_permute, dim3(grid_of(n * width)), dim3(kBlock), 0, (hipStream_t)stream, src, order, n,
+                       (int)width, dst);
+    if (hipGetLastError() != hipSuccess) return ctx_fail(ctx, "nb_permute: kernel launch failed");
+    return NB_OK;
+}
+
+extern "C" int32_t nb_permute_i64(nb_ctx* ctx, void* stream, const int64_t* src, const int32_t* order, int64_t n,
+                                  int64_t* dst)
+{
+    if (!ctx || n < 0 || src == dst) return ctx_fail(ctx, "nb_permute_i64: bad arguments");
+    if (n == 0) return NB_OK;
+    hipLaunchKernelGGL(k_permute_i64, dim3(grid_of(n)), dim3(kBlock), 0, (hipStream_t)stream, src, order, n, dst);
+    if (hipGetLastError() != hipSuccess) return ctx_fail(ctx, "nb_permute_i64: kernel launch failed");
+    return NB_OK;
+}
+
+
+// ------------------------------------------------------------------ fused step passes (host)
+extern "C" int32_t nb_hydro_density(nb_ctx* ctx, void* stream, const double* pos, const double* mass, double m_uniform,
+                                    const double* h_start, const double* h_search, int64_t n, int32_t kid,
+                                    double cnorm, double eta, double tol, double h_max, int32_t precision,
+                                    double* h_out, double* rho, double* omega, int32_t* iters, int32_t* flags,
+                                    int64_t* stats4)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (!ctx || n <= 0 || !stats4 || n >= (int64_t)kMask) return ctx_fail(ctx, "nb_hydro_density: bad arguments");
+    const Box open{{0.0, 0.0, 0.0}};
+    Grid g;
+    int64_t ncells = 0;
+    int rc = build_grid(ctx, s, pos, h_search, n, open, g, ncells);
+    if (rc) return rc;
+    const bool mixed = precision != 0;
+    const bool masses = !(m_uniform > 0.0);
+    if (ctx_reserve(ctx, (void**)&ctx->rowcnt, &ctx->rowcnt_bytes, sizeof(int32_t) * n) ||
+        ctx_reserve(ctx, (void**)&ctx->unsorted, &ctx->unsorted_bytes, n) ||
+        ctx_reserve(ctx, (void**)&ctx->pending, &ctx->pending_bytes, n) ||
+        ctx_reserve(ctx, (void**)&ctx->ecount, &ctx->ecount_bytes, sizeof(unsigned long long) * (n + 1)) ||
+        ctx_reserve(ctx, (void**)&ctx->eoff, &ctx->eoff_bytes, sizeof(unsigned long long) * (n + 1)) ||
+        ctx_reserve(ctx, (void**)&ctx->efill, &ctx->efill_bytes, sizeof(unsigned int) * n) ||
+        ctx_reserve(ctx, (void**)&ctx->scal, &ctx->scal_bytes, sizeof(int64_t) * 4))
+        return NB_ERR_HIP;
+    const int64_t nblk = (n + 63) / 64;
+    int64_t cap = ctx->row_cap > 0 ? ctx->row_cap : 192;
+    int32_t sc[4] = {0, 0, 0, 0};
+    DensArgs A{mass, m_uniform, h_start, h_max, kid, cnorm, eta, tol, h_out, rho, omega, iters, flags, ctx->pending,
+               (int32_t*)ctx->scal};
+    for (int attempt = 0; attempt < 8; attempt++) {
+        const size_t cells = (size_t)(nblk * cap * 64);
+        if (ctx_reserve(ctx, (void**)&ctx->gbuf, &ctx->gbuf_bytes, sizeof(uint32_t) * cells) ||
+            ctx_reserve(ctx, (void**)&ctx->dscratch, &ctx->dscratch_bytes, sizeof(double) * cells * (masses ? 2 : 1)))
+            return NB_ERR_HIP;
+        if (hipMemsetAsync(ctx->ecount, 0, sizeof(unsigned long long) * (n + 1), s) != hipSuccess ||
+            hipMemsetAsync(ctx->scal, 0, sizeof(int64_t) * 4, s) != hipSuccess)
+            return ctx_fail(ctx, "memset");
+        double* mbuf = masses ? ctx->dscratch + cells : nullptr;
+        if (mixed)
+            hipLaunchKernelGGL(k_gather_solve<true>, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, h_search, n, g,
+                               ctx->cell_start, ctx->order, ctx->gbuf, ctx->dscratch, mbuf, cap, ctx->rowcnt,
+                               ctx->unsorted, ctx->ecount, A);
+        else
+            hipLaunchKernelGGL(k_gather_solve<false>, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, h_search, n, g,
+                               ctx->cell_start, ctx->order, ctx->gbuf, ctx->dscratch, mbuf, cap, ctx->rowcnt,
+                               ctx->unsorted, ctx->ecount, A);
+        rc = sync_copy(ctx, s, sc, ctx->scal, sizeof(sc), "nb_hydro_density: gather failed");
+        if (rc) return rc;
+        if (sc[0] <= cap) break;
+        cap = ((sc[0] + sc[0] / 8 + 15) / 16) * 16;
+        if (attempt == 7) return ctx_fail(ctx, "row capacity did not settle");
+    }
+    ctx->row_cap = (int32_t)cap;
+    ctx->last_unsorted = sc[1];
+    const size_t cells = (size_t)(nblk * cap * 64);
+    double* mbuf = masses ? ctx->dscratch + cells : nullptr;
+    if (sc[1] > 0)
+        hipLaunchKernelGGL(k_sort_rows_payload, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->gbuf, ctx->dscratch, mbuf,
+                           cap, ctx->rowcnt, ctx->unsorted, n);
+    if (sc[3] > 0) {
+        if (mixed)
+            hipLaunchKernelGGL(k_solve_pending<true>, dim3(grid_of(n)), dim3(kBlock), 0, s, h_search, n, ctx->dscratch,
+                               mbuf, cap, ctx->rowcnt, A);
+        else
+            hipLaunchKernelGGL(k_solve_pending<false>, dim3(grid_of(n)), dim3(kBlock), 0, s, h_search, n,
+                               ctx->dscratch, mbuf, cap, ctx->rowcnt, A);
+    }
+    // extras (with d2) for the tightened rows; the longest merged row bounds the tight rows
+    if (ctx_reserve(ctx, (void**)&ctx->tot, &ctx->tot_bytes, sizeof(unsigned long long) * (n + 1))) return NB_ERR_HIP;
+    hipLaunchKernelGGL(k_row_total, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, ctx->rowcnt, ctx->ecount, ctx->tot,
+                       (int32_t*)ctx->scal, n);
+    rc = scan_u64(ctx, s, ctx->ecount, ctx->eoff, n + 1);
+    if (rc) return rc;
+    unsigned long long ne_total = 0;
+    rc = sync_copy(ctx, s, &ne_total, ctx->eoff + n, sizeof(ne_total), "extras count copy failed");
+    if (rc) return rc;
+    rc = sync_copy(ctx, s, sc, ctx->scal, sizeof(sc), "merged row length copy failed");
+    if (rc) return rc;
+    if (ctx_reserve(ctx, (void**)&ctx->offs, &ctx->offs_bytes, sizeof(unsigned long long) * (n + 1))) return NB_ERR_HIP;
+    rc = scan_u64(ctx, s, ctx->tot, ctx->offs, n + 1);
+    if (rc) return rc;
+    unsigned long long merged_total = 0;
+    rc = sync_copy(ctx, s, &merged_total, ctx->offs + n, sizeof(merged_total), "merged total copy failed");
+    if (rc) return rc;
+    if (ctx_reserve(ctx, (void**)&ctx->ebuf, &ctx->ebuf_bytes, sizeof(uint32_t) * (ne_total + 1)) ||
+        ctx_reserve(ctx, (void**)&ctx->edbuf, &ctx->edbuf_bytes, sizeof(double) * (ne_total + 1)))
+        return NB_ERR_HIP;
+    if (hipMemsetAsync(ctx->efill, 0, sizeof(unsigned int) * n, s) != hipSuccess) return ctx_fail(ctx, "memset");
+    hipLaunchKernelGGL(k_extras_fill_d2, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->gbuf, ctx->dscratch, cap,
+                       ctx->rowcnt, ctx->eoff, ctx->efill, ctx->ebuf, ctx->edbuf, n);
+    if (hipGetLastError() != hipSuccess) return ctx_fail(ctx, "nb_hydro_density: kernel launch failed");
+    ctx->lists_n = n;
+    ctx->hydro_ready = 1;
+    ctx->lists_maxrow = sc[2];
+    stats4[0] = sc[0];              // longest gather row
+    stats4[1] = ctx->last_unsorted;  // rows gathered unsorted
+    stats4[2] = (int64_t)merged_total;  // entries of the search lists (incl. self)
+    stats4[3] = sc[2];              // longest merged row: the capacity the tightened rows need
+    return NB_OK;
+}
+
+extern "C" int32_t nb_hydro_tight(nb_ctx* ctx, void* stream, const double* h_final, int32_t* out_len,
+                                  int32_t* out_index, int64_t row_cap, int64_t* total_out)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (!ctx || !ctx->hydro_ready || !total_out) return ctx_fail_code(ctx, NB_ERR_NOT_BUILT, "nb_hydro_tight: no density pass");
+    if (row_cap < ctx->lists_maxrow) return ctx_fail(ctx, "nb_hydro_tight: row capacity below the longest merged row");
+    const int64_t n = ctx->lists_n;
+    if (ctx_reserve(ctx, (void**)&ctx->scal, &ctx->scal_bytes, sizeof(int64_t) * 4)) return NB_ERR_HIP;
+    unsigned long long* tot = (unsigned long long*)ctx->scal + 3;
+    if (hipMemsetAsync(tot, 0, sizeof(unsigned long long), s) != hipSuccess) return ctx_fail(ctx, "memset");
+    hipLaunchKernelGGL(k_tight_emit, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->gbuf, ctx->dscratch,
+                       (int64_t)ctx->row_cap, ctx->rowcnt, ctx->eoff, ctx->ebuf, ctx->edbuf, h_final, n, out_len,
+                       out_index, row_cap, tot);
+    unsigned long long t = 0;
+    int rc = sync_copy(ctx, s, &t, tot, sizeof(t), "nb_hydro_tight: total copy failed");
+    if (rc) return rc;
+    *total_out = (int64_t)t;
     return NB_OK;
 }

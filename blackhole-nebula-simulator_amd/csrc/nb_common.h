@@ -67,6 +67,28 @@ struct Box {
     double l[3];  // periodic lengths, 0 = open axis
 };
 
+// Neighbour-list views of the pair passes. CSR rows (int64 offsets, the reference's layout) or
+// 64-row blocks: entry k of row i at ((i / 64) cap + k) 64 + i % 64, so that the 64 lanes of a
+// wavefront reading entry k of their own rows touch one contiguous 256-byte segment.
+struct CsrView {
+    const int64_t* off;
+    const int32_t* idx;
+    static constexpr int64_t stride = 1;
+    __device__ __forceinline__ int64_t count(int64_t i) const { return off[i + 1] - off[i]; }
+    __device__ __forceinline__ int64_t base(int64_t i) const { return off[i]; }
+    __device__ __forceinline__ int32_t at(int64_t i, int64_t k) const { return idx[off[i] + k]; }
+};
+
+struct BlkView {
+    const int32_t* idx;
+    const int32_t* len;
+    int64_t cap;
+    static constexpr int64_t stride = 64;
+    __device__ __forceinline__ int64_t count(int64_t i) const { return len[i]; }
+    __device__ __forceinline__ int64_t base(int64_t i) const { return ((i >> 6) * cap) * 64 + (i & 63); }
+    __device__ __forceinline__ int32_t at(int64_t i, int64_t k) const { return idx[base(i) + k * 64]; }
+};
+
 // Particle rows of a pass: rows[q] if a row list is given, else q.
 __device__ __forceinline__ int64_t row_of(const int64_t* rows, int64_t q)
 {

@@ -10,6 +10,8 @@
 #include <stdint.h>
 #include <string>
 
+#include "nb_common.h"
+
 struct nb_ctx {
     int device = 0;
     std::string err;
@@ -23,6 +25,9 @@ struct nb_ctx {
     // gather rows and the transposed extras
     uint32_t* gbuf = nullptr; size_t gbuf_bytes = 0;
     int32_t* rowcnt = nullptr; size_t rowcnt_bytes = 0;
+    uint8_t* unsorted = nullptr; size_t unsorted_bytes = 0;
+    uint8_t* pending = nullptr; size_t pending_bytes = 0;
+    double* edbuf = nullptr; size_t edbuf_bytes = 0;
     unsigned long long* ecount = nullptr; size_t ecount_bytes = 0;
     unsigned long long* eoff = nullptr; size_t eoff_bytes = 0;
     unsigned int* efill = nullptr; size_t efill_bytes = 0;
@@ -38,8 +43,9 @@ struct nb_ctx {
     int64_t* scal = nullptr; size_t scal_bytes = 0;
     void* cub_tmp = nullptr; size_t cub_tmp_bytes = 0;
     int32_t row_cap = 0;
-    int64_t lists_n = 0, lists_total = 0;
+    int64_t lists_n = 0, lists_total = 0, lists_maxrow = 0, last_unsorted = 0;
     int lists_ready = 0;
+    int hydro_ready = 0;
     int64_t last_grid_cells = 0;
     double last_cell = 0.0;
 };

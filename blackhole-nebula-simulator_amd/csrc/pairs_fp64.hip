@@ -13,6 +13,7 @@ static constexpr int kBlockP = 256;
 
 // ----------------------------------------------------------------- density
 // sph.py:215-228 (_sums) over the cached distances r[0:cnt] (stride 1) and masses
+template <int64_t S>
 __device__ __forceinline__ void sums64(const double* __restrict__ r, const double* __restrict__ m,
                                        double m_uniform, int64_t cnt, double h, int kid, double cnorm,
                                        double& rs_out, double& dws_out)
@@ -22,9 +23,9 @@ __device__ __forceinline__ void sums64(const double* __restrict__ r, const doubl
     const double ih3 = ipow3(inv_h);
     const double ncih4 = -cnorm * ipow4(inv_h);
     for (int64_t k = 0; k < cnt; k++) {
-        double q = r[k] * inv_h;
+        double q = r[k * S] * inv_h;
         if (q < 2.0) {
-            double mk = m ? m[k] : m_uniform;
+            double mk = m ? m[k * S] : m_uniform;
             double fq = kernel_f(q, kid);
             rs += mk * cnorm * ih3 * fq;
             dws += mk * (ncih4 * (3.0 * fq + q * kernel_df(q, kid)));
@@ -35,12 +36,14 @@ __device__ __forceinline__ void sums64(const double* __restrict__ r, const doubl
 }
 
 // sph.py:251-297 (_density_kernel). Distances of the entries closer than 2 min(h_search, h_max)
-// are cached once per row at the row's own list offset (rcache[offset[i] + k]); masses too
-// unless every particle has the same mass (m_uniform > 0).
+// are cached once per row in the slots of the row's own list entries (so in blocked rows the
+// cache is read and written 64 rows at a time, contiguously); masses too unless every particle
+// has the same mass (m_uniform > 0).
+template <class L>
 __global__ void __launch_bounds__(kBlockP)
 k_density64(const double* __restrict__ pos, const double* __restrict__ mass, double m_uniform,
             const double* __restrict__ h_start, const double* __restrict__ h_search,
-            const int64_t* __restrict__ offset, const int32_t* __restrict__ index, Box box,
+            L lists, Box box,
             const int64_t* __restrict__ rows, int64_t nrows, int kid, double cnorm, double eta,
             double tol, double h_max, double* __restrict__ rcache, double* __restrict__ mcache,
             double* __restrict__ h_out, double* __restrict__ rho, double* __restrict__ omega,
@@ -53,12 +56,13 @@ k_density64(const double* __restrict__ pos, const double* __restrict__ mass, dou
     const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
     const double r_max = 2.0 * pymin(h_search[i], h_max);
     const double r_max2 = r_max * r_max;
-    const int64_t k0 = offset[i], k1 = offset[i + 1];
-    double* r = rcache + k0;
-    double* m = mcache ? mcache + k0 : nullptr;
+    constexpr int64_t S = L::stride;
+    const int64_t kn = lists.count(i);
+    double* r = rcache + lists.base(i);
+    double* m = mcache ? mcache + lists.base(i) : nullptr;
     int64_t cnt = 0;
-    for (int64_t k = k0; k < k1; k++) {
-        int32_t j = index[k];
+    for (int64_t k = 0; k < kn; k++) {
+        const int32_t j = lists.at(i, k);
         double dx = min_image(pos[3 * j] - xi, box.l[0]);
         double dy = min_image(pos[3 * j + 1] - yi, box.l[1]);
         double dz = min_image(pos[3 * j + 2] - zi, box.l[2]);
@@ -67,8 +71,8 @@ k_density64(const double* __restrict__ pos, const double* __restrict__ mass, dou
         d2 += dy * dy;
         d2 += dz * dz;
         if (d2 < r_max2) {
-            r[cnt] = sqrt(d2);
-            if (m) m[cnt] = mass[j];
+            r[cnt * S] = sqrt(d2);
+            if (m) m[cnt * S] = mass[j];
             cnt++;
         }
     }
@@ -82,7 +86,7 @@ k_density64(const double* __restrict__ pos, const double* __restrict__ mass, dou
     double rs, dws;
     while (it < 50) {
         it += 1;
-        sums64(r, m, m_uniform, cnt, h, kid, cnorm, rs, dws);
+        sums64<S>(r, m, m_uniform, cnt, h, kid, cnorm, rs, dws);
         double f = rs - m_i * ipow3(eta / h);
         double df = dws + 3.0 * m_i * eta3 / ipow4(h);
         if (f > 0) hi = h;
@@ -99,13 +103,13 @@ k_density64(const double* __restrict__ pos, const double* __restrict__ mass, dou
     const double h_lim = pymin(h_search[i], h_max);
     if (h >= h_lim * (1 - 1e-3)) {
         double rs_lim, dum;
-        sums64(r, m, m_uniform, cnt, h_lim, kid, cnorm, rs_lim, dum);
+        sums64<S>(r, m, m_uniform, cnt, h_lim, kid, cnorm, rs_lim, dum);
         if (rs_lim - m_i * ipow3(eta / h_lim) < 0) {
             h = h_lim;
             flag = h_lim >= h_max ? 1 : 2;
         }
     }
-    sums64(r, m, m_uniform, cnt, h, kid, cnorm, rs, dws);
+    sums64<S>(r, m, m_uniform, cnt, h, kid, cnorm, rs, dws);
     h_out[q] = h;
     rho[q] = rs;
     omega[q] = flag == 1 ? 1.0 : 1.0 + h / (3.0 * rs) * dws;
@@ -115,12 +119,13 @@ k_density64(const double* __restrict__ pos, const double* __restrict__ mass, dou
 
 // ----------------------------------------------------------------- forces
 // sph.py:525-652 (_forces_kernel)
+template <class L>
 __global__ void __launch_bounds__(kBlockP)
 k_forces64(const double* __restrict__ pos, const double* __restrict__ vel, const double* __restrict__ mass,
            const double* __restrict__ h, const double* __restrict__ rho, const double* __restrict__ omega,
            const double* __restrict__ pressure, const double* __restrict__ u, const double* __restrict__ c_s,
            const double* __restrict__ alpha, const double* __restrict__ gv,
-           const int64_t* __restrict__ offset, const int32_t* __restrict__ index, Box box,
+           L lists, Box box,
            const int64_t* __restrict__ rows, int64_t nrows, nb_force_params P,
            double* __restrict__ acc, double* __restrict__ dudt, double* __restrict__ vsig_max,
            double* __restrict__ vheat, double* __restrict__ cheat)
@@ -144,8 +149,9 @@ k_forces64(const double* __restrict__ pos, const double* __restrict__ vel, const
     if (use_visc && use_recon)
         for (int k = 0; k < 9; k++) ga_m[k] = gv[9 * a + k];
     const double den_a = om_a * rho_a * rho_a;
-    for (int64_t k = offset[a]; k < offset[a + 1]; k++) {
-        const int32_t b = index[k];
+    const int64_t kn = lists.count(a);
+    for (int64_t k = 0; k < kn; k++) {
+        const int32_t b = lists.at(a, k);
         if (b == a) continue;
         double dx0 = pa0 - pos[3 * b];
         double dx1 = pa1 - pos[3 * b + 1];
@@ -318,10 +324,11 @@ __device__ __forceinline__ bool near_singular(const double* rmat, int64_t cnt)
 // ----------------------------------------------------------------- gradients
 // sph.py:837-884 (_gradients_kernel); with_acc = false means acc is zero.
 // sph.py:785-815 (_velocity_gradient_kernel) when gv_out is given (then div/curl/ddiv unused).
+template <class L>
 __global__ void __launch_bounds__(kBlockP)
 k_gradients64(const double* __restrict__ pos, const double* __restrict__ vel, const double* __restrict__ acc,
               const double* __restrict__ mass, const double* __restrict__ h,
-              const int64_t* __restrict__ offset, const int32_t* __restrict__ index, Box box,
+              L lists, Box box,
               const int64_t* __restrict__ rows, int64_t nrows, int kid, double cnorm,
               double* __restrict__ div, double* __restrict__ curl, double* __restrict__ ddiv,
               double* __restrict__ gv_out)
@@ -335,8 +342,9 @@ k_gradients64(const double* __restrict__ pos, const double* __restrict__ vel, co
     const bool want_acc = acc != nullptr && gv_out == nullptr;
     int64_t cnt = 0;
     const double h_a = h[a];
-    for (int64_t k = offset[a]; k < offset[a + 1]; k++) {
-        const int32_t b = index[k];
+    const int64_t kn = lists.count(a);
+    for (int64_t k = 0; k < kn; k++) {
+        const int32_t b = lists.at(a, k);
         if (b == a) continue;
         cnt += 1;
         double dx[3];
@@ -381,18 +389,20 @@ k_gradients64(const double* __restrict__ pos, const double* __restrict__ vel, co
 
 // ----------------------------------------------------------------- Navier-Stokes
 // sph.py:957-981 (_ns_gradient_kernel), out written at row a
+template <class L>
 __global__ void __launch_bounds__(kBlockP)
 k_ns_gradients64(const double* __restrict__ pos, const double* __restrict__ vel, const double* __restrict__ mass,
                  const double* __restrict__ h, const double* __restrict__ rho, const double* __restrict__ omega,
-                 const int64_t* __restrict__ offset, const int32_t* __restrict__ index, Box box,
+                 L lists, Box box,
                  const int64_t* __restrict__ rows, int64_t nrows, int kid, double cnorm, double* __restrict__ out)
 {
     int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (q >= nrows) return;
     const int64_t a = row_of(rows, q);
     double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int64_t k = offset[a]; k < offset[a + 1]; k++) {
-        const int32_t b = index[k];
+    const int64_t kn = lists.count(a);
+    for (int64_t k = 0; k < kn; k++) {
+        const int32_t b = lists.at(a, k);
         if (b == a) continue;
         double dx[3];
         for (int c = 0; c < 3; c++) dx[c] = min_image(pos[3 * a + c] - pos[3 * b + c], box.l[c]);
@@ -408,10 +418,11 @@ k_ns_gradients64(const double* __restrict__ pos, const double* __restrict__ vel,
 }
 
 // sph.py:1002-1027 (_ns_force_kernel)
+template <class L>
 __global__ void __launch_bounds__(kBlockP)
 k_ns_forces64(const double* __restrict__ pos, const double* __restrict__ mass, const double* __restrict__ h,
               const double* __restrict__ rho, const double* __restrict__ omega, const double* __restrict__ stress,
-              const int64_t* __restrict__ offset, const int32_t* __restrict__ index, Box box,
+              L lists, Box box,
               const int64_t* __restrict__ rows, int64_t nrows, int kid, double cnorm, double* __restrict__ acc)
 {
     int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -419,8 +430,9 @@ k_ns_forces64(const double* __restrict__ pos, const double* __restrict__ mass, c
     const int64_t a = row_of(rows, q);
     const double ia = 1.0 / (omega[a] * rho[a] * rho[a]);
     double ax[3] = {0.0, 0.0, 0.0};
-    for (int64_t k = offset[a]; k < offset[a + 1]; k++) {
-        const int32_t b = index[k];
+    const int64_t kn = lists.count(a);
+    for (int64_t k = 0; k < kn; k++) {
+        const int32_t b = lists.at(a, k);
         if (b == a) continue;
         double dx[3];
         for (int c = 0; c < 3; c++) dx[c] = min_image(pos[3 * a + c] - pos[3 * b + c], box.l[c]);
@@ -438,11 +450,11 @@ k_ns_forces64(const double* __restrict__ pos, const double* __restrict__ mass, c
 
 // ----------------------------------------------------------------- disc viscosity
 // sph.py:1084-1117 (_disc_kernel)
+template <class L>
 __global__ void __launch_bounds__(kBlockP)
 k_disc64(const double* __restrict__ pos, const double* __restrict__ vel, const double* __restrict__ mass,
          const double* __restrict__ h, const double* __restrict__ rho, const double* __restrict__ omega,
-         const double* __restrict__ c_s, double alpha_av, const int64_t* __restrict__ offset,
-         const int32_t* __restrict__ index, Box box, const int64_t* __restrict__ rows, int64_t nrows,
+         const double* __restrict__ c_s, double alpha_av, L lists, Box box, const int64_t* __restrict__ rows, int64_t nrows,
          int kid, double cnorm, double* __restrict__ acc, double* __restrict__ dudt)
 {
     int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -451,8 +463,9 @@ k_disc64(const double* __restrict__ pos, const double* __restrict__ vel, const d
     const double ia = 1.0 / (omega[a] * rho[a] * rho[a]);
     double ax[3] = {0.0, 0.0, 0.0};
     double heat = 0.0;
-    for (int64_t k = offset[a]; k < offset[a + 1]; k++) {
-        const int32_t b = index[k];
+    const int64_t kn = lists.count(a);
+    for (int64_t k = 0; k < kn; k++) {
+        const int32_t b = lists.at(a, k);
         if (b == a) continue;
         double dx[3];
         for (int c = 0; c < 3; c++) dx[c] = min_image(pos[3 * a + c] - pos[3 * b + c], box.l[c]);
@@ -494,104 +507,110 @@ static int32_t launch_check(nb_ctx* ctx, const char* what)
     return NB_OK;
 }
 
-extern "C" int32_t nb_density(nb_ctx* ctx, void* stream, const double* pos, const double* mass,
-                              double m_uniform, const double* h_start, const double* h_search,
-                              const int64_t* offset, const int32_t* index, int64_t n_entries,
-                              const double* box3, const int64_t* rows, int64_t nrows, int32_t kid,
-                              double cnorm, double eta, double tol, double h_max, double* h_out,
-                              double* rho, double* omega, int32_t* iters, int32_t* flags)
+#define NB_DISPATCH(L, KERNEL, GRID, STREAM, ...)                                                       \
+    do {                                                                                                \
+        if ((L)->length) {                                                                              \
+            BlkView v_{(L)->index, (L)->length, (L)->row_cap};                                          \
+            hipLaunchKernelGGL(KERNEL<BlkView>, dim3(GRID), dim3(kBlockP), 0, STREAM, __VA_ARGS__);      \
+        } else {                                                                                        \
+            CsrView v_{(L)->offset, (L)->index};                                                        \
+            hipLaunchKernelGGL(KERNEL<CsrView>, dim3(GRID), dim3(kBlockP), 0, STREAM, __VA_ARGS__);      \
+        }                                                                                               \
+    } while (0)
+
+static bool lists_ok(const nb_lists* L) { return L && L->index && (L->length || L->offset); }
+
+extern "C" int32_t nb_density(nb_ctx* ctx, void* stream, const double* pos, const double* mass, double m_uniform,
+                              const double* h_start, const double* h_search, const nb_lists* lists,
+                              const double* box3, const int64_t* rows, int64_t nrows, int32_t kid, double cnorm,
+                              double eta, double tol, double h_max, double* h_out, double* rho, double* omega,
+                              int32_t* iters, int32_t* flags)
 {
-    if (!ctx || nrows < 0 || n_entries < 0) return ctx_fail(ctx, "nb_density: bad arguments");
+    if (!ctx || nrows < 0 || !lists_ok(lists) || lists->n_entries < 0) return ctx_fail(ctx, "nb_density: bad arguments");
     if (nrows == 0) return NB_OK;
-    size_t need = sizeof(double) * (size_t)(n_entries + 1) * (m_uniform > 0.0 ? 1 : 2);
+    const int64_t ne = lists->n_entries;
+    size_t need = sizeof(double) * (size_t)(ne + 1) * (m_uniform > 0.0 ? 1 : 2);
     if (ctx_reserve(ctx, (void**)&ctx->dscratch, &ctx->dscratch_bytes, need)) return NB_ERR_HIP;
-    double* mcache = m_uniform > 0.0 ? nullptr : ctx->dscratch + (n_entries + 1);
-    hipLaunchKernelGGL(k_density64, dim3(grid_p(nrows)), dim3(kBlockP), 0, (hipStream_t)stream, pos, mass,
-                       m_uniform, h_start, h_search, offset, index, make_box(box3), rows, nrows, kid, cnorm,
-                       eta, tol, h_max, ctx->dscratch, mcache, h_out, rho, omega, iters, flags);
+    double* mcache = m_uniform > 0.0 ? nullptr : ctx->dscratch + (ne + 1);
+    NB_DISPATCH(lists, k_density64, grid_p(nrows), (hipStream_t)stream, pos, mass, m_uniform, h_start, h_search, v_,
+                make_box(box3), rows, nrows, kid, cnorm, eta, tol, h_max, ctx->dscratch, mcache, h_out, rho, omega,
+                iters, flags);
     return launch_check(ctx, "nb_density");
 }
 
-extern "C" int32_t nb_forces(nb_ctx* ctx, void* stream, const double* pos, const double* vel,
-                             const double* mass, const double* h, const double* rho, const double* omega,
-                             const double* pressure, const double* u, const double* c_s, const double* alpha,
-                             const double* grad_v, const int64_t* offset, const int32_t* index,
-                             const double* box3, const int64_t* rows, int64_t nrows,
-                             const nb_force_params* params, double* acc, double* dudt, double* vsig_max,
-                             double* vheat, double* cheat)
+extern "C" int32_t nb_forces(nb_ctx* ctx, void* stream, const double* pos, const double* vel, const double* mass,
+                             const double* h, const double* rho, const double* omega, const double* pressure,
+                             const double* u, const double* c_s, const double* alpha, const double* grad_v,
+                             const nb_lists* lists, const double* box3, const int64_t* rows, int64_t nrows,
+                             const nb_force_params* params, double* acc, double* dudt, double* vsig_max, double* vheat,
+                             double* cheat)
 {
-    if (!ctx || !params || nrows < 0) return ctx_fail(ctx, "nb_forces: bad arguments");
+    if (!ctx || !params || nrows < 0 || !lists_ok(lists)) return ctx_fail(ctx, "nb_forces: bad arguments");
     if (params->use_visc && (!alpha || !c_s)) return ctx_fail(ctx, "nb_forces: viscosity needs alpha and c_s");
     if (params->use_cond && !u) return ctx_fail(ctx, "nb_forces: conductivity needs u");
     if (params->use_recon && !grad_v) return ctx_fail(ctx, "nb_forces: reconstruction needs grad_v");
     if (nrows == 0) return NB_OK;
-    hipLaunchKernelGGL(k_forces64, dim3(grid_p(nrows)), dim3(kBlockP), 0, (hipStream_t)stream, pos, vel, mass,
-                       h, rho, omega, pressure, u, c_s, alpha, grad_v, offset, index, make_box(box3), rows,
-                       nrows, *params, acc, dudt, vsig_max, vheat, cheat);
+    NB_DISPATCH(lists, k_forces64, grid_p(nrows), (hipStream_t)stream, pos, vel, mass, h, rho, omega, pressure, u, c_s,
+                alpha, grad_v, v_, make_box(box3), rows, nrows, *params, acc, dudt, vsig_max, vheat, cheat);
     return launch_check(ctx, "nb_forces");
 }
 
-extern "C" int32_t nb_gradients(nb_ctx* ctx, void* stream, const double* pos, const double* vel,
-                                const double* acc, const double* mass, const double* h,
-                                const int64_t* offset, const int32_t* index, const double* box3,
-                                const int64_t* rows, int64_t nrows, int32_t kid, double cnorm,
-                                double* div, double* curl, double* ddiv)
+extern "C" int32_t nb_gradients(nb_ctx* ctx, void* stream, const double* pos, const double* vel, const double* acc,
+                                const double* mass, const double* h, const nb_lists* lists, const double* box3,
+                                const int64_t* rows, int64_t nrows, int32_t kid, double cnorm, double* div,
+                                double* curl, double* ddiv)
 {
-    if (!ctx || nrows < 0) return ctx_fail(ctx, "nb_gradients: bad arguments");
+    if (!ctx || nrows < 0 || !lists_ok(lists)) return ctx_fail(ctx, "nb_gradients: bad arguments");
     if (nrows == 0) return NB_OK;
-    hipLaunchKernelGGL(k_gradients64, dim3(grid_p(nrows)), dim3(kBlockP), 0, (hipStream_t)stream, pos, vel,
-                       acc, mass, h, offset, index, make_box(box3), rows, nrows, kid, cnorm, div, curl, ddiv,
-                       (double*)nullptr);
+    NB_DISPATCH(lists, k_gradients64, grid_p(nrows), (hipStream_t)stream, pos, vel, acc, mass, h, v_, make_box(box3),
+                rows, nrows, kid, cnorm, div, curl, ddiv, (double*)nullptr);
     return launch_check(ctx, "nb_gradients");
 }
 
 extern "C" int32_t nb_velocity_gradients(nb_ctx* ctx, void* stream, const double* pos, const double* vel,
-                                         const double* mass, const double* h, const int64_t* offset,
-                                         const int32_t* index, const double* box3, const int64_t* rows,
-                                         int64_t nrows, int32_t kid, double cnorm, double* out)
+                                         const double* mass, const double* h, const nb_lists* lists,
+                                         const double* box3, const int64_t* rows, int64_t nrows, int32_t kid,
+                                         double cnorm, double* out)
 {
-    if (!ctx || nrows < 0 || !out) return ctx_fail(ctx, "nb_velocity_gradients: bad arguments");
+    if (!ctx || nrows < 0 || !out || !lists_ok(lists)) return ctx_fail(ctx, "nb_velocity_gradients: bad arguments");
     if (nrows == 0) return NB_OK;
-    hipLaunchKernelGGL(k_gradients64, dim3(grid_p(nrows)), dim3(kBlockP), 0, (hipStream_t)stream, pos, vel,
-                       (const double*)nullptr, mass, h, offset, index, make_box(box3), rows, nrows, kid, cnorm,
-                       (double*)nullptr, (double*)nullptr, (double*)nullptr, out);
+    NB_DISPATCH(lists, k_gradients64, grid_p(nrows), (hipStream_t)stream, pos, vel, (const double*)nullptr, mass, h, v_,
+                make_box(box3), rows, nrows, kid, cnorm, (double*)nullptr, (double*)nullptr, (double*)nullptr, out);
     return launch_check(ctx, "nb_velocity_gradients");
 }
 
-extern "C" int32_t nb_ns_gradients(nb_ctx* ctx, void* stream, const double* pos, const double* vel,
-                                   const double* mass, const double* h, const double* rho, const double* omega,
-                                   const int64_t* offset, const int32_t* index, const double* box3,
-                                   const int64_t* rows, int64_t nrows, int32_t kid, double cnorm, double* out)
+extern "C" int32_t nb_ns_gradients(nb_ctx* ctx, void* stream, const double* pos, const double* vel, const double* mass,
+                                   const double* h, const double* rho, const double* omega, const nb_lists* lists,
+                                   const double* box3, const int64_t* rows, int64_t nrows, int32_t kid, double cnorm,
+                                   double* out)
 {
-    if (!ctx || nrows < 0 || !out) return ctx_fail(ctx, "nb_ns_gradients: bad arguments");
+    if (!ctx || nrows < 0 || !out || !lists_ok(lists)) return ctx_fail(ctx, "nb_ns_gradients: bad arguments");
     if (nrows == 0) return NB_OK;
-    hipLaunchKernelGGL(k_ns_gradients64, dim3(grid_p(nrows)), dim3(kBlockP), 0, (hipStream_t)stream, pos, vel,
-                       mass, h, rho, omega, offset, index, make_box(box3), rows, nrows, kid, cnorm, out);
+    NB_DISPATCH(lists, k_ns_gradients64, grid_p(nrows), (hipStream_t)stream, pos, vel, mass, h, rho, omega, v_,
+                make_box(box3), rows, nrows, kid, cnorm, out);
     return launch_check(ctx, "nb_ns_gradients");
 }
 
-extern "C" int32_t nb_ns_forces(nb_ctx* ctx, void* stream, const double* pos, const double* mass,
-                                const double* h, const double* rho, const double* omega, const double* stress,
-                                const int64_t* offset, const int32_t* index, const double* box3,
-                                const int64_t* rows, int64_t nrows, int32_t kid, double cnorm, double* acc)
+extern "C" int32_t nb_ns_forces(nb_ctx* ctx, void* stream, const double* pos, const double* mass, const double* h,
+                                const double* rho, const double* omega, const double* stress, const nb_lists* lists,
+                                const double* box3, const int64_t* rows, int64_t nrows, int32_t kid, double cnorm,
+                                double* acc)
 {
-    if (!ctx || nrows < 0) return ctx_fail(ctx, "nb_ns_forces: bad arguments");
+    if (!ctx || nrows < 0 || !lists_ok(lists)) return ctx_fail(ctx, "nb_ns_forces: bad arguments");
     if (nrows == 0) return NB_OK;
-    hipLaunchKernelGGL(k_ns_forces64, dim3(grid_p(nrows)), dim3(kBlockP), 0, (hipStream_t)stream, pos, mass, h,
-                       rho, omega, stress, offset, index, make_box(box3), rows, nrows, kid, cnorm, acc);
+    NB_DISPATCH(lists, k_ns_forces64, grid_p(nrows), (hipStream_t)stream, pos, mass, h, rho, omega, stress, v_,
+                make_box(box3), rows, nrows, kid, cnorm, acc);
     return launch_check(ctx, "nb_ns_forces");
 }
 
-extern "C" int32_t nb_disc_forces(nb_ctx* ctx, void* stream, const double* pos, const double* vel,
-                                  const double* mass, const double* h, const double* rho, const double* omega,
-                                  const double* c_s, double alpha_av, const int64_t* offset,
-                                  const int32_t* index, const double* box3, const int64_t* rows, int64_t nrows,
-                                  int32_t kid, double cnorm, double* acc, double* dudt)
+extern "C" int32_t nb_disc_forces(nb_ctx* ctx, void* stream, const double* pos, const double* vel, const double* mass,
+                                  const double* h, const double* rho, const double* omega, const double* c_s,
+                                  double alpha_av, const nb_lists* lists, const double* box3, const int64_t* rows,
+                                  int64_t nrows, int32_t kid, double cnorm, double* acc, double* dudt)
 {
-    if (!ctx || nrows < 0) return ctx_fail(ctx, "nb_disc_forces: bad arguments");
+    if (!ctx || nrows < 0 || !lists_ok(lists)) return ctx_fail(ctx, "nb_disc_forces: bad arguments");
     if (nrows == 0) return NB_OK;
-    hipLaunchKernelGGL(k_disc64, dim3(grid_p(nrows)), dim3(kBlockP), 0, (hipStream_t)stream, pos, vel, mass, h,
-                       rho, omega, c_s, alpha_av, offset, index, make_box(box3), rows, nrows, kid, cnorm, acc,
-                       dudt);
+    NB_DISPATCH(lists, k_disc64, grid_p(nrows), (hipStream_t)stream, pos, vel, mass, h, rho, omega, c_s, alpha_av, v_,
+                make_box(box3), rows, nrows, kid, cnorm, acc, dudt);
     return launch_check(ctx, "nb_disc_forces");
 }

@@ -258,27 +258,29 @@ __global__ void k_ledger_partial(const double* __restrict__ x, const double* __r
         for (int k = 0; k < 6; k++) part[6 * blockIdx.x + k] = sh[k][0];
 }
 
-// Final pass of a reduction over `nparts` partial rows of width w: op 0 = min, 1 = sum.
-// Sums use compensated (Kahan-Babuska) accumulation over the partials.
+// Final pass of a reduction over `nparts` partial rows of width w (w <= 8): op 0 = min, 1 = sum.
+// One block: min in any order; sums as per-thread partials combined in a fixed tree.
 __global__ void k_reduce_final(const double* __restrict__ part, int nparts, int w, int op, double* __restrict__ out)
 {
-    int k = threadIdx.x;
-    if (k >= w || blockIdx.x != 0) return;
-    if (op == 0) {
-        double v = INFINITY;
-        for (int b = 0; b < nparts; b++) v = fmin(v, part[w * b + k]);
-        out[k] = v;
-    } else {
-        double s = 0.0, c = 0.0;
-        for (int b = 0; b < nparts; b++) {
+    __shared__ double sh[8][kBlockS];
+    for (int k = 0; k < w; k++) {
+        double v = op == 0 ? INFINITY : 0.0;
+        for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
             double y = part[w * b + k];
-            double t = s + y;
-            if (fabs(s) >= fabs(y)) c += (s - t) + y;
-            else c += (y - t) + s;
-            s = t;
+            v = op == 0 ? fmin(v, y) : v + y;
         }
-        out[k] = s + c;
+        sh[k][threadIdx.x] = v;
     }
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int k = 0; k < w; k++)
+                sh[k][threadIdx.x] = op == 0 ? fmin(sh[k][threadIdx.x], sh[k][threadIdx.x + s])
+                                             : sh[k][threadIdx.x] + sh[k][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < w; k++) out[k] = sh[k][0];
 }
 
 static inline unsigned grid_s(int64_t n) { return (unsigned)((n + kBlockS - 1) / kBlockS); }
@@ -361,7 +363,7 @@ extern "C" int32_t nb_cool_powerlaw(nb_ctx* ctx, void* stream, const double* u, 
 static int32_t reduce_two_level(nb_ctx* ctx, hipStream_t s, int w, int op, int nparts, double* host_out)
 {
     double* fin = ctx->red + (size_t)w * nparts;
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, s, ctx->red, nparts, w, op, fin);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlockS), 0, s, ctx->red, nparts, w, op, fin);
     if (hipMemcpyAsync(host_out, fin, sizeof(double) * w, hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
         return ctx_fail(ctx, "reduction copy failed");

@@ -47,6 +47,20 @@ class ForceParams(ctypes.Structure):
     ]
 
 
+class Lists(ctypes.Structure):
+    """nb_lists (include/nebula_hip.h): CSR when length is NULL, 64-row blocks otherwise."""
+
+    _fields_ = [
+        ("offset", _vp),
+        ("index", _vp),
+        ("length", _vp),
+        ("row_cap", _i64),
+        ("n_entries", _i64),
+    ]
+
+
+_PL = ctypes.POINTER(Lists)
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "nb_ctx_create": (_vp, [_i32]),
@@ -57,20 +71,13 @@ SIGNATURES = {
     "nb_lists_build": (_i32, [_vp, _vp, _vp, _vp, _i64, _PD, _vp, _vp, _i64, _P64]),
     "nb_lists_emit": (_i32, [_vp, _vp, _vp, _i64]),
     "nb_tighten": (_i32, [_vp, _vp, _vp, _vp, _i64, _PD, _vp, _vp, _vp, _i64, _vp, _vp, _i64, _P64]),
-    "nb_density": (_i32, [_vp, _vp, _vp, _vp, _f64, _vp, _vp, _vp, _vp, _i64, _PD, _vp, _i64, _i32,
-                          _f64, _f64, _f64, _f64, _vp, _vp, _vp, _vp, _vp]),
-    "nb_forces": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-                         _PD, _vp, _i64, ctypes.POINTER(ForceParams), _vp, _vp, _vp, _vp, _vp]),
-    "nb_gradients": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _PD, _vp, _i64, _i32, _f64,
-                            _vp, _vp, _vp]),
-    "nb_velocity_gradients": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _PD, _vp, _i64, _i32,
-                                     _f64, _vp]),
-    "nb_ns_gradients": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _PD, _vp, _i64,
-                               _i32, _f64, _vp]),
-    "nb_ns_forces": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _PD, _vp, _i64, _i32,
-                            _f64, _vp]),
-    "nb_disc_forces": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f64, _vp, _vp, _PD, _vp,
-                              _i64, _i32, _f64, _vp, _vp]),
+    "nb_density": (_i32, [_vp, _vp, _vp, _vp, _f64, _vp, _vp, _PL, _PD, _vp, _i64, _i32, _f64, _f64, _f64, _f64, _vp, _vp, _vp, _vp, _vp]),
+    "nb_forces": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _PL, _PD, _vp, _i64, ctypes.POINTER(ForceParams), _vp, _vp, _vp, _vp, _vp]),
+    "nb_gradients": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _PL, _PD, _vp, _i64, _i32, _f64, _vp, _vp, _vp]),
+    "nb_velocity_gradients": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _PL, _PD, _vp, _i64, _i32, _f64, _vp]),
+    "nb_ns_gradients": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _PL, _PD, _vp, _i64, _i32, _f64, _vp]),
+    "nb_ns_forces": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _PL, _PD, _vp, _i64, _i32, _f64, _vp]),
+    "nb_disc_forces": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f64, _PL, _PD, _vp, _i64, _i32, _f64, _vp, _vp]),
     "nb_glf_step": (_i32, [_vp, _vp, _vp, _vp, _vp, _f64, _i64, _vp, _vp, _vp, _vp]),
     "nb_gn_v_from_p": (_i32, [_vp, _vp, _vp, _vp, _i64, _vp]),
     "nb_axpy": (_i32, [_vp, _vp, _vp, _vp, _f64, _i64, _vp]),
@@ -88,6 +95,16 @@ SIGNATURES = {
     "nb_nu_av": (_i32, [_vp, _vp, _vp, _vp, _f64, _i64, _vp]),
     "nb_add": (_i32, [_vp, _vp, _vp, _vp, _i64, _vp]),
     "nb_sum_into": (_i32, [_vp, _vp, _vp, _i64, _vp]),
+    "nb_lists_build_blocked": (_i32, [_vp, _vp, _vp, _vp, _i64, _PD, _vp, _vp, _i64, _P64, _P64]),
+    "nb_lists_emit_blocked": (_i32, [_vp, _vp, _vp, _vp, _i64]),
+    "nb_lists_stats": (_i32, [_vp, _P64]),
+    "nb_grid_order": (_i32, [_vp, _vp, _vp, _vp, _i64, _PD, _vp]),
+    "nb_permute": (_i32, [_vp, _vp, _vp, _vp, _i64, _i32, _vp]),
+    "nb_permute_i64": (_i32, [_vp, _vp, _vp, _vp, _i64, _vp]),
+    "nb_tighten_blocked": (_i32, [_vp, _vp, _vp, _vp, _i64, _PD, _vp, _vp, _i64, _vp, _vp, _P64]),
+    "nb_hydro_density": (_i32, [_vp, _vp, _vp, _vp, _f64, _vp, _vp, _i64, _i32, _f64, _f64, _f64, _f64, _i32, _vp,
+                                _vp, _vp, _vp, _vp, _P64]),
+    "nb_hydro_tight": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _P64]),
 }
 
 

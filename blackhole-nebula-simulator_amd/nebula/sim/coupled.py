@@ -9,6 +9,13 @@ equation of state, tightening, viscosity switch, forces, time-step minima and th
 The host only sequences the launches and reads back a few scalars (list sizes, rebuild
 counts, the step's minima).
 
+Particle order. Before each density pass the particles are put in the order of the neighbour
+grid (cells in raster order; nb_grid_order), so that every gathered neighbour row comes out
+sorted and the lanes of a wavefront work on neighbouring particles. ``GasState.perm`` maps the
+stored order back to the caller's (``to_numpy`` returns the caller's order). Sums over a row run
+in the stored order, so results equal those of the reference's arithmetic applied to the
+particles in that order; the oracle applies the same ordering.
+
 Each step (coupled.py:238-258): hydro half-kick; GN flow over dt; cooling in the middle of
 the split; hydro forces at the new positions with the predicted velocity; second half-kick.
 """
@@ -26,10 +33,10 @@ from ..physics import units
 from .fastphys import gn_v_from_p_device
 from .hydro import HydroConfig
 from .integrator import glf_device
-from .neighbours import tighten
+from .neighbours import grid_order, permute
 from .sph import (
     COND_VELOCITY,
-    compute_density_device,
+    hydro_density_device,
     density_device,
     disc_forces_device,
     forces_device,
@@ -92,17 +99,27 @@ class GasState:
     builds: int = 0
     n_search_entries: int = 0  # search-list entries of the last density pass (incl. self)
     n_tight_entries: int = 0  # tightened-list entries of the last force pass (incl. self)
+    perm: torch.Tensor = field(default=None)  # caller index of each stored particle (int64)
+    row_cap: int = 0
 
     @property
     def e_radiated(self) -> float:
         return 0.0 if self.e_radiated_dev is None else float(self.e_radiated_dev.item())
 
     def to_numpy(self) -> dict:
+        """Host copies in the caller's particle order."""
         out = {}
+        perm = _dev.host(self.perm)
         for k in ("x", "p", "u", "mass", "h", "alpha", "a_hydro", "dudt", "rho", "c_s", "vsig_max", "visc_heating",
                   "nu", "omega", "iterations", "flags"):  # fmt: skip
             v = getattr(self, k)
-            out[k] = None if v is None else _dev.host(v)
+            if v is None:
+                out[k] = None
+                continue
+            a = _dev.host(v)
+            b = np.empty_like(a)
+            b[perm] = a
+            out[k] = b
         out["t"] = self.t
         out["e_radiated"] = self.e_radiated
         return out
@@ -126,6 +143,19 @@ def _full(n, v):
     t = _dev.empty(n)
     _hip.call("nb_fill", _hip.ptr(t), float(v), n)
     return t
+
+
+def _search_lengths(cfg: CoupledConfig, h):
+    n = int(h.shape[0])
+    out = _dev.empty(n)
+    _hip.call("nb_search_lengths", _hip.ptr(h), float(cfg.hydro.h_max), float(cfg.hydro.skin), float(np.inf), n,
+              _hip.ptr(out))  # fmt: skip
+    return out
+
+
+def grid_permutation(cfg: CoupledConfig, x, h):
+    """Order of the neighbour grid the next density pass will build (search lengths from h)."""
+    return grid_order(x, _search_lengths(cfg, h), None)
 
 
 def velocity(cfg: CoupledConfig, x, p):
@@ -164,8 +194,8 @@ def _hydro(st: GasState, cfg: CoupledConfig, v, dt_alpha: float, a_prev) -> GasS
     if hc.isothermal_cs is not None or hc.eos != "ideal":
         raise ValueError("the GPU step implements the ideal-gas equation of state")
     n = int(st.x.shape[0])
-    d = compute_density_device(st.x, st.mass, st.h.clone(), None, hc.kernel, hc.eta, 1e-4, hc.h_max, hc.skin,
-                               m_uniform=st.m_uniform, capacity_hint=st.list_capacity)  # fmt: skip
+    d, n_search = hydro_density_device(st.x, st.mass, st.h.clone(), hc.kernel, hc.eta, 1e-4, hc.h_max, hc.skin,
+                                       st.m_uniform, hc.precision)  # fmt: skip
     p_gas = _dev.empty(n)
     cs = _dev.empty(n)
     nbad = torch.zeros(1, dtype=torch.int32, device=st.x.device)
@@ -174,7 +204,7 @@ def _hydro(st: GasState, cfg: CoupledConfig, v, dt_alpha: float, a_prev) -> GasS
     bad = int(nbad.item())
     if bad:
         raise FloatingPointError(f"non-positive internal energy at {bad} particles")
-    nb = tighten(d.neighbours, st.x, d.h)
+    nb = d.neighbours  # tightened at the final h
     alpha = st.alpha
     if hc.viscosity and hc.switch and dt_alpha > 0:
         g = gradients_device(st.x, v, a_prev, st.mass, d.h, nb, hc.kernel)
@@ -223,8 +253,8 @@ def _hydro(st: GasState, cfg: CoupledConfig, v, dt_alpha: float, a_prev) -> GasS
         st, h=d.h, alpha=alpha, a_hydro=acc, dudt=dudt, rho=d.rho, c_s=cs,
         vsig_max=f.vsig_max if hc.viscosity else cs, visc_heating=f.dudt_visc, nu=nu, omega=d.omega,
         iterations=d.iterations, flags=d.flags, builds=d.builds,
-        list_capacity=max(st.list_capacity, d.neighbours.n_entries), n_search_entries=d.neighbours.n_entries,
-        n_tight_entries=nb.n_entries,
+        n_search_entries=n_search,
+        n_tight_entries=nb.n_total,
     )  # fmt: skip
 
 
@@ -243,8 +273,11 @@ def initialise(x, v, u, mass, h_guess, cfg: CoupledConfig, t: float = 0.0) -> Ga
         _dev.f64(np.full(n, hc.alpha_init if hc.switch else hc.alpha_const)),
         e_radiated_dev=torch.zeros(1, dtype=torch.float64, device=_hip.torch_device()), m_uniform=m_uniform,
     )  # fmt: skip
-    x_d, p_d = st.x, st.p
-    return _hydro(st, cfg, velocity(cfg, x_d, p_d), 0.0, None)
+    order = grid_permutation(cfg, st.x, st.h)
+    st = replace(st, x=permute(st.x, order), p=permute(st.p, order), u=permute(st.u, order),
+                 mass=permute(st.mass, order), h=permute(st.h, order), alpha=permute(st.alpha, order),
+                 perm=order.to(torch.int64))  # fmt: skip
+    return _hydro(st, cfg, velocity(cfg, st.x, st.p), 0.0, None)
 
 
 def stable_dt(st: GasState, cfg: CoupledConfig) -> float:
@@ -294,8 +327,13 @@ def step(st: GasState, cfg: CoupledConfig, dt: float) -> GasState:
             _hip.call("nb_sum_into", _hip.ptr(e_part), n, _hip.ptr(e_rad))
             u_half = u_cool
     p_pred = _axpy(p_flow, st.a_hydro, half)
-    pred = replace(st, t=st.t + dt, x=x1, p=p_flow, u=_axpy(u_half, st.dudt, half))
-    new = _hydro(pred, cfg, velocity(cfg, x1, p_pred), dt, st.a_hydro)
+    u_pred = _axpy(u_half, st.dudt, half)
+    order = grid_permutation(cfg, x1, st.h)
+    x1, p_flow, p_pred, u_half, u_pred, a_prev = (permute(t, order) for t in (x1, p_flow, p_pred, u_half, u_pred,
+                                                                                st.a_hydro))  # fmt: skip
+    pred = replace(st, t=st.t + dt, x=x1, p=p_flow, u=u_pred, mass=permute(st.mass, order), h=permute(st.h, order),
+                   alpha=permute(st.alpha, order), perm=permute(st.perm, order))  # fmt: skip
+    new = _hydro(pred, cfg, velocity(cfg, x1, p_pred), dt, a_prev)
     return replace(new, p=_axpy(p_flow, new.a_hydro, half), u=_axpy(u_half, new.dudt, half), e_radiated_dev=e_rad)
 
 

@@ -73,6 +73,104 @@ class Neighbours:
     def counts(self) -> np.ndarray:
         return np.diff(self.offset)
 
+    def desc(self) -> _hip.Lists:
+        """nb_lists descriptor (CSR); the tensors stay referenced by this object."""
+        d = _hip.Lists()
+        d.offset = self.d_offset.data_ptr()
+        d.index = self.d_index.data_ptr()
+        d.length = None
+        d.row_cap = 0
+        d.n_entries = self.n_entries
+        return d
+
+
+class BlockedLists:
+    """Neighbour lists in 64-row blocks on the device (the step's own layout): row i has
+    ``length[i]`` entries, entry k at ((i // 64) * row_cap + k) * 64 + i % 64."""
+
+    def __init__(self, length: torch.Tensor, index: torch.Tensor, row_cap: int, n_total: int, box=None):
+        self.length = length
+        self.index = index
+        self.row_cap = int(row_cap)
+        self.n_total = int(n_total)
+        self.box = box
+
+    @property
+    def n_entries(self) -> int:
+        return int(self.index.numel())
+
+    def desc(self) -> _hip.Lists:
+        d = _hip.Lists()
+        d.offset = None
+        d.index = self.index.data_ptr()
+        d.length = self.length.data_ptr()
+        d.row_cap = self.row_cap
+        d.n_entries = self.n_entries
+        return d
+
+    def to_csr(self) -> Neighbours:
+        """The same lists as a CSR Neighbours (host arrays)."""
+        n = int(self.length.numel())
+        ln = self.length.cpu().numpy().astype(np.int64)
+        idx = self.index.cpu().numpy().reshape(-1, self.row_cap, 64)
+        off = np.zeros(n + 1, np.int64)
+        np.cumsum(ln, out=off[1:])
+        out = np.empty(off[-1], np.int64)
+        for i in range(n):
+            out[off[i]:off[i + 1]] = idx[i // 64, : ln[i], i % 64]
+        return Neighbours(off, out, self.box)
+
+
+def _blocked_alloc(n: int, row_cap: int):
+    nblk = (n + 63) // 64
+    return _dev.empty(n, torch.int32), _dev.empty(max(nblk * row_cap * 64, 1), torch.int32)
+
+
+def build_lists_blocked(pos_d: torch.Tensor, h_d: torch.Tensor, box_arr, row_cap_hint: int = 0) -> BlockedLists:
+    """Symmetric lists of device positions at search lengths h, in 64-row blocks."""
+    n = int(pos_d.shape[0])
+    cap = max(int(row_cap_hint), 16)
+    need = np.zeros(1, np.int64)
+    total = np.zeros(1, np.int64)
+    ln, idx = _blocked_alloc(n, cap)
+    rc = _hip.call("nb_lists_build_blocked", _hip.ptr(pos_d), _hip.ptr(h_d), n, _hip.box_arg(box_arr), _hip.ptr(ln),
+                   _hip.ptr(idx), cap, need.ctypes.data_as(_hip._P64), total.ctypes.data_as(_hip._P64))  # fmt: skip
+    if rc == _hip.NB_RETRY_CAPACITY:
+        cap = int(need[0]) + int(need[0]) // 8 + 8
+        ln, idx = _blocked_alloc(n, cap)
+        _hip.call("nb_lists_emit_blocked", _hip.ptr(ln), _hip.ptr(idx), cap)
+    return BlockedLists(ln, idx, cap, int(total[0]), box_arr)
+
+
+def tighten_blocked(nb: BlockedLists, pos_d: torch.Tensor, h_d: torch.Tensor) -> BlockedLists:
+    """tighten (neighbours.py:306-323) of blocked lists into blocked lists of the same capacity."""
+    n = int(pos_d.shape[0])
+    ln, idx = _blocked_alloc(n, nb.row_cap)
+    total = np.zeros(1, np.int64)
+    _hip.call("nb_tighten_blocked", _hip.ptr(pos_d), _hip.ptr(h_d), n, _hip.box_arg(nb.box), _hip.ptr(nb.length),
+              _hip.ptr(nb.index), nb.row_cap, _hip.ptr(ln), _hip.ptr(idx), total.ctypes.data_as(_hip._P64))  # fmt: skip
+    return BlockedLists(ln, idx, nb.row_cap, int(total[0]), nb.box)
+
+
+def grid_order(pos_d: torch.Tensor, h_d: torch.Tensor, box_arr=None) -> torch.Tensor:
+    """The neighbour grid's particle order (int32) for these positions and search lengths."""
+    n = int(pos_d.shape[0])
+    order = _dev.empty(n, torch.int32)
+    _hip.call("nb_grid_order", _hip.ptr(pos_d), _hip.ptr(h_d), n, _hip.box_arg(box_arr), _hip.ptr(order))
+    return order
+
+
+def permute(t: torch.Tensor, order: torch.Tensor) -> torch.Tensor:
+    """Rows of t (float64 (N,...) or int64 (N,)) taken in ``order``."""
+    n = int(order.numel())
+    out = torch.empty_like(t)
+    if t.dtype == torch.int64:
+        _hip.call("nb_permute_i64", _hip.ptr(t), _hip.ptr(order), n, _hip.ptr(out))
+    else:
+        width = int(t.numel() // max(n, 1))
+        _hip.call("nb_permute", _hip.ptr(t), _hip.ptr(order), n, width, _hip.ptr(out))
+    return out
+
 
 def minimum_image(dx: np.ndarray, box) -> np.ndarray:
     """Separation vectors folded into (-L/2, L/2] (neighbours.py:39-43)."""

@@ -11,12 +11,14 @@ from __future__ import annotations
 
 from dataclasses import dataclass
 
+import ctypes
+
 import numpy as np
 import torch
 
 from .. import _dev, _hip
 from ..physics.kernels import C_NORM, DEFAULT_ETA, DEFAULT_KERNEL, KERNEL_IDS
-from .neighbours import Neighbours, build_lists
+from .neighbours import Neighbours, build_lists, build_lists_blocked
 
 FLAG_OK = 0
 FLAG_AT_H_MAX = 1
@@ -75,8 +77,7 @@ def density_device(pos_d, mass_d, h_start_d, nb: Neighbours, h_search_d, kernel_
     iters = _dev.empty(nrows, torch.int32)
     flags = _dev.empty(nrows, torch.int32)
     _hip.call("nb_density", _hip.ptr(pos_d), _hip.ptr(mass_d), float(m_uniform), _hip.ptr(h_start_d),
-              _hip.ptr(h_search_d), _hip.ptr(nb.d_offset), _hip.ptr(nb.d_index), nb.n_entries,
-              _hip.box_arg(nb.box), _hip.ptr(rows_d), nrows, kid, float(C_NORM[kid]), float(eta),
+              _hip.ptr(h_search_d), ctypes.byref(nb.desc()), _hip.box_arg(nb.box), _hip.ptr(rows_d), nrows, kid, float(C_NORM[kid]), float(eta),
               float(tol), float(h_max), _hip.ptr(h_out), _hip.ptr(rho), _hip.ptr(omega),
               _hip.ptr(iters), _hip.ptr(flags))  # fmt: skip
     return h_out, rho, omega, iters, flags
@@ -112,8 +113,9 @@ def compute_density(pos, mass, h_guess, box=None, kernel_name=DEFAULT_KERNEL, et
 
 
 def compute_density_device(pos_d, mass_d, h_guess_d, box_arr, kernel_name, eta, tol, h_max, skin, rows=None,
-                           m_uniform=None, capacity_hint=0) -> DensityResult:  # fmt: skip
-    """compute_density on device tensors (h_guess_d is overwritten by the rebuild rounds)."""
+                           m_uniform=None, capacity_hint=0, blocked=False) -> DensityResult:  # fmt: skip
+    """compute_density on device tensors (h_guess_d is overwritten by the rebuild rounds).
+    ``blocked``: build the lists in 64-row blocks (the step's own layout) instead of CSR."""
     n = int(pos_d.shape[0])
     rows_d, nrows = _dev.rows_dev(rows, n)
     box_cap = np.inf if box_arr is None else 0.2499 * float(np.min(box_arr))
@@ -125,7 +127,10 @@ def compute_density_device(pos_d, mass_d, h_guess_d, box_arr, kernel_name, eta, 
     builds = 0
     grown = np.zeros(1, np.int64)
     for _ in range(12):
-        nb = build_lists(pos_d, h_search, box_arr, capacity_hint)
+        if blocked:
+            nb = build_lists_blocked(pos_d, h_search, box_arr, capacity_hint)
+        else:
+            nb = build_lists(pos_d, h_search, box_arr, capacity_hint)
         builds += 1
         h, rho, omega, it, fl = density_device(pos_d, mass_d, h_guess_d, nb, h_search, kernel_name, eta, tol,
                                                h_max, rows_d, nrows, m_uniform)  # fmt: skip
@@ -133,8 +138,45 @@ def compute_density_device(pos_d, mass_d, h_guess_d, box_arr, kernel_name, eta, 
                   _hip.ptr(h_search), _hip.ptr(h_guess_d), grown.ctypes.data_as(_hip._P64))  # fmt: skip
         if grown[0] == 0:
             return DensityResult(h, rho, omega, it, fl, nb, None, builds)
-        capacity_hint = max(capacity_hint, nb.n_entries)
+        capacity_hint = max(capacity_hint, nb.row_cap if blocked else nb.n_entries)
     raise RuntimeError("smoothing lengths kept outgrowing the neighbour search")
+
+
+def hydro_density_device(pos_d, mass_d, h_guess_d, kernel_name, eta, tol, h_max, skin, m_uniform,
+                         precision="fp64", row_cap_hint=0):  # fmt: skip
+    """The density pass of the coupled step (open box): compute_density's rebuild rounds
+    (sph.py:329-387) on the fused gather + h-rho solve kernel, then the tightened lists at the
+    final h, in 64-row blocks. h_guess_d is overwritten. Returns (DensityResult with the tight
+    lists as ``neighbours``, search-list entries)."""
+    from .neighbours import BlockedLists, _blocked_alloc
+
+    n = int(pos_d.shape[0])
+    kid = KERNEL_IDS[kernel_name]
+    h_search = _dev.empty(n)
+    _hip.call("nb_search_lengths", _hip.ptr(h_guess_d), float(h_max), float(skin), float(np.inf), n,
+              _hip.ptr(h_search))  # fmt: skip
+    h, rho, omega = (_dev.empty(n) for _ in range(3))
+    iters = _dev.empty(n, torch.int32)
+    flags = _dev.empty(n, torch.int32)
+    stats = np.zeros(4, np.int64)
+    grown = np.zeros(1, np.int64)
+    for builds in range(1, 13):
+        _hip.call("nb_hydro_density", _hip.ptr(pos_d), _hip.ptr(mass_d), float(m_uniform), _hip.ptr(h_guess_d),
+                  _hip.ptr(h_search), n, kid, float(C_NORM[kid]), float(eta), float(tol), float(h_max),
+                  PRECISIONS[precision], _hip.ptr(h), _hip.ptr(rho), _hip.ptr(omega), _hip.ptr(iters), _hip.ptr(flags),
+                  stats.ctypes.data_as(_hip._P64))  # fmt: skip
+        _hip.call("nb_grow_search", _hip.ptr(flags), _hip.ptr(h), None, n, float(np.inf), _hip.ptr(h_search),
+                  _hip.ptr(h_guess_d), grown.ctypes.data_as(_hip._P64))  # fmt: skip
+        if grown[0] == 0:
+            break
+    else:
+        raise RuntimeError("smoothing lengths kept outgrowing the neighbour search")
+    cap = max(int(stats[3]), 1)
+    ln, idx = _blocked_alloc(n, cap)
+    total = np.zeros(1, np.int64)
+    _hip.call("nb_hydro_tight", _hip.ptr(h), _hip.ptr(ln), _hip.ptr(idx), cap, total.ctypes.data_as(_hip._P64))
+    tight = BlockedLists(ln, idx, cap, int(total[0]), None)
+    return DensityResult(h, rho, omega, iters, flags, tight, None, builds), int(stats[2])
 
 
 # --------------------------------------------------------------------------- forces
@@ -197,7 +239,7 @@ def forces_device(pos_d, vel_d, mass_d, h_d, rho_d, omega_d, p_d, nb, kernel_nam
     _hip.call("nb_forces", _hip.ptr(pos_d), _hip.ptr(vel_d), _hip.ptr(mass_d), _hip.ptr(h_d), _hip.ptr(rho_d),
               _hip.ptr(omega_d), _hip.ptr(p_d), _hip.ptr(u_d if use_cond else None),
               _hip.ptr(cs_d if use_visc else None), _hip.ptr(alpha_d if use_visc else None),
-              _hip.ptr(gv_d if prm.use_recon else None), _hip.ptr(nb.d_offset), _hip.ptr(nb.d_index),
+              _hip.ptr(gv_d if prm.use_recon else None), ctypes.byref(nb.desc()),
               _hip.box_arg(nb.box), _hip.ptr(rows_d), nrows, prm, _hip.ptr(acc), _hip.ptr(dudt),
               _hip.ptr(vsig), _hip.ptr(vheat), _hip.ptr(cheat))  # fmt: skip
     return ForceResult(acc, dudt, vsig, vheat, cheat)
@@ -246,7 +288,7 @@ def gradients_device(pos_d, vel_d, acc_d, mass_d, h_d, nb, kernel_name, rows_d=N
     kid = KERNEL_IDS[kernel_name]
     div, curl, ddiv = (_dev.empty(nrows) for _ in range(3))
     _hip.call("nb_gradients", _hip.ptr(pos_d), _hip.ptr(vel_d), _hip.ptr(acc_d), _hip.ptr(mass_d), _hip.ptr(h_d),
-              _hip.ptr(nb.d_offset), _hip.ptr(nb.d_index), _hip.box_arg(nb.box), _hip.ptr(rows_d), nrows, kid,
+              ctypes.byref(nb.desc()), _hip.box_arg(nb.box), _hip.ptr(rows_d), nrows, kid,
               float(C_NORM[kid]), _hip.ptr(div), _hip.ptr(curl), _hip.ptr(ddiv))  # fmt: skip
     return Gradients(div, curl, ddiv)
 
@@ -284,7 +326,7 @@ def velocity_gradients_device(pos_d, vel_d, mass_d, h_d, nb, kernel_name, out_d,
     nrows = int(pos_d.shape[0]) if nrows is None else nrows
     kid = KERNEL_IDS[kernel_name]
     _hip.call("nb_velocity_gradients", _hip.ptr(pos_d), _hip.ptr(vel_d), _hip.ptr(mass_d), _hip.ptr(h_d),
-              _hip.ptr(nb.d_offset), _hip.ptr(nb.d_index), _hip.box_arg(nb.box), _hip.ptr(rows_d), nrows, kid,
+              ctypes.byref(nb.desc()), _hip.box_arg(nb.box), _hip.ptr(rows_d), nrows, kid,
               float(C_NORM[kid]), _hip.ptr(out_d))  # fmt: skip
     return out_d
 
@@ -311,7 +353,7 @@ def ns_gradients_device(pos_d, vel_d, mass_d, h_d, rho_d, om_d, nb, kernel_name,
     nrows = int(pos_d.shape[0]) if nrows is None else nrows
     kid = KERNEL_IDS[kernel_name]
     _hip.call("nb_ns_gradients", _hip.ptr(pos_d), _hip.ptr(vel_d), _hip.ptr(mass_d), _hip.ptr(h_d),
-              _hip.ptr(rho_d), _hip.ptr(om_d), _hip.ptr(nb.d_offset), _hip.ptr(nb.d_index), _hip.box_arg(nb.box),
+              _hip.ptr(rho_d), _hip.ptr(om_d), ctypes.byref(nb.desc()), _hip.box_arg(nb.box),
               _hip.ptr(rows_d), nrows, kid, float(C_NORM[kid]), _hip.ptr(out_d))  # fmt: skip
     return out_d
 
@@ -330,7 +372,7 @@ def ns_forces_device(pos_d, mass_d, h_d, rho_d, om_d, st_d, nb, kernel_name, row
     acc = _dev.empty((nrows, 3))
     kid = KERNEL_IDS[kernel_name]
     _hip.call("nb_ns_forces", _hip.ptr(pos_d), _hip.ptr(mass_d), _hip.ptr(h_d), _hip.ptr(rho_d), _hip.ptr(om_d),
-              _hip.ptr(st_d), _hip.ptr(nb.d_offset), _hip.ptr(nb.d_index), _hip.box_arg(nb.box), _hip.ptr(rows_d),
+              _hip.ptr(st_d), ctypes.byref(nb.desc()), _hip.box_arg(nb.box), _hip.ptr(rows_d),
               nrows, kid, float(C_NORM[kid]), _hip.ptr(acc))  # fmt: skip
     return acc
 
@@ -353,7 +395,7 @@ def disc_forces_device(pos_d, vel_d, mass_d, h_d, rho_d, om_d, cs_d, alpha_av, n
     dudt = _dev.empty(nrows)
     kid = KERNEL_IDS[kernel_name]
     _hip.call("nb_disc_forces", _hip.ptr(pos_d), _hip.ptr(vel_d), _hip.ptr(mass_d), _hip.ptr(h_d),
-              _hip.ptr(rho_d), _hip.ptr(om_d), _hip.ptr(cs_d), float(alpha_av), _hip.ptr(nb.d_offset),
-              _hip.ptr(nb.d_index), _hip.box_arg(nb.box), _hip.ptr(rows_d), nrows, kid, float(C_NORM[kid]),
+              _hip.ptr(rho_d), _hip.ptr(om_d), _hip.ptr(cs_d), float(alpha_av), ctypes.byref(nb.desc()),
+              _hip.box_arg(nb.box), _hip.ptr(rows_d), nrows, kid, float(C_NORM[kid]),
               _hip.ptr(acc), _hip.ptr(dudt))  # fmt: skip
     return acc, dudt

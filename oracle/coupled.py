@@ -3,7 +3,9 @@
 Restates coupled.py:131-270 (``gravity_flow``, ``_hydro``, ``initialise``, ``stable_dt``,
 ``_kick``, ``step``, ``ledger``) in NumPy on top of the C restatements. ``cfg`` is any object
 with the fields of the product's ``CoupledConfig`` / ``HydroConfig`` (read by attribute).
-Neighbour lists are rebuilt at every density pass (no cache), the policy the GPU path uses.
+Neighbour lists are rebuilt at every density pass (no cache), and before each density pass the
+particles are put in the order of the GPU implementation's neighbour grid (``grid_order``), the
+two policies the GPU path follows; ``State.perm`` maps back to the caller's order.
 """
 
 import math
@@ -13,7 +15,7 @@ import numpy as np
 
 from . import physics as ph
 from .integrator import glf_step_many, gn_v_from_p
-from .neighbours import tighten
+from .neighbours import grid_order, tighten
 from .sph import (
     compute_density,
     disc_forces_numba,
@@ -48,6 +50,19 @@ class State:
     iterations: np.ndarray = field(default=None)
     flags: np.ndarray = field(default=None)
     builds: int = 0
+    perm: np.ndarray = field(default=None)
+
+    def caller_order(self, name):
+        """Field ``name`` in the caller's particle order."""
+        a = getattr(self, name)
+        out = np.empty_like(a)
+        out[self.perm] = a
+        return out
+
+
+def _search_lengths(hc, h):
+    """sph.py:371 (open box)."""
+    return np.minimum(np.minimum(h, hc.h_max) * hc.skin, np.inf)
 
 
 def _gn(cfg):
@@ -163,10 +178,13 @@ def initialise(x, v, u, mass, h_guess, cfg, t=0.0) -> State:
     v = np.array(v, float)
     p = p_from_v(cfg, x, v)
     hc = cfg.hydro
-    st = State(t, x, p, np.array(u, float) * np.ones(n), np.array(mass, float) * np.ones(n),
-               np.array(h_guess, float) * np.ones(n),
-               np.full(n, hc.alpha_init if hc.switch else hc.alpha_const))  # fmt: skip
-    return _hydro(st, cfg, velocity(cfg, x, p), 0.0, None)
+    u = np.array(u, float) * np.ones(n)
+    mass = np.array(mass, float) * np.ones(n)
+    h = np.array(h_guess, float) * np.ones(n)
+    o = grid_order(x, _search_lengths(hc, h))
+    st = State(t, x[o], p[o], u[o], mass[o], h[o], np.full(n, hc.alpha_init if hc.switch else hc.alpha_const),
+               perm=o.astype(np.int64))  # fmt: skip
+    return _hydro(st, cfg, velocity(cfg, st.x, st.p), 0.0, None)
 
 
 def stable_dt(st: State, cfg) -> float:
@@ -204,8 +222,12 @@ def step(st: State, cfg, dt: float) -> State:
         e_rad += float(np.sum(st.mass * (u_half - u_cool)))
         u_half = u_cool
     p_pred = p_flow + 0.5 * dt * st.a_hydro
-    pred = replace(st, t=st.t + dt, x=x1, p=p_flow, u=u_half + 0.5 * dt * st.dudt)
-    new = _hydro(pred, cfg, velocity(cfg, x1, p_pred), dt, st.a_hydro)
+    u_pred = u_half + 0.5 * dt * st.dudt
+    o = grid_order(x1, _search_lengths(hc, st.h))
+    x1, p_flow, p_pred, u_half, u_pred, a_prev = (a[o] for a in (x1, p_flow, p_pred, u_half, u_pred, st.a_hydro))
+    pred = replace(st, t=st.t + dt, x=x1, p=p_flow, u=u_pred, mass=st.mass[o], h=st.h[o], alpha=st.alpha[o],
+                   perm=st.perm[o])  # fmt: skip
+    new = _hydro(pred, cfg, velocity(cfg, x1, p_pred), dt, a_prev)
     return replace(new, p=p_flow + 0.5 * dt * new.a_hydro, u=u_half + 0.5 * dt * new.dudt,
                    e_radiated=e_rad)  # fmt: skip
 

@@ -134,3 +134,34 @@ def tighten(nb: Neighbours, pos, h, rows=None) -> Neighbours:
     lib.tighten(lib.d(pos), lib.d(h), lib.d(box), lib.i(nb.offset), lib.i(nb.index), lib.i(rows),
                 len(rows), lib.i(off), lib.i(idx), lib.i(counts), 1)  # fmt: skip
     return Neighbours(off, idx[: off[-1]].copy(), nb.box)
+
+
+def grid_order(x, h_search):
+    """Particle order of the GPU implementation's neighbour grid (its layout policy; the
+    reference keeps the caller's order): cubic cells of the fixed-point mean search length (each
+    h quantised to h / h_max * 2^32, summed exactly as integers), at least 1/2048 of the widest
+    span, at most 2^27 cells (the cell grows by 1.25 until it fits); particles sorted by raster
+    cell id, ascending index inside a cell. Open boxes only."""
+    x = lib.f64(x)
+    h = lib.f64(h_search)
+    n = len(x)
+    lo = x.min(axis=0)
+    span = x.max(axis=0) - lo
+    hmax = float(h.max())
+    scale = 4294967296.0 / hmax if hmax > 0.0 else 0.0
+    hs = int(np.sum((h * scale).astype(np.uint64), dtype=np.uint64))
+    hmean = (float(hs) / scale) / float(n) if scale > 0.0 else 0.0
+    cell = max(max(hmean, float(np.max(span)) / 2048.0), 1e-300)
+    while True:
+        nc = [int(max(np.floor(span[c] / cell) + 1.0, 1.0)) for c in range(3)]
+        total = 1.0
+        for c in range(3):
+            total *= float(nc[c])
+        if total <= 134217728.0:
+            break
+        cell *= 1.25
+    inv = 1.0 / cell
+    f = np.floor((x - lo) * inv)
+    c = np.clip(f, 0, np.array(nc) - 1).astype(np.int64)
+    key = (c[:, 0] * nc[1] + c[:, 1]) * nc[2] + c[:, 2]
+    return np.argsort(key, kind="stable")

@@ -34,8 +34,9 @@ def _compare(st, ost, cfg, fields=("x", "p", "u", "h", "rho", "alpha", "a_hydro"
     from oracle import coupled as oc
 
     got = st.to_numpy()
+    bit_equal("state.perm (particle order of the grid)", ost.perm, st.perm.cpu().numpy())
     for k in fields:
-        bit_equal(f"state.{k}", getattr(ost, k), got[k])
+        bit_equal(f"state.{k}", ost.caller_order(k), got[k])
     lg, lo = coupled.ledger(st, cfg), oc.ledger(ost, cfg)
     scale = lo["abs_h"]
     check("ledger energy (1e-13 of sum m|H|)", lo["energy"] / scale, lg["energy"] / scale, atol=1e-13)
@@ -89,7 +90,43 @@ def test_switch_conductivity_reconstruction():
         st = c2.step(st, cfg, dt)
         ost = oc.step(ost, cfg, dt)
     got = st.to_numpy()
-    check("switch+cond+recon: x", ost.x, got["x"], rtol=1e-12, per_vector=True)
-    check("switch+cond+recon: a_hydro", ost.a_hydro, got["a_hydro"], rtol=1e-9, per_vector=True)
-    check("switch+cond+recon: rho", ost.rho, got["rho"], rtol=1e-12)
-    check("switch+cond+recon: alpha", ost.alpha, got["alpha"], rtol=1e-9, atol=1e-15)
+    check("switch+cond+recon: x", ost.caller_order("x"), got["x"], rtol=1e-12, per_vector=True)
+    check("switch+cond+recon: a_hydro", ost.caller_order("a_hydro"), got["a_hydro"], rtol=1e-9, per_vector=True)
+    check("switch+cond+recon: rho", ost.caller_order("rho"), got["rho"], rtol=1e-12)
+    check("switch+cond+recon: alpha", ost.caller_order("alpha"), got["alpha"], rtol=1e-9, atol=1e-15)
+
+
+def test_blocked_lists_grid_order_and_tighten(rng):
+    """The step's own list layout (64-row blocks) holds the reference's lists; the grid order
+    equals the oracle's restatement of it; in grid order no row needs sorting."""
+    import torch
+
+    from nebula import _dev, _hip
+    from nebula.ics import stream_crash
+    from nebula.sim import neighbours as nbm
+
+    from oracle import neighbours as on
+
+    ic = stream_crash(20000, length=400.0, seed=3)
+    hs = 1.4 * ic.h * rng.uniform(0.9, 1.1, len(ic.h))
+    x_d, h_d = _dev.f64(ic.x), _dev.f64(hs)
+    order = nbm.grid_order(x_d, h_d)
+    bit_equal("grid order = oracle restatement", on.grid_order(ic.x, hs), order.cpu().numpy())
+    xo, ho = ic.x[order.cpu().numpy()], hs[order.cpu().numpy()]
+    ref = on.find_neighbours_grid(xo, ho)
+    blk = nbm.build_lists_blocked(_dev.f64(xo), _dev.f64(ho), None)
+    stats = np.zeros(4, np.int64)
+    _hip.check(_hip.lib().nb_lists_stats(_hip.ctx(), stats.ctypes.data_as(_hip._P64)))
+    check("rows needing a sort in grid order", 0, int(stats[2]))
+    csr = blk.to_csr()
+    bit_equal("blocked lists: offsets", ref.offset, csr.offset)
+    bit_equal("blocked lists: indices", ref.index, csr.index)
+    t_ref = on.tighten(ref, xo, 0.8 * ho)
+    t_blk = nbm.tighten_blocked(blk, _dev.f64(xo), _dev.f64(0.8 * ho)).to_csr()
+    bit_equal("blocked tighten: offsets", t_ref.offset, t_blk.offset)
+    bit_equal("blocked tighten: indices", t_ref.index, t_blk.index)
+    # caller order: rows need sorting, still identical lists
+    ref2 = on.find_neighbours_grid(ic.x, hs)
+    blk2 = nbm.build_lists_blocked(x_d, h_d, None).to_csr()
+    bit_equal("blocked lists (caller order): indices", ref2.index, blk2.index)
+    del torch

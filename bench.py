@@ -44,9 +44,12 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1_048_576)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
+    ap.add_argument("--no-compare", action="store_true", help="skip timing the other precision alongside")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy 5 times (scales the PMC byte counters)")
     return ap.parse_args()
 
 
@@ -82,44 +85,64 @@ def cpu_baseline(ic, cfg, steps):
                       f"kernels, 1 warm-up + {steps} timed coupled steps"}  # fmt: skip
 
 
+def run_gpu(ic, cfg, steps, warmup):
+    """Times ``steps`` coupled steps after ``warmup``; returns (wall s, per-entry-point timing, state)."""
+    import torch
+
+    from nebula import _hip
+    from nebula.sim import coupled
+
+    st = coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg)
+    for _ in range(warmup):
+        st = coupled.step(st, cfg, coupled.stable_dt(st, cfg))
+    torch.cuda.synchronize()
+    _hip.start_timing()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = coupled.step(st, cfg, coupled.stable_dt(st, cfg))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, _hip.stop_timing(), st
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
                                    "described in DESIGN.md, not implemented)"}))  # fmt: skip
         sys.exit(2)
-    import numpy as np
-    import torch
-
     from nebula import _hip
-    from nebula.sim import coupled
 
     dev = _hip.torch_device()
+    if args.copy_calibration:
+        import torch
+
+        src = torch.ones(1 << 25, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": src.numel() * 8, "launches": 5}))
+        return
     ic = workload(args.n)
     cfg = config(args.precision)
-    print(f"bench: {args.n} particles, {args.precision}, warmup {args.warmup}, steps {args.steps}", file=sys.stderr,
-          flush=True)  # fmt: skip
-    st = coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg)
-    for _ in range(args.warmup):
-        st = coupled.step(st, cfg, coupled.stable_dt(st, cfg))
-    torch.cuda.synchronize()
-    _hip.start_timing()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        st = coupled.step(st, cfg, coupled.stable_dt(st, cfg))
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kern = _hip.stop_timing()  # name -> (launches, total ms)
     n = args.n
+    print(f"bench: {n} particles, {args.precision}, warmup {args.warmup}, steps {args.steps}", file=sys.stderr,
+          flush=True)  # fmt: skip
+    wall, kern, st = run_gpu(ic, cfg, args.steps, args.warmup)  # name -> (launches, total ms)
     value = n * args.steps / wall
+    print(f"bench: {value:.4g} particle-steps/s", file=sys.stderr, flush=True)
     # algorithmic work of the pair kernels, from this run's own lists and counters
     n_search = st.n_search_entries / n
     n_pairs = (st.n_tight_entries - n) / n
     k_it = float(st.iterations.double().mean().item())
+    dens = (FLOPS_SEARCH_ENTRY * n_search + FLOPS_NEWTON_ENTRY * (k_it + 1) * n_search) * n
     flops = {
         "nb_forces": FLOPS_FORCE_PAIR * n_pairs * n,
         "nb_gradients": FLOPS_GRAD_PAIR * n_pairs * n,
-        "nb_density": (FLOPS_SEARCH_ENTRY * n_search + FLOPS_NEWTON_ENTRY * (k_it + 1) * n_search) * n,
+        "nb_gradients_mixed": FLOPS_GRAD_PAIR * n_pairs * n,
+        "nb_hydro_density": dens,
+        "nb_hydro_density_mixed": dens,
     }
     pair = {k: v for k, v in kern.items() if k in flops}
     dom = max(pair, key=lambda k: pair[k][1] / pair[k][0])
@@ -127,6 +150,7 @@ def main():
     avg_s = total_ms / launches / 1e3
     achieved = flops[dom] / avg_s / 1e12
     peak = PEAK["fp64"] if args.precision == "fp64" else PEAK["fp32"]
+    traffic = _traffic(dom, args.precision)
     out = {
         "metric": "SPH particle-steps/s at 1M particles (plain FP64 vs mixed+learned-h; % FP32 peak)",
         "value": value,
@@ -143,16 +167,38 @@ def main():
         "config": {"workload": "stream crash, 2 x 524,288-particle streams (1,048,576)" if n == 1_048_576 else
                    f"stream crash, {n} particles", "precision": args.precision, "kernel": "cubic, eta 1.2",
                    "viscosity": "Monaghan alpha 1 beta 2 + Balsara", "cooling": "power law rho^2 T^0.5"},
-        "roofline": {"bound": "valu", "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+        "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": traffic,
+                     "note": "vector-ALU pair arithmetic, priced against MI355X's dense FP64 (78.6) or FP32 "
+                             "(157.3) TFLOP/s, which the vector and matrix units share",
                      "work": f"{flops[dom] / n:.0f} FLOP/particle-step (n_s {n_search:.1f}, pairs {n_pairs:.1f}, "
                              f"Newton iterations {k_it:.2f})"},
         "kernels_ms_per_step": {k: v[1] / args.steps for k, v in sorted(kern.items(), key=lambda kv: -kv[1][1])},
     }
+    if not args.no_compare:
+        other = "fp64" if args.precision == "mixed" else "mixed"
+        print(f"bench: {other} alongside", file=sys.stderr, flush=True)
+        del st
+        w2, k2, _ = run_gpu(ic, config(other), args.steps, args.warmup)
+        out["compare"] = {other: {"value": n * args.steps / w2, "ms_per_step": w2 / args.steps * 1e3,
+                                  "speedup_of_value": (n * args.steps / wall) / (n * args.steps / w2),
+                                  "kernels_ms_per_step": {k: v[1] / args.steps for k, v in
+                                                          sorted(k2.items(), key=lambda kv: -kv[1][1])}}}  # fmt: skip
     if not args.no_cpu:
         print("bench: CPU baseline (FP64 oracle, OpenMP)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(ic, cfg, args.cpu_steps)
     print(json.dumps(out))
+
+
+def _traffic(kernel, precision):
+    """HBM bytes per launch of ``kernel`` from the committed PMC summary (FETCH_SIZE + WRITE_SIZE,
+    scaled by the copy-kernel calibration), or None when it has not been measured."""
+    f = REPO / "profiles" / "traffic.json"
+    if not f.exists():
+        return None
+    data = json.loads(f.read_text())
+    rec = data.get(precision, {}).get(kernel)
+    return None if rec is None else rec.get("bytes_per_launch")
 
 
 if __name__ == "__main__":

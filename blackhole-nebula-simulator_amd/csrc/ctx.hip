@@ -45,7 +45,7 @@ extern "C" void nb_ctx_destroy(nb_ctx* ctx)
     if (!ctx) return;
     void* bufs[] = {ctx->cell_id, ctx->keys_out, ctx->iota, ctx->order, ctx->cell_count, ctx->cell_start,
                     ctx->gbuf, ctx->rowcnt, ctx->unsorted, ctx->pending, ctx->edbuf, ctx->ecount, ctx->eoff, ctx->efill, ctx->ebuf, ctx->tot,
-                    ctx->offs, ctx->tcount, ctx->dscratch, ctx->fscratch, ctx->red, ctx->scal, ctx->cub_tmp};
+                    ctx->offs, ctx->tcount, ctx->dscratch, ctx->fscratch, ctx->pack, ctx->red, ctx->scal, ctx->cub_tmp};
     for (void* b : bufs)
         if (b) hipFree(b);
     delete ctx;
@@ -53,7 +53,7 @@ extern "C" void nb_ctx_destroy(nb_ctx* ctx)
 
 extern "C" const char* nb_last_error(const nb_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
 
-extern "C" int32_t nb_abi_version(void) { return 1; }
+extern "C" int32_t nb_abi_version(void) { return 2; }
 
 extern "C" int64_t nb_ctx_scratch_bytes(const nb_ctx* ctx)
 {
@@ -61,7 +61,7 @@ extern "C" int64_t nb_ctx_scratch_bytes(const nb_ctx* ctx)
     size_t s = ctx->cell_id_bytes + ctx->keys_out_bytes + ctx->iota_bytes + ctx->order_bytes +
                ctx->cell_count_bytes + ctx->cell_start_bytes + ctx->gbuf_bytes + ctx->rowcnt_bytes + ctx->unsorted_bytes + ctx->pending_bytes + ctx->edbuf_bytes +
                ctx->ecount_bytes + ctx->eoff_bytes + ctx->efill_bytes + ctx->ebuf_bytes + ctx->tot_bytes +
-               ctx->offs_bytes + ctx->tcount_bytes + ctx->dscratch_bytes + ctx->fscratch_bytes +
+               ctx->offs_bytes + ctx->tcount_bytes + ctx->dscratch_bytes + ctx->fscratch_bytes + ctx->pack_bytes +
                ctx->red_bytes + ctx->scal_bytes + ctx->cub_tmp_bytes;
     return (int64_t)s;
 }

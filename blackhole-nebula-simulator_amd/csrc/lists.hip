@@ -206,8 +206,10 @@ k_gather(const double* __restrict__ pos, const double* __restrict__ h, int64_t n
          int64_t cap, int32_t* __restrict__ rowcnt, uint8_t* __restrict__ unsorted,
          unsigned long long* __restrict__ ecount, int32_t* __restrict__ scal)
 {
+    // lanes past the end redo the last particle without writing (full waves for the reductions)
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const bool live = i < n;
+    if (!live) i = n - 1;
     const double r = NB_R_KERN * h[i];
     const double r2 = r * r;
     const double xi = wrapped(pos, i, 0, box), yi = wrapped(pos, i, 1, box), zi = wrapped(pos, i, 2, box);
@@ -238,21 +240,27 @@ k_gather(const double* __restrict__ pos, const double* __restrict__ h, int64_t n
                         uint32_t e = j;
                         if (!(d2 < rj * rj)) {
                             e |= kFlag;
-                            atomicAdd(&ecount[j], 1ull);
+                            if (live) atomicAdd(&ecount[j], 1ull);
                         }
                         if ((int64_t)j < last) sorted = false;
                         last = j;
-                        if (m < cap) row[(int64_t)m * 64] = e;
+                        if (live && m < cap) row[(int64_t)m * 64] = e;
                         m++;
                     }
                 }
             }
         }
     }
-    rowcnt[i] = m;
-    unsorted[i] = sorted ? 0 : 1;
-    if (!sorted) atomicAdd(&scal[1], 1);
-    atomicMax(&scal[0], m);
+    if (live) {
+        rowcnt[i] = m;
+        unsorted[i] = sorted ? 0 : 1;
+    }
+    const int wmax = wave_max_i32(m);
+    const unsigned long long wuns = wave_sum_u64(live && !sorted ? 1ull : 0ull);
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&scal[0], wmax);
+        if (wuns) atomicAdd(&scal[1], (int)wuns);
+    }
 }
 
 // Insertion sort of the rows marked unsorted (flag bit ignored in the key).
@@ -299,12 +307,15 @@ __global__ void k_row_total(const int32_t* __restrict__ rowcnt, const unsigned l
                             unsigned long long* __restrict__ tot, int32_t* __restrict__ scal, int64_t n)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int32_t t32 = 0;
     if (i < n) {
         unsigned long long t = (unsigned long long)rowcnt[i] + ecount[i];
         tot[i] = t;
-        atomicMax(&scal[2], (int32_t)t);
+        t32 = (int32_t)t;
     }
     if (i == n) tot[i] = 0;
+    const int wm = wave_max_i32(t32);
+    if ((threadIdx.x & 63) == 0 && wm > 0) atomicMax(&scal[2], wm);
 }
 
 // Sorted extras of row i (sorted here, in place), merged with the sorted gather row. The sets are
@@ -602,8 +613,10 @@ k_gather_solve(const double* __restrict__ pos, const double* __restrict__ h, int
                double* __restrict__ dbuf, double* __restrict__ mbuf, int64_t cap, int32_t* __restrict__ rowcnt,
                uint8_t* __restrict__ unsorted, unsigned long long* __restrict__ ecount, DensArgs A)
 {
+    // lanes past the end redo the last particle without writing (full waves for the reductions)
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const bool live = i < n;
+    if (!live) i = n - 1;
     const double hs = h[i];
     const double r = NB_R_KERN * hs;
     const double r2 = r * r;
@@ -634,11 +647,11 @@ k_gather_solve(const double* __restrict__ pos, const double* __restrict__ h, int
                     uint32_t e = j;
                     if (!(d2 < rj * rj)) {
                         e |= kFlag;
-                        atomicAdd(&ecount[j], 1ull);
+                        if (live) atomicAdd(&ecount[j], 1ull);
                     }
                     if ((int64_t)j < last) sorted = false;
                     last = j;
-                    if (m < cap) {
+                    if (live && m < cap) {
                         gbuf[base + (int64_t)m * 64] = e;
                         dbuf[base + (int64_t)m * 64] = d2;
                         if (mbuf) mbuf[base + (int64_t)m * 64] = A.mass[j];
@@ -648,17 +661,20 @@ k_gather_solve(const double* __restrict__ pos, const double* __restrict__ h, int
             }
         }
     }
+    const bool done_here = sorted && m <= cap;
+    const int wmax = wave_max_i32(m);
+    const unsigned long long wuns = wave_sum_u64(live && !sorted ? 1ull : 0ull);
+    const unsigned long long wpend = wave_sum_u64(live && !done_here ? 1ull : 0ull);
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&A.scal[0], wmax);
+        if (wuns) atomicAdd(&A.scal[1], (int)wuns);
+        if (wpend) atomicAdd(&A.scal[3], (int)wpend);
+    }
+    if (!live) return;
     rowcnt[i] = m;
     unsorted[i] = sorted ? 0 : 1;
-    if (!sorted) atomicAdd(&A.scal[1], 1);
-    atomicMax(&A.scal[0], m);
-    if (sorted && m <= cap) {
-        A.pending[i] = 0;
-        solve_row<kMixed>(i, m, dbuf, mbuf, cap, hs, A);
-    } else {
-        A.pending[i] = 1;
-        atomicAdd(&A.scal[3], 1);
-    }
+    A.pending[i] = done_here ? 0 : 1;
+    if (done_here) solve_row<kMixed>(i, m, dbuf, mbuf, cap, hs, A);
 }
 
 // Insertion sort of unsorted rows, carrying d2 and mass along.
@@ -781,6 +797,355 @@ k_tight_emit(const uint32_t* __restrict__ gbuf, const double* __restrict__ dbuf,
     if (threadIdx.x == 0) atomicAdd(total, sh[0]);
 }
 
+// ----------------------------------------------------------------- mixed-precision step passes
+// Lane groups: kG consecutive lanes serve one particle. The candidates of a cell column are a
+// contiguous range of the cell-sorted particles, so the group reads kG consecutive candidates at
+// a time (coalesced), tests the FP64 list predicate (the lists stay exact), and compacts the
+// accepted ones with a ballot: lane l's slot is the number of accepting lanes below it. Rows are
+// stored row-major (entry k of row i at i * cap + k), so a group's writes are contiguous too.
+// Each lane keeps the r and mass of the entries it accepted in registers (up to kStash; a
+// row with more is solved from memory), so the h-rho Newton iterations re-read nothing: each
+// lane sums its own entries in FP32 with compensated accumulation, the group combines the
+// partials in FP64, and every lane of the group takes the same Newton step (the update and the
+// convergence test in FP64).
+static constexpr int kG = 16;
+static constexpr int kStash = 24;
+
+__device__ __forceinline__ void kernel_fdf32(float q, int kid, float& fq, float& dfq)
+{
+    if (kid == 0) {
+        if (q < 1.f) {
+            fq = 1.f - 1.5f * q * q + 0.75f * q * q * q;
+            dfq = -3.f * q + 2.25f * q * q;
+        } else {
+            const float t = 2.f - q;
+            fq = 0.25f * t * t * t;
+            dfq = -0.75f * t * t;
+        }
+    } else {
+        const float x = 1.f - 0.5f * q, x2 = x * x;
+        fq = x2 * x2 * x2 * (35.f / 12.f * q * q + 3.f * q + 1.f);
+        dfq = -7.f / 3.f * q * x2 * x2 * x * (5.f * q + 2.f);
+    }
+}
+
+__device__ __forceinline__ double group_sum(double v)
+{
+    for (int o = kG / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kG);
+    return v;
+}
+
+struct MixArgs {
+    const double* mass;
+    double m_uniform;
+    const double* h_start;
+    double h_max;
+    int kid;
+    double cnorm, eta, tol;
+    double* h_out;
+    double* rho;
+    double* omega;
+    int32_t* iters;
+    int32_t* flags;
+    int32_t* scal;  // [0] longest row, [1] rows solved from memory
+};
+
+// sums of m f(q) and m (3 f + q f') over the row, at inverse length inv_h: from the register
+// stash, or (overflow) from the row in memory read by the whole group.
+__device__ __forceinline__ void mix_sums(const float* rs_r, const float* rs_m, int nst, bool from_mem,
+                                         const float* rd2, const float* rmass, int32_t m, int glane, float rmax2f,
+                                         float inv_hf, int kid, float mu, double& s1, double& s2)
+{
+    float a1 = 0.f, c1 = 0.f, a2 = 0.f, c2 = 0.f;
+    auto add = [&](float r, float mk) {
+        const float q = r * inv_hf;
+        if (q < 2.f) {
+            float fq, dfq;
+            kernel_fdf32(q, kid, fq, dfq);
+            const float t1 = mk * fq - c1;
+            const float u1 = a1 + t1;
+            c1 = (u1 - a1) - t1;
+            a1 = u1;
+            const float t2 = mk * (3.f * fq + q * dfq) - c2;
+            const float u2 = a2 + t2;
+            c2 = (u2 - a2) - t2;
+            a2 = u2;
+        }
+    };
+    if (!from_mem) {
+#pragma unroll
+        for (int k = 0; k < kStash; k++)
+            if (k < nst) add(rs_r[k], rs_m ? rs_m[k] : mu);
+    } else {
+        for (int32_t k = glane; k < m; k += kG) {
+            const float d2 = rd2[k];
+            if (d2 < rmax2f) add(__fsqrt_rn(d2), rmass ? rmass[k] : mu);
+        }
+    }
+    s1 = group_sum((double)a1 - (double)c1);
+    s2 = group_sum((double)a2 - (double)c2);
+}
+
+template <bool kMasses>
+__global__ void __launch_bounds__(256)
+k_gather_solve_mixed(const double* __restrict__ pos, const double* __restrict__ h, int64_t n, Grid g,
+                     const uint32_t* __restrict__ start, const uint32_t* __restrict__ order,
+                     uint32_t* __restrict__ ridx, float* __restrict__ rd2, float* __restrict__ rmass, int64_t cap,
+                     int32_t* __restrict__ rowcnt, unsigned long long* __restrict__ ecount, MixArgs A)
+{
+    const int glane = threadIdx.x & (kG - 1);
+    const int gshift = (threadIdx.x & 63) & ~(kG - 1);
+    const int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kG;
+    const bool live = i < n;
+    const int64_t ii = live ? i : n - 1;
+    const double hs = h[ii];
+    const double r = NB_R_KERN * hs;
+    const double r2 = r * r;
+    const double xi = pos[3 * ii], yi = pos[3 * ii + 1], zi = pos[3 * ii + 2];
+    int64_t ax, nx, ay, ny, az, nz;
+    cell_range(xi, r, g, 0, false, ax, nx);
+    cell_range(yi, r, g, 1, false, ay, ny);
+    cell_range(zi, r, g, 2, false, az, nz);
+    if (!live) nx = 0;
+    const double r_lim = pymin(hs, A.h_max);
+    const float rmax2f = (float)((2.0 * r_lim) * (2.0 * r_lim));
+    float st_r[kStash], st_m[kStash];
+    int nst = 0;
+    bool over = false;
+    int32_t m = 0;
+    uint32_t* row = ridx + ii * cap;
+    float* drow = rd2 + ii * cap;
+    float* mrow = kMasses ? rmass + ii * cap : nullptr;
+    // all groups of the wave run the same number of ballots: loop bounds are made uniform by
+    // letting finished groups idle with pred = false
+    for (int64_t di = 0;; di++) {
+        bool more = di < nx;
+        if (__ballot(more) == 0) break;
+        for (int64_t dj = 0;; dj++) {
+            bool more2 = more && dj < ny;
+            if (__ballot(more2) == 0) break;
+            uint32_t k0 = 0, k1 = 0;
+            if (more2) {
+                const uint32_t c0 = ((uint32_t)(ax + di) * (uint32_t)g.nc[1] + (uint32_t)(ay + dj)) * (uint32_t)g.nc[2] +
+                                    (uint32_t)az;
+                k0 = start[c0];
+                k1 = start[c0 + (uint32_t)nz];
+            }
+            for (uint32_t kb = k0;; kb += kG) {
+                const bool in_range = more2 && kb < k1;
+                if (__ballot(in_range) == 0) break;
+                const uint32_t k = kb + (uint32_t)glane;
+                bool pred = false;
+                uint32_t j = 0;
+                double d2 = 0.0;
+                if (in_range && k < k1) {
+                    j = order[k];
+                    const double dx = pos[3 * j] - xi;
+                    const double dy = pos[3 * j + 1] - yi;
+                    const double dz = pos[3 * j + 2] - zi;
+                    d2 = dx * dx + dy * dy + dz * dz;
+                    pred = d2 < r2;
+                }
+                const unsigned long long bal = __ballot(pred);
+                const unsigned gb = (unsigned)((bal >> gshift) & ((1ull << kG) - 1));
+                if (pred) {
+                    const int slot = m + __popc(gb & ((1u << glane) - 1));
+                    const double rj = NB_R_KERN * h[j];
+                    uint32_t e = j;
+                    if (!(d2 < rj * rj)) {
+                        e |= kFlag;
+                        atomicAdd(&ecount[j], 1ull);
+                    }
+                    const float mj = kMasses ? (float)A.mass[j] : 0.f;
+                    if (slot < cap) {
+                        row[slot] = e;
+                        drow[slot] = (float)d2;
+                        if (kMasses) mrow[slot] = mj;
+                    }
+                    if ((float)d2 < rmax2f) {
+                        if (nst < kStash) {
+                            st_r[nst] = __fsqrt_rn((float)d2);
+                            st_m[nst] = mj;
+                            nst++;
+                        } else {
+                            over = true;
+                        }
+                    }
+                }
+                m += __popc(gb);
+            }
+        }
+    }
+    // a group overflows together: any lane past the stash, or the row past its capacity
+    const unsigned long long ob = __ballot(over);
+    const bool from_mem = ((ob >> gshift) & ((1ull << kG) - 1)) != 0;
+    if (live && glane == 0) rowcnt[i] = m;
+    const int wmax = wave_max_i32(live ? m : 0);
+    const unsigned long long wmem = wave_sum_u64(live && glane == 0 && from_mem ? 1ull : 0ull);
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&A.scal[0], wmax);
+        if (wmem) atomicAdd(&A.scal[1], (int)wmem);
+    }
+    if (m > cap) return;  // rebuilt with a larger capacity by the caller
+    // density / h solve (sph.py:251-297; sums FP32, Newton FP64)
+    const double m_i = A.mass[ii];
+    const float mu = (float)(A.m_uniform > 0.0 ? A.m_uniform : 1.0);
+    const double mscale = kMasses ? 1.0 : (A.m_uniform > 0.0 ? A.m_uniform / (double)mu : 1.0);
+    double hi = r_lim, lo = 0.0;
+    double hh = pymin(A.h_start[ii], hi);
+    const double h0 = hh;
+    int flag = 3, it = 0;
+    const double eta3 = ipow3(A.eta);
+    double s1, s2;
+    while (it < 50) {
+        it++;
+        mix_sums(st_r, kMasses ? st_m : nullptr, nst, from_mem, drow, mrow, m, glane, rmax2f, (float)(1.0 / hh),
+                 A.kid, mu, s1, s2);
+        const double inv_h = 1.0 / hh;
+        const double rs = s1 * mscale * A.cnorm * ipow3(inv_h);
+        const double dws = -s2 * mscale * A.cnorm * ipow4(inv_h);
+        const double f = rs - m_i * ipow3(A.eta / hh);
+        const double df = dws + 3.0 * m_i * eta3 / ipow4(hh);
+        if (f > 0) hi = hh;
+        else lo = hh;
+        double h_new = df > 0 ? hh - f / df : 0.5 * (lo + hi);
+        if (!(lo <= h_new && h_new <= hi)) h_new = 0.5 * (lo + hi);
+        if (fabs(h_new - hh) < A.tol * h0) {
+            hh = h_new;
+            flag = 0;
+            break;
+        }
+        hh = h_new;
+    }
+    if (hh >= r_lim * (1 - 1e-3)) {
+        mix_sums(st_r, kMasses ? st_m : nullptr, nst, from_mem, drow, mrow, m, glane, rmax2f, (float)(1.0 / r_lim),
+                 A.kid, mu, s1, s2);
+        const double rs_lim = s1 * mscale * A.cnorm * ipow3(1.0 / r_lim);
+        if (rs_lim - m_i * ipow3(A.eta / r_lim) < 0) {
+            hh = r_lim;
+            flag = r_lim >= A.h_max ? 1 : 2;
+        }
+    }
+    mix_sums(st_r, kMasses ? st_m : nullptr, nst, from_mem, drow, mrow, m, glane, rmax2f, (float)(1.0 / hh), A.kid,
+             mu, s1, s2);
+    const double inv_h = 1.0 / hh;
+    const double rs = s1 * mscale * A.cnorm * ipow3(inv_h);
+    const double dws = -s2 * mscale * A.cnorm * ipow4(inv_h);
+    if (live && glane == 0) {
+        A.h_out[i] = hh;
+        A.rho[i] = rs;
+        A.omega[i] = flag == 1 ? 1.0 : 1.0 + hh / (3.0 * rs) * dws;
+        A.iters[i] = it;
+        A.flags[i] = flag;
+    }
+}
+
+// extras of the row-major gather rows (a group reads its row, flagged entries are transposed)
+__global__ void __launch_bounds__(256)
+k_extras_fill_rm(const uint32_t* __restrict__ ridx, const float* __restrict__ rd2, int64_t cap,
+                 const int32_t* __restrict__ rowcnt, const unsigned long long* __restrict__ eoff,
+                 unsigned int* __restrict__ efill, uint32_t* __restrict__ ebuf, float* __restrict__ ed2, int64_t n)
+{
+    const int glane = threadIdx.x & (kG - 1);
+    const int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kG;
+    if (i >= n) return;
+    const int32_t m = rowcnt[i];
+    for (int32_t k = glane; k < m; k += kG) {
+        const uint32_t v = ridx[i * cap + k];
+        if (v & kFlag) {
+            const uint32_t j = v & kMask;
+            const unsigned int s = atomicAdd(&efill[j], 1u);
+            ebuf[eoff[j] + s] = (uint32_t)i;
+            ed2[eoff[j] + s] = rd2[i * cap + k];
+        }
+    }
+}
+
+// extras arrive in atomic order: sort each row's segment by index (segments are short) so that
+// the tightened rows, and every sum over them, are the same from run to run
+__global__ void k_sort_extras(uint32_t* __restrict__ ebuf, float* __restrict__ ed2,
+                              const unsigned long long* __restrict__ eoff, int64_t n)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t* e = ebuf + eoff[i];
+    float* d = ed2 + eoff[i];
+    const int64_t ne = (int64_t)(eoff[i + 1] - eoff[i]);
+    for (int64_t a = 1; a < ne; a++) {
+        const uint32_t v = e[a];
+        const float dv = d[a];
+        int64_t b = a - 1;
+        while (b >= 0 && e[b] > v) {
+            e[b + 1] = e[b];
+            d[b + 1] = d[b];
+            b--;
+        }
+        e[b + 1] = v;
+        d[b + 1] = dv;
+    }
+}
+
+// tightened rows (row-major, capacity cap_out): gather row entries, then the row's extras,
+// kept when d2 < (2 max(h_i, h_j))^2 at the final h. The stored FP32 d2 decides unless it lies
+// within its rounding error of the radius; then d2 is formed again in FP64 as the search
+// formed it, so the rows are exactly tighten() of the search lists at h_final.
+__global__ void __launch_bounds__(256)
+k_tight_emit_rm(const double* __restrict__ pos, const uint32_t* __restrict__ ridx, const float* __restrict__ rd2,
+                int64_t cap, const int32_t* __restrict__ rowcnt, const unsigned long long* __restrict__ eoff,
+                const uint32_t* __restrict__ ebuf, const float* __restrict__ ed2, const double* __restrict__ hf,
+                int64_t n, int32_t* __restrict__ out_len, int32_t* __restrict__ out_idx, int64_t cap_out,
+                unsigned long long* __restrict__ total)
+{
+    const int glane = threadIdx.x & (kG - 1);
+    const int gshift = (threadIdx.x & 63) & ~(kG - 1);
+    const int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kG;
+    const bool live = i < n;
+    const int64_t ii = live ? i : n - 1;
+    const double hi = hf[ii];
+    const int32_t ng = live ? rowcnt[ii] : 0;
+    const int64_t e0 = (int64_t)eoff[ii], ne = live ? (int64_t)(eoff[ii + 1] - eoff[ii]) : 0;
+    const int64_t len = ng + ne;
+    int32_t o = 0;
+    for (int64_t kb = 0;; kb += kG) {
+        const bool in_range = kb < len;
+        if (__ballot(in_range) == 0) break;
+        const int64_t k = kb + glane;
+        bool pred = false;
+        uint32_t j = 0;
+        if (in_range && k < len) {
+            float d2;
+            if (k < ng) {
+                j = ridx[ii * cap + k] & kMask;
+                d2 = rd2[ii * cap + k];
+            } else {
+                j = ebuf[e0 + (k - ng)];
+                d2 = ed2[e0 + (k - ng)];
+            }
+            const double rr = NB_R_KERN * pymax(hi, hf[j]);
+            const double rr2 = rr * rr;
+            const double df = (double)d2;
+            if (fabs(df - rr2) <= 2.5e-7 * rr2) {
+                const double dx = pos[3 * j] - pos[3 * ii];
+                const double dy = pos[3 * j + 1] - pos[3 * ii + 1];
+                const double dz = pos[3 * j + 2] - pos[3 * ii + 2];
+                pred = dx * dx + dy * dy + dz * dz < rr2;
+            } else {
+                pred = df < rr2;
+            }
+        }
+        const unsigned long long bal = __ballot(pred);
+        const unsigned gb = (unsigned)((bal >> gshift) & ((1ull << kG) - 1));
+        if (pred) {
+            const int slot = o + __popc(gb & ((1u << glane) - 1));
+            if (slot < cap_out) out_idx[ii * cap_out + slot] = (int32_t)j;
+        }
+        o += __popc(gb);
+    }
+    if (live && glane == 0) out_len[i] = o < cap_out ? o : (int32_t)cap_out;
+    const unsigned long long wt = wave_sum_u64(live && glane == 0 ? (unsigned long long)o : 0ull);
+    if ((threadIdx.x & 63) == 0 && wt) atomicAdd(total, wt);
+}
+
 static inline unsigned grid_of(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 }  // namespace nb
@@ -824,6 +1189,11 @@ Box make_box(const double* box3)
 int build_grid(nb_ctx* ctx, hipStream_t s, const double* pos, const double* h, int64_t n, const Box& box, Grid& g,
                int64_t& ncells)
 {
+    // every list builder starts here and reuses the context's list buffers: lists of an
+    // earlier build can no longer be emitted
+    ctx->lists_ready = 0;
+    ctx->hydro_ready = 0;
+    ctx->hydro_mixed_ready = 0;
     int nparts = (int)std::min<int64_t>(1024, (n + kBlock - 1) / kBlock);
     if (ctx_reserve(ctx, (void**)&ctx->red, &ctx->red_bytes, sizeof(double) * 8 * (nparts + 2))) return NB_ERR_HIP;
     double* fin = ctx->red + 8 * nparts;
@@ -1256,6 +1626,112 @@ extern "C" int32_t nb_hydro_tight(nb_ctx* ctx, void* stream, const double* h_fin
                        out_index, row_cap, tot);
     unsigned long long t = 0;
     int rc = sync_copy(ctx, s, &t, tot, sizeof(t), "nb_hydro_tight: total copy failed");
+    if (rc) return rc;
+    *total_out = (int64_t)t;
+    return NB_OK;
+}
+
+
+extern "C" int32_t nb_hydro_density_mixed(nb_ctx* ctx, void* stream, const double* pos, const double* mass,
+                                          double m_uniform, const double* h_start, const double* h_search, int64_t n,
+                                          int32_t kid, double cnorm, double eta, double tol, double h_max,
+                                          double* h_out, double* rho, double* omega, int32_t* iters, int32_t* flags,
+                                          int64_t* stats4)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (!ctx || n <= 0 || !stats4 || n >= (int64_t)kMask) return ctx_fail(ctx, "nb_hydro_density_mixed: bad arguments");
+    const Box open{{0.0, 0.0, 0.0}};
+    Grid g;
+    int64_t ncells = 0;
+    int rc = build_grid(ctx, s, pos, h_search, n, open, g, ncells);
+    if (rc) return rc;
+    const bool masses = !(m_uniform > 0.0);
+    if (ctx_reserve(ctx, (void**)&ctx->rowcnt, &ctx->rowcnt_bytes, sizeof(int32_t) * n) ||
+        ctx_reserve(ctx, (void**)&ctx->ecount, &ctx->ecount_bytes, sizeof(unsigned long long) * (n + 1)) ||
+        ctx_reserve(ctx, (void**)&ctx->eoff, &ctx->eoff_bytes, sizeof(unsigned long long) * (n + 1)) ||
+        ctx_reserve(ctx, (void**)&ctx->tot, &ctx->tot_bytes, sizeof(unsigned long long) * (n + 1)) ||
+        ctx_reserve(ctx, (void**)&ctx->offs, &ctx->offs_bytes, sizeof(unsigned long long) * (n + 1)) ||
+        ctx_reserve(ctx, (void**)&ctx->efill, &ctx->efill_bytes, sizeof(unsigned int) * n) ||
+        ctx_reserve(ctx, (void**)&ctx->scal, &ctx->scal_bytes, sizeof(int64_t) * 4))
+        return NB_ERR_HIP;
+    int64_t cap = ctx->row_cap_rm > 0 ? ctx->row_cap_rm : 192;
+    int32_t sc[4] = {0, 0, 0, 0};
+    MixArgs A{mass, m_uniform, h_start, h_max, kid, cnorm, eta, tol, h_out, rho, omega, iters, flags, (int32_t*)ctx->scal};
+    const unsigned blocks = (unsigned)((n * kG + 255) / 256);
+    for (int attempt = 0; attempt < 8; attempt++) {
+        const size_t cells = (size_t)(n * cap);
+        if (ctx_reserve(ctx, (void**)&ctx->gbuf, &ctx->gbuf_bytes, sizeof(uint32_t) * cells) ||
+            ctx_reserve(ctx, (void**)&ctx->fscratch, &ctx->fscratch_bytes, sizeof(float) * cells * (masses ? 2 : 1)))
+            return NB_ERR_HIP;
+        if (hipMemsetAsync(ctx->ecount, 0, sizeof(unsigned long long) * (n + 1), s) != hipSuccess ||
+            hipMemsetAsync(ctx->scal, 0, sizeof(int64_t) * 4, s) != hipSuccess)
+            return ctx_fail(ctx, "memset");
+        float* rmass = masses ? ctx->fscratch + cells : nullptr;
+        if (masses)
+            hipLaunchKernelGGL(k_gather_solve_mixed<true>, dim3(blocks), dim3(256), 0, s, pos, h_search, n, g,
+                               ctx->cell_start, ctx->order, ctx->gbuf, ctx->fscratch, rmass, cap, ctx->rowcnt,
+                               ctx->ecount, A);
+        else
+            hipLaunchKernelGGL(k_gather_solve_mixed<false>, dim3(blocks), dim3(256), 0, s, pos, h_search, n, g,
+                               ctx->cell_start, ctx->order, ctx->gbuf, ctx->fscratch, rmass, cap, ctx->rowcnt,
+                               ctx->ecount, A);
+        rc = sync_copy(ctx, s, sc, ctx->scal, sizeof(sc), "nb_hydro_density_mixed: gather failed");
+        if (rc) return rc;
+        if (sc[0] <= cap) break;
+        cap = ((sc[0] + sc[0] / 8 + 15) / 16) * 16;
+        if (attempt == 7) return ctx_fail(ctx, "row capacity did not settle");
+    }
+    ctx->row_cap_rm = (int32_t)cap;
+    const int32_t from_mem = sc[1];
+    if (hipMemsetAsync(ctx->scal, 0, sizeof(int64_t) * 4, s) != hipSuccess) return ctx_fail(ctx, "memset");
+    hipLaunchKernelGGL(k_row_total, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, ctx->rowcnt, ctx->ecount, ctx->tot,
+                       (int32_t*)ctx->scal, n);
+    rc = scan_u64(ctx, s, ctx->ecount, ctx->eoff, n + 1);
+    if (rc) return rc;
+    rc = scan_u64(ctx, s, ctx->tot, ctx->offs, n + 1);
+    if (rc) return rc;
+    unsigned long long tots[2] = {0, 0};
+    rc = sync_copy(ctx, s, &tots[0], ctx->eoff + n, sizeof(unsigned long long), "extras count copy failed");
+    if (rc) return rc;
+    rc = sync_copy(ctx, s, &tots[1], ctx->offs + n, sizeof(unsigned long long), "merged total copy failed");
+    if (rc) return rc;
+    rc = sync_copy(ctx, s, sc, ctx->scal, sizeof(sc), "merged row length copy failed");
+    if (rc) return rc;
+    if (ctx_reserve(ctx, (void**)&ctx->ebuf, &ctx->ebuf_bytes, sizeof(uint32_t) * (tots[0] + 1)) ||
+        ctx_reserve(ctx, (void**)&ctx->edbuf, &ctx->edbuf_bytes, sizeof(float) * (tots[0] + 1)))
+        return NB_ERR_HIP;
+    if (hipMemsetAsync(ctx->efill, 0, sizeof(unsigned int) * n, s) != hipSuccess) return ctx_fail(ctx, "memset");
+    hipLaunchKernelGGL(k_extras_fill_rm, dim3(blocks), dim3(256), 0, s, ctx->gbuf, ctx->fscratch, cap, ctx->rowcnt,
+                       ctx->eoff, ctx->efill, ctx->ebuf, (float*)ctx->edbuf, n);
+    hipLaunchKernelGGL(k_sort_extras, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->ebuf, (float*)ctx->edbuf, ctx->eoff,
+                       n);
+    if (hipGetLastError() != hipSuccess) return ctx_fail(ctx, "nb_hydro_density_mixed: kernel launch failed");
+    ctx->lists_n = n;
+    ctx->lists_maxrow = sc[2];
+    ctx->hydro_mixed_ready = 1;
+    stats4[0] = cap;
+    stats4[1] = from_mem;                // rows solved from memory (register stash too small)
+    stats4[2] = (int64_t)tots[1];        // search-list entries (incl. self)
+    stats4[3] = sc[2];                   // longest merged row
+    return NB_OK;
+}
+
+extern "C" int32_t nb_hydro_tight_mixed(nb_ctx* ctx, void* stream, const double* pos, const double* h_final,
+                                        int32_t* out_len, int32_t* out_index, int64_t row_cap, int64_t* total_out)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (!ctx || !ctx->hydro_mixed_ready || !total_out)
+        return ctx_fail_code(ctx, NB_ERR_NOT_BUILT, "nb_hydro_tight_mixed: no density pass");
+    if (row_cap < ctx->lists_maxrow) return ctx_fail(ctx, "nb_hydro_tight_mixed: row capacity below the longest row");
+    const int64_t n = ctx->lists_n;
+    unsigned long long* tot = (unsigned long long*)ctx->scal + 3;
+    if (hipMemsetAsync(tot, 0, sizeof(unsigned long long), s) != hipSuccess) return ctx_fail(ctx, "memset");
+    const unsigned blocks = (unsigned)((n * kG + 255) / 256);
+    hipLaunchKernelGGL(k_tight_emit_rm, dim3(blocks), dim3(256), 0, s, pos, ctx->gbuf, ctx->fscratch,
+                       (int64_t)ctx->row_cap_rm, ctx->rowcnt, ctx->eoff, ctx->ebuf, (const float*)ctx->edbuf, h_final,
+                       n, out_len, out_index, row_cap, tot);
+    unsigned long long t = 0;
+    int rc = sync_copy(ctx, s, &t, tot, sizeof(t), "nb_hydro_tight_mixed: total copy failed");
     if (rc) return rc;
     *total_out = (int64_t)t;
     return NB_OK;

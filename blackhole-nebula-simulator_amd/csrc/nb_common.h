@@ -67,9 +67,10 @@ struct Box {
     double l[3];  // periodic lengths, 0 = open axis
 };
 
-// Neighbour-list views of the pair passes. CSR rows (int64 offsets, the reference's layout) or
+// Neighbour-list views of the pair passes. CSR rows (int64 offsets, the reference's layout),
 // 64-row blocks: entry k of row i at ((i / 64) cap + k) 64 + i % 64, so that the 64 lanes of a
-// wavefront reading entry k of their own rows touch one contiguous 256-byte segment.
+// wavefront reading entry k of their own rows touch one contiguous 256-byte segment, or
+// row-major rows (entry k of row i at i cap + k) for passes where a lane group shares a row.
 struct CsrView {
     const int64_t* off;
     const int32_t* idx;
@@ -88,6 +89,30 @@ struct BlkView {
     __device__ __forceinline__ int64_t base(int64_t i) const { return ((i >> 6) * cap) * 64 + (i & 63); }
     __device__ __forceinline__ int32_t at(int64_t i, int64_t k) const { return idx[base(i) + k * 64]; }
 };
+
+struct RowView {
+    const int32_t* idx;
+    const int32_t* len;
+    int64_t cap;
+    static constexpr int64_t stride = 1;
+    __device__ __forceinline__ int64_t count(int64_t i) const { return len[i]; }
+    __device__ __forceinline__ int64_t base(int64_t i) const { return i * cap; }
+    __device__ __forceinline__ int32_t at(int64_t i, int64_t k) const { return idx[i * cap + k]; }
+};
+
+// Wave-wide reductions (every lane of the wavefront must call them); the result is valid in
+// all lanes. Used so that one lane per wavefront, not one per row, touches a global counter.
+__device__ __forceinline__ int wave_max_i32(int v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
 
 // Particle rows of a pass: rows[q] if a row list is given, else q.
 __device__ __forceinline__ int64_t row_of(const int64_t* rows, int64_t q)

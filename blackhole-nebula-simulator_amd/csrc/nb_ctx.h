@@ -38,14 +38,17 @@ struct nb_ctx {
     // density distance cache (one double per search-list entry)
     double* dscratch = nullptr; size_t dscratch_bytes = 0;
     float* fscratch = nullptr; size_t fscratch_bytes = 0;
+    void* pack = nullptr; size_t pack_bytes = 0;  // per-particle FP32 records of the mixed passes
     // reductions
     double* red = nullptr; size_t red_bytes = 0;
     int64_t* scal = nullptr; size_t scal_bytes = 0;
     void* cub_tmp = nullptr; size_t cub_tmp_bytes = 0;
     int32_t row_cap = 0;
+    int32_t row_cap_rm = 0;
     int64_t lists_n = 0, lists_total = 0, lists_maxrow = 0, last_unsorted = 0;
     int lists_ready = 0;
     int hydro_ready = 0;
+    int hydro_mixed_ready = 0;
     int64_t last_grid_cells = 0;
     double last_cell = 0.0;
 };

@@ -6,6 +6,7 @@
 #include "nb_common.h"
 #include "nb_ctx.h"
 #include "nebula_hip.h"
+#include "nb_linalg.h"
 
 namespace nb {
 
@@ -242,85 +243,6 @@ k_forces64(const double* __restrict__ pos, const double* __restrict__ vel, const
     if (cheat) cheat[q] = cond_heat;
 }
 
-// ----------------------------------------------------------------- 3x3 helpers
-// Gaussian elimination with partial pivoting (first largest |pivot|), multipliers by
-// division; determinant from the pivots; inverse by solving A X = I column by column.
-__device__ __forceinline__ void lu3(const double* a_in, double* lu, int* perm, int& sign)
-{
-    for (int k = 0; k < 9; k++) lu[k] = a_in[k];
-    for (int i = 0; i < 3; i++) perm[i] = i;
-    sign = 1;
-    for (int j = 0; j < 3; j++) {
-        int p = j;
-        double best = fabs(lu[3 * j + j]);
-        for (int i = j + 1; i < 3; i++) {
-            double v = fabs(lu[3 * i + j]);
-            if (v > best) { best = v; p = i; }
-        }
-        if (p != j) {
-            for (int k = 0; k < 3; k++) {
-                double t = lu[3 * j + k]; lu[3 * j + k] = lu[3 * p + k]; lu[3 * p + k] = t;
-            }
-            int t = perm[j]; perm[j] = perm[p]; perm[p] = t;
-            sign = -sign;
-        }
-        double piv = lu[3 * j + j];
-        if (piv != 0.0) {
-            for (int i = j + 1; i < 3; i++) {
-                double l = lu[3 * i + j] / piv;
-                lu[3 * i + j] = l;
-                for (int k = j + 1; k < 3; k++) lu[3 * i + k] -= l * lu[3 * j + k];
-            }
-        }
-    }
-}
-
-__device__ __forceinline__ double det3(const double* a)
-{
-    double lu[9]; int perm[3], sign;
-    lu3(a, lu, perm, sign);
-    double d = lu[0] * lu[4] * lu[8];
-    return sign < 0 ? -d : d;
-}
-
-__device__ __forceinline__ void inv3(const double* a, double* out)
-{
-    double lu[9]; int perm[3], sign;
-    lu3(a, lu, perm, sign);
-    for (int col = 0; col < 3; col++) {
-        double y[3];
-        for (int i = 0; i < 3; i++) {
-            double s = perm[i] == col ? 1.0 : 0.0;
-            for (int k = 0; k < i; k++) s -= lu[3 * i + k] * y[k];
-            y[i] = s;
-        }
-        for (int i = 2; i >= 0; i--) {
-            double s = y[i];
-            for (int k = i + 1; k < 3; k++) s -= lu[3 * i + k] * y[k];
-            y[i] = s / lu[3 * i + i];
-        }
-        for (int i = 0; i < 3; i++) out[3 * i + col] = y[i];
-    }
-}
-
-__device__ __forceinline__ void matmul3(const double* a, const double* b, double* c)
-{
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) {
-            double s = 0.0;
-            for (int k = 0; k < 3; k++) s += a[3 * i + k] * b[3 * k + j];
-            c[3 * i + j] = s;
-        }
-}
-
-// sph.py:776-782
-__device__ __forceinline__ bool near_singular(const double* rmat, int64_t cnt)
-{
-    double scale = (rmat[0] + rmat[4] + rmat[8]) / 3.0;
-    if (cnt < 4 || !(scale > 0.0)) return true;
-    return !(fabs(det3(rmat)) >= 1e-6 * ipow3(scale));
-}
-
 // ----------------------------------------------------------------- gradients
 // sph.py:837-884 (_gradients_kernel); with_acc = false means acc is zero.
 // sph.py:785-815 (_velocity_gradient_kernel) when gv_out is given (then div/curl/ddiv unused).
@@ -484,6 +406,13 @@ k_disc64(const double* __restrict__ pos, const double* __restrict__ vel, const d
     dudt[q] = heat;
 }
 
+// pairs_mixed.hip: the precision-1 branch of nb_forces
+int32_t forces_mixed(nb_ctx* ctx, hipStream_t s, const double* pos, const double* vel, const double* mass,
+                     const double* h, const double* rho, const double* omega, const double* pressure, const double* u,
+                     const double* c_s, const double* alpha, const nb_lists* lists, const double* box3,
+                     const int64_t* rows, int64_t nrows, int64_t n, const nb_force_params* P, double* acc,
+                     double* dudt, double* vsig_max, double* vheat, double* cheat);
+
 static inline unsigned grid_p(int64_t n) { return (unsigned)((n + kBlockP - 1) / kBlockP); }
 
 static inline Box make_box(const double* box3)
@@ -509,7 +438,10 @@ static int32_t launch_check(nb_ctx* ctx, const char* what)
 
 #define NB_DISPATCH(L, KERNEL, GRID, STREAM, ...)                                                       \
     do {                                                                                                \
-        if ((L)->length) {                                                                              \
+        if ((L)->length && (L)->row_major) {                                                            \
+            RowView v_{(L)->index, (L)->length, (L)->row_cap};                                          \
+            hipLaunchKernelGGL(KERNEL<RowView>, dim3(GRID), dim3(kBlockP), 0, STREAM, __VA_ARGS__);      \
+        } else if ((L)->length) {                                                                       \
             BlkView v_{(L)->index, (L)->length, (L)->row_cap};                                          \
             hipLaunchKernelGGL(KERNEL<BlkView>, dim3(GRID), dim3(kBlockP), 0, STREAM, __VA_ARGS__);      \
         } else {                                                                                        \
@@ -550,6 +482,12 @@ extern "C" int32_t nb_forces(nb_ctx* ctx, void* stream, const double* pos, const
     if (params->use_cond && !u) return ctx_fail(ctx, "nb_forces: conductivity needs u");
     if (params->use_recon && !grad_v) return ctx_fail(ctx, "nb_forces: reconstruction needs grad_v");
     if (nrows == 0) return NB_OK;
+    if (params->precision == 1 && !params->use_recon) {
+        const int64_t n = rows ? (int64_t)lists->n_rows : nrows;
+        if (n <= 0) return ctx_fail(ctx, "nb_forces: mixed precision on a row subset needs nb_lists.n_rows");
+        return forces_mixed(ctx, (hipStream_t)stream, pos, vel, mass, h, rho, omega, pressure, u, c_s, alpha, lists,
+                            box3, rows, nrows, n, params, acc, dudt, vsig_max, vheat, cheat);
+    }
     NB_DISPATCH(lists, k_forces64, grid_p(nrows), (hipStream_t)stream, pos, vel, mass, h, rho, omega, pressure, u, c_s,
                 alpha, grad_v, v_, make_box(box3), rows, nrows, *params, acc, dudt, vsig_max, vheat, cheat);
     return launch_check(ctx, "nb_forces");

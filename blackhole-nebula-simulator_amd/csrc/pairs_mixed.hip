@@ -1,0 +1,326 @@
+// Mixed-precision pair passes: gradients (for the viscosity switches) and forces with FP32 pair
+// arithmetic.
+//
+// A group of 16 lanes serves one row: lane l takes entries l, l + 16, ... so that the group
+// reads a row-major or CSR row contiguously. Each lane accumulates its pairs in FP32 with
+// compensated (Kahan) sums and the group combines the lane partials in FP64. Separations are
+// formed in FP64 (positions stay FP64: at r ~ 2000 an FP32 position resolves only ~1e-4 of a
+// unit length) and then rounded to FP32; every other per-neighbour quantity comes from a packed
+// FP32 record prepared once per pass (48 bytes for forces, 32 for gradients: whole 16-byte loads).
+#include <string>
+
+#include "nb_common.h"
+#include "nb_ctx.h"
+#include "nb_linalg.h"
+#include "nebula_hip.h"
+
+namespace nb {
+
+static constexpr int kGM = 16;
+static constexpr int kBlockM = 256;
+
+struct __align__(16) PF {
+    float4 a;  // vx, vy, vz, 1/h
+    float4 b;  // cnorm / h^4, P / (Omega rho^2), 1 / (Omega rho), alpha c_s
+    float4 c;  // m, u, P, rho
+};
+
+struct __align__(16) PG {
+    float4 a;  // vx, vy, vz, 1/h
+    float4 b;  // cnorm / h^4, m, 0, 0
+};
+
+__global__ void k_pack_forces(const double* __restrict__ vel, const double* __restrict__ mass,
+                              const double* __restrict__ h, const double* __restrict__ rho,
+                              const double* __restrict__ omega, const double* __restrict__ pressure,
+                              const double* __restrict__ u, const double* __restrict__ c_s,
+                              const double* __restrict__ alpha, double cnorm, int64_t n, PF* __restrict__ out)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double hi = h[i], r = rho[i], om = omega[i], p = pressure[i];
+    PF v;
+    v.a = make_float4((float)vel[3 * i], (float)vel[3 * i + 1], (float)vel[3 * i + 2], (float)(1.0 / hi));
+    v.b = make_float4((float)(cnorm / ipow4(hi)), (float)(p / (om * r * r)), (float)(1.0 / (om * r)),
+                      (float)(alpha ? alpha[i] * c_s[i] : 0.0));
+    v.c = make_float4((float)mass[i], (float)(u ? u[i] : 0.0), (float)p, (float)r);
+    out[i] = v;
+}
+
+__global__ void k_pack_grad(const double* __restrict__ vel, const double* __restrict__ mass,
+                            const double* __restrict__ h, double cnorm, int64_t n, PG* __restrict__ out)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double hi = h[i];
+    PG v;
+    v.a = make_float4((float)vel[3 * i], (float)vel[3 * i + 1], (float)vel[3 * i + 2], (float)(1.0 / hi));
+    v.b = make_float4((float)(cnorm / ipow4(hi)), (float)mass[i], 0.f, 0.f);
+    out[i] = v;
+}
+
+// kernels_nb.py:31-43 in FP32
+__device__ __forceinline__ float df32(float q, int kid)
+{
+    if (q >= 2.f) return 0.f;
+    if (kid == 0) {
+        if (q < 1.f) return -3.f * q + 2.25f * q * q;
+        const float t = 2.f - q;
+        return -0.75f * t * t;
+    }
+    const float x = 1.f - 0.5f * q, x2 = x * x;
+    return -7.f / 3.f * q * x2 * x2 * x * (5.f * q + 2.f);
+}
+
+struct Kahan {
+    float s = 0.f, c = 0.f;
+    __device__ __forceinline__ void add(float v)
+    {
+        const float t = v - c;
+        const float u = s + t;
+        c = (u - s) - t;
+        s = u;
+    }
+    __device__ __forceinline__ double value() const { return (double)s - (double)c; }
+};
+
+// butterfly over the 16 lanes of a group: every lane ends with the same value
+__device__ __forceinline__ double gsum(double v)
+{
+    for (int o = kGM / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kGM);
+    return v;
+}
+
+__device__ __forceinline__ float gmax(float v)
+{
+    for (int o = kGM / 2; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, kGM));
+    return v;
+}
+
+// separation x_a - x_b (minimum image), formed in FP64, rounded to FP32
+__device__ __forceinline__ void sep32(const double* pos, double pa0, double pa1, double pa2, int64_t b, const Box& box,
+                                      float& d0, float& d1, float& d2)
+{
+    d0 = (float)min_image(pa0 - pos[3 * b], box.l[0]);
+    d1 = (float)min_image(pa1 - pos[3 * b + 1], box.l[1]);
+    d2 = (float)min_image(pa2 - pos[3 * b + 2], box.l[2]);
+}
+
+// sph.py:525-652 without reconstruction: pressure force with grad-h terms, Monaghan viscosity on
+// approaching pairs, conductivity (pressure or velocity signal), P dV work, largest v_sig.
+template <class L>
+__global__ void __launch_bounds__(kBlockM)
+k_forces_mixed(const double* __restrict__ pos, const PF* __restrict__ pf, const double* __restrict__ c_s, L lists,
+               Box box, const int64_t* __restrict__ rows, int64_t nrows, nb_force_params P, double* __restrict__ acc,
+               double* __restrict__ dudt, double* __restrict__ vsig_max, double* __restrict__ vheat,
+               double* __restrict__ cheat)
+{
+    const int glane = threadIdx.x & (kGM - 1);
+    const int64_t q = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kGM;
+    if (q >= nrows) return;  // whole groups leave together
+    const int64_t a = row_of(rows, q);
+    const double pa0 = pos[3 * a], pa1 = pos[3 * a + 1], pa2 = pos[3 * a + 2];
+    const PF A = pf[a];
+    const float beta = (float)P.beta, alpha_u = (float)P.alpha_u;
+    const bool use_visc = P.use_visc, use_cond = P.use_cond;
+    const int kid = P.kid;
+    Kahan ax, ay, az, work, visc, cond;
+    float vmax = 0.f;
+    const int64_t m = lists.count(a);
+    for (int64_t k = glane; k < m; k += kGM) {
+        const int32_t b = lists.at(a, k);
+        if (b == a) continue;
+        const PF B = pf[b];
+        float dx0, dx1, dx2;
+        sep32(pos, pa0, pa1, pa2, b, box, dx0, dx1, dx2);
+        const float r2 = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
+        const float inv_r = 1.f / __fsqrt_rn(r2);
+        const float r = r2 * inv_r;
+        const float rx = dx0 * inv_r, ry = dx1 * inv_r, rz = dx2 * inv_r;
+        const float fa = A.b.x * df32(r * A.a.w, kid);
+        const float fb = B.b.x * df32(r * B.a.w, kid);
+        const float vdotr = (A.a.x - B.a.x) * rx + (A.a.y - B.a.y) * ry + (A.a.z - B.a.z) * rz;
+        float na = A.b.y, nbb = B.b.y;  // (P - rho v_sig v.r / 2) / (Omega rho^2) of either side
+        if (use_visc) {
+            const float bv = beta * fabsf(vdotr);
+            const float vsa = A.b.w + bv;
+            vmax = fmaxf(vmax, vsa);
+            if (vdotr < 0.f) {
+                const float vsb = B.b.w + bv;
+                na += -0.5f * vsa * vdotr * A.b.z;
+                nbb += -0.5f * vsb * vdotr * B.b.z;
+                visc.add(-(B.c.x * vsa * 0.5f * vdotr * vdotr * fa * A.b.z));
+            }
+        }
+        if (use_cond) {
+            const float vu =
+                P.cond_signal == 0 ? __fsqrt_rn(fabsf(A.c.z - B.c.z) / (0.5f * (A.c.w + B.c.w))) : fabsf(vdotr);
+            cond.add(B.c.x * alpha_u * vu * (A.c.y - B.c.y) * 0.5f * (fa * A.b.z + fb * B.b.z));
+        }
+        const float f = B.c.x * (na * fa + nbb * fb);
+        ax.add(-f * rx);
+        ay.add(-f * ry);
+        az.add(-f * rz);
+        work.add(B.c.x * vdotr * fa);
+    }
+    const double sx = gsum(ax.value()), sy = gsum(ay.value()), sz = gsum(az.value());
+    const double sw = gsum(work.value()), sv = gsum(visc.value()), sc = gsum(cond.value());
+    const float vm = gmax(vmax);
+    if (glane == 0) {
+        acc[3 * q] = sx;
+        acc[3 * q + 1] = sy;
+        acc[3 * q + 2] = sz;
+        dudt[q] = (double)A.b.y * sw + sv + sc;
+        vsig_max[q] = use_visc ? fmax(c_s[a], (double)vm) : 0.0;
+        if (vheat) vheat[q] = sv;
+        if (cheat) cheat[q] = sc;
+    }
+}
+
+// sph.py:837-884: R, G_v and G_a of a row from FP32 pair terms, inverted in FP64.
+template <class L>
+__global__ void __launch_bounds__(kBlockM)
+k_gradients_mixed(const double* __restrict__ pos, const double* __restrict__ accel, const PG* __restrict__ pg, L lists,
+                  Box box, const int64_t* __restrict__ rows, int64_t nrows, int kid, double* __restrict__ div,
+                  double* __restrict__ curl, double* __restrict__ ddiv)
+{
+    const int glane = threadIdx.x & (kGM - 1);
+    const int64_t q = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kGM;
+    if (q >= nrows) return;
+    const int64_t a = row_of(rows, q);
+    const double pa0 = pos[3 * a], pa1 = pos[3 * a + 1], pa2 = pos[3 * a + 2];
+    const PG A = pg[a];
+    Kahan R[9], Gv[9], Ga[9];
+    float aa[3] = {0.f, 0.f, 0.f};
+    if (accel)
+        for (int c = 0; c < 3; c++) aa[c] = (float)accel[3 * a + c];
+    int cnt = 0;
+    const int64_t m = lists.count(a);
+    for (int64_t k = glane; k < m; k += kGM) {
+        const int32_t b = lists.at(a, k);
+        if (b == a) continue;
+        cnt++;
+        const PG B = pg[b];
+        float dx[3];
+        sep32(pos, pa0, pa1, pa2, b, box, dx[0], dx[1], dx[2]);
+        const float r2 = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
+        const float inv_r = 1.f / __fsqrt_rn(r2);
+        const float g = A.b.x * df32(r2 * inv_r * A.a.w, kid) * inv_r;
+        const float dv[3] = {B.a.x - A.a.x, B.a.y - A.a.y, B.a.z - A.a.z};
+        float da[3] = {0.f, 0.f, 0.f};
+        if (accel)
+            for (int c = 0; c < 3; c++) da[c] = (float)accel[3 * b + c] - aa[c];
+        const float mg = B.b.y * g;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const float gj = mg * dx[j];
+                R[3 * i + j].add(-(dx[i] * gj));
+                Gv[3 * i + j].add(dv[i] * gj);
+                if (accel) Ga[3 * i + j].add(da[i] * gj);
+            }
+    }
+    double rm[9], gvm[9], gam[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        rm[k] = gsum(R[k].value());
+        gvm[k] = gsum(Gv[k].value());
+        gam[k] = gsum(Ga[k].value());
+    }
+    for (int o = kGM / 2; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, kGM);
+    if (glane != 0) return;
+    double dvm[9], dam[9];
+    if (near_singular(rm, cnt)) {
+        const double scale = (rm[0] + rm[4] + rm[8]) / 3.0;
+        const double inv_s = scale > 0.0 ? 1.0 / scale : 0.0;
+        for (int k = 0; k < 9; k++) {
+            dvm[k] = gvm[k] * inv_s;
+            dam[k] = gam[k] * inv_s;
+        }
+    } else {
+        double rinv[9];
+        inv3(rm, rinv);
+        matmul3(gvm, rinv, dvm);
+        matmul3(gam, rinv, dam);
+    }
+    div[q] = dvm[0] + dvm[4] + dvm[8];
+    const double c0 = dvm[7] - dvm[5], c1 = dvm[2] - dvm[6], c2 = dvm[3] - dvm[1];
+    curl[q] = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+    double ss = 0.0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) ss += dvm[3 * i + j] * dvm[3 * j + i];
+    ddiv[q] = dam[0] + dam[4] + dam[8] - ss;
+}
+
+static inline unsigned grid_m(int64_t nrows) { return (unsigned)((nrows * kGM + kBlockM - 1) / kBlockM); }
+static inline unsigned grid_1(int64_t n) { return (unsigned)((n + kBlockM - 1) / kBlockM); }
+
+static inline Box box_of(const double* box3)
+{
+    Box b{{0.0, 0.0, 0.0}};
+    if (box3)
+        for (int c = 0; c < 3; c++) b.l[c] = box3[c];
+    return b;
+}
+
+#define NB_DISPATCH_M(L, KERNEL, GRID, STREAM, ...)                                                     \
+    do {                                                                                                \
+        if ((L)->length && (L)->row_major) {                                                            \
+            RowView v_{(L)->index, (L)->length, (L)->row_cap};                                          \
+            hipLaunchKernelGGL(KERNEL<RowView>, dim3(GRID), dim3(kBlockM), 0, STREAM, __VA_ARGS__);      \
+        } else if ((L)->length) {                                                                       \
+            BlkView v_{(L)->index, (L)->length, (L)->row_cap};                                          \
+            hipLaunchKernelGGL(KERNEL<BlkView>, dim3(GRID), dim3(kBlockM), 0, STREAM, __VA_ARGS__);      \
+        } else {                                                                                        \
+            CsrView v_{(L)->offset, (L)->index};                                                        \
+            hipLaunchKernelGGL(KERNEL<CsrView>, dim3(GRID), dim3(kBlockM), 0, STREAM, __VA_ARGS__);      \
+        }                                                                                               \
+    } while (0)
+
+static int32_t launched(nb_ctx* ctx, const char* what)
+{
+    const hipError_t e = hipGetLastError();
+    if (e == hipSuccess) return NB_OK;
+    const std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    return ctx_fail(ctx, m.c_str());
+}
+
+// the mixed branch of nb_forces (precision 1, no reconstruction); arguments checked by the caller
+int32_t forces_mixed(nb_ctx* ctx, hipStream_t s, const double* pos, const double* vel, const double* mass,
+                     const double* h, const double* rho, const double* omega, const double* pressure, const double* u,
+                     const double* c_s, const double* alpha, const nb_lists* lists, const double* box3,
+                     const int64_t* rows, int64_t nrows, int64_t n, const nb_force_params* P, double* acc,
+                     double* dudt, double* vsig_max, double* vheat, double* cheat)
+{
+    if (ctx_reserve(ctx, (void**)&ctx->pack, &ctx->pack_bytes, sizeof(PF) * (size_t)n)) return NB_ERR_HIP;
+    PF* pf = (PF*)ctx->pack;
+    hipLaunchKernelGGL(k_pack_forces, dim3(grid_1(n)), dim3(kBlockM), 0, s, vel, mass, h, rho, omega, pressure,
+                       P->use_cond ? u : nullptr, c_s, P->use_visc ? alpha : nullptr, P->cnorm, n, pf);
+    NB_DISPATCH_M(lists, k_forces_mixed, grid_m(nrows), s, pos, pf, c_s, v_, box_of(box3), rows, nrows, *P, acc, dudt,
+                  vsig_max, vheat, cheat);
+    return launched(ctx, "nb_forces (mixed)");
+}
+
+}  // namespace nb
+
+using namespace nb;
+
+extern "C" int32_t nb_gradients_mixed(nb_ctx* ctx, void* stream, const double* pos, const double* vel,
+                                      const double* acc, const double* mass, const double* h, const nb_lists* lists,
+                                      const double* box3, const int64_t* rows, int64_t nrows, int32_t kid, double cnorm,
+                                      double* div, double* curl, double* ddiv)
+{
+    if (!ctx || nrows < 0 || !lists || !lists->index || !(lists->length || lists->offset))
+        return ctx_fail(ctx, "nb_gradients_mixed: bad arguments");
+    if (nrows == 0) return NB_OK;
+    const int64_t n = rows ? (int64_t)lists->n_rows : nrows;
+    if (n <= 0) return ctx_fail(ctx, "nb_gradients_mixed: a row subset needs nb_lists.n_rows");
+    if (ctx_reserve(ctx, (void**)&ctx->pack, &ctx->pack_bytes, sizeof(PG) * (size_t)n)) return NB_ERR_HIP;
+    hipStream_t s = (hipStream_t)stream;
+    PG* pg = (PG*)ctx->pack;
+    hipLaunchKernelGGL(k_pack_grad, dim3(grid_1(n)), dim3(kBlockM), 0, s, vel, mass, h, cnorm, n, pg);
+    NB_DISPATCH_M(lists, k_gradients_mixed, grid_m(nrows), s, pos, acc, pg, v_, box_of(box3), rows, nrows, kid, div,
+                  curl, ddiv);
+    return launched(ctx, "nb_gradients_mixed");
+}

@@ -48,7 +48,8 @@ class ForceParams(ctypes.Structure):
 
 
 class Lists(ctypes.Structure):
-    """nb_lists (include/nebula_hip.h): CSR when length is NULL, 64-row blocks otherwise."""
+    """nb_lists (include/nebula_hip.h): CSR when length is NULL, else rows of capacity row_cap in
+    64-row blocks (row_major 0) or row-major (row_major 1)."""
 
     _fields_ = [
         ("offset", _vp),
@@ -56,6 +57,8 @@ class Lists(ctypes.Structure):
         ("length", _vp),
         ("row_cap", _i64),
         ("n_entries", _i64),
+        ("row_major", _i32),
+        ("n_rows", _i32),
     ]
 
 
@@ -105,7 +108,15 @@ SIGNATURES = {
     "nb_hydro_density": (_i32, [_vp, _vp, _vp, _vp, _f64, _vp, _vp, _i64, _i32, _f64, _f64, _f64, _f64, _i32, _vp,
                                 _vp, _vp, _vp, _vp, _P64]),
     "nb_hydro_tight": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _P64]),
+    "nb_hydro_density_mixed": (_i32, [_vp, _vp, _vp, _vp, _f64, _vp, _vp, _i64, _i32, _f64, _f64, _f64, _f64, _vp,
+                                      _vp, _vp, _vp, _vp, _P64]),
+    "nb_hydro_tight_mixed": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _P64]),
+    "nb_gradients_mixed": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _PL, _PD, _vp, _i64, _i32, _f64, _vp, _vp,
+                                  _vp]),
 }
+
+
+ABI_VERSION = 2
 
 
 def load_library() -> ctypes.CDLL:
@@ -120,6 +131,8 @@ def load_library() -> ctypes.CDLL:
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    if lib.nb_abi_version() != ABI_VERSION:
+        raise RuntimeError(f"{LIB_PATH} has ABI {lib.nb_abi_version()}, this binding expects {ABI_VERSION}: rebuild")
     return lib
 
 

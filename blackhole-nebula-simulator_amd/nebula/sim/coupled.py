@@ -37,6 +37,7 @@ from .neighbours import grid_order, permute
 from .sph import (
     COND_VELOCITY,
     hydro_density_device,
+    hydro_density_mixed_device,
     density_device,
     disc_forces_device,
     forces_device,
@@ -194,8 +195,12 @@ def _hydro(st: GasState, cfg: CoupledConfig, v, dt_alpha: float, a_prev) -> GasS
     if hc.isothermal_cs is not None or hc.eos != "ideal":
         raise ValueError("the GPU step implements the ideal-gas equation of state")
     n = int(st.x.shape[0])
-    d, n_search = hydro_density_device(st.x, st.mass, st.h.clone(), hc.kernel, hc.eta, 1e-4, hc.h_max, hc.skin,
-                                       st.m_uniform, hc.precision)  # fmt: skip
+    if hc.precision == "mixed":
+        d, n_search = hydro_density_mixed_device(st.x, st.mass, st.h.clone(), hc.kernel, hc.eta, 1e-4, hc.h_max,
+                                                 hc.skin, st.m_uniform)  # fmt: skip
+    else:
+        d, n_search = hydro_density_device(st.x, st.mass, st.h.clone(), hc.kernel, hc.eta, 1e-4, hc.h_max, hc.skin,
+                                           st.m_uniform, hc.precision)  # fmt: skip
     p_gas = _dev.empty(n)
     cs = _dev.empty(n)
     nbad = torch.zeros(1, dtype=torch.int32, device=st.x.device)
@@ -207,13 +212,13 @@ def _hydro(st: GasState, cfg: CoupledConfig, v, dt_alpha: float, a_prev) -> GasS
     nb = d.neighbours  # tightened at the final h
     alpha = st.alpha
     if hc.viscosity and hc.switch and dt_alpha > 0:
-        g = gradients_device(st.x, v, a_prev, st.mass, d.h, nb, hc.kernel)
+        g = gradients_device(st.x, v, a_prev, st.mass, d.h, nb, hc.kernel, precision=hc.precision)
         alpha = _dev.empty(n)
         _hip.call("nb_alpha_switch", _hip.ptr(g.div_v), _hip.ptr(g.curl_v), _hip.ptr(g.ddivv_dt), _hip.ptr(cs),
                   _hip.ptr(d.h), _hip.ptr(st.alpha), float(dt_alpha), 1.0, 0.1, n, _hip.ptr(alpha))  # fmt: skip
     elif hc.viscosity and not hc.switch:
         if hc.balsara:
-            g = gradients_device(st.x, v, None, st.mass, d.h, nb, hc.kernel)
+            g = gradients_device(st.x, v, None, st.mass, d.h, nb, hc.kernel, precision=hc.precision)
             alpha = _dev.empty(n)
             _hip.call("nb_alpha_balsara", _hip.ptr(g.div_v), _hip.ptr(g.curl_v), _hip.ptr(cs), _hip.ptr(d.h),
                       float(hc.alpha_const), BALSARA_EPS, n, _hip.ptr(alpha))  # fmt: skip

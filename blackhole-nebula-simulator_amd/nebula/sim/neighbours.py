@@ -81,6 +81,7 @@ class Neighbours:
         d.length = None
         d.row_cap = 0
         d.n_entries = self.n_entries
+        d.n_rows = int(self.d_offset.numel()) - 1
         return d
 
 
@@ -106,6 +107,7 @@ class BlockedLists:
         d.length = self.length.data_ptr()
         d.row_cap = self.row_cap
         d.n_entries = self.n_entries
+        d.n_rows = int(self.length.numel())
         return d
 
     def to_csr(self) -> Neighbours:
@@ -118,6 +120,46 @@ class BlockedLists:
         out = np.empty(off[-1], np.int64)
         for i in range(n):
             out[off[i]:off[i + 1]] = idx[i // 64, : ln[i], i % 64]
+        return Neighbours(off, out, self.box)
+
+
+class RowLists:
+    """Neighbour lists in row-major rows of capacity ``row_cap`` (the mixed-precision step's
+    layout): row i has ``length[i]`` entries at index[i * row_cap : i * row_cap + length[i]],
+    not sorted."""
+
+    def __init__(self, length: torch.Tensor, index: torch.Tensor, row_cap: int, n_total: int, box=None):
+        self.length = length
+        self.index = index
+        self.row_cap = int(row_cap)
+        self.n_total = int(n_total)
+        self.box = box
+
+    @property
+    def n_entries(self) -> int:
+        return int(self.index.numel())
+
+    def desc(self) -> _hip.Lists:
+        d = _hip.Lists()
+        d.offset = None
+        d.index = self.index.data_ptr()
+        d.length = self.length.data_ptr()
+        d.row_cap = self.row_cap
+        d.n_entries = self.n_entries
+        d.row_major = 1
+        d.n_rows = int(self.length.numel())
+        return d
+
+    def to_csr(self) -> Neighbours:
+        """The same lists as a CSR Neighbours (host arrays, rows sorted)."""
+        n = int(self.length.numel())
+        ln = self.length.cpu().numpy().astype(np.int64)
+        idx = self.index.cpu().numpy().reshape(n, self.row_cap)
+        off = np.zeros(n + 1, np.int64)
+        np.cumsum(ln, out=off[1:])
+        out = np.empty(off[-1], np.int64)
+        for i in range(n):
+            out[off[i]:off[i + 1]] = np.sort(idx[i, : ln[i]])
         return Neighbours(off, out, self.box)
 
 

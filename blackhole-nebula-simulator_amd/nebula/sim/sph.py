@@ -179,6 +179,43 @@ def hydro_density_device(pos_d, mass_d, h_guess_d, kernel_name, eta, tol, h_max,
     return DensityResult(h, rho, omega, iters, flags, tight, None, builds), int(stats[2])
 
 
+def hydro_density_mixed_device(pos_d, mass_d, h_guess_d, kernel_name, eta, tol, h_max, skin, m_uniform):
+    """hydro_density_device in mixed precision: the same exact search lists, FP32 compensated
+    sums in the h-rho solve (Newton steps and convergence test in FP64), tight lists row-major.
+    Returns (DensityResult with RowLists as ``neighbours``, search-list entries)."""
+    from .neighbours import RowLists
+
+    n = int(pos_d.shape[0])
+    kid = KERNEL_IDS[kernel_name]
+    h_search = _dev.empty(n)
+    _hip.call("nb_search_lengths", _hip.ptr(h_guess_d), float(h_max), float(skin), float(np.inf), n,
+              _hip.ptr(h_search))  # fmt: skip
+    h, rho, omega = (_dev.empty(n) for _ in range(3))
+    iters = _dev.empty(n, torch.int32)
+    flags = _dev.empty(n, torch.int32)
+    stats = np.zeros(4, np.int64)
+    grown = np.zeros(1, np.int64)
+    for builds in range(1, 13):
+        _hip.call("nb_hydro_density_mixed", _hip.ptr(pos_d), _hip.ptr(mass_d), float(m_uniform), _hip.ptr(h_guess_d),
+                  _hip.ptr(h_search), n, kid, float(C_NORM[kid]), float(eta), float(tol), float(h_max), _hip.ptr(h),
+                  _hip.ptr(rho), _hip.ptr(omega), _hip.ptr(iters), _hip.ptr(flags),
+                  stats.ctypes.data_as(_hip._P64))  # fmt: skip
+        _hip.call("nb_grow_search", _hip.ptr(flags), _hip.ptr(h), None, n, float(np.inf), _hip.ptr(h_search),
+                  _hip.ptr(h_guess_d), grown.ctypes.data_as(_hip._P64))  # fmt: skip
+        if grown[0] == 0:
+            break
+    else:
+        raise RuntimeError("smoothing lengths kept outgrowing the neighbour search")
+    cap = max(int(stats[3]), 1)
+    ln = _dev.empty(n, torch.int32)
+    idx = _dev.empty(n * cap, torch.int32)
+    total = np.zeros(1, np.int64)
+    _hip.call("nb_hydro_tight_mixed", _hip.ptr(pos_d), _hip.ptr(h), _hip.ptr(ln), _hip.ptr(idx), cap,
+              total.ctypes.data_as(_hip._P64))  # fmt: skip
+    tight = RowLists(ln, idx, cap, int(total[0]), None)
+    return DensityResult(h, rho, omega, iters, flags, tight, None, builds), int(stats[2])
+
+
 # --------------------------------------------------------------------------- forces
 
 
@@ -282,25 +319,29 @@ class Gradients:
 SINGULAR_TOL = 1e-6
 
 
-def gradients_device(pos_d, vel_d, acc_d, mass_d, h_d, nb, kernel_name, rows_d=None, nrows=None) -> Gradients:
+def gradients_device(pos_d, vel_d, acc_d, mass_d, h_d, nb, kernel_name, rows_d=None, nrows=None,
+                     precision="fp64") -> Gradients:  # fmt: skip
     n = int(pos_d.shape[0])
     nrows = n if nrows is None else nrows
     kid = KERNEL_IDS[kernel_name]
     div, curl, ddiv = (_dev.empty(nrows) for _ in range(3))
-    _hip.call("nb_gradients", _hip.ptr(pos_d), _hip.ptr(vel_d), _hip.ptr(acc_d), _hip.ptr(mass_d), _hip.ptr(h_d),
+    entry = "nb_gradients_mixed" if PRECISIONS[precision] == 1 else "nb_gradients"
+    _hip.call(entry, _hip.ptr(pos_d), _hip.ptr(vel_d), _hip.ptr(acc_d), _hip.ptr(mass_d), _hip.ptr(h_d),
               ctypes.byref(nb.desc()), _hip.box_arg(nb.box), _hip.ptr(rows_d), nrows, kid,
               float(C_NORM[kid]), _hip.ptr(div), _hip.ptr(curl), _hip.ptr(ddiv))  # fmt: skip
     return Gradients(div, curl, ddiv)
 
 
-def gradients_numba(pos, vel, acc, mass, h, nb: Neighbours, kernel_name=DEFAULT_KERNEL, rows=None) -> Gradients:
+def gradients_numba(pos, vel, acc, mass, h, nb: Neighbours, kernel_name=DEFAULT_KERNEL, rows=None,
+                    precision="fp64") -> Gradients:  # fmt: skip
     """Corrected velocity / acceleration gradients (sph.py:887-902): div v, |curl v|,
-    d(div v)/dt. ``acc`` None means zero accelerations."""
+    d(div v)/dt. ``acc`` None means zero accelerations. ``precision`` "mixed": FP32 pair
+    terms with compensated sums."""
     as_dev = _dev.is_dev(pos)
     pos_d = _dev.f64(pos)
     rows_d, nrows = _dev.rows_dev(rows, int(pos_d.shape[0]))
     g = gradients_device(pos_d, _dev.f64(vel), None if acc is None else _dev.f64(acc), _dev.f64(mass),
-                         _dev.f64(h), nb, kernel_name, rows_d, nrows)  # fmt: skip
+                         _dev.f64(h), nb, kernel_name, rows_d, nrows, precision)  # fmt: skip
     return g if as_dev else Gradients(_dev.host(g.div_v), _dev.host(g.curl_v), _dev.host(g.ddivv_dt))
 
 

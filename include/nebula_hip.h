@@ -46,7 +46,7 @@ typedef struct nb_force_params {
     int32_t use_recon;    /* velocity reconstruction (needs grad_v) */
     int32_t closeness;
     int32_t gate;         /* 0 both, 1 published, 2 min-mod (sph.py:406) */
-    int32_t precision;    /* 0 FP64; 1 mixed (FP32 pair sums, compensated) */
+    int32_t precision;    /* 0 FP64; 1 mixed (FP32 pair terms, compensated sums; FP64 with use_recon) */
     double cnorm;
     double beta;
     double alpha_u;
@@ -54,14 +54,20 @@ typedef struct nb_force_params {
     double eta_fold;
 } nb_force_params;
 
-/* A neighbour list: CSR when length == NULL, blocked rows otherwise. n_entries: CSR offset[N],
- * or 64 * row_cap * ceil(N / 64) for blocked rows (the size of per-entry scratch). */
+/* A neighbour list: CSR when length == NULL; else rows of capacity row_cap, in 64-row blocks
+ * (row_major 0: entry k of row i at ((i / 64) row_cap + k) 64 + i % 64) or row-major (row_major 1:
+ * entry k of row i at i row_cap + k; the mixed-precision step's layout, rows not sorted).
+ * n_entries: CSR offset[N], or the size of the entry array (the size of per-entry scratch).
+ * n_rows: N, the particles the lists cover (0 = not given; the mixed-precision passes need it
+ * when they run on a row subset). */
 typedef struct nb_lists {
     const int64_t *offset;
     const int32_t *index;
     const int32_t *length;
     int64_t row_cap;
     int64_t n_entries;
+    int32_t row_major;
+    int32_t n_rows;
 } nb_lists;
 
 /* ------------------------------------------------------------------ context */
@@ -130,6 +136,29 @@ int32_t nb_hydro_density(nb_ctx *ctx, void *stream, const double *pos, const dou
  * lengths, straight into blocked rows (row_cap >= the longest merged row). */
 int32_t nb_hydro_tight(nb_ctx *ctx, void *stream, const double *h_final, int32_t *out_len, int32_t *out_index,
                        int64_t row_cap, int64_t *total_out);
+
+/* ------------------------------------------------------------------ mixed-precision step passes */
+/* The density pass of coupled.step for the mixed-precision mode (open box): the same search
+ * lists as nb_hydro_density (FP64 predicate, exact), gathered by 16-lane groups into row-major
+ * rows (entry k of row i at i * cap + k) with FP32 d2; the h-rho solve sums in FP32 with
+ * compensated accumulation and takes its Newton steps in FP64. stats4 (host) = [row capacity
+ * used, rows solved from memory, search-list entries, longest merged row]. */
+int32_t nb_hydro_density_mixed(nb_ctx *ctx, void *stream, const double *pos, const double *mass, double m_uniform,
+                               const double *h_start, const double *h_search, int64_t n, int32_t kid, double cnorm,
+                               double eta, double tol, double h_max, double *h_out, double *rho, double *omega,
+                               int32_t *iters, int32_t *flags, int64_t *stats4);
+/* tighten (neighbours.py:306-323) of the last nb_hydro_density_mixed lists at h_final, into
+ * row-major rows of capacity row_cap (>= the longest merged row); pos as in the density pass.
+ * Rows are exactly tighten() of the search lists (borderline FP32 d2 re-formed in FP64). */
+int32_t nb_hydro_tight_mixed(nb_ctx *ctx, void *stream, const double *pos, const double *h_final, int32_t *out_len,
+                             int32_t *out_index, int64_t row_cap, int64_t *total_out);
+/* gradients_numba (sph.py:837-884) in mixed precision: FP32 pair terms over 16-lane row
+ * groups, compensated sums, partials combined and the 3x3 inverse taken in FP64. Arguments as
+ * nb_gradients. (The force pass takes its precision from nb_force_params.precision.) */
+int32_t nb_gradients_mixed(nb_ctx *ctx, void *stream, const double *pos, const double *vel, const double *acc,
+                           const double *mass, const double *h, const nb_lists *lists, const double *box3,
+                           const int64_t *rows, int64_t nrows, int32_t kid, double cnorm, double *div, double *curl,
+                           double *ddiv);
 
 /* ------------------------------------------------------------------ pair passes */
 /* density_numba (sph.py:300-326 -> _density_kernel sph.py:251-297): Newton-bisection h-rho

@@ -86,7 +86,9 @@ def cpu_baseline(ic, cfg, steps):
 
 
 def run_gpu(ic, cfg, steps, warmup):
-    """Times ``steps`` coupled steps after ``warmup``; returns (wall s, per-entry-point timing, state)."""
+    """Times ``steps`` coupled steps after ``warmup`` (no instrumentation in the timed region), then
+    runs ``steps`` more with the library's kernel timing on (HIP events around each main kernel on
+    its stream) and the entry-point timing. Returns (wall s, kernel times, entry-point times, state)."""
     import torch
 
     from nebula import _hip
@@ -96,13 +98,23 @@ def run_gpu(ic, cfg, steps, warmup):
     for _ in range(warmup):
         st = coupled.step(st, cfg, coupled.stable_dt(st, cfg))
     torch.cuda.synchronize()
-    _hip.start_timing()
     t0 = time.perf_counter()
     for _ in range(steps):
         st = coupled.step(st, cfg, coupled.stable_dt(st, cfg))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    return wall, _hip.stop_timing(), st
+    _hip.start_kernel_timing()
+    _hip.start_timing()
+    for _ in range(steps):
+        st = coupled.step(st, cfg, coupled.stable_dt(st, cfg))
+    torch.cuda.synchronize()
+    entry = _hip.stop_timing()
+    kern = _hip.stop_kernel_timing()
+    return wall, kern, entry, st
+
+
+def _per_step(d, steps):
+    return {k: round(v[1] / steps, 4) for k, v in sorted(d.items(), key=lambda kv: -kv[1][1])}
 
 
 def main():
@@ -129,7 +141,7 @@ def main():
     n = args.n
     print(f"bench: {n} particles, {args.precision}, warmup {args.warmup}, steps {args.steps}", file=sys.stderr,
           flush=True)  # fmt: skip
-    wall, kern, st = run_gpu(ic, cfg, args.steps, args.warmup)  # name -> (launches, total ms)
+    wall, kern, entry, st = run_gpu(ic, cfg, args.steps, args.warmup)
     value = n * args.steps / wall
     print(f"bench: {value:.4g} particle-steps/s", file=sys.stderr, flush=True)
     # algorithmic work of the pair kernels, from this run's own lists and counters
@@ -137,20 +149,28 @@ def main():
     n_pairs = (st.n_tight_entries - n) / n
     k_it = float(st.iterations.double().mean().item())
     dens = (FLOPS_SEARCH_ENTRY * n_search + FLOPS_NEWTON_ENTRY * (k_it + 1) * n_search) * n
-    flops = {
-        "nb_forces": FLOPS_FORCE_PAIR * n_pairs * n,
-        "nb_gradients": FLOPS_GRAD_PAIR * n_pairs * n,
-        "nb_gradients_mixed": FLOPS_GRAD_PAIR * n_pairs * n,
-        "nb_hydro_density": dens,
-        "nb_hydro_density_mixed": dens,
+    fp64 = args.precision == "fp64"
+    # kernel -> (bound, work per launch, unit)
+    models = {
+        "k_gather_solve": ("mfma", dens, "TFLOP/s"),
+        "k_gather_solve_mixed": ("mfma", dens, "TFLOP/s"),
+        "k_forces64": ("mfma", FLOPS_FORCE_PAIR * n_pairs * n, "TFLOP/s"),
+        "k_forces_mixed": ("mfma", FLOPS_FORCE_PAIR * n_pairs * n, "TFLOP/s"),
+        "k_gradients64": ("mfma", FLOPS_GRAD_PAIR * n_pairs * n, "TFLOP/s"),
+        "k_gradients_mixed": ("mfma", FLOPS_GRAD_PAIR * n_pairs * n, "TFLOP/s"),
+        # tightening: read each search entry (index, d2) and the neighbour's h, write the kept index
+        "k_tight_emit": ("hbm", (4 + 8 + 8) * n_search * n + 4 * (n_pairs + 1) * n, "GB/s"),
+        "k_tight_emit_rm": ("hbm", (4 + 4 + 8) * n_search * n + 4 * (n_pairs + 1) * n, "GB/s"),
     }
-    pair = {k: v for k, v in kern.items() if k in flops}
-    dom = max(pair, key=lambda k: pair[k][1] / pair[k][0])
-    launches, total_ms = pair[dom]
+    timed = {k: v for k, v in kern.items() if k in models}
+    dom = max(timed, key=lambda k: timed[k][1])
+    launches, total_ms = timed[dom]
     avg_s = total_ms / launches / 1e3
-    achieved = flops[dom] / avg_s / 1e12
-    peak = PEAK["fp64"] if args.precision == "fp64" else PEAK["fp32"]
-    traffic = _traffic(dom, args.precision)
+    bound, work, unit = models[dom]
+    if bound == "hbm":
+        achieved, peak = work / avg_s / 1e9, HBM_PEAK
+    else:
+        achieved, peak = work / avg_s / 1e12, PEAK["fp64"] if fp64 else PEAK["fp32"]
     out = {
         "metric": "SPH particle-steps/s at 1M particles (plain FP64 vs mixed+learned-h; % FP32 peak)",
         "value": value,
@@ -162,28 +182,30 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "dtype": "f64" if fp64 else "f32+f64",
         "data": "synthetic (seeded stream-crash initial conditions)",
         "config": {"workload": "stream crash, 2 x 524,288-particle streams (1,048,576)" if n == 1_048_576 else
                    f"stream crash, {n} particles", "precision": args.precision, "kernel": "cubic, eta 1.2",
                    "viscosity": "Monaghan alpha 1 beta 2 + Balsara", "cooling": "power law rho^2 T^0.5"},
-        "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": traffic,
-                     "note": "vector-ALU pair arithmetic, priced against MI355X's dense FP64 (78.6) or FP32 "
-                             "(157.3) TFLOP/s, which the vector and matrix units share",
-                     "work": f"{flops[dom] / n:.0f} FLOP/particle-step (n_s {n_search:.1f}, pairs {n_pairs:.1f}, "
-                             f"Newton iterations {k_it:.2f})"},
-        "kernels_ms_per_step": {k: v[1] / args.steps for k, v in sorted(kern.items(), key=lambda kv: -kv[1][1])},
+        "roofline": {"bound": bound, "kernel": dom, "achieved": achieved, "peak": peak, "unit": unit,
+                     "frac": achieved / peak, "traffic": _traffic(dom, args.precision),
+                     "avg_launch_ms": avg_s * 1e3,
+                     "note": "compute-bound pair arithmetic runs on the vector ALUs and is priced against "
+                             "MI355X's dense FP64 (78.6) or FP32 (157.3) TFLOP/s, which the vector and matrix "
+                             "units share",
+                     "work": f"{work / n:.0f} {'FLOP' if bound != 'hbm' else 'B'}/particle per launch "
+                             f"(n_s {n_search:.1f}, pairs {n_pairs:.1f}, Newton iterations {k_it:.2f})"},
+        "kernels_ms_per_step": _per_step(kern, args.steps),
+        "entry_points_ms_per_step": _per_step(entry, args.steps),
     }
     if not args.no_compare:
         other = "fp64" if args.precision == "mixed" else "mixed"
         print(f"bench: {other} alongside", file=sys.stderr, flush=True)
         del st
-        w2, k2, _ = run_gpu(ic, config(other), args.steps, args.warmup)
-        out["compare"] = {other: {"value": n * args.steps / w2, "ms_per_step": w2 / args.steps * 1e3,
-                                  "speedup_of_value": (n * args.steps / wall) / (n * args.steps / w2),
-                                  "kernels_ms_per_step": {k: v[1] / args.steps for k, v in
-                                                          sorted(k2.items(), key=lambda kv: -kv[1][1])}}}  # fmt: skip
+        w2, k2, _, _ = run_gpu(ic, config(other), args.steps, args.warmup)
+        v2 = n * args.steps / w2
+        out["compare"] = {other: {"value": v2, "ms_per_step": w2 / args.steps * 1e3, "value_ratio": value / v2,
+                                  "kernels_ms_per_step": _per_step(k2, args.steps)}}  # fmt: skip
     if not args.no_cpu:
         print("bench: CPU baseline (FP64 oracle, OpenMP)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(ic, cfg, args.cpu_steps)

@@ -151,18 +151,66 @@ __device__ __forceinline__ int32_t cell_of(double x, const Grid& g, int c)
     return (int32_t)f;
 }
 
-__global__ void k_cell_id(const double* __restrict__ pos, int64_t n, Box box, Grid g, uint32_t* __restrict__ cell_id,
-                          uint32_t* __restrict__ iota, uint32_t* __restrict__ cell_count)
+__device__ __forceinline__ uint32_t cell_key(const double* pos, int64_t i, const Box& box, const Grid& g)
 {
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int32_t cx = cell_of(wrapped(pos, i, 0, box), g, 0);
-    int32_t cy = cell_of(wrapped(pos, i, 1, box), g, 1);
-    int32_t cz = cell_of(wrapped(pos, i, 2, box), g, 2);
-    uint32_t c = ((uint32_t)cx * (uint32_t)g.nc[1] + (uint32_t)cy) * (uint32_t)g.nc[2] + (uint32_t)cz;
-    cell_id[i] = c;
-    iota[i] = (uint32_t)i;
-    atomicAdd(&cell_count[c], 1u);
+    const int32_t cx = cell_of(wrapped(pos, i, 0, box), g, 0);
+    const int32_t cy = cell_of(wrapped(pos, i, 1, box), g, 1);
+    const int32_t cz = cell_of(wrapped(pos, i, 2, box), g, 2);
+    return ((uint32_t)cx * (uint32_t)g.nc[1] + (uint32_t)cy) * (uint32_t)g.nc[2] + (uint32_t)cz;
+}
+
+// cell of each particle, particles per cell, and (part, one partial per workgroup) the number of
+// neighbouring pairs i, i + 1 out of cell order: zero when the particles are stored sorted.
+__global__ void __launch_bounds__(kBlock)
+k_cell_id(const double* __restrict__ pos, int64_t n, Box box, Grid g, uint32_t* __restrict__ cell_id,
+          uint32_t* __restrict__ iota, uint32_t* __restrict__ cell_count, long long* __restrict__ part)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    long long bad = 0;
+    if (i < n) {
+        const uint32_t c = cell_key(pos, i, box, g);
+        cell_id[i] = c;
+        iota[i] = (uint32_t)i;
+        atomicAdd(&cell_count[c], 1u);
+        if (part && i + 1 < n && cell_key(pos, i + 1, box, g) < c) bad = 1;
+    }
+    if (part) block_partial<false>(bad, part);
+}
+
+// Sum (bit s of maxmask clear) or max (set) of the per-workgroup partials of nslots slots
+// (slot s at part[s * nb + b]) into out[s]; one workgroup.
+__global__ void __launch_bounds__(kBlock)
+k_partials_final(const long long* __restrict__ part, int64_t nb, int nslots, unsigned maxmask, long long* out)
+{
+    __shared__ long long sh[kBlock];
+    for (int sl = 0; sl < nslots; sl++) {
+        const bool mx = (maxmask >> sl) & 1u;
+        long long v = 0;
+        for (int64_t b = threadIdx.x; b < nb; b += blockDim.x) {
+            const long long t = part[sl * nb + b];
+            v = mx ? (t > v ? t : v) : v + t;
+        }
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+            if ((int)threadIdx.x < k) {
+                const long long t = sh[threadIdx.x + k];
+                sh[threadIdx.x] = mx ? (t > sh[threadIdx.x] ? t : sh[threadIdx.x]) : sh[threadIdx.x] + t;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[sl] = sh[0];
+        __syncthreads();
+    }
+}
+
+// out[k] = src[k] for a few device scalars (collects values for one host copy)
+__global__ void k_collect(long long* out, const unsigned long long* a, const unsigned long long* b)
+{
+    if (threadIdx.x == 0) {
+        if (a) out[0] = (long long)*a;
+        if (b) out[1] = (long long)*b;
+    }
 }
 
 // neighbours.py:129-145: first cell and count along one axis; periodic searches that span the
@@ -199,15 +247,16 @@ __device__ __forceinline__ int32_t wrap_cell(int64_t raw, int32_t nc)
 // One work-item per particle; row i (blocked, capacity cap) holds {j : d2 < (2 h_i)^2} in
 // traversal order; entries carry kFlag when d2 >= (2 h_j)^2 (pair unseen from j's side) and
 // ecount[j] counts those. rowcnt[i] is the true count even past cap (then the caller rebuilds
-// with a larger cap; scal[0] holds the largest). Rows that come out unsorted are marked.
+// with a larger cap; the longest row goes to the workgroup partials). Rows that come out
+// unsorted are marked and counted. order NULL: particles stored in cell order.
 __global__ void __launch_bounds__(kBlock)
 k_gather(const double* __restrict__ pos, const double* __restrict__ h, int64_t n, Box box, Grid g,
          const uint32_t* __restrict__ start, const uint32_t* __restrict__ order, uint32_t* __restrict__ gbuf,
          int64_t cap, int32_t* __restrict__ rowcnt, uint8_t* __restrict__ unsorted,
-         unsigned long long* __restrict__ ecount, int32_t* __restrict__ scal)
+         unsigned long long* __restrict__ ecount, long long* __restrict__ part)
 {
-    // lanes past the end redo the last particle without writing (full waves for the reductions)
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    // lanes past the end redo the last particle without writing (full blocks for the reductions)
+    int64_t i = xcd_tile() * (int64_t)blockDim.x + threadIdx.x;
     const bool live = i < n;
     if (!live) i = n - 1;
     const double r = NB_R_KERN * h[i];
@@ -230,7 +279,7 @@ k_gather(const double* __restrict__ pos, const double* __restrict__ h, int64_t n
                 const uint32_t c = base + (uint32_t)wrap_cell(az + dk, g.nc[2]);
                 const uint32_t k1 = start[c + 1];
                 for (uint32_t k = start[c]; k < k1; k++) {
-                    const uint32_t j = order[k];
+                    const uint32_t j = order ? order[k] : k;
                     const double dx = min_image(wrapped(pos, j, 0, box) - xi, box.l[0]);
                     const double dy = min_image(wrapped(pos, j, 1, box) - yi, box.l[1]);
                     const double dz = min_image(wrapped(pos, j, 2, box) - zi, box.l[2]);
@@ -255,12 +304,8 @@ k_gather(const double* __restrict__ pos, const double* __restrict__ h, int64_t n
         rowcnt[i] = m;
         unsorted[i] = sorted ? 0 : 1;
     }
-    const int wmax = wave_max_i32(m);
-    const unsigned long long wuns = wave_sum_u64(live && !sorted ? 1ull : 0ull);
-    if ((threadIdx.x & 63) == 0) {
-        atomicMax(&scal[0], wmax);
-        if (wuns) atomicAdd(&scal[1], (int)wuns);
-    }
+    block_partial<true>(m, part);
+    block_partial<false>(live && !sorted ? 1 : 0, part + gridDim.x);
 }
 
 // Insertion sort of the rows marked unsorted (flag bit ignored in the key).
@@ -301,21 +346,20 @@ __global__ void k_extras_fill(const uint32_t* __restrict__ gbuf, int64_t cap, co
     }
 }
 
-// final row lengths (gather + extras), their largest value (scal[2]) and, for the CSR scan,
+// final row lengths (gather + extras), their largest value (workgroup partials) and, for the CSR scan,
 // tot[n] = 0.
-__global__ void k_row_total(const int32_t* __restrict__ rowcnt, const unsigned long long* __restrict__ ecount,
-                            unsigned long long* __restrict__ tot, int32_t* __restrict__ scal, int64_t n)
+__global__ void __launch_bounds__(kBlock)
+k_row_total(const int32_t* __restrict__ rowcnt, const unsigned long long* __restrict__ ecount,
+            unsigned long long* __restrict__ tot, long long* __restrict__ part, int64_t n)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    int32_t t32 = 0;
+    long long t = 0;
     if (i < n) {
-        unsigned long long t = (unsigned long long)rowcnt[i] + ecount[i];
-        tot[i] = t;
-        t32 = (int32_t)t;
+        t = (long long)rowcnt[i] + (long long)ecount[i];
+        tot[i] = (unsigned long long)t;
     }
     if (i == n) tot[i] = 0;
-    const int wm = wave_max_i32(t32);
-    if ((threadIdx.x & 63) == 0 && wm > 0) atomicMax(&scal[2], wm);
+    block_partial<true>(t, part);
 }
 
 // Sorted extras of row i (sorted here, in place), merged with the sorted gather row. The sets are
@@ -396,9 +440,9 @@ __global__ void k_tighten_csr(const double* __restrict__ pos, const double* __re
 __global__ void __launch_bounds__(kBlock)
 k_tighten_blk(const double* __restrict__ pos, const double* __restrict__ h, Box box, const int32_t* __restrict__ in_len,
               const int32_t* __restrict__ in_idx, int64_t cap, int64_t n, int32_t* __restrict__ out_len,
-              int32_t* __restrict__ out_idx, unsigned long long* __restrict__ total)
+              int32_t* __restrict__ out_idx, long long* __restrict__ part)
 {
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t i = xcd_tile() * (int64_t)blockDim.x + threadIdx.x;
     int32_t m = 0;
     if (i < n) {
         const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
@@ -423,14 +467,7 @@ k_tighten_blk(const double* __restrict__ pos, const double* __restrict__ h, Box 
         }
         out_len[i] = m;
     }
-    __shared__ unsigned long long sh[kBlock];
-    sh[threadIdx.x] = (unsigned long long)m;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicAdd(total, sh[0]);
+    block_partial<false>(m, part);
 }
 
 __global__ void k_fill_u64(unsigned long long* p, int64_t n, unsigned long long v)
@@ -480,7 +517,7 @@ struct DensArgs {
     int32_t* iters;
     int32_t* flags;
     uint8_t* pending;  // 1: row not solved yet
-    int32_t* scal;     // scal[3] counts rows left pending
+    long long* part;   // per-workgroup partials: longest row, unsorted rows, rows left pending
 };
 
 // sph.py:215-228 on the cached d2 of a blocked row: FP64 (exact order) or FP32 with
@@ -613,8 +650,8 @@ k_gather_solve(const double* __restrict__ pos, const double* __restrict__ h, int
                double* __restrict__ dbuf, double* __restrict__ mbuf, int64_t cap, int32_t* __restrict__ rowcnt,
                uint8_t* __restrict__ unsorted, unsigned long long* __restrict__ ecount, DensArgs A)
 {
-    // lanes past the end redo the last particle without writing (full waves for the reductions)
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    // lanes past the end redo the last particle without writing (full blocks for the reductions)
+    int64_t i = xcd_tile() * (int64_t)blockDim.x + threadIdx.x;
     const bool live = i < n;
     if (!live) i = n - 1;
     const double hs = h[i];
@@ -637,7 +674,7 @@ k_gather_solve(const double* __restrict__ pos, const double* __restrict__ h, int
             // the z-run of cells is one contiguous range of the cell-sorted particles
             const uint32_t k0 = start[c0], k1 = start[c0 + (uint32_t)nz];
             for (uint32_t k = k0; k < k1; k++) {
-                const uint32_t j = order[k];
+                const uint32_t j = order ? order[k] : k;
                 const double dx = pos[3 * j] - xi;
                 const double dy = pos[3 * j + 1] - yi;
                 const double dz = pos[3 * j + 2] - zi;
@@ -662,14 +699,9 @@ k_gather_solve(const double* __restrict__ pos, const double* __restrict__ h, int
         }
     }
     const bool done_here = sorted && m <= cap;
-    const int wmax = wave_max_i32(m);
-    const unsigned long long wuns = wave_sum_u64(live && !sorted ? 1ull : 0ull);
-    const unsigned long long wpend = wave_sum_u64(live && !done_here ? 1ull : 0ull);
-    if ((threadIdx.x & 63) == 0) {
-        atomicMax(&A.scal[0], wmax);
-        if (wuns) atomicAdd(&A.scal[1], (int)wuns);
-        if (wpend) atomicAdd(&A.scal[3], (int)wpend);
-    }
+    block_partial<true>(m, A.part);
+    block_partial<false>(live && !sorted ? 1 : 0, A.part + gridDim.x);
+    block_partial<false>(live && !done_here ? 1 : 0, A.part + 2 * gridDim.x);
     if (!live) return;
     rowcnt[i] = m;
     unsorted[i] = sorted ? 0 : 1;
@@ -741,9 +773,9 @@ __global__ void __launch_bounds__(kBlock)
 k_tight_emit(const uint32_t* __restrict__ gbuf, const double* __restrict__ dbuf, int64_t cap,
              const int32_t* __restrict__ rowcnt, const unsigned long long* __restrict__ eoff, uint32_t* __restrict__ ebuf,
              double* __restrict__ edbuf, const double* __restrict__ hf, int64_t n, int32_t* __restrict__ out_len,
-             int32_t* __restrict__ out_idx, int64_t cap_out, unsigned long long* __restrict__ total)
+             int32_t* __restrict__ out_idx, int64_t cap_out, long long* __restrict__ part)
 {
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t i = xcd_tile() * (int64_t)blockDim.x + threadIdx.x;
     int32_t o = 0;
     if (i < n) {
         uint32_t* e = ebuf + eoff[i];
@@ -787,29 +819,27 @@ k_tight_emit(const uint32_t* __restrict__ gbuf, const double* __restrict__ dbuf,
         }
         out_len[i] = o;
     }
-    __shared__ unsigned long long sh[kBlock];
-    sh[threadIdx.x] = (unsigned long long)o;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicAdd(total, sh[0]);
+    block_partial<false>(o, part);
 }
 
 // ----------------------------------------------------------------- mixed-precision step passes
-// Lane groups: kG consecutive lanes serve one particle. The candidates of a cell column are a
-// contiguous range of the cell-sorted particles, so the group reads kG consecutive candidates at
-// a time (coalesced), tests the FP64 list predicate (the lists stay exact), and compacts the
-// accepted ones with a ballot: lane l's slot is the number of accepting lanes below it. Rows are
-// stored row-major (entry k of row i at i * cap + k), so a group's writes are contiguous too.
-// Each lane keeps the r and mass of the entries it accepted in registers (up to kStash; a
-// row with more is solved from memory), so the h-rho Newton iterations re-read nothing: each
-// lane sums its own entries in FP32 with compensated accumulation, the group combines the
-// partials in FP64, and every lane of the group takes the same Newton step (the update and the
-// convergence test in FP64).
+// Lane groups: kG consecutive lanes serve one particle. The search sphere is cut into grid
+// columns (x-y cells); in each column only the z-cells the sphere reaches at that column's x-y
+// distance are visited, and that z-run is one contiguous range of the cell-sorted particles.
+// The group lays the ranges of a chunk of kCols columns end to end and walks the concatenation
+// kG candidates at a time (lane l takes candidate l of each batch; a binary search over the
+// chunk's prefix sums, held in the lanes' registers, finds its column), so every lane tests a
+// candidate on every pass whatever the column lengths. Candidate positions and search lengths
+// come in one 32-byte record. The FP64 list predicate keeps the lists exact; accepted entries
+// are compacted with a ballot (lane l's slot is the number of accepting lanes below it) into
+// row-major rows (entry k of row i at i * cap + k) with their FP32 d2 (and mass).
+// The first evaluation of the h-rho Newton solve (sph.py:251-297, at the starting h) is summed
+// on the fly, in FP32 with compensated accumulation per lane, combined in FP64 by the group,
+// and the group takes the Newton step in FP64. Rows converged after that step (the common case)
+// get rho and Omega at their final h from the emission pass, which reads the row anyway; the
+// others are solved in full by k_solve_slow_mixed.
 static constexpr int kG = 16;
-static constexpr int kStash = 24;
+static constexpr int kCols = 2 * kG;  // columns per chunk: two per lane
 
 __device__ __forceinline__ void kernel_fdf32(float q, int kid, float& fq, float& dfq)
 {
@@ -835,6 +865,31 @@ __device__ __forceinline__ double group_sum(double v)
     return v;
 }
 
+// compensated FP32 sums of m f(q) and m (3 f + q f') (the two density-solve sums)
+struct DensSums {
+    float a1 = 0.f, c1 = 0.f, a2 = 0.f, c2 = 0.f;
+    __device__ __forceinline__ void add(float q, float mk, int kid)
+    {
+        if (!(q < 2.f)) return;
+        float fq, dfq;
+        kernel_fdf32(q, kid, fq, dfq);
+        const float t1 = mk * fq - c1;
+        const float u1 = a1 + t1;
+        c1 = (u1 - a1) - t1;
+        a1 = u1;
+        const float t2 = mk * (3.f * fq + q * dfq) - c2;
+        const float u2 = a2 + t2;
+        c2 = (u2 - a2) - t2;
+        a2 = u2;
+    }
+    // group totals (FP64), identical on every lane of the group
+    __device__ __forceinline__ void total(double& s1, double& s2) const
+    {
+        s1 = group_sum((double)a1 - (double)c1);
+        s2 = group_sum((double)a2 - (double)c2);
+    }
+};
+
 struct MixArgs {
     const double* mass;
     double m_uniform;
@@ -847,160 +902,208 @@ struct MixArgs {
     double* omega;
     int32_t* iters;
     int32_t* flags;
-    int32_t* scal;  // [0] longest row, [1] rows solved from memory
+    uint8_t* slow;    // 1: row solved by k_solve_slow_mixed (not converged in one Newton step)
+    long long* part;  // per-workgroup partials: longest row, slow rows
 };
 
-// sums of m f(q) and m (3 f + q f') over the row, at inverse length inv_h: from the register
-// stash, or (overflow) from the row in memory read by the whole group.
-__device__ __forceinline__ void mix_sums(const float* rs_r, const float* rs_m, int nst, bool from_mem,
-                                         const float* rd2, const float* rmass, int32_t m, int glane, float rmax2f,
-                                         float inv_hf, int kid, float mu, double& s1, double& s2)
+// (x, y, z, h_search) records of the candidates
+__global__ void k_pack_posh(const double* __restrict__ pos, const double* __restrict__ h, int64_t n,
+                            double4* __restrict__ out)
 {
-    float a1 = 0.f, c1 = 0.f, a2 = 0.f, c2 = 0.f;
-    auto add = [&](float r, float mk) {
-        const float q = r * inv_hf;
-        if (q < 2.f) {
-            float fq, dfq;
-            kernel_fdf32(q, kid, fq, dfq);
-            const float t1 = mk * fq - c1;
-            const float u1 = a1 + t1;
-            c1 = (u1 - a1) - t1;
-            a1 = u1;
-            const float t2 = mk * (3.f * fq + q * dfq) - c2;
-            const float u2 = a2 + t2;
-            c2 = (u2 - a2) - t2;
-            a2 = u2;
-        }
-    };
-    if (!from_mem) {
-#pragma unroll
-        for (int k = 0; k < kStash; k++)
-            if (k < nst) add(rs_r[k], rs_m ? rs_m[k] : mu);
-    } else {
-        for (int32_t k = glane; k < m; k += kG) {
-            const float d2 = rd2[k];
-            if (d2 < rmax2f) add(__fsqrt_rn(d2), rmass ? rmass[k] : mu);
-        }
-    }
-    s1 = group_sum((double)a1 - (double)c1);
-    s2 = group_sum((double)a2 - (double)c2);
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) out[i] = make_double4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], h[i]);
+}
+
+// The z-cells [z0, z1] that a sphere (centre at cell coordinates ux, uy and height zc, radius^2
+// r2) reaches in grid column (cx, cy) of an open grid; false when it misses the column. The x-y
+// distance to the column is shortened by 1e-9 of a cell and the z half-extent lengthened by as
+// much, so rounding can only widen the range (extra candidates fail the exact predicate).
+__device__ __forceinline__ bool column_zrange(double ux, double uy, double zc, double r2, int64_t cx, int64_t cy,
+                                              const Grid& g, int64_t& z0, int64_t& z1)
+{
+    const double cell = 1.0 / g.inv[0];
+    const double ex = fmax(fmax((double)cx - ux, ux - (double)(cx + 1)) - 1e-9, 0.0) * cell;
+    const double ey = fmax(fmax((double)cy - uy, uy - (double)(cy + 1)) - 1e-9, 0.0) * cell;
+    const double rest = r2 - (ex * ex + ey * ey);
+    if (!(rest > 0.0)) return false;
+    const double hz = sqrt(rest) + 1e-9 * cell;
+    const double fa = fmax(floor((zc - hz - g.lo[2]) * g.inv[2]), 0.0);
+    const double fb = fmin(floor((zc + hz - g.lo[2]) * g.inv[2]), (double)(g.nc[2] - 1));
+    if (fb < fa) return false;
+    z0 = (int64_t)fa;
+    z1 = (int64_t)fb;
+    return true;
 }
 
 template <bool kMasses>
 __global__ void __launch_bounds__(256)
-k_gather_solve_mixed(const double* __restrict__ pos, const double* __restrict__ h, int64_t n, Grid g,
-                     const uint32_t* __restrict__ start, const uint32_t* __restrict__ order,
-                     uint32_t* __restrict__ ridx, float* __restrict__ rd2, float* __restrict__ rmass, int64_t cap,
-                     int32_t* __restrict__ rowcnt, unsigned long long* __restrict__ ecount, MixArgs A)
+k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g, const uint32_t* __restrict__ start,
+               const uint32_t* __restrict__ order, uint32_t* __restrict__ ridx, float* __restrict__ rd2,
+               float* __restrict__ rmass, int64_t cap, int32_t* __restrict__ rowcnt,
+               unsigned long long* __restrict__ ecount, MixArgs A)
 {
     const int glane = threadIdx.x & (kG - 1);
     const int gshift = (threadIdx.x & 63) & ~(kG - 1);
-    const int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kG;
+    const int64_t i = (xcd_tile() * (int64_t)blockDim.x + threadIdx.x) / kG;
     const bool live = i < n;
     const int64_t ii = live ? i : n - 1;
-    const double hs = h[ii];
+    const double4 pi = posh[ii];
+    const double xi = pi.x, yi = pi.y, zi = pi.z, hs = pi.w;
     const double r = NB_R_KERN * hs;
     const double r2 = r * r;
-    const double xi = pos[3 * ii], yi = pos[3 * ii + 1], zi = pos[3 * ii + 2];
-    int64_t ax, nx, ay, ny, az, nz;
+    int64_t ax, nx, ay, ny;
     cell_range(xi, r, g, 0, false, ax, nx);
     cell_range(yi, r, g, 1, false, ay, ny);
-    cell_range(zi, r, g, 2, false, az, nz);
-    if (!live) nx = 0;
+    const int64_t ncol = live ? nx * ny : 0;
+    const double ux = (xi - g.lo[0]) * g.inv[0], uy = (yi - g.lo[1]) * g.inv[1];
+    const uint32_t nc1 = (uint32_t)g.nc[1], nc2 = (uint32_t)g.nc[2];
     const double r_lim = pymin(hs, A.h_max);
-    const float rmax2f = (float)((2.0 * r_lim) * (2.0 * r_lim));
-    float st_r[kStash], st_m[kStash];
-    int nst = 0;
-    bool over = false;
+    const double h0 = pymin(A.h_start[ii], r_lim);
+    const float inv_h0 = (float)(1.0 / h0);
+    const float mu = (float)(A.m_uniform > 0.0 ? A.m_uniform : 1.0);
+    DensSums sums;
     int32_t m = 0;
     uint32_t* row = ridx + ii * cap;
     float* drow = rd2 + ii * cap;
     float* mrow = kMasses ? rmass + ii * cap : nullptr;
-    // all groups of the wave run the same number of ballots: loop bounds are made uniform by
-    // letting finished groups idle with pred = false
-    for (int64_t di = 0;; di++) {
-        bool more = di < nx;
-        if (__ballot(more) == 0) break;
-        for (int64_t dj = 0;; dj++) {
-            bool more2 = more && dj < ny;
-            if (__ballot(more2) == 0) break;
-            uint32_t k0 = 0, k1 = 0;
-            if (more2) {
-                const uint32_t c0 = ((uint32_t)(ax + di) * (uint32_t)g.nc[1] + (uint32_t)(ay + dj)) * (uint32_t)g.nc[2] +
-                                    (uint32_t)az;
-                k0 = start[c0];
-                k1 = start[c0 + (uint32_t)nz];
-            }
-            for (uint32_t kb = k0;; kb += kG) {
-                const bool in_range = more2 && kb < k1;
-                if (__ballot(in_range) == 0) break;
-                const uint32_t k = kb + (uint32_t)glane;
-                bool pred = false;
-                uint32_t j = 0;
-                double d2 = 0.0;
-                if (in_range && k < k1) {
-                    j = order[k];
-                    const double dx = pos[3 * j] - xi;
-                    const double dy = pos[3 * j + 1] - yi;
-                    const double dz = pos[3 * j + 2] - zi;
-                    d2 = dx * dx + dy * dy + dz * dz;
-                    pred = d2 < r2;
-                }
-                const unsigned long long bal = __ballot(pred);
-                const unsigned gb = (unsigned)((bal >> gshift) & ((1ull << kG) - 1));
-                if (pred) {
-                    const int slot = m + __popc(gb & ((1u << glane) - 1));
-                    const double rj = NB_R_KERN * h[j];
-                    uint32_t e = j;
-                    if (!(d2 < rj * rj)) {
-                        e |= kFlag;
-                        atomicAdd(&ecount[j], 1ull);
-                    }
-                    const float mj = kMasses ? (float)A.mass[j] : 0.f;
-                    if (slot < cap) {
-                        row[slot] = e;
-                        drow[slot] = (float)d2;
-                        if (kMasses) mrow[slot] = mj;
-                    }
-                    if ((float)d2 < rmax2f) {
-                        if (nst < kStash) {
-                            st_r[nst] = __fsqrt_rn((float)d2);
-                            st_m[nst] = mj;
-                            nst++;
-                        } else {
-                            over = true;
-                        }
-                    }
-                }
-                m += __popc(gb);
+    // every loop below runs the same number of times on all lanes of the wave (ballot bounds),
+    // so the shuffles always see their whole group active
+    for (int64_t c0 = 0;; c0 += kCols) {
+        if (__ballot(c0 < ncol) == 0) break;
+        uint32_t k0a = 0, lena = 0, k0b = 0, lenb = 0;
+        int64_t z0, z1;
+        const int64_t ca = c0 + glane, cb = c0 + kG + glane;
+        if (ca < ncol && column_zrange(ux, uy, zi, r2, ax + ca / ny, ay + ca % ny, g, z0, z1)) {
+            const uint32_t base = ((uint32_t)(ax + ca / ny) * nc1 + (uint32_t)(ay + ca % ny)) * nc2;
+            k0a = start[base + (uint32_t)z0];
+            lena = start[base + (uint32_t)z1 + 1u] - k0a;
+        }
+        if (cb < ncol && column_zrange(ux, uy, zi, r2, ax + cb / ny, ay + cb % ny, g, z0, z1)) {
+            const uint32_t base = ((uint32_t)(ax + cb / ny) * nc1 + (uint32_t)(ay + cb % ny)) * nc2;
+            k0b = start[base + (uint32_t)z0];
+            lenb = start[base + (uint32_t)z1 + 1u] - k0b;
+        }
+        uint32_t ia = lena, ib = lenb;  // inclusive prefix sums over the group
+        for (int o = 1; o < kG; o <<= 1) {
+            const uint32_t ta = __shfl_up(ia, o, kG), tb = __shfl_up(ib, o, kG);
+            if (glane >= o) {
+                ia += ta;
+                ib += tb;
             }
         }
+        const uint32_t tota = __shfl(ia, kG - 1, kG);
+        const uint32_t total = tota + __shfl(ib, kG - 1, kG);
+        const uint32_t pa = ia - lena, pb = tota + ib - lenb;  // exclusive prefixes of both columns
+        for (uint32_t vb = 0;; vb += kG) {
+            if (__ballot(vb < total) == 0) break;
+            const uint32_t v = vb + (uint32_t)glane;
+            const uint32_t vs = total > 0 ? (v < total ? v : total - 1) : 0;
+            // the chunk column holding candidate vs: the last one whose prefix is <= vs
+            int lo = 0;
+            for (int step = kCols / 2; step > 0; step >>= 1) {
+                const int mid = lo + step;
+                const uint32_t qa = __shfl(pa, mid & (kG - 1), kG), qb = __shfl(pb, mid & (kG - 1), kG);
+                if ((mid < kG ? qa : qb) <= vs) lo = mid;
+            }
+            const uint32_t qa = __shfl(pa, lo & (kG - 1), kG), qb = __shfl(pb, lo & (kG - 1), kG);
+            const uint32_t ka = __shfl(k0a, lo & (kG - 1), kG), kb = __shfl(k0b, lo & (kG - 1), kG);
+            bool pred = false, flag = false;
+            uint32_t j = 0;
+            double d2 = 0.0;
+            float mj = 0.f;
+            if (v < total) {
+                const uint32_t k = (lo < kG ? ka : kb) + (v - (lo < kG ? qa : qb));
+                j = order ? order[k] : k;
+                const double4 pj = posh[j];
+                if (kMasses) mj = (float)A.mass[j];
+                const double dx = pj.x - xi;
+                const double dy = pj.y - yi;
+                const double dz = pj.z - zi;
+                d2 = dx * dx + dy * dy + dz * dz;
+                pred = d2 < r2;
+                const double rj = NB_R_KERN * pj.w;
+                flag = pred && !(d2 < rj * rj);
+            }
+            const unsigned long long bal = __ballot(pred);
+            const unsigned gb = (unsigned)((bal >> gshift) & ((1ull << kG) - 1));
+            if (pred) {
+                const int slot = m + __popc(gb & ((1u << glane) - 1));
+                if (flag) atomicAdd(&ecount[j], 1ull);
+                const float d2f = (float)d2;
+                if (slot < cap) {
+                    row[slot] = flag ? (j | kFlag) : j;
+                    drow[slot] = d2f;
+                    if (kMasses) mrow[slot] = mj;
+                }
+                sums.add(__fsqrt_rn(d2f) * inv_h0, kMasses ? mj : mu, A.kid);
+            }
+            m += __popc(gb);
+        }
     }
-    // a group overflows together: any lane past the stash, or the row past its capacity
-    const unsigned long long ob = __ballot(over);
-    const bool from_mem = ((ob >> gshift) & ((1ull << kG) - 1)) != 0;
     if (live && glane == 0) rowcnt[i] = m;
-    const int wmax = wave_max_i32(live ? m : 0);
-    const unsigned long long wmem = wave_sum_u64(live && glane == 0 && from_mem ? 1ull : 0ull);
-    if ((threadIdx.x & 63) == 0) {
-        atomicMax(&A.scal[0], wmax);
-        if (wmem) atomicAdd(&A.scal[1], (int)wmem);
-    }
-    if (m > cap) return;  // rebuilt with a larger capacity by the caller
-    // density / h solve (sph.py:251-297; sums FP32, Newton FP64)
+    // the loop's first pass at h0 (sph.py:260-290), the Newton update and convergence test in FP64
+    double S1, S2;
+    sums.total(S1, S2);
     const double m_i = A.mass[ii];
+    const double mscale = kMasses ? 1.0 : (A.m_uniform > 0.0 ? A.m_uniform / (double)mu : 1.0);
+    const double inv_h = 1.0 / h0;
+    const double rs = S1 * mscale * A.cnorm * ipow3(inv_h);
+    const double dws = -S2 * mscale * A.cnorm * ipow4(inv_h);
+    const double f = rs - m_i * ipow3(A.eta / h0);
+    const double df = dws + 3.0 * m_i * ipow3(A.eta) / ipow4(h0);
+    double lo_b = 0.0, hi_b = r_lim;
+    if (f > 0) hi_b = h0;
+    else lo_b = h0;
+    double h_new = df > 0 ? h0 - f / df : 0.5 * (lo_b + hi_b);
+    if (!(lo_b <= h_new && h_new <= hi_b)) h_new = 0.5 * (lo_b + hi_b);
+    const bool fast = m <= cap && fabs(h_new - h0) < A.tol * h0 && !(h_new >= r_lim * (1 - 1e-3));
+    block_partial<true>(live ? m : 0, A.part);
+    block_partial<false>(live && glane == 0 && !fast ? 1 : 0, A.part + gridDim.x);
+    if (live && glane == 0) {
+        A.slow[i] = fast ? 0 : 1;
+        if (fast) {
+            A.h_out[i] = h_new;
+            A.iters[i] = 1;
+            A.flags[i] = 0;
+        }
+    }
+}
+
+// the two density sums of row i at inverse length inv_h, from the stored row (group-wide)
+__device__ __forceinline__ void row_dens_sums(const float* drow, const float* mrow, int32_t m, int glane, float inv_h,
+                                              int kid, float mu, double& s1, double& s2)
+{
+    DensSums sums;
+    for (int32_t k = glane; k < m; k += kG) sums.add(__fsqrt_rn(drow[k]) * inv_h, mrow ? mrow[k] : mu, kid);
+    sums.total(s1, s2);
+}
+
+// Rows not converged in one Newton step: the whole solve of sph.py:251-297 over the stored row
+// (sums FP32 compensated, Newton steps and tests FP64), rho and Omega at the final h.
+template <bool kMasses>
+__global__ void __launch_bounds__(256)
+k_solve_slow_mixed(const double* __restrict__ h_search, int64_t n, const float* __restrict__ rd2,
+                   const float* __restrict__ rmass, int64_t cap, const int32_t* __restrict__ rowcnt, MixArgs A)
+{
+    const int glane = threadIdx.x & (kG - 1);
+    const int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kG;
+    if (i >= n || !A.slow[i]) return;  // whole groups leave together
+    const int32_t m = rowcnt[i];
+    const float* drow = rd2 + i * cap;
+    const float* mrow = kMasses ? rmass + i * cap : nullptr;
     const float mu = (float)(A.m_uniform > 0.0 ? A.m_uniform : 1.0);
     const double mscale = kMasses ? 1.0 : (A.m_uniform > 0.0 ? A.m_uniform / (double)mu : 1.0);
+    const double m_i = A.mass[i];
+    const double r_lim = pymin(h_search[i], A.h_max);
     double hi = r_lim, lo = 0.0;
-    double hh = pymin(A.h_start[ii], hi);
+    double hh = pymin(A.h_start[i], hi);
     const double h0 = hh;
     int flag = 3, it = 0;
     const double eta3 = ipow3(A.eta);
     double s1, s2;
     while (it < 50) {
         it++;
-        mix_sums(st_r, kMasses ? st_m : nullptr, nst, from_mem, drow, mrow, m, glane, rmax2f, (float)(1.0 / hh),
-                 A.kid, mu, s1, s2);
+        row_dens_sums(drow, mrow, m, glane, (float)(1.0 / hh), A.kid, mu, s1, s2);
         const double inv_h = 1.0 / hh;
         const double rs = s1 * mscale * A.cnorm * ipow3(inv_h);
         const double dws = -s2 * mscale * A.cnorm * ipow4(inv_h);
@@ -1018,20 +1121,18 @@ k_gather_solve_mixed(const double* __restrict__ pos, const double* __restrict__ 
         hh = h_new;
     }
     if (hh >= r_lim * (1 - 1e-3)) {
-        mix_sums(st_r, kMasses ? st_m : nullptr, nst, from_mem, drow, mrow, m, glane, rmax2f, (float)(1.0 / r_lim),
-                 A.kid, mu, s1, s2);
+        row_dens_sums(drow, mrow, m, glane, (float)(1.0 / r_lim), A.kid, mu, s1, s2);
         const double rs_lim = s1 * mscale * A.cnorm * ipow3(1.0 / r_lim);
         if (rs_lim - m_i * ipow3(A.eta / r_lim) < 0) {
             hh = r_lim;
             flag = r_lim >= A.h_max ? 1 : 2;
         }
     }
-    mix_sums(st_r, kMasses ? st_m : nullptr, nst, from_mem, drow, mrow, m, glane, rmax2f, (float)(1.0 / hh), A.kid,
-             mu, s1, s2);
+    row_dens_sums(drow, mrow, m, glane, (float)(1.0 / hh), A.kid, mu, s1, s2);
     const double inv_h = 1.0 / hh;
     const double rs = s1 * mscale * A.cnorm * ipow3(inv_h);
     const double dws = -s2 * mscale * A.cnorm * ipow4(inv_h);
-    if (live && glane == 0) {
+    if (glane == 0) {
         A.h_out[i] = hh;
         A.rho[i] = rs;
         A.omega[i] = flag == 1 ? 1.0 : 1.0 + hh / (3.0 * rs) * dws;
@@ -1047,7 +1148,7 @@ k_extras_fill_rm(const uint32_t* __restrict__ ridx, const float* __restrict__ rd
                  unsigned int* __restrict__ efill, uint32_t* __restrict__ ebuf, float* __restrict__ ed2, int64_t n)
 {
     const int glane = threadIdx.x & (kG - 1);
-    const int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kG;
+    const int64_t i = (xcd_tile() * (int64_t)blockDim.x + threadIdx.x) / kG;
     if (i >= n) return;
     const int32_t m = rowcnt[i];
     for (int32_t k = glane; k < m; k += kG) {
@@ -1061,16 +1162,51 @@ k_extras_fill_rm(const uint32_t* __restrict__ ridx, const float* __restrict__ rd
     }
 }
 
-// extras arrive in atomic order: sort each row's segment by index (segments are short) so that
-// the tightened rows, and every sum over them, are the same from run to run
-__global__ void k_sort_extras(uint32_t* __restrict__ ebuf, float* __restrict__ ed2,
-                              const unsigned long long* __restrict__ eoff, int64_t n)
+// Extras arrive in atomic order: each row's segment is put in index order (so that the tightened
+// rows, and every sum over them, are the same from run to run). A 16-lane group per row ranks up
+// to 64 entries held in registers (entries of a segment are distinct); longer segments are sorted
+// by the group's first lane.
+static constexpr int kSortRegs = 4;
+
+__global__ void __launch_bounds__(256)
+k_sort_extras(uint32_t* __restrict__ ebuf, float* __restrict__ ed2, const unsigned long long* __restrict__ eoff,
+              int64_t n)
 {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int glane = threadIdx.x & (kG - 1);
+    const int64_t i = (xcd_tile() * (int64_t)blockDim.x + threadIdx.x) / kG;
+    if (i >= n) return;  // whole groups leave together
     uint32_t* e = ebuf + eoff[i];
     float* d = ed2 + eoff[i];
     const int64_t ne = (int64_t)(eoff[i + 1] - eoff[i]);
+    if (ne <= 1) return;
+    if (ne <= kG * kSortRegs) {
+        uint32_t v[kSortRegs];
+        float dv[kSortRegs];
+#pragma unroll
+        for (int t = 0; t < kSortRegs; t++) {
+            const int64_t k = glane + kG * t;
+            v[t] = k < ne ? e[k] : 0xffffffffu;
+            dv[t] = k < ne ? d[k] : 0.f;
+        }
+        int rank[kSortRegs] = {0, 0, 0, 0};
+#pragma unroll
+        for (int t2 = 0; t2 < kSortRegs; t2++)
+            for (int l = 0; l < kG; l++) {
+                const uint32_t w = __shfl(v[t2], l, kG);
+#pragma unroll
+                for (int t = 0; t < kSortRegs; t++) rank[t] += w < v[t] ? 1 : 0;
+            }
+#pragma unroll
+        for (int t = 0; t < kSortRegs; t++) {
+            const int64_t k = glane + kG * t;
+            if (k < ne) {
+                e[rank[t]] = v[t];
+                d[rank[t]] = dv[t];
+            }
+        }
+        return;
+    }
+    if (glane != 0) return;
     for (int64_t a = 1; a < ne; a++) {
         const uint32_t v = e[a];
         const float dv = d[a];
@@ -1085,26 +1221,43 @@ __global__ void k_sort_extras(uint32_t* __restrict__ ebuf, float* __restrict__ e
     }
 }
 
-// tightened rows (row-major, capacity cap_out): gather row entries, then the row's extras,
-// kept when d2 < (2 max(h_i, h_j))^2 at the final h. The stored FP32 d2 decides unless it lies
-// within its rounding error of the radius; then d2 is formed again in FP64 as the search
-// formed it, so the rows are exactly tighten() of the search lists at h_final.
+// what the emission pass needs to finish the rows converged in one Newton step
+struct FinArgs {
+    const float* rmass;  // stored row masses (NULL: uniform masses)
+    double m_uniform;
+    int kid;
+    double cnorm;
+    const uint8_t* slow;
+    double* rho;
+    double* omega;
+};
+
+// Tightened rows (row-major, capacity cap_out): gather row entries, then the row's extras, kept
+// when d2 < (2 max(h_i, h_j))^2 at the final h. The stored FP32 d2 decides unless it lies within
+// its rounding error of the radius; then d2 is formed again in FP64 as the search formed it, so
+// the rows are exactly tighten() of the search lists at h_final. The same pass sums rho and
+// Omega at the final h of the rows the gather finished in one Newton step (extras never count:
+// their d2 >= (2 h_search_i)^2 >= (2 h_i)^2).
 __global__ void __launch_bounds__(256)
 k_tight_emit_rm(const double* __restrict__ pos, const uint32_t* __restrict__ ridx, const float* __restrict__ rd2,
                 int64_t cap, const int32_t* __restrict__ rowcnt, const unsigned long long* __restrict__ eoff,
                 const uint32_t* __restrict__ ebuf, const float* __restrict__ ed2, const double* __restrict__ hf,
                 int64_t n, int32_t* __restrict__ out_len, int32_t* __restrict__ out_idx, int64_t cap_out,
-                unsigned long long* __restrict__ total)
+                FinArgs F, long long* __restrict__ part)
 {
     const int glane = threadIdx.x & (kG - 1);
     const int gshift = (threadIdx.x & 63) & ~(kG - 1);
-    const int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kG;
+    const int64_t i = (xcd_tile() * (int64_t)blockDim.x + threadIdx.x) / kG;
     const bool live = i < n;
     const int64_t ii = live ? i : n - 1;
     const double hi = hf[ii];
     const int32_t ng = live ? rowcnt[ii] : 0;
     const int64_t e0 = (int64_t)eoff[ii], ne = live ? (int64_t)(eoff[ii + 1] - eoff[ii]) : 0;
     const int64_t len = ng + ne;
+    const bool fin = live && !F.slow[ii];
+    const float inv_hf = (float)(1.0 / hi);
+    const float mu = (float)(F.m_uniform > 0.0 ? F.m_uniform : 1.0);
+    DensSums sums;
     int32_t o = 0;
     for (int64_t kb = 0;; kb += kG) {
         const bool in_range = kb < len;
@@ -1117,6 +1270,7 @@ k_tight_emit_rm(const double* __restrict__ pos, const uint32_t* __restrict__ rid
             if (k < ng) {
                 j = ridx[ii * cap + k] & kMask;
                 d2 = rd2[ii * cap + k];
+                if (fin) sums.add(__fsqrt_rn(d2) * inv_hf, F.rmass ? F.rmass[ii * cap + k] : mu, F.kid);
             } else {
                 j = ebuf[e0 + (k - ng)];
                 d2 = ed2[e0 + (k - ng)];
@@ -1141,9 +1295,20 @@ k_tight_emit_rm(const double* __restrict__ pos, const uint32_t* __restrict__ rid
         }
         o += __popc(gb);
     }
-    if (live && glane == 0) out_len[i] = o < cap_out ? o : (int32_t)cap_out;
-    const unsigned long long wt = wave_sum_u64(live && glane == 0 ? (unsigned long long)o : 0ull);
-    if ((threadIdx.x & 63) == 0 && wt) atomicAdd(total, wt);
+    double s1, s2;
+    sums.total(s1, s2);
+    if (live && glane == 0) {
+        out_len[i] = o < cap_out ? o : (int32_t)cap_out;
+        if (fin) {
+            const double mscale = F.rmass ? 1.0 : (F.m_uniform > 0.0 ? F.m_uniform / (double)mu : 1.0);
+            const double inv_h = 1.0 / hi;
+            const double rs = s1 * mscale * F.cnorm * ipow3(inv_h);
+            const double dws = -s2 * mscale * F.cnorm * ipow4(inv_h);
+            F.rho[i] = rs;
+            F.omega[i] = 1.0 + hi / (3.0 * rs) * dws;
+        }
+    }
+    block_partial<false>(live && glane == 0 ? o : 0, part);
 }
 
 static inline unsigned grid_of(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
@@ -1186,16 +1351,41 @@ Box make_box(const double* box3)
 
 // Grid of the given positions and search lengths, particles sorted by cell into ctx->order and
 // cell starts into ctx->cell_start.
+// Scratch of the per-workgroup partials (4 slots for launches of up to n / 16 + 1 workgroups) and
+// of the collected scalars (ctx->scal, 16 values).
+int reserve_reductions(nb_ctx* ctx, int64_t n)
+{
+    if (ctx_reserve(ctx, (void**)&ctx->part, &ctx->part_bytes, sizeof(long long) * 4 * (size_t)(n / 16 + 1024)) ||
+        ctx_reserve(ctx, (void**)&ctx->scal, &ctx->scal_bytes, sizeof(long long) * 16))
+        return NB_ERR_HIP;
+    return NB_OK;
+}
+
+inline void finish_partials(nb_ctx* ctx, hipStream_t s, int64_t nb, int nslots, unsigned maxmask, int at)
+{
+    hipLaunchKernelGGL(k_partials_final, dim3(1), dim3(kBlock), 0, s, ctx->part, nb, nslots, maxmask,
+                       (long long*)ctx->scal + at);
+}
+
+// The particle order of the last grid for the gather kernels: NULL when the particles were stored
+// in cell order already (identity).
+inline const uint32_t* order_arg(const nb_ctx* ctx) { return ctx->order_ident ? nullptr : ctx->order; }
+
+// Grid of the given positions and search lengths and the particles' cell order (ctx->order,
+// cell_start). check_sorted: when the particles are already stored in cell order (the step
+// keeps them so) the sort is skipped and ctx->order_ident set.
 int build_grid(nb_ctx* ctx, hipStream_t s, const double* pos, const double* h, int64_t n, const Box& box, Grid& g,
-               int64_t& ncells)
+               int64_t& ncells, bool check_sorted)
 {
     // every list builder starts here and reuses the context's list buffers: lists of an
     // earlier build can no longer be emitted
     ctx->lists_ready = 0;
     ctx->hydro_ready = 0;
     ctx->hydro_mixed_ready = 0;
+    ctx->order_ident = 0;
     int nparts = (int)std::min<int64_t>(1024, (n + kBlock - 1) / kBlock);
     if (ctx_reserve(ctx, (void**)&ctx->red, &ctx->red_bytes, sizeof(double) * 8 * (nparts + 2))) return NB_ERR_HIP;
+    if (reserve_reductions(ctx, n)) return NB_ERR_HIP;
     double* fin = ctx->red + 8 * nparts;
     unsigned long long* hsum = (unsigned long long*)(fin + 8);
     hipLaunchKernelGGL(k_bbox, dim3(nparts), dim3(kBlock), 0, s, pos, h, n, box, ctx->red);
@@ -1249,19 +1439,30 @@ int build_grid(nb_ctx* ctx, hipStream_t s, const double* pos, const double* h, i
         return NB_ERR_HIP;
     if (hipMemsetAsync(ctx->cell_count, 0, sizeof(uint32_t) * (ncells + 1), s) != hipSuccess)
         return ctx_fail(ctx, "memset cell_count");
-    hipLaunchKernelGGL(k_cell_id, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, n, box, g, ctx->cell_id, ctx->iota,
-                       ctx->cell_count);
-    int end_bit = 1;
-    while ((int64_t(1) << end_bit) < ncells && end_bit < 32) end_bit++;
+    NB_TIMED(ctx, s, "k_cell_id", k_cell_id, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, n, box, g, ctx->cell_id, ctx->iota,
+                       ctx->cell_count, check_sorted ? ctx->part : nullptr);
+    bool sorted = false;
+    if (check_sorted) {
+        finish_partials(ctx, s, grid_of(n), 1, 0u, 0);
+        long long bad = 0;
+        rc = sync_copy(ctx, s, &bad, ctx->scal, sizeof(bad), "grid: order check failed");
+        if (rc) return rc;
+        sorted = bad == 0;
+    }
+    if (!sorted) {
+        int end_bit = 1;
+        while ((int64_t(1) << end_bit) < ncells && end_bit < 32) end_bit++;
+        size_t bytes = 0;
+        if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, ctx->cell_id, ctx->keys_out, ctx->iota, ctx->order,
+                                               (int)n, 0, end_bit, s) != hipSuccess)
+            return ctx_fail(ctx, "hipcub sort sizing failed");
+        if (ctx_reserve(ctx, (void**)&ctx->cub_tmp, &ctx->cub_tmp_bytes, bytes)) return NB_ERR_HIP;
+        if (hipcub::DeviceRadixSort::SortPairs(ctx->cub_tmp, bytes, ctx->cell_id, ctx->keys_out, ctx->iota, ctx->order,
+                                               (int)n, 0, end_bit, s) != hipSuccess)
+            return ctx_fail(ctx, "hipcub sort failed");
+    }
+    ctx->order_ident = sorted ? 1 : 0;
     size_t bytes = 0;
-    if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, ctx->cell_id, ctx->keys_out, ctx->iota, ctx->order, (int)n,
-                                           0, end_bit, s) != hipSuccess)
-        return ctx_fail(ctx, "hipcub sort sizing failed");
-    if (ctx_reserve(ctx, (void**)&ctx->cub_tmp, &ctx->cub_tmp_bytes, bytes)) return NB_ERR_HIP;
-    if (hipcub::DeviceRadixSort::SortPairs(ctx->cub_tmp, bytes, ctx->cell_id, ctx->keys_out, ctx->iota, ctx->order,
-                                           (int)n, 0, end_bit, s) != hipSuccess)
-        return ctx_fail(ctx, "hipcub sort failed");
-    bytes = 0;
     if (hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, ctx->cell_count, ctx->cell_start, (int)(ncells + 1), s) !=
         hipSuccess)
         return ctx_fail(ctx, "hipcub scan sizing failed");
@@ -1277,7 +1478,7 @@ int gather_stage(nb_ctx* ctx, hipStream_t s, const double* pos, const double* h,
 {
     Grid g;
     int64_t ncells = 0;
-    int rc = build_grid(ctx, s, pos, h, n, box, g, ncells);
+    int rc = build_grid(ctx, s, pos, h, n, box, g, ncells, true);
     if (rc) return rc;
     if (ctx_reserve(ctx, (void**)&ctx->rowcnt, &ctx->rowcnt_bytes, sizeof(int32_t) * n) ||
         ctx_reserve(ctx, (void**)&ctx->unsorted, &ctx->unsorted_bytes, n) ||
@@ -1285,21 +1486,21 @@ int gather_stage(nb_ctx* ctx, hipStream_t s, const double* pos, const double* h,
         ctx_reserve(ctx, (void**)&ctx->eoff, &ctx->eoff_bytes, sizeof(unsigned long long) * (n + 1)) ||
         ctx_reserve(ctx, (void**)&ctx->tot, &ctx->tot_bytes, sizeof(unsigned long long) * (n + 1)) ||
         ctx_reserve(ctx, (void**)&ctx->efill, &ctx->efill_bytes, sizeof(unsigned int) * n) ||
-        ctx_reserve(ctx, (void**)&ctx->offs, &ctx->offs_bytes, sizeof(unsigned long long) * (n + 1)) ||
-        ctx_reserve(ctx, (void**)&ctx->scal, &ctx->scal_bytes, sizeof(int64_t) * 4))
+        ctx_reserve(ctx, (void**)&ctx->offs, &ctx->offs_bytes, sizeof(unsigned long long) * (n + 1)))
         return NB_ERR_HIP;
     const int64_t nblk = (n + 63) / 64;
+    const int64_t nb = grid_of(n);
     int64_t cap = ctx->row_cap > 0 ? ctx->row_cap : 192;
-    int32_t sc[4] = {0, 0, 0, 0};
+    long long sc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int attempt = 0; attempt < 8; attempt++) {
         if (ctx_reserve(ctx, (void**)&ctx->gbuf, &ctx->gbuf_bytes, sizeof(uint32_t) * (size_t)(nblk * cap * 64)))
             return NB_ERR_HIP;
-        if (hipMemsetAsync(ctx->ecount, 0, sizeof(unsigned long long) * (n + 1), s) != hipSuccess ||
-            hipMemsetAsync(ctx->scal, 0, sizeof(int64_t) * 4, s) != hipSuccess)
+        if (hipMemsetAsync(ctx->ecount, 0, sizeof(unsigned long long) * (n + 1), s) != hipSuccess)
             return ctx_fail(ctx, "memset");
-        hipLaunchKernelGGL(k_gather, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, h, n, box, g, ctx->cell_start, ctx->order,
-                           ctx->gbuf, cap, ctx->rowcnt, ctx->unsorted, ctx->ecount, (int32_t*)ctx->scal);
-        rc = sync_copy(ctx, s, sc, ctx->scal, sizeof(sc), "gather failed");
+        NB_TIMED(ctx, s, "k_gather", k_gather, dim3(nb), dim3(kBlock), 0, s, pos, h, n, box, g, ctx->cell_start, order_arg(ctx),
+                           ctx->gbuf, cap, ctx->rowcnt, ctx->unsorted, ctx->ecount, ctx->part);
+        finish_partials(ctx, s, nb, 2, 1u, 0);  // [0] longest row, [1] unsorted rows
+        rc = sync_copy(ctx, s, sc, ctx->scal, sizeof(long long) * 2, "gather failed");
         if (rc) return rc;
         if (sc[0] <= cap) break;
         cap = ((sc[0] + sc[0] / 8 + 15) / 16) * 16;
@@ -1308,27 +1509,25 @@ int gather_stage(nb_ctx* ctx, hipStream_t s, const double* pos, const double* h,
     ctx->row_cap = cap;
     ctx->last_unsorted = sc[1];
     if (sc[1] > 0)
-        hipLaunchKernelGGL(k_sort_rows, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->gbuf, cap, ctx->rowcnt, ctx->unsorted, n);
+        NB_TIMED(ctx, s, "k_sort_rows", k_sort_rows, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->gbuf, cap, ctx->rowcnt, ctx->unsorted, n);
     rc = scan_u64(ctx, s, ctx->ecount, ctx->eoff, n + 1);
     if (rc) return rc;
-    unsigned long long ne_total = 0;
-    rc = sync_copy(ctx, s, &ne_total, ctx->eoff + n, sizeof(ne_total), "extras count copy failed");
-    if (rc) return rc;
-    if (ctx_reserve(ctx, (void**)&ctx->ebuf, &ctx->ebuf_bytes, sizeof(uint32_t) * (ne_total + 1))) return NB_ERR_HIP;
-    if (hipMemsetAsync(ctx->efill, 0, sizeof(unsigned int) * n, s) != hipSuccess) return ctx_fail(ctx, "memset");
-    hipLaunchKernelGGL(k_extras_fill, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->gbuf, cap, ctx->rowcnt, ctx->eoff,
-                       ctx->efill, ctx->ebuf, n);
-    hipLaunchKernelGGL(k_row_total, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, ctx->rowcnt, ctx->ecount, ctx->tot,
-                       (int32_t*)ctx->scal, n);
+    NB_TIMED(ctx, s, "k_row_total", k_row_total, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, ctx->rowcnt, ctx->ecount, ctx->tot,
+                       ctx->part, n);
+    finish_partials(ctx, s, grid_of(n + 1), 1, 1u, 2);  // [2] longest merged row
     rc = scan_u64(ctx, s, ctx->tot, ctx->offs, n + 1);
     if (rc) return rc;
-    unsigned long long total = 0;
-    rc = sync_copy(ctx, s, &total, ctx->offs + n, sizeof(total), "total copy failed");
+    hipLaunchKernelGGL(k_collect, dim3(1), dim3(64), 0, s, (long long*)ctx->scal + 3, ctx->eoff + n, ctx->offs + n);
+    rc = sync_copy(ctx, s, sc, ctx->scal, sizeof(long long) * 5, "list totals copy failed");
     if (rc) return rc;
-    rc = sync_copy(ctx, s, sc, ctx->scal, sizeof(sc), "max row copy failed");
-    if (rc) return rc;
+    const unsigned long long ne_total = (unsigned long long)sc[3];
+    if (ctx_reserve(ctx, (void**)&ctx->ebuf, &ctx->ebuf_bytes, sizeof(uint32_t) * (ne_total + 1))) return NB_ERR_HIP;
+    if (hipMemsetAsync(ctx->efill, 0, sizeof(unsigned int) * n, s) != hipSuccess) return ctx_fail(ctx, "memset");
+    NB_TIMED(ctx, s, "k_extras_fill", k_extras_fill, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->gbuf, cap, ctx->rowcnt, ctx->eoff,
+                       ctx->efill, ctx->ebuf, n);
+    if (hipGetLastError() != hipSuccess) return ctx_fail(ctx, "gather: kernel launch failed");
     ctx->lists_n = n;
-    ctx->lists_total = (int64_t)total;
+    ctx->lists_total = sc[4];
     ctx->lists_maxrow = sc[2];
     ctx->lists_ready = 1;
     return NB_OK;
@@ -1345,7 +1544,7 @@ extern "C" int32_t nb_grid_order(nb_ctx* ctx, void* stream, const double* pos, c
     if (n >= (int64_t)kMask) return ctx_fail(ctx, "nb_grid_order: too many particles");
     Grid g;
     int64_t ncells;
-    int rc = build_grid(ctx, s, pos, h, n, make_box(box3), g, ncells);
+    int rc = build_grid(ctx, s, pos, h, n, make_box(box3), g, ncells, false);
     if (rc) return rc;
     if (hipMemcpyAsync(order_out, ctx->order, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s) != hipSuccess)
         return ctx_fail(ctx, "nb_grid_order: copy failed");
@@ -1384,7 +1583,7 @@ extern "C" int32_t nb_lists_emit(nb_ctx* ctx, void* stream, int32_t* out_index, 
     if (ctx->lists_total > capacity) return ctx_fail(ctx, "nb_lists_emit: capacity too small");
     const int64_t n = ctx->lists_n;
     if (n == 0) return NB_OK;
-    hipLaunchKernelGGL(k_merge<false>, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->gbuf, (int64_t)ctx->row_cap,
+    NB_TIMED(ctx, s, "k_merge", k_merge<false>, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->gbuf, (int64_t)ctx->row_cap,
                        ctx->rowcnt, ctx->eoff, ctx->ebuf, ctx->offs, out_index, (int32_t*)nullptr, (int64_t)1 << 40, n);
     if (hipGetLastError() != hipSuccess) return ctx_fail(ctx, "nb_lists_emit: kernel launch failed");
     ctx->lists_ready = 2;
@@ -1423,7 +1622,7 @@ extern "C" int32_t nb_lists_emit_blocked(nb_ctx* ctx, void* stream, int32_t* out
     if (ctx->lists_maxrow > row_cap) return ctx_fail(ctx, "nb_lists_emit_blocked: row capacity too small");
     const int64_t n = ctx->lists_n;
     if (n == 0) return NB_OK;
-    hipLaunchKernelGGL(k_merge<true>, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->gbuf, (int64_t)ctx->row_cap,
+    NB_TIMED(ctx, s, "k_merge", k_merge<true>, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->gbuf, (int64_t)ctx->row_cap,
                        ctx->rowcnt, ctx->eoff, ctx->ebuf, ctx->offs, out_index, out_len, row_cap, n);
     if (hipGetLastError() != hipSuccess) return ctx_fail(ctx, "nb_lists_emit_blocked: kernel launch failed");
     ctx->lists_ready = 2;
@@ -1476,13 +1675,12 @@ extern "C" int32_t nb_tighten_blocked(nb_ctx* ctx, void* stream, const double* p
     if (!ctx || n < 0 || !total_out) return ctx_fail(ctx, "nb_tighten_blocked: bad arguments");
     *total_out = 0;
     if (n == 0) return NB_OK;
-    if (ctx_reserve(ctx, (void**)&ctx->scal, &ctx->scal_bytes, sizeof(int64_t) * 4)) return NB_ERR_HIP;
-    unsigned long long* tot = (unsigned long long*)ctx->scal + 3;
-    if (hipMemsetAsync(tot, 0, sizeof(unsigned long long), s) != hipSuccess) return ctx_fail(ctx, "memset");
-    hipLaunchKernelGGL(k_tighten_blk, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, h, make_box(box3), in_len, in_index,
-                       row_cap, n, out_len, out_index, tot);
-    unsigned long long t = 0;
-    int rc = sync_copy(ctx, s, &t, tot, sizeof(t), "nb_tighten_blocked: total copy failed");
+    if (reserve_reductions(ctx, n)) return NB_ERR_HIP;
+    NB_TIMED(ctx, s, "k_tighten_blk", k_tighten_blk, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, h, make_box(box3), in_len, in_index,
+                       row_cap, n, out_len, out_index, ctx->part);
+    finish_partials(ctx, s, grid_of(n), 1, 0u, 8);
+    long long t = 0;
+    int rc = sync_copy(ctx, s, &t, (long long*)ctx->scal + 8, sizeof(t), "nb_tighten_blocked: total copy failed");
     if (rc) return rc;
     *total_out = (int64_t)t;
     return NB_OK;
@@ -1522,7 +1720,7 @@ extern "C" int32_t nb_hydro_density(nb_ctx* ctx, void* stream, const double* pos
     const Box open{{0.0, 0.0, 0.0}};
     Grid g;
     int64_t ncells = 0;
-    int rc = build_grid(ctx, s, pos, h_search, n, open, g, ncells);
+    int rc = build_grid(ctx, s, pos, h_search, n, open, g, ncells, true);
     if (rc) return rc;
     const bool mixed = precision != 0;
     const bool masses = !(m_uniform > 0.0);
@@ -1532,31 +1730,33 @@ extern "C" int32_t nb_hydro_density(nb_ctx* ctx, void* stream, const double* pos
         ctx_reserve(ctx, (void**)&ctx->ecount, &ctx->ecount_bytes, sizeof(unsigned long long) * (n + 1)) ||
         ctx_reserve(ctx, (void**)&ctx->eoff, &ctx->eoff_bytes, sizeof(unsigned long long) * (n + 1)) ||
         ctx_reserve(ctx, (void**)&ctx->efill, &ctx->efill_bytes, sizeof(unsigned int) * n) ||
-        ctx_reserve(ctx, (void**)&ctx->scal, &ctx->scal_bytes, sizeof(int64_t) * 4))
+        ctx_reserve(ctx, (void**)&ctx->tot, &ctx->tot_bytes, sizeof(unsigned long long) * (n + 1)) ||
+        ctx_reserve(ctx, (void**)&ctx->offs, &ctx->offs_bytes, sizeof(unsigned long long) * (n + 1)))
         return NB_ERR_HIP;
     const int64_t nblk = (n + 63) / 64;
+    const int64_t nb = grid_of(n);
     int64_t cap = ctx->row_cap > 0 ? ctx->row_cap : 192;
-    int32_t sc[4] = {0, 0, 0, 0};
+    long long sc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     DensArgs A{mass, m_uniform, h_start, h_max, kid, cnorm, eta, tol, h_out, rho, omega, iters, flags, ctx->pending,
-               (int32_t*)ctx->scal};
+               ctx->part};
     for (int attempt = 0; attempt < 8; attempt++) {
         const size_t cells = (size_t)(nblk * cap * 64);
         if (ctx_reserve(ctx, (void**)&ctx->gbuf, &ctx->gbuf_bytes, sizeof(uint32_t) * cells) ||
             ctx_reserve(ctx, (void**)&ctx->dscratch, &ctx->dscratch_bytes, sizeof(double) * cells * (masses ? 2 : 1)))
             return NB_ERR_HIP;
-        if (hipMemsetAsync(ctx->ecount, 0, sizeof(unsigned long long) * (n + 1), s) != hipSuccess ||
-            hipMemsetAsync(ctx->scal, 0, sizeof(int64_t) * 4, s) != hipSuccess)
+        if (hipMemsetAsync(ctx->ecount, 0, sizeof(unsigned long long) * (n + 1), s) != hipSuccess)
             return ctx_fail(ctx, "memset");
         double* mbuf = masses ? ctx->dscratch + cells : nullptr;
         if (mixed)
-            hipLaunchKernelGGL(k_gather_solve<true>, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, h_search, n, g,
-                               ctx->cell_start, ctx->order, ctx->gbuf, ctx->dscratch, mbuf, cap, ctx->rowcnt,
+            NB_TIMED(ctx, s, "k_gather_solve", k_gather_solve<true>, dim3(nb), dim3(kBlock), 0, s, pos, h_search, n, g,
+                               ctx->cell_start, order_arg(ctx), ctx->gbuf, ctx->dscratch, mbuf, cap, ctx->rowcnt,
                                ctx->unsorted, ctx->ecount, A);
         else
-            hipLaunchKernelGGL(k_gather_solve<false>, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, h_search, n, g,
-                               ctx->cell_start, ctx->order, ctx->gbuf, ctx->dscratch, mbuf, cap, ctx->rowcnt,
+            NB_TIMED(ctx, s, "k_gather_solve", k_gather_solve<false>, dim3(nb), dim3(kBlock), 0, s, pos, h_search, n, g,
+                               ctx->cell_start, order_arg(ctx), ctx->gbuf, ctx->dscratch, mbuf, cap, ctx->rowcnt,
                                ctx->unsorted, ctx->ecount, A);
-        rc = sync_copy(ctx, s, sc, ctx->scal, sizeof(sc), "nb_hydro_density: gather failed");
+        finish_partials(ctx, s, nb, 3, 1u, 0);  // [0] longest gather row, [1] unsorted rows, [2] rows pending
+        rc = sync_copy(ctx, s, sc, ctx->scal, sizeof(long long) * 3, "nb_hydro_density: gather failed");
         if (rc) return rc;
         if (sc[0] <= cap) break;
         cap = ((sc[0] + sc[0] / 8 + 15) / 16) * 16;
@@ -1567,47 +1767,42 @@ extern "C" int32_t nb_hydro_density(nb_ctx* ctx, void* stream, const double* pos
     const size_t cells = (size_t)(nblk * cap * 64);
     double* mbuf = masses ? ctx->dscratch + cells : nullptr;
     if (sc[1] > 0)
-        hipLaunchKernelGGL(k_sort_rows_payload, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->gbuf, ctx->dscratch, mbuf,
+        NB_TIMED(ctx, s, "k_sort_rows_payload", k_sort_rows_payload, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->gbuf, ctx->dscratch, mbuf,
                            cap, ctx->rowcnt, ctx->unsorted, n);
-    if (sc[3] > 0) {
+    if (sc[2] > 0) {
         if (mixed)
-            hipLaunchKernelGGL(k_solve_pending<true>, dim3(grid_of(n)), dim3(kBlock), 0, s, h_search, n, ctx->dscratch,
+            NB_TIMED(ctx, s, "k_solve_pending", k_solve_pending<true>, dim3(grid_of(n)), dim3(kBlock), 0, s, h_search, n, ctx->dscratch,
                                mbuf, cap, ctx->rowcnt, A);
         else
-            hipLaunchKernelGGL(k_solve_pending<false>, dim3(grid_of(n)), dim3(kBlock), 0, s, h_search, n,
+            NB_TIMED(ctx, s, "k_solve_pending", k_solve_pending<false>, dim3(grid_of(n)), dim3(kBlock), 0, s, h_search, n,
                                ctx->dscratch, mbuf, cap, ctx->rowcnt, A);
     }
     // extras (with d2) for the tightened rows; the longest merged row bounds the tight rows
-    if (ctx_reserve(ctx, (void**)&ctx->tot, &ctx->tot_bytes, sizeof(unsigned long long) * (n + 1))) return NB_ERR_HIP;
-    hipLaunchKernelGGL(k_row_total, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, ctx->rowcnt, ctx->ecount, ctx->tot,
-                       (int32_t*)ctx->scal, n);
+    NB_TIMED(ctx, s, "k_row_total", k_row_total, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, ctx->rowcnt, ctx->ecount, ctx->tot,
+                       ctx->part, n);
+    finish_partials(ctx, s, grid_of(n + 1), 1, 1u, 3);  // [3] longest merged row
     rc = scan_u64(ctx, s, ctx->ecount, ctx->eoff, n + 1);
     if (rc) return rc;
-    unsigned long long ne_total = 0;
-    rc = sync_copy(ctx, s, &ne_total, ctx->eoff + n, sizeof(ne_total), "extras count copy failed");
-    if (rc) return rc;
-    rc = sync_copy(ctx, s, sc, ctx->scal, sizeof(sc), "merged row length copy failed");
-    if (rc) return rc;
-    if (ctx_reserve(ctx, (void**)&ctx->offs, &ctx->offs_bytes, sizeof(unsigned long long) * (n + 1))) return NB_ERR_HIP;
     rc = scan_u64(ctx, s, ctx->tot, ctx->offs, n + 1);
     if (rc) return rc;
-    unsigned long long merged_total = 0;
-    rc = sync_copy(ctx, s, &merged_total, ctx->offs + n, sizeof(merged_total), "merged total copy failed");
+    hipLaunchKernelGGL(k_collect, dim3(1), dim3(64), 0, s, (long long*)ctx->scal + 4, ctx->eoff + n, ctx->offs + n);
+    rc = sync_copy(ctx, s, sc, ctx->scal, sizeof(long long) * 6, "nb_hydro_density: totals copy failed");
     if (rc) return rc;
+    const unsigned long long ne_total = (unsigned long long)sc[4];
     if (ctx_reserve(ctx, (void**)&ctx->ebuf, &ctx->ebuf_bytes, sizeof(uint32_t) * (ne_total + 1)) ||
         ctx_reserve(ctx, (void**)&ctx->edbuf, &ctx->edbuf_bytes, sizeof(double) * (ne_total + 1)))
         return NB_ERR_HIP;
     if (hipMemsetAsync(ctx->efill, 0, sizeof(unsigned int) * n, s) != hipSuccess) return ctx_fail(ctx, "memset");
-    hipLaunchKernelGGL(k_extras_fill_d2, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->gbuf, ctx->dscratch, cap,
+    NB_TIMED(ctx, s, "k_extras_fill_d2", k_extras_fill_d2, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->gbuf, ctx->dscratch, cap,
                        ctx->rowcnt, ctx->eoff, ctx->efill, ctx->ebuf, ctx->edbuf, n);
     if (hipGetLastError() != hipSuccess) return ctx_fail(ctx, "nb_hydro_density: kernel launch failed");
     ctx->lists_n = n;
     ctx->hydro_ready = 1;
-    ctx->lists_maxrow = sc[2];
-    stats4[0] = sc[0];              // longest gather row
-    stats4[1] = ctx->last_unsorted;  // rows gathered unsorted
-    stats4[2] = (int64_t)merged_total;  // entries of the search lists (incl. self)
-    stats4[3] = sc[2];              // longest merged row: the capacity the tightened rows need
+    ctx->lists_maxrow = sc[3];
+    stats4[0] = sc[0];   // longest gather row
+    stats4[1] = sc[1];   // rows gathered unsorted
+    stats4[2] = sc[5];   // entries of the search lists (incl. self)
+    stats4[3] = sc[3];   // longest merged row: the capacity the tightened rows need
     return NB_OK;
 }
 
@@ -1618,14 +1813,12 @@ extern "C" int32_t nb_hydro_tight(nb_ctx* ctx, void* stream, const double* h_fin
     if (!ctx || !ctx->hydro_ready || !total_out) return ctx_fail_code(ctx, NB_ERR_NOT_BUILT, "nb_hydro_tight: no density pass");
     if (row_cap < ctx->lists_maxrow) return ctx_fail(ctx, "nb_hydro_tight: row capacity below the longest merged row");
     const int64_t n = ctx->lists_n;
-    if (ctx_reserve(ctx, (void**)&ctx->scal, &ctx->scal_bytes, sizeof(int64_t) * 4)) return NB_ERR_HIP;
-    unsigned long long* tot = (unsigned long long*)ctx->scal + 3;
-    if (hipMemsetAsync(tot, 0, sizeof(unsigned long long), s) != hipSuccess) return ctx_fail(ctx, "memset");
-    hipLaunchKernelGGL(k_tight_emit, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->gbuf, ctx->dscratch,
+    NB_TIMED(ctx, s, "k_tight_emit", k_tight_emit, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->gbuf, ctx->dscratch,
                        (int64_t)ctx->row_cap, ctx->rowcnt, ctx->eoff, ctx->ebuf, ctx->edbuf, h_final, n, out_len,
-                       out_index, row_cap, tot);
-    unsigned long long t = 0;
-    int rc = sync_copy(ctx, s, &t, tot, sizeof(t), "nb_hydro_tight: total copy failed");
+                       out_index, row_cap, ctx->part);
+    finish_partials(ctx, s, grid_of(n), 1, 0u, 8);
+    long long t = 0;
+    int rc = sync_copy(ctx, s, &t, (long long*)ctx->scal + 8, sizeof(t), "nb_hydro_tight: total copy failed");
     if (rc) return rc;
     *total_out = (int64_t)t;
     return NB_OK;
@@ -1643,95 +1836,115 @@ extern "C" int32_t nb_hydro_density_mixed(nb_ctx* ctx, void* stream, const doubl
     const Box open{{0.0, 0.0, 0.0}};
     Grid g;
     int64_t ncells = 0;
-    int rc = build_grid(ctx, s, pos, h_search, n, open, g, ncells);
+    int rc = build_grid(ctx, s, pos, h_search, n, open, g, ncells, true);
     if (rc) return rc;
     const bool masses = !(m_uniform > 0.0);
     if (ctx_reserve(ctx, (void**)&ctx->rowcnt, &ctx->rowcnt_bytes, sizeof(int32_t) * n) ||
+        ctx_reserve(ctx, (void**)&ctx->pending, &ctx->pending_bytes, n) ||
+        ctx_reserve(ctx, (void**)&ctx->posh, &ctx->posh_bytes, sizeof(double4) * n) ||
         ctx_reserve(ctx, (void**)&ctx->ecount, &ctx->ecount_bytes, sizeof(unsigned long long) * (n + 1)) ||
         ctx_reserve(ctx, (void**)&ctx->eoff, &ctx->eoff_bytes, sizeof(unsigned long long) * (n + 1)) ||
         ctx_reserve(ctx, (void**)&ctx->tot, &ctx->tot_bytes, sizeof(unsigned long long) * (n + 1)) ||
         ctx_reserve(ctx, (void**)&ctx->offs, &ctx->offs_bytes, sizeof(unsigned long long) * (n + 1)) ||
-        ctx_reserve(ctx, (void**)&ctx->efill, &ctx->efill_bytes, sizeof(unsigned int) * n) ||
-        ctx_reserve(ctx, (void**)&ctx->scal, &ctx->scal_bytes, sizeof(int64_t) * 4))
+        ctx_reserve(ctx, (void**)&ctx->efill, &ctx->efill_bytes, sizeof(unsigned int) * n))
         return NB_ERR_HIP;
+    NB_TIMED(ctx, s, "k_pack_posh", k_pack_posh, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, h_search, n,
+             (double4*)ctx->posh);
     int64_t cap = ctx->row_cap_rm > 0 ? ctx->row_cap_rm : 192;
-    int32_t sc[4] = {0, 0, 0, 0};
-    MixArgs A{mass, m_uniform, h_start, h_max, kid, cnorm, eta, tol, h_out, rho, omega, iters, flags, (int32_t*)ctx->scal};
-    const unsigned blocks = (unsigned)((n * kG + 255) / 256);
+    long long sc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    MixArgs A{mass, m_uniform, h_start, h_max, kid, cnorm, eta, tol, h_out, rho, omega, iters, flags, ctx->pending,
+              ctx->part};
+    const int64_t blocks = (n * kG + 255) / 256;
     for (int attempt = 0; attempt < 8; attempt++) {
         const size_t cells = (size_t)(n * cap);
         if (ctx_reserve(ctx, (void**)&ctx->gbuf, &ctx->gbuf_bytes, sizeof(uint32_t) * cells) ||
             ctx_reserve(ctx, (void**)&ctx->fscratch, &ctx->fscratch_bytes, sizeof(float) * cells * (masses ? 2 : 1)))
             return NB_ERR_HIP;
-        if (hipMemsetAsync(ctx->ecount, 0, sizeof(unsigned long long) * (n + 1), s) != hipSuccess ||
-            hipMemsetAsync(ctx->scal, 0, sizeof(int64_t) * 4, s) != hipSuccess)
+        if (hipMemsetAsync(ctx->ecount, 0, sizeof(unsigned long long) * (n + 1), s) != hipSuccess)
             return ctx_fail(ctx, "memset");
         float* rmass = masses ? ctx->fscratch + cells : nullptr;
         if (masses)
-            hipLaunchKernelGGL(k_gather_solve_mixed<true>, dim3(blocks), dim3(256), 0, s, pos, h_search, n, g,
-                               ctx->cell_start, ctx->order, ctx->gbuf, ctx->fscratch, rmass, cap, ctx->rowcnt,
-                               ctx->ecount, A);
+            NB_TIMED(ctx, s, "k_gather_mixed", k_gather_mixed<true>, dim3(blocks), dim3(256), 0, s,
+                     (const double4*)ctx->posh, n, g, ctx->cell_start, order_arg(ctx), ctx->gbuf, ctx->fscratch, rmass,
+                     cap, ctx->rowcnt, ctx->ecount, A);
         else
-            hipLaunchKernelGGL(k_gather_solve_mixed<false>, dim3(blocks), dim3(256), 0, s, pos, h_search, n, g,
-                               ctx->cell_start, ctx->order, ctx->gbuf, ctx->fscratch, rmass, cap, ctx->rowcnt,
-                               ctx->ecount, A);
-        rc = sync_copy(ctx, s, sc, ctx->scal, sizeof(sc), "nb_hydro_density_mixed: gather failed");
+            NB_TIMED(ctx, s, "k_gather_mixed", k_gather_mixed<false>, dim3(blocks), dim3(256), 0, s,
+                     (const double4*)ctx->posh, n, g, ctx->cell_start, order_arg(ctx), ctx->gbuf, ctx->fscratch, rmass,
+                     cap, ctx->rowcnt, ctx->ecount, A);
+        finish_partials(ctx, s, blocks, 2, 1u, 0);  // [0] longest gather row, [1] rows left slow
+        rc = sync_copy(ctx, s, sc, ctx->scal, sizeof(long long) * 2, "nb_hydro_density_mixed: gather failed");
         if (rc) return rc;
         if (sc[0] <= cap) break;
         cap = ((sc[0] + sc[0] / 8 + 15) / 16) * 16;
         if (attempt == 7) return ctx_fail(ctx, "row capacity did not settle");
     }
     ctx->row_cap_rm = (int32_t)cap;
-    const int32_t from_mem = sc[1];
-    if (hipMemsetAsync(ctx->scal, 0, sizeof(int64_t) * 4, s) != hipSuccess) return ctx_fail(ctx, "memset");
-    hipLaunchKernelGGL(k_row_total, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, ctx->rowcnt, ctx->ecount, ctx->tot,
-                       (int32_t*)ctx->scal, n);
+    const long long slow_rows = sc[1];
+    const size_t cells = (size_t)(n * cap);
+    float* rmass = masses ? ctx->fscratch + cells : nullptr;
+    if (slow_rows > 0) {
+        if (masses)
+            NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<true>, dim3(blocks), dim3(256), 0, s, h_search, n,
+                     ctx->fscratch, rmass, cap, ctx->rowcnt, A);
+        else
+            NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<false>, dim3(blocks), dim3(256), 0, s, h_search,
+                     n, ctx->fscratch, rmass, cap, ctx->rowcnt, A);
+    }
+    NB_TIMED(ctx, s, "k_row_total", k_row_total, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, ctx->rowcnt, ctx->ecount,
+             ctx->tot, ctx->part, n);
+    finish_partials(ctx, s, grid_of(n + 1), 1, 1u, 2);  // [2] longest merged row
     rc = scan_u64(ctx, s, ctx->ecount, ctx->eoff, n + 1);
     if (rc) return rc;
     rc = scan_u64(ctx, s, ctx->tot, ctx->offs, n + 1);
     if (rc) return rc;
-    unsigned long long tots[2] = {0, 0};
-    rc = sync_copy(ctx, s, &tots[0], ctx->eoff + n, sizeof(unsigned long long), "extras count copy failed");
+    hipLaunchKernelGGL(k_collect, dim3(1), dim3(64), 0, s, (long long*)ctx->scal + 3, ctx->eoff + n, ctx->offs + n);
+    rc = sync_copy(ctx, s, sc, ctx->scal, sizeof(long long) * 5, "nb_hydro_density_mixed: totals copy failed");
     if (rc) return rc;
-    rc = sync_copy(ctx, s, &tots[1], ctx->offs + n, sizeof(unsigned long long), "merged total copy failed");
-    if (rc) return rc;
-    rc = sync_copy(ctx, s, sc, ctx->scal, sizeof(sc), "merged row length copy failed");
-    if (rc) return rc;
-    if (ctx_reserve(ctx, (void**)&ctx->ebuf, &ctx->ebuf_bytes, sizeof(uint32_t) * (tots[0] + 1)) ||
-        ctx_reserve(ctx, (void**)&ctx->edbuf, &ctx->edbuf_bytes, sizeof(float) * (tots[0] + 1)))
+    const unsigned long long ne_total = (unsigned long long)sc[3];
+    if (ctx_reserve(ctx, (void**)&ctx->ebuf, &ctx->ebuf_bytes, sizeof(uint32_t) * (ne_total + 1)) ||
+        ctx_reserve(ctx, (void**)&ctx->edbuf, &ctx->edbuf_bytes, sizeof(float) * (ne_total + 1)))
         return NB_ERR_HIP;
     if (hipMemsetAsync(ctx->efill, 0, sizeof(unsigned int) * n, s) != hipSuccess) return ctx_fail(ctx, "memset");
-    hipLaunchKernelGGL(k_extras_fill_rm, dim3(blocks), dim3(256), 0, s, ctx->gbuf, ctx->fscratch, cap, ctx->rowcnt,
-                       ctx->eoff, ctx->efill, ctx->ebuf, (float*)ctx->edbuf, n);
-    hipLaunchKernelGGL(k_sort_extras, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->ebuf, (float*)ctx->edbuf, ctx->eoff,
-                       n);
+    NB_TIMED(ctx, s, "k_extras_fill_rm", k_extras_fill_rm, dim3(blocks), dim3(256), 0, s, ctx->gbuf, ctx->fscratch, cap,
+             ctx->rowcnt, ctx->eoff, ctx->efill, ctx->ebuf, (float*)ctx->edbuf, n);
+    NB_TIMED(ctx, s, "k_sort_extras", k_sort_extras, dim3(blocks), dim3(256), 0, s, ctx->ebuf, (float*)ctx->edbuf,
+             ctx->eoff, n);
     if (hipGetLastError() != hipSuccess) return ctx_fail(ctx, "nb_hydro_density_mixed: kernel launch failed");
     ctx->lists_n = n;
     ctx->lists_maxrow = sc[2];
     ctx->hydro_mixed_ready = 1;
+    ctx->mix_kid = kid;
+    ctx->mix_cnorm = cnorm;
+    ctx->mix_m_uniform = m_uniform;
     stats4[0] = cap;
-    stats4[1] = from_mem;                // rows solved from memory (register stash too small)
-    stats4[2] = (int64_t)tots[1];        // search-list entries (incl. self)
-    stats4[3] = sc[2];                   // longest merged row
+    stats4[1] = slow_rows;  // rows not converged in one Newton step (solved in full from the stored row)
+    stats4[2] = sc[4];      // search-list entries (incl. self)
+    stats4[3] = sc[2];      // longest merged row
     return NB_OK;
 }
 
 extern "C" int32_t nb_hydro_tight_mixed(nb_ctx* ctx, void* stream, const double* pos, const double* h_final,
-                                        int32_t* out_len, int32_t* out_index, int64_t row_cap, int64_t* total_out)
+                                        double* rho, double* omega, int32_t* out_len, int32_t* out_index,
+                                        int64_t row_cap, int64_t* total_out)
 {
     hipStream_t s = (hipStream_t)stream;
     if (!ctx || !ctx->hydro_mixed_ready || !total_out)
         return ctx_fail_code(ctx, NB_ERR_NOT_BUILT, "nb_hydro_tight_mixed: no density pass");
+    if (!pos || !h_final || !rho || !omega || !out_len || !out_index)
+        return ctx_fail(ctx, "nb_hydro_tight_mixed: bad arguments");
     if (row_cap < ctx->lists_maxrow) return ctx_fail(ctx, "nb_hydro_tight_mixed: row capacity below the longest row");
     const int64_t n = ctx->lists_n;
-    unsigned long long* tot = (unsigned long long*)ctx->scal + 3;
-    if (hipMemsetAsync(tot, 0, sizeof(unsigned long long), s) != hipSuccess) return ctx_fail(ctx, "memset");
+    const int64_t cap = ctx->row_cap_rm;
+    const bool masses = !(ctx->mix_m_uniform > 0.0);
+    const FinArgs F{masses ? ctx->fscratch + (size_t)(n * cap) : nullptr, ctx->mix_m_uniform, ctx->mix_kid,
+                    ctx->mix_cnorm, ctx->pending, rho, omega};
     const unsigned blocks = (unsigned)((n * kG + 255) / 256);
-    hipLaunchKernelGGL(k_tight_emit_rm, dim3(blocks), dim3(256), 0, s, pos, ctx->gbuf, ctx->fscratch,
-                       (int64_t)ctx->row_cap_rm, ctx->rowcnt, ctx->eoff, ctx->ebuf, (const float*)ctx->edbuf, h_final,
-                       n, out_len, out_index, row_cap, tot);
-    unsigned long long t = 0;
-    int rc = sync_copy(ctx, s, &t, tot, sizeof(t), "nb_hydro_tight_mixed: total copy failed");
+    NB_TIMED(ctx, s, "k_tight_emit_rm", k_tight_emit_rm, dim3(blocks), dim3(256), 0, s, pos, ctx->gbuf, ctx->fscratch,
+             cap, ctx->rowcnt, ctx->eoff, ctx->ebuf, (const float*)ctx->edbuf, h_final, n, out_len, out_index, row_cap,
+             F, ctx->part);
+    finish_partials(ctx, s, blocks, 1, 0u, 8);
+    long long t = 0;
+    int rc = sync_copy(ctx, s, &t, (long long*)ctx->scal + 8, sizeof(t), "nb_hydro_tight_mixed: total copy failed");
     if (rc) return rc;
     *total_out = (int64_t)t;
     return NB_OK;

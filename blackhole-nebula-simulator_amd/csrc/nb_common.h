@@ -114,6 +114,39 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
     return v;
 }
 
+// Workgroups of a launch are dispatched to the 8 XCDs round-robin (workgroup b on XCD b % 8),
+// and each XCD has its own L2. xcd_tile() renumbers the workgroups so that each XCD works on one
+// contiguous range of tiles: consecutive tiles hold neighbouring particles (grid order), whose
+// neighbour data then stays in one L2 instead of being fetched by all eight. A bijection of
+// [0, gridDim.x) for any grid size (1-D grids).
+__device__ __forceinline__ int64_t xcd_tile()
+{
+    const int64_t nb = gridDim.x, b = blockIdx.x;
+    const int64_t q = nb >> 3, r = nb & 7, x = b & 7;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+// Block-wide sum or max of one value per thread, written by thread 0 to part[blockIdx.x] (a
+// per-workgroup partial, combined afterwards by one small kernel: hot kernels make no
+// same-address atomics). Every thread of the block must call it; blockDim.x a multiple of 64.
+template <bool kMax>
+__device__ __forceinline__ void block_partial(long long v, long long* part)
+{
+    __shared__ long long sh_part[16];
+    for (int o = 32; o > 0; o >>= 1) {
+        const long long t = __shfl_xor(v, o, 64);
+        v = kMax ? (t > v ? t : v) : v + t;
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh_part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long r = sh_part[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = kMax ? (sh_part[w] > r ? sh_part[w] : r) : r + sh_part[w];
+        part[blockIdx.x] = r;
+    }
+}
+
 // Particle rows of a pass: rows[q] if a row list is given, else q.
 __device__ __forceinline__ int64_t row_of(const int64_t* rows, int64_t q)
 {

@@ -9,12 +9,30 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string>
+#include <vector>
 
 #include "nb_common.h"
+
+// one timed launch: HIP events recorded around it on its stream
+struct nb_timed {
+    const char* name;
+    hipEvent_t start, stop;
+};
+
+struct nb_kernel_time {
+    std::string name;
+    long long launches = 0;
+    double ms = 0.0;
+};
 
 struct nb_ctx {
     int device = 0;
     std::string err;
+    // kernel timing (nb_timing)
+    int timing = 0;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<nb_timed> ev_open;
+    std::vector<nb_kernel_time> ktimes;
     // neighbour grid
     uint32_t* cell_id = nullptr; size_t cell_id_bytes = 0;
     uint32_t* keys_out = nullptr; size_t keys_out_bytes = 0;
@@ -39,6 +57,8 @@ struct nb_ctx {
     double* dscratch = nullptr; size_t dscratch_bytes = 0;
     float* fscratch = nullptr; size_t fscratch_bytes = 0;
     void* pack = nullptr; size_t pack_bytes = 0;  // per-particle FP32 records of the mixed passes
+    long long* part = nullptr; size_t part_bytes = 0;  // per-workgroup partials of reductions
+    void* posh = nullptr; size_t posh_bytes = 0;  // (x, y, z, h_search) records of the mixed gather
     // reductions
     double* red = nullptr; size_t red_bytes = 0;
     int64_t* scal = nullptr; size_t scal_bytes = 0;
@@ -49,6 +69,10 @@ struct nb_ctx {
     int lists_ready = 0;
     int hydro_ready = 0;
     int hydro_mixed_ready = 0;
+    int order_ident = 0;  // last grid: particles already stored in cell order (no order array)
+    // settings of the last mixed density pass, for its emission pass
+    int32_t mix_kid = 0;
+    double mix_cnorm = 0.0, mix_m_uniform = 0.0;
     int64_t last_grid_cells = 0;
     double last_cell = 0.0;
 };
@@ -57,3 +81,16 @@ int32_t ctx_fail(nb_ctx* ctx, const char* msg);
 int32_t ctx_fail_code(nb_ctx* ctx, int32_t code, const char* msg);
 // grow *p to at least `bytes` (contents are not kept); nonzero on failure
 int ctx_reserve(nb_ctx* ctx, void** p, size_t* have, size_t bytes);
+
+// Kernel timing: with ctx->timing set, events around a launch (same stream); their durations are
+// added to ctx->ktimes when the events are read (ktime_flush, or when many are open).
+int ktime_begin(nb_ctx* ctx, hipStream_t s, const char* name);
+void ktime_end(nb_ctx* ctx, hipStream_t s, int slot);
+void ktime_flush(nb_ctx* ctx);
+
+#define NB_TIMED(CTX, STREAM, NAME, ...)                      \
+    do {                                                      \
+        const int ts_ = ktime_begin((CTX), (STREAM), (NAME)); \
+        hipLaunchKernelGGL(__VA_ARGS__);                      \
+        ktime_end((CTX), (STREAM), ts_);                      \
+    } while (0)

@@ -50,7 +50,7 @@ k_density64(const double* __restrict__ pos, const double* __restrict__ mass, dou
             double* __restrict__ h_out, double* __restrict__ rho, double* __restrict__ omega,
             int32_t* __restrict__ iters, int32_t* __restrict__ flags)
 {
-    int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t q = xcd_tile() * (int64_t)blockDim.x + threadIdx.x;
     if (q >= nrows) return;
     const int64_t i = row_of(rows, q);
     const double m_i = mass[i];
@@ -131,7 +131,7 @@ k_forces64(const double* __restrict__ pos, const double* __restrict__ vel, const
            double* __restrict__ acc, double* __restrict__ dudt, double* __restrict__ vsig_max,
            double* __restrict__ vheat, double* __restrict__ cheat)
 {
-    int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t q = xcd_tile() * (int64_t)blockDim.x + threadIdx.x;
     if (q >= nrows) return;
     const int64_t a = row_of(rows, q);
     const int kid = P.kid;
@@ -255,7 +255,7 @@ k_gradients64(const double* __restrict__ pos, const double* __restrict__ vel, co
               double* __restrict__ div, double* __restrict__ curl, double* __restrict__ ddiv,
               double* __restrict__ gv_out)
 {
-    int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t q = xcd_tile() * (int64_t)blockDim.x + threadIdx.x;
     if (q >= nrows) return;
     const int64_t a = row_of(rows, q);
     double rmat[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -318,7 +318,7 @@ k_ns_gradients64(const double* __restrict__ pos, const double* __restrict__ vel,
                  L lists, Box box,
                  const int64_t* __restrict__ rows, int64_t nrows, int kid, double cnorm, double* __restrict__ out)
 {
-    int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t q = xcd_tile() * (int64_t)blockDim.x + threadIdx.x;
     if (q >= nrows) return;
     const int64_t a = row_of(rows, q);
     double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -347,7 +347,7 @@ k_ns_forces64(const double* __restrict__ pos, const double* __restrict__ mass, c
               L lists, Box box,
               const int64_t* __restrict__ rows, int64_t nrows, int kid, double cnorm, double* __restrict__ acc)
 {
-    int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t q = xcd_tile() * (int64_t)blockDim.x + threadIdx.x;
     if (q >= nrows) return;
     const int64_t a = row_of(rows, q);
     const double ia = 1.0 / (omega[a] * rho[a] * rho[a]);
@@ -379,7 +379,7 @@ k_disc64(const double* __restrict__ pos, const double* __restrict__ vel, const d
          const double* __restrict__ c_s, double alpha_av, L lists, Box box, const int64_t* __restrict__ rows, int64_t nrows,
          int kid, double cnorm, double* __restrict__ acc, double* __restrict__ dudt)
 {
-    int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t q = xcd_tile() * (int64_t)blockDim.x + threadIdx.x;
     if (q >= nrows) return;
     const int64_t a = row_of(rows, q);
     const double ia = 1.0 / (omega[a] * rho[a] * rho[a]);
@@ -438,6 +438,7 @@ static int32_t launch_check(nb_ctx* ctx, const char* what)
 
 #define NB_DISPATCH(L, KERNEL, GRID, STREAM, ...)                                                       \
     do {                                                                                                \
+        const int ts_ = ktime_begin(ctx, (STREAM), #KERNEL);                                            \
         if ((L)->length && (L)->row_major) {                                                            \
             RowView v_{(L)->index, (L)->length, (L)->row_cap};                                          \
             hipLaunchKernelGGL(KERNEL<RowView>, dim3(GRID), dim3(kBlockP), 0, STREAM, __VA_ARGS__);      \
@@ -448,6 +449,7 @@ static int32_t launch_check(nb_ctx* ctx, const char* what)
             CsrView v_{(L)->offset, (L)->index};                                                        \
             hipLaunchKernelGGL(KERNEL<CsrView>, dim3(GRID), dim3(kBlockP), 0, STREAM, __VA_ARGS__);      \
         }                                                                                               \
+        ktime_end(ctx, (STREAM), ts_);                                                                  \
     } while (0)
 
 static bool lists_ok(const nb_lists* L) { return L && L->index && (L->length || L->offset); }

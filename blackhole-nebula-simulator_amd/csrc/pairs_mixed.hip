@@ -116,7 +116,7 @@ k_forces_mixed(const double* __restrict__ pos, const PF* __restrict__ pf, const 
                double* __restrict__ cheat)
 {
     const int glane = threadIdx.x & (kGM - 1);
-    const int64_t q = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kGM;
+    const int64_t q = (xcd_tile() * (int64_t)blockDim.x + threadIdx.x) / kGM;
     if (q >= nrows) return;  // whole groups leave together
     const int64_t a = row_of(rows, q);
     const double pa0 = pos[3 * a], pa1 = pos[3 * a + 1], pa2 = pos[3 * a + 2];
@@ -185,7 +185,7 @@ k_gradients_mixed(const double* __restrict__ pos, const double* __restrict__ acc
                   double* __restrict__ curl, double* __restrict__ ddiv)
 {
     const int glane = threadIdx.x & (kGM - 1);
-    const int64_t q = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kGM;
+    const int64_t q = (xcd_tile() * (int64_t)blockDim.x + threadIdx.x) / kGM;
     if (q >= nrows) return;
     const int64_t a = row_of(rows, q);
     const double pa0 = pos[3 * a], pa1 = pos[3 * a + 1], pa2 = pos[3 * a + 2];
@@ -266,6 +266,7 @@ static inline Box box_of(const double* box3)
 
 #define NB_DISPATCH_M(L, KERNEL, GRID, STREAM, ...)                                                     \
     do {                                                                                                \
+        const int ts_ = ktime_begin(ctx, (STREAM), #KERNEL);                                            \
         if ((L)->length && (L)->row_major) {                                                            \
             RowView v_{(L)->index, (L)->length, (L)->row_cap};                                          \
             hipLaunchKernelGGL(KERNEL<RowView>, dim3(GRID), dim3(kBlockM), 0, STREAM, __VA_ARGS__);      \
@@ -276,6 +277,7 @@ static inline Box box_of(const double* box3)
             CsrView v_{(L)->offset, (L)->index};                                                        \
             hipLaunchKernelGGL(KERNEL<CsrView>, dim3(GRID), dim3(kBlockM), 0, STREAM, __VA_ARGS__);      \
         }                                                                                               \
+        ktime_end(ctx, (STREAM), ts_);                                                                  \
     } while (0)
 
 static int32_t launched(nb_ctx* ctx, const char* what)

@@ -110,7 +110,9 @@ SIGNATURES = {
     "nb_hydro_tight": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _P64]),
     "nb_hydro_density_mixed": (_i32, [_vp, _vp, _vp, _vp, _f64, _vp, _vp, _i64, _i32, _f64, _f64, _f64, _f64, _vp,
                                       _vp, _vp, _vp, _vp, _P64]),
-    "nb_hydro_tight_mixed": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _P64]),
+    "nb_hydro_tight_mixed": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _P64]),
+    "nb_timing": (_i32, [_vp, _i32]),
+    "nb_timing_report": (_i64, [_vp, ctypes.c_char_p, _i64]),
     "nb_gradients_mixed": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _PL, _PD, _vp, _i64, _i32, _f64, _vp, _vp,
                                   _vp]),
 }
@@ -221,6 +223,25 @@ def stop_timing() -> dict:
     torch.cuda.synchronize()
     out = {k: (len(v), sum(a.elapsed_time(b) for a, b in v)) for k, v in (_TIMING or {}).items()}
     _TIMING = None
+    return out
+
+
+def start_kernel_timing() -> None:
+    """Have the library time its main kernels with HIP events on their own stream (clears)."""
+    check(lib().nb_timing(ctx(), 1), "nb_timing")
+
+
+def stop_kernel_timing() -> dict:
+    """Stop kernel timing; returns kernel name -> (launches, total milliseconds)."""
+    L = lib()
+    need = int(L.nb_timing_report(ctx(), None, 0))
+    buf = ctypes.create_string_buffer(max(need, 1))
+    L.nb_timing_report(ctx(), buf, need)
+    check(L.nb_timing(ctx(), 0), "nb_timing")
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, launches, ms = line.split()
+        out[name] = (int(launches), float(ms))
     return out
 
 

@@ -210,8 +210,8 @@ def hydro_density_mixed_device(pos_d, mass_d, h_guess_d, kernel_name, eta, tol, 
     ln = _dev.empty(n, torch.int32)
     idx = _dev.empty(n * cap, torch.int32)
     total = np.zeros(1, np.int64)
-    _hip.call("nb_hydro_tight_mixed", _hip.ptr(pos_d), _hip.ptr(h), _hip.ptr(ln), _hip.ptr(idx), cap,
-              total.ctypes.data_as(_hip._P64))  # fmt: skip
+    _hip.call("nb_hydro_tight_mixed", _hip.ptr(pos_d), _hip.ptr(h), _hip.ptr(rho), _hip.ptr(omega), _hip.ptr(ln),
+              _hip.ptr(idx), cap, total.ctypes.data_as(_hip._P64))  # fmt: skip
     tight = RowLists(ln, idx, cap, int(total[0]), None)
     return DensityResult(h, rho, omega, iters, flags, tight, None, builds), int(stats[2])
 

@@ -77,6 +77,13 @@ const char *nb_last_error(const nb_ctx *ctx);
 int32_t nb_abi_version(void);
 /* bytes of scratch the context currently holds */
 int64_t nb_ctx_scratch_bytes(const nb_ctx *ctx);
+/* Kernel timing (no counterpart in the reference; for benchmarks): with enable != 0 the library
+ * records HIP events around its main kernels, on the stream each runs on, and accumulates their
+ * durations; every call clears the totals. */
+int32_t nb_timing(nb_ctx *ctx, int32_t enable);
+/* The accumulated kernel times as "kernel launches total_ms" lines in buf (cap bytes,
+ * NUL-terminated; waits for the pending events). Returns the bytes the full report needs. */
+int64_t nb_timing_report(nb_ctx *ctx, char *buf, int64_t cap);
 
 /* ------------------------------------------------------------------ neighbour lists */
 /* find_neighbours_numba (neighbours.py:240-281): symmetric lists {j : |x_i - x_j| <
@@ -141,17 +148,22 @@ int32_t nb_hydro_tight(nb_ctx *ctx, void *stream, const double *h_final, int32_t
 /* The density pass of coupled.step for the mixed-precision mode (open box): the same search
  * lists as nb_hydro_density (FP64 predicate, exact), gathered by 16-lane groups into row-major
  * rows (entry k of row i at i * cap + k) with FP32 d2; the h-rho solve sums in FP32 with
- * compensated accumulation and takes its Newton steps in FP64. stats4 (host) = [row capacity
- * used, rows solved from memory, search-list entries, longest merged row]. */
+ * compensated accumulation and takes its Newton steps in FP64. Writes h_out, iters and flags of
+ * every row; rho and omega of the rows that needed more than one Newton step (the others are
+ * completed by nb_hydro_tight_mixed, which evaluates them at the final h while it reads the
+ * rows). stats4 (host) = [row capacity used, rows needing more than one step, search-list
+ * entries, longest merged row]. */
 int32_t nb_hydro_density_mixed(nb_ctx *ctx, void *stream, const double *pos, const double *mass, double m_uniform,
                                const double *h_start, const double *h_search, int64_t n, int32_t kid, double cnorm,
                                double eta, double tol, double h_max, double *h_out, double *rho, double *omega,
                                int32_t *iters, int32_t *flags, int64_t *stats4);
-/* tighten (neighbours.py:306-323) of the last nb_hydro_density_mixed lists at h_final, into
- * row-major rows of capacity row_cap (>= the longest merged row); pos as in the density pass.
- * Rows are exactly tighten() of the search lists (borderline FP32 d2 re-formed in FP64). */
-int32_t nb_hydro_tight_mixed(nb_ctx *ctx, void *stream, const double *pos, const double *h_final, int32_t *out_len,
-                             int32_t *out_index, int64_t row_cap, int64_t *total_out);
+/* tighten (neighbours.py:306-323) of the last nb_hydro_density_mixed lists at h_final (its
+ * h_out), into row-major rows of capacity row_cap (>= the longest merged row); pos as in the
+ * density pass. Rows are exactly tighten() of the search lists (borderline FP32 d2 re-formed in
+ * FP64). Completes rho and omega (the same arrays given to the density pass). */
+int32_t nb_hydro_tight_mixed(nb_ctx *ctx, void *stream, const double *pos, const double *h_final, double *rho,
+                             double *omega, int32_t *out_len, int32_t *out_index, int64_t row_cap,
+                             int64_t *total_out);
 /* gradients_numba (sph.py:837-884) in mixed precision: FP32 pair terms over 16-lane row
  * groups, compensated sums, partials combined and the 3x3 inverse taken in FP64. Arguments as
  * nb_gradients. (The force pass takes its precision from nb_force_params.precision.) */

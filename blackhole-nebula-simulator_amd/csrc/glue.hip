@@ -141,12 +141,19 @@ __global__ void k_sum_partial(const double* __restrict__ x, int64_t n, double* _
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-__global__ void k_sum_into(const double* __restrict__ part, int nparts, double* __restrict__ acc)
+// acc[0] += the sum of the partials (one workgroup, fixed tree order)
+__global__ void __launch_bounds__(kBlockG) k_sum_into(const double* __restrict__ part, int nparts, double* __restrict__ acc)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ double sh[kBlockG];
     double s = 0.0;
-    for (int b = 0; b < nparts; b++) s += part[b];
-    acc[0] += s;
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) s += part[b];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) acc[0] += sh[0];
 }
 
 static inline unsigned grid_g(int64_t n) { return (unsigned)((n + kBlockG - 1) / kBlockG); }
@@ -262,6 +269,6 @@ extern "C" int32_t nb_sum_into(nb_ctx* ctx, void* stream, const double* x, int64
     int nparts = (int)std::min<int64_t>(1024, (n + kBlockG - 1) / kBlockG);
     if (ctx_reserve(ctx, (void**)&ctx->red, &ctx->red_bytes, sizeof(double) * (nparts + 8))) return NB_ERR_HIP;
     hipLaunchKernelGGL(k_sum_partial, dim3(nparts), dim3(kBlockG), 0, s, x, n, ctx->red);
-    hipLaunchKernelGGL(k_sum_into, dim3(1), dim3(1), 0, s, ctx->red, nparts, acc_dev);
+    hipLaunchKernelGGL(k_sum_into, dim3(1), dim3(kBlockG), 0, s, ctx->red, nparts, acc_dev);
     return launched(ctx, "nb_sum_into");
 }

@@ -178,7 +178,8 @@ k_cell_id(const double* __restrict__ pos, int64_t n, Box box, Grid g, uint32_t* 
 }
 
 // Sum (bit s of maxmask clear) or max (set) of the per-workgroup partials of nslots slots
-// (slot s at part[s * nb + b]) into out[s]; one workgroup.
+// (slot s at part[s * nb + b]) into out[s] (zeroed beforehand): each workgroup reduces a strided
+// share in LDS and adds its result with one atomic per slot (a few hundred at most).
 __global__ void __launch_bounds__(kBlock)
 k_partials_final(const long long* __restrict__ part, int64_t nb, int nslots, unsigned maxmask, long long* out)
 {
@@ -186,7 +187,7 @@ k_partials_final(const long long* __restrict__ part, int64_t nb, int nslots, uns
     for (int sl = 0; sl < nslots; sl++) {
         const bool mx = (maxmask >> sl) & 1u;
         long long v = 0;
-        for (int64_t b = threadIdx.x; b < nb; b += blockDim.x) {
+        for (int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; b < nb; b += (int64_t)gridDim.x * blockDim.x) {
             const long long t = part[sl * nb + b];
             v = mx ? (t > v ? t : v) : v + t;
         }
@@ -199,7 +200,10 @@ k_partials_final(const long long* __restrict__ part, int64_t nb, int nslots, uns
             }
             __syncthreads();
         }
-        if (threadIdx.x == 0) out[sl] = sh[0];
+        if (threadIdx.x == 0) {
+            if (mx) atomicMax(&out[sl], sh[0]);
+            else atomicAdd((unsigned long long*)&out[sl], (unsigned long long)sh[0]);
+        }
         __syncthreads();
     }
 }
@@ -492,6 +496,31 @@ __global__ void k_permute_i64(const int64_t* __restrict__ src, const int32_t* __
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i < n) dst[i] = src[order[i]];
+}
+
+// several arrays of 8-byte values at once (bit copies: doubles or int64)
+static constexpr int kMaxPermute = 12;
+struct PermSet {
+    const uint64_t* src[kMaxPermute];
+    uint64_t* dst[kMaxPermute];
+    int32_t width[kMaxPermute];
+    int32_t count;
+    int32_t total;  // sum of widths
+};
+
+__global__ void k_permute_many(PermSet P, const int32_t* __restrict__ order, int64_t n)
+{
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= n * P.total) return;
+    const int64_t i = t / P.total;
+    int c = (int)(t - i * P.total);
+    int k = 0;
+    while (c >= P.width[k]) {
+        c -= P.width[k];
+        k++;
+    }
+    const int w = P.width[k];
+    P.dst[k][i * w + c] = P.src[k][(int64_t)order[i] * w + c];
 }
 
 // ----------------------------------------------------------------- fused step passes
@@ -935,12 +964,15 @@ __device__ __forceinline__ bool column_zrange(double ux, double uy, double zc, d
     return true;
 }
 
+// Extras (pairs seen only from the other side, d2 >= (2 h_search_j)^2) go straight to row j's
+// extras segment (capacity ecap, filled with an atomic counter efill[j]).
 template <bool kMasses>
 __global__ void __launch_bounds__(256)
 k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g, const uint32_t* __restrict__ start,
                const uint32_t* __restrict__ order, uint32_t* __restrict__ ridx, float* __restrict__ rd2,
                float* __restrict__ rmass, int64_t cap, int32_t* __restrict__ rowcnt,
-               unsigned long long* __restrict__ ecount, MixArgs A)
+               unsigned int* __restrict__ efill, uint32_t* __restrict__ ebuf, float* __restrict__ ed2, int64_t ecap,
+               MixArgs A)
 {
     const int glane = threadIdx.x & (kG - 1);
     const int gshift = (threadIdx.x & 63) & ~(kG - 1);
@@ -963,6 +995,7 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g, const uint32
     const float mu = (float)(A.m_uniform > 0.0 ? A.m_uniform : 1.0);
     DensSums sums;
     int32_t m = 0;
+    unsigned int emax = 0;
     uint32_t* row = ridx + ii * cap;
     float* drow = rd2 + ii * cap;
     float* mrow = kMasses ? rmass + ii * cap : nullptr;
@@ -1028,10 +1061,17 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g, const uint32
             const unsigned gb = (unsigned)((bal >> gshift) & ((1ull << kG) - 1));
             if (pred) {
                 const int slot = m + __popc(gb & ((1u << glane) - 1));
-                if (flag) atomicAdd(&ecount[j], 1ull);
                 const float d2f = (float)d2;
+                if (flag) {
+                    const unsigned int es = atomicAdd(&efill[j], 1u);
+                    if (es < ecap) {
+                        ebuf[(int64_t)j * ecap + es] = (uint32_t)i;
+                        ed2[(int64_t)j * ecap + es] = d2f;
+                    }
+                    emax = es + 1 > emax ? es + 1 : emax;
+                }
                 if (slot < cap) {
-                    row[slot] = flag ? (j | kFlag) : j;
+                    row[slot] = j;
                     drow[slot] = d2f;
                     if (kMasses) mrow[slot] = mj;
                 }
@@ -1059,6 +1099,7 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g, const uint32
     const bool fast = m <= cap && fabs(h_new - h0) < A.tol * h0 && !(h_new >= r_lim * (1 - 1e-3));
     block_partial<true>(live ? m : 0, A.part);
     block_partial<false>(live && glane == 0 && !fast ? 1 : 0, A.part + gridDim.x);
+    block_partial<true>(emax, A.part + 2 * gridDim.x);
     if (live && glane == 0) {
         A.slow[i] = fast ? 0 : 1;
         if (fast) {
@@ -1141,25 +1182,17 @@ k_solve_slow_mixed(const double* __restrict__ h_search, int64_t n, const float* 
     }
 }
 
-// extras of the row-major gather rows (a group reads its row, flagged entries are transposed)
-__global__ void __launch_bounds__(256)
-k_extras_fill_rm(const uint32_t* __restrict__ ridx, const float* __restrict__ rd2, int64_t cap,
-                 const int32_t* __restrict__ rowcnt, const unsigned long long* __restrict__ eoff,
-                 unsigned int* __restrict__ efill, uint32_t* __restrict__ ebuf, float* __restrict__ ed2, int64_t n)
+// final row lengths of the mixed lists (gather row + extras): partials of their largest value
+// and of their sum (the search-list entries)
+__global__ void __launch_bounds__(kBlock)
+k_row_total_rm(const int32_t* __restrict__ rowcnt, const unsigned int* __restrict__ efill, int64_t ecap,
+               long long* __restrict__ part, int64_t n)
 {
-    const int glane = threadIdx.x & (kG - 1);
-    const int64_t i = (xcd_tile() * (int64_t)blockDim.x + threadIdx.x) / kG;
-    if (i >= n) return;
-    const int32_t m = rowcnt[i];
-    for (int32_t k = glane; k < m; k += kG) {
-        const uint32_t v = ridx[i * cap + k];
-        if (v & kFlag) {
-            const uint32_t j = v & kMask;
-            const unsigned int s = atomicAdd(&efill[j], 1u);
-            ebuf[eoff[j] + s] = (uint32_t)i;
-            ed2[eoff[j] + s] = rd2[i * cap + k];
-        }
-    }
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    long long t = 0;
+    if (i < n) t = (long long)rowcnt[i] + (long long)(efill[i] < ecap ? efill[i] : ecap);
+    block_partial<true>(t, part);
+    block_partial<false>(t, part + gridDim.x);
 }
 
 // Extras arrive in atomic order: each row's segment is put in index order (so that the tightened
@@ -1169,15 +1202,15 @@ k_extras_fill_rm(const uint32_t* __restrict__ ridx, const float* __restrict__ rd
 static constexpr int kSortRegs = 4;
 
 __global__ void __launch_bounds__(256)
-k_sort_extras(uint32_t* __restrict__ ebuf, float* __restrict__ ed2, const unsigned long long* __restrict__ eoff,
-              int64_t n)
+k_sort_extras(uint32_t* __restrict__ ebuf, float* __restrict__ ed2, const unsigned int* __restrict__ efill,
+              int64_t ecap, int64_t n)
 {
     const int glane = threadIdx.x & (kG - 1);
     const int64_t i = (xcd_tile() * (int64_t)blockDim.x + threadIdx.x) / kG;
     if (i >= n) return;  // whole groups leave together
-    uint32_t* e = ebuf + eoff[i];
-    float* d = ed2 + eoff[i];
-    const int64_t ne = (int64_t)(eoff[i + 1] - eoff[i]);
+    uint32_t* e = ebuf + i * ecap;
+    float* d = ed2 + i * ecap;
+    const int64_t ne = efill[i] < ecap ? efill[i] : ecap;
     if (ne <= 1) return;
     if (ne <= kG * kSortRegs) {
         uint32_t v[kSortRegs];
@@ -1240,7 +1273,7 @@ struct FinArgs {
 // their d2 >= (2 h_search_i)^2 >= (2 h_i)^2).
 __global__ void __launch_bounds__(256)
 k_tight_emit_rm(const double* __restrict__ pos, const uint32_t* __restrict__ ridx, const float* __restrict__ rd2,
-                int64_t cap, const int32_t* __restrict__ rowcnt, const unsigned long long* __restrict__ eoff,
+                int64_t cap, const int32_t* __restrict__ rowcnt, const unsigned int* __restrict__ efill, int64_t ecap,
                 const uint32_t* __restrict__ ebuf, const float* __restrict__ ed2, const double* __restrict__ hf,
                 int64_t n, int32_t* __restrict__ out_len, int32_t* __restrict__ out_idx, int64_t cap_out,
                 FinArgs F, long long* __restrict__ part)
@@ -1252,7 +1285,7 @@ k_tight_emit_rm(const double* __restrict__ pos, const uint32_t* __restrict__ rid
     const int64_t ii = live ? i : n - 1;
     const double hi = hf[ii];
     const int32_t ng = live ? rowcnt[ii] : 0;
-    const int64_t e0 = (int64_t)eoff[ii], ne = live ? (int64_t)(eoff[ii + 1] - eoff[ii]) : 0;
+    const int64_t e0 = ii * ecap, ne = live ? (int64_t)(efill[ii] < ecap ? efill[ii] : ecap) : 0;
     const int64_t len = ng + ne;
     const bool fin = live && !F.slow[ii];
     const float inv_hf = (float)(1.0 / hi);
@@ -1268,7 +1301,7 @@ k_tight_emit_rm(const double* __restrict__ pos, const uint32_t* __restrict__ rid
         if (in_range && k < len) {
             float d2;
             if (k < ng) {
-                j = ridx[ii * cap + k] & kMask;
+                j = ridx[ii * cap + k];
                 d2 = rd2[ii * cap + k];
                 if (fin) sums.add(__fsqrt_rn(d2) * inv_hf, F.rmass ? F.rmass[ii * cap + k] : mu, F.kid);
             } else {
@@ -1363,8 +1396,11 @@ int reserve_reductions(nb_ctx* ctx, int64_t n)
 
 inline void finish_partials(nb_ctx* ctx, hipStream_t s, int64_t nb, int nslots, unsigned maxmask, int at)
 {
-    hipLaunchKernelGGL(k_partials_final, dim3(1), dim3(kBlock), 0, s, ctx->part, nb, nslots, maxmask,
-                       (long long*)ctx->scal + at);
+    long long* out = (long long*)ctx->scal + at;
+    hipMemsetAsync(out, 0, sizeof(long long) * nslots, s);
+    const int64_t blocks = std::min<int64_t>(256, (nb + 4 * kBlock - 1) / (4 * kBlock));
+    hipLaunchKernelGGL(k_partials_final, dim3((unsigned)blocks), dim3(kBlock), 0, s, ctx->part, nb, nslots, maxmask,
+                       out);
 }
 
 // The particle order of the last grid for the gather kernels: NULL when the particles were stored
@@ -1697,6 +1733,27 @@ extern "C" int32_t nb_permute(nb_ctx* ctx, void* stream, const double* src, cons
     return NB_OK;
 }
 
+extern "C" int32_t nb_permute_many(nb_ctx* ctx, void* stream, int32_t count, const void* const* src, void* const* dst,
+                                   const int32_t* width, const int32_t* order, int64_t n)
+{
+    if (!ctx || count < 0 || count > kMaxPermute || n < 0 || !order || (count && (!src || !dst || !width)))
+        return ctx_fail(ctx, "nb_permute_many: bad arguments");
+    PermSet P{};
+    P.count = count;
+    P.total = 0;
+    for (int k = 0; k < count; k++) {
+        if (!src[k] || !dst[k] || src[k] == dst[k] || width[k] <= 0) return ctx_fail(ctx, "nb_permute_many: bad array");
+        P.src[k] = (const uint64_t*)src[k];
+        P.dst[k] = (uint64_t*)dst[k];
+        P.width[k] = width[k];
+        P.total += width[k];
+    }
+    if (n == 0 || count == 0) return NB_OK;
+    hipLaunchKernelGGL(k_permute_many, dim3(grid_of(n * P.total)), dim3(kBlock), 0, (hipStream_t)stream, P, order, n);
+    if (hipGetLastError() != hipSuccess) return ctx_fail(ctx, "nb_permute_many: kernel launch failed");
+    return NB_OK;
+}
+
 extern "C" int32_t nb_permute_i64(nb_ctx* ctx, void* stream, const int64_t* src, const int32_t* order, int64_t n,
                                   int64_t* dst)
 {
@@ -1842,43 +1899,45 @@ extern "C" int32_t nb_hydro_density_mixed(nb_ctx* ctx, void* stream, const doubl
     if (ctx_reserve(ctx, (void**)&ctx->rowcnt, &ctx->rowcnt_bytes, sizeof(int32_t) * n) ||
         ctx_reserve(ctx, (void**)&ctx->pending, &ctx->pending_bytes, n) ||
         ctx_reserve(ctx, (void**)&ctx->posh, &ctx->posh_bytes, sizeof(double4) * n) ||
-        ctx_reserve(ctx, (void**)&ctx->ecount, &ctx->ecount_bytes, sizeof(unsigned long long) * (n + 1)) ||
-        ctx_reserve(ctx, (void**)&ctx->eoff, &ctx->eoff_bytes, sizeof(unsigned long long) * (n + 1)) ||
-        ctx_reserve(ctx, (void**)&ctx->tot, &ctx->tot_bytes, sizeof(unsigned long long) * (n + 1)) ||
-        ctx_reserve(ctx, (void**)&ctx->offs, &ctx->offs_bytes, sizeof(unsigned long long) * (n + 1)) ||
         ctx_reserve(ctx, (void**)&ctx->efill, &ctx->efill_bytes, sizeof(unsigned int) * n))
         return NB_ERR_HIP;
     NB_TIMED(ctx, s, "k_pack_posh", k_pack_posh, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, h_search, n,
              (double4*)ctx->posh);
     int64_t cap = ctx->row_cap_rm > 0 ? ctx->row_cap_rm : 192;
+    int64_t ecap = ctx->ecap_rm > 0 ? ctx->ecap_rm : 32;
     long long sc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     MixArgs A{mass, m_uniform, h_start, h_max, kid, cnorm, eta, tol, h_out, rho, omega, iters, flags, ctx->pending,
               ctx->part};
     const int64_t blocks = (n * kG + 255) / 256;
     for (int attempt = 0; attempt < 8; attempt++) {
         const size_t cells = (size_t)(n * cap);
+        const size_t ecells = (size_t)(n * ecap);
         if (ctx_reserve(ctx, (void**)&ctx->gbuf, &ctx->gbuf_bytes, sizeof(uint32_t) * cells) ||
-            ctx_reserve(ctx, (void**)&ctx->fscratch, &ctx->fscratch_bytes, sizeof(float) * cells * (masses ? 2 : 1)))
+            ctx_reserve(ctx, (void**)&ctx->fscratch, &ctx->fscratch_bytes, sizeof(float) * cells * (masses ? 2 : 1)) ||
+            ctx_reserve(ctx, (void**)&ctx->ebuf, &ctx->ebuf_bytes, sizeof(uint32_t) * ecells) ||
+            ctx_reserve(ctx, (void**)&ctx->edbuf, &ctx->edbuf_bytes, sizeof(float) * ecells))
             return NB_ERR_HIP;
-        if (hipMemsetAsync(ctx->ecount, 0, sizeof(unsigned long long) * (n + 1), s) != hipSuccess)
-            return ctx_fail(ctx, "memset");
+        if (hipMemsetAsync(ctx->efill, 0, sizeof(unsigned int) * n, s) != hipSuccess) return ctx_fail(ctx, "memset");
         float* rmass = masses ? ctx->fscratch + cells : nullptr;
         if (masses)
             NB_TIMED(ctx, s, "k_gather_mixed", k_gather_mixed<true>, dim3(blocks), dim3(256), 0, s,
                      (const double4*)ctx->posh, n, g, ctx->cell_start, order_arg(ctx), ctx->gbuf, ctx->fscratch, rmass,
-                     cap, ctx->rowcnt, ctx->ecount, A);
+                     cap, ctx->rowcnt, ctx->efill, ctx->ebuf, (float*)ctx->edbuf, ecap, A);
         else
             NB_TIMED(ctx, s, "k_gather_mixed", k_gather_mixed<false>, dim3(blocks), dim3(256), 0, s,
                      (const double4*)ctx->posh, n, g, ctx->cell_start, order_arg(ctx), ctx->gbuf, ctx->fscratch, rmass,
-                     cap, ctx->rowcnt, ctx->ecount, A);
-        finish_partials(ctx, s, blocks, 2, 1u, 0);  // [0] longest gather row, [1] rows left slow
-        rc = sync_copy(ctx, s, sc, ctx->scal, sizeof(long long) * 2, "nb_hydro_density_mixed: gather failed");
+                     cap, ctx->rowcnt, ctx->efill, ctx->ebuf, (float*)ctx->edbuf, ecap, A);
+        // [0] longest gather row, [1] rows left slow, [2] longest extras segment
+        finish_partials(ctx, s, blocks, 3, 5u, 0);
+        rc = sync_copy(ctx, s, sc, ctx->scal, sizeof(long long) * 3, "nb_hydro_density_mixed: gather failed");
         if (rc) return rc;
-        if (sc[0] <= cap) break;
-        cap = ((sc[0] + sc[0] / 8 + 15) / 16) * 16;
+        if (sc[0] <= cap && sc[2] <= ecap) break;
+        if (sc[0] > cap) cap = ((sc[0] + sc[0] / 8 + 15) / 16) * 16;
+        if (sc[2] > ecap) ecap = ((sc[2] + sc[2] / 8 + 15) / 16) * 16;
         if (attempt == 7) return ctx_fail(ctx, "row capacity did not settle");
     }
     ctx->row_cap_rm = (int32_t)cap;
+    ctx->ecap_rm = ecap;
     const long long slow_rows = sc[1];
     const size_t cells = (size_t)(n * cap);
     float* rmass = masses ? ctx->fscratch + cells : nullptr;
@@ -1890,28 +1949,17 @@ extern "C" int32_t nb_hydro_density_mixed(nb_ctx* ctx, void* stream, const doubl
             NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<false>, dim3(blocks), dim3(256), 0, s, h_search,
                      n, ctx->fscratch, rmass, cap, ctx->rowcnt, A);
     }
-    NB_TIMED(ctx, s, "k_row_total", k_row_total, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, ctx->rowcnt, ctx->ecount,
-             ctx->tot, ctx->part, n);
-    finish_partials(ctx, s, grid_of(n + 1), 1, 1u, 2);  // [2] longest merged row
-    rc = scan_u64(ctx, s, ctx->ecount, ctx->eoff, n + 1);
-    if (rc) return rc;
-    rc = scan_u64(ctx, s, ctx->tot, ctx->offs, n + 1);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_collect, dim3(1), dim3(64), 0, s, (long long*)ctx->scal + 3, ctx->eoff + n, ctx->offs + n);
-    rc = sync_copy(ctx, s, sc, ctx->scal, sizeof(long long) * 5, "nb_hydro_density_mixed: totals copy failed");
-    if (rc) return rc;
-    const unsigned long long ne_total = (unsigned long long)sc[3];
-    if (ctx_reserve(ctx, (void**)&ctx->ebuf, &ctx->ebuf_bytes, sizeof(uint32_t) * (ne_total + 1)) ||
-        ctx_reserve(ctx, (void**)&ctx->edbuf, &ctx->edbuf_bytes, sizeof(float) * (ne_total + 1)))
-        return NB_ERR_HIP;
-    if (hipMemsetAsync(ctx->efill, 0, sizeof(unsigned int) * n, s) != hipSuccess) return ctx_fail(ctx, "memset");
-    NB_TIMED(ctx, s, "k_extras_fill_rm", k_extras_fill_rm, dim3(blocks), dim3(256), 0, s, ctx->gbuf, ctx->fscratch, cap,
-             ctx->rowcnt, ctx->eoff, ctx->efill, ctx->ebuf, (float*)ctx->edbuf, n);
     NB_TIMED(ctx, s, "k_sort_extras", k_sort_extras, dim3(blocks), dim3(256), 0, s, ctx->ebuf, (float*)ctx->edbuf,
-             ctx->eoff, n);
+             ctx->efill, ecap, n);
+    NB_TIMED(ctx, s, "k_row_total_rm", k_row_total_rm, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->rowcnt, ctx->efill,
+             ecap, ctx->part, n);
+    finish_partials(ctx, s, grid_of(n), 2, 1u, 3);  // [3] longest merged row, [4] search-list entries
+    rc = sync_copy(ctx, s, sc + 3, (long long*)ctx->scal + 3, sizeof(long long) * 2,
+                   "nb_hydro_density_mixed: totals copy failed");
+    if (rc) return rc;
     if (hipGetLastError() != hipSuccess) return ctx_fail(ctx, "nb_hydro_density_mixed: kernel launch failed");
     ctx->lists_n = n;
-    ctx->lists_maxrow = sc[2];
+    ctx->lists_maxrow = sc[3];
     ctx->hydro_mixed_ready = 1;
     ctx->mix_kid = kid;
     ctx->mix_cnorm = cnorm;
@@ -1919,7 +1967,7 @@ extern "C" int32_t nb_hydro_density_mixed(nb_ctx* ctx, void* stream, const doubl
     stats4[0] = cap;
     stats4[1] = slow_rows;  // rows not converged in one Newton step (solved in full from the stored row)
     stats4[2] = sc[4];      // search-list entries (incl. self)
-    stats4[3] = sc[2];      // longest merged row
+    stats4[3] = sc[3];      // longest merged row
     return NB_OK;
 }
 
@@ -1940,8 +1988,8 @@ extern "C" int32_t nb_hydro_tight_mixed(nb_ctx* ctx, void* stream, const double*
                     ctx->mix_cnorm, ctx->pending, rho, omega};
     const unsigned blocks = (unsigned)((n * kG + 255) / 256);
     NB_TIMED(ctx, s, "k_tight_emit_rm", k_tight_emit_rm, dim3(blocks), dim3(256), 0, s, pos, ctx->gbuf, ctx->fscratch,
-             cap, ctx->rowcnt, ctx->eoff, ctx->ebuf, (const float*)ctx->edbuf, h_final, n, out_len, out_index, row_cap,
-             F, ctx->part);
+             cap, ctx->rowcnt, ctx->efill, ctx->ecap_rm, ctx->ebuf, (const float*)ctx->edbuf, h_final, n, out_len,
+             out_index, row_cap, F, ctx->part);
     finish_partials(ctx, s, blocks, 1, 0u, 8);
     long long t = 0;
     int rc = sync_copy(ctx, s, &t, (long long*)ctx->scal + 8, sizeof(t), "nb_hydro_tight_mixed: total copy failed");

@@ -65,6 +65,7 @@ struct nb_ctx {
     void* cub_tmp = nullptr; size_t cub_tmp_bytes = 0;
     int32_t row_cap = 0;
     int32_t row_cap_rm = 0;
+    int64_t ecap_rm = 0;  // extras segment capacity of the mixed lists
     int64_t lists_n = 0, lists_total = 0, lists_maxrow = 0, last_unsorted = 0;
     int lists_ready = 0;
     int hydro_ready = 0;

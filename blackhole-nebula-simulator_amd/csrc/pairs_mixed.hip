@@ -177,12 +177,14 @@ k_forces_mixed(const double* __restrict__ pos, const PF* __restrict__ pf, const 
     }
 }
 
-// sph.py:837-884: R, G_v and G_a of a row from FP32 pair terms, inverted in FP64.
-template <class L>
-__global__ void __launch_bounds__(kBlockM)
-k_gradients_mixed(const double* __restrict__ pos, const double* __restrict__ accel, const PG* __restrict__ pg, L lists,
-                  Box box, const int64_t* __restrict__ rows, int64_t nrows, int kid, double* __restrict__ div,
-                  double* __restrict__ curl, double* __restrict__ ddiv)
+// sph.py:837-884: R, G_v and G_a of a row from FP32 pair terms, inverted in FP64 (kAcc false:
+// accelerations zero, G_a not summed).
+template <class L, bool kAcc>
+__device__ __forceinline__ void gradients_mixed_row(const double* __restrict__ pos, const double* __restrict__ accel,
+                                                    const PG* __restrict__ pg, const L& lists, const Box& box,
+                                                    const int64_t* __restrict__ rows, int64_t nrows, int kid,
+                                                    double* __restrict__ div, double* __restrict__ curl,
+                                                    double* __restrict__ ddiv)
 {
     const int glane = threadIdx.x & (kGM - 1);
     const int64_t q = (xcd_tile() * (int64_t)blockDim.x + threadIdx.x) / kGM;
@@ -190,9 +192,9 @@ k_gradients_mixed(const double* __restrict__ pos, const double* __restrict__ acc
     const int64_t a = row_of(rows, q);
     const double pa0 = pos[3 * a], pa1 = pos[3 * a + 1], pa2 = pos[3 * a + 2];
     const PG A = pg[a];
-    Kahan R[9], Gv[9], Ga[9];
+    Kahan R[9], Gv[9], Ga[kAcc ? 9 : 1];
     float aa[3] = {0.f, 0.f, 0.f};
-    if (accel)
+    if (kAcc)
         for (int c = 0; c < 3; c++) aa[c] = (float)accel[3 * a + c];
     int cnt = 0;
     const int64_t m = lists.count(a);
@@ -208,7 +210,7 @@ k_gradients_mixed(const double* __restrict__ pos, const double* __restrict__ acc
         const float g = A.b.x * df32(r2 * inv_r * A.a.w, kid) * inv_r;
         const float dv[3] = {B.a.x - A.a.x, B.a.y - A.a.y, B.a.z - A.a.z};
         float da[3] = {0.f, 0.f, 0.f};
-        if (accel)
+        if (kAcc)
             for (int c = 0; c < 3; c++) da[c] = (float)accel[3 * b + c] - aa[c];
         const float mg = B.b.y * g;
 #pragma unroll
@@ -218,7 +220,7 @@ k_gradients_mixed(const double* __restrict__ pos, const double* __restrict__ acc
                 const float gj = mg * dx[j];
                 R[3 * i + j].add(-(dx[i] * gj));
                 Gv[3 * i + j].add(dv[i] * gj);
-                if (accel) Ga[3 * i + j].add(da[i] * gj);
+                if (kAcc) Ga[kAcc ? 3 * i + j : 0].add(da[i] * gj);
             }
     }
     double rm[9], gvm[9], gam[9];
@@ -226,7 +228,7 @@ k_gradients_mixed(const double* __restrict__ pos, const double* __restrict__ acc
     for (int k = 0; k < 9; k++) {
         rm[k] = gsum(R[k].value());
         gvm[k] = gsum(Gv[k].value());
-        gam[k] = gsum(Ga[k].value());
+        gam[k] = kAcc ? gsum(Ga[kAcc ? k : 0].value()) : 0.0;
     }
     for (int o = kGM / 2; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, kGM);
     if (glane != 0) return;
@@ -251,6 +253,24 @@ k_gradients_mixed(const double* __restrict__ pos, const double* __restrict__ acc
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++) ss += dvm[3 * i + j] * dvm[3 * j + i];
     ddiv[q] = dam[0] + dam[4] + dam[8] - ss;
+}
+
+template <class L>
+__global__ void __launch_bounds__(kBlockM)
+k_gradients_mixed(const double* __restrict__ pos, const double* __restrict__ accel, const PG* __restrict__ pg, L lists,
+                  Box box, const int64_t* __restrict__ rows, int64_t nrows, int kid, double* __restrict__ div,
+                  double* __restrict__ curl, double* __restrict__ ddiv)
+{
+    gradients_mixed_row<L, true>(pos, accel, pg, lists, box, rows, nrows, kid, div, curl, ddiv);
+}
+
+template <class L>
+__global__ void __launch_bounds__(kBlockM)
+k_divcurl_mixed(const double* __restrict__ pos, const double* __restrict__ accel, const PG* __restrict__ pg, L lists,
+                Box box, const int64_t* __restrict__ rows, int64_t nrows, int kid, double* __restrict__ div,
+                double* __restrict__ curl, double* __restrict__ ddiv)
+{
+    gradients_mixed_row<L, false>(pos, accel, pg, lists, box, rows, nrows, kid, div, curl, ddiv);
 }
 
 static inline unsigned grid_m(int64_t nrows) { return (unsigned)((nrows * kGM + kBlockM - 1) / kBlockM); }
@@ -322,7 +342,11 @@ extern "C" int32_t nb_gradients_mixed(nb_ctx* ctx, void* stream, const double* p
     hipStream_t s = (hipStream_t)stream;
     PG* pg = (PG*)ctx->pack;
     hipLaunchKernelGGL(k_pack_grad, dim3(grid_1(n)), dim3(kBlockM), 0, s, vel, mass, h, cnorm, n, pg);
-    NB_DISPATCH_M(lists, k_gradients_mixed, grid_m(nrows), s, pos, acc, pg, v_, box_of(box3), rows, nrows, kid, div,
-                  curl, ddiv);
+    if (acc)
+        NB_DISPATCH_M(lists, k_gradients_mixed, grid_m(nrows), s, pos, acc, pg, v_, box_of(box3), rows, nrows, kid,
+                      div, curl, ddiv);
+    else
+        NB_DISPATCH_M(lists, k_divcurl_mixed, grid_m(nrows), s, pos, acc, pg, v_, box_of(box3), rows, nrows, kid, div,
+                      curl, ddiv);
     return launched(ctx, "nb_gradients_mixed");
 }

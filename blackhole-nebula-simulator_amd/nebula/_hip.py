@@ -112,6 +112,7 @@ SIGNATURES = {
                                       _vp, _vp, _vp, _vp, _P64]),
     "nb_hydro_tight_mixed": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _P64]),
     "nb_timing": (_i32, [_vp, _i32]),
+    "nb_permute_many": (_i32, [_vp, _vp, _i32, _vp, _vp, _vp, _vp, _i64]),
     "nb_timing_report": (_i64, [_vp, ctypes.c_char_p, _i64]),
     "nb_gradients_mixed": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _PL, _PD, _vp, _i64, _i32, _f64, _vp, _vp,
                                   _vp]),

@@ -33,7 +33,7 @@ from ..physics import units
 from .fastphys import gn_v_from_p_device
 from .hydro import HydroConfig
 from .integrator import glf_device
-from .neighbours import grid_order, permute
+from .neighbours import grid_order, permute, permute_many
 from .sph import (
     COND_VELOCITY,
     hydro_density_device,
@@ -334,10 +334,9 @@ def step(st: GasState, cfg: CoupledConfig, dt: float) -> GasState:
     p_pred = _axpy(p_flow, st.a_hydro, half)
     u_pred = _axpy(u_half, st.dudt, half)
     order = grid_permutation(cfg, x1, st.h)
-    x1, p_flow, p_pred, u_half, u_pred, a_prev = (permute(t, order) for t in (x1, p_flow, p_pred, u_half, u_pred,
-                                                                                st.a_hydro))  # fmt: skip
-    pred = replace(st, t=st.t + dt, x=x1, p=p_flow, u=u_pred, mass=permute(st.mass, order), h=permute(st.h, order),
-                   alpha=permute(st.alpha, order), perm=permute(st.perm, order))  # fmt: skip
+    x1, p_flow, p_pred, u_half, u_pred, a_prev, mass, h, alpha, perm = permute_many(
+        (x1, p_flow, p_pred, u_half, u_pred, st.a_hydro, st.mass, st.h, st.alpha, st.perm), order)
+    pred = replace(st, t=st.t + dt, x=x1, p=p_flow, u=u_pred, mass=mass, h=h, alpha=alpha, perm=perm)
     new = _hydro(pred, cfg, velocity(cfg, x1, p_pred), dt, a_prev)
     return replace(new, p=_axpy(p_flow, new.a_hydro, half), u=_axpy(u_half, new.dudt, half), e_radiated_dev=e_rad)
 

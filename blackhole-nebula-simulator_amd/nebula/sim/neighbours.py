@@ -214,6 +214,23 @@ def permute(t: torch.Tensor, order: torch.Tensor) -> torch.Tensor:
     return out
 
 
+def permute_many(tensors, order: torch.Tensor) -> list:
+    """permute() of several float64 / int64 tensors with one kernel launch."""
+    import ctypes
+
+    n = int(order.numel())
+    outs = [torch.empty_like(t) for t in tensors]
+    k = len(tensors)
+    src = (ctypes.c_void_p * k)(*[t.data_ptr() for t in tensors])
+    dst = (ctypes.c_void_p * k)(*[o.data_ptr() for o in outs])
+    width = (ctypes.c_int32 * k)(*[int(t.numel() // max(n, 1)) for t in tensors])
+    for t in tensors:
+        if t.dtype not in (torch.float64, torch.int64) or not t.is_contiguous() or t.shape[0] != n:
+            raise ValueError("permute_many takes contiguous float64 / int64 tensors of one length")
+    _hip.call("nb_permute_many", k, src, dst, width, _hip.ptr(order), n)
+    return outs
+
+
 def minimum_image(dx: np.ndarray, box) -> np.ndarray:
     """Separation vectors folded into (-L/2, L/2] (neighbours.py:39-43)."""
     if box is None:

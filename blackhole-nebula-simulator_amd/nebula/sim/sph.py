@@ -142,6 +142,12 @@ def compute_density_device(pos_d, mass_d, h_guess_d, box_arr, kernel_name, eta, 
     raise RuntimeError("smoothing lengths kept outgrowing the neighbour search")
 
 
+def _round_cap(cap: int) -> int:
+    """Row capacity rounded up to a multiple of 32, so the list buffers keep their size from step
+    to step (the caching allocator then reuses them instead of allocating anew)."""
+    return max(((cap + 31) // 32) * 32, 32)
+
+
 def hydro_density_device(pos_d, mass_d, h_guess_d, kernel_name, eta, tol, h_max, skin, m_uniform,
                          precision="fp64", row_cap_hint=0):  # fmt: skip
     """The density pass of the coupled step (open box): compute_density's rebuild rounds
@@ -171,7 +177,7 @@ def hydro_density_device(pos_d, mass_d, h_guess_d, kernel_name, eta, tol, h_max,
             break
     else:
         raise RuntimeError("smoothing lengths kept outgrowing the neighbour search")
-    cap = max(int(stats[3]), 1)
+    cap = _round_cap(int(stats[3]))
     ln, idx = _blocked_alloc(n, cap)
     total = np.zeros(1, np.int64)
     _hip.call("nb_hydro_tight", _hip.ptr(h), _hip.ptr(ln), _hip.ptr(idx), cap, total.ctypes.data_as(_hip._P64))
@@ -206,7 +212,7 @@ def hydro_density_mixed_device(pos_d, mass_d, h_guess_d, kernel_name, eta, tol, 
             break
     else:
         raise RuntimeError("smoothing lengths kept outgrowing the neighbour search")
-    cap = max(int(stats[3]), 1)
+    cap = _round_cap(int(stats[3]))
     ln = _dev.empty(n, torch.int32)
     idx = _dev.empty(n * cap, torch.int32)
     total = np.zeros(1, np.int64)

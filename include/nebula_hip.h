@@ -114,6 +114,10 @@ int32_t nb_grid_order(nb_ctx *ctx, void *stream, const double *pos, const double
 int32_t nb_permute(nb_ctx *ctx, void *stream, const double *src, const int32_t *order, int64_t n, int32_t width,
                    double *dst);
 int32_t nb_permute_i64(nb_ctx *ctx, void *stream, const int64_t *src, const int32_t *order, int64_t n, int64_t *dst);
+/* The same for count (<= 12) arrays of 8-byte values (doubles or int64) in one launch: dst[k] row i
+ * = src[k] row order[i], rows of width[k] values (src, dst, width: host arrays of count). */
+int32_t nb_permute_many(nb_ctx *ctx, void *stream, int32_t count, const void *const *src, void *const *dst,
+                        const int32_t *width, const int32_t *order, int64_t n);
 
 /* tighten (neighbours.py:306-323): keep pairs with |x_i - x_j| < 2 max(h_i, h_j); rows not in
  * ``rows`` are left empty. Same capacity protocol (the input list total always suffices). */

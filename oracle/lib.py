@@ -67,6 +67,7 @@ disc_forces = _sig("orc_disc_forces", _D, _D, _D, _D, _D, _D, _D, _f64, _I, _I, 
                    _I, _i64, _D, _D)  # fmt: skip
 glf_step_many = _sig("orc_glf_step_many", _D, _D, _D, _i64, _D, _D, _I)
 gn_v_from_p = _sig("orc_gn_v_from_p", _D, _D, _i64, _D)
+inv3_det3 = _sig("orc_inv3_det3", _D, _i64, _D, _D)
 
 
 def d(a: np.ndarray):

@@ -577,6 +577,15 @@ static void matmul3(const double *a, const double *b, double *c)
         }
 }
 
+/* the 3x3 helpers on n matrices (row-major), for the tests that hold them to LAPACK */
+void orc_inv3_det3(const double *a, int64_t n, double *inv, double *det)
+{
+    for (int64_t m = 0; m < n; m++) {
+        inv3(a + 9 * m, inv + 9 * m);
+        det[m] = det3(a + 9 * m);
+    }
+}
+
 /* sph.py:776-782 (_near_singular) */
 static int near_singular(const double *rmat, int64_t cnt)
 {

@@ -381,3 +381,57 @@ def test_disc_and_ns_conserve(rng):
     heat = oph.viscous_heating(g, nu)
     pw = np.sum(mass * (np.sum(vel * an, axis=1) + heat))
     check("NS: sum m (v.a + Lambda) = 0 (Phantom Eq. 132)", 0.0, abs(pw) / np.sum(mass * np.abs(heat)), atol=1e-12)
+
+
+def test_inverse_and_determinant_match_lapack(rng):
+    """The gradient passes invert R with the oracle's own 3x3 elimination (the reference calls
+    np.linalg.inv / det, i.e. LAPACK): both agree to round-off on well-conditioned and on
+    badly scaled matrices, and a transposed or sign-flipped result would fail."""
+    from oracle import lib
+
+    n = 2000
+    a = rng.normal(0.0, 1.0, (n, 3, 3)) + 3.0 * np.eye(3)
+    a[: n // 4] *= rng.uniform(1e-6, 1e6, (n // 4, 1, 1))  # scale does not matter
+    inv = np.empty_like(a)
+    det = np.empty(n)
+    lib.inv3_det3(lib.d(np.ascontiguousarray(a)), n, lib.d(inv), lib.d(det))
+    ref_inv = np.linalg.inv(a)
+    ref_det = np.linalg.det(a)
+    rel = np.abs(inv - ref_inv).max(axis=(1, 2)) / np.abs(ref_inv).max(axis=(1, 2))
+    check("3x3 inverse vs LAPACK (max rel.)", 0.0, float(rel.max()), atol=1e-13)
+    check("3x3 determinant vs LAPACK", ref_det, det, rtol=1e-13)
+    eye = np.einsum("nij,njk->nik", a, inv)
+    check("A A^-1 = I", np.broadcast_to(np.eye(3), eye.shape), eye, atol=1e-12)
+
+
+def test_balsara_closed_forms():
+    """Balsara (1995) limiter on gradients of closed-form flows (the BASELINE C3 switch, which the
+    reference lacks): pure compression keeps alpha (f -> 1 up to the eps c_s/h term), pure
+    rotation removes it (f = 0 exactly on the perfect lattice)."""
+    from oracle import physics as ph
+    from oracle import sph as osph
+    from oracle.neighbours import tighten
+
+    k = 12
+    g = (np.arange(k) + 0.5) / k
+    pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    n = len(pos)
+    mass = np.full(n, 1.0 / n)
+    d = osph.compute_density(pos, mass, np.full(n, 1.5 / k), box=1.0, kernel_name="cubic", eta=1.2)
+    nb = tighten(d.neighbours, pos, d.h)
+    cs = np.full(n, 1e-3)
+    # compression v = -x about the centre (periodic images far from the probes)
+    inner = np.linalg.norm(pos - 0.5, axis=1) < 0.25
+    v = -(pos - 0.5)
+    gr = osph.gradients_numba(pos, v, None, mass, d.h, nb, "cubic")
+    f = ph.balsara_factor(gr.div_v, gr.curl_v, cs, d.h)
+    check("compression: div v = -3", -3.0 * np.ones(inner.sum()), gr.div_v[inner], rtol=1e-10)
+    expect = 3.0 / (3.0 + ph.BALSARA_EPS * cs[inner] / d.h[inner])
+    check("compression: f = 3 / (3 + eps c_s/h)", expect, f[inner], rtol=1e-9)
+    # rigid rotation about z: v = omega x (x - c)
+    w = np.array([0.0, 0.0, 2.0])
+    v = np.cross(w, pos - 0.5)
+    gr = osph.gradients_numba(pos, v, None, mass, d.h, nb, "cubic")
+    f = ph.balsara_factor(gr.div_v, gr.curl_v, cs, d.h)
+    check("rotation: |curl v| = 2 |omega|", np.full(inner.sum(), 4.0), gr.curl_v[inner], rtol=1e-10)
+    check("rotation: f = 0", np.zeros(inner.sum()), f[inner], atol=1e-12)

@@ -499,7 +499,7 @@ __global__ void k_permute_i64(const int64_t* __restrict__ src, const int32_t* __
 }
 
 // several arrays of 8-byte values at once (bit copies: doubles or int64)
-static constexpr int kMaxPermute = 12;
+static constexpr int kMaxPermute = 16;
 struct PermSet {
     const uint64_t* src[kMaxPermute];
     uint64_t* dst[kMaxPermute];

@@ -102,6 +102,12 @@ class GasState:
     n_tight_entries: int = 0  # tightened-list entries of the last force pass (incl. self)
     perm: torch.Tensor = field(default=None)  # caller index of each stored particle (int64)
     row_cap: int = 0
+    # learned-h inputs kept from the last hydro pass, and its statistics
+    div_v: torch.Tensor = field(default=None)
+    curl_v: torch.Tensor = field(default=None)
+    nneigh: torch.Tensor = field(default=None)  # tight-list length per particle (float64)
+    dt_prev: float = 0.0
+    learned_fallbacks: int = 0
 
     @property
     def e_radiated(self) -> float:
@@ -189,18 +195,19 @@ def gravity_flow(cfg: CoupledConfig, x, p, dt):
     return x1, p1
 
 
-def _hydro(st: GasState, cfg: CoupledConfig, v, dt_alpha: float, a_prev) -> GasState:
-    """coupled.py:145-195 on device tensors."""
+def _hydro(st: GasState, cfg: CoupledConfig, v, dt_alpha: float, a_prev, h_start=None) -> GasState:
+    """coupled.py:145-195 on device tensors. h_start: the Newton solve's starting h (default:
+    the previous h, as in the reference); the search lists always come from the previous h."""
     hc = cfg.hydro
     if hc.isothermal_cs is not None or hc.eos != "ideal":
         raise ValueError("the GPU step implements the ideal-gas equation of state")
     n = int(st.x.shape[0])
     if hc.precision == "mixed":
         d, n_search = hydro_density_mixed_device(st.x, st.mass, st.h.clone(), hc.kernel, hc.eta, 1e-4, hc.h_max,
-                                                 hc.skin, st.m_uniform)  # fmt: skip
+                                                 hc.skin, st.m_uniform, h_start_d=h_start)  # fmt: skip
     else:
         d, n_search = hydro_density_device(st.x, st.mass, st.h.clone(), hc.kernel, hc.eta, 1e-4, hc.h_max, hc.skin,
-                                           st.m_uniform, hc.precision)  # fmt: skip
+                                           st.m_uniform, hc.precision, h_start_d=h_start)  # fmt: skip
     p_gas = _dev.empty(n)
     cs = _dev.empty(n)
     nbad = torch.zeros(1, dtype=torch.int32, device=st.x.device)
@@ -211,6 +218,7 @@ def _hydro(st: GasState, cfg: CoupledConfig, v, dt_alpha: float, a_prev) -> GasS
         raise FloatingPointError(f"non-positive internal energy at {bad} particles")
     nb = d.neighbours  # tightened at the final h
     alpha = st.alpha
+    g = None
     if hc.viscosity and hc.switch and dt_alpha > 0:
         g = gradients_device(st.x, v, a_prev, st.mass, d.h, nb, hc.kernel, precision=hc.precision)
         alpha = _dev.empty(n)
@@ -260,7 +268,15 @@ def _hydro(st: GasState, cfg: CoupledConfig, v, dt_alpha: float, a_prev) -> GasS
         iterations=d.iterations, flags=d.flags, builds=d.builds,
         n_search_entries=n_search,
         n_tight_entries=nb.n_total,
+        div_v=None if g is None else g.div_v, curl_v=None if g is None else g.curl_v,
+        nneigh=nb.length.double(),
     )  # fmt: skip
+
+
+def _learned(cfg: CoupledConfig):
+    """The learned-h network of the configuration, or None."""
+    hc = cfg.hydro
+    return hc.learned_model if (hc.learned_h or getattr(hc.learned_model, "collect", None)) else None
 
 
 def initialise(x, v, u, mass, h_guess, cfg: CoupledConfig, t: float = 0.0) -> GasState:
@@ -334,11 +350,32 @@ def step(st: GasState, cfg: CoupledConfig, dt: float) -> GasState:
     p_pred = _axpy(p_flow, st.a_hydro, half)
     u_pred = _axpy(u_half, st.dudt, half)
     order = grid_permutation(cfg, x1, st.h)
-    x1, p_flow, p_pred, u_half, u_pred, a_prev, mass, h, alpha, perm = permute_many(
-        (x1, p_flow, p_pred, u_half, u_pred, st.a_hydro, st.mass, st.h, st.alpha, st.perm), order)
+    model = _learned(cfg)
+    fields = [x1, p_flow, p_pred, u_half, u_pred, st.a_hydro, st.mass, st.h, st.alpha, st.perm]
+    extra = []
+    if model is not None:
+        # the learned start's inputs from the last hydro pass, in the new order
+        extra = [t for t in (st.rho, st.div_v, st.curl_v, st.nneigh)]
+    moved = permute_many(fields + [t for t in extra if t is not None], order)
+    x1, p_flow, p_pred, u_half, u_pred, a_prev, mass, h, alpha, perm = moved[:10]
     pred = replace(st, t=st.t + dt, x=x1, p=p_flow, u=u_pred, mass=mass, h=h, alpha=alpha, perm=perm)
-    new = _hydro(pred, cfg, velocity(cfg, x1, p_pred), dt, a_prev)
-    return replace(new, p=_axpy(p_flow, new.a_hydro, half), u=_axpy(u_half, new.dudt, half), e_radiated_dev=e_rad)
+    h_start = feat = None
+    fallbacks = 0
+    if model is not None:
+        it = iter(moved[10:])
+        rho_p, div_p, curl_p, nn_p = (None if t is None else next(it) for t in extra)
+        stride = model.collect.stride if model.collect is not None else 1
+        feat = model.features(h, rho_p, div_p, curl_p, nn_p, x1, u_pred, dt, st.dt_prev, stride=1)
+        if hc.learned_h:
+            h_start = model.h_start(h, feat)
+            fallbacks = model.last_fallbacks
+        if model.collect is not None and stride > 1:
+            feat = feat[::stride].contiguous()
+    new = _hydro(pred, cfg, velocity(cfg, x1, p_pred), dt, a_prev, h_start=h_start)
+    if model is not None and model.collect is not None:
+        model.collect.add(feat, model.targets(h, new.h, model.collect.stride))
+    return replace(new, p=_axpy(p_flow, new.a_hydro, half), u=_axpy(u_half, new.dudt, half), e_radiated_dev=e_rad,
+                   dt_prev=float(dt), learned_fallbacks=fallbacks)
 
 
 def ledger(st: GasState, cfg: CoupledConfig) -> dict:

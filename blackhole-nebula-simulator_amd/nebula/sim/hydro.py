@@ -8,7 +8,7 @@ network-predicted starting h of the Newton solve).
 
 from __future__ import annotations
 
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 
 import numpy as np
 
@@ -49,6 +49,9 @@ class HydroConfig:
     balsara: bool = False
     precision: str = "fp64"
     learned_h: bool = False
+    # the network (nebula.sim.learned_h.LearnedH) used when learned_h is set, or that collects
+    # training pairs when its ``collect`` log is set
+    learned_model: object = field(default=None, compare=False, repr=False)
 
     def shock(self) -> ShockCapturing:
         """The force pass's shock-capturing settings (hydro.py:92-98)."""

@@ -1,0 +1,156 @@
+"""Learned starting h for the h-rho Newton solve.
+
+A small MLP (8 -> 64 -> 64 -> 1, tanh) predicts delta = log h_final - log h_prev of every particle
+from the previous step's state:
+
+    log h, log rho, div v dt, |curl v| dt, log(neighbour count), log r, log u, dt / dt_prev.
+
+The Newton iteration of the density pass then starts from h_prev exp(delta) instead of h_prev.
+It still runs to the same tolerance and stays the judge; a prediction that is not finite, or
+that changes h by a factor 2 or more, falls back to h_prev (counted). The search lists are
+built from h_prev as before, so the lists are the plain method's.
+
+Inference is the BF16 MFMA kernel ``nb_mlp_infer``. Training uses this project's own FP32
+kernels (``nb_mlp_train_step``: forward, backward and Adam). The training pairs are logged by
+the simulator itself (``PairLog``, filled by ``coupled.step`` when ``collect`` is set).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .. import _dev, _hip
+
+N_FEATURES = 8
+
+
+def param_count() -> int:
+    return int(_hip.lib().nb_mlp_param_count())
+
+
+class PairLog:
+    """(features, target) pairs logged from simulation steps (device tensors, float32)."""
+
+    def __init__(self, stride: int = 1):
+        self.stride = int(stride)
+        self._feat: list[torch.Tensor] = []
+        self._target: list[torch.Tensor] = []
+
+    def add(self, feat: torch.Tensor, target: torch.Tensor) -> None:
+        self._feat.append(feat)
+        self._target.append(target)
+
+    @property
+    def size(self) -> int:
+        return int(sum(t.numel() for t in self._target))
+
+    def tensors(self) -> tuple[torch.Tensor, torch.Tensor]:
+        return torch.cat(self._feat), torch.cat(self._target)
+
+
+class LearnedH:
+    """The network, its normalisation and its Adam state (device tensors)."""
+
+    def __init__(self, seed: int = 0):
+        n = param_count()
+        rng = np.random.default_rng(seed)
+        p = np.zeros(n, np.float32)
+        # W1 (8 x 64), W2 (64 x 64), w3 (64): Glorot-uniform; biases zero
+        p[0:512] = rng.uniform(-1, 1, 512) * np.sqrt(6.0 / (8 + 64))
+        p[576:4672] = rng.uniform(-1, 1, 4096) * np.sqrt(6.0 / (64 + 64))
+        p[4736:4800] = rng.uniform(-1, 1, 64) * np.sqrt(6.0 / (64 + 1))
+        dev = _hip.torch_device()
+        self.params = torch.as_tensor(p, device=dev)
+        self.norm = torch.as_tensor(np.concatenate([np.zeros(8), np.ones(8), [0.0, 1.0]]).astype(np.float32), device=dev)
+        self.adam_m = torch.zeros(n, dtype=torch.float32, device=dev)
+        self.adam_v = torch.zeros(n, dtype=torch.float32, device=dev)
+        self.step_count = 0
+        self.collect: PairLog | None = None
+        self.last_fallbacks = 0
+
+    # ------------------------------------------------------------------ data
+    def features(self, h_prev, rho_prev, div_v, curl_v, nneigh, x, u, dt, dt_prev, stride: int = 1) -> torch.Tensor:
+        n = int(h_prev.shape[0])
+        rows = (n + stride - 1) // stride
+        out = _dev.empty((rows, N_FEATURES), torch.float32)
+        _hip.call("nb_mlp_features", _hip.ptr(h_prev), _hip.ptr(rho_prev), _hip.ptr(div_v), _hip.ptr(curl_v),
+                  _hip.ptr(nneigh), _hip.ptr(x), _hip.ptr(u), n, int(stride), float(dt), float(dt_prev),
+                  _hip.ptr(out))  # fmt: skip
+        return out
+
+    @staticmethod
+    def targets(h_prev, h_final, stride: int = 1) -> torch.Tensor:
+        n = int(h_prev.shape[0])
+        out = _dev.empty((n + stride - 1) // stride, torch.float32)
+        _hip.call("nb_mlp_target", _hip.ptr(h_prev), _hip.ptr(h_final), n, int(stride), _hip.ptr(out))
+        return out
+
+    # ------------------------------------------------------------------ inference
+    def predict(self, feat: torch.Tensor) -> torch.Tensor:
+        """delta = log h_final - log h_prev for each feature row (BF16 MFMA)."""
+        n = int(feat.shape[0])
+        out = _dev.empty(n, torch.float32)
+        _hip.call("nb_mlp_infer", _hip.ptr(self.params), _hip.ptr(self.norm), _hip.ptr(feat), n, _hip.ptr(out))
+        return out
+
+    def h_start(self, h_prev: torch.Tensor, feat: torch.Tensor) -> torch.Tensor:
+        """Starting h of the Newton solve; counts the fallbacks into ``last_fallbacks``."""
+        delta = self.predict(feat)
+        out = torch.empty_like(h_prev)
+        fb = np.zeros(1, np.int64)
+        _hip.call("nb_mlp_h_start", _hip.ptr(h_prev), _hip.ptr(delta), int(h_prev.shape[0]), _hip.ptr(out),
+                  fb.ctypes.data_as(_hip._P64))  # fmt: skip
+        self.last_fallbacks = int(fb[0])
+        return out
+
+    # ------------------------------------------------------------------ training
+    def fit_normalisation(self, feat: torch.Tensor, target: torch.Tensor) -> None:
+        """Feature means and 1/std, target mean and std, from a training set."""
+        f = feat.double()
+        mean = f.mean(0)
+        std = f.std(0).clamp_min(1e-12)
+        t = target.double()
+        tstd = float(t.std().clamp_min(1e-12)) if t.numel() > 1 else 1.0
+        vals = torch.cat([mean, 1.0 / std, torch.tensor([float(t.mean()), tstd], dtype=torch.float64, device=f.device)])
+        self.norm = vals.float().contiguous()
+
+    def grad(self, feat: torch.Tensor, target: torch.Tensor) -> tuple[float, torch.Tensor]:
+        """Loss and FP32 gradient over the batch (no update)."""
+        g = torch.empty_like(self.params)
+        loss = _dev.empty(1, torch.float32)
+        _hip.call("nb_mlp_grad", _hip.ptr(self.params), _hip.ptr(self.norm), _hip.ptr(feat.contiguous()),
+                  _hip.ptr(target.contiguous()), int(target.numel()), _hip.ptr(g), _hip.ptr(loss))  # fmt: skip
+        return float(loss.item()), g
+
+    def train(self, feat: torch.Tensor, target: torch.Tensor, steps: int = 400, batch: int = 4096,
+              lr: float = 3e-3, seed: int = 0) -> list[float]:  # fmt: skip
+        """Adam on random minibatches; returns the loss of each step (normalised target)."""
+        gen = torch.Generator(device=feat.device)
+        gen.manual_seed(seed)
+        n = int(target.numel())
+        batch = min(batch, n)
+        losses = []
+        loss = _dev.empty(1, torch.float32)
+        for _ in range(steps):
+            idx = torch.randint(0, n, (batch,), generator=gen, device=feat.device)
+            fb = feat.index_select(0, idx).contiguous()
+            tb = target.index_select(0, idx).contiguous()
+            self.step_count += 1
+            _hip.call("nb_mlp_train_step", _hip.ptr(self.params), _hip.ptr(self.adam_m), _hip.ptr(self.adam_v),
+                      self.step_count, _hip.ptr(self.norm), _hip.ptr(fb), _hip.ptr(tb), batch, float(lr),
+                      _hip.ptr(loss))  # fmt: skip
+            losses.append(float(loss.item()))
+        return losses
+
+    # ------------------------------------------------------------------ persistence
+    def save(self, path) -> None:
+        np.savez(path, params=self.params.cpu().numpy(), norm=self.norm.cpu().numpy())
+
+    @classmethod
+    def load(cls, path) -> "LearnedH":
+        d = np.load(path, allow_pickle=False)
+        m = cls()
+        m.params = torch.as_tensor(d["params"], device=m.params.device).contiguous()
+        m.norm = torch.as_tensor(d["norm"], device=m.params.device).contiguous()
+        return m

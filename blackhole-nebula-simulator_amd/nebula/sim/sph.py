@@ -149,7 +149,7 @@ def _round_cap(cap: int) -> int:
 
 
 def hydro_density_device(pos_d, mass_d, h_guess_d, kernel_name, eta, tol, h_max, skin, m_uniform,
-                         precision="fp64", row_cap_hint=0):  # fmt: skip
+                         precision="fp64", row_cap_hint=0, h_start_d=None):  # fmt: skip
     """The density pass of the coupled step (open box): compute_density's rebuild rounds
     (sph.py:329-387) on the fused gather + h-rho solve kernel, then the tightened lists at the
     final h, in 64-row blocks. h_guess_d is overwritten. Returns (DensityResult with the tight
@@ -167,7 +167,8 @@ def hydro_density_device(pos_d, mass_d, h_guess_d, kernel_name, eta, tol, h_max,
     stats = np.zeros(4, np.int64)
     grown = np.zeros(1, np.int64)
     for builds in range(1, 13):
-        _hip.call("nb_hydro_density", _hip.ptr(pos_d), _hip.ptr(mass_d), float(m_uniform), _hip.ptr(h_guess_d),
+        start = h_start_d if (h_start_d is not None and builds == 1) else h_guess_d
+        _hip.call("nb_hydro_density", _hip.ptr(pos_d), _hip.ptr(mass_d), float(m_uniform), _hip.ptr(start),
                   _hip.ptr(h_search), n, kid, float(C_NORM[kid]), float(eta), float(tol), float(h_max),
                   PRECISIONS[precision], _hip.ptr(h), _hip.ptr(rho), _hip.ptr(omega), _hip.ptr(iters), _hip.ptr(flags),
                   stats.ctypes.data_as(_hip._P64))  # fmt: skip
@@ -185,7 +186,8 @@ def hydro_density_device(pos_d, mass_d, h_guess_d, kernel_name, eta, tol, h_max,
     return DensityResult(h, rho, omega, iters, flags, tight, None, builds), int(stats[2])
 
 
-def hydro_density_mixed_device(pos_d, mass_d, h_guess_d, kernel_name, eta, tol, h_max, skin, m_uniform):
+def hydro_density_mixed_device(pos_d, mass_d, h_guess_d, kernel_name, eta, tol, h_max, skin, m_uniform,
+                               h_start_d=None):  # fmt: skip
     """hydro_density_device in mixed precision: the same exact search lists, FP32 compensated
     sums in the h-rho solve (Newton steps and convergence test in FP64), tight lists row-major.
     Returns (DensityResult with RowLists as ``neighbours``, search-list entries)."""
@@ -202,7 +204,8 @@ def hydro_density_mixed_device(pos_d, mass_d, h_guess_d, kernel_name, eta, tol, 
     stats = np.zeros(4, np.int64)
     grown = np.zeros(1, np.int64)
     for builds in range(1, 13):
-        _hip.call("nb_hydro_density_mixed", _hip.ptr(pos_d), _hip.ptr(mass_d), float(m_uniform), _hip.ptr(h_guess_d),
+        start = h_start_d if (h_start_d is not None and builds == 1) else h_guess_d
+        _hip.call("nb_hydro_density_mixed", _hip.ptr(pos_d), _hip.ptr(mass_d), float(m_uniform), _hip.ptr(start),
                   _hip.ptr(h_search), n, kid, float(C_NORM[kid]), float(eta), float(tol), float(h_max), _hip.ptr(h),
                   _hip.ptr(rho), _hip.ptr(omega), _hip.ptr(iters), _hip.ptr(flags),
                   stats.ctypes.data_as(_hip._P64))  # fmt: skip

@@ -114,7 +114,7 @@ int32_t nb_grid_order(nb_ctx *ctx, void *stream, const double *pos, const double
 int32_t nb_permute(nb_ctx *ctx, void *stream, const double *src, const int32_t *order, int64_t n, int32_t width,
                    double *dst);
 int32_t nb_permute_i64(nb_ctx *ctx, void *stream, const int64_t *src, const int32_t *order, int64_t n, int64_t *dst);
-/* The same for count (<= 12) arrays of 8-byte values (doubles or int64) in one launch: dst[k] row i
+/* The same for count (<= 16) arrays of 8-byte values (doubles or int64) in one launch: dst[k] row i
  * = src[k] row order[i], rows of width[k] values (src, dst, width: host arrays of count). */
 int32_t nb_permute_many(nb_ctx *ctx, void *stream, int32_t count, const void *const *src, void *const *dst,
                         const int32_t *width, const int32_t *order, int64_t n);
@@ -293,6 +293,38 @@ int32_t nb_nu_av(nb_ctx *ctx, void *stream, const double *cs, const double *h, d
 int32_t nb_add(nb_ctx *ctx, void *stream, const double *a, const double *b, int64_t m, double *out);
 /* acc_dev[0] += sum(x) on the device (no host synchronization). */
 int32_t nb_sum_into(nb_ctx *ctx, void *stream, const double *x, int64_t n, double *acc_dev);
+
+/* ------------------------------------------------------------------ learned starting h
+ * No counterpart in the reference: the learned-h mode of this implementation. A small MLP
+ * (8 -> 64 -> 64 -> 1, tanh) predicts delta = log h_final - log h_prev from the previous step's
+ * state; the Newton iteration starts from h_prev exp(delta) and still converges to the same
+ * tolerance (it stays the judge). params: nb_mlp_param_count() floats, W1[8][64], b1[64],
+ * W2[64][64], b2[64], w3[64], b3. norm: 18 floats, feature means[8], feature 1/std[8], target
+ * mean, target scale. Features (raw, float32, 8 per row): log h, log rho, div v dt, |curl v| dt,
+ * log(neighbour count), log r, log u, dt / dt_prev. */
+int32_t nb_mlp_param_count(void);
+/* raw features of particles 0, stride, 2 stride, ... (nneigh: neighbour counts as doubles; div_v,
+ * curl_v, nneigh may be NULL: 0, 0, 1) */
+int32_t nb_mlp_features(nb_ctx *ctx, void *stream, const double *h_prev, const double *rho_prev, const double *div_v,
+                        const double *curl_v, const double *nneigh, const double *x, const double *u, int64_t n,
+                        int64_t stride, double dt, double dt_prev, float *out);
+/* training target log h_final - log h_prev of the same rows */
+int32_t nb_mlp_target(nb_ctx *ctx, void *stream, const double *h_prev, const double *h_final, int64_t n,
+                      int64_t stride, float *out);
+/* delta for n feature rows: BF16 MFMA (FP32 accumulation) */
+int32_t nb_mlp_infer(nb_ctx *ctx, void *stream, const float *params, const float *norm, const float *feat, int64_t n,
+                     float *out);
+/* Newton starting h = h_prev exp(delta); the previous h where delta is not finite or |delta| >= ln 2
+ * (fallback: *fallbacks_out, host, counts them) */
+int32_t nb_mlp_h_start(nb_ctx *ctx, void *stream, const double *h_prev, const float *delta, int64_t n,
+                       double *h_start, int64_t *fallbacks_out);
+/* FP32 gradient of the mean squared error (normalised target) over a batch, and the loss */
+int32_t nb_mlp_grad(nb_ctx *ctx, void *stream, const float *params, const float *norm, const float *feat,
+                    const float *target, int64_t batch, float *grad_out, float *loss_out);
+/* one Adam step (beta 0.9 / 0.999, eps 1e-8; step counts from 1) on a batch; loss before the step */
+int32_t nb_mlp_train_step(nb_ctx *ctx, void *stream, float *params, float *adam_m, float *adam_v, int64_t step,
+                          const float *norm, const float *feat, const float *target, int64_t batch, float lr,
+                          float *loss_out);
 
 #ifdef __cplusplus
 }

@@ -46,6 +46,7 @@ def parse():
     ap.add_argument("--n", type=int, default=1_048_576)
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--no-compare", action="store_true", help="skip timing the other precision alongside")
+    ap.add_argument("--no-learned", action="store_true", help="mixed mode without the learned starting h")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -85,18 +86,43 @@ def cpu_baseline(ic, cfg, steps):
                       f"kernels, 1 warm-up + {steps} timed coupled steps"}  # fmt: skip
 
 
-def run_gpu(ic, cfg, steps, warmup):
+def run_gpu(ic, cfg, steps, warmup, learned=False, log_stride=8, train_steps=300):
     """Times ``steps`` coupled steps after ``warmup`` (no instrumentation in the timed region), then
     runs ``steps`` more with the library's kernel timing on (HIP events around each main kernel on
-    its stream) and the entry-point timing. Returns (wall s, kernel times, entry-point times, state)."""
+    its stream) and the entry-point timing. With ``learned``, the warm-up steps log (features,
+    converged h) pairs (every ``log_stride``-th particle), the network is trained on them with the
+    FP32 training kernels, and the timed steps start each Newton solve from its BF16 MFMA
+    prediction. Returns (wall s, kernel times, entry-point times, state, info)."""
     import torch
 
     from nebula import _hip
     from nebula.sim import coupled
+    from nebula.sim.learned_h import LearnedH, PairLog
 
     st = coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg)
-    for _ in range(warmup):
-        st = coupled.step(st, cfg, coupled.stable_dt(st, cfg))
+    info = {}
+    if learned:
+        model = LearnedH(seed=1)
+        model.collect = PairLog(stride=log_stride)
+        cfg_log = replace(cfg, hydro=replace(cfg.hydro, learned_model=model))
+        plain_iters = []
+        for _ in range(max(warmup, 2)):
+            st = coupled.step(st, cfg_log, coupled.stable_dt(st, cfg_log))
+            plain_iters.append(float(st.iterations.double().mean().item()))
+        feat, target = model.collect.tensors()
+        model.collect = None
+        t0 = time.perf_counter()
+        model.fit_normalisation(feat, target)
+        losses = model.train(feat, target, steps=train_steps, batch=8192, lr=3e-3)
+        torch.cuda.synchronize()
+        info = {"pairs": int(target.numel()), "logged_steps": len(plain_iters), "train_steps": train_steps,
+                "train_s": time.perf_counter() - t0, "loss_first": losses[0], "loss_last": losses[-1],
+                "newton_iters_plain": sum(plain_iters) / len(plain_iters)}  # fmt: skip
+        cfg = replace(cfg, hydro=replace(cfg.hydro, learned_h=True, learned_model=model))
+        st = coupled.step(st, cfg, coupled.stable_dt(st, cfg))  # first learned step: warm-up
+    else:
+        for _ in range(warmup):
+            st = coupled.step(st, cfg, coupled.stable_dt(st, cfg))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -105,12 +131,18 @@ def run_gpu(ic, cfg, steps, warmup):
     wall = time.perf_counter() - t0
     _hip.start_kernel_timing()
     _hip.start_timing()
+    iters, fallbacks = [], 0
     for _ in range(steps):
         st = coupled.step(st, cfg, coupled.stable_dt(st, cfg))
+        iters.append(st.iterations.double().mean())
+        fallbacks += st.learned_fallbacks if learned else 0
     torch.cuda.synchronize()
     entry = _hip.stop_timing()
     kern = _hip.stop_kernel_timing()
-    return wall, kern, entry, st
+    info["newton_iters"] = float(sum(float(x) for x in iters) / len(iters))
+    if learned:
+        info["fallback_rate"] = fallbacks / (steps * len(ic.x))
+    return wall, kern, entry, st, info
 
 
 def _per_step(d, steps):
@@ -141,7 +173,8 @@ def main():
     n = args.n
     print(f"bench: {n} particles, {args.precision}, warmup {args.warmup}, steps {args.steps}", file=sys.stderr,
           flush=True)  # fmt: skip
-    wall, kern, entry, st = run_gpu(ic, cfg, args.steps, args.warmup)
+    learned = args.precision == "mixed" and not args.no_learned
+    wall, kern, entry, st, info = run_gpu(ic, cfg, args.steps, args.warmup, learned=learned)
     value = n * args.steps / wall
     print(f"bench: {value:.4g} particle-steps/s", file=sys.stderr, flush=True)
     # algorithmic work of the pair kernels, from this run's own lists and counters
@@ -152,8 +185,10 @@ def main():
     fp64 = args.precision == "fp64"
     # kernel -> (bound, work per launch, unit)
     models = {
-        "k_gather_solve": ("mfma", dens, "TFLOP/s"),
-        "k_gather_solve_mixed": ("mfma", dens, "TFLOP/s"),
+        # neighbour gather (+ the first Newton pass): write each search entry (index, d2, mass; FP64 d2
+        # in the bit-exact layout), read each particle's candidate record once
+        "k_gather_mixed": ("hbm", (4 + 4 + 4) * n_search * n + 40 * n, "GB/s"),
+        "k_gather_solve": ("hbm", (4 + 8 + 8) * n_search * n + 32 * n, "GB/s"),
         "k_forces64": ("mfma", FLOPS_FORCE_PAIR * n_pairs * n, "TFLOP/s"),
         "k_forces_mixed": ("mfma", FLOPS_FORCE_PAIR * n_pairs * n, "TFLOP/s"),
         "k_gradients64": ("mfma", FLOPS_GRAD_PAIR * n_pairs * n, "TFLOP/s"),
@@ -182,7 +217,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if fp64 else "f32+f64",
+        "dtype": "f64" if fp64 else ("f32+f64+bf16" if learned else "f32+f64"),
         "data": "synthetic (seeded stream-crash initial conditions)",
         "config": {"workload": "stream crash, 2 x 524,288-particle streams (1,048,576)" if n == 1_048_576 else
                    f"stream crash, {n} particles", "precision": args.precision, "kernel": "cubic, eta 1.2",
@@ -196,15 +231,17 @@ def main():
                      "work": f"{work / n:.0f} {'FLOP' if bound != 'hbm' else 'B'}/particle per launch "
                              f"(n_s {n_search:.1f}, pairs {n_pairs:.1f}, Newton iterations {k_it:.2f})"},
         "kernels_ms_per_step": _per_step(kern, args.steps),
-        "entry_points_ms_per_step": _per_step(entry, args.steps),
+        "learned_h": info if learned else None,
+        "newton_iters": info["newton_iters"],
     }
     if not args.no_compare:
         other = "fp64" if args.precision == "mixed" else "mixed"
         print(f"bench: {other} alongside", file=sys.stderr, flush=True)
         del st
-        w2, k2, _, _ = run_gpu(ic, config(other), args.steps, args.warmup)
+        w2, k2, _, _, info2 = run_gpu(ic, config(other), args.steps, args.warmup)
         v2 = n * args.steps / w2
         out["compare"] = {other: {"value": v2, "ms_per_step": w2 / args.steps * 1e3, "value_ratio": value / v2,
+                                  "newton_iters": info2["newton_iters"],
                                   "kernels_ms_per_step": _per_step(k2, args.steps)}}  # fmt: skip
     if not args.no_cpu:
         print("bench: CPU baseline (FP64 oracle, OpenMP)", file=sys.stderr, flush=True)

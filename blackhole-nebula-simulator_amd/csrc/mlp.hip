@@ -77,15 +77,18 @@ __global__ void k_mlp_target(const double* __restrict__ h_prev, const double* __
 }
 
 // Newton starting h: h_prev exp(delta); the previous h when the prediction is not finite or
-// changes h by more than a factor 2 (fallback, counted)
+// changes h by more than a factor 2 (fallback, counted), and also when |delta| < gate: a change
+// below the Newton tolerance converges in one step from h_prev anyway.
 __global__ void k_mlp_h_start(const double* __restrict__ h_prev, const float* __restrict__ delta, int64_t n,
-                              double* __restrict__ h_start, long long* __restrict__ part)
+                              float gate, double* __restrict__ h_start, long long* __restrict__ part)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     long long fb = 0;
     if (i < n) {
         const float d = delta[i];
-        if (isfinite(d) && fabsf(d) < 0.69314718f) {
+        if (isfinite(d) && fabsf(d) < gate) {
+            h_start[i] = h_prev[i];
+        } else if (isfinite(d) && fabsf(d) < 0.69314718f) {
             h_start[i] = h_prev[i] * exp((double)d);
         } else {
             h_start[i] = h_prev[i];
@@ -93,6 +96,21 @@ __global__ void k_mlp_h_start(const double* __restrict__ h_prev, const float* __
         }
     }
     block_partial<false>(fb, part);
+}
+
+// *out = sum of the per-workgroup counts (one workgroup, fixed order)
+__global__ void __launch_bounds__(256) k_mlp_count(const long long* __restrict__ part, int64_t nb, long long* out)
+{
+    __shared__ long long sh[256];
+    long long v = 0;
+    for (int64_t b = threadIdx.x; b < nb; b += blockDim.x) v += part[b];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
 }
 
 // ---------------------------------------------------------------- inference (BF16 MFMA)
@@ -407,29 +425,19 @@ extern "C" int32_t nb_mlp_infer(nb_ctx* ctx, void* stream, const float* params, 
 }
 
 extern "C" int32_t nb_mlp_h_start(nb_ctx* ctx, void* stream, const double* h_prev, const float* delta, int64_t n,
-                                  double* h_start, int64_t* fallbacks_out)
+                                  float gate, double* h_start, long long* fallbacks_dev)
 {
-    if (!ctx || n < 0 || !h_prev || !delta || !h_start || !fallbacks_out)
+    if (!ctx || n < 0 || !h_prev || !delta || !h_start || !fallbacks_dev)
         return ctx_fail(ctx, "nb_mlp_h_start: bad arguments");
-    *fallbacks_out = 0;
-    if (n == 0) return NB_OK;
     hipStream_t s = (hipStream_t)stream;
     const int64_t nb = blocks_of(n, 256);
-    if (ctx_reserve(ctx, (void**)&ctx->part, &ctx->part_bytes, sizeof(long long) * (size_t)(nb + 1)) ||
-        ctx_reserve(ctx, (void**)&ctx->scal, &ctx->scal_bytes, sizeof(long long) * 16))
+    if (ctx_reserve(ctx, (void**)&ctx->mlp_part, &ctx->mlp_part_bytes, sizeof(long long) * (size_t)(nb + 1)))
         return NB_ERR_HIP;
-    hipLaunchKernelGGL(k_mlp_h_start, dim3((unsigned)nb), dim3(256), 0, s, h_prev, delta, n, h_start, ctx->part);
-    int32_t rc = mlp_launched(ctx, "nb_mlp_h_start");
-    if (rc) return rc;
-    // host sum of the per-workgroup counts
-    std::vector<long long> host((size_t)nb);
-    if (hipMemcpyAsync(host.data(), ctx->part, sizeof(long long) * nb, hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess)
-        return ctx_fail(ctx, "nb_mlp_h_start: count copy failed");
-    long long t = 0;
-    for (long long c : host) t += c;
-    *fallbacks_out = t;
-    return NB_OK;
+    if (n > 0)
+        hipLaunchKernelGGL(k_mlp_h_start, dim3((unsigned)nb), dim3(256), 0, s, h_prev, delta, n, gate, h_start,
+                           ctx->mlp_part);
+    hipLaunchKernelGGL(k_mlp_count, dim3(1), dim3(256), 0, s, ctx->mlp_part, nb, fallbacks_dev);
+    return mlp_launched(ctx, "nb_mlp_h_start");
 }
 
 extern "C" int32_t nb_mlp_grad(nb_ctx* ctx, void* stream, const float* params, const float* norm, const float* feat,

@@ -61,6 +61,7 @@ struct nb_ctx {
     void* posh = nullptr; size_t posh_bytes = 0;  // (x, y, z, h_search) records of the mixed gather
     void* mlp_scratch = nullptr; size_t mlp_scratch_bytes = 0;  // training activations and partials
     void* mlp_pack = nullptr; size_t mlp_pack_bytes = 0;        // BF16 MFMA operands of the network
+    long long* mlp_part = nullptr; size_t mlp_part_bytes = 0;   // fallback counts per workgroup
     // reductions
     double* red = nullptr; size_t red_bytes = 0;
     int64_t* scal = nullptr; size_t scal_bytes = 0;

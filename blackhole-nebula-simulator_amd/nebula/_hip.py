@@ -116,7 +116,7 @@ SIGNATURES = {
     "nb_mlp_features": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _f64, _f64, _vp]),
     "nb_mlp_target": (_i32, [_vp, _vp, _vp, _vp, _i64, _i64, _vp]),
     "nb_mlp_infer": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _vp]),
-    "nb_mlp_h_start": (_i32, [_vp, _vp, _vp, _vp, _i64, _vp, _P64]),
+    "nb_mlp_h_start": (_i32, [_vp, _vp, _vp, _vp, _i64, ctypes.c_float, _vp, _vp]),
     "nb_mlp_grad": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp]),
     "nb_mlp_train_step": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _i64, ctypes.c_float, _vp]),
     "nb_permute_many": (_i32, [_vp, _vp, _i32, _vp, _vp, _vp, _vp, _i64]),

@@ -107,7 +107,11 @@ class GasState:
     curl_v: torch.Tensor = field(default=None)
     nneigh: torch.Tensor = field(default=None)  # tight-list length per particle (float64)
     dt_prev: float = 0.0
-    learned_fallbacks: int = 0
+    learned_fallbacks_dev: torch.Tensor = field(default=None)  # (1,) int64: particles that fell back
+
+    @property
+    def learned_fallbacks(self) -> int:
+        return 0 if self.learned_fallbacks_dev is None else int(self.learned_fallbacks_dev.item())
 
     @property
     def e_radiated(self) -> float:
@@ -360,7 +364,7 @@ def step(st: GasState, cfg: CoupledConfig, dt: float) -> GasState:
     x1, p_flow, p_pred, u_half, u_pred, a_prev, mass, h, alpha, perm = moved[:10]
     pred = replace(st, t=st.t + dt, x=x1, p=p_flow, u=u_pred, mass=mass, h=h, alpha=alpha, perm=perm)
     h_start = feat = None
-    fallbacks = 0
+    fallbacks = None
     if model is not None:
         it = iter(moved[10:])
         rho_p, div_p, curl_p, nn_p = (None if t is None else next(it) for t in extra)
@@ -375,7 +379,7 @@ def step(st: GasState, cfg: CoupledConfig, dt: float) -> GasState:
     if model is not None and model.collect is not None:
         model.collect.add(feat, model.targets(h, new.h, model.collect.stride))
     return replace(new, p=_axpy(p_flow, new.a_hydro, half), u=_axpy(u_half, new.dudt, half), e_radiated_dev=e_rad,
-                   dt_prev=float(dt), learned_fallbacks=fallbacks)
+                   dt_prev=float(dt), learned_fallbacks_dev=fallbacks)
 
 
 def ledger(st: GasState, cfg: CoupledConfig) -> dict:

@@ -67,7 +67,7 @@ class LearnedH:
         self.adam_v = torch.zeros(n, dtype=torch.float32, device=dev)
         self.step_count = 0
         self.collect: PairLog | None = None
-        self.last_fallbacks = 0
+        self.last_fallbacks = None  # device count of the last h_start call
 
     # ------------------------------------------------------------------ data
     def features(self, h_prev, rho_prev, div_v, curl_v, nneigh, x, u, dt, dt_prev, stride: int = 1) -> torch.Tensor:
@@ -94,14 +94,17 @@ class LearnedH:
         _hip.call("nb_mlp_infer", _hip.ptr(self.params), _hip.ptr(self.norm), _hip.ptr(feat), n, _hip.ptr(out))
         return out
 
-    def h_start(self, h_prev: torch.Tensor, feat: torch.Tensor) -> torch.Tensor:
-        """Starting h of the Newton solve; counts the fallbacks into ``last_fallbacks``."""
+    def h_start(self, h_prev: torch.Tensor, feat: torch.Tensor, gate: float = 1e-4) -> torch.Tensor:
+        """Starting h of the Newton solve: h_prev exp(delta), or h_prev where the predicted change
+        is below ``gate`` (the Newton tolerance: one step from h_prev suffices there).
+        ``last_fallbacks`` (a device int64 tensor, read lazily) counts the particles whose
+        prediction was rejected (not finite, or a change by a factor 2 or more)."""
         delta = self.predict(feat)
         out = torch.empty_like(h_prev)
-        fb = np.zeros(1, np.int64)
-        _hip.call("nb_mlp_h_start", _hip.ptr(h_prev), _hip.ptr(delta), int(h_prev.shape[0]), _hip.ptr(out),
-                  fb.ctypes.data_as(_hip._P64))  # fmt: skip
-        self.last_fallbacks = int(fb[0])
+        fb = _dev.empty(1, torch.int64)
+        _hip.call("nb_mlp_h_start", _hip.ptr(h_prev), _hip.ptr(delta), int(h_prev.shape[0]), float(gate),
+                  _hip.ptr(out), _hip.ptr(fb))  # fmt: skip
+        self.last_fallbacks = fb
         return out
 
     # ------------------------------------------------------------------ training

@@ -314,10 +314,12 @@ int32_t nb_mlp_target(nb_ctx *ctx, void *stream, const double *h_prev, const dou
 /* delta for n feature rows: BF16 MFMA (FP32 accumulation) */
 int32_t nb_mlp_infer(nb_ctx *ctx, void *stream, const float *params, const float *norm, const float *feat, int64_t n,
                      float *out);
-/* Newton starting h = h_prev exp(delta); the previous h where delta is not finite or |delta| >= ln 2
- * (fallback: *fallbacks_out, host, counts them) */
-int32_t nb_mlp_h_start(nb_ctx *ctx, void *stream, const double *h_prev, const float *delta, int64_t n,
-                       double *h_start, int64_t *fallbacks_out);
+/* Newton starting h = h_prev exp(delta). The previous h where |delta| < gate (a change below the
+ * Newton tolerance converges in one step from it anyway), and where delta is not finite or
+ * |delta| >= ln 2 (fallbacks: their number, written to the device int64 *fallbacks_dev; no host
+ * synchronisation). */
+int32_t nb_mlp_h_start(nb_ctx *ctx, void *stream, const double *h_prev, const float *delta, int64_t n, float gate,
+                       double *h_start, long long *fallbacks_dev);
 /* FP32 gradient of the mean squared error (normalised target) over a batch, and the loss */
 int32_t nb_mlp_grad(nb_ctx *ctx, void *stream, const float *params, const float *norm, const float *feat,
                     const float *target, int64_t batch, float *grad_out, float *loss_out);

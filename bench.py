@@ -49,6 +49,9 @@ def parse():
     ap.add_argument("--no-learned", action="store_true", help="mixed mode without the learned starting h")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--ensemble", action="store_true",
+                    help="the batched-ensemble workload (64 clouds) against sequential single-cloud runs")
+    ap.add_argument("--ensemble-clouds", type=int, default=64)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy 5 times (scales the PMC byte counters)")
     return ap.parse_args()
@@ -145,6 +148,77 @@ def run_gpu(ic, cfg, steps, warmup, learned=False, log_stride=8, train_steps=300
     return wall, kern, entry, st, info
 
 
+def run_ensemble(args):
+    """64 clouds of the plain cloud physics (particle counts log-uniform in [8192, 262144],
+    pericentres 1000-4000, radii 100-200), stepped together with a step per cloud, against the same
+    clouds stepped one after the other. Cell factor and tile come from the autotuner."""
+    import numpy as np
+    import torch
+
+    from nebula.sim import autotune, coupled
+    from nebula.sim.ensemble import Ensemble, ensemble_ics
+
+    cfg = coupled.baseline_config(precision="mixed")
+    ics = ensemble_ics(args.ensemble_clouds, 8192, 262144)
+    sizes = np.array([len(ic.x) for ic in ics])
+    total = int(sizes.sum())
+    print(f"bench: ensemble of {len(ics)} clouds, {total} particles", file=sys.stderr, flush=True)
+    ens = Ensemble(ics, cfg)
+    choice = autotune.tune(ens.state, cfg, repeats=2)
+    ens.state = autotune.apply(ens.state, choice)
+    print(f"bench: autotuned cell factor {choice.cell_factor}, tile {choice.tile}", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        ens.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ens.step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    batched = total * args.steps / wall
+    print(f"bench: batched {batched:.4g} particle-steps/s", file=sys.stderr, flush=True)
+    del ens
+    # sequential: each cloud alone (same tuned settings), timed over a few steps after one warm-up
+    seq_steps = max(2, min(5, args.steps))
+    per_cloud_ms = []
+    for k, ic in enumerate(ics):
+        st = coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg)
+        st = replace(st, cell_factor=choice.cell_factor)
+        st = coupled.step(st, cfg, coupled.stable_dt(st, cfg))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(seq_steps):
+            st = coupled.step(st, cfg, coupled.stable_dt(st, cfg))
+        torch.cuda.synchronize()
+        per_cloud_ms.append((time.perf_counter() - t0) / seq_steps * 1e3)
+        if k % 16 == 15:
+            print(f"bench: sequential clouds {k + 1}/{len(ics)}", file=sys.stderr, flush=True)
+    seq_ms = float(np.sum(per_cloud_ms))
+    sequential = total / (seq_ms / 1e3)
+    out = {
+        "metric": "batched-ensemble particle-steps/s (64 clouds, one launch per pass) vs 64 sequential runs",
+        "value": batched,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (seeded clouds on parabolic orbits)",
+        "config": {"workload": f"ensemble of {len(ics)} clouds, {total} particles "
+                               f"({int(sizes.min())}..{int(sizes.max())} per cloud)",
+                   "physics": "cubic, eta 1.2, Monaghan alpha 1 beta 2, no cooling", "precision": "mixed",
+                   "cell_factor": choice.cell_factor, "tile": choice.tile},
+        "sequential": {"value": sequential, "ms_per_ensemble_step": seq_ms, "steps_per_cloud": seq_steps,
+                       "speedup_batched": batched / sequential},
+        "autotune_ms_per_hydro_pass": {f"{f}x{t}": round(v, 3) for (f, t), v in choice.table.items()},
+    }
+    print(json.dumps(out))
+
+
 def _per_step(d, steps):
     return {k: round(v[1] / steps, 4) for k, v in sorted(d.items(), key=lambda kv: -kv[1][1])}
 
@@ -158,6 +232,9 @@ def main():
     from nebula import _hip
 
     dev = _hip.torch_device()
+    if args.ensemble:
+        run_ensemble(args)
+        return
     if args.copy_calibration:
         import torch
 

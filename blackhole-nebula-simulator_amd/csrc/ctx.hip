@@ -80,6 +80,13 @@ void ktime_flush(nb_ctx* ctx)
     ctx->ev_open.clear();
 }
 
+extern "C" int32_t nb_set_tile(nb_ctx* ctx, int32_t tile)
+{
+    if (!ctx || (tile != 64 && tile != 128 && tile != 256)) return ctx_fail(ctx, "nb_set_tile: 64, 128 or 256");
+    ctx->tile = tile;
+    return NB_OK;
+}
+
 extern "C" int32_t nb_timing(nb_ctx* ctx, int32_t enable)
 {
     if (!ctx) return NB_ERR_ARG;
@@ -122,7 +129,7 @@ extern "C" void nb_ctx_destroy(nb_ctx* ctx)
     for (hipEvent_t e : ctx->ev_pool) hipEventDestroy(e);
     void* bufs[] = {ctx->cell_id, ctx->keys_out, ctx->iota, ctx->order, ctx->cell_count, ctx->cell_start,
                     ctx->gbuf, ctx->rowcnt, ctx->unsorted, ctx->pending, ctx->edbuf, ctx->ecount, ctx->eoff, ctx->efill, ctx->ebuf, ctx->tot,
-                    ctx->offs, ctx->tcount, ctx->dscratch, ctx->fscratch, ctx->pack, ctx->part, ctx->posh, ctx->mlp_scratch, ctx->mlp_pack, ctx->mlp_part, ctx->red, ctx->scal, ctx->cub_tmp};
+                    ctx->offs, ctx->tcount, ctx->dscratch, ctx->fscratch, ctx->pack, ctx->part, ctx->posh, ctx->mlp_scratch, ctx->mlp_pack, ctx->mlp_part, ctx->segs, ctx->red, ctx->scal, ctx->cub_tmp};
     for (void* b : bufs)
         if (b) hipFree(b);
     delete ctx;
@@ -138,7 +145,7 @@ extern "C" int64_t nb_ctx_scratch_bytes(const nb_ctx* ctx)
     size_t s = ctx->cell_id_bytes + ctx->keys_out_bytes + ctx->iota_bytes + ctx->order_bytes +
                ctx->cell_count_bytes + ctx->cell_start_bytes + ctx->gbuf_bytes + ctx->rowcnt_bytes + ctx->unsorted_bytes + ctx->pending_bytes + ctx->edbuf_bytes +
                ctx->ecount_bytes + ctx->eoff_bytes + ctx->efill_bytes + ctx->ebuf_bytes + ctx->tot_bytes +
-               ctx->offs_bytes + ctx->tcount_bytes + ctx->dscratch_bytes + ctx->fscratch_bytes + ctx->pack_bytes + ctx->part_bytes + ctx->posh_bytes + ctx->mlp_scratch_bytes + ctx->mlp_pack_bytes + ctx->mlp_part_bytes +
+               ctx->offs_bytes + ctx->tcount_bytes + ctx->dscratch_bytes + ctx->fscratch_bytes + ctx->pack_bytes + ctx->part_bytes + ctx->posh_bytes + ctx->mlp_scratch_bytes + ctx->mlp_pack_bytes + ctx->mlp_part_bytes + ctx->segs_bytes +
                ctx->red_bytes + ctx->scal_bytes + ctx->cub_tmp_bytes;
     return (int64_t)s;
 }

@@ -143,6 +143,13 @@ struct Grid {
     int32_t nc[3];
 };
 
+// One particle system of an ensemble in the segmented grid: its own cells, numbered from base.
+struct GridSeg {
+    Grid g;
+    uint32_t base;
+    uint32_t pad;
+};
+
 __device__ __forceinline__ int32_t cell_of(double x, const Grid& g, int c)
 {
     double f = floor((x - g.lo[c]) * g.inv[c]);
@@ -157,6 +164,75 @@ __device__ __forceinline__ uint32_t cell_key(const double* pos, int64_t i, const
     const int32_t cy = cell_of(wrapped(pos, i, 1, box), g, 1);
     const int32_t cz = cell_of(wrapped(pos, i, 2, box), g, 2);
     return ((uint32_t)cx * (uint32_t)g.nc[1] + (uint32_t)cy) * (uint32_t)g.nc[2] + (uint32_t)cz;
+}
+
+// Bounding box, largest h and the fixed-point sum of h (as k_bbox / k_hsum) of each segment
+// [off[s], off[s + 1]): one workgroup per segment; out[9 s + 0..6] box and h_max, the u64 sum in
+// out[9 s + 8].
+__global__ void __launch_bounds__(kBlock)
+k_seg_bbox(const double* __restrict__ pos, const double* __restrict__ h, const int64_t* __restrict__ off,
+           double* __restrict__ out)
+{
+    const int64_t a = off[blockIdx.x], b = off[blockIdx.x + 1];
+    double v[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0};
+    for (int64_t i = a + threadIdx.x; i < b; i += blockDim.x) {
+        for (int c = 0; c < 3; c++) {
+            v[c] = fmin(v[c], pos[3 * i + c]);
+            v[3 + c] = fmax(v[3 + c], pos[3 * i + c]);
+        }
+        v[6] = fmax(v[6], h[i]);
+    }
+    __shared__ double sh[7][kBlock];
+    __shared__ unsigned long long su[kBlock];
+    for (int k = 0; k < 7; k++) sh[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            for (int k = 0; k < 3; k++) sh[k][threadIdx.x] = fmin(sh[k][threadIdx.x], sh[k][threadIdx.x + st]);
+            for (int k = 3; k < 7; k++) sh[k][threadIdx.x] = fmax(sh[k][threadIdx.x], sh[k][threadIdx.x + st]);
+        }
+        __syncthreads();
+    }
+    const double hmax = sh[6][0];
+    const double scale = hmax > 0.0 ? 4294967296.0 / hmax : 0.0;
+    unsigned long long hs = 0;
+    for (int64_t i = a + threadIdx.x; i < b; i += blockDim.x) hs += (unsigned long long)(h[i] * scale);
+    su[threadIdx.x] = hs;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) su[threadIdx.x] += su[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 7; k++) out[9 * blockIdx.x + k] = sh[k][0];
+        out[9 * blockIdx.x + 7] = 0.0;
+        ((unsigned long long*)out)[9 * blockIdx.x + 8] = su[0];
+    }
+}
+
+__device__ __forceinline__ uint32_t seg_key(const double* pos, int64_t i, const GridSeg* segs, const int32_t* cloud)
+{
+    const GridSeg& G = segs[cloud[i]];
+    const Box open{{0.0, 0.0, 0.0}};
+    return G.base + cell_key(pos, i, open, G.g);
+}
+
+// segmented cell ids (global numbering), as k_cell_id
+__global__ void __launch_bounds__(kBlock)
+k_cell_id_seg(const double* __restrict__ pos, int64_t n, const GridSeg* __restrict__ segs,
+              const int32_t* __restrict__ cloud, uint32_t* __restrict__ cell_id, uint32_t* __restrict__ iota,
+              uint32_t* __restrict__ cell_count, long long* __restrict__ part)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    long long bad = 0;
+    if (i < n) {
+        const uint32_t c = seg_key(pos, i, segs, cloud);
+        cell_id[i] = c;
+        iota[i] = (uint32_t)i;
+        atomicAdd(&cell_count[c], 1u);
+        if (part && i + 1 < n && seg_key(pos, i + 1, segs, cloud) < c) bad = 1;
+    }
+    if (part) block_partial<false>(bad, part);
 }
 
 // cell of each particle, particles per cell, and (part, one partial per workgroup) the number of
@@ -965,10 +1041,13 @@ __device__ __forceinline__ bool column_zrange(double ux, double uy, double zc, d
 }
 
 // Extras (pairs seen only from the other side, d2 >= (2 h_search_j)^2) go straight to row j's
-// extras segment (capacity ecap, filled with an atomic counter efill[j]).
-template <bool kMasses>
+// extras segment (capacity ecap, filled with an atomic counter efill[j]). kSeg: an ensemble of
+// independent systems; particle i searches only the cells of its own system cloud[i], whose grid
+// and first cell come from segs.
+template <bool kMasses, bool kSeg>
 __global__ void __launch_bounds__(256)
-k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g, const uint32_t* __restrict__ start,
+k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const GridSeg* __restrict__ segs,
+               const int32_t* __restrict__ cloud, const uint32_t* __restrict__ start,
                const uint32_t* __restrict__ order, uint32_t* __restrict__ ridx, float* __restrict__ rd2,
                float* __restrict__ rmass, int64_t cap, int32_t* __restrict__ rowcnt,
                unsigned int* __restrict__ efill, uint32_t* __restrict__ ebuf, float* __restrict__ ed2, int64_t ecap,
@@ -979,6 +1058,8 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g, const uint32
     const int64_t i = (xcd_tile() * (int64_t)blockDim.x + threadIdx.x) / kG;
     const bool live = i < n;
     const int64_t ii = live ? i : n - 1;
+    const Grid g = kSeg ? segs[cloud[ii]].g : g_one;
+    const uint32_t cbase = kSeg ? segs[cloud[ii]].base : 0u;
     const double4 pi = posh[ii];
     const double xi = pi.x, yi = pi.y, zi = pi.z, hs = pi.w;
     const double r = NB_R_KERN * hs;
@@ -1007,12 +1088,12 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g, const uint32
         int64_t z0, z1;
         const int64_t ca = c0 + glane, cb = c0 + kG + glane;
         if (ca < ncol && column_zrange(ux, uy, zi, r2, ax + ca / ny, ay + ca % ny, g, z0, z1)) {
-            const uint32_t base = ((uint32_t)(ax + ca / ny) * nc1 + (uint32_t)(ay + ca % ny)) * nc2;
+            const uint32_t base = cbase + ((uint32_t)(ax + ca / ny) * nc1 + (uint32_t)(ay + ca % ny)) * nc2;
             k0a = start[base + (uint32_t)z0];
             lena = start[base + (uint32_t)z1 + 1u] - k0a;
         }
         if (cb < ncol && column_zrange(ux, uy, zi, r2, ax + cb / ny, ay + cb % ny, g, z0, z1)) {
-            const uint32_t base = ((uint32_t)(ax + cb / ny) * nc1 + (uint32_t)(ay + cb % ny)) * nc2;
+            const uint32_t base = cbase + ((uint32_t)(ax + cb / ny) * nc1 + (uint32_t)(ay + cb % ny)) * nc2;
             k0b = start[base + (uint32_t)z0];
             lenb = start[base + (uint32_t)z1 + 1u] - k0b;
         }
@@ -1388,7 +1469,8 @@ Box make_box(const double* box3)
 // of the collected scalars (ctx->scal, 16 values).
 int reserve_reductions(nb_ctx* ctx, int64_t n)
 {
-    if (ctx_reserve(ctx, (void**)&ctx->part, &ctx->part_bytes, sizeof(long long) * 4 * (size_t)(n / 16 + 1024)) ||
+    // 4 slots for launches of up to n / 4 + 1 workgroups (lane groups of 16 in 64-thread workgroups)
+    if (ctx_reserve(ctx, (void**)&ctx->part, &ctx->part_bytes, sizeof(long long) * 4 * (size_t)(n / 4 + 1024)) ||
         ctx_reserve(ctx, (void**)&ctx->scal, &ctx->scal_bytes, sizeof(long long) * 16))
         return NB_ERR_HIP;
     return NB_OK;
@@ -1411,7 +1493,7 @@ inline const uint32_t* order_arg(const nb_ctx* ctx) { return ctx->order_ident ? 
 // cell_start). check_sorted: when the particles are already stored in cell order (the step
 // keeps them so) the sort is skipped and ctx->order_ident set.
 int build_grid(nb_ctx* ctx, hipStream_t s, const double* pos, const double* h, int64_t n, const Box& box, Grid& g,
-               int64_t& ncells, bool check_sorted)
+               int64_t& ncells, bool check_sorted, double cell_factor = 1.0)
 {
     // every list builder starts here and reuses the context's list buffers: lists of an
     // earlier build can no longer be emitted
@@ -1446,7 +1528,7 @@ int build_grid(nb_ctx* ctx, hipStream_t s, const double* pos, const double* h, i
     const double scale = hmax > 0.0 ? 4294967296.0 / hmax : 0.0;
     const double hmean = scale > 0.0 ? ((double)hs / scale) / (double)n : 0.0;
     const double maxspan = std::max(span[0], std::max(span[1], span[2]));
-    double cell = std::max(std::max(hmean, maxspan / (double)kMaxAxisCells), 1e-300);
+    double cell = std::max(std::max(cell_factor * hmean, maxspan / (double)kMaxAxisCells), 1e-300);
     for (;;) {
         double total = 1.0;
         for (int c = 0; c < 3; c++) {
@@ -1482,6 +1564,114 @@ int build_grid(nb_ctx* ctx, hipStream_t s, const double* pos, const double* h, i
         finish_partials(ctx, s, grid_of(n), 1, 0u, 0);
         long long bad = 0;
         rc = sync_copy(ctx, s, &bad, ctx->scal, sizeof(bad), "grid: order check failed");
+        if (rc) return rc;
+        sorted = bad == 0;
+    }
+    if (!sorted) {
+        int end_bit = 1;
+        while ((int64_t(1) << end_bit) < ncells && end_bit < 32) end_bit++;
+        size_t bytes = 0;
+        if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, ctx->cell_id, ctx->keys_out, ctx->iota, ctx->order,
+                                               (int)n, 0, end_bit, s) != hipSuccess)
+            return ctx_fail(ctx, "hipcub sort sizing failed");
+        if (ctx_reserve(ctx, (void**)&ctx->cub_tmp, &ctx->cub_tmp_bytes, bytes)) return NB_ERR_HIP;
+        if (hipcub::DeviceRadixSort::SortPairs(ctx->cub_tmp, bytes, ctx->cell_id, ctx->keys_out, ctx->iota, ctx->order,
+                                               (int)n, 0, end_bit, s) != hipSuccess)
+            return ctx_fail(ctx, "hipcub sort failed");
+    }
+    ctx->order_ident = sorted ? 1 : 0;
+    size_t bytes = 0;
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, ctx->cell_count, ctx->cell_start, (int)(ncells + 1), s) !=
+        hipSuccess)
+        return ctx_fail(ctx, "hipcub scan sizing failed");
+    if (ctx_reserve(ctx, (void**)&ctx->cub_tmp, &ctx->cub_tmp_bytes, bytes)) return NB_ERR_HIP;
+    if (hipcub::DeviceScan::ExclusiveSum(ctx->cub_tmp, bytes, ctx->cell_count, ctx->cell_start, (int)(ncells + 1), s) !=
+        hipSuccess)
+        return ctx_fail(ctx, "hipcub scan failed");
+    return NB_OK;
+}
+
+// Grids of the segments of an ensemble (open boxes), each planned as build_grid plans one grid
+// from its own box and search lengths, numbered one after the other (GridSeg.base), and the
+// particles' order by global cell id (ctx->order, cell_start, order_ident as build_grid). cloud[i]:
+// the segment of particle i; off (host, nseg + 1): the segments' particle ranges.
+int build_grid_seg(nb_ctx* ctx, hipStream_t s, const double* pos, const double* h, int64_t n, const int32_t* cloud,
+                   const int64_t* off, int32_t nseg, double cell_factor, bool check_sorted, int64_t& ncells)
+{
+    ctx->lists_ready = 0;
+    ctx->hydro_ready = 0;
+    ctx->hydro_mixed_ready = 0;
+    ctx->order_ident = 0;
+    if (nseg < 1 || off[0] != 0 || off[nseg] != n) return ctx_fail(ctx, "segmented grid: bad segment offsets");
+    for (int k = 0; k < nseg; k++)
+        if (off[k + 1] <= off[k]) return ctx_fail(ctx, "segmented grid: empty segment");
+    if (reserve_reductions(ctx, n) ||
+        ctx_reserve(ctx, (void**)&ctx->red, &ctx->red_bytes, sizeof(double) * 9 * nseg + sizeof(int64_t) * (nseg + 1)) ||
+        ctx_reserve(ctx, (void**)&ctx->segs, &ctx->segs_bytes, sizeof(GridSeg) * nseg))
+        return NB_ERR_HIP;
+    int64_t* off_dev = (int64_t*)(ctx->red + 9 * nseg);
+    if (hipMemcpyAsync(off_dev, off, sizeof(int64_t) * (nseg + 1), hipMemcpyHostToDevice, s) != hipSuccess)
+        return ctx_fail(ctx, "segmented grid: offset copy failed");
+    hipLaunchKernelGGL(k_seg_bbox, dim3(nseg), dim3(kBlock), 0, s, pos, h, off_dev, ctx->red);
+    std::vector<double> bb((size_t)9 * nseg);
+    int rc = sync_copy(ctx, s, bb.data(), ctx->red, sizeof(double) * 9 * nseg, "segmented grid: box copy failed");
+    if (rc) return rc;
+    ctx->host_segs.resize((size_t)nseg * sizeof(GridSeg));
+    GridSeg* hs = (GridSeg*)ctx->host_segs.data();
+    double total_cells = 0.0;
+    for (int k = 0; k < nseg; k++) {
+        const double* b = &bb[(size_t)9 * k];
+        unsigned long long hsum;
+        std::memcpy(&hsum, &b[8], sizeof(hsum));
+        const double hmax = b[6];
+        if (!std::isfinite(b[0] + b[1] + b[2] + b[3] + b[4] + b[5]) || !std::isfinite(hmax))
+            return ctx_fail(ctx, "segmented grid: non-finite positions or smoothing lengths");
+        double span[3];
+        for (int c = 0; c < 3; c++) {
+            hs[k].g.lo[c] = b[c];
+            span[c] = b[3 + c] - b[c];
+        }
+        const double scale = hmax > 0.0 ? 4294967296.0 / hmax : 0.0;
+        const double hmean = scale > 0.0 ? ((double)hsum / scale) / (double)(off[k + 1] - off[k]) : 0.0;
+        const double maxspan = std::max(span[0], std::max(span[1], span[2]));
+        double cell = std::max(std::max(cell_factor * hmean, maxspan / (double)kMaxAxisCells), 1e-300);
+        double cells;
+        for (;;) {
+            cells = 1.0;
+            for (int c = 0; c < 3; c++) {
+                hs[k].g.nc[c] = (int32_t)std::max(std::floor(span[c] / cell) + 1.0, 1.0);
+                hs[k].g.inv[c] = 1.0 / cell;
+                cells *= (double)hs[k].g.nc[c];
+            }
+            if (cells <= kMaxTotalCells) break;
+            cell *= 1.25;
+        }
+        hs[k].base = (uint32_t)total_cells;
+        hs[k].pad = 0;
+        total_cells += cells;
+        if (total_cells > 2147483648.0) return ctx_fail(ctx, "segmented grid: more than 2^31 cells in total");
+    }
+    ncells = (int64_t)total_cells;
+    ctx->last_grid_cells = ncells;
+    if (hipMemcpyAsync(ctx->segs, hs, sizeof(GridSeg) * nseg, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return ctx_fail(ctx, "segmented grid: table copy failed");
+    if (ctx_reserve(ctx, (void**)&ctx->cell_id, &ctx->cell_id_bytes, sizeof(uint32_t) * n) ||
+        ctx_reserve(ctx, (void**)&ctx->order, &ctx->order_bytes, sizeof(uint32_t) * n) ||
+        ctx_reserve(ctx, (void**)&ctx->keys_out, &ctx->keys_out_bytes, sizeof(uint32_t) * n) ||
+        ctx_reserve(ctx, (void**)&ctx->iota, &ctx->iota_bytes, sizeof(uint32_t) * n) ||
+        ctx_reserve(ctx, (void**)&ctx->cell_count, &ctx->cell_count_bytes, sizeof(uint32_t) * (ncells + 1)) ||
+        ctx_reserve(ctx, (void**)&ctx->cell_start, &ctx->cell_start_bytes, sizeof(uint32_t) * (ncells + 1)))
+        return NB_ERR_HIP;
+    if (hipMemsetAsync(ctx->cell_count, 0, sizeof(uint32_t) * (ncells + 1), s) != hipSuccess)
+        return ctx_fail(ctx, "memset cell_count");
+    NB_TIMED(ctx, s, "k_cell_id", k_cell_id_seg, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, n, (const GridSeg*)ctx->segs,
+             cloud, ctx->cell_id, ctx->iota, ctx->cell_count, check_sorted ? ctx->part : nullptr);
+    bool sorted = false;
+    if (check_sorted) {
+        finish_partials(ctx, s, grid_of(n), 1, 0u, 0);
+        long long bad = 0;
+        rc = sync_copy(ctx, s, &bad, ctx->scal, sizeof(bad), "segmented grid: order check failed");
         if (rc) return rc;
         sorted = bad == 0;
     }
@@ -1572,18 +1762,34 @@ int gather_stage(nb_ctx* ctx, hipStream_t s, const double* pos, const double* h,
 }  // namespace
 
 extern "C" int32_t nb_grid_order(nb_ctx* ctx, void* stream, const double* pos, const double* h, int64_t n,
-                                 const double* box3, int32_t* order_out)
+                                 const double* box3, double cell_factor, int32_t* order_out)
 {
     hipStream_t s = (hipStream_t)stream;
-    if (!ctx || n < 0 || !order_out) return ctx_fail(ctx, "nb_grid_order: bad arguments");
+    if (!ctx || n < 0 || !order_out || !(cell_factor > 0.0)) return ctx_fail(ctx, "nb_grid_order: bad arguments");
     if (n == 0) return NB_OK;
     if (n >= (int64_t)kMask) return ctx_fail(ctx, "nb_grid_order: too many particles");
     Grid g;
     int64_t ncells;
-    int rc = build_grid(ctx, s, pos, h, n, make_box(box3), g, ncells, false);
+    int rc = build_grid(ctx, s, pos, h, n, make_box(box3), g, ncells, false, cell_factor);
     if (rc) return rc;
     if (hipMemcpyAsync(order_out, ctx->order, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s) != hipSuccess)
         return ctx_fail(ctx, "nb_grid_order: copy failed");
+    return NB_OK;
+}
+
+extern "C" int32_t nb_grid_order_seg(nb_ctx* ctx, void* stream, const double* pos, const double* h, int64_t n,
+                                     const int32_t* cloud, const int64_t* seg_off, int32_t nseg, double cell_factor,
+                                     int32_t* order_out)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (!ctx || n <= 0 || !order_out || !cloud || !seg_off || !(cell_factor > 0.0))
+        return ctx_fail(ctx, "nb_grid_order_seg: bad arguments");
+    if (n >= (int64_t)kMask) return ctx_fail(ctx, "nb_grid_order_seg: too many particles");
+    int64_t ncells;
+    int rc = build_grid_seg(ctx, s, pos, h, n, cloud, seg_off, nseg, cell_factor, false, ncells);
+    if (rc) return rc;
+    if (hipMemcpyAsync(order_out, ctx->order, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return ctx_fail(ctx, "nb_grid_order_seg: copy failed");
     return NB_OK;
 }
 
@@ -1884,16 +2090,20 @@ extern "C" int32_t nb_hydro_tight(nb_ctx* ctx, void* stream, const double* h_fin
 
 extern "C" int32_t nb_hydro_density_mixed(nb_ctx* ctx, void* stream, const double* pos, const double* mass,
                                           double m_uniform, const double* h_start, const double* h_search, int64_t n,
-                                          int32_t kid, double cnorm, double eta, double tol, double h_max,
-                                          double* h_out, double* rho, double* omega, int32_t* iters, int32_t* flags,
-                                          int64_t* stats4)
+                                          const int32_t* cloud, const int64_t* seg_off, int32_t nseg,
+                                          double cell_factor, int32_t kid, double cnorm, double eta, double tol,
+                                          double h_max, double* h_out, double* rho, double* omega, int32_t* iters,
+                                          int32_t* flags, int64_t* stats4)
 {
     hipStream_t s = (hipStream_t)stream;
-    if (!ctx || n <= 0 || !stats4 || n >= (int64_t)kMask) return ctx_fail(ctx, "nb_hydro_density_mixed: bad arguments");
+    if (!ctx || n <= 0 || !stats4 || n >= (int64_t)kMask || !(cell_factor > 0.0) || (cloud && !seg_off))
+        return ctx_fail(ctx, "nb_hydro_density_mixed: bad arguments");
     const Box open{{0.0, 0.0, 0.0}};
-    Grid g;
+    Grid g{};
     int64_t ncells = 0;
-    int rc = build_grid(ctx, s, pos, h_search, n, open, g, ncells, true);
+    const bool segmented = cloud != nullptr;
+    int rc = segmented ? build_grid_seg(ctx, s, pos, h_search, n, cloud, seg_off, nseg, cell_factor, true, ncells)
+                       : build_grid(ctx, s, pos, h_search, n, open, g, ncells, true, cell_factor);
     if (rc) return rc;
     const bool masses = !(m_uniform > 0.0);
     if (ctx_reserve(ctx, (void**)&ctx->rowcnt, &ctx->rowcnt_bytes, sizeof(int32_t) * n) ||
@@ -1908,7 +2118,8 @@ extern "C" int32_t nb_hydro_density_mixed(nb_ctx* ctx, void* stream, const doubl
     long long sc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     MixArgs A{mass, m_uniform, h_start, h_max, kid, cnorm, eta, tol, h_out, rho, omega, iters, flags, ctx->pending,
               ctx->part};
-    const int64_t blocks = (n * kG + 255) / 256;
+    const int tile = ctx->tile;  // workgroup size of the lane-group kernels (autotuned: 64, 128 or 256)
+    const int64_t blocks = (n * kG + tile - 1) / tile;
     for (int attempt = 0; attempt < 8; attempt++) {
         const size_t cells = (size_t)(n * cap);
         const size_t ecells = (size_t)(n * ecap);
@@ -1919,14 +2130,16 @@ extern "C" int32_t nb_hydro_density_mixed(nb_ctx* ctx, void* stream, const doubl
             return NB_ERR_HIP;
         if (hipMemsetAsync(ctx->efill, 0, sizeof(unsigned int) * n, s) != hipSuccess) return ctx_fail(ctx, "memset");
         float* rmass = masses ? ctx->fscratch + cells : nullptr;
-        if (masses)
-            NB_TIMED(ctx, s, "k_gather_mixed", k_gather_mixed<true>, dim3(blocks), dim3(256), 0, s,
-                     (const double4*)ctx->posh, n, g, ctx->cell_start, order_arg(ctx), ctx->gbuf, ctx->fscratch, rmass,
-                     cap, ctx->rowcnt, ctx->efill, ctx->ebuf, (float*)ctx->edbuf, ecap, A);
-        else
-            NB_TIMED(ctx, s, "k_gather_mixed", k_gather_mixed<false>, dim3(blocks), dim3(256), 0, s,
-                     (const double4*)ctx->posh, n, g, ctx->cell_start, order_arg(ctx), ctx->gbuf, ctx->fscratch, rmass,
-                     cap, ctx->rowcnt, ctx->efill, ctx->ebuf, (float*)ctx->edbuf, ecap, A);
+        const GridSeg* segs = (const GridSeg*)ctx->segs;
+#define NB_GATHER_MIXED(M, S)                                                                                        \
+    NB_TIMED(ctx, s, "k_gather_mixed", (k_gather_mixed<M, S>), dim3(blocks), dim3(tile), 0, s,                      \
+             (const double4*)ctx->posh, n, g, segs, cloud, ctx->cell_start, order_arg(ctx), ctx->gbuf, ctx->fscratch, \
+             rmass, cap, ctx->rowcnt, ctx->efill, ctx->ebuf, (float*)ctx->edbuf, ecap, A)
+        if (masses && segmented) NB_GATHER_MIXED(true, true);
+        else if (masses) NB_GATHER_MIXED(true, false);
+        else if (segmented) NB_GATHER_MIXED(false, true);
+        else NB_GATHER_MIXED(false, false);
+#undef NB_GATHER_MIXED
         // [0] longest gather row, [1] rows left slow, [2] longest extras segment
         finish_partials(ctx, s, blocks, 3, 5u, 0);
         rc = sync_copy(ctx, s, sc, ctx->scal, sizeof(long long) * 3, "nb_hydro_density_mixed: gather failed");
@@ -1943,13 +2156,13 @@ extern "C" int32_t nb_hydro_density_mixed(nb_ctx* ctx, void* stream, const doubl
     float* rmass = masses ? ctx->fscratch + cells : nullptr;
     if (slow_rows > 0) {
         if (masses)
-            NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<true>, dim3(blocks), dim3(256), 0, s, h_search, n,
+            NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<true>, dim3(blocks), dim3(tile), 0, s, h_search, n,
                      ctx->fscratch, rmass, cap, ctx->rowcnt, A);
         else
-            NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<false>, dim3(blocks), dim3(256), 0, s, h_search,
+            NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<false>, dim3(blocks), dim3(tile), 0, s, h_search,
                      n, ctx->fscratch, rmass, cap, ctx->rowcnt, A);
     }
-    NB_TIMED(ctx, s, "k_sort_extras", k_sort_extras, dim3(blocks), dim3(256), 0, s, ctx->ebuf, (float*)ctx->edbuf,
+    NB_TIMED(ctx, s, "k_sort_extras", k_sort_extras, dim3(blocks), dim3(tile), 0, s, ctx->ebuf, (float*)ctx->edbuf,
              ctx->efill, ecap, n);
     NB_TIMED(ctx, s, "k_row_total_rm", k_row_total_rm, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->rowcnt, ctx->efill,
              ecap, ctx->part, n);
@@ -1986,8 +2199,11 @@ extern "C" int32_t nb_hydro_tight_mixed(nb_ctx* ctx, void* stream, const double*
     const bool masses = !(ctx->mix_m_uniform > 0.0);
     const FinArgs F{masses ? ctx->fscratch + (size_t)(n * cap) : nullptr, ctx->mix_m_uniform, ctx->mix_kid,
                     ctx->mix_cnorm, ctx->pending, rho, omega};
-    const unsigned blocks = (unsigned)((n * kG + 255) / 256);
-    NB_TIMED(ctx, s, "k_tight_emit_rm", k_tight_emit_rm, dim3(blocks), dim3(256), 0, s, pos, ctx->gbuf, ctx->fscratch,
+    const int tile = ctx->tile;
+    const unsigned blocks = (unsigned)((n * kG + tile - 1) / tile);
+    if (ctx->part_bytes < sizeof(long long) * (size_t)blocks)
+        return ctx_fail(ctx, "nb_hydro_tight_mixed: reduction buffer smaller than the launch");
+    NB_TIMED(ctx, s, "k_tight_emit_rm", k_tight_emit_rm, dim3(blocks), dim3(tile), 0, s, pos, ctx->gbuf, ctx->fscratch,
              cap, ctx->rowcnt, ctx->efill, ctx->ecap_rm, ctx->ebuf, (const float*)ctx->edbuf, h_final, n, out_len,
              out_index, row_cap, F, ctx->part);
     finish_partials(ctx, s, blocks, 1, 0u, 8);

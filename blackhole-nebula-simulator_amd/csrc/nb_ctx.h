@@ -58,6 +58,8 @@ struct nb_ctx {
     float* fscratch = nullptr; size_t fscratch_bytes = 0;
     void* pack = nullptr; size_t pack_bytes = 0;  // per-particle FP32 records of the mixed passes
     long long* part = nullptr; size_t part_bytes = 0;  // per-workgroup partials of reductions
+    void* segs = nullptr; size_t segs_bytes = 0;       // grid table of an ensemble's segments
+    std::vector<char> host_segs;                       // its host copy
     void* posh = nullptr; size_t posh_bytes = 0;  // (x, y, z, h_search) records of the mixed gather
     void* mlp_scratch = nullptr; size_t mlp_scratch_bytes = 0;  // training activations and partials
     void* mlp_pack = nullptr; size_t mlp_pack_bytes = 0;        // BF16 MFMA operands of the network
@@ -74,6 +76,7 @@ struct nb_ctx {
     int hydro_ready = 0;
     int hydro_mixed_ready = 0;
     int order_ident = 0;  // last grid: particles already stored in cell order (no order array)
+    int tile = 256;       // workgroup size of the mixed-precision lane-group kernels (nb_set_tile)
     // settings of the last mixed density pass, for its emission pass
     int32_t mix_kid = 0;
     double mix_cnorm = 0.0, mix_m_uniform = 0.0;

@@ -273,7 +273,7 @@ k_divcurl_mixed(const double* __restrict__ pos, const double* __restrict__ accel
     gradients_mixed_row<L, false>(pos, accel, pg, lists, box, rows, nrows, kid, div, curl, ddiv);
 }
 
-static inline unsigned grid_m(int64_t nrows) { return (unsigned)((nrows * kGM + kBlockM - 1) / kBlockM); }
+static inline unsigned grid_m(int64_t nrows, int tile) { return (unsigned)((nrows * kGM + tile - 1) / tile); }
 static inline unsigned grid_1(int64_t n) { return (unsigned)((n + kBlockM - 1) / kBlockM); }
 
 static inline Box box_of(const double* box3)
@@ -289,13 +289,13 @@ static inline Box box_of(const double* box3)
         const int ts_ = ktime_begin(ctx, (STREAM), #KERNEL);                                            \
         if ((L)->length && (L)->row_major) {                                                            \
             RowView v_{(L)->index, (L)->length, (L)->row_cap};                                          \
-            hipLaunchKernelGGL(KERNEL<RowView>, dim3(GRID), dim3(kBlockM), 0, STREAM, __VA_ARGS__);      \
+            hipLaunchKernelGGL(KERNEL<RowView>, dim3(GRID), dim3(ctx->tile), 0, STREAM, __VA_ARGS__);    \
         } else if ((L)->length) {                                                                       \
             BlkView v_{(L)->index, (L)->length, (L)->row_cap};                                          \
-            hipLaunchKernelGGL(KERNEL<BlkView>, dim3(GRID), dim3(kBlockM), 0, STREAM, __VA_ARGS__);      \
+            hipLaunchKernelGGL(KERNEL<BlkView>, dim3(GRID), dim3(ctx->tile), 0, STREAM, __VA_ARGS__);    \
         } else {                                                                                        \
             CsrView v_{(L)->offset, (L)->index};                                                        \
-            hipLaunchKernelGGL(KERNEL<CsrView>, dim3(GRID), dim3(kBlockM), 0, STREAM, __VA_ARGS__);      \
+            hipLaunchKernelGGL(KERNEL<CsrView>, dim3(GRID), dim3(ctx->tile), 0, STREAM, __VA_ARGS__);    \
         }                                                                                               \
         ktime_end(ctx, (STREAM), ts_);                                                                  \
     } while (0)
@@ -319,7 +319,7 @@ int32_t forces_mixed(nb_ctx* ctx, hipStream_t s, const double* pos, const double
     PF* pf = (PF*)ctx->pack;
     hipLaunchKernelGGL(k_pack_forces, dim3(grid_1(n)), dim3(kBlockM), 0, s, vel, mass, h, rho, omega, pressure,
                        P->use_cond ? u : nullptr, c_s, P->use_visc ? alpha : nullptr, P->cnorm, n, pf);
-    NB_DISPATCH_M(lists, k_forces_mixed, grid_m(nrows), s, pos, pf, c_s, v_, box_of(box3), rows, nrows, *P, acc, dudt,
+    NB_DISPATCH_M(lists, k_forces_mixed, grid_m(nrows, ctx->tile), s, pos, pf, c_s, v_, box_of(box3), rows, nrows, *P, acc, dudt,
                   vsig_max, vheat, cheat);
     return launched(ctx, "nb_forces (mixed)");
 }
@@ -343,10 +343,10 @@ extern "C" int32_t nb_gradients_mixed(nb_ctx* ctx, void* stream, const double* p
     PG* pg = (PG*)ctx->pack;
     hipLaunchKernelGGL(k_pack_grad, dim3(grid_1(n)), dim3(kBlockM), 0, s, vel, mass, h, cnorm, n, pg);
     if (acc)
-        NB_DISPATCH_M(lists, k_gradients_mixed, grid_m(nrows), s, pos, acc, pg, v_, box_of(box3), rows, nrows, kid,
+        NB_DISPATCH_M(lists, k_gradients_mixed, grid_m(nrows, ctx->tile), s, pos, acc, pg, v_, box_of(box3), rows, nrows, kid,
                       div, curl, ddiv);
     else
-        NB_DISPATCH_M(lists, k_divcurl_mixed, grid_m(nrows), s, pos, acc, pg, v_, box_of(box3), rows, nrows, kid, div,
+        NB_DISPATCH_M(lists, k_divcurl_mixed, grid_m(nrows, ctx->tile), s, pos, acc, pg, v_, box_of(box3), rows, nrows, kid, div,
                       curl, ddiv);
     return launched(ctx, "nb_gradients_mixed");
 }

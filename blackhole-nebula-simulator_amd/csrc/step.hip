@@ -216,6 +216,53 @@ __global__ void k_dt_partial(const double* __restrict__ h, const double* __restr
         for (int k = 0; k < 4; k++) part[4 * blockIdx.x + k] = sh[k][0];
 }
 
+// k_dt_partial per segment of an ensemble: one workgroup per segment [off[s], off[s + 1]) writes
+// its four minima to out[4 s + k].
+__global__ void __launch_bounds__(kBlockS)
+k_dt_seg(const double* __restrict__ h, const double* __restrict__ vsig, const double* __restrict__ acc,
+         const double* __restrict__ nu, double c_visc, const double* __restrict__ x, const int64_t* __restrict__ off,
+         double* __restrict__ out)
+{
+    double v[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+    for (int64_t i = off[blockIdx.x] + threadIdx.x; i < off[blockIdx.x + 1]; i += blockDim.x) {
+        const double hi = h[i];
+        v[0] = fmin(v[0], hi / vsig[i]);
+        const double a0 = acc[3 * i], a1 = acc[3 * i + 1], a2 = acc[3 * i + 2];
+        const double am = sqrt(a0 * a0 + a1 * a1 + a2 * a2);
+        if (am > 0.0) v[1] = fmin(v[1], sqrt(hi / am));
+        if (nu && nu[i] > 0.0) v[2] = fmin(v[2], c_visc * (hi * hi) / nu[i]);
+        const double x0 = x[3 * i], x1 = x[3 * i + 1], x2 = x[3 * i + 2];
+        v[3] = fmin(v[3], sqrt(x0 * x0 + x1 * x1 + x2 * x2));
+    }
+    __shared__ double sh[4][kBlockS];
+    for (int k = 0; k < 4; k++) sh[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int k = 0; k < 4; k++) sh[k][threadIdx.x] = fmin(sh[k][threadIdx.x], sh[k][threadIdx.x + st]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; k++) out[4 * blockIdx.x + k] = sh[k][0];
+}
+
+// out row i = y row i + s[i] a row i (rows of w values): a kick with a step per particle
+__global__ void k_axpy_rows(const double* __restrict__ y, const double* __restrict__ a, const double* __restrict__ s,
+                            int64_t n, int w, double* __restrict__ out)
+{
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= n * w) return;
+    out[t] = y[t] + s[t / w] * a[t];
+}
+
+// out[i] = scale * per_seg[cloud[i]]
+__global__ void k_expand_seg(const double* __restrict__ per_seg, const int32_t* __restrict__ cloud, int64_t n,
+                             double scale, double* __restrict__ out)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) out[i] = scale * per_seg[cloud[i]];
+}
+
 // Ledger sums (coupled.py:261-270): out = [sum m (H + u), sum m |H|, sum m (x cross p)_xyz,
 // sum m (u - u_cool) extra term (e_part, may be null)]. H is the GN Hamiltonian
 // (gravity.py:92-102) or |p|^2/2 without the hole.
@@ -379,6 +426,45 @@ extern "C" int32_t nb_dt_minima(nb_ctx* ctx, void* stream, const double* h, cons
     if (ctx_reserve(ctx, (void**)&ctx->red, &ctx->red_bytes, sizeof(double) * 4 * (nparts + 1))) return NB_ERR_HIP;
     hipLaunchKernelGGL(k_dt_partial, dim3(nparts), dim3(kBlockS), 0, s, h, vsig, acc, nu, c_visc, x, n, ctx->red);
     return reduce_two_level(ctx, s, 4, 0, nparts, out4);
+}
+
+extern "C" int32_t nb_dt_minima_seg(nb_ctx* ctx, void* stream, const double* h, const double* vsig,
+                                    const double* acc, const double* nu, double c_visc, const double* x,
+                                    const int64_t* seg_off, int32_t nseg, double* out)
+{
+    if (!ctx || nseg < 1 || !seg_off || !out || !h || !vsig || !acc || !x)
+        return ctx_fail(ctx, "nb_dt_minima_seg: bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    if (ctx_reserve(ctx, (void**)&ctx->red, &ctx->red_bytes, sizeof(double) * 4 * nseg + sizeof(int64_t) * (nseg + 1)))
+        return NB_ERR_HIP;
+    int64_t* off = (int64_t*)(ctx->red + 4 * nseg);
+    if (hipMemcpyAsync(off, seg_off, sizeof(int64_t) * (nseg + 1), hipMemcpyHostToDevice, s) != hipSuccess)
+        return ctx_fail(ctx, "nb_dt_minima_seg: offset copy failed");
+    hipLaunchKernelGGL(k_dt_seg, dim3(nseg), dim3(kBlockS), 0, s, h, vsig, acc, nu, c_visc, x, off, ctx->red);
+    if (hipMemcpyAsync(out, ctx->red, sizeof(double) * 4 * nseg, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return ctx_fail(ctx, "nb_dt_minima_seg: result copy failed");
+    return NB_OK;
+}
+
+extern "C" int32_t nb_axpy_rows(nb_ctx* ctx, void* stream, const double* y, const double* a, const double* s_rows,
+                                int64_t n, int32_t width, double* out)
+{
+    if (!ctx || n < 0 || width <= 0 || !y || !a || !s_rows || !out) return ctx_fail(ctx, "nb_axpy_rows: bad arguments");
+    if (n == 0) return NB_OK;
+    hipLaunchKernelGGL(k_axpy_rows, dim3(grid_s(n * width)), dim3(kBlockS), 0, (hipStream_t)stream, y, a, s_rows, n,
+                       (int)width, out);
+    return launch_check_s(ctx, "nb_axpy_rows");
+}
+
+extern "C" int32_t nb_expand_seg(nb_ctx* ctx, void* stream, const double* per_seg, const int32_t* cloud, int64_t n,
+                                 double scale, double* out)
+{
+    if (!ctx || n < 0 || !per_seg || !cloud || !out) return ctx_fail(ctx, "nb_expand_seg: bad arguments");
+    if (n == 0) return NB_OK;
+    hipLaunchKernelGGL(k_expand_seg, dim3(grid_s(n)), dim3(kBlockS), 0, (hipStream_t)stream, per_seg, cloud, n, scale,
+                       out);
+    return launch_check_s(ctx, "nb_expand_seg");
 }
 
 extern "C" int32_t nb_ledger(nb_ctx* ctx, void* stream, const double* x, const double* p, const double* u,

@@ -101,17 +101,22 @@ SIGNATURES = {
     "nb_lists_build_blocked": (_i32, [_vp, _vp, _vp, _vp, _i64, _PD, _vp, _vp, _i64, _P64, _P64]),
     "nb_lists_emit_blocked": (_i32, [_vp, _vp, _vp, _vp, _i64]),
     "nb_lists_stats": (_i32, [_vp, _P64]),
-    "nb_grid_order": (_i32, [_vp, _vp, _vp, _vp, _i64, _PD, _vp]),
+    "nb_grid_order": (_i32, [_vp, _vp, _vp, _vp, _i64, _PD, _f64, _vp]),
+    "nb_grid_order_seg": (_i32, [_vp, _vp, _vp, _vp, _i64, _vp, _P64, _i32, _f64, _vp]),
     "nb_permute": (_i32, [_vp, _vp, _vp, _vp, _i64, _i32, _vp]),
     "nb_permute_i64": (_i32, [_vp, _vp, _vp, _vp, _i64, _vp]),
     "nb_tighten_blocked": (_i32, [_vp, _vp, _vp, _vp, _i64, _PD, _vp, _vp, _i64, _vp, _vp, _P64]),
     "nb_hydro_density": (_i32, [_vp, _vp, _vp, _vp, _f64, _vp, _vp, _i64, _i32, _f64, _f64, _f64, _f64, _i32, _vp,
                                 _vp, _vp, _vp, _vp, _P64]),
     "nb_hydro_tight": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _P64]),
-    "nb_hydro_density_mixed": (_i32, [_vp, _vp, _vp, _vp, _f64, _vp, _vp, _i64, _i32, _f64, _f64, _f64, _f64, _vp,
-                                      _vp, _vp, _vp, _vp, _P64]),
+    "nb_hydro_density_mixed": (_i32, [_vp, _vp, _vp, _vp, _f64, _vp, _vp, _i64, _vp, _P64, _i32, _f64, _i32, _f64,
+                                      _f64, _f64, _f64, _vp, _vp, _vp, _vp, _vp, _P64]),
     "nb_hydro_tight_mixed": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _P64]),
     "nb_timing": (_i32, [_vp, _i32]),
+    "nb_set_tile": (_i32, [_vp, _i32]),
+    "nb_dt_minima_seg": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _f64, _vp, _P64, _i32, _PD]),
+    "nb_axpy_rows": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _i32, _vp]),
+    "nb_expand_seg": (_i32, [_vp, _vp, _vp, _vp, _i64, _f64, _vp]),
     "nb_mlp_param_count": (_i32, []),
     "nb_mlp_features": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _f64, _f64, _vp]),
     "nb_mlp_target": (_i32, [_vp, _vp, _vp, _vp, _i64, _i64, _vp]),
@@ -232,6 +237,11 @@ def stop_timing() -> dict:
     out = {k: (len(v), sum(a.elapsed_time(b) for a, b in v)) for k, v in (_TIMING or {}).items()}
     _TIMING = None
     return out
+
+
+def set_tile(tile: int) -> None:
+    """Workgroup size (64, 128 or 256) of the mixed-precision lane-group kernels."""
+    check(lib().nb_set_tile(ctx(), int(tile)), "nb_set_tile")
 
 
 def start_kernel_timing() -> None:

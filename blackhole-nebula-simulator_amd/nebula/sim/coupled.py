@@ -33,7 +33,7 @@ from ..physics import units
 from .fastphys import gn_v_from_p_device
 from .hydro import HydroConfig
 from .integrator import glf_device
-from .neighbours import grid_order, permute, permute_many
+from .neighbours import grid_order, grid_order_seg, permute, permute_many
 from .sph import (
     COND_VELOCITY,
     hydro_density_device,
@@ -108,6 +108,10 @@ class GasState:
     nneigh: torch.Tensor = field(default=None)  # tight-list length per particle (float64)
     dt_prev: float = 0.0
     learned_fallbacks_dev: torch.Tensor = field(default=None)  # (1,) int64: particles that fell back
+    # ensembles of independent clouds (nebula.sim.ensemble): system of each particle, their ranges
+    cloud: torch.Tensor = field(default=None)
+    seg_off: np.ndarray = field(default=None)
+    cell_factor: float = 1.0  # neighbour-grid cell edge relative to the default (mixed path; autotuned)
 
     @property
     def learned_fallbacks(self) -> int:
@@ -164,9 +168,11 @@ def _search_lengths(cfg: CoupledConfig, h):
     return out
 
 
-def grid_permutation(cfg: CoupledConfig, x, h):
+def grid_permutation(cfg: CoupledConfig, x, h, st: GasState | None = None):
     """Order of the neighbour grid the next density pass will build (search lengths from h)."""
-    return grid_order(x, _search_lengths(cfg, h), None)
+    if st is not None and st.cloud is not None:
+        return grid_order_seg(x, _search_lengths(cfg, h), st.cloud, st.seg_off, st.cell_factor)
+    return grid_order(x, _search_lengths(cfg, h), None, 1.0 if st is None else st.cell_factor)
 
 
 def velocity(cfg: CoupledConfig, x, p):
@@ -208,8 +214,11 @@ def _hydro(st: GasState, cfg: CoupledConfig, v, dt_alpha: float, a_prev, h_start
     n = int(st.x.shape[0])
     if hc.precision == "mixed":
         d, n_search = hydro_density_mixed_device(st.x, st.mass, st.h.clone(), hc.kernel, hc.eta, 1e-4, hc.h_max,
-                                                 hc.skin, st.m_uniform, h_start_d=h_start)  # fmt: skip
+                                                 hc.skin, st.m_uniform, h_start_d=h_start, cloud_d=st.cloud,
+                                                 seg_off=st.seg_off, cell_factor=st.cell_factor)  # fmt: skip
     else:
+        if st.cloud is not None or st.cell_factor != 1.0:
+            raise ValueError("ensembles and tuned grids run in the mixed-precision mode")
         d, n_search = hydro_density_device(st.x, st.mass, st.h.clone(), hc.kernel, hc.eta, 1e-4, hc.h_max, hc.skin,
                                            st.m_uniform, hc.precision, h_start_d=h_start)  # fmt: skip
     p_gas = _dev.empty(n)
@@ -283,15 +292,17 @@ def _learned(cfg: CoupledConfig):
     return hc.learned_model if (hc.learned_h or getattr(hc.learned_model, "collect", None)) else None
 
 
-def initialise(x, v, u, mass, h_guess, cfg: CoupledConfig, t: float = 0.0) -> GasState:
-    """Start from positions and velocities, p computed from v (coupled.py:198-210)."""
+def initialise(x, v, u, mass, h_guess, cfg: CoupledConfig, t: float = 0.0, uniform_mass: bool = True) -> GasState:
+    """Start from positions and velocities, p computed from v (coupled.py:198-210). Equal masses
+    are passed to the pair kernels as one number unless ``uniform_mass`` is False (the per-particle
+    form an ensemble of clouds uses)."""
     x = np.array(x, float)
     n = len(x)
     v = np.array(v, float)
     p = p_from_v(cfg, x, v)
     hc = cfg.hydro
     mass_h = np.array(mass, float) * np.ones(n)
-    m_uniform = float(mass_h[0]) if n and np.all(mass_h == mass_h[0]) and mass_h[0] > 0 else 0.0
+    m_uniform = float(mass_h[0]) if uniform_mass and n and np.all(mass_h == mass_h[0]) and mass_h[0] > 0 else 0.0
     st = GasState(
         t, _dev.f64(x), _dev.f64(p), _dev.f64(np.array(u, float) * np.ones(n)), _dev.f64(mass_h),
         _dev.f64(np.array(h_guess, float) * np.ones(n)),
@@ -353,7 +364,7 @@ def step(st: GasState, cfg: CoupledConfig, dt: float) -> GasState:
             u_half = u_cool
     p_pred = _axpy(p_flow, st.a_hydro, half)
     u_pred = _axpy(u_half, st.dudt, half)
-    order = grid_permutation(cfg, x1, st.h)
+    order = grid_permutation(cfg, x1, st.h, st)
     model = _learned(cfg)
     fields = [x1, p_flow, p_pred, u_half, u_pred, st.a_hydro, st.mass, st.h, st.alpha, st.perm]
     extra = []

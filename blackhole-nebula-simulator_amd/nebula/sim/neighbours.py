@@ -194,11 +194,23 @@ def tighten_blocked(nb: BlockedLists, pos_d: torch.Tensor, h_d: torch.Tensor) ->
     return BlockedLists(ln, idx, nb.row_cap, int(total[0]), nb.box)
 
 
-def grid_order(pos_d: torch.Tensor, h_d: torch.Tensor, box_arr=None) -> torch.Tensor:
+def grid_order(pos_d: torch.Tensor, h_d: torch.Tensor, box_arr=None, cell_factor: float = 1.0) -> torch.Tensor:
     """The neighbour grid's particle order (int32) for these positions and search lengths."""
     n = int(pos_d.shape[0])
     order = _dev.empty(n, torch.int32)
-    _hip.call("nb_grid_order", _hip.ptr(pos_d), _hip.ptr(h_d), n, _hip.box_arg(box_arr), _hip.ptr(order))
+    _hip.call("nb_grid_order", _hip.ptr(pos_d), _hip.ptr(h_d), n, _hip.box_arg(box_arr), float(cell_factor),
+              _hip.ptr(order))  # fmt: skip
+    return order
+
+
+def grid_order_seg(pos_d: torch.Tensor, h_d: torch.Tensor, cloud_d: torch.Tensor, seg_off: np.ndarray,
+                   cell_factor: float = 1.0) -> torch.Tensor:  # fmt: skip
+    """grid_order for an ensemble of independent systems (seg_off: host int64 ranges)."""
+    n = int(pos_d.shape[0])
+    order = _dev.empty(n, torch.int32)
+    off = np.ascontiguousarray(seg_off, np.int64)
+    _hip.call("nb_grid_order_seg", _hip.ptr(pos_d), _hip.ptr(h_d), n, _hip.ptr(cloud_d),
+              off.ctypes.data_as(_hip._P64), len(off) - 1, float(cell_factor), _hip.ptr(order))  # fmt: skip
     return order
 
 

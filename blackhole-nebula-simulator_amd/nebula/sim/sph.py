@@ -187,7 +187,7 @@ def hydro_density_device(pos_d, mass_d, h_guess_d, kernel_name, eta, tol, h_max,
 
 
 def hydro_density_mixed_device(pos_d, mass_d, h_guess_d, kernel_name, eta, tol, h_max, skin, m_uniform,
-                               h_start_d=None):  # fmt: skip
+                               h_start_d=None, cloud_d=None, seg_off=None, cell_factor=1.0):  # fmt: skip
     """hydro_density_device in mixed precision: the same exact search lists, FP32 compensated
     sums in the h-rho solve (Newton steps and convergence test in FP64), tight lists row-major.
     Returns (DensityResult with RowLists as ``neighbours``, search-list entries)."""
@@ -195,6 +195,7 @@ def hydro_density_mixed_device(pos_d, mass_d, h_guess_d, kernel_name, eta, tol, 
 
     n = int(pos_d.shape[0])
     kid = KERNEL_IDS[kernel_name]
+    off = None if seg_off is None else np.ascontiguousarray(seg_off, np.int64)
     h_search = _dev.empty(n)
     _hip.call("nb_search_lengths", _hip.ptr(h_guess_d), float(h_max), float(skin), float(np.inf), n,
               _hip.ptr(h_search))  # fmt: skip
@@ -206,7 +207,9 @@ def hydro_density_mixed_device(pos_d, mass_d, h_guess_d, kernel_name, eta, tol, 
     for builds in range(1, 13):
         start = h_start_d if (h_start_d is not None and builds == 1) else h_guess_d
         _hip.call("nb_hydro_density_mixed", _hip.ptr(pos_d), _hip.ptr(mass_d), float(m_uniform), _hip.ptr(start),
-                  _hip.ptr(h_search), n, kid, float(C_NORM[kid]), float(eta), float(tol), float(h_max), _hip.ptr(h),
+                  _hip.ptr(h_search), n, _hip.ptr(cloud_d), None if off is None else off.ctypes.data_as(_hip._P64),
+                  0 if off is None else len(off) - 1, float(cell_factor), kid, float(C_NORM[kid]), float(eta),
+                  float(tol), float(h_max), _hip.ptr(h),
                   _hip.ptr(rho), _hip.ptr(omega), _hip.ptr(iters), _hip.ptr(flags),
                   stats.ctypes.data_as(_hip._P64))  # fmt: skip
         _hip.call("nb_grow_search", _hip.ptr(flags), _hip.ptr(h), None, n, float(np.inf), _hip.ptr(h_search),

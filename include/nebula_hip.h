@@ -77,6 +77,9 @@ const char *nb_last_error(const nb_ctx *ctx);
 int32_t nb_abi_version(void);
 /* bytes of scratch the context currently holds */
 int64_t nb_ctx_scratch_bytes(const nb_ctx *ctx);
+/* Workgroup size (64, 128 or 256 threads = 4, 8 or 16 rows) of the mixed-precision lane-group
+ * kernels of this context (no counterpart in the reference; set by the autotuner). */
+int32_t nb_set_tile(nb_ctx *ctx, int32_t tile);
 /* Kernel timing (no counterpart in the reference; for benchmarks): with enable != 0 the library
  * records HIP events around its main kernels, on the stream each runs on, and accumulates their
  * durations; every call clears the totals. */
@@ -107,9 +110,16 @@ int32_t nb_lists_stats(const nb_ctx *ctx, int64_t *out4);
 /* The particle order of the neighbour grid for these positions and search lengths: particles by
  * cell (raster order), ascending index inside a cell. Storing particles in this order makes the
  * gathered rows come out sorted. (A layout choice of this implementation: the reference keeps
- * the caller's order.) */
+ * the caller's order.) cell_factor scales the cell edge (1 for the FP64 passes; the mixed passes
+ * may take another, which their density pass must then use too). */
 int32_t nb_grid_order(nb_ctx *ctx, void *stream, const double *pos, const double *h, int64_t n, const double *box3,
-                      int32_t *order_out);
+                      double cell_factor, int32_t *order_out);
+/* The same for an ensemble of independent systems (no counterpart in the reference): system k holds
+ * particles seg_off[k] .. seg_off[k+1] - 1 (seg_off: host array of nseg + 1), cloud[i] (device) the
+ * system of particle i; each system gets its own grid, cells numbered system after system. */
+int32_t nb_grid_order_seg(nb_ctx *ctx, void *stream, const double *pos, const double *h, int64_t n,
+                          const int32_t *cloud, const int64_t *seg_off, int32_t nseg, double cell_factor,
+                          int32_t *order_out);
 /* dst row i = src row order[i], rows of `width` doubles (or int64 values). */
 int32_t nb_permute(nb_ctx *ctx, void *stream, const double *src, const int32_t *order, int64_t n, int32_t width,
                    double *dst);
@@ -152,13 +162,16 @@ int32_t nb_hydro_tight(nb_ctx *ctx, void *stream, const double *h_final, int32_t
 /* The density pass of coupled.step for the mixed-precision mode (open box): the same search
  * lists as nb_hydro_density (FP64 predicate, exact), gathered by 16-lane groups into row-major
  * rows (entry k of row i at i * cap + k) with FP32 d2; the h-rho solve sums in FP32 with
- * compensated accumulation and takes its Newton steps in FP64. Writes h_out, iters and flags of
+ * compensated accumulation and takes its Newton steps in FP64. cloud / seg_off / nseg: an ensemble
+ * of independent systems as in nb_grid_order_seg (cloud NULL: one system); cell_factor as in
+ * nb_grid_order. Writes h_out, iters and flags of
  * every row; rho and omega of the rows that needed more than one Newton step (the others are
  * completed by nb_hydro_tight_mixed, which evaluates them at the final h while it reads the
  * rows). stats4 (host) = [row capacity used, rows needing more than one step, search-list
  * entries, longest merged row]. */
 int32_t nb_hydro_density_mixed(nb_ctx *ctx, void *stream, const double *pos, const double *mass, double m_uniform,
-                               const double *h_start, const double *h_search, int64_t n, int32_t kid, double cnorm,
+                               const double *h_start, const double *h_search, int64_t n, const int32_t *cloud,
+                               const int64_t *seg_off, int32_t nseg, double cell_factor, int32_t kid, double cnorm,
                                double eta, double tol, double h_max, double *h_out, double *rho, double *omega,
                                int32_t *iters, int32_t *flags, int64_t *stats4);
 /* tighten (neighbours.py:306-323) of the last nb_hydro_density_mixed lists at h_final (its
@@ -259,6 +272,18 @@ int32_t nb_cool_powerlaw(nb_ctx *ctx, void *stream, const double *u, const doubl
  * min c_visc h^2/nu over nu>0 (nu may be NULL), min |x|]. */
 int32_t nb_dt_minima(nb_ctx *ctx, void *stream, const double *h, const double *vsig, const double *acc,
                      const double *nu, double c_visc, const double *x, int64_t n, double *out4);
+
+/* Ensemble helpers (no counterpart in the reference; the systems of nb_grid_order_seg):
+ * nb_dt_minima_seg: nb_dt_minima per system, out (host) nseg x 4; nb_axpy_rows: out row i = y row i
+ * + s_rows[i] a row i (rows of width values, a kick with a step per particle); nb_expand_seg:
+ * out[i] = scale per_seg[cloud[i]] (per_seg on the device). */
+int32_t nb_dt_minima_seg(nb_ctx *ctx, void *stream, const double *h, const double *vsig, const double *acc,
+                         const double *nu, double c_visc, const double *x, const int64_t *seg_off, int32_t nseg,
+                         double *out);
+int32_t nb_axpy_rows(nb_ctx *ctx, void *stream, const double *y, const double *a, const double *s_rows, int64_t n,
+                     int32_t width, double *out);
+int32_t nb_expand_seg(nb_ctx *ctx, void *stream, const double *per_seg, const int32_t *cloud, int64_t n, double scale,
+                      double *out);
 
 /* ledger sums (coupled.py:261-270): out6 (host) = [sum m (H + u), sum m |H|, sum m (x cross p)
  * (3), sum e_part]. gn = 1: GN Hamiltonian (gravity.py:92-102); 0: |p|^2/2. */

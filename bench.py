@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--no-learned", action="store_true", help="mixed mode without the learned starting h")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--drift-steps", type=int, default=1000,
+                    help="steps of the energy-ledger drift runs (both precisions, from the initial state); 0: skip")
     ap.add_argument("--ensemble", action="store_true",
                     help="the batched-ensemble workload (64 clouds) against sequential single-cloud runs")
     ap.add_argument("--ensemble-clouds", type=int, default=64)
@@ -122,6 +124,7 @@ def run_gpu(ic, cfg, steps, warmup, learned=False, log_stride=8, train_steps=300
                 "train_s": time.perf_counter() - t0, "loss_first": losses[0], "loss_last": losses[-1],
                 "newton_iters_plain": sum(plain_iters) / len(plain_iters)}  # fmt: skip
         cfg = replace(cfg, hydro=replace(cfg.hydro, learned_h=True, learned_model=model))
+        info["_cfg"] = cfg
         st = coupled.step(st, cfg, coupled.stable_dt(st, cfg))  # first learned step: warm-up
     else:
         for _ in range(warmup):
@@ -219,6 +222,29 @@ def run_ensemble(args):
     print(json.dumps(out))
 
 
+def drift_run(ic, cfg, steps):
+    """Energy and angular-momentum ledgers (coupled.py:261-270) over ``steps`` steps from the
+    initial state: the maximum over the run of |E - E0| / sum m|H| and |L - L0| / |L0| (the
+    reference's coupled ledger tests bound the former by 1e-5 and the latter by 1e-10)."""
+    import numpy as np
+
+    from nebula.sim import coupled
+
+    st = coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg)
+    l0 = coupled.ledger(st, cfg)
+    e_max = l_max = 0.0
+    t0 = time.perf_counter()
+    for k in range(steps):
+        st = coupled.step(st, cfg, coupled.stable_dt(st, cfg))
+        lg = coupled.ledger(st, cfg)
+        e_max = max(e_max, abs(lg["energy"] - l0["energy"]) / l0["abs_h"])
+        l_max = max(l_max, float(np.linalg.norm(lg["angmom"] - l0["angmom"]) / np.linalg.norm(l0["angmom"])))
+        if k % 250 == 249:
+            print(f"bench: drift run {k + 1}/{steps}, max |dE| / sum m|H| {e_max:.3g}", file=sys.stderr, flush=True)
+    return {"steps": steps, "t_end": st.t, "max_energy_error": e_max, "max_angmom_error": l_max,
+            "e_radiated_over_abs_h": lg["e_radiated"] / l0["abs_h"], "wall_s": time.perf_counter() - t0}
+
+
 def _per_step(d, steps):
     return {k: round(v[1] / steps, 4) for k, v in sorted(d.items(), key=lambda kv: -kv[1][1])}
 
@@ -308,7 +334,7 @@ def main():
                      "work": f"{work / n:.0f} {'FLOP' if bound != 'hbm' else 'B'}/particle per launch "
                              f"(n_s {n_search:.1f}, pairs {n_pairs:.1f}, Newton iterations {k_it:.2f})"},
         "kernels_ms_per_step": _per_step(kern, args.steps),
-        "learned_h": info if learned else None,
+        "learned_h": {k: v for k, v in info.items() if not k.startswith("_")} if learned else None,
         "newton_iters": info["newton_iters"],
     }
     if not args.no_compare:
@@ -320,6 +346,15 @@ def main():
         out["compare"] = {other: {"value": v2, "ms_per_step": w2 / args.steps * 1e3, "value_ratio": value / v2,
                                   "newton_iters": info2["newton_iters"],
                                   "kernels_ms_per_step": _per_step(k2, args.steps)}}  # fmt: skip
+    if args.drift_steps > 0:
+        runs = {args.precision + ("+learned" if learned else ""): info["_cfg"] if learned else cfg}
+        if not args.no_compare:
+            runs[other] = config(other)
+        out["drift"] = {}
+        for name, c in runs.items():
+            print(f"bench: {args.drift_steps}-step ledger run, {name}", file=sys.stderr, flush=True)
+            out["drift"][name] = drift_run(ic, c, args.drift_steps)
+        out["drift"]["tolerance"] = "max |dE| / sum m|H| < 1e-5 (the reference's coupled ledger tests)"
     if not args.no_cpu:
         print("bench: CPU baseline (FP64 oracle, OpenMP)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(ic, cfg, args.cpu_steps)

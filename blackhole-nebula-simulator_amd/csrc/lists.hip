@@ -937,7 +937,7 @@ k_tight_emit(const uint32_t* __restrict__ gbuf, const double* __restrict__ dbuf,
 // candidate on every pass whatever the column lengths. Candidate positions and search lengths
 // come in one 32-byte record. The FP64 list predicate keeps the lists exact; accepted entries
 // are compacted with a ballot (lane l's slot is the number of accepting lanes below it) into
-// row-major rows (entry k of row i at i * cap + k) with their FP32 d2 (and mass).
+// rows of their own capacity (row i at goff[i] .. goff[i + 1]) with their FP32 d2 (and mass).
 // The first evaluation of the h-rho Newton solve (sph.py:251-297, at the starting h) is summed
 // on the fly, in FP32 with compensated accumulation per lane, combined in FP64 by the group,
 // and the group takes the Newton step in FP64. Rows converged after that step (the common case)
@@ -1011,6 +1011,38 @@ struct MixArgs {
     long long* part;  // per-workgroup partials: longest row, slow rows
 };
 
+// Row capacities of the mixed lists (gather row gc, extras segment ec): uniform (mode 0), from
+// the previous pass's counts of the same particles with headroom (mode 1; hint[2 i] gather row,
+// hint[2 i + 1] extras), or from this pass's exact counts after an overflow (mode 2). Entry n is
+// 0, so the exclusive scans end in the totals. Rows are kept 16-byte aligned.
+__global__ void k_caps_mixed(const int32_t* __restrict__ hint, const int32_t* __restrict__ rowcnt,
+                             const unsigned int* __restrict__ efill, int64_t n, int mode, int64_t cap_u,
+                             int64_t ecap_u, unsigned long long* __restrict__ gc, unsigned long long* __restrict__ ec)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i > n) return;
+    unsigned long long g = 0, e = 0;
+    if (i < n) {
+        if (mode == 0) {
+            g = (unsigned long long)cap_u;
+            e = (unsigned long long)ecap_u;
+        } else if (mode == 1) {
+            const unsigned long long a = hint[2 * i] > 0 ? (unsigned long long)hint[2 * i] : 0ull;
+            const unsigned long long b = hint[2 * i + 1] > 0 ? (unsigned long long)hint[2 * i + 1] : 0ull;
+            g = a + a / 8 + 8;
+            e = b + b / 4 + 4;
+        } else {
+            const unsigned long long a = (unsigned long long)rowcnt[i], b = efill[i];
+            g = a + a / 16 + 4;
+            e = b + b / 8 + 2;
+        }
+        g = (g + 3) & ~3ull;
+        e = (e + 3) & ~3ull;
+    }
+    gc[i] = g;
+    ec[i] = e;
+}
+
 // (x, y, z, h_search) records of the candidates
 __global__ void k_pack_posh(const double* __restrict__ pos, const double* __restrict__ h, int64_t n,
                             double4* __restrict__ out)
@@ -1041,7 +1073,9 @@ __device__ __forceinline__ bool column_zrange(double ux, double uy, double zc, d
 }
 
 // Extras (pairs seen only from the other side, d2 >= (2 h_search_j)^2) go straight to row j's
-// extras segment (capacity ecap, filled with an atomic counter efill[j]). kSeg: an ensemble of
+// extras segment (eoff[j] .. eoff[j + 1], filled with an atomic counter efill[j]). Rows and
+// segments that outgrow their capacity keep counting (the pass is repeated with capacities from
+// the exact counts). kSeg: an ensemble of
 // independent systems; particle i searches only the cells of its own system cloud[i], whose grid
 // and first cell come from segs.
 template <bool kMasses, bool kSeg>
@@ -1049,9 +1083,9 @@ __global__ void __launch_bounds__(256)
 k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const GridSeg* __restrict__ segs,
                const int32_t* __restrict__ cloud, const uint32_t* __restrict__ start,
                const uint32_t* __restrict__ order, uint32_t* __restrict__ ridx, float* __restrict__ rd2,
-               float* __restrict__ rmass, int64_t cap, int32_t* __restrict__ rowcnt,
-               unsigned int* __restrict__ efill, uint32_t* __restrict__ ebuf, float* __restrict__ ed2, int64_t ecap,
-               MixArgs A)
+               float* __restrict__ rmass, const unsigned long long* __restrict__ goff, int32_t* __restrict__ rowcnt,
+               unsigned int* __restrict__ efill, uint32_t* __restrict__ ebuf, float* __restrict__ ed2,
+               const unsigned long long* __restrict__ eoff, MixArgs A)
 {
     const int glane = threadIdx.x & (kG - 1);
     const int gshift = (threadIdx.x & 63) & ~(kG - 1);
@@ -1076,10 +1110,12 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const Gr
     const float mu = (float)(A.m_uniform > 0.0 ? A.m_uniform : 1.0);
     DensSums sums;
     int32_t m = 0;
-    unsigned int emax = 0;
-    uint32_t* row = ridx + ii * cap;
-    float* drow = rd2 + ii * cap;
-    float* mrow = kMasses ? rmass + ii * cap : nullptr;
+    unsigned int emax = 0, eover = 0;
+    const unsigned long long gb0 = goff[ii];
+    const int64_t cap = (int64_t)(goff[ii + 1] - gb0);
+    uint32_t* row = ridx + gb0;
+    float* drow = rd2 + gb0;
+    float* mrow = kMasses ? rmass + gb0 : nullptr;
     // every loop below runs the same number of times on all lanes of the wave (ballot bounds),
     // so the shuffles always see their whole group active
     for (int64_t c0 = 0;; c0 += kCols) {
@@ -1145,9 +1181,12 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const Gr
                 const float d2f = (float)d2;
                 if (flag) {
                     const unsigned int es = atomicAdd(&efill[j], 1u);
-                    if (es < ecap) {
-                        ebuf[(int64_t)j * ecap + es] = (uint32_t)i;
-                        ed2[(int64_t)j * ecap + es] = d2f;
+                    const unsigned long long eb = eoff[j];
+                    if (es < eoff[j + 1] - eb) {
+                        ebuf[eb + es] = (uint32_t)i;
+                        ed2[eb + es] = d2f;
+                    } else {
+                        eover = 1;
                     }
                     emax = es + 1 > emax ? es + 1 : emax;
                 }
@@ -1178,9 +1217,13 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const Gr
     double h_new = df > 0 ? h0 - f / df : 0.5 * (lo_b + hi_b);
     if (!(lo_b <= h_new && h_new <= hi_b)) h_new = 0.5 * (lo_b + hi_b);
     const bool fast = m <= cap && fabs(h_new - h0) < A.tol * h0 && !(h_new >= r_lim * (1 - 1e-3));
+    // [0] longest gather row, [1] rows left slow, [2] an extras segment overflowed, [3] gather rows
+    // over capacity, [4] longest extras segment
     block_partial<true>(live ? m : 0, A.part);
     block_partial<false>(live && glane == 0 && !fast ? 1 : 0, A.part + gridDim.x);
-    block_partial<true>(emax, A.part + 2 * gridDim.x);
+    block_partial<true>(eover, A.part + 2 * gridDim.x);
+    block_partial<false>(live && glane == 0 && m > cap ? 1 : 0, A.part + 3 * gridDim.x);
+    block_partial<true>(emax, A.part + 4 * gridDim.x);
     if (live && glane == 0) {
         A.slow[i] = fast ? 0 : 1;
         if (fast) {
@@ -1205,14 +1248,15 @@ __device__ __forceinline__ void row_dens_sums(const float* drow, const float* mr
 template <bool kMasses>
 __global__ void __launch_bounds__(256)
 k_solve_slow_mixed(const double* __restrict__ h_search, int64_t n, const float* __restrict__ rd2,
-                   const float* __restrict__ rmass, int64_t cap, const int32_t* __restrict__ rowcnt, MixArgs A)
+                   const float* __restrict__ rmass, const unsigned long long* __restrict__ goff,
+                   const int32_t* __restrict__ rowcnt, MixArgs A)
 {
     const int glane = threadIdx.x & (kG - 1);
     const int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kG;
     if (i >= n || !A.slow[i]) return;  // whole groups leave together
     const int32_t m = rowcnt[i];
-    const float* drow = rd2 + i * cap;
-    const float* mrow = kMasses ? rmass + i * cap : nullptr;
+    const float* drow = rd2 + goff[i];
+    const float* mrow = kMasses ? rmass + goff[i] : nullptr;
     const float mu = (float)(A.m_uniform > 0.0 ? A.m_uniform : 1.0);
     const double mscale = kMasses ? 1.0 : (A.m_uniform > 0.0 ? A.m_uniform / (double)mu : 1.0);
     const double m_i = A.mass[i];
@@ -1263,36 +1307,53 @@ k_solve_slow_mixed(const double* __restrict__ h_search, int64_t n, const float* 
     }
 }
 
-// final row lengths of the mixed lists (gather row + extras): partials of their largest value
-// and of their sum (the search-list entries)
+// final row lengths of the mixed lists (gather row + extras) into mc (entry n: 0, for the scan
+// of the tight rows' offsets), the two counts per row for the next pass's capacities
+// (row_counts, may be NULL), and partials of their largest value and of their sum (the
+// search-list entries). Runs after a pass without overflow: the counts are the stored entries.
 __global__ void __launch_bounds__(kBlock)
-k_row_total_rm(const int32_t* __restrict__ rowcnt, const unsigned int* __restrict__ efill, int64_t ecap,
-               long long* __restrict__ part, int64_t n)
+k_row_total_rm(const int32_t* __restrict__ rowcnt, const unsigned int* __restrict__ efill,
+               unsigned long long* __restrict__ mc, int32_t* __restrict__ row_counts, long long* __restrict__ part,
+               int64_t n)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     long long t = 0;
-    if (i < n) t = (long long)rowcnt[i] + (long long)(efill[i] < ecap ? efill[i] : ecap);
+    if (i < n) {
+        t = (long long)rowcnt[i] + (long long)efill[i];
+        mc[i] = (unsigned long long)t;
+        if (row_counts) {
+            row_counts[2 * i] = rowcnt[i];
+            row_counts[2 * i + 1] = (int32_t)efill[i];
+        }
+    } else if (i == n) {
+        mc[n] = 0;
+    }
     block_partial<true>(t, part);
     block_partial<false>(t, part + gridDim.x);
 }
 
 // Extras arrive in atomic order: each row's segment is put in index order (so that the tightened
-// rows, and every sum over them, are the same from run to run). A 16-lane group per row ranks up
-// to 64 entries held in registers (entries of a segment are distinct); longer segments are sorted
-// by the group's first lane.
+// rows, and every sum over them, are the same from run to run), out of place into (es, eds). Every
+// entry goes to its rank, the number of smaller entries (entries of a segment are distinct). A
+// 16-lane group per row ranks up to 64 entries held in registers against each other; a longer
+// segment is ranked by its lanes, 16 entries at a time, against the whole segment.
 static constexpr int kSortRegs = 4;
 
 __global__ void __launch_bounds__(256)
-k_sort_extras(uint32_t* __restrict__ ebuf, float* __restrict__ ed2, const unsigned int* __restrict__ efill,
-              int64_t ecap, int64_t n)
+k_sort_extras(const uint32_t* __restrict__ ebuf, const float* __restrict__ ed2, const unsigned int* __restrict__ efill,
+              const unsigned long long* __restrict__ eoff, int64_t n, uint32_t* __restrict__ es,
+              float* __restrict__ eds)
 {
     const int glane = threadIdx.x & (kG - 1);
     const int64_t i = (xcd_tile() * (int64_t)blockDim.x + threadIdx.x) / kG;
     if (i >= n) return;  // whole groups leave together
-    uint32_t* e = ebuf + i * ecap;
-    float* d = ed2 + i * ecap;
-    const int64_t ne = efill[i] < ecap ? efill[i] : ecap;
-    if (ne <= 1) return;
+    const unsigned long long b0 = eoff[i];
+    const uint32_t* e = ebuf + b0;
+    const float* d = ed2 + b0;
+    uint32_t* eo = es + b0;
+    float* dout = eds + b0;
+    const int64_t ne = efill[i];
+    if (ne == 0) return;
     if (ne <= kG * kSortRegs) {
         uint32_t v[kSortRegs];
         float dv[kSortRegs];
@@ -1314,24 +1375,21 @@ k_sort_extras(uint32_t* __restrict__ ebuf, float* __restrict__ ed2, const unsign
         for (int t = 0; t < kSortRegs; t++) {
             const int64_t k = glane + kG * t;
             if (k < ne) {
-                e[rank[t]] = v[t];
-                d[rank[t]] = dv[t];
+                eo[rank[t]] = v[t];
+                dout[rank[t]] = dv[t];
             }
         }
         return;
     }
-    if (glane != 0) return;
-    for (int64_t a = 1; a < ne; a++) {
-        const uint32_t v = e[a];
-        const float dv = d[a];
-        int64_t b = a - 1;
-        while (b >= 0 && e[b] > v) {
-            e[b + 1] = e[b];
-            d[b + 1] = d[b];
-            b--;
+    for (int64_t kb = 0; kb < ne; kb += kG) {
+        const int64_t k = kb + glane;
+        const uint32_t v = k < ne ? e[k] : 0u;
+        int64_t rank = 0;
+        for (int64_t q = 0; q < ne; q++) rank += e[q] < v ? 1 : 0;
+        if (k < ne) {
+            eo[rank] = v;
+            dout[rank] = d[k];
         }
-        e[b + 1] = v;
-        d[b + 1] = dv;
     }
 }
 
@@ -1346,7 +1404,8 @@ struct FinArgs {
     double* omega;
 };
 
-// Tightened rows (row-major, capacity cap_out): gather row entries, then the row's extras, kept
+// Tightened rows (row i at toff[i], capacity its merged length): gather row entries, then the
+// row's extras (sorted), kept
 // when d2 < (2 max(h_i, h_j))^2 at the final h. The stored FP32 d2 decides unless it lies within
 // its rounding error of the radius; then d2 is formed again in FP64 as the search formed it, so
 // the rows are exactly tighten() of the search lists at h_final. The same pass sums rho and
@@ -1354,10 +1413,11 @@ struct FinArgs {
 // their d2 >= (2 h_search_i)^2 >= (2 h_i)^2).
 __global__ void __launch_bounds__(256)
 k_tight_emit_rm(const double* __restrict__ pos, const uint32_t* __restrict__ ridx, const float* __restrict__ rd2,
-                int64_t cap, const int32_t* __restrict__ rowcnt, const unsigned int* __restrict__ efill, int64_t ecap,
+                const unsigned long long* __restrict__ goff, const int32_t* __restrict__ rowcnt,
+                const unsigned int* __restrict__ efill, const unsigned long long* __restrict__ eoff,
                 const uint32_t* __restrict__ ebuf, const float* __restrict__ ed2, const double* __restrict__ hf,
-                int64_t n, int32_t* __restrict__ out_len, int32_t* __restrict__ out_idx, int64_t cap_out,
-                FinArgs F, long long* __restrict__ part)
+                int64_t n, const unsigned long long* __restrict__ toff, int64_t* __restrict__ out_off,
+                int32_t* __restrict__ out_len, int32_t* __restrict__ out_idx, FinArgs F, long long* __restrict__ part)
 {
     const int glane = threadIdx.x & (kG - 1);
     const int gshift = (threadIdx.x & 63) & ~(kG - 1);
@@ -1366,7 +1426,8 @@ k_tight_emit_rm(const double* __restrict__ pos, const uint32_t* __restrict__ rid
     const int64_t ii = live ? i : n - 1;
     const double hi = hf[ii];
     const int32_t ng = live ? rowcnt[ii] : 0;
-    const int64_t e0 = ii * ecap, ne = live ? (int64_t)(efill[ii] < ecap ? efill[ii] : ecap) : 0;
+    const unsigned long long g0 = goff[ii], e0 = eoff[ii], t0 = toff[ii];
+    const int64_t ne = live ? (int64_t)efill[ii] : 0;
     const int64_t len = ng + ne;
     const bool fin = live && !F.slow[ii];
     const float inv_hf = (float)(1.0 / hi);
@@ -1382,9 +1443,9 @@ k_tight_emit_rm(const double* __restrict__ pos, const uint32_t* __restrict__ rid
         if (in_range && k < len) {
             float d2;
             if (k < ng) {
-                j = ridx[ii * cap + k];
-                d2 = rd2[ii * cap + k];
-                if (fin) sums.add(__fsqrt_rn(d2) * inv_hf, F.rmass ? F.rmass[ii * cap + k] : mu, F.kid);
+                j = ridx[g0 + k];
+                d2 = rd2[g0 + k];
+                if (fin) sums.add(__fsqrt_rn(d2) * inv_hf, F.rmass ? F.rmass[g0 + k] : mu, F.kid);
             } else {
                 j = ebuf[e0 + (k - ng)];
                 d2 = ed2[e0 + (k - ng)];
@@ -1403,16 +1464,15 @@ k_tight_emit_rm(const double* __restrict__ pos, const uint32_t* __restrict__ rid
         }
         const unsigned long long bal = __ballot(pred);
         const unsigned gb = (unsigned)((bal >> gshift) & ((1ull << kG) - 1));
-        if (pred) {
-            const int slot = o + __popc(gb & ((1u << glane) - 1));
-            if (slot < cap_out) out_idx[ii * cap_out + slot] = (int32_t)j;
-        }
+        if (pred) out_idx[t0 + o + __popc(gb & ((1u << glane) - 1))] = (int32_t)j;
         o += __popc(gb);
     }
     double s1, s2;
     sums.total(s1, s2);
     if (live && glane == 0) {
-        out_len[i] = o < cap_out ? o : (int32_t)cap_out;
+        out_len[i] = o;
+        out_off[i] = (int64_t)t0;
+        if (i == n - 1) out_off[n] = (int64_t)toff[n];
         if (fin) {
             const double mscale = F.rmass ? 1.0 : (F.m_uniform > 0.0 ? F.m_uniform / (double)mu : 1.0);
             const double inv_h = 1.0 / hi;
@@ -1469,8 +1529,8 @@ Box make_box(const double* box3)
 // of the collected scalars (ctx->scal, 16 values).
 int reserve_reductions(nb_ctx* ctx, int64_t n)
 {
-    // 4 slots for launches of up to n / 4 + 1 workgroups (lane groups of 16 in 64-thread workgroups)
-    if (ctx_reserve(ctx, (void**)&ctx->part, &ctx->part_bytes, sizeof(long long) * 4 * (size_t)(n / 4 + 1024)) ||
+    // 5 slots for launches of up to n / 4 + 1 workgroups (lane groups of 16 in 64-thread workgroups)
+    if (ctx_reserve(ctx, (void**)&ctx->part, &ctx->part_bytes, sizeof(long long) * 5 * (size_t)(n / 4 + 1024)) ||
         ctx_reserve(ctx, (void**)&ctx->scal, &ctx->scal_bytes, sizeof(long long) * 16))
         return NB_ERR_HIP;
     return NB_OK;
@@ -2091,9 +2151,10 @@ extern "C" int32_t nb_hydro_tight(nb_ctx* ctx, void* stream, const double* h_fin
 extern "C" int32_t nb_hydro_density_mixed(nb_ctx* ctx, void* stream, const double* pos, const double* mass,
                                           double m_uniform, const double* h_start, const double* h_search, int64_t n,
                                           const int32_t* cloud, const int64_t* seg_off, int32_t nseg,
-                                          double cell_factor, int32_t kid, double cnorm, double eta, double tol,
-                                          double h_max, double* h_out, double* rho, double* omega, int32_t* iters,
-                                          int32_t* flags, int64_t* stats4)
+                                          double cell_factor, const int32_t* row_hint, int32_t* row_counts,
+                                          int32_t kid, double cnorm, double eta, double tol, double h_max,
+                                          double* h_out, double* rho, double* omega, int32_t* iters, int32_t* flags,
+                                          int64_t* stats4)
 {
     hipStream_t s = (hipStream_t)stream;
     if (!ctx || n <= 0 || !stats4 || n >= (int64_t)kMask || !(cell_factor > 0.0) || (cloud && !seg_off))
@@ -2106,23 +2167,44 @@ extern "C" int32_t nb_hydro_density_mixed(nb_ctx* ctx, void* stream, const doubl
                        : build_grid(ctx, s, pos, h_search, n, open, g, ncells, true, cell_factor);
     if (rc) return rc;
     const bool masses = !(m_uniform > 0.0);
+    const size_t u64n = sizeof(unsigned long long) * (size_t)(n + 1);
     if (ctx_reserve(ctx, (void**)&ctx->rowcnt, &ctx->rowcnt_bytes, sizeof(int32_t) * n) ||
         ctx_reserve(ctx, (void**)&ctx->pending, &ctx->pending_bytes, n) ||
         ctx_reserve(ctx, (void**)&ctx->posh, &ctx->posh_bytes, sizeof(double4) * n) ||
-        ctx_reserve(ctx, (void**)&ctx->efill, &ctx->efill_bytes, sizeof(unsigned int) * n))
+        ctx_reserve(ctx, (void**)&ctx->efill, &ctx->efill_bytes, sizeof(unsigned int) * n) ||
+        ctx_reserve(ctx, (void**)&ctx->rm_gc, &ctx->rm_gc_bytes, u64n) ||
+        ctx_reserve(ctx, (void**)&ctx->rm_go, &ctx->rm_go_bytes, u64n) ||
+        ctx_reserve(ctx, (void**)&ctx->rm_ec, &ctx->rm_ec_bytes, u64n) ||
+        ctx_reserve(ctx, (void**)&ctx->rm_eo, &ctx->rm_eo_bytes, u64n) ||
+        ctx_reserve(ctx, (void**)&ctx->rm_to, &ctx->rm_to_bytes, u64n))
         return NB_ERR_HIP;
     NB_TIMED(ctx, s, "k_pack_posh", k_pack_posh, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, h_search, n,
              (double4*)ctx->posh);
-    int64_t cap = ctx->row_cap_rm > 0 ? ctx->row_cap_rm : 192;
-    int64_t ecap = ctx->ecap_rm > 0 ? ctx->ecap_rm : 32;
+    const int64_t cap_u = ctx->row_cap_rm > 0 ? ctx->row_cap_rm : 192;
+    const int64_t ecap_u = ctx->ecap_rm > 0 ? ctx->ecap_rm : 32;
     long long sc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     MixArgs A{mass, m_uniform, h_start, h_max, kid, cnorm, eta, tol, h_out, rho, omega, iters, flags, ctx->pending,
               ctx->part};
     const int tile = ctx->tile;  // workgroup size of the lane-group kernels (autotuned: 64, 128 or 256)
     const int64_t blocks = (n * kG + tile - 1) / tile;
-    for (int attempt = 0; attempt < 8; attempt++) {
-        const size_t cells = (size_t)(n * cap);
-        const size_t ecells = (size_t)(n * ecap);
+    const GridSeg* segs = (const GridSeg*)ctx->segs;
+    // capacities: the previous pass's counts (row_hint) or uniform; after an overflow, this pass's
+    // exact counts, which the repeated pass cannot exceed
+    int mode = row_hint ? 1 : 0;
+    unsigned long long tot[2] = {0, 0};
+    for (int attempt = 0;; attempt++) {
+        hipLaunchKernelGGL(k_caps_mixed, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, row_hint, ctx->rowcnt, ctx->efill, n,
+                           mode, cap_u, ecap_u, ctx->rm_gc, ctx->rm_ec);
+        rc = scan_u64(ctx, s, ctx->rm_gc, ctx->rm_go, n + 1);
+        if (rc) return rc;
+        rc = scan_u64(ctx, s, ctx->rm_ec, ctx->rm_eo, n + 1);
+        if (rc) return rc;
+        if (hipMemcpyAsync(&tot[0], ctx->rm_go + n, sizeof(unsigned long long), hipMemcpyDeviceToHost, s) != hipSuccess)
+            return ctx_fail(ctx, "nb_hydro_density_mixed: capacity copy failed");
+        rc = sync_copy(ctx, s, &tot[1], ctx->rm_eo + n, sizeof(unsigned long long),
+                       "nb_hydro_density_mixed: capacity copy failed");
+        if (rc) return rc;
+        const size_t cells = (size_t)tot[0], ecells = (size_t)tot[1];
         if (ctx_reserve(ctx, (void**)&ctx->gbuf, &ctx->gbuf_bytes, sizeof(uint32_t) * cells) ||
             ctx_reserve(ctx, (void**)&ctx->fscratch, &ctx->fscratch_bytes, sizeof(float) * cells * (masses ? 2 : 1)) ||
             ctx_reserve(ctx, (void**)&ctx->ebuf, &ctx->ebuf_bytes, sizeof(uint32_t) * ecells) ||
@@ -2130,82 +2212,89 @@ extern "C" int32_t nb_hydro_density_mixed(nb_ctx* ctx, void* stream, const doubl
             return NB_ERR_HIP;
         if (hipMemsetAsync(ctx->efill, 0, sizeof(unsigned int) * n, s) != hipSuccess) return ctx_fail(ctx, "memset");
         float* rmass = masses ? ctx->fscratch + cells : nullptr;
-        const GridSeg* segs = (const GridSeg*)ctx->segs;
 #define NB_GATHER_MIXED(M, S)                                                                                        \
     NB_TIMED(ctx, s, "k_gather_mixed", (k_gather_mixed<M, S>), dim3(blocks), dim3(tile), 0, s,                      \
              (const double4*)ctx->posh, n, g, segs, cloud, ctx->cell_start, order_arg(ctx), ctx->gbuf, ctx->fscratch, \
-             rmass, cap, ctx->rowcnt, ctx->efill, ctx->ebuf, (float*)ctx->edbuf, ecap, A)
+             rmass, ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->ebuf, (float*)ctx->edbuf, ctx->rm_eo, A)
         if (masses && segmented) NB_GATHER_MIXED(true, true);
         else if (masses) NB_GATHER_MIXED(true, false);
         else if (segmented) NB_GATHER_MIXED(false, true);
         else NB_GATHER_MIXED(false, false);
 #undef NB_GATHER_MIXED
-        // [0] longest gather row, [1] rows left slow, [2] longest extras segment
-        finish_partials(ctx, s, blocks, 3, 5u, 0);
-        rc = sync_copy(ctx, s, sc, ctx->scal, sizeof(long long) * 3, "nb_hydro_density_mixed: gather failed");
+        // [0] longest gather row, [1] rows left slow, [2] extras overflow, [3] rows over capacity,
+        // [4] longest extras segment
+        finish_partials(ctx, s, blocks, 5, 1u | 4u | 16u, 0);
+        rc = sync_copy(ctx, s, sc, ctx->scal, sizeof(long long) * 5, "nb_hydro_density_mixed: gather failed");
         if (rc) return rc;
-        if (sc[0] <= cap && sc[2] <= ecap) break;
-        if (sc[0] > cap) cap = ((sc[0] + sc[0] / 8 + 15) / 16) * 16;
-        if (sc[2] > ecap) ecap = ((sc[2] + sc[2] / 8 + 15) / 16) * 16;
-        if (attempt == 7) return ctx_fail(ctx, "row capacity did not settle");
+        if (sc[2] == 0 && sc[3] == 0) break;
+        if (attempt == 2) return ctx_fail(ctx, "row capacity did not settle");
+        mode = 2;
     }
-    ctx->row_cap_rm = (int32_t)cap;
-    ctx->ecap_rm = ecap;
+    // uniform capacities for a later pass without counts
+    ctx->row_cap_rm = (int32_t)(sc[0] + sc[0] / 8 + 8);
+    ctx->ecap_rm = sc[4] + sc[4] / 4 + 4;
+    ctx->rm_gtotal = (int64_t)tot[0];
     const long long slow_rows = sc[1];
-    const size_t cells = (size_t)(n * cap);
-    float* rmass = masses ? ctx->fscratch + cells : nullptr;
+    float* rmass = masses ? ctx->fscratch + (size_t)tot[0] : nullptr;
     if (slow_rows > 0) {
         if (masses)
             NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<true>, dim3(blocks), dim3(tile), 0, s, h_search, n,
-                     ctx->fscratch, rmass, cap, ctx->rowcnt, A);
+                     ctx->fscratch, rmass, ctx->rm_go, ctx->rowcnt, A);
         else
             NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<false>, dim3(blocks), dim3(tile), 0, s, h_search,
-                     n, ctx->fscratch, rmass, cap, ctx->rowcnt, A);
+                     n, ctx->fscratch, rmass, ctx->rm_go, ctx->rowcnt, A);
     }
-    NB_TIMED(ctx, s, "k_sort_extras", k_sort_extras, dim3(blocks), dim3(tile), 0, s, ctx->ebuf, (float*)ctx->edbuf,
-             ctx->efill, ecap, n);
-    NB_TIMED(ctx, s, "k_row_total_rm", k_row_total_rm, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->rowcnt, ctx->efill,
-             ecap, ctx->part, n);
-    finish_partials(ctx, s, grid_of(n), 2, 1u, 3);  // [3] longest merged row, [4] search-list entries
-    rc = sync_copy(ctx, s, sc + 3, (long long*)ctx->scal + 3, sizeof(long long) * 2,
+    if (ctx_reserve(ctx, (void**)&ctx->ebuf_s, &ctx->ebuf_s_bytes, sizeof(uint32_t) * (size_t)tot[1]) ||
+        ctx_reserve(ctx, (void**)&ctx->edbuf_s, &ctx->edbuf_s_bytes, sizeof(float) * (size_t)tot[1]))
+        return NB_ERR_HIP;
+    NB_TIMED(ctx, s, "k_sort_extras", k_sort_extras, dim3(blocks), dim3(tile), 0, s, ctx->ebuf,
+             (const float*)ctx->edbuf, ctx->efill, ctx->rm_eo, n, ctx->ebuf_s, ctx->edbuf_s);
+    // merged row lengths (reusing the gather capacities' array), their scan: the tight rows' offsets
+    NB_TIMED(ctx, s, "k_row_total_rm", k_row_total_rm, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, ctx->rowcnt,
+             ctx->efill, ctx->rm_gc, row_counts, ctx->part, n);
+    finish_partials(ctx, s, grid_of(n + 1), 2, 1u, 5);  // [5] longest merged row, [6] search-list entries
+    rc = scan_u64(ctx, s, ctx->rm_gc, ctx->rm_to, n + 1);
+    if (rc) return rc;
+    rc = sync_copy(ctx, s, sc + 5, (long long*)ctx->scal + 5, sizeof(long long) * 2,
                    "nb_hydro_density_mixed: totals copy failed");
     if (rc) return rc;
     if (hipGetLastError() != hipSuccess) return ctx_fail(ctx, "nb_hydro_density_mixed: kernel launch failed");
     ctx->lists_n = n;
-    ctx->lists_maxrow = sc[3];
+    ctx->lists_maxrow = sc[5];
+    ctx->lists_total = sc[6];
     ctx->hydro_mixed_ready = 1;
     ctx->mix_kid = kid;
     ctx->mix_cnorm = cnorm;
     ctx->mix_m_uniform = m_uniform;
-    stats4[0] = cap;
-    stats4[1] = slow_rows;  // rows not converged in one Newton step (solved in full from the stored row)
-    stats4[2] = sc[4];      // search-list entries (incl. self)
-    stats4[3] = sc[3];      // longest merged row
+    stats4[0] = (int64_t)tot[0];  // gather-row entries allocated
+    stats4[1] = slow_rows;        // rows not converged in one Newton step (solved in full from the stored row)
+    stats4[2] = sc[6];            // search-list entries (incl. self): the capacity the tight rows need
+    stats4[3] = sc[5];            // longest merged row
     return NB_OK;
 }
 
 extern "C" int32_t nb_hydro_tight_mixed(nb_ctx* ctx, void* stream, const double* pos, const double* h_final,
-                                        double* rho, double* omega, int32_t* out_len, int32_t* out_index,
-                                        int64_t row_cap, int64_t* total_out)
+                                        double* rho, double* omega, int64_t* out_off, int32_t* out_len,
+                                        int32_t* out_index, int64_t index_cap, int64_t* total_out)
 {
     hipStream_t s = (hipStream_t)stream;
     if (!ctx || !ctx->hydro_mixed_ready || !total_out)
         return ctx_fail_code(ctx, NB_ERR_NOT_BUILT, "nb_hydro_tight_mixed: no density pass");
-    if (!pos || !h_final || !rho || !omega || !out_len || !out_index)
+    if (!pos || !h_final || !rho || !omega || !out_off || !out_len || !out_index)
         return ctx_fail(ctx, "nb_hydro_tight_mixed: bad arguments");
-    if (row_cap < ctx->lists_maxrow) return ctx_fail(ctx, "nb_hydro_tight_mixed: row capacity below the longest row");
+    if (index_cap < ctx->lists_total)
+        return ctx_fail(ctx, "nb_hydro_tight_mixed: index capacity below the search-list entries");
     const int64_t n = ctx->lists_n;
-    const int64_t cap = ctx->row_cap_rm;
     const bool masses = !(ctx->mix_m_uniform > 0.0);
-    const FinArgs F{masses ? ctx->fscratch + (size_t)(n * cap) : nullptr, ctx->mix_m_uniform, ctx->mix_kid,
+    const FinArgs F{masses ? ctx->fscratch + (size_t)ctx->rm_gtotal : nullptr, ctx->mix_m_uniform, ctx->mix_kid,
                     ctx->mix_cnorm, ctx->pending, rho, omega};
     const int tile = ctx->tile;
     const unsigned blocks = (unsigned)((n * kG + tile - 1) / tile);
     if (ctx->part_bytes < sizeof(long long) * (size_t)blocks)
         return ctx_fail(ctx, "nb_hydro_tight_mixed: reduction buffer smaller than the launch");
     NB_TIMED(ctx, s, "k_tight_emit_rm", k_tight_emit_rm, dim3(blocks), dim3(tile), 0, s, pos, ctx->gbuf, ctx->fscratch,
-             cap, ctx->rowcnt, ctx->efill, ctx->ecap_rm, ctx->ebuf, (const float*)ctx->edbuf, h_final, n, out_len,
-             out_index, row_cap, F, ctx->part);
+             ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->rm_eo, ctx->ebuf_s, ctx->edbuf_s, h_final, n, ctx->rm_to,
+             out_off, out_len, out_index, F, ctx->part);
     finish_partials(ctx, s, blocks, 1, 0u, 8);
     long long t = 0;
     int rc = sync_copy(ctx, s, &t, (long long*)ctx->scal + 8, sizeof(t), "nb_hydro_tight_mixed: total copy failed");

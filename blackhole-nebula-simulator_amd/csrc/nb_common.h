@@ -90,14 +90,15 @@ struct BlkView {
     __device__ __forceinline__ int32_t at(int64_t i, int64_t k) const { return idx[base(i) + k * 64]; }
 };
 
+// rows of their own capacity: row i has len[i] entries from off[i]
 struct RowView {
     const int32_t* idx;
     const int32_t* len;
-    int64_t cap;
+    const int64_t* off;
     static constexpr int64_t stride = 1;
     __device__ __forceinline__ int64_t count(int64_t i) const { return len[i]; }
-    __device__ __forceinline__ int64_t base(int64_t i) const { return i * cap; }
-    __device__ __forceinline__ int32_t at(int64_t i, int64_t k) const { return idx[i * cap + k]; }
+    __device__ __forceinline__ int64_t base(int64_t i) const { return off[i]; }
+    __device__ __forceinline__ int32_t at(int64_t i, int64_t k) const { return idx[off[i] + k]; }
 };
 
 // Wave-wide reductions (every lane of the wavefront must call them); the result is valid in

@@ -58,6 +58,15 @@ struct nb_ctx {
     float* fscratch = nullptr; size_t fscratch_bytes = 0;
     void* pack = nullptr; size_t pack_bytes = 0;  // per-particle FP32 records of the mixed passes
     long long* part = nullptr; size_t part_bytes = 0;  // per-workgroup partials of reductions
+    // mixed lists: per-row capacities and offsets (gather rows, extras segments, tight rows)
+    unsigned long long* rm_gc = nullptr; size_t rm_gc_bytes = 0;
+    unsigned long long* rm_go = nullptr; size_t rm_go_bytes = 0;
+    unsigned long long* rm_ec = nullptr; size_t rm_ec_bytes = 0;
+    unsigned long long* rm_eo = nullptr; size_t rm_eo_bytes = 0;
+    unsigned long long* rm_to = nullptr; size_t rm_to_bytes = 0;
+    uint32_t* ebuf_s = nullptr; size_t ebuf_s_bytes = 0;  // extras in index order
+    float* edbuf_s = nullptr; size_t edbuf_s_bytes = 0;
+    int64_t rm_gtotal = 0;  // gather-row entries of the last mixed pass (start of the mass rows)
     void* segs = nullptr; size_t segs_bytes = 0;       // grid table of an ensemble's segments
     std::vector<char> host_segs;                       // its host copy
     void* posh = nullptr; size_t posh_bytes = 0;  // (x, y, z, h_search) records of the mixed gather

@@ -440,7 +440,7 @@ static int32_t launch_check(nb_ctx* ctx, const char* what)
     do {                                                                                                \
         const int ts_ = ktime_begin(ctx, (STREAM), #KERNEL);                                            \
         if ((L)->length && (L)->row_major) {                                                            \
-            RowView v_{(L)->index, (L)->length, (L)->row_cap};                                          \
+            RowView v_{(L)->index, (L)->length, (L)->offset};                                           \
             hipLaunchKernelGGL(KERNEL<RowView>, dim3(GRID), dim3(kBlockP), 0, STREAM, __VA_ARGS__);      \
         } else if ((L)->length) {                                                                       \
             BlkView v_{(L)->index, (L)->length, (L)->row_cap};                                          \

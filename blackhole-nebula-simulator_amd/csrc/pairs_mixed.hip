@@ -288,7 +288,7 @@ static inline Box box_of(const double* box3)
     do {                                                                                                \
         const int ts_ = ktime_begin(ctx, (STREAM), #KERNEL);                                            \
         if ((L)->length && (L)->row_major) {                                                            \
-            RowView v_{(L)->index, (L)->length, (L)->row_cap};                                          \
+            RowView v_{(L)->index, (L)->length, (L)->offset};                                           \
             hipLaunchKernelGGL(KERNEL<RowView>, dim3(GRID), dim3(ctx->tile), 0, STREAM, __VA_ARGS__);    \
         } else if ((L)->length) {                                                                       \
             BlkView v_{(L)->index, (L)->length, (L)->row_cap};                                          \

@@ -109,9 +109,9 @@ SIGNATURES = {
     "nb_hydro_density": (_i32, [_vp, _vp, _vp, _vp, _f64, _vp, _vp, _i64, _i32, _f64, _f64, _f64, _f64, _i32, _vp,
                                 _vp, _vp, _vp, _vp, _P64]),
     "nb_hydro_tight": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _P64]),
-    "nb_hydro_density_mixed": (_i32, [_vp, _vp, _vp, _vp, _f64, _vp, _vp, _i64, _vp, _P64, _i32, _f64, _i32, _f64,
-                                      _f64, _f64, _f64, _vp, _vp, _vp, _vp, _vp, _P64]),
-    "nb_hydro_tight_mixed": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _P64]),
+    "nb_hydro_density_mixed": (_i32, [_vp, _vp, _vp, _vp, _f64, _vp, _vp, _i64, _vp, _P64, _i32, _f64, _vp, _vp, _i32,
+                                      _f64, _f64, _f64, _f64, _vp, _vp, _vp, _vp, _vp, _P64]),
+    "nb_hydro_tight_mixed": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _P64]),
     "nb_timing": (_i32, [_vp, _i32]),
     "nb_set_tile": (_i32, [_vp, _i32]),
     "nb_dt_minima_seg": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _f64, _vp, _P64, _i32, _PD]),
@@ -131,7 +131,7 @@ SIGNATURES = {
 }
 
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 
 def load_library() -> ctypes.CDLL:

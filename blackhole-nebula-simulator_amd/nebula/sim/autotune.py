@@ -61,7 +61,11 @@ def tune(st, cfg, factors=FACTORS, tiles=TILES, repeats: int = 3) -> Choice:
         trial = replace(st, cell_factor=float(f))
         order = coupled.grid_permutation(cfg, trial.x, trial.h, trial)
         x, p, u, m, h, a, acc = permute_many((st.x, st.p, st.u, st.mass, st.h, st.alpha, st.a_hydro), order)
-        trial = replace(trial, x=x, p=p, u=u, mass=m, h=h, alpha=a, a_hydro=acc)
+        counts = None
+        if st.row_counts is not None:
+            n = int(st.x.shape[0])
+            counts = permute_many((st.row_counts.view(torch.int64).view(n),), order)[0].view(torch.int32).view(n, 2)
+        trial = replace(trial, x=x, p=p, u=u, mass=m, h=h, alpha=a, a_hydro=acc, row_counts=counts)
         for t in tiles:
             _hip.set_tile(t)
             table[(float(f), int(t))] = _time_pass(trial, cfg, repeats)

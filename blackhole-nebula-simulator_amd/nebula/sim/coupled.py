@@ -33,7 +33,7 @@ from ..physics import units
 from .fastphys import gn_v_from_p_device
 from .hydro import HydroConfig
 from .integrator import glf_device
-from .neighbours import grid_order, grid_order_seg, permute, permute_many
+from .neighbours import IndexBuffer, grid_order, grid_order_seg, permute, permute_many
 from .sph import (
     COND_VELOCITY,
     hydro_density_device,
@@ -112,6 +112,10 @@ class GasState:
     cloud: torch.Tensor = field(default=None)
     seg_off: np.ndarray = field(default=None)
     cell_factor: float = 1.0  # neighbour-grid cell edge relative to the default (mixed path; autotuned)
+    # storage of the tight lists, reused from step to step (neighbours.IndexBuffer)
+    list_buffer: object = field(default=None, compare=False, repr=False)
+    # mixed mode: (N, 2) int32 search-row and extras counts of the last density pass (sizes the next)
+    row_counts: torch.Tensor = field(default=None, compare=False, repr=False)
 
     @property
     def learned_fallbacks(self) -> int:
@@ -212,15 +216,21 @@ def _hydro(st: GasState, cfg: CoupledConfig, v, dt_alpha: float, a_prev, h_start
     if hc.isothermal_cs is not None or hc.eos != "ideal":
         raise ValueError("the GPU step implements the ideal-gas equation of state")
     n = int(st.x.shape[0])
+    buf = st.list_buffer if st.list_buffer is not None else IndexBuffer()
+    counts = None
     if hc.precision == "mixed":
+        counts = _dev.empty((n, 2), torch.int32)
         d, n_search = hydro_density_mixed_device(st.x, st.mass, st.h.clone(), hc.kernel, hc.eta, 1e-4, hc.h_max,
                                                  hc.skin, st.m_uniform, h_start_d=h_start, cloud_d=st.cloud,
-                                                 seg_off=st.seg_off, cell_factor=st.cell_factor)  # fmt: skip
+                                                 seg_off=st.seg_off, cell_factor=st.cell_factor,
+                                                 index_buffer=buf, row_hint=st.row_counts,
+                                                 row_counts=counts)  # fmt: skip
     else:
         if st.cloud is not None or st.cell_factor != 1.0:
             raise ValueError("ensembles and tuned grids run in the mixed-precision mode")
         d, n_search = hydro_density_device(st.x, st.mass, st.h.clone(), hc.kernel, hc.eta, 1e-4, hc.h_max, hc.skin,
-                                           st.m_uniform, hc.precision, h_start_d=h_start)  # fmt: skip
+                                           st.m_uniform, hc.precision, h_start_d=h_start,
+                                           index_buffer=buf)  # fmt: skip
     p_gas = _dev.empty(n)
     cs = _dev.empty(n)
     nbad = torch.zeros(1, dtype=torch.int32, device=st.x.device)
@@ -282,7 +292,7 @@ def _hydro(st: GasState, cfg: CoupledConfig, v, dt_alpha: float, a_prev, h_start
         n_search_entries=n_search,
         n_tight_entries=nb.n_total,
         div_v=None if g is None else g.div_v, curl_v=None if g is None else g.curl_v,
-        nneigh=nb.length.double(),
+        nneigh=nb.length.double(), list_buffer=buf, row_counts=counts,
     )  # fmt: skip
 
 
@@ -367,17 +377,23 @@ def step(st: GasState, cfg: CoupledConfig, dt: float) -> GasState:
     order = grid_permutation(cfg, x1, st.h, st)
     model = _learned(cfg)
     fields = [x1, p_flow, p_pred, u_half, u_pred, st.a_hydro, st.mass, st.h, st.alpha, st.perm]
+    counts = st.row_counts
+    if counts is not None:
+        fields.append(counts.view(torch.int64).view(n))  # two int32 counts per particle move as one
     extra = []
     if model is not None:
         # the learned start's inputs from the last hydro pass, in the new order
         extra = [t for t in (st.rho, st.div_v, st.curl_v, st.nneigh)]
     moved = permute_many(fields + [t for t in extra if t is not None], order)
     x1, p_flow, p_pred, u_half, u_pred, a_prev, mass, h, alpha, perm = moved[:10]
-    pred = replace(st, t=st.t + dt, x=x1, p=p_flow, u=u_pred, mass=mass, h=h, alpha=alpha, perm=perm)
+    if counts is not None:
+        counts = moved[10].view(torch.int32).view(n, 2)
+    pred = replace(st, t=st.t + dt, x=x1, p=p_flow, u=u_pred, mass=mass, h=h, alpha=alpha, perm=perm,
+                   row_counts=counts)  # fmt: skip
     h_start = feat = None
     fallbacks = None
     if model is not None:
-        it = iter(moved[10:])
+        it = iter(moved[len(fields):])
         rho_p, div_p, curl_p, nn_p = (None if t is None else next(it) for t in extra)
         stride = model.collect.stride if model.collect is not None else 1
         feat = model.features(h, rho_p, div_p, curl_p, nn_p, x1, u_pred, dt, st.dt_prev, stride=1)

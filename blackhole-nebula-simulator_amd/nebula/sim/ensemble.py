@@ -108,9 +108,11 @@ def step(st: GasState, cfg: coupled.CoupledConfig, dt_clouds: np.ndarray) -> Gas
     p_pred = kick(p_flow, st.a_hydro, 3)
     u_pred = kick(u_half, st.dudt, 1)
     order = coupled.grid_permutation(cfg, x1, st.h, st)
-    x1, p_flow, p_pred, u_half, u_pred, a_prev, mass, h, alpha, perm, half = permute_many(
-        (x1, p_flow, p_pred, u_half, u_pred, st.a_hydro, st.mass, st.h, st.alpha, st.perm, half), order)
-    pred = replace(st, x=x1, p=p_flow, u=u_pred, mass=mass, h=h, alpha=alpha, perm=perm)
+    counts = st.row_counts.view(torch.int64).view(n)  # two int32 counts per particle move as one
+    x1, p_flow, p_pred, u_half, u_pred, a_prev, mass, h, alpha, perm, half, counts = permute_many(
+        (x1, p_flow, p_pred, u_half, u_pred, st.a_hydro, st.mass, st.h, st.alpha, st.perm, half, counts), order)
+    pred = replace(st, x=x1, p=p_flow, u=u_pred, mass=mass, h=h, alpha=alpha, perm=perm,
+                   row_counts=counts.view(torch.int32).view(n, 2))  # fmt: skip
     new = _hydro(pred, cfg, velocity(cfg, x1, p_pred), 0.0, a_prev)
 
     return replace(new, p=kick(p_flow, new.a_hydro, 3), u=kick(u_half, new.dudt, 1),

@@ -124,14 +124,14 @@ class BlockedLists:
 
 
 class RowLists:
-    """Neighbour lists in row-major rows of capacity ``row_cap`` (the mixed-precision step's
-    layout): row i has ``length[i]`` entries at index[i * row_cap : i * row_cap + length[i]],
-    not sorted."""
+    """Neighbour lists in rows of their own capacity (the mixed-precision step's layout): row i
+    has ``length[i]`` entries at index[offset[i] : offset[i] + length[i]], not sorted. ``offset``
+    has N + 1 entries."""
 
-    def __init__(self, length: torch.Tensor, index: torch.Tensor, row_cap: int, n_total: int, box=None):
+    def __init__(self, length: torch.Tensor, index: torch.Tensor, offset: torch.Tensor, n_total: int, box=None):
         self.length = length
         self.index = index
-        self.row_cap = int(row_cap)
+        self.offset = offset
         self.n_total = int(n_total)
         self.box = box
 
@@ -141,10 +141,10 @@ class RowLists:
 
     def desc(self) -> _hip.Lists:
         d = _hip.Lists()
-        d.offset = None
+        d.offset = self.offset.data_ptr()
         d.index = self.index.data_ptr()
         d.length = self.length.data_ptr()
-        d.row_cap = self.row_cap
+        d.row_cap = 0
         d.n_entries = self.n_entries
         d.row_major = 1
         d.n_rows = int(self.length.numel())
@@ -154,18 +154,39 @@ class RowLists:
         """The same lists as a CSR Neighbours (host arrays, rows sorted)."""
         n = int(self.length.numel())
         ln = self.length.cpu().numpy().astype(np.int64)
-        idx = self.index.cpu().numpy().reshape(n, self.row_cap)
+        start = self.offset.cpu().numpy()
+        idx = self.index.cpu().numpy()
         off = np.zeros(n + 1, np.int64)
         np.cumsum(ln, out=off[1:])
         out = np.empty(off[-1], np.int64)
         for i in range(n):
-            out[off[i]:off[i + 1]] = np.sort(idx[i, : ln[i]])
+            out[off[i]:off[i + 1]] = np.sort(idx[start[i]:start[i] + ln[i]])
         return Neighbours(off, out, self.box)
 
 
-def _blocked_alloc(n: int, row_cap: int):
+class IndexBuffer:
+    """Reusable device storage for the tight lists of a time-stepping loop. The list size drifts
+    from step to step; a fresh allocation per step would leave the caching allocator with ever
+    larger blocks split by the step's smaller tensors. take() hands out a view of one buffer that
+    grows by a quarter beyond the request when it is too small. The lists of a step are only valid
+    until the next take()."""
+
+    def __init__(self):
+        self._t = None
+
+    def take(self, numel: int) -> torch.Tensor:
+        numel = max(int(numel), 1)
+        if self._t is None or self._t.numel() < numel:
+            self._t = None
+            self._t = _dev.empty(numel + numel // 4 + 4096, torch.int32)
+        return self._t[:numel]
+
+
+def _blocked_alloc(n: int, row_cap: int, index_buffer: IndexBuffer | None = None):
     nblk = (n + 63) // 64
-    return _dev.empty(n, torch.int32), _dev.empty(max(nblk * row_cap * 64, 1), torch.int32)
+    size = max(nblk * row_cap * 64, 1)
+    idx = _dev.empty(size, torch.int32) if index_buffer is None else index_buffer.take(size)
+    return _dev.empty(n, torch.int32), idx
 
 
 def build_lists_blocked(pos_d: torch.Tensor, h_d: torch.Tensor, box_arr, row_cap_hint: int = 0) -> BlockedLists:

@@ -149,7 +149,7 @@ def _round_cap(cap: int) -> int:
 
 
 def hydro_density_device(pos_d, mass_d, h_guess_d, kernel_name, eta, tol, h_max, skin, m_uniform,
-                         precision="fp64", row_cap_hint=0, h_start_d=None):  # fmt: skip
+                         precision="fp64", row_cap_hint=0, h_start_d=None, index_buffer=None):  # fmt: skip
     """The density pass of the coupled step (open box): compute_density's rebuild rounds
     (sph.py:329-387) on the fused gather + h-rho solve kernel, then the tightened lists at the
     final h, in 64-row blocks. h_guess_d is overwritten. Returns (DensityResult with the tight
@@ -179,7 +179,7 @@ def hydro_density_device(pos_d, mass_d, h_guess_d, kernel_name, eta, tol, h_max,
     else:
         raise RuntimeError("smoothing lengths kept outgrowing the neighbour search")
     cap = _round_cap(int(stats[3]))
-    ln, idx = _blocked_alloc(n, cap)
+    ln, idx = _blocked_alloc(n, cap, index_buffer)
     total = np.zeros(1, np.int64)
     _hip.call("nb_hydro_tight", _hip.ptr(h), _hip.ptr(ln), _hip.ptr(idx), cap, total.ctypes.data_as(_hip._P64))
     tight = BlockedLists(ln, idx, cap, int(total[0]), None)
@@ -187,10 +187,13 @@ def hydro_density_device(pos_d, mass_d, h_guess_d, kernel_name, eta, tol, h_max,
 
 
 def hydro_density_mixed_device(pos_d, mass_d, h_guess_d, kernel_name, eta, tol, h_max, skin, m_uniform,
-                               h_start_d=None, cloud_d=None, seg_off=None, cell_factor=1.0):  # fmt: skip
+                               h_start_d=None, cloud_d=None, seg_off=None, cell_factor=1.0,
+                               index_buffer=None, row_hint=None, row_counts=None):  # fmt: skip
     """hydro_density_device in mixed precision: the same exact search lists, FP32 compensated
-    sums in the h-rho solve (Newton steps and convergence test in FP64), tight lists row-major.
-    Returns (DensityResult with RowLists as ``neighbours``, search-list entries)."""
+    sums in the h-rho solve (Newton steps and convergence test in FP64), tight lists in rows of
+    their own capacity. row_hint / row_counts: (N, 2) int32 device tensors, the previous pass's
+    row counts of the same particles (sizes the rows) and this pass's (filled). Returns
+    (DensityResult with RowLists as ``neighbours``, search-list entries)."""
     from .neighbours import RowLists
 
     n = int(pos_d.shape[0])
@@ -208,7 +211,9 @@ def hydro_density_mixed_device(pos_d, mass_d, h_guess_d, kernel_name, eta, tol, 
         start = h_start_d if (h_start_d is not None and builds == 1) else h_guess_d
         _hip.call("nb_hydro_density_mixed", _hip.ptr(pos_d), _hip.ptr(mass_d), float(m_uniform), _hip.ptr(start),
                   _hip.ptr(h_search), n, _hip.ptr(cloud_d), None if off is None else off.ctypes.data_as(_hip._P64),
-                  0 if off is None else len(off) - 1, float(cell_factor), kid, float(C_NORM[kid]), float(eta),
+                  0 if off is None else len(off) - 1, float(cell_factor),
+                  _hip.ptr(row_hint if builds == 1 or row_counts is None else row_counts), _hip.ptr(row_counts),
+                  kid, float(C_NORM[kid]), float(eta),
                   float(tol), float(h_max), _hip.ptr(h),
                   _hip.ptr(rho), _hip.ptr(omega), _hip.ptr(iters), _hip.ptr(flags),
                   stats.ctypes.data_as(_hip._P64))  # fmt: skip
@@ -218,13 +223,14 @@ def hydro_density_mixed_device(pos_d, mass_d, h_guess_d, kernel_name, eta, tol, 
             break
     else:
         raise RuntimeError("smoothing lengths kept outgrowing the neighbour search")
-    cap = _round_cap(int(stats[3]))
+    entries = int(stats[2])
     ln = _dev.empty(n, torch.int32)
-    idx = _dev.empty(n * cap, torch.int32)
+    off_t = _dev.empty(n + 1, torch.int64)
+    idx = _dev.empty(entries, torch.int32) if index_buffer is None else index_buffer.take(entries)
     total = np.zeros(1, np.int64)
-    _hip.call("nb_hydro_tight_mixed", _hip.ptr(pos_d), _hip.ptr(h), _hip.ptr(rho), _hip.ptr(omega), _hip.ptr(ln),
-              _hip.ptr(idx), cap, total.ctypes.data_as(_hip._P64))  # fmt: skip
-    tight = RowLists(ln, idx, cap, int(total[0]), None)
+    _hip.call("nb_hydro_tight_mixed", _hip.ptr(pos_d), _hip.ptr(h), _hip.ptr(rho), _hip.ptr(omega),
+              _hip.ptr(off_t), _hip.ptr(ln), _hip.ptr(idx), entries, total.ctypes.data_as(_hip._P64))  # fmt: skip
+    tight = RowLists(ln, idx, off_t, int(total[0]), None)
     return DensityResult(h, rho, omega, iters, flags, tight, None, builds), int(stats[2])
 
 

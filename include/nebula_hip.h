@@ -54,9 +54,10 @@ typedef struct nb_force_params {
     double eta_fold;
 } nb_force_params;
 
-/* A neighbour list: CSR when length == NULL; else rows of capacity row_cap, in 64-row blocks
- * (row_major 0: entry k of row i at ((i / 64) row_cap + k) 64 + i % 64) or row-major (row_major 1:
- * entry k of row i at i row_cap + k; the mixed-precision step's layout, rows not sorted).
+/* A neighbour list: CSR when length == NULL; else rows in 64-row blocks of capacity row_cap
+ * (row_major 0: entry k of row i at ((i / 64) row_cap + k) 64 + i % 64) or rows of their own
+ * capacity (row_major 1: entry k of row i at offset[i] + k, k < length[i]; the mixed-precision
+ * step's layout, rows not sorted, row_cap unused).
  * n_entries: CSR offset[N], or the size of the entry array (the size of per-entry scratch).
  * n_rows: N, the particles the lists cover (0 = not given; the mixed-precision passes need it
  * when they run on a row subset). */
@@ -160,27 +161,34 @@ int32_t nb_hydro_tight(nb_ctx *ctx, void *stream, const double *h_final, int32_t
 
 /* ------------------------------------------------------------------ mixed-precision step passes */
 /* The density pass of coupled.step for the mixed-precision mode (open box): the same search
- * lists as nb_hydro_density (FP64 predicate, exact), gathered by 16-lane groups into row-major
- * rows (entry k of row i at i * cap + k) with FP32 d2; the h-rho solve sums in FP32 with
- * compensated accumulation and takes its Newton steps in FP64. cloud / seg_off / nseg: an ensemble
- * of independent systems as in nb_grid_order_seg (cloud NULL: one system); cell_factor as in
- * nb_grid_order. Writes h_out, iters and flags of
- * every row; rho and omega of the rows that needed more than one Newton step (the others are
- * completed by nb_hydro_tight_mixed, which evaluates them at the final h while it reads the
- * rows). stats4 (host) = [row capacity used, rows needing more than one step, search-list
- * entries, longest merged row]. */
+ * lists as nb_hydro_density (FP64 predicate, exact), gathered by 16-lane groups into rows of their
+ * own capacity with FP32 d2; the h-rho solve sums in FP32 with compensated accumulation and takes
+ * its Newton steps in FP64. cloud / seg_off / nseg: an ensemble of independent systems as in
+ * nb_grid_order_seg (cloud NULL: one system); cell_factor as in nb_grid_order.
+ * Row capacities come from row_hint (device, 2 N: gather-row and extras counts of each particle
+ * from a previous pass, e.g. row_counts of the last step carried through the particles'
+ * permutation) with headroom, or are uniform when row_hint is NULL; a row that outgrows its
+ * capacity makes the pass repeat with the exact counts. row_counts (device, 2 N, may be NULL)
+ * receives this pass's counts. Writes h_out, iters and flags of every row; rho and omega of the
+ * rows that needed more than one Newton step (the others are completed by nb_hydro_tight_mixed,
+ * which evaluates them at the final h while it reads the rows). stats4 (host) = [gather-row
+ * entries allocated, rows needing more than one step, search-list entries (the index capacity
+ * nb_hydro_tight_mixed needs), longest merged row]. */
 int32_t nb_hydro_density_mixed(nb_ctx *ctx, void *stream, const double *pos, const double *mass, double m_uniform,
                                const double *h_start, const double *h_search, int64_t n, const int32_t *cloud,
-                               const int64_t *seg_off, int32_t nseg, double cell_factor, int32_t kid, double cnorm,
-                               double eta, double tol, double h_max, double *h_out, double *rho, double *omega,
-                               int32_t *iters, int32_t *flags, int64_t *stats4);
+                               const int64_t *seg_off, int32_t nseg, double cell_factor, const int32_t *row_hint,
+                               int32_t *row_counts, int32_t kid, double cnorm, double eta, double tol, double h_max,
+                               double *h_out, double *rho, double *omega, int32_t *iters, int32_t *flags,
+                               int64_t *stats4);
 /* tighten (neighbours.py:306-323) of the last nb_hydro_density_mixed lists at h_final (its
- * h_out), into row-major rows of capacity row_cap (>= the longest merged row); pos as in the
- * density pass. Rows are exactly tighten() of the search lists (borderline FP32 d2 re-formed in
- * FP64). Completes rho and omega (the same arrays given to the density pass). */
+ * h_out), into rows of their own capacity: row i at out_off[i] (N + 1 entries written; row i's
+ * capacity is its merged search row) with out_len[i] entries; out_index holds index_cap >= the
+ * search-list entries (stats4[2]). pos as in the density pass. Rows are exactly tighten() of the
+ * search lists (borderline FP32 d2 re-formed in FP64). Completes rho and omega (the same arrays
+ * given to the density pass). */
 int32_t nb_hydro_tight_mixed(nb_ctx *ctx, void *stream, const double *pos, const double *h_final, double *rho,
-                             double *omega, int32_t *out_len, int32_t *out_index, int64_t row_cap,
-                             int64_t *total_out);
+                             double *omega, int64_t *out_off, int32_t *out_len, int32_t *out_index,
+                             int64_t index_cap, int64_t *total_out);
 /* gradients_numba (sph.py:837-884) in mixed precision: FP32 pair terms over 16-lane row
  * groups, compensated sums, partials combined and the 3x3 inverse taken in FP64. Arguments as
  * nb_gradients. (The force pass takes its precision from nb_force_params.precision.) */

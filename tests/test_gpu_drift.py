@@ -1,0 +1,72 @@
+"""1000-step energy and angular-momentum ledgers of the GPU step (coupled.py:261-270), plain FP64
+against mixed precision with the learned starting h.
+
+The workload is the plain-method cloud: a 4096-particle sphere of 0.1 M_sun and radius 150 on
+its parabolic orbit from r = 20000 with pericentre 2000, cubic spline, eta 1.2, Monaghan
+alpha 1, beta 2, at the default step factors. Over 1000 steps the ledger error of this run is set
+by the integrator's step (the reference's 1e-5 bound of tests/sim/test_coupled.py:44-88 holds at
+its halved and quartered step factors, not at these), so the FP64 GPU run, bit-identical to the
+FP64 oracle, sets the tolerance: the mixed run's maximum energy error may exceed the FP64 run's
+by at most 1e-3 of it plus 1e-8 of sum m|H|. Angular momentum: both runs within 1e-10 (the
+reference's bound).
+"""
+
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+from helpers import check
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="needs a GPU")]
+
+STEPS = 1000
+
+
+def _ledger_run(ic, cfg, steps):
+    from nebula.sim import coupled
+
+    st = coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg)
+    l0 = coupled.ledger(st, cfg)
+    e_max = l_max = 0.0
+    iters = []
+    for _ in range(steps):
+        st = coupled.step(st, cfg, coupled.stable_dt(st, cfg))
+        lg = coupled.ledger(st, cfg)
+        e_max = max(e_max, abs(lg["energy"] - l0["energy"]) / l0["abs_h"])
+        l_max = max(l_max, float(np.linalg.norm(lg["angmom"] - l0["angmom"]) / np.linalg.norm(l0["angmom"])))
+        iters.append(float(st.iterations.double().mean().item()))
+    return st, e_max, l_max, float(np.mean(iters))
+
+
+def test_ledger_drift_1000_steps_mixed_learned_vs_fp64():
+    from dataclasses import replace
+
+    from nebula.ics import jittered_sphere_on_parabola
+    from nebula.sim import coupled
+    from nebula.sim.learned_h import LearnedH, PairLog
+
+    ic = jittered_sphere_on_parabola(4096)
+    cfg64 = coupled.baseline_config()
+    cfg_m = coupled.baseline_config(precision="mixed")
+    # the network: pairs logged over 60 plain mixed steps of another seed of the cloud
+    model = LearnedH(seed=3)
+    model.collect = PairLog()
+    other = jittered_sphere_on_parabola(4096, seed=99)
+    print("drift: logging 60 steps for the learned start", flush=True)
+    _ledger_run(other, replace(cfg_m, hydro=replace(cfg_m.hydro, learned_model=model)), 60)
+    feat, target = model.collect.tensors()
+    model.collect = None
+    model.fit_normalisation(feat, target)
+    model.train(feat, target, steps=300, batch=4096)
+    cfg_l = replace(cfg_m, hydro=replace(cfg_m.hydro, learned_h=True, learned_model=model))
+    print(f"drift: {STEPS} steps, FP64", flush=True)
+    s64, e64, l64, it64 = _ledger_run(ic, cfg64, STEPS)
+    print(f"drift: {STEPS} steps, mixed + learned h", flush=True)
+    sml, eml, lml, itml = _ledger_run(ic, cfg_l, STEPS)
+    print(f"max |dE| / sum m|H| over {STEPS} steps: FP64 {e64:.3g}, mixed+learned {eml:.3g}; "
+          f"max |dL| / |L|: FP64 {l64:.3g}, mixed+learned {lml:.3g}; t_end {s64.t:.6g} / {sml.t:.6g}; "
+          f"Newton iterations {it64:.3f} / {itml:.3f}")  # fmt: skip
+    check("FP64: energy error finite and below 1e-2 of sum m|H|", True, bool(e64 < 1e-2))
+    check("mixed+learned energy error within the FP64 run's (x 1.001 + 1e-8)", True, eml <= 1.001 * e64 + 1e-8)
+    check("FP64: max angular-momentum error < 1e-10", True, l64 < 1e-10)
+    check("mixed+learned: max angular-momentum error < 1e-10", True, lml < 1e-10)

@@ -123,6 +123,9 @@ def run_gpu(ic, cfg, steps, warmup, learned=False, log_stride=8, train_steps=300
         info = {"pairs": int(target.numel()), "logged_steps": len(plain_iters), "train_steps": train_steps,
                 "train_s": time.perf_counter() - t0, "loss_first": losses[0], "loss_last": losses[-1],
                 "newton_iters_plain": sum(plain_iters) / len(plain_iters)}  # fmt: skip
+        online = dict(stride=max(1, len(ic.x) // 16384), every=8, steps=4)
+        model.start_online(**online)  # keeps learning from every step's own pairs
+        info["online"] = online
         cfg = replace(cfg, hydro=replace(cfg.hydro, learned_h=True, learned_model=model))
         info["_cfg"] = cfg
         st = coupled.step(st, cfg, coupled.stable_dt(st, cfg))  # first learned step: warm-up
@@ -334,18 +337,23 @@ def main():
                      "work": f"{work / n:.0f} {'FLOP' if bound != 'hbm' else 'B'}/particle per launch "
                              f"(n_s {n_search:.1f}, pairs {n_pairs:.1f}, Newton iterations {k_it:.2f})"},
         "kernels_ms_per_step": _per_step(kern, args.steps),
+        "gather_launches_per_step": kern["k_gather_mixed" if not fp64 else "k_gather_solve"][0] / args.steps,
         "learned_h": {k: v for k, v in info.items() if not k.startswith("_")} if learned else None,
         "newton_iters": info["newton_iters"],
     }
     if not args.no_compare:
         other = "fp64" if args.precision == "mixed" else "mixed"
-        print(f"bench: {other} alongside", file=sys.stderr, flush=True)
         del st
-        w2, k2, _, _, info2 = run_gpu(ic, config(other), args.steps, args.warmup)
-        v2 = n * args.steps / w2
-        out["compare"] = {other: {"value": v2, "ms_per_step": w2 / args.steps * 1e3, "value_ratio": value / v2,
-                                  "newton_iters": info2["newton_iters"],
-                                  "kernels_ms_per_step": _per_step(k2, args.steps)}}  # fmt: skip
+        out["compare"] = {}
+        alongside = [other] + (["mixed"] if learned else [])  # and the mixed mode without the learned start
+        for name in alongside:
+            print(f"bench: {name} alongside", file=sys.stderr, flush=True)
+            w2, k2, _, _, info2 = run_gpu(ic, config(name), args.steps, args.warmup)
+            v2 = n * args.steps / w2
+            gk = k2.get("k_gather_mixed", k2.get("k_gather_solve", (0, 0.0)))
+            out["compare"][name] = {"value": v2, "ms_per_step": w2 / args.steps * 1e3, "value_ratio": value / v2,
+                                    "newton_iters": info2["newton_iters"], "gather_launches_per_step": gk[0] / args.steps,
+                                    "kernels_ms_per_step": _per_step(k2, args.steps)}  # fmt: skip
     if args.drift_steps > 0:
         runs = {args.precision + ("+learned" if learned else ""): info["_cfg"] if learned else cfg}
         if not args.no_compare:

@@ -1014,7 +1014,9 @@ struct MixArgs {
 // Row capacities of the mixed lists (gather row gc, extras segment ec): uniform (mode 0), from
 // the previous pass's counts of the same particles with headroom (mode 1; hint[2 i] gather row,
 // hint[2 i + 1] extras), or from this pass's exact counts after an overflow (mode 2). Entry n is
-// 0, so the exclusive scans end in the totals. Rows are kept 16-byte aligned.
+// 0, so the exclusive scans end in the totals. Rows are kept 16-byte aligned. The headroom of
+// mode 1 (a quarter plus 32 entries; extras doubled plus 16) is sized so that a single row of a
+// million outgrowing it, which repeats the whole pass, stays rare.
 __global__ void k_caps_mixed(const int32_t* __restrict__ hint, const int32_t* __restrict__ rowcnt,
                              const unsigned int* __restrict__ efill, int64_t n, int mode, int64_t cap_u,
                              int64_t ecap_u, unsigned long long* __restrict__ gc, unsigned long long* __restrict__ ec)
@@ -1029,8 +1031,8 @@ __global__ void k_caps_mixed(const int32_t* __restrict__ hint, const int32_t* __
         } else if (mode == 1) {
             const unsigned long long a = hint[2 * i] > 0 ? (unsigned long long)hint[2 * i] : 0ull;
             const unsigned long long b = hint[2 * i + 1] > 0 ? (unsigned long long)hint[2 * i + 1] : 0ull;
-            g = a + a / 8 + 8;
-            e = b + b / 4 + 4;
+            g = a + a / 4 + 32;
+            e = 2 * b + 16;
         } else {
             const unsigned long long a = (unsigned long long)rowcnt[i], b = efill[i];
             g = a + a / 16 + 4;

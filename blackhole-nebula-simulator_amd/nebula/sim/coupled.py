@@ -255,6 +255,9 @@ def _hydro(st: GasState, cfg: CoupledConfig, v, dt_alpha: float, a_prev, h_start
                       float(hc.alpha_const), BALSARA_EPS, n, _hip.ptr(alpha))  # fmt: skip
         else:
             alpha = _full(n, hc.alpha_const)
+    if g is None and _learned(cfg) is not None:
+        # div v and |curl v| for the learned start's features
+        g = gradients_device(st.x, v, None, st.mass, d.h, nb, hc.kernel, precision=hc.precision)
     shock = hc.shock()
     gv = None
     if shock.reconstruction:
@@ -395,16 +398,17 @@ def step(st: GasState, cfg: CoupledConfig, dt: float) -> GasState:
     if model is not None:
         it = iter(moved[len(fields):])
         rho_p, div_p, curl_p, nn_p = (None if t is None else next(it) for t in extra)
-        stride = model.collect.stride if model.collect is not None else 1
         feat = model.features(h, rho_p, div_p, curl_p, nn_p, x1, u_pred, dt, st.dt_prev, stride=1)
         if hc.learned_h:
             h_start = model.h_start(h, feat)
             fallbacks = model.last_fallbacks
-        if model.collect is not None and stride > 1:
-            feat = feat[::stride].contiguous()
     new = _hydro(pred, cfg, velocity(cfg, x1, p_pred), dt, a_prev, h_start=h_start)
     if model is not None and model.collect is not None:
-        model.collect.add(feat, model.targets(h, new.h, model.collect.stride))
+        s = model.collect.stride
+        model.collect.add(feat[::s].contiguous() if s > 1 else feat, model.targets(h, new.h, s))
+    if model is not None and model.online is not None and hc.learned_h:
+        s = model.online.stride
+        model.observe(feat[::s].contiguous() if s > 1 else feat, model.targets(h, new.h, s))
     return replace(new, p=_axpy(p_flow, new.a_hydro, half), u=_axpy(u_half, new.dudt, half), e_radiated_dev=e_rad,
                    dt_prev=float(dt), learned_fallbacks_dev=fallbacks)
 

@@ -5,6 +5,10 @@ from the previous step's state:
 
     log h, log rho, div v dt, |curl v| dt, log(neighbour count), log r, log u, dt / dt_prev.
 
+With ``residual`` (the default) the network learns the correction to the continuity-equation
+predictor d log h = div v dt / 3 (h rho^(1/3) fixed, d log rho / dt = -div v): delta = div v dt / 3
++ MLP(features). An untrained or zero network then still starts from the physical predictor.
+
 The Newton iteration of the density pass then starts from h_prev exp(delta) instead of h_prev.
 It still runs to the same tolerance and stays the judge; a prediction that is not finite, or
 that changes h by a factor 2 or more, falls back to h_prev (counted). The search lists are
@@ -12,7 +16,10 @@ built from h_prev as before, so the lists are the plain method's.
 
 Inference is the BF16 MFMA kernel ``nb_mlp_infer``. Training uses this project's own FP32
 kernels (``nb_mlp_train_step``: forward, backward and Adam). The training pairs are logged by
-the simulator itself (``PairLog``, filled by ``coupled.step`` when ``collect`` is set).
+the simulator itself (``PairLog``, filled by ``coupled.step`` when ``collect`` is set). With
+``online`` set, the model keeps learning while it is used: ``coupled.step`` hands it every
+step's pairs (``observe``), which go into a ring buffer of the latest pairs, and every few steps
+a few Adam steps run on it, so the network follows the flow as it changes.
 """
 
 from __future__ import annotations
@@ -49,10 +56,22 @@ class PairLog:
         return torch.cat(self._feat), torch.cat(self._target)
 
 
+class OnlineSchedule:
+    """Online training while the model is used: the last ``capacity`` pairs are kept (pairs of
+    every ``stride``-th particle); every ``every`` steps, ``steps`` Adam steps of ``batch`` pairs
+    at learning rate ``lr``."""
+
+    def __init__(self, stride: int = 1, capacity: int = 131072, every: int = 4, steps: int = 8, batch: int = 4096,
+                 lr: float = 1e-3):  # fmt: skip
+        self.stride, self.capacity, self.every = int(stride), int(capacity), int(every)
+        self.steps, self.batch, self.lr = int(steps), int(batch), float(lr)
+
+
 class LearnedH:
     """The network, its normalisation and its Adam state (device tensors)."""
 
-    def __init__(self, seed: int = 0):
+    def __init__(self, seed: int = 0, residual: bool = True):
+        self.residual = bool(residual)
         n = param_count()
         rng = np.random.default_rng(seed)
         p = np.zeros(n, np.float32)
@@ -68,6 +87,37 @@ class LearnedH:
         self.step_count = 0
         self.collect: PairLog | None = None
         self.last_fallbacks = None  # device count of the last h_start call
+        self.online: OnlineSchedule | None = None
+        self._ring = None
+
+    def start_online(self, **kw) -> None:
+        """Keep training while the model is used (OnlineSchedule fields as keywords)."""
+        self.online = OnlineSchedule(**kw)
+        cap = self.online.capacity
+        self._ring = [_dev.empty((cap, N_FEATURES), torch.float32), _dev.empty(cap, torch.float32), 0, 0, 0]
+
+    def observe(self, feat: torch.Tensor, target: torch.Tensor) -> None:
+        """One step's (features, converged delta) pairs: into the ring buffer; every
+        ``online.every`` calls, ``online.steps`` Adam steps on random batches of it."""
+        rf, rt, pos, size, calls = self._ring
+        m = int(target.numel())
+        cap = int(rt.numel())
+        if m > cap:
+            feat, target, m = feat[:cap], target[:cap], cap
+        a = min(m, cap - pos)
+        rf[pos:pos + a] = feat[:a]
+        rt[pos:pos + a] = target[:a]
+        if a < m:
+            rf[: m - a] = feat[a:m]
+            rt[: m - a] = target[a:m]
+        pos = (pos + m) % cap
+        size = min(size + m, cap)
+        calls += 1
+        self._ring = [rf, rt, pos, size, calls]
+        o = self.online
+        if calls % o.every == 0:
+            self.train(rf[:size], rt[:size], steps=o.steps, batch=min(o.batch, size), lr=o.lr, seed=calls,
+                       record=False)  # fmt: skip
 
     # ------------------------------------------------------------------ data
     def features(self, h_prev, rho_prev, div_v, curl_v, nneigh, x, u, dt, dt_prev, stride: int = 1) -> torch.Tensor:
@@ -86,13 +136,23 @@ class LearnedH:
         _hip.call("nb_mlp_target", _hip.ptr(h_prev), _hip.ptr(h_final), n, int(stride), _hip.ptr(out))
         return out
 
+    def _base(self, feat: torch.Tensor) -> torch.Tensor | None:
+        """The continuity-equation predictor div v dt / 3 of each row (None without ``residual``)."""
+        return feat[:, 2] * (1.0 / 3.0) if self.residual else None
+
+    def net_target(self, feat: torch.Tensor, target: torch.Tensor) -> torch.Tensor:
+        """What the network itself is trained on: target minus the predictor."""
+        b = self._base(feat)
+        return target if b is None else (target - b).contiguous()
+
     # ------------------------------------------------------------------ inference
     def predict(self, feat: torch.Tensor) -> torch.Tensor:
-        """delta = log h_final - log h_prev for each feature row (BF16 MFMA)."""
+        """delta = log h_final - log h_prev for each feature row (network: BF16 MFMA)."""
         n = int(feat.shape[0])
         out = _dev.empty(n, torch.float32)
         _hip.call("nb_mlp_infer", _hip.ptr(self.params), _hip.ptr(self.norm), _hip.ptr(feat), n, _hip.ptr(out))
-        return out
+        b = self._base(feat)
+        return out if b is None else out + b
 
     def h_start(self, h_prev: torch.Tensor, feat: torch.Tensor, gate: float = 1e-4) -> torch.Tensor:
         """Starting h of the Newton solve: h_prev exp(delta), or h_prev where the predicted change
@@ -109,17 +169,18 @@ class LearnedH:
 
     # ------------------------------------------------------------------ training
     def fit_normalisation(self, feat: torch.Tensor, target: torch.Tensor) -> None:
-        """Feature means and 1/std, target mean and std, from a training set."""
+        """Feature means and 1/std, (network) target mean and std, from a training set."""
         f = feat.double()
         mean = f.mean(0)
         std = f.std(0).clamp_min(1e-12)
-        t = target.double()
+        t = self.net_target(feat, target).double()
         tstd = float(t.std().clamp_min(1e-12)) if t.numel() > 1 else 1.0
         vals = torch.cat([mean, 1.0 / std, torch.tensor([float(t.mean()), tstd], dtype=torch.float64, device=f.device)])
         self.norm = vals.float().contiguous()
 
     def grad(self, feat: torch.Tensor, target: torch.Tensor) -> tuple[float, torch.Tensor]:
         """Loss and FP32 gradient over the batch (no update)."""
+        target = self.net_target(feat, target)
         g = torch.empty_like(self.params)
         loss = _dev.empty(1, torch.float32)
         _hip.call("nb_mlp_grad", _hip.ptr(self.params), _hip.ptr(self.norm), _hip.ptr(feat.contiguous()),
@@ -127,8 +188,10 @@ class LearnedH:
         return float(loss.item()), g
 
     def train(self, feat: torch.Tensor, target: torch.Tensor, steps: int = 400, batch: int = 4096,
-              lr: float = 3e-3, seed: int = 0) -> list[float]:  # fmt: skip
-        """Adam on random minibatches; returns the loss of each step (normalised target)."""
+              lr: float = 3e-3, seed: int = 0, record: bool = True) -> list[float]:  # fmt: skip
+        """Adam on random minibatches; returns the loss of each step (normalised target), or
+        nothing without ``record`` (no host synchronisation)."""
+        target = self.net_target(feat, target)
         gen = torch.Generator(device=feat.device)
         gen.manual_seed(seed)
         n = int(target.numel())
@@ -143,17 +206,19 @@ class LearnedH:
             _hip.call("nb_mlp_train_step", _hip.ptr(self.params), _hip.ptr(self.adam_m), _hip.ptr(self.adam_v),
                       self.step_count, _hip.ptr(self.norm), _hip.ptr(fb), _hip.ptr(tb), batch, float(lr),
                       _hip.ptr(loss))  # fmt: skip
-            losses.append(float(loss.item()))
+            if record:
+                losses.append(float(loss.item()))
         return losses
 
     # ------------------------------------------------------------------ persistence
     def save(self, path) -> None:
-        np.savez(path, params=self.params.cpu().numpy(), norm=self.norm.cpu().numpy())
+        np.savez(path, params=self.params.cpu().numpy(), norm=self.norm.cpu().numpy(),
+                 residual=np.array(self.residual))  # fmt: skip
 
     @classmethod
     def load(cls, path) -> "LearnedH":
         d = np.load(path, allow_pickle=False)
-        m = cls()
+        m = cls(residual=bool(d["residual"]) if "residual" in d else True)
         m.params = torch.as_tensor(d["params"], device=m.params.device).contiguous()
         m.norm = torch.as_tensor(d["norm"], device=m.params.device).contiguous()
         return m

@@ -25,7 +25,7 @@ def _random_model(rng, seed=5):
 
     from oracle import mlp as omlp
 
-    m = LearnedH(seed=seed)
+    m = LearnedH(seed=seed, residual=False)  # the bare network, as the oracle computes it
     p = rng.normal(0.0, 0.25, omlp.N_PARAMS).astype(np.float32)
     norm = np.concatenate([rng.normal(0, 1, 8), rng.uniform(0.5, 2.0, 8), [0.01, 0.2]]).astype(np.float32)
     m.params = torch.as_tensor(p, device=m.params.device)
@@ -86,7 +86,7 @@ def test_training_reduces_loss(rng):
     target = (0.3 * np.tanh(feat[:, 0] - feat[:, 1]) + 0.2 * feat[:, 2] - 0.1 * feat[:, 5] * feat[:, 6]).astype(np.float32)
     dev = torch.device("cuda")
     f, t = torch.as_tensor(feat, device=dev), torch.as_tensor(target, device=dev)
-    m = LearnedH(seed=2)
+    m = LearnedH(seed=2, residual=False)
     m.fit_normalisation(f, t)
     print("MLP training: 20000 synthetic pairs, 300 Adam steps", flush=True)
     losses = m.train(f, t, steps=300, batch=2048, lr=3e-3)

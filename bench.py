@@ -51,6 +51,11 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--drift-steps", type=int, default=1000,
                     help="steps of the energy-ledger drift runs (both precisions, from the initial state); 0: skip")
+    ap.add_argument("--long-run", type=int, default=0,
+                    help="instead of the headline line: step cost and device memory in windows of 50 steps over "
+                         "this many steps from the initial state (the --precision mode, learned start as usual)")
+    ap.add_argument("--report", choices=["sphere262k", "stream300", "torus4m"], default=None,
+                    help="instead of the headline line: the error / Newton report of one of the other workloads")
     ap.add_argument("--ensemble", action="store_true",
                     help="the batched-ensemble workload (64 clouds) against sequential single-cloud runs")
     ap.add_argument("--ensemble-clouds", type=int, default=64)
@@ -225,6 +230,52 @@ def run_ensemble(args):
     print(json.dumps(out))
 
 
+def _err_stats(ref, got, kind):
+    """median, p99 and max of the per-particle error: relative ("rel"), absolute ("abs"), or per
+    vector over the rms size of ref ("rms")."""
+    import numpy as np
+
+    ref = np.asarray(ref, float)
+    got = np.asarray(got, float)
+    d = got - ref
+    if kind == "rel":
+        e = np.abs(d) / np.maximum(np.abs(ref), 1e-300)
+    elif kind == "abs":
+        e = np.abs(d)
+    else:
+        e = (np.linalg.norm(d, axis=1) if d.ndim == 2 else np.abs(d))
+        e = e / float(np.sqrt(np.mean(np.sum(ref.reshape(len(ref), -1) ** 2, axis=1))))
+    return {"median": float(np.median(e)), "p99": float(np.percentile(e, 99)), "max": float(e.max())}
+
+
+def parity_report(st, cfg_learned):
+    """Per-field errors of the mixed mode against the FP64 path (bit-identical to the FP64 oracle)
+    on the same state (one hydro pass each), and of the learned start against the plain start
+    (one step each from the same state)."""
+    from nebula import _dev
+    from nebula.sim import coupled
+
+    v = coupled.velocity(cfg_learned, st.x, st.p)
+    a = coupled._hydro(st, config("fp64"), v, 0.0, st.a_hydro)
+    b = coupled._hydro(st, config("mixed"), v, 0.0, st.a_hydro)
+    host = _dev.host
+    out = {"mixed_vs_fp64 (same state, one hydro pass)": {
+        "h": _err_stats(host(a.h), host(b.h), "rel"), "rho": _err_stats(host(a.rho), host(b.rho), "rel"),
+        "omega": _err_stats(host(a.omega), host(b.omega), "abs"),
+        "acc (per vector / rms|a|)": _err_stats(host(a.a_hydro), host(b.a_hydro), "rms"),
+        "dudt (/ rms)": _err_stats(host(a.dudt), host(b.dudt), "rms"),
+        "vsig_max": _err_stats(host(a.vsig_max), host(b.vsig_max), "rel")}}
+    del a, b
+    plain = replace(cfg_learned, hydro=replace(cfg_learned.hydro, learned_h=False, learned_model=None))
+    dt = coupled.stable_dt(st, plain)
+    p = coupled.step(st, plain, dt)
+    q = coupled.step(st, cfg_learned, dt)
+    out["learned_vs_plain_start (one step, same state)"] = {
+        "h": _err_stats(host(p.h), host(q.h), "rel"), "rho": _err_stats(host(p.rho), host(q.rho), "rel"),
+        "newton_iters": [float(p.iterations.double().mean()), float(q.iterations.double().mean())]}
+    return out
+
+
 def drift_run(ic, cfg, steps):
     """Energy and angular-momentum ledgers (coupled.py:261-270) over ``steps`` steps from the
     initial state: the maximum over the run of |E - E0| / sum m|H| and |L - L0| / |L0| (the
@@ -248,6 +299,186 @@ def drift_run(ic, cfg, steps):
             "e_radiated_over_abs_h": lg["e_radiated"] / l0["abs_h"], "wall_s": time.perf_counter() - t0}
 
 
+def _lockstep(ic, cfg_a, cfg_b, steps, every, compare, on_sample=None):
+    """Runs ``cfg_a`` (FP64) and ``cfg_b`` from the same initial state with the FP64 run's dt at
+    every step; every ``every`` steps calls compare(step, state_a, state_b)."""
+    from nebula.sim import coupled
+
+    a = coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg_a)
+    b = coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg_b)
+    out = [compare(0, a, b)]
+    iters_b = []
+    for k in range(1, steps + 1):
+        dt = coupled.stable_dt(a, cfg_a)
+        a = coupled.step(a, cfg_a, dt)
+        b = coupled.step(b, cfg_b, dt)
+        iters_b.append(float(b.iterations.double().mean().item()))
+        if k % every == 0 or k == steps:
+            out.append(compare(k, a, b))
+            print(f"bench: step {k}/{steps}", file=sys.stderr, flush=True)
+    return a, b, out, iters_b
+
+
+def _same_state(st, cfg_a, cfg_b):
+    """One hydro pass of each configuration on the same state: (rho, a_hydro) errors of b vs a."""
+    from nebula import _dev
+    from nebula.sim import coupled
+
+    v = coupled.velocity(cfg_a, st.x, st.p)
+    a = coupled._hydro(st, cfg_a, v, 0.0, st.a_hydro)
+    b = coupled._hydro(st, cfg_b, v, 0.0, st.a_hydro)
+    return {"rho": _err_stats(_dev.host(a.rho), _dev.host(b.rho), "rel"),
+            "acc": _err_stats(_dev.host(a.a_hydro), _dev.host(b.a_hydro), "rms"),
+            "dudt": _err_stats(_dev.host(a.dudt), _dev.host(b.dudt), "rms")}
+
+
+def run_report(args):
+    """Reports of the other workloads (printed as one JSON line):
+
+    - sphere262k: the plain-method cloud at 262,144 particles with power-law cooling, started at
+      r = 2100 on its parabolic orbit so that 500 steps span the pericentre (2000); FP64 and mixed
+      in lockstep (the FP64 run's dt), density and acceleration errors of mixed against FP64
+      (bit-identical to the FP64 oracle) every 100 steps, along the two trajectories and for one
+      hydro pass on the FP64 state.
+    - stream300: the 1M stream crash, 300 steps, FP64 and mixed + learned h in lockstep:
+      shock-heated mass fraction (T > 1e5 K), energy-ledger drift, final per-field errors.
+    - torus4m: the 4,194,304-particle torus with Shakura-Sunyaev viscosity (alpha 0.1, option A)
+      and power-law cooling in mixed mode: Newton iterations per particle with the plain and the
+      learned start (trained on 3 logged steps, then online), fallback rate, h / rho of the
+      learned start against the plain start (one step from the same state)."""
+    import numpy as np
+
+    from nebula import _dev
+    from nebula.ics import jittered_sphere_on_parabola, torus
+    from nebula.physics import units
+    from nebula.sim import coupled
+
+    rep = {"report": args.report}
+    if args.report == "sphere262k":
+        ic = jittered_sphere_on_parabola(262144, r0=2100.0)
+        c64 = replace(coupled.baseline_config(), cooling_powerlaw=True)
+        cm = replace(coupled.baseline_config(precision="mixed"), cooling_powerlaw=True)
+
+        def compare(k, a, b):
+            ha, hb = a.to_numpy(), b.to_numpy()
+            return {"step": k, "t": a.t, "r_centre": float(np.linalg.norm(ha["x"].mean(axis=0))),
+                    "trajectory": {"rho": _err_stats(ha["rho"], hb["rho"], "rel"),
+                                   "acc": _err_stats(ha["a_hydro"], hb["a_hydro"], "rms"),
+                                   "x (/ h)": _err_stats(np.zeros(len(ha["h"])),
+                                                         np.linalg.norm(hb["x"] - ha["x"], axis=1) / ha["h"], "abs")},
+                    "same_state": _same_state(a, c64, cm)}  # fmt: skip
+
+        a, b, rows, it = _lockstep(ic, c64, cm, 500, 100, compare)
+        la, lb = coupled.ledger(a, c64), coupled.ledger(b, cm)
+        rep.update({"workload": "sphere 262,144, r0 2100 -> pericentre 2000, power-law cooling, 500 steps",
+                    "samples": rows, "energy_ledger_end": [la["energy"], lb["energy"]], "abs_h": la["abs_h"]})
+    elif args.report == "stream300":
+        ic = workload(1 << 20)
+        c64 = config("fp64")
+        _, _, _, _, info = run_gpu(ic, config("mixed"), 1, 1, learned=True)
+        cl = info["_cfg"]
+        u_hot = units.gas_u_from_temperature(1e5)
+        l0 = {}
+
+        def compare(k, a, b):
+            ha, hb = a.to_numpy(), b.to_numpy()
+            la, lb = coupled.ledger(a, c64), coupled.ledger(b, cl)
+            if k == 0:
+                l0.update(a=la, b=lb)
+            m = ha["mass"] if "mass" in ha else np.asarray(_dev.host(a.mass))
+            return {"step": k, "t": a.t,
+                    "shock_heated_mass_fraction": [float(np.sum(m[ha["u"] > u_hot]) / np.sum(m)),
+                                                   float(np.sum(m[hb["u"] > u_hot]) / np.sum(m))],
+                    "energy_drift": [(la["energy"] - l0["a"]["energy"]) / l0["a"]["abs_h"],
+                                     (lb["energy"] - l0["b"]["energy"]) / l0["b"]["abs_h"]],
+                    "trajectory": {"rho": _err_stats(ha["rho"], hb["rho"], "rel"),
+                                   "acc": _err_stats(ha["a_hydro"], hb["a_hydro"], "rms"),
+                                   "u": _err_stats(ha["u"], hb["u"], "rel")}}  # fmt: skip
+
+        a, b, rows, it = _lockstep(ic, c64, cl, 300, 100, compare)
+        rep.update({"workload": "stream crash 1,048,576, 300 steps, [FP64, mixed + learned h]", "samples": rows,
+                    "newton_iters_mixed_learned": float(np.mean(it)), "hot": "T > 1e5 K"})  # fmt: skip
+    else:
+        n = args.n if args.n != 1_048_576 else 4_194_304
+        print(f"bench: torus of {n} particles", file=sys.stderr, flush=True)
+        ic = torus(n)
+        base = replace(coupled.baseline_config(precision="mixed", ns_alpha=0.1), cooling_powerlaw=True)
+        # The density solve (sph.py:251-297) tests convergence against tol * h_start: from a
+        # random start, a first Newton step that lands near h = 0 "converges" there (the FP64
+        # oracle does the same for these rows), and the next step cannot grow h back within 12
+        # rebuilds. Such rows (h below 1 % of their start) get a start 1.3x larger, until none.
+        relaxed = 0
+        for _ in range(8):
+            h_got = coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, base).to_numpy()["h"]
+            bad = h_got < 0.01 * ic.h
+            if not bad.any():
+                break
+            relaxed += int(bad.sum())
+            ic = replace(ic, h=np.where(bad, 1.3 * ic.h, ic.h))
+        rep["collapsed_starts_moved"] = relaxed
+        _, _, _, _, info = run_gpu(ic, base, 1, 1, learned=True)
+        cl = info["_cfg"]
+        steps = max(args.steps, 20)
+        res = {}
+        for name, c in (("plain", base), ("learned", cl)):
+            st = coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, c)
+            its, fb = [], 0
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                st = coupled.step(st, c, coupled.stable_dt(st, c))
+                its.append(float(st.iterations.double().mean().item()))
+                fb += st.learned_fallbacks
+            res[name] = {"newton_iters": float(np.mean(its[1:])), "fallback_rate": fb / (steps * n),
+                         "ms_per_step": (time.perf_counter() - t0) / steps * 1e3}
+            if name == "plain":
+                keep = st
+        dt = coupled.stable_dt(keep, base)
+        p = coupled.step(keep, base, dt)
+        q = coupled.step(keep, cl, dt)
+        rep.update({"workload": f"torus {n}: r_in 12, r_max 25, constant l, polytrope, 2% velocity noise, "
+                                f"Shakura-Sunyaev alpha 0.1 (option A), power-law cooling; {steps} steps",
+                    "runs": res, "learned_vs_plain (one step, same state)": {
+                        "h": _err_stats(_dev.host(p.h), _dev.host(q.h), "rel"),
+                        "rho": _err_stats(_dev.host(p.rho), _dev.host(q.rho), "rel")}})  # fmt: skip
+    print(json.dumps(rep))
+
+
+def run_long(args):
+    """Step cost and memory over a long run of the headline workload: windows of 50 steps."""
+    import torch
+
+    from nebula.sim import coupled
+
+    ic = workload(args.n)
+    cfg = config(args.precision)
+    learned = args.precision == "mixed" and not args.no_learned
+    if learned:  # the headline's learned start: trained on 3 logged steps, then online
+        _, _, _, _, info = run_gpu(ic, cfg, 1, 1, learned=True)
+        cfg = info["_cfg"]
+    st = coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg)
+    windows = []
+    done = 0
+    while done < args.long_run:
+        w = min(50, args.long_run - done)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        iters = 0.0
+        for _ in range(w):
+            st = coupled.step(st, cfg, coupled.stable_dt(st, cfg))
+            iters += float(st.iterations.double().mean().item())
+        torch.cuda.synchronize()
+        done += w
+        free, total = torch.cuda.mem_get_info()
+        rec = {"steps": done, "t": round(st.t, 3), "ms_per_step": round((time.perf_counter() - t0) / w * 1e3, 3),
+               "newton_iters": round(iters / w, 4), "search_entries_per_particle": round(st.n_search_entries / args.n, 1),
+               "torch_allocated_GB": round(torch.cuda.memory_allocated() / 1e9, 2),
+               "device_used_GB": round((total - free) / 1e9, 2)}  # fmt: skip
+        windows.append(rec)
+        print(f"bench: {json.dumps(rec)}", file=sys.stderr, flush=True)
+    print(json.dumps({"workload": f"stream crash, {args.n} particles", "precision": args.precision,
+                      "learned_h": learned, "windows": windows}))
+
+
 def _per_step(d, steps):
     return {k: round(v[1] / steps, 4) for k, v in sorted(d.items(), key=lambda kv: -kv[1][1])}
 
@@ -263,6 +494,12 @@ def main():
     dev = _hip.torch_device()
     if args.ensemble:
         run_ensemble(args)
+        return
+    if args.long_run > 0:
+        run_long(args)
+        return
+    if args.report:
+        run_report(args)
         return
     if args.copy_calibration:
         import torch
@@ -341,6 +578,9 @@ def main():
         "learned_h": {k: v for k, v in info.items() if not k.startswith("_")} if learned else None,
         "newton_iters": info["newton_iters"],
     }
+    if learned:
+        print("bench: per-field parity (mixed vs FP64, learned vs plain start)", file=sys.stderr, flush=True)
+        out["parity"] = parity_report(st, info["_cfg"])
     if not args.no_compare:
         other = "fp64" if args.precision == "mixed" else "mixed"
         del st

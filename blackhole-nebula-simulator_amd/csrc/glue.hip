@@ -92,7 +92,7 @@ __global__ void k_ns_glue(const double* __restrict__ g, const double* __restrict
     double nu;
     if (alpha_ss > 0.0) {
         double r = sqrt(x[3 * i] * x[3 * i] + x[3 * i + 1] * x[3 * i + 1] + x[3 * i + 2] * x[3 * i + 2]);
-        nu = alpha_ss * (cs[i] * cs[i]) * pow(r, 1.5);
+        nu = alpha_ss * (cs[i] * cs[i]) * pow15(r);
     } else {
         nu = nu_const;
     }

@@ -16,6 +16,17 @@ namespace nb {
 
 __device__ __forceinline__ double ipow3(double x) { return x * (x * x); }
 __device__ __forceinline__ double ipow4(double x) { double x2 = x * x; return x2 * x2; }
+// r^1.5 (r >= 0) rounded once, as a correctly rounded pow(r, 1.5) gives it: sqrt(r) = s + sl and
+// r s = p + e carried exactly with FMA residuals, then p + (e + r sl) rounds the ~106-bit value.
+__device__ __forceinline__ double pow15(double r)
+{
+    if (!(r > 0.0)) return r == 0.0 ? 0.0 : sqrt(r);  // 0, or NaN for r < 0 / NaN
+    const double s = sqrt(r);
+    const double sl = fma(-s, s, r) / (2.0 * s);
+    const double p = r * s;
+    const double e = fma(r, s, -p);
+    return p + (e + r * sl);
+}
 __device__ __forceinline__ double ipow5(double x) { double x2 = x * x; return x * (x2 * x2); }
 
 // Python's builtin min/max of two floats (second argument only if strictly smaller/larger)

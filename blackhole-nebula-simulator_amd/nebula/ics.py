@@ -8,6 +8,9 @@
   noise, meeting at a point at radius r_meet with a given opening angle between their
   directions of motion, each at a given speed.
 
+- ``torus``: a thick constant-angular-momentum torus in hydrostatic equilibrium in the GN
+  potential, polytropic (gamma 5/3), with seeded velocity noise.
+
 Particles are returned in the order of a cubic cell grid of about one smoothing length
 (raster order, as the neighbour search traverses cells), which keeps each neighbour list
 nearly sorted and the pair loops' gathers local.
@@ -130,3 +133,78 @@ def stream_crash(n: int = 1_048_576, radius: float = 20.0, length: float = 2000.
     order = raster_order(x, 1.4 * h0)
     return IC(x=x[order], v=v[order], u=np.full(n, units.gas_u_from_temperature(temperature)), mass=mass[order],
               h=np.full(n, h0))  # fmt: skip
+
+
+def _torus_phi_eff(R, z, ell2):
+    """Effective potential of azimuthal motion with specific angular momentum ell in the GN
+    Hamiltonian: H = A |p|^2 / 2 - 1/r with p . x = 0, |p| = ell / R, A = (r - 2) / r."""
+    r = np.sqrt(R * R + z * z)
+    return 0.5 * (r - 2.0) / r * ell2 / (R * R) - 1.0 / r
+
+
+def torus(n: int = 4_194_304, r_in: float = 12.0, r_max: float = 25.0, mass_msun: float = 0.05,
+          noise: float = 0.02, gamma: float = 5.0 / 3.0, eta: float = 1.2, seed: int = 20261014) -> IC:  # fmt: skip
+    """Thick torus of constant specific angular momentum ell, ell^2 = r_max^2 / (r_max - 3) (the
+    circular orbit at r_max, where the effective potential of the GN Hamiltonian has its
+    minimum in the plane), inner edge r_in. Hydrostatic polytrope P = K rho^gamma: the enthalpy
+    w = gamma / (gamma - 1) K rho^(gamma - 1) = Phi_eff(r_in, 0) - Phi_eff(R, z) inside, so the
+    specific energy u = w / gamma follows from the geometry alone (K scales the density only;
+    K is fixed by the torus mass). Equal-mass particles sampled with probability proportional
+    to rho (seeded rejection sampling in the bounding cylinder); velocity = dH/dp of
+    p = ell / R along phi, plus Gaussian noise of ``noise`` times its size per component;
+    starting h from the distance to the particle's 58th neighbour."""
+    rng = np.random.default_rng(seed)
+    ell2 = r_max * r_max / (r_max - 3.0)
+    phi_in = _torus_phi_eff(r_in, 0.0, ell2)
+    w_max = phi_in - _torus_phi_eff(r_max, 0.0, ell2)
+    # outer edge in the plane and half height at r_max (w = 0)
+    Rg = np.linspace(r_max, 50.0 * r_max, 200001)
+    r_out = float(Rg[np.argmax(phi_in - _torus_phi_eff(Rg, 0.0, ell2) <= 0.0)])
+    zg = np.linspace(0.0, r_out, 200001)
+    z_top = float(zg[np.argmax(phi_in - _torus_phi_eff(r_max, zg, ell2) <= 0.0)])
+    z_top = max(z_top, float(np.max(np.where(phi_in - _torus_phi_eff(np.linspace(r_in, r_out, 401)[:, None],
+                                                                         zg[None, ::500], ell2) > 0,
+                                                zg[None, ::500], 0.0))))  # fmt: skip
+    xs = []
+    got = 0
+    while got < n:
+        m = 4 * (n - got) + 1024
+        R = np.sqrt(rng.uniform(r_in * r_in, r_out * r_out, m))  # uniform in volume
+        z = rng.uniform(-z_top, z_top, m)
+        w = phi_in - _torus_phi_eff(R, z, ell2)
+        keep = (w > 0.0) & (rng.uniform(0.0, 1.0, m) < (np.maximum(w, 0.0) / w_max) ** (1.0 / (gamma - 1.0)))
+        ph = rng.uniform(0.0, 2.0 * np.pi, int(keep.sum()))
+        xs.append(np.stack([R[keep] * np.cos(ph), R[keep] * np.sin(ph), z[keep]], axis=1))
+        got += int(keep.sum())
+    x = np.concatenate(xs)[:n]
+    R = np.hypot(x[:, 0], x[:, 1])
+    r = np.linalg.norm(x, axis=1)
+    w = phi_in - _torus_phi_eff(R, x[:, 2], ell2)
+    shape = (w / w_max) ** (1.0 / (gamma - 1.0))  # rho / rho_max
+    # rho_max from the mass: M = rho_max * integral of shape dV (Monte Carlo over the cylinder)
+    mc = 4_000_000
+    Rm = np.sqrt(rng.uniform(r_in * r_in, r_out * r_out, mc))
+    zm = rng.uniform(-z_top, z_top, mc)
+    wm = np.maximum(phi_in - _torus_phi_eff(Rm, zm, ell2), 0.0)
+    vol = np.pi * (r_out * r_out - r_in * r_in) * 2.0 * z_top
+    integral = vol * float(np.mean((wm / w_max) ** (1.0 / (gamma - 1.0))))
+    total = units.msun_to_code_mass(mass_msun)
+    rho = total / integral * shape
+    m = total / n
+    # p along phi with |p| = ell / R; v = A p (p . x = 0), A = (r - 2) / r
+    ell = np.sqrt(ell2)
+    phat = np.stack([-x[:, 1] / R, x[:, 0] / R, np.zeros(n)], axis=1)
+    v = ((r - 2.0) / r * ell / R)[:, None] * phat
+    v = v + noise * np.linalg.norm(v, axis=1, keepdims=True) * rng.standard_normal((n, 3))
+    u = w / gamma
+    # h from the sampled particles themselves (half the distance to the 58th neighbour: the
+    # cubic kernel's support holds about 58 neighbours at eta 1.2), not from the analytic
+    # density: random sampling makes the local density differ from it, and a start far from
+    # the consistent h can send the first Newton step of the density solve towards h = 0
+    from scipy.spatial import cKDTree
+
+    k = int(round(4.0 / 3.0 * np.pi * (2.0 * eta) ** 3))
+    dist, _ = cKDTree(x).query(x, k=[min(k, n)], workers=-1)
+    h = np.minimum(0.5 * dist[:, 0], eta * (m / np.maximum(rho, 1e-300)) ** (1.0 / 3.0) * 4.0)
+    order = raster_order(x, 1.4 * float(np.median(h)))
+    return IC(x=x[order], v=v[order], u=u[order], mass=np.full(n, m), h=h[order])

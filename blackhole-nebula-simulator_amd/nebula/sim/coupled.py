@@ -271,12 +271,12 @@ def _hydro(st: GasState, cfg: CoupledConfig, v, dt_alpha: float, a_prev, h_start
     if (hc.ns_nu > 0) + (hc.ns_alpha > 0) + (hc.disc_alpha_av > 0) > 1:
         raise ValueError("set only one of ns_nu, ns_alpha, disc_alpha_av")
     if hc.ns_nu > 0 or hc.ns_alpha > 0:
-        g = torch.zeros((n, 3, 3), dtype=torch.float64, device=st.x.device)
-        ns_gradients_device(st.x, v, st.mass, d.h, d.rho, d.omega, nb, hc.kernel, g)
+        gns = torch.zeros((n, 3, 3), dtype=torch.float64, device=st.x.device)
+        ns_gradients_device(st.x, v, st.mass, d.h, d.rho, d.omega, nb, hc.kernel, gns)
         nu = _dev.empty(n)
         stress = _dev.empty((n, 3, 3))
         heat = _dev.empty(n)
-        _hip.call("nb_ns_stress", _hip.ptr(g), _hip.ptr(d.rho), _hip.ptr(cs), _hip.ptr(st.x), float(hc.ns_alpha),
+        _hip.call("nb_ns_stress", _hip.ptr(gns), _hip.ptr(d.rho), _hip.ptr(cs), _hip.ptr(st.x), float(hc.ns_alpha),
                   float(hc.ns_nu), n, _hip.ptr(nu), _hip.ptr(stress), _hip.ptr(heat))  # fmt: skip
         a_ns = ns_forces_device(st.x, st.mass, d.h, d.rho, d.omega, stress, nb, hc.kernel)
         _hip.call("nb_add", _hip.ptr(acc), _hip.ptr(a_ns), 3 * n, _hip.ptr(acc))

@@ -130,3 +130,48 @@ def test_blocked_lists_grid_order_and_tighten(rng):
     blk2 = nbm.build_lists_blocked(x_d, h_d, None).to_csr()
     bit_equal("blocked lists (caller order): indices", ref2.index, blk2.index)
     del torch
+
+
+def test_torus_shakura_sunyaev_cooling():
+    """The torus workload's physics: Shakura-Sunyaev viscosity (alpha 0.1, stress form, which
+    adds the N-S gradient and force passes) with power-law cooling, against the oracle; then two
+    steps of the same in mixed mode with the learned start (finite).
+
+    nu = alpha c_s^2 r^1.5 takes a non-integer power. The GPU rounds r^1.5 once (correctly
+    rounded); the reference's NumPy pow (glibc, within 0.52 ulp) differs from that by one ulp for
+    about 1 value in 1000, so this comparison is at 1e-12 relative (the FP64 mode's bound), not
+    bitwise: x, p, u, h, rho per value (acc and du/dt per vector over their rms)."""
+    from dataclasses import replace
+
+    from nebula.ics import torus
+    from nebula.sim import coupled
+    from nebula.sim.learned_h import LearnedH
+
+    ic = torus(4096, seed=5)
+    cfg = replace(coupled.baseline_config(ns_alpha=0.1), cooling_powerlaw=True)
+    print("coupled: 4096-particle torus, Shakura-Sunyaev alpha 0.1 + power-law cooling, 3 steps", flush=True)
+    from oracle import coupled as oc
+
+    st = coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg)
+    ost = oc.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg)
+    for k in range(3):
+        dt, odt = coupled.stable_dt(st, cfg), oc.stable_dt(ost, cfg)
+        check(f"step {k}: stable_dt (1e-12 relative)", odt, dt, rtol=1e-12)
+        st = coupled.step(st, cfg, odt)
+        ost = oc.step(ost, cfg, odt)
+    got = st.to_numpy()
+    for k in ("x", "p", "u", "h", "rho", "nu"):
+        check(f"state.{k} (1e-12 relative)", ost.caller_order(k), got[k], rtol=1e-12, per_vector=k in ("x", "p"))
+    for k in ("a_hydro", "dudt"):
+        ref = ost.caller_order(k)
+        rms = float(np.sqrt(np.mean(np.sum(ref.reshape(len(ref), -1) ** 2, axis=1))))
+        check(f"state.{k} (1e-12 of rms)", ref, got[k], atol=1e-12 * rms, per_vector=k == "a_hydro")
+    model = LearnedH(seed=1)
+    cm = replace(cfg, hydro=replace(cfg.hydro, precision="mixed", learned_h=True, learned_model=model))
+    print("coupled: the same torus, mixed + learned start, 2 steps", flush=True)
+    sm = coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cm)
+    for _ in range(2):
+        sm = coupled.step(sm, cm, coupled.stable_dt(sm, cm))
+    got = sm.to_numpy()
+    check("mixed torus: all fields finite", True,
+          all(bool(np.all(np.isfinite(got[k]))) for k in ("x", "p", "u", "h", "rho", "a_hydro", "dudt", "nu")))

@@ -188,7 +188,21 @@ def run_ensemble(args):
     wall = time.perf_counter() - t0
     batched = total * args.steps / wall
     print(f"bench: batched {batched:.4g} particle-steps/s", file=sys.stderr, flush=True)
-    del ens
+    # the same ensemble step recorded as one hipGraph and replayed
+    from nebula.sim.captured import CapturedEnsemble
+
+    cap = CapturedEnsemble(ens)
+    cap.step()  # first replay: untimed
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        cap.step()
+    torch.cuda.synchronize()
+    wall_c = time.perf_counter() - t0
+    captured = {"value": total * args.steps / wall_c, "ms_per_step": wall_c / args.steps * 1e3,
+                "fallbacks": cap.fallbacks, "graph_nodes": cap.node_counts(), "capacities": cap.capacities}
+    print(f"bench: captured {captured['value']:.4g} particle-steps/s", file=sys.stderr, flush=True)
+    del cap, ens
     # sequential: each cloud alone (same tuned settings), timed over a few steps after one warm-up
     seq_steps = max(2, min(5, args.steps))
     per_cloud_ms = []
@@ -225,6 +239,7 @@ def run_ensemble(args):
                    "cell_factor": choice.cell_factor, "tile": choice.tile},
         "sequential": {"value": sequential, "ms_per_ensemble_step": seq_ms, "steps_per_cloud": seq_steps,
                        "speedup_batched": batched / sequential},
+        "captured": captured,
         "autotune_ms_per_hydro_pass": {f"{f}x{t}": round(v, 3) for (f, t), v in choice.table.items()},
     }
     print(json.dumps(out))

@@ -945,6 +945,20 @@ k_tight_emit(const uint32_t* __restrict__ gbuf, const double* __restrict__ dbuf,
 // others are solved in full by k_solve_slow_mixed.
 static constexpr int kG = 16;
 static constexpr int kCols = 2 * kG;  // columns per chunk: two per lane
+static constexpr int kTab = 256;      // candidate slots per lane group in LDS (multiple of kG)
+
+// Status words (int32, device) of a step recorded for replay: what did not fit, or what the
+// caller must redo eagerly. Words 6 and 7 are written by nb_glf_step (failures) and
+// nb_eos_ideal (u <= 0) when given status + 6 / status + 7.
+enum : int { kStGather = 0, kStExtras = 1, kStIndex = 2, kStCells = 3, kStGrow = 4, kStUnsorted = 5 };
+
+// Orders the LDS accesses of one wavefront (lanes read what other lanes of the wave wrote).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 __device__ __forceinline__ void kernel_fdf32(float q, int kid, float& fq, float& dfq)
 {
@@ -1081,7 +1095,7 @@ __device__ __forceinline__ bool column_zrange(double ux, double uy, double zc, d
 // independent systems; particle i searches only the cells of its own system cloud[i], whose grid
 // and first cell come from segs.
 template <bool kMasses, bool kSeg>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 5)
 k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const GridSeg* __restrict__ segs,
                const int32_t* __restrict__ cloud, const uint32_t* __restrict__ start,
                const uint32_t* __restrict__ order, uint32_t* __restrict__ ridx, float* __restrict__ rd2,
@@ -1118,6 +1132,8 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const Gr
     uint32_t* row = ridx + gb0;
     float* drow = rd2 + gb0;
     float* mrow = kMasses ? rmass + gb0 : nullptr;
+    __shared__ uint32_t ctab[256 / kG][kTab];  // one candidate table per lane group
+    uint32_t* tab = ctab[threadIdx.x / kG];
     // every loop below runs the same number of times on all lanes of the wave (ballot bounds),
     // so the shuffles always see their whole group active
     for (int64_t c0 = 0;; c0 += kCols) {
@@ -1146,25 +1162,36 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const Gr
         const uint32_t tota = __shfl(ia, kG - 1, kG);
         const uint32_t total = tota + __shfl(ib, kG - 1, kG);
         const uint32_t pa = ia - lena, pb = tota + ib - lenb;  // exclusive prefixes of both columns
+        // the chunk's first kTab candidates by position: each lane lists its two columns' runs
+        wave_sync();  // the previous chunk's reads are done
+        for (uint32_t t = 0; t < lena && pa + t < (uint32_t)kTab; t++) tab[pa + t] = k0a + t;
+        for (uint32_t t = 0; t < lenb && pb + t < (uint32_t)kTab; t++) tab[pb + t] = k0b + t;
+        wave_sync();
         for (uint32_t vb = 0;; vb += kG) {
             if (__ballot(vb < total) == 0) break;
             const uint32_t v = vb + (uint32_t)glane;
-            const uint32_t vs = total > 0 ? (v < total ? v : total - 1) : 0;
-            // the chunk column holding candidate vs: the last one whose prefix is <= vs
-            int lo = 0;
-            for (int step = kCols / 2; step > 0; step >>= 1) {
-                const int mid = lo + step;
-                const uint32_t qa = __shfl(pa, mid & (kG - 1), kG), qb = __shfl(pb, mid & (kG - 1), kG);
-                if ((mid < kG ? qa : qb) <= vs) lo = mid;
+            uint32_t kc = 0;
+            if (vb < (uint32_t)kTab) {  // the same on every lane of the group
+                kc = v < total ? tab[v] : 0u;
+            } else {
+                const uint32_t vs = v < total ? v : total - 1;
+                // the chunk column holding candidate vs: the last one whose prefix is <= vs
+                int lo = 0;
+                for (int step = kCols / 2; step > 0; step >>= 1) {
+                    const int mid = lo + step;
+                    const uint32_t qa = __shfl(pa, mid & (kG - 1), kG), qb = __shfl(pb, mid & (kG - 1), kG);
+                    if ((mid < kG ? qa : qb) <= vs) lo = mid;
+                }
+                const uint32_t qa = __shfl(pa, lo & (kG - 1), kG), qb = __shfl(pb, lo & (kG - 1), kG);
+                const uint32_t ka = __shfl(k0a, lo & (kG - 1), kG), kb = __shfl(k0b, lo & (kG - 1), kG);
+                kc = (lo < kG ? ka : kb) + (vs - (lo < kG ? qa : qb));
             }
-            const uint32_t qa = __shfl(pa, lo & (kG - 1), kG), qb = __shfl(pb, lo & (kG - 1), kG);
-            const uint32_t ka = __shfl(k0a, lo & (kG - 1), kG), kb = __shfl(k0b, lo & (kG - 1), kG);
             bool pred = false, flag = false;
             uint32_t j = 0;
             double d2 = 0.0;
             float mj = 0.f;
             if (v < total) {
-                const uint32_t k = (lo < kG ? ka : kb) + (v - (lo < kG ? qa : qb));
+                const uint32_t k = kc;
                 j = order ? order[k] : k;
                 const double4 pj = posh[j];
                 if (kMasses) mj = (float)A.mass[j];
@@ -1256,7 +1283,8 @@ k_solve_slow_mixed(const double* __restrict__ h_search, int64_t n, const float* 
     const int glane = threadIdx.x & (kG - 1);
     const int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kG;
     if (i >= n || !A.slow[i]) return;  // whole groups leave together
-    const int32_t m = rowcnt[i];
+    // the stored row (shorter than rowcnt only after an overflow the caller is told about)
+    const int32_t m = (int32_t)min((unsigned long long)rowcnt[i], goff[i + 1] - goff[i]);
     const float* drow = rd2 + goff[i];
     const float* mrow = kMasses ? rmass + goff[i] : nullptr;
     const float mu = (float)(A.m_uniform > 0.0 ? A.m_uniform : 1.0);
@@ -1315,13 +1343,16 @@ k_solve_slow_mixed(const double* __restrict__ h_search, int64_t n, const float* 
 // search-list entries). Runs after a pass without overflow: the counts are the stored entries.
 __global__ void __launch_bounds__(kBlock)
 k_row_total_rm(const int32_t* __restrict__ rowcnt, const unsigned int* __restrict__ efill,
+               const unsigned long long* __restrict__ goff, const unsigned long long* __restrict__ eoff,
                unsigned long long* __restrict__ mc, int32_t* __restrict__ row_counts, long long* __restrict__ part,
                int64_t n)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     long long t = 0;
     if (i < n) {
-        t = (long long)rowcnt[i] + (long long)efill[i];
+        // the stored entries (the counts exceed them only after an overflow the caller is told about)
+        t = (long long)min((unsigned long long)rowcnt[i], goff[i + 1] - goff[i]) +
+            (long long)min((unsigned long long)efill[i], eoff[i + 1] - eoff[i]);
         mc[i] = (unsigned long long)t;
         if (row_counts) {
             row_counts[2 * i] = rowcnt[i];
@@ -1354,7 +1385,7 @@ k_sort_extras(const uint32_t* __restrict__ ebuf, const float* __restrict__ ed2, 
     const float* d = ed2 + b0;
     uint32_t* eo = es + b0;
     float* dout = eds + b0;
-    const int64_t ne = efill[i];
+    const int64_t ne = (int64_t)min((unsigned long long)efill[i], eoff[i + 1] - b0);  // the stored entries
     if (ne == 0) return;
     if (ne <= kG * kSortRegs) {
         uint32_t v[kSortRegs];
@@ -1419,7 +1450,8 @@ k_tight_emit_rm(const double* __restrict__ pos, const uint32_t* __restrict__ rid
                 const unsigned int* __restrict__ efill, const unsigned long long* __restrict__ eoff,
                 const uint32_t* __restrict__ ebuf, const float* __restrict__ ed2, const double* __restrict__ hf,
                 int64_t n, const unsigned long long* __restrict__ toff, int64_t* __restrict__ out_off,
-                int32_t* __restrict__ out_len, int32_t* __restrict__ out_idx, FinArgs F, long long* __restrict__ part)
+                int32_t* __restrict__ out_len, int32_t* __restrict__ out_idx, unsigned long long idx_cap,
+                int32_t* __restrict__ status, FinArgs F, long long* __restrict__ part)
 {
     const int glane = threadIdx.x & (kG - 1);
     const int gshift = (threadIdx.x & 63) & ~(kG - 1);
@@ -1427,9 +1459,10 @@ k_tight_emit_rm(const double* __restrict__ pos, const uint32_t* __restrict__ rid
     const bool live = i < n;
     const int64_t ii = live ? i : n - 1;
     const double hi = hf[ii];
-    const int32_t ng = live ? rowcnt[ii] : 0;
     const unsigned long long g0 = goff[ii], e0 = eoff[ii], t0 = toff[ii];
-    const int64_t ne = live ? (int64_t)efill[ii] : 0;
+    // the stored entries (the counts exceed them only after an overflow the caller is told about)
+    const int32_t ng = live ? (int32_t)min((unsigned long long)rowcnt[ii], goff[ii + 1] - g0) : 0;
+    const int64_t ne = live ? (int64_t)min((unsigned long long)efill[ii], eoff[ii + 1] - e0) : 0;
     const int64_t len = ng + ne;
     const bool fin = live && !F.slow[ii];
     const float inv_hf = (float)(1.0 / hi);
@@ -1466,13 +1499,18 @@ k_tight_emit_rm(const double* __restrict__ pos, const uint32_t* __restrict__ rid
         }
         const unsigned long long bal = __ballot(pred);
         const unsigned gb = (unsigned)((bal >> gshift) & ((1ull << kG) - 1));
-        if (pred) out_idx[t0 + o + __popc(gb & ((1u << glane) - 1))] = (int32_t)j;
+        if (pred) {
+            const unsigned long long at = t0 + o + __popc(gb & ((1u << glane) - 1));
+            if (at < idx_cap) out_idx[at] = (int32_t)j;
+            else if (status) status[kStIndex] = 1;
+        }
         o += __popc(gb);
     }
     double s1, s2;
     sums.total(s1, s2);
     if (live && glane == 0) {
-        out_len[i] = o;
+        // a row cut by the index capacity (reported in status) keeps what was written
+        out_len[i] = t0 + (unsigned long long)o <= idx_cap ? o : (t0 < idx_cap ? (int32_t)(idx_cap - t0) : 0);
         out_off[i] = (int64_t)t0;
         if (i == n - 1) out_off[n] = (int64_t)toff[n];
         if (fin) {
@@ -1488,6 +1526,222 @@ k_tight_emit_rm(const double* __restrict__ pos, const uint32_t* __restrict__ rid
 }
 
 static inline unsigned grid_of(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+// ------------------------------------------------------------------ fixed-capacity passes
+__global__ void k_fill_i64(long long* __restrict__ p, long long v, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Graph-safe device-wide exclusive scan and stable radix sort (kernels only). Library sorts and
+// scans of large inputs clear their state with memsets, which misbehave in replayed graphs.
+static constexpr int kScanThreads = 256;
+static constexpr int kScanItems = 4;
+static constexpr int kScanTile = kScanThreads * kScanItems;
+
+// exclusive scan of v over the workgroup (kScanThreads lanes); total returned to every lane
+template <class T>
+__device__ __forceinline__ T block_exclusive(T v, T& total)
+{
+    __shared__ T wsum[kScanThreads / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    T x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const T t = __shfl_up(x, o, 64);
+        if (lane >= o) x += t;
+    }
+    __syncthreads();
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    T before = 0, all = 0;
+    for (int k = 0; k < kScanThreads / 64; k++) {
+        if (k < w) before += wsum[k];
+        all += wsum[k];
+    }
+    total = all;
+    return before + x - v;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kScanThreads) k_scan_tiles(const T* __restrict__ in, int64_t count, T* __restrict__ sums)
+{
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    T v = 0;
+    for (int k = 0; k < kScanItems; k++)
+        if (base + k < count) v += in[base + k];
+    T total;
+    block_exclusive(v, total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// exclusive scan of the tile sums in place, one workgroup
+template <class T>
+__global__ void __launch_bounds__(kScanThreads) k_scan_sums(T* __restrict__ sums, int64_t ntiles)
+{
+    T carry = 0;
+    for (int64_t c = 0; c < ntiles; c += kScanThreads) {
+        const int64_t i = c + threadIdx.x;
+        const T v = i < ntiles ? sums[i] : T(0);
+        T total;
+        const T ex = block_exclusive(v, total);
+        if (i < ntiles) sums[i] = carry + ex;
+        carry += total;
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kScanThreads)
+k_scan_apply(const T* __restrict__ in, int64_t count, const T* __restrict__ sums, T* __restrict__ out)
+{
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    T v[kScanItems];
+    T local = 0;
+    for (int k = 0; k < kScanItems; k++) {
+        v[k] = base + k < count ? in[base + k] : T(0);
+        local += v[k];
+    }
+    T total;
+    T run = sums[blockIdx.x] + block_exclusive(local, total);
+    for (int k = 0; k < kScanItems; k++) {
+        if (base + k < count) out[base + k] = run;
+        run += v[k];
+    }
+}
+
+static constexpr int kSortThreads = 256;
+static constexpr int kRadixBits = 4;
+static constexpr int kRadix = 1 << kRadixBits;
+
+// digit counts of each workgroup's keys: hist[d * nblocks + block]
+__global__ void __launch_bounds__(kSortThreads)
+k_radix_hist(const uint32_t* __restrict__ keys, int64_t n, int shift, uint32_t* __restrict__ hist)
+{
+    __shared__ uint32_t cnt[kRadix];
+    if (threadIdx.x < kRadix) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * kSortThreads + threadIdx.x;
+    if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & (kRadix - 1)], 1u);
+    __syncthreads();
+    if (threadIdx.x < kRadix) hist[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// stable scatter by digit: position = start of (digit, block) + rank among the workgroup's
+// earlier keys with the same digit (waves in order, lanes in order by ballot)
+__global__ void __launch_bounds__(kSortThreads)
+k_radix_scatter(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, int64_t n, int shift,
+                const uint32_t* __restrict__ offs, uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out)
+{
+    __shared__ uint32_t wcnt[kSortThreads / 64][kRadix];
+    const int64_t i = (int64_t)blockIdx.x * kSortThreads + threadIdx.x;
+    const bool live = i < n;
+    const uint32_t key = live ? keys[i] : 0u;
+    const int d = live ? (int)((key >> shift) & (kRadix - 1)) : -1;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    uint32_t rank = 0;
+    for (int b = 0; b < kRadix; b++) {
+        const unsigned long long m = __ballot(d == b);
+        if (d == b) rank = (uint32_t)__popcll(m & lt);
+        if (lane == 0) wcnt[w][b] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    if (live) {
+        uint32_t before = 0;
+        for (int k = 0; k < w; k++) before += wcnt[k][d];
+        const uint32_t pos = offs[(int64_t)d * gridDim.x + blockIdx.x] + before + rank;
+        keys_out[pos] = key;
+        vals_out[pos] = vals[i];
+    }
+}
+
+__global__ void k_copy_u32(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, int64_t n)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+// The segmented grid plan of build_grid_seg on the device (same arithmetic, so the same grids),
+// from k_seg_bbox's boxes. When the cells of all segments exceed cmax, every segment's starting
+// cell edge grows by 1.25 until they fit (any cell edge gives the same lists; only the visit
+// order, hence FP32 sums, change) and status[kStCells] is set.
+__global__ void k_plan_seg(const double* __restrict__ bb, const int64_t* __restrict__ off, int nseg,
+                           double cell_factor, double cmax, double max_axis, double max_cells,
+                           GridSeg* __restrict__ segs, int32_t* __restrict__ status)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double grow = 1.0;
+    for (int attempt = 0; attempt < 200; attempt++) {
+        double total = 0.0;
+        for (int k = 0; k < nseg; k++) {
+            const double* b = bb + 9 * k;
+            const unsigned long long hsum = ((const unsigned long long*)bb)[9 * k + 8];
+            const double hmax = b[6];
+            double span[3];
+            for (int c = 0; c < 3; c++) {
+                segs[k].g.lo[c] = b[c];
+                span[c] = b[3 + c] - b[c];
+            }
+            const double scale = hmax > 0.0 ? 4294967296.0 / hmax : 0.0;
+            const double hmean = scale > 0.0 ? ((double)hsum / scale) / (double)(off[k + 1] - off[k]) : 0.0;
+            const double maxspan = fmax(span[0], fmax(span[1], span[2]));
+            double cell = fmax(fmax(cell_factor * hmean, maxspan / max_axis), 1e-300) * grow;
+            double cells;
+            for (;;) {
+                cells = 1.0;
+                for (int c = 0; c < 3; c++) {
+                    segs[k].g.nc[c] = (int32_t)fmax(floor(span[c] / cell) + 1.0, 1.0);
+                    segs[k].g.inv[c] = 1.0 / cell;
+                    cells *= (double)segs[k].g.nc[c];
+                }
+                if (cells <= max_cells) break;
+                cell *= 1.25;
+            }
+            segs[k].base = (uint32_t)total;
+            segs[k].pad = 0;
+            total += cells;
+        }
+        if (total <= cmax) {
+            if (attempt > 0) status[kStCells] = 1;
+            return;
+        }
+        grow *= 1.25;
+    }
+    status[kStCells] = 1;
+}
+
+// offsets (exclusive scan, n + 1 entries) clamped to the fixed capacity; status[word] set when
+// the total did not fit (rows past the capacity then get none: the pass overflows and says so)
+__global__ void k_clamp_off(unsigned long long* __restrict__ off, int64_t n1, unsigned long long cap,
+                            int32_t* __restrict__ status, int word)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n1) return;
+    const unsigned long long v = off[i];
+    if (v > cap) {
+        off[i] = cap;
+        if (i == n1 - 1) status[word] = 1;
+    }
+}
+
+// status[word] = 1 when the collected scalar is nonzero
+__global__ void k_status_from(const long long* __restrict__ scal, int at, int32_t* __restrict__ status, int word)
+{
+    if (threadIdx.x == 0 && scal[at] != 0) status[word] = 1;
+}
+
+// rows whose h-rho root lies beyond their search (flag 2): the caller redoes the step eagerly
+__global__ void k_status_grow(const int32_t* __restrict__ flags, int64_t n, int32_t* __restrict__ status)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n && flags[i] == 2) atomicAdd(&status[kStGrow], 1);
+}
 
 }  // namespace nb
 
@@ -1541,7 +1795,7 @@ int reserve_reductions(nb_ctx* ctx, int64_t n)
 inline void finish_partials(nb_ctx* ctx, hipStream_t s, int64_t nb, int nslots, unsigned maxmask, int at)
 {
     long long* out = (long long*)ctx->scal + at;
-    hipMemsetAsync(out, 0, sizeof(long long) * nslots, s);
+    hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(64), 0, s, out, 0ll, (int64_t)nslots);  // a kernel: graph-safe
     const int64_t blocks = std::min<int64_t>(256, (nb + 4 * kBlock - 1) / (4 * kBlock));
     hipLaunchKernelGGL(k_partials_final, dim3((unsigned)blocks), dim3(kBlock), 0, s, ctx->part, nb, nslots, maxmask,
                        out);
@@ -2236,6 +2490,7 @@ extern "C" int32_t nb_hydro_density_mixed(nb_ctx* ctx, void* stream, const doubl
     ctx->row_cap_rm = (int32_t)(sc[0] + sc[0] / 8 + 8);
     ctx->ecap_rm = sc[4] + sc[4] / 4 + 4;
     ctx->rm_gtotal = (int64_t)tot[0];
+    ctx->rm_etotal = (int64_t)tot[1];
     const long long slow_rows = sc[1];
     float* rmass = masses ? ctx->fscratch + (size_t)tot[0] : nullptr;
     if (slow_rows > 0) {
@@ -2253,7 +2508,7 @@ extern "C" int32_t nb_hydro_density_mixed(nb_ctx* ctx, void* stream, const doubl
              (const float*)ctx->edbuf, ctx->efill, ctx->rm_eo, n, ctx->ebuf_s, ctx->edbuf_s);
     // merged row lengths (reusing the gather capacities' array), their scan: the tight rows' offsets
     NB_TIMED(ctx, s, "k_row_total_rm", k_row_total_rm, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, ctx->rowcnt,
-             ctx->efill, ctx->rm_gc, row_counts, ctx->part, n);
+             ctx->efill, ctx->rm_go, ctx->rm_eo, ctx->rm_gc, row_counts, ctx->part, n);
     finish_partials(ctx, s, grid_of(n + 1), 2, 1u, 5);  // [5] longest merged row, [6] search-list entries
     rc = scan_u64(ctx, s, ctx->rm_gc, ctx->rm_to, n + 1);
     if (rc) return rc;
@@ -2296,11 +2551,304 @@ extern "C" int32_t nb_hydro_tight_mixed(nb_ctx* ctx, void* stream, const double*
         return ctx_fail(ctx, "nb_hydro_tight_mixed: reduction buffer smaller than the launch");
     NB_TIMED(ctx, s, "k_tight_emit_rm", k_tight_emit_rm, dim3(blocks), dim3(tile), 0, s, pos, ctx->gbuf, ctx->fscratch,
              ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->rm_eo, ctx->ebuf_s, ctx->edbuf_s, h_final, n, ctx->rm_to,
-             out_off, out_len, out_index, F, ctx->part);
+             out_off, out_len, out_index, (unsigned long long)index_cap, (int32_t*)nullptr, F, ctx->part);
     finish_partials(ctx, s, blocks, 1, 0u, 8);
     long long t = 0;
     int rc = sync_copy(ctx, s, &t, (long long*)ctx->scal + 8, sizeof(t), "nb_hydro_tight_mixed: total copy failed");
     if (rc) return rc;
     *total_out = (int64_t)t;
+    return NB_OK;
+}
+
+// ===================================================================== fixed-capacity passes
+// The mixed step's grid and density passes with capacities fixed beforehand (nb_fixed_reserve):
+// no host synchronisation, no allocation and no memset, so that a caller can record a whole step
+// in a hipGraph and replay it. What does not fit is reported in the int32 status words (see
+// kStGather ...); the caller then redoes the step eagerly.
+
+namespace {
+
+int fixed_check(nb_ctx* ctx, int64_t n, int32_t nseg, const char* what)
+{
+    if (!ctx->fx_ready || n > ctx->fx_n || nseg > ctx->fx_nseg || n <= 0 || nseg < 1)
+        return ctx_fail(ctx, what);
+    return NB_OK;
+}
+
+// scratch of the fixed passes' scans and sort histograms, in bytes
+size_t fixed_tmp_bytes(int64_t n, int64_t cells)
+{
+    const int64_t m = std::max(std::max(n + 1, cells + 1), (int64_t)kRadix * ((n + kSortThreads - 1) / kSortThreads));
+    const int64_t tiles = (m + kScanTile - 1) / kScanTile;
+    return sizeof(unsigned long long) * (size_t)(tiles + 1) + 2 * sizeof(uint32_t) * (size_t)(m + 1) + 256;
+}
+
+// exclusive scan with kernels only (count entries of in into out; in may not alias out)
+template <class T>
+int scan_fixed(nb_ctx* ctx, hipStream_t s, const T* in, T* out, int64_t count)
+{
+    const int64_t tiles = (count + kScanTile - 1) / kScanTile;
+    if (sizeof(T) * (size_t)(tiles + 1) > ctx->fx_tmp_bytes) return ctx_fail(ctx, "fixed pass: scan scratch not reserved");
+    T* sums = (T*)ctx->fx_tmp;
+    hipLaunchKernelGGL(k_scan_tiles<T>, dim3((unsigned)tiles), dim3(kScanThreads), 0, s, in, count, sums);
+    hipLaunchKernelGGL(k_scan_sums<T>, dim3(1), dim3(kScanThreads), 0, s, sums, tiles);
+    hipLaunchKernelGGL(k_scan_apply<T>, dim3((unsigned)tiles), dim3(kScanThreads), 0, s, in, count, sums, out);
+    return NB_OK;
+}
+
+// stable LSD radix sort of (ctx->cell_id, ctx->iota) by the key's low fx_bits bits, values into
+// ctx->order (keys and ping-pong buffers clobbered)
+int sort_fixed(nb_ctx* ctx, hipStream_t s, int64_t n)
+{
+    const int64_t nblocks = (n + kSortThreads - 1) / kSortThreads;
+    const int64_t hn = (int64_t)kRadix * nblocks;
+    const int64_t tiles = (hn + kScanTile - 1) / kScanTile;
+    const size_t need = sizeof(uint32_t) * (size_t)(tiles + 1) + 64 + 2 * sizeof(uint32_t) * (size_t)(hn + 1);
+    if (need > ctx->fx_tmp_bytes) return ctx_fail(ctx, "fixed pass: sort scratch not reserved");
+    uint32_t* hist = (uint32_t*)((char*)ctx->fx_tmp + sizeof(uint32_t) * (size_t)(tiles + 1) + 64);
+    uint32_t* offs = hist + hn + 1;
+    uint32_t *ka = ctx->cell_id, *kb = ctx->keys_out, *va = ctx->iota, *vb = ctx->order;
+    const int passes = (ctx->fx_bits + kRadixBits - 1) / kRadixBits;
+    for (int p = 0; p < passes; p++) {
+        const int shift = p * kRadixBits;
+        hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)nblocks), dim3(kSortThreads), 0, s, ka, n, shift, hist);
+        // scan_fixed's partials live at the start of fx_tmp, before hist
+        const int64_t t = (hn + kScanTile - 1) / kScanTile;
+        uint32_t* sums = (uint32_t*)ctx->fx_tmp;
+        hipLaunchKernelGGL(k_scan_tiles<uint32_t>, dim3((unsigned)t), dim3(kScanThreads), 0, s, hist, hn, sums);
+        hipLaunchKernelGGL(k_scan_sums<uint32_t>, dim3(1), dim3(kScanThreads), 0, s, sums, t);
+        hipLaunchKernelGGL(k_scan_apply<uint32_t>, dim3((unsigned)t), dim3(kScanThreads), 0, s, hist, hn, sums, offs);
+        hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)nblocks), dim3(kSortThreads), 0, s, ka, va, n, shift, offs,
+                           kb, vb);
+        std::swap(ka, kb);
+        std::swap(va, vb);
+    }
+    if (va != ctx->order)  // an even number of passes ends in the iota buffer
+        hipLaunchKernelGGL(k_copy_u32, dim3(grid_of(n)), dim3(kBlock), 0, s, va, ctx->order, n);
+    return NB_OK;
+}
+
+// build_grid_seg with the plan made on the device. sort: order the particles by cell (the grid
+// order pass); else they must be stored in cell order already (the density pass after the step's
+// permutation), which is checked (status[kStUnsorted]).
+int build_grid_seg_fixed(nb_ctx* ctx, hipStream_t s, const double* pos, const double* h, int64_t n,
+                         const int32_t* cloud, const int64_t* off_dev, int32_t nseg, double cell_factor, bool sort,
+                         int32_t* status)
+{
+    ctx->lists_ready = 0;
+    ctx->hydro_ready = 0;
+    ctx->hydro_mixed_ready = 0;
+    hipLaunchKernelGGL(k_seg_bbox, dim3(nseg), dim3(kBlock), 0, s, pos, h, off_dev, ctx->red);
+    hipLaunchKernelGGL(k_plan_seg, dim3(1), dim3(64), 0, s, ctx->red, off_dev, nseg, cell_factor, (double)ctx->fx_cells,
+                       (double)kMaxAxisCells, kMaxTotalCells, (GridSeg*)ctx->segs, status);
+    const int64_t nc1 = ctx->fx_cells + 1;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)std::min<int64_t>(4096, (nc1 + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       s, ctx->cell_count, 0u, nc1);
+    NB_TIMED(ctx, s, "k_cell_id", k_cell_id_seg, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, n, (const GridSeg*)ctx->segs,
+             cloud, ctx->cell_id, ctx->iota, ctx->cell_count, sort ? nullptr : ctx->part);
+    if (sort) {
+        int rc = sort_fixed(ctx, s, n);
+        if (rc) return rc;
+        ctx->order_ident = 0;
+    } else {
+        finish_partials(ctx, s, grid_of(n), 1, 0u, 0);
+        hipLaunchKernelGGL(k_status_from, dim3(1), dim3(64), 0, s, (const long long*)ctx->scal, 0, status, kStUnsorted);
+        ctx->order_ident = 1;
+    }
+    return scan_fixed(ctx, s, (const uint32_t*)ctx->cell_count, ctx->cell_start, nc1);
+}
+
+}  // namespace
+
+extern "C" int32_t nb_fixed_reserve(nb_ctx* ctx, int64_t n, int32_t nseg, int64_t gather_cap, int64_t extras_cap,
+                                    int64_t index_cap, int64_t cells_cap, int32_t masses)
+{
+    if (!ctx || n <= 0 || n >= (int64_t)kMask || nseg < 1 || gather_cap <= 0 || extras_cap <= 0 || index_cap <= 0 ||
+        cells_cap <= 0 || cells_cap >= 2147483647ll)
+        return ctx_fail(ctx, "nb_fixed_reserve: bad arguments");
+    ctx->fx_ready = 0;
+    const size_t u64n = sizeof(unsigned long long) * (size_t)(n + 1);
+    if (reserve_reductions(ctx, n) ||
+        ctx_reserve(ctx, (void**)&ctx->red, &ctx->red_bytes, sizeof(double) * 9 * (size_t)nseg + 8 * (size_t)(nseg + 1) + 4096) ||
+        ctx_reserve(ctx, (void**)&ctx->segs, &ctx->segs_bytes, sizeof(GridSeg) * (size_t)nseg) ||
+        ctx_reserve(ctx, (void**)&ctx->cell_id, &ctx->cell_id_bytes, sizeof(uint32_t) * n) ||
+        ctx_reserve(ctx, (void**)&ctx->order, &ctx->order_bytes, sizeof(uint32_t) * n) ||
+        ctx_reserve(ctx, (void**)&ctx->keys_out, &ctx->keys_out_bytes, sizeof(uint32_t) * n) ||
+        ctx_reserve(ctx, (void**)&ctx->iota, &ctx->iota_bytes, sizeof(uint32_t) * n) ||
+        ctx_reserve(ctx, (void**)&ctx->cell_count, &ctx->cell_count_bytes, sizeof(uint32_t) * (size_t)(cells_cap + 1)) ||
+        ctx_reserve(ctx, (void**)&ctx->cell_start, &ctx->cell_start_bytes, sizeof(uint32_t) * (size_t)(cells_cap + 1)) ||
+        ctx_reserve(ctx, (void**)&ctx->rowcnt, &ctx->rowcnt_bytes, sizeof(int32_t) * n) ||
+        ctx_reserve(ctx, (void**)&ctx->pending, &ctx->pending_bytes, n) ||
+        ctx_reserve(ctx, (void**)&ctx->posh, &ctx->posh_bytes, sizeof(double4) * n) ||
+        ctx_reserve(ctx, (void**)&ctx->efill, &ctx->efill_bytes, sizeof(unsigned int) * n) ||
+        ctx_reserve(ctx, (void**)&ctx->rm_gc, &ctx->rm_gc_bytes, u64n) ||
+        ctx_reserve(ctx, (void**)&ctx->rm_go, &ctx->rm_go_bytes, u64n) ||
+        ctx_reserve(ctx, (void**)&ctx->rm_ec, &ctx->rm_ec_bytes, u64n) ||
+        ctx_reserve(ctx, (void**)&ctx->rm_eo, &ctx->rm_eo_bytes, u64n) ||
+        ctx_reserve(ctx, (void**)&ctx->rm_to, &ctx->rm_to_bytes, u64n) ||
+        ctx_reserve(ctx, (void**)&ctx->gbuf, &ctx->gbuf_bytes, sizeof(uint32_t) * (size_t)gather_cap) ||
+        ctx_reserve(ctx, (void**)&ctx->fscratch, &ctx->fscratch_bytes,
+                    sizeof(float) * (size_t)gather_cap * (masses ? 2 : 1)) ||
+        ctx_reserve(ctx, (void**)&ctx->ebuf, &ctx->ebuf_bytes, sizeof(uint32_t) * (size_t)extras_cap) ||
+        ctx_reserve(ctx, (void**)&ctx->edbuf, &ctx->edbuf_bytes, sizeof(float) * (size_t)extras_cap) ||
+        ctx_reserve(ctx, (void**)&ctx->ebuf_s, &ctx->ebuf_s_bytes, sizeof(uint32_t) * (size_t)extras_cap) ||
+        ctx_reserve(ctx, (void**)&ctx->edbuf_s, &ctx->edbuf_s_bytes, sizeof(float) * (size_t)extras_cap) ||
+        ctx_reserve(ctx, (void**)&ctx->pack, &ctx->pack_bytes, 48 * (size_t)n))
+        return NB_ERR_HIP;
+    int bits = 1;
+    while ((int64_t(1) << bits) < cells_cap + 1 && bits < 32) bits++;
+    if (ctx_reserve(ctx, (void**)&ctx->fx_tmp, &ctx->fx_tmp_bytes, fixed_tmp_bytes(n, cells_cap))) return NB_ERR_HIP;
+    ctx->fx_n = n;
+    ctx->fx_nseg = nseg;
+    ctx->fx_gather = gather_cap;
+    ctx->fx_extras = extras_cap;
+    ctx->fx_index = index_cap;
+    ctx->fx_cells = cells_cap;
+    ctx->fx_bits = bits;
+    ctx->fx_ready = 1;
+    return NB_OK;
+}
+
+extern "C" int32_t nb_ctx_sizes(const nb_ctx* ctx, int64_t* out6)
+{
+    if (!ctx || !out6) return -1;
+    out6[0] = ctx->rm_gtotal;        // gather-row entries allocated by the last mixed density pass
+    out6[1] = ctx->rm_etotal;        // extras entries allocated by it
+    out6[2] = ctx->lists_total;      // its search-list entries (the tight index capacity it needs)
+    out6[3] = ctx->last_grid_cells;  // cells of the last grid
+    out6[4] = ctx->lists_n;
+    out6[5] = ctx->fx_ready;
+    return NB_OK;
+}
+
+extern "C" int32_t nb_grid_order_seg_fixed(nb_ctx* ctx, void* stream, const double* pos, const double* h, int64_t n,
+                                           const int32_t* cloud, const int64_t* seg_off_dev, int32_t nseg,
+                                           double cell_factor, int32_t* status, int32_t* order_out)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (!cloud || !seg_off_dev || !status || !order_out || !(cell_factor > 0.0))
+        return ctx_fail(ctx, "nb_grid_order_seg_fixed: bad arguments");
+    int rc = fixed_check(ctx, n, nseg, "nb_grid_order_seg_fixed: capacities not reserved for this size");
+    if (rc) return rc;
+    rc = build_grid_seg_fixed(ctx, s, pos, h, n, cloud, seg_off_dev, nseg, cell_factor, true, status);
+    if (rc) return rc;
+    if (hipMemcpyAsync(order_out, ctx->order, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return ctx_fail(ctx, "nb_grid_order_seg_fixed: copy failed");
+    return NB_OK;
+}
+
+extern "C" int32_t nb_hydro_density_mixed_fixed(nb_ctx* ctx, void* stream, const double* pos, const double* mass,
+                                                double m_uniform, const double* h_start, const double* h_search,
+                                                int64_t n, const int32_t* cloud, const int64_t* seg_off_dev,
+                                                int32_t nseg, double cell_factor, const int32_t* row_hint,
+                                                int32_t* row_counts, int32_t kid, double cnorm, double eta,
+                                                double tol, double h_max, double* h_out, double* rho, double* omega,
+                                                int32_t* iters, int32_t* flags, int32_t* status)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (!cloud || !seg_off_dev || !row_hint || !row_counts || !status || !(cell_factor > 0.0))
+        return ctx_fail(ctx, "nb_hydro_density_mixed_fixed: bad arguments");
+    int rc = fixed_check(ctx, n, nseg, "nb_hydro_density_mixed_fixed: capacities not reserved for this size");
+    if (rc) return rc;
+    const bool masses = !(m_uniform > 0.0);
+    if (masses && ctx->fscratch_bytes < sizeof(float) * 2 * (size_t)ctx->fx_gather)
+        return ctx_fail(ctx, "nb_hydro_density_mixed_fixed: mass rows not reserved");
+    rc = build_grid_seg_fixed(ctx, s, pos, h_search, n, cloud, seg_off_dev, nseg, cell_factor, false, status);
+    if (rc) return rc;
+    NB_TIMED(ctx, s, "k_pack_posh", k_pack_posh, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, h_search, n,
+             (double4*)ctx->posh);
+    hipLaunchKernelGGL(k_caps_mixed, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, row_hint, ctx->rowcnt, ctx->efill, n, 1,
+                       (int64_t)0, (int64_t)0, ctx->rm_gc, ctx->rm_ec);
+    rc = scan_fixed(ctx, s, (const unsigned long long*)ctx->rm_gc, ctx->rm_go, n + 1);
+    if (rc) return rc;
+    rc = scan_fixed(ctx, s, (const unsigned long long*)ctx->rm_ec, ctx->rm_eo, n + 1);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_clamp_off, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, ctx->rm_go, n + 1,
+                       (unsigned long long)ctx->fx_gather, status, (int)kStGather);
+    hipLaunchKernelGGL(k_clamp_off, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, ctx->rm_eo, n + 1,
+                       (unsigned long long)ctx->fx_extras, status, (int)kStExtras);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_of(n)), dim3(kBlock), 0, s, (uint32_t*)ctx->efill, 0u, n);
+    MixArgs A{mass, m_uniform, h_start, h_max, kid, cnorm, eta, tol, h_out, rho, omega, iters, flags, ctx->pending,
+              ctx->part};
+    const int tile = ctx->tile;
+    const int64_t blocks = (n * kG + tile - 1) / tile;
+    const GridSeg* segs = (const GridSeg*)ctx->segs;
+    float* rmass = masses ? ctx->fscratch + (size_t)ctx->fx_gather : nullptr;
+    const Grid g_none{};
+    if (masses)
+        NB_TIMED(ctx, s, "k_gather_mixed", (k_gather_mixed<true, true>), dim3(blocks), dim3(tile), 0, s,
+                 (const double4*)ctx->posh, n, g_none, segs, cloud, ctx->cell_start, (const uint32_t*)nullptr, ctx->gbuf,
+                 ctx->fscratch, rmass, ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->ebuf, (float*)ctx->edbuf, ctx->rm_eo, A);
+    else
+        NB_TIMED(ctx, s, "k_gather_mixed", (k_gather_mixed<false, true>), dim3(blocks), dim3(tile), 0, s,
+                 (const double4*)ctx->posh, n, g_none, segs, cloud, ctx->cell_start, (const uint32_t*)nullptr, ctx->gbuf,
+                 ctx->fscratch, rmass, ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->ebuf, (float*)ctx->edbuf, ctx->rm_eo, A);
+    // [2] extras overflow, [3] rows over capacity
+    finish_partials(ctx, s, blocks, 5, 1u | 4u | 16u, 0);
+    hipLaunchKernelGGL(k_status_from, dim3(1), dim3(64), 0, s, (const long long*)ctx->scal, 2, status, (int)kStExtras);
+    hipLaunchKernelGGL(k_status_from, dim3(1), dim3(64), 0, s, (const long long*)ctx->scal, 3, status, (int)kStGather);
+    if (masses)
+        NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<true>, dim3(blocks), dim3(tile), 0, s, h_search, n,
+                 ctx->fscratch, rmass, ctx->rm_go, ctx->rowcnt, A);
+    else
+        NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<false>, dim3(blocks), dim3(tile), 0, s, h_search, n,
+                 ctx->fscratch, rmass, ctx->rm_go, ctx->rowcnt, A);
+    hipLaunchKernelGGL(k_status_grow, dim3(grid_of(n)), dim3(kBlock), 0, s, flags, n, status);
+    NB_TIMED(ctx, s, "k_sort_extras", k_sort_extras, dim3(blocks), dim3(tile), 0, s, ctx->ebuf, (const float*)ctx->edbuf,
+             ctx->efill, ctx->rm_eo, n, ctx->ebuf_s, ctx->edbuf_s);
+    NB_TIMED(ctx, s, "k_row_total_rm", k_row_total_rm, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, ctx->rowcnt,
+             ctx->efill, ctx->rm_go, ctx->rm_eo, ctx->rm_gc, row_counts, ctx->part, n);
+    rc = scan_fixed(ctx, s, (const unsigned long long*)ctx->rm_gc, ctx->rm_to, n + 1);
+    if (rc) return rc;
+    if (hipGetLastError() != hipSuccess) return ctx_fail(ctx, "nb_hydro_density_mixed_fixed: kernel launch failed");
+    ctx->lists_n = n;
+    ctx->lists_total = 0;
+    ctx->lists_maxrow = 0;
+    ctx->rm_gtotal = ctx->fx_gather;
+    ctx->hydro_mixed_ready = 1;
+    ctx->mix_kid = kid;
+    ctx->mix_cnorm = cnorm;
+    ctx->mix_m_uniform = m_uniform;
+    return NB_OK;
+}
+
+extern "C" int32_t nb_hydro_tight_mixed_fixed(nb_ctx* ctx, void* stream, const double* pos, const double* h_final,
+                                              double* rho, double* omega, int64_t* out_off, int32_t* out_len,
+                                              int32_t* out_index, int32_t* status)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (!ctx || !ctx->hydro_mixed_ready || !ctx->fx_ready)
+        return ctx_fail_code(ctx, NB_ERR_NOT_BUILT, "nb_hydro_tight_mixed_fixed: no fixed density pass");
+    if (!pos || !h_final || !rho || !omega || !out_off || !out_len || !out_index || !status)
+        return ctx_fail(ctx, "nb_hydro_tight_mixed_fixed: bad arguments");
+    const int64_t n = ctx->lists_n;
+    const bool masses = !(ctx->mix_m_uniform > 0.0);
+    const FinArgs F{masses ? ctx->fscratch + (size_t)ctx->rm_gtotal : nullptr, ctx->mix_m_uniform, ctx->mix_kid,
+                    ctx->mix_cnorm, ctx->pending, rho, omega};
+    const int tile = ctx->tile;
+    const unsigned blocks = (unsigned)((n * kG + tile - 1) / tile);
+    NB_TIMED(ctx, s, "k_tight_emit_rm", k_tight_emit_rm, dim3(blocks), dim3(tile), 0, s, pos, ctx->gbuf, ctx->fscratch,
+             ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->rm_eo, ctx->ebuf_s, ctx->edbuf_s, h_final, n, ctx->rm_to,
+             out_off, out_len, out_index, (unsigned long long)ctx->fx_index, status, F, ctx->part);
+    if (hipGetLastError() != hipSuccess) return ctx_fail(ctx, "nb_hydro_tight_mixed_fixed: kernel launch failed");
+    return NB_OK;
+}
+
+// Node counts of a recorded graph by type (hipGraphNodeType order, 16 entries): a caller checks
+// that a captured step holds no memset nodes.
+extern "C" int32_t nb_graph_node_counts(void* graph, int32_t* counts16)
+{
+    if (!graph || !counts16) return -1;
+    for (int k = 0; k < 16; k++) counts16[k] = 0;
+    size_t num = 0;
+    if (hipGraphGetNodes((hipGraph_t)graph, nullptr, &num) != hipSuccess) return -1;
+    std::vector<hipGraphNode_t> nodes(num);
+    if (num && hipGraphGetNodes((hipGraph_t)graph, nodes.data(), &num) != hipSuccess) return -1;
+    for (size_t k = 0; k < num; k++) {
+        hipGraphNodeType t;
+        if (hipGraphNodeGetType(nodes[k], &t) != hipSuccess) return -1;
+        const int it = (int)t;
+        counts16[it >= 0 && it < 16 ? it : 15]++;
+    }
     return NB_OK;
 }

@@ -67,6 +67,12 @@ struct nb_ctx {
     uint32_t* ebuf_s = nullptr; size_t ebuf_s_bytes = 0;  // extras in index order
     float* edbuf_s = nullptr; size_t edbuf_s_bytes = 0;
     int64_t rm_gtotal = 0;  // gather-row entries of the last mixed pass (start of the mass rows)
+    int64_t rm_etotal = 0;  // extras entries of the last mixed pass
+    // fixed capacities of the passes that a captured step records (nb_fixed_reserve): no host
+    // synchronisation or allocation inside them; what does not fit is reported in a status word
+    int fx_ready = 0;
+    int64_t fx_n = 0, fx_nseg = 0, fx_gather = 0, fx_extras = 0, fx_index = 0, fx_cells = 0;
+    int fx_bits = 32;  // radix-sort key bits of fx_cells
     void* segs = nullptr; size_t segs_bytes = 0;       // grid table of an ensemble's segments
     std::vector<char> host_segs;                       // its host copy
     void* posh = nullptr; size_t posh_bytes = 0;  // (x, y, z, h_search) records of the mixed gather
@@ -77,6 +83,7 @@ struct nb_ctx {
     double* red = nullptr; size_t red_bytes = 0;
     int64_t* scal = nullptr; size_t scal_bytes = 0;
     void* cub_tmp = nullptr; size_t cub_tmp_bytes = 0;
+    void* fx_tmp = nullptr; size_t fx_tmp_bytes = 0;  // scan partials and sort histograms of the fixed passes
     int32_t row_cap = 0;
     int32_t row_cap_rm = 0;
     int64_t ecap_rm = 0;  // extras segment capacity of the mixed lists

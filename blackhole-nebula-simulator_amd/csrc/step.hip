@@ -447,6 +447,17 @@ extern "C" int32_t nb_dt_minima_seg(nb_ctx* ctx, void* stream, const double* h, 
     return NB_OK;
 }
 
+extern "C" int32_t nb_dt_minima_seg_dev(nb_ctx* ctx, void* stream, const double* h, const double* vsig,
+                                        const double* acc, const double* nu, double c_visc, const double* x,
+                                        const int64_t* seg_off_dev, int32_t nseg, double* out_dev)
+{
+    if (!ctx || nseg < 1 || !h || !vsig || !acc || !x || !seg_off_dev || !out_dev)
+        return ctx_fail(ctx, "nb_dt_minima_seg_dev: bad arguments");
+    hipLaunchKernelGGL(k_dt_seg, dim3(nseg), dim3(kBlockS), 0, (hipStream_t)stream, h, vsig, acc, nu, c_visc, x,
+                       seg_off_dev, out_dev);
+    return launch_check_s(ctx, "nb_dt_minima_seg_dev");
+}
+
 extern "C" int32_t nb_axpy_rows(nb_ctx* ctx, void* stream, const double* y, const double* a, const double* s_rows,
                                 int64_t n, int32_t width, double* out)
 {

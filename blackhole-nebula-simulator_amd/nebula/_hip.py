@@ -114,6 +114,14 @@ SIGNATURES = {
     "nb_hydro_tight_mixed": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _P64]),
     "nb_timing": (_i32, [_vp, _i32]),
     "nb_set_tile": (_i32, [_vp, _i32]),
+    "nb_fixed_reserve": (_i32, [_vp, _i64, _i32, _i64, _i64, _i64, _i64, _i32]),
+    "nb_ctx_sizes": (_i32, [_vp, _P64]),
+    "nb_grid_order_seg_fixed": (_i32, [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _i32, _f64, _vp, _vp]),
+    "nb_hydro_density_mixed_fixed": (_i32, [_vp, _vp, _vp, _vp, _f64, _vp, _vp, _i64, _vp, _vp, _i32, _f64, _vp, _vp,
+                                            _i32, _f64, _f64, _f64, _f64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "nb_hydro_tight_mixed_fixed": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "nb_dt_minima_seg_dev": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _f64, _vp, _vp, _i32, _vp]),
+    "nb_graph_node_counts": (_i32, [_vp, _vp]),
     "nb_dt_minima_seg": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _f64, _vp, _P64, _i32, _PD]),
     "nb_axpy_rows": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _i32, _vp]),
     "nb_expand_seg": (_i32, [_vp, _vp, _vp, _vp, _i64, _f64, _vp]),

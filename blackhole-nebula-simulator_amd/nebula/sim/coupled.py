@@ -209,16 +209,21 @@ def gravity_flow(cfg: CoupledConfig, x, p, dt):
     return x1, p1
 
 
-def _hydro(st: GasState, cfg: CoupledConfig, v, dt_alpha: float, a_prev, h_start=None) -> GasState:
+def _hydro(st: GasState, cfg: CoupledConfig, v, dt_alpha: float, a_prev, h_start=None, fixed=None) -> GasState:
     """coupled.py:145-195 on device tensors. h_start: the Newton solve's starting h (default:
-    the previous h, as in the reference); the search lists always come from the previous h."""
+    the previous h, as in the reference); the search lists always come from the previous h.
+    fixed: the fixed-capacity passes of a step recorded for replay (nebula.sim.captured): one
+    search round, no host synchronisation; what does not fit goes to fixed.status."""
     hc = cfg.hydro
     if hc.isothermal_cs is not None or hc.eos != "ideal":
         raise ValueError("the GPU step implements the ideal-gas equation of state")
     n = int(st.x.shape[0])
     buf = st.list_buffer if st.list_buffer is not None else IndexBuffer()
     counts = None
-    if hc.precision == "mixed":
+    if fixed is not None:
+        counts = _dev.empty((n, 2), torch.int32)
+        d, n_search = fixed.density(st, cfg, h_start, counts)
+    elif hc.precision == "mixed":
         counts = _dev.empty((n, 2), torch.int32)
         d, n_search = hydro_density_mixed_device(st.x, st.mass, st.h.clone(), hc.kernel, hc.eta, 1e-4, hc.h_max,
                                                  hc.skin, st.m_uniform, h_start_d=h_start, cloud_d=st.cloud,
@@ -233,10 +238,10 @@ def _hydro(st: GasState, cfg: CoupledConfig, v, dt_alpha: float, a_prev, h_start
                                            index_buffer=buf)  # fmt: skip
     p_gas = _dev.empty(n)
     cs = _dev.empty(n)
-    nbad = torch.zeros(1, dtype=torch.int32, device=st.x.device)
+    nbad = torch.zeros(1, dtype=torch.int32, device=st.x.device) if fixed is None else fixed.status[7:]
     _hip.call("nb_eos_ideal", _hip.ptr(d.rho), _hip.ptr(st.u), float(hc.gamma), n, _hip.ptr(p_gas), _hip.ptr(cs),
               _hip.ptr(nbad))  # fmt: skip
-    bad = int(nbad.item())
+    bad = int(nbad.item()) if fixed is None else 0  # a recorded step reports it in its status
     if bad:
         raise FloatingPointError(f"non-positive internal energy at {bad} particles")
     nb = d.neighbours  # tightened at the final h

@@ -189,6 +189,43 @@ int32_t nb_hydro_density_mixed(nb_ctx *ctx, void *stream, const double *pos, con
 int32_t nb_hydro_tight_mixed(nb_ctx *ctx, void *stream, const double *pos, const double *h_final, double *rho,
                              double *omega, int64_t *out_off, int32_t *out_len, int32_t *out_index,
                              int64_t index_cap, int64_t *total_out);
+/* Fixed-capacity passes for steps recorded in a hipGraph (no counterpart in the reference; for
+ * callers that replay a captured step). nb_fixed_reserve sizes every buffer of the passes below
+ * once (N particles, nseg systems, gather-row, extras and tight-index entries, grid cells;
+ * masses != 0: per-particle masses). The passes then make no host synchronisation, allocation or
+ * memset; what does not fit, or must be redone eagerly, is flagged in the int32 device status
+ * words: [0] gather rows, [1] extras, [2] tight index, [3] grid cells (the grid was coarsened:
+ * valid lists, other FP32 sums), [4] rows whose h outgrew the search, [5] particles not stored in
+ * cell order. nb_glf_step and nb_eos_ideal given status + 6 / status + 7 add theirs. */
+int32_t nb_fixed_reserve(nb_ctx *ctx, int64_t n, int32_t nseg, int64_t gather_cap, int64_t extras_cap,
+                         int64_t index_cap, int64_t cells_cap, int32_t masses);
+/* out6 = [gather-row entries, extras entries and search-list entries of the last mixed density
+ * pass, cells of the last grid, its N, fixed capacities reserved (0/1)]: sizes for nb_fixed_reserve. */
+int32_t nb_ctx_sizes(const nb_ctx *ctx, int64_t *out6);
+/* nb_grid_order_seg with the grid planned on the device (seg_off_dev: device, nseg + 1). */
+int32_t nb_grid_order_seg_fixed(nb_ctx *ctx, void *stream, const double *pos, const double *h, int64_t n,
+                                const int32_t *cloud, const int64_t *seg_off_dev, int32_t nseg, double cell_factor,
+                                int32_t *status, int32_t *order_out);
+/* nb_hydro_density_mixed of a segmented system stored in cell order, with the fixed capacities:
+ * row capacities from row_hint (required), one search round (rows that need more are flagged). */
+int32_t nb_hydro_density_mixed_fixed(nb_ctx *ctx, void *stream, const double *pos, const double *mass,
+                                     double m_uniform, const double *h_start, const double *h_search, int64_t n,
+                                     const int32_t *cloud, const int64_t *seg_off_dev, int32_t nseg,
+                                     double cell_factor, const int32_t *row_hint, int32_t *row_counts, int32_t kid,
+                                     double cnorm, double eta, double tol, double h_max, double *h_out, double *rho,
+                                     double *omega, int32_t *iters, int32_t *flags, int32_t *status);
+/* nb_hydro_tight_mixed after nb_hydro_density_mixed_fixed; out_index holds the reserved index
+ * capacity. */
+int32_t nb_hydro_tight_mixed_fixed(nb_ctx *ctx, void *stream, const double *pos, const double *h_final, double *rho,
+                                   double *omega, int64_t *out_off, int32_t *out_len, int32_t *out_index,
+                                   int32_t *status);
+/* nb_dt_minima_seg into device memory (out_dev: nseg x 4). */
+int32_t nb_dt_minima_seg_dev(nb_ctx *ctx, void *stream, const double *h, const double *vsig, const double *acc,
+                             const double *nu, double c_visc, const double *x, const int64_t *seg_off_dev,
+                             int32_t nseg, double *out_dev);
+/* Node counts by hipGraphNodeType (16 entries) of a recorded graph (hipGraph_t). */
+int32_t nb_graph_node_counts(void *graph, int32_t *counts16);
+
 /* gradients_numba (sph.py:837-884) in mixed precision: FP32 pair terms over 16-lane row
  * groups, compensated sums, partials combined and the 3x3 inverse taken in FP64. Arguments as
  * nb_gradients. (The force pass takes its precision from nb_force_params.precision.) */

@@ -159,6 +159,37 @@ def run_gpu(ic, cfg, steps, warmup, learned=False, log_stride=8, train_steps=300
     return wall, kern, entry, st, info
 
 
+def _small_ensemble(cfg, args):
+    """64 small clouds (1024..4096 particles): where launches and host work dominate a step,
+    eager against recorded steps."""
+    import numpy as np
+    import torch
+
+    from nebula.sim.captured import CapturedEnsemble
+    from nebula.sim.ensemble import Ensemble, ensemble_ics
+
+    ics = ensemble_ics(64, 1024, 4096, seed=7)
+    total = int(sum(len(ic.x) for ic in ics))
+    out = {"workload": f"64 clouds, {total} particles (1024..4096 per cloud)"}
+    for mode in ("eager", "captured"):
+        ens = Ensemble(ics, cfg)
+        ens.step()
+        runner = ens if mode == "eager" else CapturedEnsemble(ens)
+        runner.step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            runner.step()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        out[mode] = {"value": total * args.steps / wall, "ms_per_step": wall / args.steps * 1e3}
+        del runner, ens
+    out["speedup_captured"] = out["captured"]["value"] / out["eager"]["value"]
+    print(f"bench: small ensemble eager {out['eager']['ms_per_step']:.2f} ms, captured "
+          f"{out['captured']['ms_per_step']:.2f} ms per step", file=sys.stderr, flush=True)  # fmt: skip
+    return out
+
+
 def run_ensemble(args):
     """64 clouds of the plain cloud physics (particle counts log-uniform in [8192, 262144],
     pericentres 1000-4000, radii 100-200), stepped together with a step per cloud, against the same
@@ -203,6 +234,7 @@ def run_ensemble(args):
                 "fallbacks": cap.fallbacks, "graph_nodes": cap.node_counts(), "capacities": cap.capacities}
     print(f"bench: captured {captured['value']:.4g} particle-steps/s", file=sys.stderr, flush=True)
     del cap, ens
+    small = _small_ensemble(cfg, args)
     # sequential: each cloud alone (same tuned settings), timed over a few steps after one warm-up
     seq_steps = max(2, min(5, args.steps))
     per_cloud_ms = []
@@ -240,6 +272,7 @@ def run_ensemble(args):
         "sequential": {"value": sequential, "ms_per_ensemble_step": seq_ms, "steps_per_cloud": seq_steps,
                        "speedup_batched": batched / sequential},
         "captured": captured,
+        "small_ensemble": small,
         "autotune_ms_per_hydro_pass": {f"{f}x{t}": round(v, 3) for (f, t), v in choice.table.items()},
     }
     print(json.dumps(out))

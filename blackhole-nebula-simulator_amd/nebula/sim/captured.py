@@ -218,7 +218,7 @@ class CapturedEnsemble:
         dt = hc.courant * m[:, 0]
         dt = np.where(np.isfinite(m[:, 1]), np.minimum(dt, hc.c_force * m[:, 1]), dt)
         if self.cfg.hole:
-            dt = np.minimum(dt, self.cfg.eta_orb * m[:, 3] ** 1.5)
+            dt = np.minimum(dt, np.array([self.cfg.eta_orb * float(r) ** 1.5 for r in m[:, 3]]))
         return dt
 
     def _set_dt(self):

@@ -349,7 +349,7 @@ def stable_dt(st: GasState, cfg: CoupledConfig) -> float:
     if st.nu is not None:
         dt = min(dt, float(out[2]))
     if cfg.hole:
-        dt = min(dt, cfg.eta_orb * out[3] ** 1.5)
+        dt = min(dt, cfg.eta_orb * float(out[3]) ** 1.5)  # scalar (libm) pow, as the oracle
     return float(dt)
 
 

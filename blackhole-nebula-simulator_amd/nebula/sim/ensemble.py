@@ -79,7 +79,9 @@ def stable_dt_clouds(st: GasState, cfg: coupled.CoupledConfig) -> np.ndarray:
     if st.nu is not None:
         dt = np.minimum(dt, m[:, 2])
     if cfg.hole:
-        dt = np.minimum(dt, cfg.eta_orb * m[:, 3] ** 1.5)
+        # scalar pow per cloud, as coupled.stable_dt takes it: NumPy's vectorised power may differ
+        # from the scalar (libm) pow in the last bit on some hosts
+        dt = np.minimum(dt, np.array([cfg.eta_orb * float(r) ** 1.5 for r in m[:, 3]]))
     return dt
 
 

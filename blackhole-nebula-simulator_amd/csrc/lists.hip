@@ -430,13 +430,18 @@ __global__ void k_extras_fill(const uint32_t* __restrict__ gbuf, int64_t cap, co
 // tot[n] = 0.
 __global__ void __launch_bounds__(kBlock)
 k_row_total(const int32_t* __restrict__ rowcnt, const unsigned long long* __restrict__ ecount,
-            unsigned long long* __restrict__ tot, long long* __restrict__ part, int64_t n)
+            unsigned long long* __restrict__ tot, long long* __restrict__ part, int64_t n,
+            int32_t* __restrict__ row_counts)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     long long t = 0;
     if (i < n) {
         t = (long long)rowcnt[i] + (long long)ecount[i];
         tot[i] = (unsigned long long)t;
+        if (row_counts) {  // (gather row, extras) of the particle, the next pass's capacity hint
+            row_counts[2 * i] = rowcnt[i];
+            row_counts[2 * i + 1] = (int32_t)ecount[i];
+        }
     }
     if (i == n) tot[i] = 0;
     block_partial<true>(t, part);
@@ -699,13 +704,13 @@ __device__ __forceinline__ void row_sums(const double* __restrict__ d2row, const
 // sph.py:251-297 for one row whose gathered d2 (and masses) sit in blocked buffers.
 template <bool kMixed>
 __device__ void solve_row(int64_t i, int32_t m, const double* __restrict__ dbuf, const double* __restrict__ mbuf,
-                          int64_t cap, double h_search_i, const DensArgs& A)
+                          int64_t base, double h_search_i, const DensArgs& A)
 {
     const double m_i = A.mass[i];
     const double r_max = 2.0 * pymin(h_search_i, A.h_max);
     const double r_max2 = r_max * r_max;
-    const double* d2row = dbuf + ((i >> 6) * cap) * 64 + (i & 63);
-    const double* mrow = mbuf ? mbuf + ((i >> 6) * cap) * 64 + (i & 63) : nullptr;
+    const double* d2row = dbuf + base;
+    const double* mrow = mbuf ? mbuf + base : nullptr;
     double hi = pymin(h_search_i, A.h_max);
     double lo = 0.0;
     double h = pymin(A.h_start[i], hi);
@@ -747,13 +752,45 @@ __device__ void solve_row(int64_t i, int32_t m, const double* __restrict__ dbuf,
     A.flags[i] = flag;
 }
 
-// Gather (as k_gather, open boxes) keeping d2 (and masses), then the row's density solve.
+// Capacities of 64-row blocks with capacities of their own: bsize[b] = 64 x the largest row need
+// of block b (row i of the block at boff[b] + i % 64, entry k 64 further per k; boff = exclusive
+// scan of bsize). Row need: mode 0, cap0; mode 1, a + a / 4 + 16 rounded up to a multiple of 4
+// (a = cnt[i * cstride], the previous pass's count of the same particle: headroom for the drift
+// between steps); mode 2, cnt[i * cstride] (+ add[i] when given), exact. bsize[nblk] = 0.
+__global__ void k_block_caps(const int32_t* __restrict__ cnt, int cstride, const unsigned long long* __restrict__ add,
+                             int64_t n, int mode, int64_t cap0, unsigned long long* __restrict__ bsize, int64_t nblk)
+{
+    const int64_t blk = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);  // one wavefront per block
+    if (blk > nblk) return;
+    const int64_t i = blk * 64 + (threadIdx.x & 63);
+    long long need = 0;
+    if (blk < nblk && i < n) {
+        if (mode == 0) {
+            need = cap0;
+        } else {
+            const long long a = cnt[i * cstride];
+            need = mode == 1 ? ((a + a / 4 + 16 + 3) / 4) * 4 : a + (add ? (long long)add[i] : 0);
+        }
+        need = need < 1 ? 1 : need;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const long long t = __shfl_xor(need, o, 64);
+        need = t > need ? t : need;
+    }
+    if ((threadIdx.x & 63) == 0) bsize[blk] = (unsigned long long)need * 64ull;
+}
+
+// Gather (as k_gather, open boxes) keeping d2 (and masses) in 64-row blocks of capacities of
+// their own (boff), then the row's density solve. Rows longer than their block's capacity are
+// counted in full (rowcnt) and stored up to the capacity; the caller sizes the blocks from those
+// counts and gathers again.
 template <bool kMixed>
 __global__ void __launch_bounds__(kBlock)
 k_gather_solve(const double* __restrict__ pos, const double* __restrict__ h, int64_t n, Grid g,
                const uint32_t* __restrict__ start, const uint32_t* __restrict__ order, uint32_t* __restrict__ gbuf,
-               double* __restrict__ dbuf, double* __restrict__ mbuf, int64_t cap, int32_t* __restrict__ rowcnt,
-               uint8_t* __restrict__ unsorted, unsigned long long* __restrict__ ecount, DensArgs A)
+               double* __restrict__ dbuf, double* __restrict__ mbuf, const unsigned long long* __restrict__ boff,
+               int32_t* __restrict__ rowcnt, uint8_t* __restrict__ unsorted, unsigned long long* __restrict__ ecount,
+               DensArgs A)
 {
     // lanes past the end redo the last particle without writing (full blocks for the reductions)
     int64_t i = xcd_tile() * (int64_t)blockDim.x + threadIdx.x;
@@ -770,7 +807,8 @@ k_gather_solve(const double* __restrict__ pos, const double* __restrict__ h, int
     int32_t m = 0;
     int64_t last = -1;
     bool sorted = true;
-    const int64_t base = ((i >> 6) * cap) * 64 + (i & 63);
+    const int64_t base = (int64_t)boff[i >> 6] + (i & 63);
+    const int32_t cap = (int32_t)((boff[(i >> 6) + 1] - boff[i >> 6]) >> 6);
     for (int64_t di = 0; di < nx; di++) {
         const uint32_t cx = (uint32_t)(ax + di);
         for (int64_t dj = 0; dj < ny; dj++) {
@@ -807,21 +845,22 @@ k_gather_solve(const double* __restrict__ pos, const double* __restrict__ h, int
     block_partial<true>(m, A.part);
     block_partial<false>(live && !sorted ? 1 : 0, A.part + gridDim.x);
     block_partial<false>(live && !done_here ? 1 : 0, A.part + 2 * gridDim.x);
+    block_partial<false>(live && m > cap ? 1 : 0, A.part + 3 * gridDim.x);
     if (!live) return;
     rowcnt[i] = m;
     unsorted[i] = sorted ? 0 : 1;
     A.pending[i] = done_here ? 0 : 1;
-    if (done_here) solve_row<kMixed>(i, m, dbuf, mbuf, cap, hs, A);
+    if (done_here) solve_row<kMixed>(i, m, dbuf, mbuf, base, hs, A);
 }
 
-// Insertion sort of unsorted rows, carrying d2 and mass along.
+// Insertion sort of unsorted rows (blocks at boff), carrying d2 and mass along.
 __global__ void k_sort_rows_payload(uint32_t* __restrict__ gbuf, double* __restrict__ dbuf, double* __restrict__ mbuf,
-                                    int64_t cap, const int32_t* __restrict__ rowcnt,
+                                    const unsigned long long* __restrict__ boff, const int32_t* __restrict__ rowcnt,
                                     const uint8_t* __restrict__ unsorted, int64_t n)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n || !unsorted[i]) return;
-    const int64_t base = ((i >> 6) * cap) * 64 + (i & 63);
+    const int64_t base = (int64_t)boff[i >> 6] + (i & 63);
     const int32_t m = rowcnt[i];
     for (int32_t a = 1; a < m; a++) {
         const uint32_t v = gbuf[base + (int64_t)a * 64];
@@ -843,23 +882,23 @@ __global__ void k_sort_rows_payload(uint32_t* __restrict__ gbuf, double* __restr
 
 template <bool kMixed>
 __global__ void k_solve_pending(const double* __restrict__ h, int64_t n, const double* __restrict__ dbuf,
-                                const double* __restrict__ mbuf, int64_t cap, const int32_t* __restrict__ rowcnt,
-                                DensArgs A)
+                                const double* __restrict__ mbuf, const unsigned long long* __restrict__ boff,
+                                const int32_t* __restrict__ rowcnt, DensArgs A)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n || !A.pending[i]) return;
-    solve_row<kMixed>(i, rowcnt[i], dbuf, mbuf, cap, h[i], A);
+    solve_row<kMixed>(i, rowcnt[i], dbuf, mbuf, (int64_t)boff[i >> 6] + (i & 63), h[i], A);
 }
 
 // extras with their d2
-__global__ void k_extras_fill_d2(const uint32_t* __restrict__ gbuf, const double* __restrict__ dbuf, int64_t cap,
-                                 const int32_t* __restrict__ rowcnt, const unsigned long long* __restrict__ eoff,
-                                 unsigned int* __restrict__ efill, uint32_t* __restrict__ ebuf,
-                                 double* __restrict__ edbuf, int64_t n)
+__global__ void k_extras_fill_d2(const uint32_t* __restrict__ gbuf, const double* __restrict__ dbuf,
+                                 const unsigned long long* __restrict__ boff, const int32_t* __restrict__ rowcnt,
+                                 const unsigned long long* __restrict__ eoff, unsigned int* __restrict__ efill,
+                                 uint32_t* __restrict__ ebuf, double* __restrict__ edbuf, int64_t n)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int64_t base = ((i >> 6) * cap) * 64 + (i & 63);
+    const int64_t base = (int64_t)boff[i >> 6] + (i & 63);
     const int32_t m = rowcnt[i];
     for (int32_t a = 0; a < m; a++) {
         const uint32_t v = gbuf[base + (int64_t)a * 64];
@@ -873,12 +912,14 @@ __global__ void k_extras_fill_d2(const uint32_t* __restrict__ gbuf, const double
 }
 
 // neighbours.py:284-303 on the merged row (gather row + sorted extras, both with d2), with the
-// final smoothing lengths hf: keep d2 < (2 max(hf_i, hf_j))^2, into blocked rows (cap_out).
+// final smoothing lengths hf: keep d2 < (2 max(hf_i, hf_j))^2, into 64-row blocks at toff (each
+// block's capacity the longest merged row of its rows, so every kept entry has a slot).
 __global__ void __launch_bounds__(kBlock)
-k_tight_emit(const uint32_t* __restrict__ gbuf, const double* __restrict__ dbuf, int64_t cap,
-             const int32_t* __restrict__ rowcnt, const unsigned long long* __restrict__ eoff, uint32_t* __restrict__ ebuf,
-             double* __restrict__ edbuf, const double* __restrict__ hf, int64_t n, int32_t* __restrict__ out_len,
-             int32_t* __restrict__ out_idx, int64_t cap_out, long long* __restrict__ part)
+k_tight_emit(const uint32_t* __restrict__ gbuf, const double* __restrict__ dbuf,
+             const unsigned long long* __restrict__ boff, const int32_t* __restrict__ rowcnt,
+             const unsigned long long* __restrict__ eoff, uint32_t* __restrict__ ebuf, double* __restrict__ edbuf,
+             const double* __restrict__ hf, int64_t n, const unsigned long long* __restrict__ toff,
+             int32_t* __restrict__ out_len, int32_t* __restrict__ out_idx, long long* __restrict__ part)
 {
     int64_t i = xcd_tile() * (int64_t)blockDim.x + threadIdx.x;
     int32_t o = 0;
@@ -898,8 +939,9 @@ k_tight_emit(const uint32_t* __restrict__ gbuf, const double* __restrict__ dbuf,
             e[b + 1] = v;
             ed[b + 1] = dv;
         }
-        const int64_t base = ((i >> 6) * cap) * 64 + (i & 63);
-        const int64_t obase = ((i >> 6) * cap_out) * 64 + (i & 63);
+        const int64_t base = (int64_t)boff[i >> 6] + (i & 63);
+        const int64_t obase = (int64_t)toff[i >> 6] + (i & 63);
+        const int32_t cap_out = (int32_t)((toff[(i >> 6) + 1] - toff[i >> 6]) >> 6);
         const int32_t ng = rowcnt[i];
         const double hi = hf[i];
         int32_t p = 0;
@@ -2055,7 +2097,7 @@ int gather_stage(nb_ctx* ctx, hipStream_t s, const double* pos, const double* h,
     rc = scan_u64(ctx, s, ctx->ecount, ctx->eoff, n + 1);
     if (rc) return rc;
     NB_TIMED(ctx, s, "k_row_total", k_row_total, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, ctx->rowcnt, ctx->ecount, ctx->tot,
-                       ctx->part, n);
+                       ctx->part, n, (int32_t*)nullptr);
     finish_partials(ctx, s, grid_of(n + 1), 1, 1u, 2);  // [2] longest merged row
     rc = scan_u64(ctx, s, ctx->tot, ctx->offs, n + 1);
     if (rc) return rc;
@@ -2288,11 +2330,29 @@ extern "C" int32_t nb_permute_i64(nb_ctx* ctx, void* stream, const int64_t* src,
 
 
 // ------------------------------------------------------------------ fused step passes (host)
+// Exclusive offsets ctx->fb_* (nblk + 1 entries) of 64-row blocks sized by k_block_caps; their
+// total (entries) in *total. Synchronises with the stream.
+static int32_t block_offsets(nb_ctx* ctx, hipStream_t s, const int32_t* cnt, int cstride,
+                             const unsigned long long* add, int64_t n, int mode, int64_t cap0,
+                             unsigned long long* off, int64_t& total, const char* what)
+{
+    const int64_t nblk = (n + 63) / 64;
+    const int64_t wg = (nblk + 1 + 3) / 4;  // four blocks (wavefronts) per workgroup
+    hipLaunchKernelGGL(k_block_caps, dim3((unsigned)wg), dim3(kBlock), 0, s, cnt, cstride, add, n, mode, cap0,
+                       ctx->fb_sz, nblk);
+    int rc = scan_u64(ctx, s, ctx->fb_sz, off, nblk + 1);
+    if (rc) return rc;
+    unsigned long long t = 0;
+    rc = sync_copy(ctx, s, &t, off + nblk, sizeof(t), what);
+    total = (int64_t)t;
+    return rc;
+}
+
 extern "C" int32_t nb_hydro_density(nb_ctx* ctx, void* stream, const double* pos, const double* mass, double m_uniform,
-                                    const double* h_start, const double* h_search, int64_t n, int32_t kid,
-                                    double cnorm, double eta, double tol, double h_max, int32_t precision,
-                                    double* h_out, double* rho, double* omega, int32_t* iters, int32_t* flags,
-                                    int64_t* stats4)
+                                    const double* h_start, const double* h_search, int64_t n,
+                                    const int32_t* row_hint, int32_t* row_counts, int32_t kid, double cnorm,
+                                    double eta, double tol, double h_max, int32_t precision, double* h_out,
+                                    double* rho, double* omega, int32_t* iters, int32_t* flags, int64_t* stats4)
 {
     hipStream_t s = (hipStream_t)stream;
     if (!ctx || n <= 0 || !stats4 || n >= (int64_t)kMask) return ctx_fail(ctx, "nb_hydro_density: bad arguments");
@@ -2303,6 +2363,7 @@ extern "C" int32_t nb_hydro_density(nb_ctx* ctx, void* stream, const double* pos
     if (rc) return rc;
     const bool mixed = precision != 0;
     const bool masses = !(m_uniform > 0.0);
+    const int64_t nblk = (n + 63) / 64;
     if (ctx_reserve(ctx, (void**)&ctx->rowcnt, &ctx->rowcnt_bytes, sizeof(int32_t) * n) ||
         ctx_reserve(ctx, (void**)&ctx->unsorted, &ctx->unsorted_bytes, n) ||
         ctx_reserve(ctx, (void**)&ctx->pending, &ctx->pending_bytes, n) ||
@@ -2310,91 +2371,112 @@ extern "C" int32_t nb_hydro_density(nb_ctx* ctx, void* stream, const double* pos
         ctx_reserve(ctx, (void**)&ctx->eoff, &ctx->eoff_bytes, sizeof(unsigned long long) * (n + 1)) ||
         ctx_reserve(ctx, (void**)&ctx->efill, &ctx->efill_bytes, sizeof(unsigned int) * n) ||
         ctx_reserve(ctx, (void**)&ctx->tot, &ctx->tot_bytes, sizeof(unsigned long long) * (n + 1)) ||
-        ctx_reserve(ctx, (void**)&ctx->offs, &ctx->offs_bytes, sizeof(unsigned long long) * (n + 1)))
+        ctx_reserve(ctx, (void**)&ctx->offs, &ctx->offs_bytes, sizeof(unsigned long long) * (n + 1)) ||
+        ctx_reserve(ctx, (void**)&ctx->fb_sz, &ctx->fb_sz_bytes, sizeof(unsigned long long) * (nblk + 1)) ||
+        ctx_reserve(ctx, (void**)&ctx->fb_go, &ctx->fb_go_bytes, sizeof(unsigned long long) * (nblk + 1)) ||
+        ctx_reserve(ctx, (void**)&ctx->fb_to, &ctx->fb_to_bytes, sizeof(unsigned long long) * (nblk + 1)))
         return NB_ERR_HIP;
-    const int64_t nblk = (n + 63) / 64;
     const int64_t nb = grid_of(n);
-    int64_t cap = ctx->row_cap > 0 ? ctx->row_cap : 192;
     long long sc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     DensArgs A{mass, m_uniform, h_start, h_max, kid, cnorm, eta, tol, h_out, rho, omega, iters, flags, ctx->pending,
                ctx->part};
-    for (int attempt = 0; attempt < 8; attempt++) {
-        const size_t cells = (size_t)(nblk * cap * 64);
-        if (ctx_reserve(ctx, (void**)&ctx->gbuf, &ctx->gbuf_bytes, sizeof(uint32_t) * cells) ||
-            ctx_reserve(ctx, (void**)&ctx->dscratch, &ctx->dscratch_bytes, sizeof(double) * cells * (masses ? 2 : 1)))
+    // block capacities: from the hint (the previous pass's counts of the same particles), else a
+    // uniform first guess; a pass with rows over their block's capacity is repeated with exact ones
+    int64_t cells = 0;
+    for (int attempt = 0; attempt < 2; attempt++) {
+        const int mode = attempt ? 2 : (row_hint ? 1 : 0);
+        rc = block_offsets(ctx, s, attempt ? ctx->rowcnt : row_hint, attempt ? 1 : 2, nullptr, n, mode, 192,
+                           ctx->fb_go, cells, "nb_hydro_density: block capacities");
+        if (rc) return rc;
+        if (ctx_reserve(ctx, (void**)&ctx->gbuf, &ctx->gbuf_bytes, sizeof(uint32_t) * (size_t)cells) ||
+            ctx_reserve(ctx, (void**)&ctx->dscratch, &ctx->dscratch_bytes,
+                        sizeof(double) * (size_t)cells * (masses ? 2 : 1)))
             return NB_ERR_HIP;
-        if (hipMemsetAsync(ctx->ecount, 0, sizeof(unsigned long long) * (n + 1), s) != hipSuccess)
-            return ctx_fail(ctx, "memset");
+        hipLaunchKernelGGL(k_fill_i64, dim3((unsigned)grid_of(n + 1)), dim3(kBlock), 0, s, (long long*)ctx->ecount, 0ll,
+                           n + 1);
         double* mbuf = masses ? ctx->dscratch + cells : nullptr;
         if (mixed)
             NB_TIMED(ctx, s, "k_gather_solve", k_gather_solve<true>, dim3(nb), dim3(kBlock), 0, s, pos, h_search, n, g,
-                               ctx->cell_start, order_arg(ctx), ctx->gbuf, ctx->dscratch, mbuf, cap, ctx->rowcnt,
-                               ctx->unsorted, ctx->ecount, A);
+                               ctx->cell_start, order_arg(ctx), ctx->gbuf, ctx->dscratch, mbuf, ctx->fb_go,
+                               ctx->rowcnt, ctx->unsorted, ctx->ecount, A);
         else
             NB_TIMED(ctx, s, "k_gather_solve", k_gather_solve<false>, dim3(nb), dim3(kBlock), 0, s, pos, h_search, n, g,
-                               ctx->cell_start, order_arg(ctx), ctx->gbuf, ctx->dscratch, mbuf, cap, ctx->rowcnt,
-                               ctx->unsorted, ctx->ecount, A);
-        finish_partials(ctx, s, nb, 3, 1u, 0);  // [0] longest gather row, [1] unsorted rows, [2] rows pending
-        rc = sync_copy(ctx, s, sc, ctx->scal, sizeof(long long) * 3, "nb_hydro_density: gather failed");
+                               ctx->cell_start, order_arg(ctx), ctx->gbuf, ctx->dscratch, mbuf, ctx->fb_go,
+                               ctx->rowcnt, ctx->unsorted, ctx->ecount, A);
+        // [0] longest gather row, [1] unsorted rows, [2] rows pending, [3] rows over capacity
+        finish_partials(ctx, s, nb, 4, 1u, 0);
+        rc = sync_copy(ctx, s, sc, ctx->scal, sizeof(long long) * 4, "nb_hydro_density: gather failed");
         if (rc) return rc;
-        if (sc[0] <= cap) break;
-        cap = ((sc[0] + sc[0] / 8 + 15) / 16) * 16;
-        if (attempt == 7) return ctx_fail(ctx, "row capacity did not settle");
+        if (sc[3] == 0) break;
+        if (attempt == 1) return ctx_fail(ctx, "nb_hydro_density: rows exceed exact capacities");
     }
-    ctx->row_cap = (int32_t)cap;
+    ctx->row_cap = (int32_t)sc[0];
+    ctx->fb_gtotal = cells;
     ctx->last_unsorted = sc[1];
-    const size_t cells = (size_t)(nblk * cap * 64);
     double* mbuf = masses ? ctx->dscratch + cells : nullptr;
     if (sc[1] > 0)
-        NB_TIMED(ctx, s, "k_sort_rows_payload", k_sort_rows_payload, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->gbuf, ctx->dscratch, mbuf,
-                           cap, ctx->rowcnt, ctx->unsorted, n);
+        NB_TIMED(ctx, s, "k_sort_rows_payload", k_sort_rows_payload, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->gbuf,
+                           ctx->dscratch, mbuf, ctx->fb_go, ctx->rowcnt, ctx->unsorted, n);
     if (sc[2] > 0) {
         if (mixed)
-            NB_TIMED(ctx, s, "k_solve_pending", k_solve_pending<true>, dim3(grid_of(n)), dim3(kBlock), 0, s, h_search, n, ctx->dscratch,
-                               mbuf, cap, ctx->rowcnt, A);
+            NB_TIMED(ctx, s, "k_solve_pending", k_solve_pending<true>, dim3(grid_of(n)), dim3(kBlock), 0, s, h_search, n,
+                               ctx->dscratch, mbuf, ctx->fb_go, ctx->rowcnt, A);
         else
             NB_TIMED(ctx, s, "k_solve_pending", k_solve_pending<false>, dim3(grid_of(n)), dim3(kBlock), 0, s, h_search, n,
-                               ctx->dscratch, mbuf, cap, ctx->rowcnt, A);
+                               ctx->dscratch, mbuf, ctx->fb_go, ctx->rowcnt, A);
     }
     // extras (with d2) for the tightened rows; the longest merged row bounds the tight rows
     NB_TIMED(ctx, s, "k_row_total", k_row_total, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, ctx->rowcnt, ctx->ecount, ctx->tot,
-                       ctx->part, n);
+                       ctx->part, n, row_counts);
     finish_partials(ctx, s, grid_of(n + 1), 1, 1u, 3);  // [3] longest merged row
     rc = scan_u64(ctx, s, ctx->ecount, ctx->eoff, n + 1);
     if (rc) return rc;
     rc = scan_u64(ctx, s, ctx->tot, ctx->offs, n + 1);
     if (rc) return rc;
     hipLaunchKernelGGL(k_collect, dim3(1), dim3(64), 0, s, (long long*)ctx->scal + 4, ctx->eoff + n, ctx->offs + n);
-    rc = sync_copy(ctx, s, sc, ctx->scal, sizeof(long long) * 6, "nb_hydro_density: totals copy failed");
+    // tight-row blocks: capacity of each the longest merged row of its rows
+    const int64_t wg = (nblk + 1 + 3) / 4;
+    hipLaunchKernelGGL(k_block_caps, dim3((unsigned)wg), dim3(kBlock), 0, s, ctx->rowcnt, 1, ctx->ecount, n, 2,
+                       (int64_t)0, ctx->fb_sz, nblk);
+    rc = scan_u64(ctx, s, ctx->fb_sz, ctx->fb_to, nblk + 1);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_collect, dim3(1), dim3(64), 0, s, (long long*)ctx->scal + 6, ctx->fb_to + nblk,
+                       ctx->fb_to + nblk);
+    rc = sync_copy(ctx, s, sc, ctx->scal, sizeof(long long) * 7, "nb_hydro_density: totals copy failed");
     if (rc) return rc;
     const unsigned long long ne_total = (unsigned long long)sc[4];
     if (ctx_reserve(ctx, (void**)&ctx->ebuf, &ctx->ebuf_bytes, sizeof(uint32_t) * (ne_total + 1)) ||
         ctx_reserve(ctx, (void**)&ctx->edbuf, &ctx->edbuf_bytes, sizeof(double) * (ne_total + 1)))
         return NB_ERR_HIP;
-    if (hipMemsetAsync(ctx->efill, 0, sizeof(unsigned int) * n, s) != hipSuccess) return ctx_fail(ctx, "memset");
-    NB_TIMED(ctx, s, "k_extras_fill_d2", k_extras_fill_d2, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->gbuf, ctx->dscratch, cap,
-                       ctx->rowcnt, ctx->eoff, ctx->efill, ctx->ebuf, ctx->edbuf, n);
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)grid_of(n)), dim3(kBlock), 0, s, (uint32_t*)ctx->efill, 0u, n);
+    NB_TIMED(ctx, s, "k_extras_fill_d2", k_extras_fill_d2, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->gbuf, ctx->dscratch,
+                       ctx->fb_go, ctx->rowcnt, ctx->eoff, ctx->efill, ctx->ebuf, ctx->edbuf, n);
     if (hipGetLastError() != hipSuccess) return ctx_fail(ctx, "nb_hydro_density: kernel launch failed");
     ctx->lists_n = n;
     ctx->hydro_ready = 1;
     ctx->lists_maxrow = sc[3];
+    ctx->fb_ttotal = sc[6];
     stats4[0] = sc[0];   // longest gather row
     stats4[1] = sc[1];   // rows gathered unsorted
     stats4[2] = sc[5];   // entries of the search lists (incl. self)
-    stats4[3] = sc[3];   // longest merged row: the capacity the tightened rows need
+    stats4[3] = sc[6];   // entries of the tight rows' blocks (each block as long as its longest merged row)
     return NB_OK;
 }
 
-extern "C" int32_t nb_hydro_tight(nb_ctx* ctx, void* stream, const double* h_final, int32_t* out_len,
-                                  int32_t* out_index, int64_t row_cap, int64_t* total_out)
+extern "C" int32_t nb_hydro_tight(nb_ctx* ctx, void* stream, const double* h_final, int64_t* out_off,
+                                  int32_t* out_len, int32_t* out_index, int64_t index_cap, int64_t* total_out)
 {
     hipStream_t s = (hipStream_t)stream;
     if (!ctx || !ctx->hydro_ready || !total_out) return ctx_fail_code(ctx, NB_ERR_NOT_BUILT, "nb_hydro_tight: no density pass");
-    if (row_cap < ctx->lists_maxrow) return ctx_fail(ctx, "nb_hydro_tight: row capacity below the longest merged row");
+    if (index_cap < ctx->fb_ttotal) return ctx_fail(ctx, "nb_hydro_tight: index capacity below the tight rows' blocks");
+    if (!out_off || !out_len || !out_index) return ctx_fail(ctx, "nb_hydro_tight: bad arguments");
     const int64_t n = ctx->lists_n;
+    const int64_t nblk = (n + 63) / 64;
+    if (hipMemcpyAsync(out_off, ctx->fb_to, sizeof(int64_t) * (nblk + 1), hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return ctx_fail(ctx, "nb_hydro_tight: offset copy failed");
     NB_TIMED(ctx, s, "k_tight_emit", k_tight_emit, dim3(grid_of(n)), dim3(kBlock), 0, s, ctx->gbuf, ctx->dscratch,
-                       (int64_t)ctx->row_cap, ctx->rowcnt, ctx->eoff, ctx->ebuf, ctx->edbuf, h_final, n, out_len,
-                       out_index, row_cap, ctx->part);
+                       ctx->fb_go, ctx->rowcnt, ctx->eoff, ctx->ebuf, ctx->edbuf, h_final, n, ctx->fb_to, out_len,
+                       out_index, ctx->part);
     finish_partials(ctx, s, grid_of(n), 1, 0u, 8);
     long long t = 0;
     int rc = sync_copy(ctx, s, &t, (long long*)ctx->scal + 8, sizeof(t), "nb_hydro_tight: total copy failed");
@@ -2402,7 +2484,6 @@ extern "C" int32_t nb_hydro_tight(nb_ctx* ctx, void* stream, const double* h_fin
     *total_out = (int64_t)t;
     return NB_OK;
 }
-
 
 extern "C" int32_t nb_hydro_density_mixed(nb_ctx* ctx, void* stream, const double* pos, const double* mass,
                                           double m_uniform, const double* h_start, const double* h_search, int64_t n,

@@ -79,7 +79,8 @@ struct Box {
 };
 
 // Neighbour-list views of the pair passes. CSR rows (int64 offsets, the reference's layout),
-// 64-row blocks: entry k of row i at ((i / 64) cap + k) 64 + i % 64, so that the 64 lanes of a
+// 64-row blocks: entry k of row i at ((i / 64) cap + k) 64 + i % 64 (or, for blocks with
+// capacities of their own, at boff[i / 64] + 64 k + i % 64), so that the 64 lanes of a
 // wavefront reading entry k of their own rows touch one contiguous 256-byte segment, or
 // row-major rows (entry k of row i at i cap + k) for passes where a lane group shares a row.
 struct CsrView {
@@ -95,9 +96,13 @@ struct BlkView {
     const int32_t* idx;
     const int32_t* len;
     int64_t cap;
+    const int64_t* boff;  // non-null: block b starts at boff[b] (blocks with capacities of their own)
     static constexpr int64_t stride = 64;
     __device__ __forceinline__ int64_t count(int64_t i) const { return len[i]; }
-    __device__ __forceinline__ int64_t base(int64_t i) const { return ((i >> 6) * cap) * 64 + (i & 63); }
+    __device__ __forceinline__ int64_t base(int64_t i) const
+    {
+        return boff ? boff[i >> 6] + (i & 63) : ((i >> 6) * cap) * 64 + (i & 63);
+    }
     __device__ __forceinline__ int32_t at(int64_t i, int64_t k) const { return idx[base(i) + k * 64]; }
 };
 

@@ -64,6 +64,11 @@ struct nb_ctx {
     unsigned long long* rm_ec = nullptr; size_t rm_ec_bytes = 0;
     unsigned long long* rm_eo = nullptr; size_t rm_eo_bytes = 0;
     unsigned long long* rm_to = nullptr; size_t rm_to_bytes = 0;
+    // FP64 step lists: 64-row blocks with capacities of their own (sizes, gather and tight offsets)
+    unsigned long long* fb_sz = nullptr; size_t fb_sz_bytes = 0;
+    unsigned long long* fb_go = nullptr; size_t fb_go_bytes = 0;
+    unsigned long long* fb_to = nullptr; size_t fb_to_bytes = 0;
+    int64_t fb_gtotal = 0, fb_ttotal = 0;
     uint32_t* ebuf_s = nullptr; size_t ebuf_s_bytes = 0;  // extras in index order
     float* edbuf_s = nullptr; size_t edbuf_s_bytes = 0;
     int64_t rm_gtotal = 0;  // gather-row entries of the last mixed pass (start of the mass rows)

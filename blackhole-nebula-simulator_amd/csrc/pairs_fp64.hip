@@ -443,7 +443,7 @@ static int32_t launch_check(nb_ctx* ctx, const char* what)
             RowView v_{(L)->index, (L)->length, (L)->offset};                                           \
             hipLaunchKernelGGL(KERNEL<RowView>, dim3(GRID), dim3(kBlockP), 0, STREAM, __VA_ARGS__);      \
         } else if ((L)->length) {                                                                       \
-            BlkView v_{(L)->index, (L)->length, (L)->row_cap};                                          \
+            BlkView v_{(L)->index, (L)->length, (L)->row_cap, (L)->offset};                                          \
             hipLaunchKernelGGL(KERNEL<BlkView>, dim3(GRID), dim3(kBlockP), 0, STREAM, __VA_ARGS__);      \
         } else {                                                                                        \
             CsrView v_{(L)->offset, (L)->index};                                                        \

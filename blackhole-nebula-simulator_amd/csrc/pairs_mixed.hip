@@ -291,7 +291,7 @@ static inline Box box_of(const double* box3)
             RowView v_{(L)->index, (L)->length, (L)->offset};                                           \
             hipLaunchKernelGGL(KERNEL<RowView>, dim3(GRID), dim3(ctx->tile), 0, STREAM, __VA_ARGS__);    \
         } else if ((L)->length) {                                                                       \
-            BlkView v_{(L)->index, (L)->length, (L)->row_cap};                                          \
+            BlkView v_{(L)->index, (L)->length, (L)->row_cap, (L)->offset};                                          \
             hipLaunchKernelGGL(KERNEL<BlkView>, dim3(GRID), dim3(ctx->tile), 0, STREAM, __VA_ARGS__);    \
         } else {                                                                                        \
             CsrView v_{(L)->offset, (L)->index};                                                        \

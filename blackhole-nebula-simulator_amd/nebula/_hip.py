@@ -106,9 +106,9 @@ SIGNATURES = {
     "nb_permute": (_i32, [_vp, _vp, _vp, _vp, _i64, _i32, _vp]),
     "nb_permute_i64": (_i32, [_vp, _vp, _vp, _vp, _i64, _vp]),
     "nb_tighten_blocked": (_i32, [_vp, _vp, _vp, _vp, _i64, _PD, _vp, _vp, _i64, _vp, _vp, _P64]),
-    "nb_hydro_density": (_i32, [_vp, _vp, _vp, _vp, _f64, _vp, _vp, _i64, _i32, _f64, _f64, _f64, _f64, _i32, _vp,
-                                _vp, _vp, _vp, _vp, _P64]),
-    "nb_hydro_tight": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _P64]),
+    "nb_hydro_density": (_i32, [_vp, _vp, _vp, _vp, _f64, _vp, _vp, _i64, _vp, _vp, _i32, _f64, _f64, _f64, _f64,
+                                _i32, _vp, _vp, _vp, _vp, _vp, _P64]),
+    "nb_hydro_tight": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _P64]),
     "nb_hydro_density_mixed": (_i32, [_vp, _vp, _vp, _vp, _f64, _vp, _vp, _i64, _vp, _P64, _i32, _f64, _vp, _vp, _i32,
                                       _f64, _f64, _f64, _f64, _vp, _vp, _vp, _vp, _vp, _P64]),
     "nb_hydro_tight_mixed": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _P64]),
@@ -139,7 +139,7 @@ SIGNATURES = {
 }
 
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 
 def load_library() -> ctypes.CDLL:

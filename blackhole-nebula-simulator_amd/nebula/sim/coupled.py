@@ -233,9 +233,10 @@ def _hydro(st: GasState, cfg: CoupledConfig, v, dt_alpha: float, a_prev, h_start
     else:
         if st.cloud is not None or st.cell_factor != 1.0:
             raise ValueError("ensembles and tuned grids run in the mixed-precision mode")
+        counts = _dev.empty((n, 2), torch.int32)
         d, n_search = hydro_density_device(st.x, st.mass, st.h.clone(), hc.kernel, hc.eta, 1e-4, hc.h_max, hc.skin,
-                                           st.m_uniform, hc.precision, h_start_d=h_start,
-                                           index_buffer=buf)  # fmt: skip
+                                           st.m_uniform, hc.precision, h_start_d=h_start, index_buffer=buf,
+                                           row_hint=st.row_counts, row_counts=counts)  # fmt: skip
     p_gas = _dev.empty(n)
     cs = _dev.empty(n)
     nbad = torch.zeros(1, dtype=torch.int32, device=st.x.device) if fixed is None else fixed.status[7:]

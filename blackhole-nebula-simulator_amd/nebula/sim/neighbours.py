@@ -86,15 +86,19 @@ class Neighbours:
 
 
 class BlockedLists:
-    """Neighbour lists in 64-row blocks on the device (the step's own layout): row i has
-    ``length[i]`` entries, entry k at ((i // 64) * row_cap + k) * 64 + i % 64."""
+    """Neighbour lists in 64-row blocks on the device (the FP64 step's layout): row i has
+    ``length[i]`` entries, entry k at ((i // 64) * row_cap + k) * 64 + i % 64, or, when
+    ``block_offset`` (N // 64 + 1 int64 block starts) is given, at block_offset[i // 64] +
+    64 k + i % 64 (blocks with capacities of their own)."""
 
-    def __init__(self, length: torch.Tensor, index: torch.Tensor, row_cap: int, n_total: int, box=None):
+    def __init__(self, length: torch.Tensor, index: torch.Tensor, row_cap: int, n_total: int, box=None,
+                 block_offset: torch.Tensor | None = None):  # fmt: skip
         self.length = length
         self.index = index
         self.row_cap = int(row_cap)
         self.n_total = int(n_total)
         self.box = box
+        self.block_offset = block_offset
 
     @property
     def n_entries(self) -> int:
@@ -102,7 +106,7 @@ class BlockedLists:
 
     def desc(self) -> _hip.Lists:
         d = _hip.Lists()
-        d.offset = None
+        d.offset = None if self.block_offset is None else self.block_offset.data_ptr()
         d.index = self.index.data_ptr()
         d.length = self.length.data_ptr()
         d.row_cap = self.row_cap
@@ -114,12 +118,17 @@ class BlockedLists:
         """The same lists as a CSR Neighbours (host arrays)."""
         n = int(self.length.numel())
         ln = self.length.cpu().numpy().astype(np.int64)
-        idx = self.index.cpu().numpy().reshape(-1, self.row_cap, 64)
+        idx = self.index.cpu().numpy()
+        if self.block_offset is None:
+            start = np.arange(0, (n + 63) // 64, dtype=np.int64) * self.row_cap * 64
+        else:
+            start = self.block_offset.cpu().numpy()
         off = np.zeros(n + 1, np.int64)
         np.cumsum(ln, out=off[1:])
         out = np.empty(off[-1], np.int64)
         for i in range(n):
-            out[off[i]:off[i + 1]] = idx[i // 64, : ln[i], i % 64]
+            b = start[i // 64] + i % 64
+            out[off[i]:off[i + 1]] = idx[b : b + 64 * ln[i] : 64]
         return Neighbours(off, out, self.box)
 
 

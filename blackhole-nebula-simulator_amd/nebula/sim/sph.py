@@ -142,19 +142,16 @@ def compute_density_device(pos_d, mass_d, h_guess_d, box_arr, kernel_name, eta, 
     raise RuntimeError("smoothing lengths kept outgrowing the neighbour search")
 
 
-def _round_cap(cap: int) -> int:
-    """Row capacity rounded up to a multiple of 32, so the list buffers keep their size from step
-    to step (the caching allocator then reuses them instead of allocating anew)."""
-    return max(((cap + 31) // 32) * 32, 32)
-
-
 def hydro_density_device(pos_d, mass_d, h_guess_d, kernel_name, eta, tol, h_max, skin, m_uniform,
-                         precision="fp64", row_cap_hint=0, h_start_d=None, index_buffer=None):  # fmt: skip
+                         precision="fp64", h_start_d=None, index_buffer=None, row_hint=None,
+                         row_counts=None):  # fmt: skip
     """The density pass of the coupled step (open box): compute_density's rebuild rounds
     (sph.py:329-387) on the fused gather + h-rho solve kernel, then the tightened lists at the
-    final h, in 64-row blocks. h_guess_d is overwritten. Returns (DensityResult with the tight
-    lists as ``neighbours``, search-list entries)."""
-    from .neighbours import BlockedLists, _blocked_alloc
+    final h, in 64-row blocks with capacities of their own. h_guess_d is overwritten.
+    row_hint / row_counts: (N, 2) int32 device tensors, the previous pass's row counts of the
+    same particles (sizes the blocks) and this pass's (filled). Returns (DensityResult with the
+    tight lists as ``neighbours``, search-list entries)."""
+    from .neighbours import BlockedLists
 
     n = int(pos_d.shape[0])
     kid = KERNEL_IDS[kernel_name]
@@ -169,7 +166,8 @@ def hydro_density_device(pos_d, mass_d, h_guess_d, kernel_name, eta, tol, h_max,
     for builds in range(1, 13):
         start = h_start_d if (h_start_d is not None and builds == 1) else h_guess_d
         _hip.call("nb_hydro_density", _hip.ptr(pos_d), _hip.ptr(mass_d), float(m_uniform), _hip.ptr(start),
-                  _hip.ptr(h_search), n, kid, float(C_NORM[kid]), float(eta), float(tol), float(h_max),
+                  _hip.ptr(h_search), n, _hip.ptr(row_hint if builds == 1 or row_counts is None else row_counts),
+                  _hip.ptr(row_counts), kid, float(C_NORM[kid]), float(eta), float(tol), float(h_max),
                   PRECISIONS[precision], _hip.ptr(h), _hip.ptr(rho), _hip.ptr(omega), _hip.ptr(iters), _hip.ptr(flags),
                   stats.ctypes.data_as(_hip._P64))  # fmt: skip
         _hip.call("nb_grow_search", _hip.ptr(flags), _hip.ptr(h), None, n, float(np.inf), _hip.ptr(h_search),
@@ -178,11 +176,14 @@ def hydro_density_device(pos_d, mass_d, h_guess_d, kernel_name, eta, tol, h_max,
             break
     else:
         raise RuntimeError("smoothing lengths kept outgrowing the neighbour search")
-    cap = _round_cap(int(stats[3]))
-    ln, idx = _blocked_alloc(n, cap, index_buffer)
+    entries = max(int(stats[3]), 1)
+    ln = _dev.empty(n, torch.int32)
+    boff = _dev.empty((n + 63) // 64 + 1, torch.int64)
+    idx = _dev.empty(entries, torch.int32) if index_buffer is None else index_buffer.take(entries)
     total = np.zeros(1, np.int64)
-    _hip.call("nb_hydro_tight", _hip.ptr(h), _hip.ptr(ln), _hip.ptr(idx), cap, total.ctypes.data_as(_hip._P64))
-    tight = BlockedLists(ln, idx, cap, int(total[0]), None)
+    _hip.call("nb_hydro_tight", _hip.ptr(h), _hip.ptr(boff), _hip.ptr(ln), _hip.ptr(idx), entries,
+              total.ctypes.data_as(_hip._P64))  # fmt: skip
+    tight = BlockedLists(ln, idx, 0, int(total[0]), None, block_offset=boff)
     return DensityResult(h, rho, omega, iters, flags, tight, None, builds), int(stats[2])
 
 

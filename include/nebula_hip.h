@@ -54,10 +54,12 @@ typedef struct nb_force_params {
     double eta_fold;
 } nb_force_params;
 
-/* A neighbour list: CSR when length == NULL; else rows in 64-row blocks of capacity row_cap
- * (row_major 0: entry k of row i at ((i / 64) row_cap + k) 64 + i % 64) or rows of their own
- * capacity (row_major 1: entry k of row i at offset[i] + k, k < length[i]; the mixed-precision
- * step's layout, rows not sorted, row_cap unused).
+/* A neighbour list: CSR when length == NULL; else rows in 64-row blocks (row_major 0), of
+ * capacity row_cap (offset NULL: entry k of row i at ((i / 64) row_cap + k) 64 + i % 64) or of
+ * capacities of their own (offset = N / 64 + 1 block starts, the FP64 step's layout: entry k of
+ * row i at offset[i / 64] + 64 k + i % 64), or rows of their own capacity (row_major 1: entry k
+ * of row i at offset[i] + k, k < length[i]; the mixed-precision step's layout, rows not sorted,
+ * row_cap unused).
  * n_entries: CSR offset[N], or the size of the entry array (the size of per-entry scratch).
  * n_rows: N, the particles the lists cover (0 = not given; the mixed-precision passes need it
  * when they run on a row subset). */
@@ -148,16 +150,24 @@ int32_t nb_tighten_blocked(nb_ctx *ctx, void *stream, const double *pos, const d
  * cell grid at search lengths h_search, gather of every row, and each row's h-rho solve right
  * after it, summing over the gathered d2 (FP64 in the reference's order for precision 0, FP32
  * with compensated sums for precision 1); the lists stay in the context for nb_hydro_tight.
+ * The gathered rows sit in 64-row blocks with capacities of their own, sized from row_hint
+ * (N x 2 int32, the previous pass's row_counts of the same particles; NULL: a uniform first
+ * guess) and gathered again with exact capacities when a row does not fit. row_counts (N x 2
+ * int32, may be NULL) receives this pass's (gather row, extras) counts per particle.
  * Outputs per particle; stats4 (host) = [longest gather row, rows gathered unsorted, search-list
- * entries, longest merged row]. Rows flagged 2 must be redone by the caller (sph.py:382-386). */
+ * entries, entries of the tight rows' blocks (the index capacity nb_hydro_tight needs)]. Rows
+ * flagged 2 must be redone by the caller (sph.py:382-386). */
 int32_t nb_hydro_density(nb_ctx *ctx, void *stream, const double *pos, const double *mass, double m_uniform,
-                         const double *h_start, const double *h_search, int64_t n, int32_t kid, double cnorm,
-                         double eta, double tol, double h_max, int32_t precision, double *h_out, double *rho,
-                         double *omega, int32_t *iters, int32_t *flags, int64_t *stats4);
+                         const double *h_start, const double *h_search, int64_t n, const int32_t *row_hint,
+                         int32_t *row_counts, int32_t kid, double cnorm, double eta, double tol, double h_max,
+                         int32_t precision, double *h_out, double *rho, double *omega, int32_t *iters,
+                         int32_t *flags, int64_t *stats4);
 /* tighten (neighbours.py:306-323) of the last nb_hydro_density lists at the final smoothing
- * lengths, straight into blocked rows (row_cap >= the longest merged row). */
-int32_t nb_hydro_tight(nb_ctx *ctx, void *stream, const double *h_final, int32_t *out_len, int32_t *out_index,
-                       int64_t row_cap, int64_t *total_out);
+ * lengths, straight into 64-row blocks with capacities of their own: out_off (N / 64 + 1 int64,
+ * device) receives the blocks' starts (the nb_lists offset of that layout), out_index holds
+ * index_cap >= stats4[3] entries. */
+int32_t nb_hydro_tight(nb_ctx *ctx, void *stream, const double *h_final, int64_t *out_off, int32_t *out_len,
+                       int32_t *out_index, int64_t index_cap, int64_t *total_out);
 
 /* ------------------------------------------------------------------ mixed-precision step passes */
 /* The density pass of coupled.step for the mixed-precision mode (open box): the same search

@@ -24,7 +24,7 @@ def test_library_exports_every_declared_symbol():
     check("header declares entry points", True, len(names) > 20)
     check("every declared entry point exported", [], missing)
     check("binding declares exactly the header's entry points", names, sorted(_hip.SIGNATURES))
-    check("ABI version", 3, lib.nb_abi_version())
+    check("ABI version", 4, lib.nb_abi_version())
 
 
 def test_product_never_imports_the_oracle():

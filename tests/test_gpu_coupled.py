@@ -132,6 +132,59 @@ def test_blocked_lists_grid_order_and_tighten(rng):
     del torch
 
 
+@pytest.mark.parametrize("masses", ["uniform", "per-particle"])
+def test_fp64_step_lists_in_blocks_of_their_own_capacity(masses):
+    """The FP64 step's density pass sizes its 64-row blocks from the previous pass's row counts
+    (two per particle), gathers again at exact capacities when a row outgrows its block, and
+    writes the tight rows into blocks as long as their longest merged row. Nothing depends on the
+    hint (none, the pass's own counts, a hint far too small): outputs and lists are bit-identical,
+    and the tight rows are the reference's lists at the final h (neighbours.py:240-281: r_ij <
+    2 max(h_i, h_j), which a search at h_search >= h holds in full)."""
+    import torch
+
+    from nebula import _dev
+    from nebula.ics import stream_crash
+    from nebula.sim import coupled, sph
+
+    from oracle import neighbours as on
+
+    ic = stream_crash(20000, length=400.0, seed=5)
+    n = len(ic.h)
+    mass = ic.mass if masses == "uniform" else ic.mass * np.random.default_rng(2).uniform(0.5, 1.5, n)
+    hc = coupled.baseline_config().hydro
+    x_d, m_d = _dev.f64(ic.x), _dev.f64(mass)
+    mu = _dev.uniform_mass(m_d)
+
+    def run(hint):
+        counts = _dev.empty((n, 2), torch.int32)
+        d, _ = sph.hydro_density_device(x_d, m_d, _dev.f64(ic.h), hc.kernel, hc.eta, 1e-4, hc.h_max, hc.skin, mu,
+                                        "fp64", row_hint=hint, row_counts=counts)  # fmt: skip
+        return d, counts
+
+    print(f"fp64 blocks ({masses} masses): no hint, own counts, hint too small", flush=True)
+    d0, c0 = run(None)
+    d1, c1 = run(c0)
+    d2, c2 = run(torch.zeros_like(c0))
+    for name, d, c in (("own counts", d1, c1), ("hint too small", d2, c2)):
+        for k in ("h", "rho", "omega", "iterations", "flags"):
+            bit_equal(f"{name}: {k}", getattr(d0, k).cpu().numpy(), getattr(d, k).cpu().numpy())
+        bit_equal(f"{name}: row counts", c0.cpu().numpy(), c.cpu().numpy())
+        t0, t = d0.neighbours.to_csr(), d.neighbours.to_csr()
+        bit_equal(f"{name}: tight offsets", t0.offset, t.offset)
+        bit_equal(f"{name}: tight indices", t0.index, t.index)
+    ref = on.find_neighbours_grid(ic.x, d0.h.cpu().numpy())
+    t0 = d0.neighbours.to_csr()
+    bit_equal("tight rows = reference lists at the final h: offsets", ref.offset, t0.offset)
+    bit_equal("tight rows = reference lists at the final h: indices", ref.index, t0.index)
+    boff = d0.neighbours.block_offset.cpu().numpy()
+    ln = d0.neighbours.length.cpu().numpy()
+    cap = np.diff(boff) // 64
+    check("block starts are multiples of 64 apart", True, bool(np.all(np.diff(boff) % 64 == 0)))
+    check("every tight row fits its block", True, bool(np.all(ln <= np.repeat(cap, 64)[:n])))
+    check("merged rows (gather + extras) hold the tight rows", True, bool(np.all(c0.cpu().numpy().sum(1) >= ln)))
+    print(f"tight entries allocated {boff[-1]} vs uniform blocks {((n + 63) // 64) * 64 * int(cap.max())}")
+
+
 def test_torus_shakura_sunyaev_cooling():
     """The torus workload's physics: Shakura-Sunyaev viscosity (alpha 0.1, stress form, which
     adds the N-S gradient and force passes) with power-law cooling, against the oracle; then two

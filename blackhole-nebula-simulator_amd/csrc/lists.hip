@@ -167,23 +167,12 @@ __device__ __forceinline__ uint32_t cell_key(const double* pos, int64_t i, const
 }
 
 // Bounding box, largest h and the fixed-point sum of h (as k_bbox / k_hsum) of each segment
-// [off[s], off[s + 1]): one workgroup per segment; out[9 s + 0..6] box and h_max, the u64 sum in
-// out[9 s + 8].
-__global__ void __launch_bounds__(kBlock)
-k_seg_bbox(const double* __restrict__ pos, const double* __restrict__ h, const int64_t* __restrict__ off,
-           double* __restrict__ out)
+// [off[s], off[s + 1]), in `parts` workgroups per segment (grid nseg x parts): k_seg_bbox_part
+// writes each workgroup's box and h_max (8 doubles per workgroup), k_seg_bbox_final combines them
+// into out[9 s + 0..6]; k_seg_hsum_part / _final do the same for the u64 sum (out[9 s + 8]).
+// Minima, maxima and integer sums do not depend on the split.
+__device__ __forceinline__ void bbox_block_reduce(double* v, double (*sh)[kBlock])
 {
-    const int64_t a = off[blockIdx.x], b = off[blockIdx.x + 1];
-    double v[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0};
-    for (int64_t i = a + threadIdx.x; i < b; i += blockDim.x) {
-        for (int c = 0; c < 3; c++) {
-            v[c] = fmin(v[c], pos[3 * i + c]);
-            v[3 + c] = fmax(v[3 + c], pos[3 * i + c]);
-        }
-        v[6] = fmax(v[6], h[i]);
-    }
-    __shared__ double sh[7][kBlock];
-    __shared__ unsigned long long su[kBlock];
     for (int k = 0; k < 7; k++) sh[k][threadIdx.x] = v[k];
     __syncthreads();
     for (int st = blockDim.x / 2; st > 0; st >>= 1) {
@@ -193,21 +182,80 @@ k_seg_bbox(const double* __restrict__ pos, const double* __restrict__ h, const i
         }
         __syncthreads();
     }
-    const double hmax = sh[6][0];
-    const double scale = hmax > 0.0 ? 4294967296.0 / hmax : 0.0;
-    unsigned long long hs = 0;
-    for (int64_t i = a + threadIdx.x; i < b; i += blockDim.x) hs += (unsigned long long)(h[i] * scale);
-    su[threadIdx.x] = hs;
+}
+
+__device__ __forceinline__ unsigned long long u64_block_sum(unsigned long long v, unsigned long long* su)
+{
+    su[threadIdx.x] = v;
     __syncthreads();
     for (int st = blockDim.x / 2; st > 0; st >>= 1) {
         if ((int)threadIdx.x < st) su[threadIdx.x] += su[threadIdx.x + st];
         __syncthreads();
     }
+    return su[0];
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_seg_bbox_part(const double* __restrict__ pos, const double* __restrict__ h, const int64_t* __restrict__ off,
+                int parts, double* __restrict__ part)
+{
+    const int64_t a = off[blockIdx.x], b = off[blockIdx.x + 1];
+    double v[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0};
+    for (int64_t i = a + (int64_t)blockIdx.y * blockDim.x + threadIdx.x; i < b; i += (int64_t)parts * blockDim.x) {
+        for (int c = 0; c < 3; c++) {
+            v[c] = fmin(v[c], pos[3 * i + c]);
+            v[3 + c] = fmax(v[3 + c], pos[3 * i + c]);
+        }
+        v[6] = fmax(v[6], h[i]);
+    }
+    __shared__ double sh[7][kBlock];
+    bbox_block_reduce(v, sh);
+    if (threadIdx.x == 0) {
+        double* o = part + 8 * ((int64_t)blockIdx.x * parts + blockIdx.y);
+        for (int k = 0; k < 7; k++) o[k] = sh[k][0];
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_seg_bbox_final(const double* __restrict__ part, int parts,
+                                                           double* __restrict__ out)
+{
+    double v[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0};
+    for (int q = threadIdx.x; q < parts; q += blockDim.x) {
+        const double* o = part + 8 * ((int64_t)blockIdx.x * parts + q);
+        for (int k = 0; k < 3; k++) v[k] = fmin(v[k], o[k]);
+        for (int k = 3; k < 7; k++) v[k] = fmax(v[k], o[k]);
+    }
+    __shared__ double sh[7][kBlock];
+    bbox_block_reduce(v, sh);
     if (threadIdx.x == 0) {
         for (int k = 0; k < 7; k++) out[9 * blockIdx.x + k] = sh[k][0];
         out[9 * blockIdx.x + 7] = 0.0;
-        ((unsigned long long*)out)[9 * blockIdx.x + 8] = su[0];
     }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_seg_hsum_part(const double* __restrict__ h, const int64_t* __restrict__ off, const double* __restrict__ out,
+                int parts, unsigned long long* __restrict__ hpart)
+{
+    const int64_t a = off[blockIdx.x], b = off[blockIdx.x + 1];
+    const double hmax = out[9 * blockIdx.x + 6];
+    const double scale = hmax > 0.0 ? 4294967296.0 / hmax : 0.0;
+    unsigned long long hs = 0;
+    for (int64_t i = a + (int64_t)blockIdx.y * blockDim.x + threadIdx.x; i < b; i += (int64_t)parts * blockDim.x)
+        hs += (unsigned long long)(h[i] * scale);
+    __shared__ unsigned long long su[kBlock];
+    const unsigned long long t = u64_block_sum(hs, su);
+    if (threadIdx.x == 0) hpart[(int64_t)blockIdx.x * parts + blockIdx.y] = t;
+}
+
+__global__ void __launch_bounds__(kBlock) k_seg_hsum_final(const unsigned long long* __restrict__ hpart, int parts,
+                                                           double* __restrict__ out)
+{
+    unsigned long long hs = 0;
+    for (int q = threadIdx.x; q < parts; q += blockDim.x) hs += hpart[(int64_t)blockIdx.x * parts + q];
+    __shared__ unsigned long long su[kBlock];
+    const unsigned long long t = u64_block_sum(hs, su);
+    if (threadIdx.x == 0) ((unsigned long long*)out)[9 * blockIdx.x + 8] = t;
 }
 
 __device__ __forceinline__ uint32_t seg_key(const double* pos, int64_t i, const GridSeg* segs, const int32_t* cloud)
@@ -1952,6 +2000,35 @@ int build_grid(nb_ctx* ctx, hipStream_t s, const double* pos, const double* h, i
 // Grids of the segments of an ensemble (open boxes), each planned as build_grid plans one grid
 // from its own box and search lengths, numbered one after the other (GridSeg.base), and the
 // particles' order by global cell id (ctx->order, cell_start, order_ident as build_grid). cloud[i]:
+// Workgroups per segment of the segment box passes, and the bytes of their partials.
+static int seg_parts(int64_t n, int64_t nseg)
+{
+    const int64_t p = (n + nseg * 8192 - 1) / (nseg * 8192);
+    return (int)std::max<int64_t>(1, std::min<int64_t>(128, p));
+}
+
+static size_t seg_part_bytes(int64_t n, int64_t nseg)
+{
+    return (size_t)nseg * (size_t)seg_parts(n, nseg) * (8 * sizeof(double) + sizeof(unsigned long long));
+}
+
+// Each segment's box, h_max and fixed-point h sum into out (9 values per segment), from the
+// partials in ctx->segpart (seg_part_bytes reserved).
+static void launch_seg_bbox(nb_ctx* ctx, hipStream_t s, const double* pos, const double* h, int64_t n,
+                            const int64_t* off_dev, int32_t nseg, double* out)
+{
+    const int parts = seg_parts(n, nseg);
+    double* part = (double*)ctx->segpart;
+    unsigned long long* hpart = (unsigned long long*)(part + 8 * (size_t)nseg * parts);
+    hipLaunchKernelGGL(k_seg_bbox_part, dim3((unsigned)nseg, (unsigned)parts), dim3(kBlock), 0, s, pos, h, off_dev,
+                       parts, part);
+    hipLaunchKernelGGL(k_seg_bbox_final, dim3((unsigned)nseg), dim3(kBlock), 0, s, (const double*)part, parts, out);
+    hipLaunchKernelGGL(k_seg_hsum_part, dim3((unsigned)nseg, (unsigned)parts), dim3(kBlock), 0, s, h, off_dev,
+                       (const double*)out, parts, hpart);
+    hipLaunchKernelGGL(k_seg_hsum_final, dim3((unsigned)nseg), dim3(kBlock), 0, s,
+                       (const unsigned long long*)hpart, parts, out);
+}
+
 // the segment of particle i; off (host, nseg + 1): the segments' particle ranges.
 int build_grid_seg(nb_ctx* ctx, hipStream_t s, const double* pos, const double* h, int64_t n, const int32_t* cloud,
                    const int64_t* off, int32_t nseg, double cell_factor, bool check_sorted, int64_t& ncells)
@@ -1965,12 +2042,13 @@ int build_grid_seg(nb_ctx* ctx, hipStream_t s, const double* pos, const double* 
         if (off[k + 1] <= off[k]) return ctx_fail(ctx, "segmented grid: empty segment");
     if (reserve_reductions(ctx, n) ||
         ctx_reserve(ctx, (void**)&ctx->red, &ctx->red_bytes, sizeof(double) * 9 * nseg + sizeof(int64_t) * (nseg + 1)) ||
-        ctx_reserve(ctx, (void**)&ctx->segs, &ctx->segs_bytes, sizeof(GridSeg) * nseg))
+        ctx_reserve(ctx, (void**)&ctx->segs, &ctx->segs_bytes, sizeof(GridSeg) * nseg) ||
+        ctx_reserve(ctx, &ctx->segpart, &ctx->segpart_bytes, seg_part_bytes(n, nseg)))
         return NB_ERR_HIP;
     int64_t* off_dev = (int64_t*)(ctx->red + 9 * nseg);
     if (hipMemcpyAsync(off_dev, off, sizeof(int64_t) * (nseg + 1), hipMemcpyHostToDevice, s) != hipSuccess)
         return ctx_fail(ctx, "segmented grid: offset copy failed");
-    hipLaunchKernelGGL(k_seg_bbox, dim3(nseg), dim3(kBlock), 0, s, pos, h, off_dev, ctx->red);
+    launch_seg_bbox(ctx, s, pos, h, n, off_dev, nseg, ctx->red);
     std::vector<double> bb((size_t)9 * nseg);
     int rc = sync_copy(ctx, s, bb.data(), ctx->red, sizeof(double) * 9 * nseg, "segmented grid: box copy failed");
     if (rc) return rc;
@@ -2719,7 +2797,8 @@ int build_grid_seg_fixed(nb_ctx* ctx, hipStream_t s, const double* pos, const do
     ctx->lists_ready = 0;
     ctx->hydro_ready = 0;
     ctx->hydro_mixed_ready = 0;
-    hipLaunchKernelGGL(k_seg_bbox, dim3(nseg), dim3(kBlock), 0, s, pos, h, off_dev, ctx->red);
+    if (ctx->segpart_bytes < seg_part_bytes(n, nseg)) return ctx_fail(ctx, "fixed pass: box partials not reserved");
+    launch_seg_bbox(ctx, s, pos, h, n, off_dev, nseg, ctx->red);
     hipLaunchKernelGGL(k_plan_seg, dim3(1), dim3(64), 0, s, ctx->red, off_dev, nseg, cell_factor, (double)ctx->fx_cells,
                        (double)kMaxAxisCells, kMaxTotalCells, (GridSeg*)ctx->segs, status);
     const int64_t nc1 = ctx->fx_cells + 1;
@@ -2752,6 +2831,7 @@ extern "C" int32_t nb_fixed_reserve(nb_ctx* ctx, int64_t n, int32_t nseg, int64_
     if (reserve_reductions(ctx, n) ||
         ctx_reserve(ctx, (void**)&ctx->red, &ctx->red_bytes, sizeof(double) * 9 * (size_t)nseg + 8 * (size_t)(nseg + 1) + 4096) ||
         ctx_reserve(ctx, (void**)&ctx->segs, &ctx->segs_bytes, sizeof(GridSeg) * (size_t)nseg) ||
+        ctx_reserve(ctx, &ctx->segpart, &ctx->segpart_bytes, seg_part_bytes(n, nseg)) ||
         ctx_reserve(ctx, (void**)&ctx->cell_id, &ctx->cell_id_bytes, sizeof(uint32_t) * n) ||
         ctx_reserve(ctx, (void**)&ctx->order, &ctx->order_bytes, sizeof(uint32_t) * n) ||
         ctx_reserve(ctx, (void**)&ctx->keys_out, &ctx->keys_out_bytes, sizeof(uint32_t) * n) ||

@@ -428,6 +428,19 @@ extern "C" int32_t nb_dt_minima(nb_ctx* ctx, void* stream, const double* h, cons
     return reduce_two_level(ctx, s, 4, 0, nparts, out4);
 }
 
+extern "C" int32_t nb_dt_minima_dev(nb_ctx* ctx, void* stream, const double* h, const double* vsig,
+                                    const double* acc, const double* nu, double c_visc, const double* x, int64_t n,
+                                    double* out4_dev)
+{
+    if (!ctx || n <= 0 || !out4_dev || !h || !vsig || !acc || !x) return ctx_fail(ctx, "nb_dt_minima_dev: bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    int nparts = (int)std::min<int64_t>(1024, (n + kBlockS - 1) / kBlockS);
+    if (ctx_reserve(ctx, (void**)&ctx->red, &ctx->red_bytes, sizeof(double) * 4 * (nparts + 1))) return NB_ERR_HIP;
+    hipLaunchKernelGGL(k_dt_partial, dim3(nparts), dim3(kBlockS), 0, s, h, vsig, acc, nu, c_visc, x, n, ctx->red);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlockS), 0, s, ctx->red, nparts, 4, 0, out4_dev);
+    return launch_check_s(ctx, "nb_dt_minima_dev");
+}
+
 extern "C" int32_t nb_dt_minima_seg(nb_ctx* ctx, void* stream, const double* h, const double* vsig,
                                     const double* acc, const double* nu, double c_visc, const double* x,
                                     const int64_t* seg_off, int32_t nseg, double* out)

@@ -38,13 +38,26 @@ STATUS_NAMES = ("gather rows", "extras", "tight index", "grid cells", "h outgrew
 
 
 class _Fixed:
-    """The fixed-capacity density of a recorded step (coupled._hydro's ``fixed`` hook)."""
+    """The fixed-capacity passes of a step with no host synchronisation (the ``fixed`` hook of
+    coupled.step / coupled._hydro): grid order and density of the systems seg_off_dev delimits
+    (cloud: the particles' system, or the state's own ``cloud``), what does not fit counted in
+    ``status``."""
 
-    def __init__(self, status, seg_off_dev, nseg, index):
+    def __init__(self, status, seg_off_dev, nseg, index, cloud=None):
         self.status = status
         self.seg_off_dev = seg_off_dev
         self.nseg = nseg
         self.index = index
+        self.cloud = cloud
+        self.pairs = None
+
+    def order(self, x, h_search, cell_factor):
+        n = int(x.shape[0])
+        order = _dev.empty(n, torch.int32)
+        _hip.call("nb_grid_order_seg_fixed", _hip.ptr(x), _hip.ptr(h_search), n, _hip.ptr(self.cloud),
+                  _hip.ptr(self.seg_off_dev), self.nseg, float(cell_factor), _hip.ptr(self.status),
+                  _hip.ptr(order))  # fmt: skip
+        return order
 
     def density(self, st, cfg, h_start, counts):
         hc = cfg.hydro
@@ -57,7 +70,8 @@ class _Fixed:
         flags = _dev.empty(n, torch.int32)
         start = h_start if h_start is not None else h_guess
         _hip.call("nb_hydro_density_mixed_fixed", _hip.ptr(st.x), _hip.ptr(st.mass), float(st.m_uniform),
-                  _hip.ptr(start), _hip.ptr(h_search), n, _hip.ptr(st.cloud), _hip.ptr(self.seg_off_dev), self.nseg,
+                  _hip.ptr(start), _hip.ptr(h_search), n, _hip.ptr(st.cloud if self.cloud is None else self.cloud),
+                  _hip.ptr(self.seg_off_dev), self.nseg,
                   float(st.cell_factor), _hip.ptr(st.row_counts), _hip.ptr(counts), kid, float(C_NORM[kid]),
                   float(hc.eta), 1e-4, float(hc.h_max), _hip.ptr(h), _hip.ptr(rho), _hip.ptr(omega), _hip.ptr(iters),
                   _hip.ptr(flags), _hip.ptr(self.status))  # fmt: skip

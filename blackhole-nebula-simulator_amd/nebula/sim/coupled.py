@@ -172,8 +172,11 @@ def _search_lengths(cfg: CoupledConfig, h):
     return out
 
 
-def grid_permutation(cfg: CoupledConfig, x, h, st: GasState | None = None):
-    """Order of the neighbour grid the next density pass will build (search lengths from h)."""
+def grid_permutation(cfg: CoupledConfig, x, h, st: GasState | None = None, fixed=None):
+    """Order of the neighbour grid the next density pass will build (search lengths from h).
+    fixed: the fixed-capacity passes (no host synchronisation; see _hydro)."""
+    if fixed is not None:
+        return fixed.order(x, _search_lengths(cfg, h), 1.0 if st is None else st.cell_factor)
     if st is not None and st.cloud is not None:
         return grid_order_seg(x, _search_lengths(cfg, h), st.cloud, st.seg_off, st.cell_factor)
     return grid_order(x, _search_lengths(cfg, h), None, 1.0 if st is None else st.cell_factor)
@@ -197,9 +200,16 @@ def p_from_v(cfg: CoupledConfig, x: np.ndarray, v: np.ndarray) -> np.ndarray:
     return f * v + g * rdot * x / r
 
 
-def gravity_flow(cfg: CoupledConfig, x, p, dt):
-    """coupled.py:131-142 (GN generalized leapfrog, or a straight drift without the hole)."""
-    if _gn(cfg):
+def gravity_flow(cfg: CoupledConfig, x, p, dt, fixed=None):
+    """coupled.py:131-142 (GN generalized leapfrog, or a straight drift without the hole). fixed:
+    failures are counted in fixed.status[6] instead of raising (no host synchronisation)."""
+    if _gn(cfg) and fixed is not None:
+        n = int(x.shape[0])
+        x1, p1 = _dev.empty((n, 3)), _dev.empty((n, 3))
+        iters = _dev.empty(n, torch.int32)
+        _hip.call("nb_glf_step", _hip.ptr(x), _hip.ptr(p), None, float(dt), n, _hip.ptr(x1), _hip.ptr(p1),
+                  _hip.ptr(iters), _hip.ptr(fixed.status[6:]))  # fmt: skip
+    elif _gn(cfg):
         x1, p1, _ = glf_device(x, p, float(dt))
     else:
         x1 = _axpy(x, p, dt)
@@ -212,8 +222,9 @@ def gravity_flow(cfg: CoupledConfig, x, p, dt):
 def _hydro(st: GasState, cfg: CoupledConfig, v, dt_alpha: float, a_prev, h_start=None, fixed=None) -> GasState:
     """coupled.py:145-195 on device tensors. h_start: the Newton solve's starting h (default:
     the previous h, as in the reference); the search lists always come from the previous h.
-    fixed: the fixed-capacity passes of a step recorded for replay (nebula.sim.captured): one
-    search round, no host synchronisation; what does not fit goes to fixed.status."""
+    fixed: the fixed-capacity passes (nebula.sim.captured: a step recorded for replay, or one
+    checked after the fact by nebula.sim.stepper): one search round, no host synchronisation;
+    what does not fit goes to fixed.status."""
     hc = cfg.hydro
     if hc.isothermal_cs is not None or hc.eos != "ideal":
         raise ValueError("the GPU step implements the ideal-gas equation of state")
@@ -339,30 +350,38 @@ def stable_dt(st: GasState, cfg: CoupledConfig) -> float:
     """Shared step: min over particles of the Courant, force, viscous and orbit criteria
     (coupled.py:213-226). The minima are taken on the device (exact in any order); r^1.5 of the
     smallest radius equals the smallest r^1.5 (pow is monotone)."""
-    hc = cfg.hydro
     n = int(st.x.shape[0])
     out = np.zeros(4)
     _hip.call("nb_dt_minima", _hip.ptr(st.h), _hip.ptr(st.vsig_max), _hip.ptr(st.a_hydro), _hip.ptr(st.nu), 0.25,
               _hip.ptr(st.x), n, out.ctypes.data_as(_hip._PD))  # fmt: skip
-    dt = hc.courant * out[0]
+    return dt_from_minima(out, cfg, st.nu is not None)
+
+
+def dt_from_minima(out, cfg: CoupledConfig, viscous: bool) -> float:
+    """stable_dt from the four device minima (min h / vsig, min sqrt(h / |a|), min c h^2 / nu,
+    min |x|)."""
+    hc = cfg.hydro
+    dt = hc.courant * float(out[0])
     if np.isfinite(out[1]):
-        dt = min(dt, hc.c_force * out[1])
-    if st.nu is not None:
+        dt = min(dt, hc.c_force * float(out[1]))
+    if viscous:
         dt = min(dt, float(out[2]))
     if cfg.hole:
         dt = min(dt, cfg.eta_orb * float(out[3]) ** 1.5)  # scalar (libm) pow, as the oracle
     return float(dt)
 
 
-def step(st: GasState, cfg: CoupledConfig, dt: float) -> GasState:
+def step(st: GasState, cfg: CoupledConfig, dt: float, fixed=None) -> GasState:
     """One Strang-split step (coupled.py:238-258), with optional power-law cooling in the same
-    place of the split as the reference's beta cooling."""
+    place of the split as the reference's beta cooling. fixed: the fixed-capacity passes of
+    nebula.sim.stepper (no host synchronisation; the learned start's pairs are handed to
+    fixed.pairs for the caller to log once the step is known to be good)."""
     hc = cfg.hydro
     n = int(st.x.shape[0])
     half = 0.5 * dt
     p_half = _axpy(st.p, st.a_hydro, half)
     u_half = _axpy(st.u, st.dudt, half)
-    x1, p_flow = gravity_flow(cfg, st.x, p_half, dt)
+    x1, p_flow = gravity_flow(cfg, st.x, p_half, dt, fixed)
     e_rad = st.e_radiated_dev.clone()
     if math.isfinite(cfg.cooling_beta) or cfg.cooling_powerlaw:
         u_floor = units.ideal_u_floor(cfg.t_floor, hc.gamma)
@@ -383,7 +402,7 @@ def step(st: GasState, cfg: CoupledConfig, dt: float) -> GasState:
             u_half = u_cool
     p_pred = _axpy(p_flow, st.a_hydro, half)
     u_pred = _axpy(u_half, st.dudt, half)
-    order = grid_permutation(cfg, x1, st.h, st)
+    order = grid_permutation(cfg, x1, st.h, st, fixed)
     model = _learned(cfg)
     fields = [x1, p_flow, p_pred, u_half, u_pred, st.a_hydro, st.mass, st.h, st.alpha, st.perm]
     counts = st.row_counts
@@ -408,15 +427,26 @@ def step(st: GasState, cfg: CoupledConfig, dt: float) -> GasState:
         if hc.learned_h:
             h_start = model.h_start(h, feat)
             fallbacks = model.last_fallbacks
-    new = _hydro(pred, cfg, velocity(cfg, x1, p_pred), dt, a_prev, h_start=h_start)
-    if model is not None and model.collect is not None:
-        s = model.collect.stride
-        model.collect.add(feat[::s].contiguous() if s > 1 else feat, model.targets(h, new.h, s))
-    if model is not None and model.online is not None and hc.learned_h:
-        s = model.online.stride
-        model.observe(feat[::s].contiguous() if s > 1 else feat, model.targets(h, new.h, s))
+    new = _hydro(pred, cfg, velocity(cfg, x1, p_pred), dt, a_prev, h_start=h_start, fixed=fixed)
+    if fixed is not None:
+        fixed.pairs = (model, feat, h, new.h)
+    else:
+        log_pairs(cfg, model, feat, h, new.h)
     return replace(new, p=_axpy(p_flow, new.a_hydro, half), u=_axpy(u_half, new.dudt, half), e_radiated_dev=e_rad,
                    dt_prev=float(dt), learned_fallbacks_dev=fallbacks)
+
+
+def log_pairs(cfg: CoupledConfig, model, feat, h_prev, h_final) -> None:
+    """The learned start's (features, converged h) pairs of a step: into the model's PairLog, and
+    to its online training when the start is in use."""
+    if model is None:
+        return
+    if model.collect is not None:
+        s = model.collect.stride
+        model.collect.add(feat[::s].contiguous() if s > 1 else feat, model.targets(h_prev, h_final, s))
+    if model.online is not None and cfg.hydro.learned_h:
+        s = model.online.stride
+        model.observe(feat[::s].contiguous() if s > 1 else feat, model.targets(h_prev, h_final, s))
 
 
 def ledger(st: GasState, cfg: CoupledConfig) -> dict:

@@ -327,6 +327,10 @@ int32_t nb_cool_powerlaw(nb_ctx *ctx, void *stream, const double *u, const doubl
  * min c_visc h^2/nu over nu>0 (nu may be NULL), min |x|]. */
 int32_t nb_dt_minima(nb_ctx *ctx, void *stream, const double *h, const double *vsig, const double *acc,
                      const double *nu, double c_visc, const double *x, int64_t n, double *out4);
+/* nb_dt_minima into device memory (out4_dev), no host synchronisation: a step driver reads it
+ * with its other per-step words in one copy. */
+int32_t nb_dt_minima_dev(nb_ctx *ctx, void *stream, const double *h, const double *vsig, const double *acc,
+                         const double *nu, double c_visc, const double *x, int64_t n, double *out4_dev);
 
 /* Ensemble helpers (no counterpart in the reference; the systems of nb_grid_order_seg):
  * nb_dt_minima_seg: nb_dt_minima per system, out (host) nseg x 4; nb_axpy_rows: out row i = y row i

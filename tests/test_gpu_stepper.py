@@ -1,0 +1,110 @@
+"""GPU tests of the single-synchronisation step driver (nebula.sim.stepper.Stepper).
+
+- Its steps give coupled.step's bits in the mixed-precision mode: every field, the time steps
+  (coupled.stable_dt of each state), the list totals, with the benchmark's physics (stream crash,
+  Balsara limiter, power-law cooling, GN flow) and the learned start learning online.
+- A step whose fixed capacities are too small reports it, is discarded and done again by
+  coupled.step; the results are still coupled.step's.
+"""
+
+import copy
+
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+from helpers import bit_equal, check
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="needs a GPU")]
+
+FIELDS = ("x", "p", "u", "h", "rho", "alpha", "a_hydro", "dudt", "vsig_max", "c_s", "omega", "perm", "row_counts")
+
+
+def _config(learned_model=None):
+    from dataclasses import replace
+
+    from nebula.sim import coupled
+
+    cfg = replace(coupled.baseline_config(balsara=True, precision="mixed"), cooling_powerlaw=True)
+    if learned_model is not None:
+        cfg = replace(cfg, hydro=replace(cfg.hydro, learned_h=True, learned_model=learned_model))
+    return cfg
+
+
+def _compare(label, a, b):
+    for f in FIELDS:
+        bit_equal(f"{label}: {f}", getattr(a, f).cpu().numpy(), getattr(b, f).cpu().numpy())
+    check(f"{label}: t", a.t, b.t)
+    check(f"{label}: radiated energy", float(a.e_radiated), float(b.e_radiated))
+    check(f"{label}: search entries", a.n_search_entries, b.n_search_entries)
+    check(f"{label}: tight entries", a.n_tight_entries, b.n_tight_entries)
+
+
+def _trained_model(ic, cfg):
+    from dataclasses import replace
+
+    from nebula.sim import coupled
+    from nebula.sim.learned_h import LearnedH, PairLog
+
+    model = LearnedH(seed=5)
+    model.collect = PairLog(stride=2)
+    c = replace(cfg, hydro=replace(cfg.hydro, learned_model=model))
+    st = coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, c)
+    for _ in range(3):
+        st = coupled.step(st, c, coupled.stable_dt(st, c))
+    f, t = model.collect.tensors()
+    model.collect = None
+    model.fit_normalisation(f, t)
+    model.train(f, t, steps=60, batch=2048, record=False)
+    model.start_online(stride=2, every=3, steps=2, batch=1024)
+    return model
+
+
+@pytest.mark.parametrize("learned", [False, True])
+def test_stepper_equals_coupled_step(learned):
+    from nebula.ics import stream_crash
+    from nebula.sim import coupled
+    from nebula.sim.stepper import Stepper
+
+    ic = stream_crash(16384, length=200.0, seed=20261014)
+    base = _config()
+    model_a = _trained_model(ic, base) if learned else None
+    model_b = copy.deepcopy(model_a) if learned else None
+    cfg_a, cfg_b = _config(model_a), _config(model_b)
+    print(f"stepper: 16384-particle stream crash, learned start {learned}, 10 steps", flush=True)
+    st = coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg_a)
+    sp = Stepper(coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg_b), cfg_b)
+    dts, sdts = [], []
+    for _ in range(10):
+        dt = coupled.stable_dt(st, cfg_a)
+        dts.append(dt)
+        st = coupled.step(st, cfg_a, dt)
+        sdts.append(sp.step())
+    bit_equal("time steps", np.array(dts), np.array(sdts))
+    _compare("stepper vs coupled.step", st, sp.state)
+    # the first steps from the initial conditions may outgrow their rows or search (the eager
+    # step then regathers or rebuilds): those steps are done again by coupled.step
+    print("fallbacks:", sp.fallbacks, sp.fallback_reasons)
+    check("most steps without a fallback", True, sp.fallbacks <= 3)
+    if learned:
+        bit_equal("online-trained weights", model_a.params.cpu().numpy(), model_b.params.cpu().numpy())
+
+
+def test_stepper_overflow_falls_back_to_coupled_step():
+    from nebula.ics import stream_crash
+    from nebula.sim import coupled
+    from nebula.sim.stepper import Stepper
+
+    ic = stream_crash(16384, length=200.0, seed=7)
+    cfg = _config()
+    print("stepper: capacities a third of the need, 4 steps", flush=True)
+    st = coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg)
+    sp = Stepper(coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg), cfg, headroom=0.3)
+    for _ in range(4):
+        st = coupled.step(st, cfg, coupled.stable_dt(st, cfg))
+        sp.step()
+    print("fallbacks:", sp.fallbacks, sp.fallback_reasons)
+    check("the undersized first step fell back", True, sp.fallbacks >= 1)
+    check("reported as a capacity overflow", True,
+          bool(set(sp.fallback_reasons[0]) & {"gather rows", "extras", "tight index", "grid cells"}))
+    _compare("stepper (with fallback) vs coupled.step", st, sp.state)

@@ -143,6 +143,12 @@ struct Grid {
     int32_t nc[3];
 };
 
+// The segmented grids (the mixed-precision passes: an ensemble, or one cloud as one segment) cut
+// each x-y cell's column into kSegZSub cells along z. The search walks columns and takes one
+// contiguous run of cells per column, so shorter z cells trim each run closer to the search
+// sphere's chord at no extra cost per column.
+static constexpr double kSegZSub = 4.0;
+
 // One particle system of an ensemble in the segmented grid: its own cells, numbered from base.
 struct GridSeg {
     Grid g;
@@ -1314,7 +1320,7 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const Gr
                     drow[slot] = d2f;
                     if (kMasses) mrow[slot] = mj;
                 }
-                sums.add(__fsqrt_rn(d2f) * inv_h0, kMasses ? mj : mu, A.kid);
+                sums.add(sqrt32(d2f) * inv_h0, kMasses ? mj : mu, A.kid);
             }
             m += __popc(gb);
         }
@@ -1358,7 +1364,7 @@ __device__ __forceinline__ void row_dens_sums(const float* drow, const float* mr
                                               int kid, float mu, double& s1, double& s2)
 {
     DensSums sums;
-    for (int32_t k = glane; k < m; k += kG) sums.add(__fsqrt_rn(drow[k]) * inv_h, mrow ? mrow[k] : mu, kid);
+    for (int32_t k = glane; k < m; k += kG) sums.add(sqrt32(drow[k]) * inv_h, mrow ? mrow[k] : mu, kid);
     sums.total(s1, s2);
 }
 
@@ -1456,64 +1462,190 @@ k_row_total_rm(const int32_t* __restrict__ rowcnt, const unsigned int* __restric
 }
 
 // Extras arrive in atomic order: each row's segment is put in index order (so that the tightened
-// rows, and every sum over them, are the same from run to run), out of place into (es, eds). Every
-// entry goes to its rank, the number of smaller entries (entries of a segment are distinct). A
-// 16-lane group per row ranks up to 64 entries held in registers against each other; a longer
-// segment is ranked by its lanes, 16 entries at a time, against the whole segment.
+// rows, and every sum over them, are the same from run to run), out of place into (es, eds). A
+// 16-lane group per row ranks up to 64 entries held in registers against each other (every entry
+// goes to its rank, the number of smaller entries; entries of a segment are distinct). Longer
+// segments, few and clustered where large search spheres meet a dense region, are sorted by the
+// whole workgroup.
 static constexpr int kSortRegs = 4;
+static constexpr int kSortWave = 512;  // longest extras segment sorted by one wavefront (bitonic, LDS)
+static constexpr int kSortLds = 2048;  // longest extras segment sorted by the workgroup (bitonic, LDS)
 
+// the stored extras of row i (its count exceeds the segment only after a reported overflow)
+__device__ __forceinline__ int64_t extras_of(const unsigned int* efill, const unsigned long long* eoff, int64_t i,
+                                             unsigned long long& c0)
+{
+    c0 = eoff[i];
+    return (int64_t)min((unsigned long long)efill[i], eoff[i + 1] - c0);
+}
+
+// Short segments; longer ones go to the work lists of k_sort_extras_mid (rows up to kSortWave
+// entries, list[0 ..], count cnt[0]) and k_sort_extras_long (list[n ..], count cnt[1]).
 __global__ void __launch_bounds__(256)
 k_sort_extras(const uint32_t* __restrict__ ebuf, const float* __restrict__ ed2, const unsigned int* __restrict__ efill,
               const unsigned long long* __restrict__ eoff, int64_t n, uint32_t* __restrict__ es,
-              float* __restrict__ eds)
+              float* __restrict__ eds, uint32_t* __restrict__ list, unsigned int* __restrict__ cnt)
 {
     const int glane = threadIdx.x & (kG - 1);
     const int64_t i = (xcd_tile() * (int64_t)blockDim.x + threadIdx.x) / kG;
     if (i >= n) return;  // whole groups leave together
-    const unsigned long long b0 = eoff[i];
-    const uint32_t* e = ebuf + b0;
-    const float* d = ed2 + b0;
-    uint32_t* eo = es + b0;
-    float* dout = eds + b0;
-    const int64_t ne = (int64_t)min((unsigned long long)efill[i], eoff[i + 1] - b0);  // the stored entries
+    unsigned long long b0;
+    const int64_t ne = extras_of(efill, eoff, i, b0);
     if (ne == 0) return;
-    if (ne <= kG * kSortRegs) {
-        uint32_t v[kSortRegs];
-        float dv[kSortRegs];
-#pragma unroll
-        for (int t = 0; t < kSortRegs; t++) {
-            const int64_t k = glane + kG * t;
-            v[t] = k < ne ? e[k] : 0xffffffffu;
-            dv[t] = k < ne ? d[k] : 0.f;
-        }
-        int rank[kSortRegs] = {0, 0, 0, 0};
-#pragma unroll
-        for (int t2 = 0; t2 < kSortRegs; t2++)
-            for (int l = 0; l < kG; l++) {
-                const uint32_t w = __shfl(v[t2], l, kG);
-#pragma unroll
-                for (int t = 0; t < kSortRegs; t++) rank[t] += w < v[t] ? 1 : 0;
-            }
-#pragma unroll
-        for (int t = 0; t < kSortRegs; t++) {
-            const int64_t k = glane + kG * t;
-            if (k < ne) {
-                eo[rank[t]] = v[t];
-                dout[rank[t]] = dv[t];
-            }
+    if (ne > kG * kSortRegs) {
+        if (glane == 0) {
+            const int which = ne > kSortWave ? 1 : 0;
+            list[which * n + atomicAdd(&cnt[which], 1u)] = (uint32_t)i;
         }
         return;
     }
-    for (int64_t kb = 0; kb < ne; kb += kG) {
-        const int64_t k = kb + glane;
-        const uint32_t v = k < ne ? e[k] : 0u;
-        int64_t rank = 0;
-        for (int64_t q = 0; q < ne; q++) rank += e[q] < v ? 1 : 0;
+    const uint32_t* e = ebuf + b0;
+    const float* d = ed2 + b0;
+    uint32_t v[kSortRegs];
+    float dv[kSortRegs];
+#pragma unroll
+    for (int t = 0; t < kSortRegs; t++) {
+        const int64_t k = glane + kG * t;
+        v[t] = k < ne ? e[k] : 0xffffffffu;
+        dv[t] = k < ne ? d[k] : 0.f;
+    }
+    int rank[kSortRegs] = {0, 0, 0, 0};
+#pragma unroll
+    for (int t2 = 0; t2 < kSortRegs; t2++)
+        for (int l = 0; l < kG; l++) {
+            const uint32_t w = __shfl(v[t2], l, kG);
+#pragma unroll
+            for (int t = 0; t < kSortRegs; t++) rank[t] += w < v[t] ? 1 : 0;
+        }
+#pragma unroll
+    for (int t = 0; t < kSortRegs; t++) {
+        const int64_t k = glane + kG * t;
         if (k < ne) {
-            eo[rank] = v;
-            dout[rank] = d[k];
+            es[b0 + rank[t]] = v[t];
+            eds[b0 + rank[t]] = dv[t];
         }
     }
+}
+
+// Bitonic sort of (index, position) pairs in LDS (p2 a power of two >= m, pads last), then the
+// segment written in index order. sync(): the barrier of the threads taking part (tid of nt).
+template <class Sync>
+__device__ __forceinline__ void bitonic_segment(const uint32_t* e, const float* d, int m, uint32_t* sk, uint32_t* sp,
+                                                int tid, int nt, uint32_t* eo, float* dout, Sync sync)
+{
+    int p2 = 1;
+    while (p2 < m) p2 <<= 1;
+    for (int t = tid; t < p2; t += nt) {
+        sk[t] = t < m ? e[t] : 0xffffffffu;
+        sp[t] = (uint32_t)t;
+    }
+    sync();
+    for (int k = 2; k <= p2; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = tid; t < p2; t += nt) {
+                const int u = t ^ j;
+                if (u > t) {
+                    const bool up = (t & k) == 0;
+                    const uint32_t a = sk[t], b = sk[u];
+                    if ((a > b) == up) {
+                        sk[t] = b;
+                        sk[u] = a;
+                        const uint32_t pa = sp[t];
+                        sp[t] = sp[u];
+                        sp[u] = pa;
+                    }
+                }
+            }
+            sync();
+        }
+    for (int t = tid; t < m; t += nt) {
+        eo[t] = sk[t];
+        dout[t] = d[sp[t]];
+    }
+    sync();  // the LDS slice is free again
+}
+
+// Segments of kG * kSortRegs .. kSortWave entries (a particle seen by many others with larger
+// search lengths; few, and clustered where large search spheres meet a dense region): one
+// wavefront per row, the rows of the list spread over the whole launch.
+__global__ void __launch_bounds__(256)
+k_sort_extras_mid(const uint32_t* __restrict__ ebuf, const float* __restrict__ ed2,
+                  const unsigned int* __restrict__ efill, const unsigned long long* __restrict__ eoff,
+                  uint32_t* __restrict__ es, float* __restrict__ eds, const uint32_t* __restrict__ list,
+                  const unsigned int* __restrict__ cnt)
+{
+    __shared__ uint32_t sk[4][kSortWave];
+    __shared__ uint32_t sp[4][kSortWave];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const unsigned int count = cnt[0];
+    const unsigned int stride = gridDim.x * (blockDim.x >> 6);
+    for (unsigned int r = blockIdx.x * (blockDim.x >> 6) + wave; r < count; r += stride) {
+        const int64_t row = list[r];
+        unsigned long long c0;
+        const int m = (int)extras_of(efill, eoff, row, c0);
+        bitonic_segment(ebuf + c0, ed2 + c0, m, sk[wave], sp[wave], lane, 64, es + c0, eds + c0, [] { wave_sync(); });
+    }
+}
+
+// Segments longer than kSortWave: one workgroup per row (bitonic in LDS up to kSortLds entries,
+// else ranks counted against the segment read through LDS in tiles).
+__global__ void __launch_bounds__(256)
+k_sort_extras_long(const uint32_t* __restrict__ ebuf, const float* __restrict__ ed2,
+                   const unsigned int* __restrict__ efill, const unsigned long long* __restrict__ eoff, int64_t n,
+                   uint32_t* __restrict__ es, float* __restrict__ eds, const uint32_t* __restrict__ list,
+                   const unsigned int* __restrict__ cnt)
+{
+    __shared__ uint32_t sk[kSortLds];
+    __shared__ uint32_t sp[kSortLds];
+    const int tid = threadIdx.x, bs = blockDim.x;
+    const unsigned int count = cnt[1];
+    for (unsigned int r = blockIdx.x; r < count; r += gridDim.x) {
+        const int64_t row = list[n + r];
+        unsigned long long c0;
+        const int64_t m = extras_of(efill, eoff, row, c0);
+        const uint32_t* e = ebuf + c0;
+        const float* d = ed2 + c0;
+        if (m <= kSortLds) {
+            bitonic_segment(e, d, (int)m, sk, sp, tid, bs, es + c0, eds + c0, [] { __syncthreads(); });
+            continue;
+        }
+        for (int64_t kb = 0; kb < m; kb += bs) {
+            const int64_t k = kb + tid;
+            const uint32_t v = k < m ? e[k] : 0u;
+            int64_t rank = 0;
+            for (int64_t tb = 0; tb < m; tb += kSortLds) {
+                const int tl = (int)min((int64_t)kSortLds, m - tb);
+                __syncthreads();
+                for (int t = tid; t < tl; t += bs) sk[t] = e[tb + t];
+                __syncthreads();
+                if (k < m)
+                    for (int q = 0; q < tl; q++) rank += sk[q] < v ? 1 : 0;
+            }
+            if (k < m) {
+                es[c0 + rank] = v;
+                eds[c0 + rank] = d[k];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, int64_t n);
+
+// The three passes of the extras sort (work-list counters cleared by a kernel: graph-safe).
+static void sort_extras(nb_ctx* ctx, hipStream_t s, int64_t n, unsigned blocks, int tile)
+{
+    uint32_t* list = ctx->xlist;
+    unsigned int* cnt = (unsigned int*)(ctx->xlist + 2 * n);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, (uint32_t*)cnt, 0u, (int64_t)2);
+    NB_TIMED(ctx, s, "k_sort_extras", k_sort_extras, dim3(blocks), dim3(tile), 0, s, ctx->ebuf,
+             (const float*)ctx->edbuf, ctx->efill, ctx->rm_eo, n, ctx->ebuf_s, ctx->edbuf_s, list, cnt);
+    NB_TIMED(ctx, s, "k_sort_extras_mid", k_sort_extras_mid, dim3(1024), dim3(256), 0, s, ctx->ebuf,
+             (const float*)ctx->edbuf, ctx->efill, ctx->rm_eo, ctx->ebuf_s, ctx->edbuf_s, (const uint32_t*)list,
+             (const unsigned int*)cnt);
+    NB_TIMED(ctx, s, "k_sort_extras_long", k_sort_extras_long, dim3(256), dim3(256), 0, s, ctx->ebuf,
+             (const float*)ctx->edbuf, ctx->efill, ctx->rm_eo, n, ctx->ebuf_s, ctx->edbuf_s, (const uint32_t*)list,
+             (const unsigned int*)cnt);
 }
 
 // what the emission pass needs to finish the rows converged in one Newton step
@@ -1570,7 +1702,7 @@ k_tight_emit_rm(const double* __restrict__ pos, const uint32_t* __restrict__ rid
             if (k < ng) {
                 j = ridx[g0 + k];
                 d2 = rd2[g0 + k];
-                if (fin) sums.add(__fsqrt_rn(d2) * inv_hf, F.rmass ? F.rmass[g0 + k] : mu, F.kid);
+                if (fin) sums.add(sqrt32(d2) * inv_hf, F.rmass ? F.rmass[g0 + k] : mu, F.kid);
             } else {
                 j = ebuf[e0 + (k - ng)];
                 d2 = ed2[e0 + (k - ng)];
@@ -1786,8 +1918,9 @@ __global__ void k_plan_seg(const double* __restrict__ bb, const int64_t* __restr
             for (;;) {
                 cells = 1.0;
                 for (int c = 0; c < 3; c++) {
-                    segs[k].g.nc[c] = (int32_t)fmax(floor(span[c] / cell) + 1.0, 1.0);
-                    segs[k].g.inv[c] = 1.0 / cell;
+                    const double cc = c == 2 ? cell / kSegZSub : cell;
+                    segs[k].g.nc[c] = (int32_t)fmax(floor(span[c] / cc) + 1.0, 1.0);
+                    segs[k].g.inv[c] = 1.0 / cc;
                     cells *= (double)segs[k].g.nc[c];
                 }
                 if (cells <= max_cells) break;
@@ -2075,8 +2208,9 @@ int build_grid_seg(nb_ctx* ctx, hipStream_t s, const double* pos, const double* 
         for (;;) {
             cells = 1.0;
             for (int c = 0; c < 3; c++) {
-                hs[k].g.nc[c] = (int32_t)std::max(std::floor(span[c] / cell) + 1.0, 1.0);
-                hs[k].g.inv[c] = 1.0 / cell;
+                const double cc = c == 2 ? cell / kSegZSub : cell;
+                hs[k].g.nc[c] = (int32_t)std::max(std::floor(span[c] / cc) + 1.0, 1.0);
+                hs[k].g.inv[c] = 1.0 / cc;
                 cells *= (double)hs[k].g.nc[c];
             }
             if (cells <= kMaxTotalCells) break;
@@ -2587,6 +2721,7 @@ extern "C" int32_t nb_hydro_density_mixed(nb_ctx* ctx, void* stream, const doubl
         ctx_reserve(ctx, (void**)&ctx->pending, &ctx->pending_bytes, n) ||
         ctx_reserve(ctx, (void**)&ctx->posh, &ctx->posh_bytes, sizeof(double4) * n) ||
         ctx_reserve(ctx, (void**)&ctx->efill, &ctx->efill_bytes, sizeof(unsigned int) * n) ||
+        ctx_reserve(ctx, (void**)&ctx->xlist, &ctx->xlist_bytes, sizeof(uint32_t) * (size_t)(2 * n + 2)) ||
         ctx_reserve(ctx, (void**)&ctx->rm_gc, &ctx->rm_gc_bytes, u64n) ||
         ctx_reserve(ctx, (void**)&ctx->rm_go, &ctx->rm_go_bytes, u64n) ||
         ctx_reserve(ctx, (void**)&ctx->rm_ec, &ctx->rm_ec_bytes, u64n) ||
@@ -2663,8 +2798,7 @@ extern "C" int32_t nb_hydro_density_mixed(nb_ctx* ctx, void* stream, const doubl
     if (ctx_reserve(ctx, (void**)&ctx->ebuf_s, &ctx->ebuf_s_bytes, sizeof(uint32_t) * (size_t)tot[1]) ||
         ctx_reserve(ctx, (void**)&ctx->edbuf_s, &ctx->edbuf_s_bytes, sizeof(float) * (size_t)tot[1]))
         return NB_ERR_HIP;
-    NB_TIMED(ctx, s, "k_sort_extras", k_sort_extras, dim3(blocks), dim3(tile), 0, s, ctx->ebuf,
-             (const float*)ctx->edbuf, ctx->efill, ctx->rm_eo, n, ctx->ebuf_s, ctx->edbuf_s);
+    sort_extras(ctx, s, n, (unsigned)blocks, tile);
     // merged row lengths (reusing the gather capacities' array), their scan: the tight rows' offsets
     NB_TIMED(ctx, s, "k_row_total_rm", k_row_total_rm, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, ctx->rowcnt,
              ctx->efill, ctx->rm_go, ctx->rm_eo, ctx->rm_gc, row_counts, ctx->part, n);
@@ -2842,6 +2976,7 @@ extern "C" int32_t nb_fixed_reserve(nb_ctx* ctx, int64_t n, int32_t nseg, int64_
         ctx_reserve(ctx, (void**)&ctx->pending, &ctx->pending_bytes, n) ||
         ctx_reserve(ctx, (void**)&ctx->posh, &ctx->posh_bytes, sizeof(double4) * n) ||
         ctx_reserve(ctx, (void**)&ctx->efill, &ctx->efill_bytes, sizeof(unsigned int) * n) ||
+        ctx_reserve(ctx, (void**)&ctx->xlist, &ctx->xlist_bytes, sizeof(uint32_t) * (size_t)(2 * n + 2)) ||
         ctx_reserve(ctx, (void**)&ctx->rm_gc, &ctx->rm_gc_bytes, u64n) ||
         ctx_reserve(ctx, (void**)&ctx->rm_go, &ctx->rm_go_bytes, u64n) ||
         ctx_reserve(ctx, (void**)&ctx->rm_ec, &ctx->rm_ec_bytes, u64n) ||
@@ -2955,8 +3090,7 @@ extern "C" int32_t nb_hydro_density_mixed_fixed(nb_ctx* ctx, void* stream, const
         NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<false>, dim3(blocks), dim3(tile), 0, s, h_search, n,
                  ctx->fscratch, rmass, ctx->rm_go, ctx->rowcnt, A);
     hipLaunchKernelGGL(k_status_grow, dim3(grid_of(n)), dim3(kBlock), 0, s, flags, n, status);
-    NB_TIMED(ctx, s, "k_sort_extras", k_sort_extras, dim3(blocks), dim3(tile), 0, s, ctx->ebuf, (const float*)ctx->edbuf,
-             ctx->efill, ctx->rm_eo, n, ctx->ebuf_s, ctx->edbuf_s);
+    sort_extras(ctx, s, n, (unsigned)blocks, tile);
     NB_TIMED(ctx, s, "k_row_total_rm", k_row_total_rm, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, ctx->rowcnt,
              ctx->efill, ctx->rm_go, ctx->rm_eo, ctx->rm_gc, row_counts, ctx->part, n);
     rc = scan_fixed(ctx, s, (const unsigned long long*)ctx->rm_gc, ctx->rm_to, n + 1);

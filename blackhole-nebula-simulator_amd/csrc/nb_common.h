@@ -74,6 +74,13 @@ __device__ __forceinline__ double min_image(double d, double box_c)
     return d;
 }
 
+// FP32 square root, reciprocal square root and reciprocal of the hardware (about 1 ulp): the
+// pair terms of the mixed-precision passes carry FP32 rounding anyway, and the correctly rounded
+// forms take several instructions each.
+__device__ __forceinline__ float sqrt32(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float rsqrt32(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float rcp32(float x) { return __builtin_amdgcn_rcpf(x); }
+
 struct Box {
     double l[3];  // periodic lengths, 0 = open axis
 };

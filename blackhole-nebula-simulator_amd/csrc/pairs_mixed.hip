@@ -1,7 +1,7 @@
 // Mixed-precision pair passes: gradients (for the viscosity switches) and forces with FP32 pair
 // arithmetic.
 //
-// A group of 16 lanes serves one row: lane l takes entries l, l + 16, ... so that the group
+// A group of 8 lanes serves one row: lane l takes entries l, l + 8, ... so that the group
 // reads a row-major or CSR row contiguously. Each lane accumulates its pairs in FP32 with
 // compensated (Kahan) sums and the group combines the lane partials in FP64. Separations are
 // formed in FP64 (positions stay FP64: at r ~ 2000 an FP32 position resolves only ~1e-4 of a
@@ -16,7 +16,10 @@
 
 namespace nb {
 
-static constexpr int kGM = 16;
+// Lanes per row of the gradient and force passes: 8 (rows of ~60 pairs give each lane 7-8; the
+// per-row combination of the lane sums costs a quarter less than with 16 lanes, and the FP64
+// 3 x 3 solve of the gradients runs on 8 rows per wavefront instead of 4).
+static constexpr int kPairW = 8;
 static constexpr int kBlockM = 256;
 
 struct __align__(16) PF {
@@ -84,16 +87,18 @@ struct Kahan {
     __device__ __forceinline__ double value() const { return (double)s - (double)c; }
 };
 
-// butterfly over the 16 lanes of a group: every lane ends with the same value
+// butterfly over the W lanes of a group: every lane ends with the same value
+template <int W>
 __device__ __forceinline__ double gsum(double v)
 {
-    for (int o = kGM / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kGM);
+    for (int o = W / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, W);
     return v;
 }
 
+template <int W>
 __device__ __forceinline__ float gmax(float v)
 {
-    for (int o = kGM / 2; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, kGM));
+    for (int o = W / 2; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, W));
     return v;
 }
 
@@ -108,15 +113,15 @@ __device__ __forceinline__ void sep32(const double* pos, double pa0, double pa1,
 
 // sph.py:525-652 without reconstruction: pressure force with grad-h terms, Monaghan viscosity on
 // approaching pairs, conductivity (pressure or velocity signal), P dV work, largest v_sig.
-template <class L>
+template <class L, int W>
 __global__ void __launch_bounds__(kBlockM)
 k_forces_mixed(const double* __restrict__ pos, const PF* __restrict__ pf, const double* __restrict__ c_s, L lists,
                Box box, const int64_t* __restrict__ rows, int64_t nrows, nb_force_params P, double* __restrict__ acc,
                double* __restrict__ dudt, double* __restrict__ vsig_max, double* __restrict__ vheat,
                double* __restrict__ cheat)
 {
-    const int glane = threadIdx.x & (kGM - 1);
-    const int64_t q = (xcd_tile() * (int64_t)blockDim.x + threadIdx.x) / kGM;
+    const int glane = threadIdx.x & (W - 1);
+    const int64_t q = (xcd_tile() * (int64_t)blockDim.x + threadIdx.x) / W;
     if (q >= nrows) return;  // whole groups leave together
     const int64_t a = row_of(rows, q);
     const double pa0 = pos[3 * a], pa1 = pos[3 * a + 1], pa2 = pos[3 * a + 2];
@@ -127,14 +132,14 @@ k_forces_mixed(const double* __restrict__ pos, const PF* __restrict__ pf, const 
     Kahan ax, ay, az, work, visc, cond;
     float vmax = 0.f;
     const int64_t m = lists.count(a);
-    for (int64_t k = glane; k < m; k += kGM) {
+    for (int64_t k = glane; k < m; k += W) {
         const int32_t b = lists.at(a, k);
         if (b == a) continue;
         const PF B = pf[b];
         float dx0, dx1, dx2;
         sep32(pos, pa0, pa1, pa2, b, box, dx0, dx1, dx2);
         const float r2 = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
-        const float inv_r = 1.f / __fsqrt_rn(r2);
+        const float inv_r = rsqrt32(r2);
         const float r = r2 * inv_r;
         const float rx = dx0 * inv_r, ry = dx1 * inv_r, rz = dx2 * inv_r;
         const float fa = A.b.x * df32(r * A.a.w, kid);
@@ -154,7 +159,7 @@ k_forces_mixed(const double* __restrict__ pos, const PF* __restrict__ pf, const 
         }
         if (use_cond) {
             const float vu =
-                P.cond_signal == 0 ? __fsqrt_rn(fabsf(A.c.z - B.c.z) / (0.5f * (A.c.w + B.c.w))) : fabsf(vdotr);
+                P.cond_signal == 0 ? sqrt32(fabsf(A.c.z - B.c.z) * rcp32(0.5f * (A.c.w + B.c.w))) : fabsf(vdotr);
             cond.add(B.c.x * alpha_u * vu * (A.c.y - B.c.y) * 0.5f * (fa * A.b.z + fb * B.b.z));
         }
         const float f = B.c.x * (na * fa + nbb * fb);
@@ -163,9 +168,9 @@ k_forces_mixed(const double* __restrict__ pos, const PF* __restrict__ pf, const 
         az.add(-f * rz);
         work.add(B.c.x * vdotr * fa);
     }
-    const double sx = gsum(ax.value()), sy = gsum(ay.value()), sz = gsum(az.value());
-    const double sw = gsum(work.value()), sv = gsum(visc.value()), sc = gsum(cond.value());
-    const float vm = gmax(vmax);
+    const double sx = gsum<W>(ax.value()), sy = gsum<W>(ay.value()), sz = gsum<W>(az.value());
+    const double sw = gsum<W>(work.value()), sv = gsum<W>(visc.value()), sc = gsum<W>(cond.value());
+    const float vm = gmax<W>(vmax);
     if (glane == 0) {
         acc[3 * q] = sx;
         acc[3 * q + 1] = sy;
@@ -179,26 +184,26 @@ k_forces_mixed(const double* __restrict__ pos, const PF* __restrict__ pf, const 
 
 // sph.py:837-884: R, G_v and G_a of a row from FP32 pair terms, inverted in FP64 (kAcc false:
 // accelerations zero, G_a not summed).
-template <class L, bool kAcc>
+template <class L, bool kAcc, int W>
 __device__ __forceinline__ void gradients_mixed_row(const double* __restrict__ pos, const double* __restrict__ accel,
                                                     const PG* __restrict__ pg, const L& lists, const Box& box,
                                                     const int64_t* __restrict__ rows, int64_t nrows, int kid,
                                                     double* __restrict__ div, double* __restrict__ curl,
                                                     double* __restrict__ ddiv)
 {
-    const int glane = threadIdx.x & (kGM - 1);
-    const int64_t q = (xcd_tile() * (int64_t)blockDim.x + threadIdx.x) / kGM;
+    const int glane = threadIdx.x & (W - 1);
+    const int64_t q = (xcd_tile() * (int64_t)blockDim.x + threadIdx.x) / W;
     if (q >= nrows) return;
     const int64_t a = row_of(rows, q);
     const double pa0 = pos[3 * a], pa1 = pos[3 * a + 1], pa2 = pos[3 * a + 2];
     const PG A = pg[a];
-    Kahan R[9], Gv[9], Ga[kAcc ? 9 : 1];
+    Kahan R[6], Gv[9], Ga[kAcc ? 9 : 1];  // R is symmetric: its upper triangle (00 01 02 11 12 22)
     float aa[3] = {0.f, 0.f, 0.f};
     if (kAcc)
         for (int c = 0; c < 3; c++) aa[c] = (float)accel[3 * a + c];
     int cnt = 0;
     const int64_t m = lists.count(a);
-    for (int64_t k = glane; k < m; k += kGM) {
+    for (int64_t k = glane; k < m; k += W) {
         const int32_t b = lists.at(a, k);
         if (b == a) continue;
         cnt++;
@@ -206,31 +211,42 @@ __device__ __forceinline__ void gradients_mixed_row(const double* __restrict__ p
         float dx[3];
         sep32(pos, pa0, pa1, pa2, b, box, dx[0], dx[1], dx[2]);
         const float r2 = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
-        const float inv_r = 1.f / __fsqrt_rn(r2);
+        const float inv_r = rsqrt32(r2);
         const float g = A.b.x * df32(r2 * inv_r * A.a.w, kid) * inv_r;
         const float dv[3] = {B.a.x - A.a.x, B.a.y - A.a.y, B.a.z - A.a.z};
         float da[3] = {0.f, 0.f, 0.f};
         if (kAcc)
             for (int c = 0; c < 3; c++) da[c] = (float)accel[3 * b + c] - aa[c];
         const float mg = B.b.y * g;
+        const float gx[3] = {mg * dx[0], mg * dx[1], mg * dx[2]};
+        R[0].add(-(dx[0] * gx[0]));
+        R[1].add(-(dx[0] * gx[1]));
+        R[2].add(-(dx[0] * gx[2]));
+        R[3].add(-(dx[1] * gx[1]));
+        R[4].add(-(dx[1] * gx[2]));
+        R[5].add(-(dx[2] * gx[2]));
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
             for (int j = 0; j < 3; j++) {
-                const float gj = mg * dx[j];
-                R[3 * i + j].add(-(dx[i] * gj));
-                Gv[3 * i + j].add(dv[i] * gj);
-                if (kAcc) Ga[kAcc ? 3 * i + j : 0].add(da[i] * gj);
+                Gv[3 * i + j].add(dv[i] * gx[j]);
+                if (kAcc) Ga[kAcc ? 3 * i + j : 0].add(da[i] * gx[j]);
             }
     }
     double rm[9], gvm[9], gam[9];
+    {
+        const double r00 = gsum<W>(R[0].value()), r01 = gsum<W>(R[1].value()), r02 = gsum<W>(R[2].value());
+        const double r11 = gsum<W>(R[3].value()), r12 = gsum<W>(R[4].value()), r22 = gsum<W>(R[5].value());
+        rm[0] = r00, rm[1] = r01, rm[2] = r02;
+        rm[3] = r01, rm[4] = r11, rm[5] = r12;
+        rm[6] = r02, rm[7] = r12, rm[8] = r22;
+    }
 #pragma unroll
     for (int k = 0; k < 9; k++) {
-        rm[k] = gsum(R[k].value());
-        gvm[k] = gsum(Gv[k].value());
-        gam[k] = kAcc ? gsum(Ga[kAcc ? k : 0].value()) : 0.0;
+        gvm[k] = gsum<W>(Gv[k].value());
+        gam[k] = kAcc ? gsum<W>(Ga[kAcc ? k : 0].value()) : 0.0;
     }
-    for (int o = kGM / 2; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, kGM);
+    for (int o = W / 2; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, W);
     if (glane != 0) return;
     double dvm[9], dam[9];
     if (near_singular(rm, cnt)) {
@@ -255,25 +271,25 @@ __device__ __forceinline__ void gradients_mixed_row(const double* __restrict__ p
     ddiv[q] = dam[0] + dam[4] + dam[8] - ss;
 }
 
-template <class L>
+template <class L, int W>
 __global__ void __launch_bounds__(kBlockM)
 k_gradients_mixed(const double* __restrict__ pos, const double* __restrict__ accel, const PG* __restrict__ pg, L lists,
                   Box box, const int64_t* __restrict__ rows, int64_t nrows, int kid, double* __restrict__ div,
                   double* __restrict__ curl, double* __restrict__ ddiv)
 {
-    gradients_mixed_row<L, true>(pos, accel, pg, lists, box, rows, nrows, kid, div, curl, ddiv);
+    gradients_mixed_row<L, true, W>(pos, accel, pg, lists, box, rows, nrows, kid, div, curl, ddiv);
 }
 
-template <class L>
+template <class L, int W>
 __global__ void __launch_bounds__(kBlockM)
 k_divcurl_mixed(const double* __restrict__ pos, const double* __restrict__ accel, const PG* __restrict__ pg, L lists,
                 Box box, const int64_t* __restrict__ rows, int64_t nrows, int kid, double* __restrict__ div,
                 double* __restrict__ curl, double* __restrict__ ddiv)
 {
-    gradients_mixed_row<L, false>(pos, accel, pg, lists, box, rows, nrows, kid, div, curl, ddiv);
+    gradients_mixed_row<L, false, W>(pos, accel, pg, lists, box, rows, nrows, kid, div, curl, ddiv);
 }
 
-static inline unsigned grid_m(int64_t nrows, int tile) { return (unsigned)((nrows * kGM + tile - 1) / tile); }
+static inline unsigned grid_m(int64_t nrows, int tile, int w) { return (unsigned)((nrows * w + tile - 1) / tile); }
 static inline unsigned grid_1(int64_t n) { return (unsigned)((n + kBlockM - 1) / kBlockM); }
 
 static inline Box box_of(const double* box3)
@@ -284,19 +300,25 @@ static inline Box box_of(const double* box3)
     return b;
 }
 
-#define NB_DISPATCH_M(L, KERNEL, GRID, STREAM, ...)                                                     \
+#define NB_DISPATCH_W(L, KERNEL, W, NROWS, STREAM, ...)                                                 \
     do {                                                                                                \
-        const int ts_ = ktime_begin(ctx, (STREAM), #KERNEL);                                            \
+        const dim3 grid_(grid_m((NROWS), ctx->tile, (W)));                                              \
         if ((L)->length && (L)->row_major) {                                                            \
             RowView v_{(L)->index, (L)->length, (L)->offset};                                           \
-            hipLaunchKernelGGL(KERNEL<RowView>, dim3(GRID), dim3(ctx->tile), 0, STREAM, __VA_ARGS__);    \
+            hipLaunchKernelGGL((KERNEL<RowView, W>), grid_, dim3(ctx->tile), 0, STREAM, __VA_ARGS__);    \
         } else if ((L)->length) {                                                                       \
-            BlkView v_{(L)->index, (L)->length, (L)->row_cap, (L)->offset};                                          \
-            hipLaunchKernelGGL(KERNEL<BlkView>, dim3(GRID), dim3(ctx->tile), 0, STREAM, __VA_ARGS__);    \
+            BlkView v_{(L)->index, (L)->length, (L)->row_cap, (L)->offset};                             \
+            hipLaunchKernelGGL((KERNEL<BlkView, W>), grid_, dim3(ctx->tile), 0, STREAM, __VA_ARGS__);    \
         } else {                                                                                        \
             CsrView v_{(L)->offset, (L)->index};                                                        \
-            hipLaunchKernelGGL(KERNEL<CsrView>, dim3(GRID), dim3(ctx->tile), 0, STREAM, __VA_ARGS__);    \
+            hipLaunchKernelGGL((KERNEL<CsrView, W>), grid_, dim3(ctx->tile), 0, STREAM, __VA_ARGS__);    \
         }                                                                                               \
+    } while (0)
+
+#define NB_DISPATCH_M(L, KERNEL, NROWS, STREAM, ...)                                                    \
+    do {                                                                                                \
+        const int ts_ = ktime_begin(ctx, (STREAM), #KERNEL);                                            \
+        NB_DISPATCH_W(L, KERNEL, kPairW, NROWS, STREAM, __VA_ARGS__);                                   \
         ktime_end(ctx, (STREAM), ts_);                                                                  \
     } while (0)
 
@@ -319,7 +341,7 @@ int32_t forces_mixed(nb_ctx* ctx, hipStream_t s, const double* pos, const double
     PF* pf = (PF*)ctx->pack;
     hipLaunchKernelGGL(k_pack_forces, dim3(grid_1(n)), dim3(kBlockM), 0, s, vel, mass, h, rho, omega, pressure,
                        P->use_cond ? u : nullptr, c_s, P->use_visc ? alpha : nullptr, P->cnorm, n, pf);
-    NB_DISPATCH_M(lists, k_forces_mixed, grid_m(nrows, ctx->tile), s, pos, pf, c_s, v_, box_of(box3), rows, nrows, *P, acc, dudt,
+    NB_DISPATCH_M(lists, k_forces_mixed, nrows, s, pos, pf, c_s, v_, box_of(box3), rows, nrows, *P, acc, dudt,
                   vsig_max, vheat, cheat);
     return launched(ctx, "nb_forces (mixed)");
 }
@@ -343,10 +365,10 @@ extern "C" int32_t nb_gradients_mixed(nb_ctx* ctx, void* stream, const double* p
     PG* pg = (PG*)ctx->pack;
     hipLaunchKernelGGL(k_pack_grad, dim3(grid_1(n)), dim3(kBlockM), 0, s, vel, mass, h, cnorm, n, pg);
     if (acc)
-        NB_DISPATCH_M(lists, k_gradients_mixed, grid_m(nrows, ctx->tile), s, pos, acc, pg, v_, box_of(box3), rows, nrows, kid,
-                      div, curl, ddiv);
+        NB_DISPATCH_M(lists, k_gradients_mixed, nrows, s, pos, acc, pg, v_, box_of(box3), rows, nrows, kid, div, curl,
+                      ddiv);
     else
-        NB_DISPATCH_M(lists, k_divcurl_mixed, grid_m(nrows, ctx->tile), s, pos, acc, pg, v_, box_of(box3), rows, nrows, kid, div,
-                      curl, ddiv);
+        NB_DISPATCH_M(lists, k_divcurl_mixed, nrows, s, pos, acc, pg, v_, box_of(box3), rows, nrows, kid, div, curl,
+                      ddiv);
     return launched(ctx, "nb_gradients_mixed");
 }

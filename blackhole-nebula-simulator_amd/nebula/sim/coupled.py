@@ -14,7 +14,9 @@ grid (cells in raster order; nb_grid_order), so that every gathered neighbour ro
 sorted and the lanes of a wavefront work on neighbouring particles. ``GasState.perm`` maps the
 stored order back to the caller's (``to_numpy`` returns the caller's order). Sums over a row run
 in the stored order, so results equal those of the reference's arithmetic applied to the
-particles in that order; the oracle applies the same ordering.
+particles in that order; the oracle applies the same ordering. The mixed-precision mode orders
+by the segmented grid of an ensemble (nb_grid_order_seg), the cloud being its one segment: the
+same lists, with z cells a quarter of the x-y cell for a tighter candidate walk.
 
 Each step (coupled.py:238-258): hydro half-kick; GN flow over dt; cooling in the middle of
 the split; hydro forces at the new positions with the predicted velocity; second half-kick.
@@ -172,13 +174,37 @@ def _search_lengths(cfg: CoupledConfig, h):
     return out
 
 
+_ONE_CLOUD: dict = {}
+
+
+def one_cloud(n: int, device):
+    """(cloud, seg_off) of a single system as one segment: the mixed-precision passes always run on
+    the segmented grid (an ensemble's, here with one segment), the FP64 passes on the plain grid
+    that the oracle restates."""
+    key = (int(n), str(device))
+    if key not in _ONE_CLOUD:
+        _ONE_CLOUD.clear()
+        _ONE_CLOUD[key] = (torch.zeros(n, dtype=torch.int32, device=device), np.array([0, n], np.int64))
+    return _ONE_CLOUD[key]
+
+
+def _segments(cfg: CoupledConfig, st: GasState | None, n: int, device):
+    """The (cloud, seg_off) the grid passes of this state use, or (None, None) for the plain grid."""
+    if st is not None and st.cloud is not None:
+        return st.cloud, st.seg_off
+    if cfg.hydro.precision == "mixed":
+        return one_cloud(n, device)
+    return None, None
+
+
 def grid_permutation(cfg: CoupledConfig, x, h, st: GasState | None = None, fixed=None):
     """Order of the neighbour grid the next density pass will build (search lengths from h).
     fixed: the fixed-capacity passes (no host synchronisation; see _hydro)."""
     if fixed is not None:
         return fixed.order(x, _search_lengths(cfg, h), 1.0 if st is None else st.cell_factor)
-    if st is not None and st.cloud is not None:
-        return grid_order_seg(x, _search_lengths(cfg, h), st.cloud, st.seg_off, st.cell_factor)
+    cloud, seg_off = _segments(cfg, st, int(x.shape[0]), x.device)
+    if cloud is not None:
+        return grid_order_seg(x, _search_lengths(cfg, h), cloud, seg_off, 1.0 if st is None else st.cell_factor)
     return grid_order(x, _search_lengths(cfg, h), None, 1.0 if st is None else st.cell_factor)
 
 
@@ -236,9 +262,10 @@ def _hydro(st: GasState, cfg: CoupledConfig, v, dt_alpha: float, a_prev, h_start
         d, n_search = fixed.density(st, cfg, h_start, counts)
     elif hc.precision == "mixed":
         counts = _dev.empty((n, 2), torch.int32)
+        cloud, seg_off = _segments(cfg, st, n, st.x.device)
         d, n_search = hydro_density_mixed_device(st.x, st.mass, st.h.clone(), hc.kernel, hc.eta, 1e-4, hc.h_max,
-                                                 hc.skin, st.m_uniform, h_start_d=h_start, cloud_d=st.cloud,
-                                                 seg_off=st.seg_off, cell_factor=st.cell_factor,
+                                                 hc.skin, st.m_uniform, h_start_d=h_start, cloud_d=cloud,
+                                                 seg_off=seg_off, cell_factor=st.cell_factor,
                                                  index_buffer=buf, row_hint=st.row_counts,
                                                  row_counts=counts)  # fmt: skip
     else:

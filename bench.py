@@ -96,13 +96,15 @@ def cpu_baseline(ic, cfg, steps):
                       f"kernels, 1 warm-up + {steps} timed coupled steps"}  # fmt: skip
 
 
-def run_gpu(ic, cfg, steps, warmup, learned=False, log_stride=8, train_steps=300):
+def run_gpu(ic, cfg, steps, warmup, learned=False, log_stride=8, train_steps=300, driver="eager"):
     """Times ``steps`` coupled steps after ``warmup`` (no instrumentation in the timed region), then
     runs ``steps`` more with the library's kernel timing on (HIP events around each main kernel on
     its stream) and the entry-point timing. With ``learned``, the warm-up steps log (features,
     converged h) pairs (every ``log_stride``-th particle), the network is trained on them with the
     FP32 training kernels, and the timed steps start each Newton solve from its BF16 MFMA
-    prediction. Returns (wall s, kernel times, entry-point times, state, info)."""
+    prediction. driver: "eager" (coupled.stable_dt + coupled.step) or "stepper" (the same steps,
+    bit for bit, through nebula.sim.stepper.Stepper: one host synchronisation per step).
+    Returns (wall s, kernel times, entry-point times, state, info)."""
     import torch
 
     from nebula import _hip
@@ -137,23 +139,43 @@ def run_gpu(ic, cfg, steps, warmup, learned=False, log_stride=8, train_steps=300
     else:
         for _ in range(warmup):
             st = coupled.step(st, cfg, coupled.stable_dt(st, cfg))
+    if driver == "stepper":
+        from nebula.sim.stepper import Stepper
+
+        sp = Stepper(st, cfg)
+        for _ in range(2):  # warm-up of the fixed-capacity passes
+            sp.step()
+
+        def advance():
+            sp.step()
+            return sp.state
+    else:
+        holder = [st]
+
+        def advance():
+            holder[0] = coupled.step(holder[0], cfg, coupled.stable_dt(holder[0], cfg))
+            return holder[0]
+
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        st = coupled.step(st, cfg, coupled.stable_dt(st, cfg))
+        advance()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     _hip.start_kernel_timing()
     _hip.start_timing()
     iters, fallbacks = [], 0
     for _ in range(steps):
-        st = coupled.step(st, cfg, coupled.stable_dt(st, cfg))
+        st = advance()
         iters.append(st.iterations.double().mean())
         fallbacks += st.learned_fallbacks if learned else 0
     torch.cuda.synchronize()
     entry = _hip.stop_timing()
     kern = _hip.stop_kernel_timing()
     info["newton_iters"] = float(sum(float(x) for x in iters) / len(iters))
+    info["driver"] = driver
+    if driver == "stepper":
+        info["stepper_fallbacks"] = sp.fallbacks
     if learned:
         info["fallback_rate"] = fallbacks / (steps * len(ic.x))
     return wall, kern, entry, st, info
@@ -565,7 +587,8 @@ def main():
     print(f"bench: {n} particles, {args.precision}, warmup {args.warmup}, steps {args.steps}", file=sys.stderr,
           flush=True)  # fmt: skip
     learned = args.precision == "mixed" and not args.no_learned
-    wall, kern, entry, st, info = run_gpu(ic, cfg, args.steps, args.warmup, learned=learned)
+    driver = "stepper" if args.precision == "mixed" else "eager"
+    wall, kern, entry, st, info = run_gpu(ic, cfg, args.steps, args.warmup, learned=learned, driver=driver)
     value = n * args.steps / wall
     print(f"bench: {value:.4g} particle-steps/s", file=sys.stderr, flush=True)
     # algorithmic work of the pair kernels, from this run's own lists and counters
@@ -612,7 +635,8 @@ def main():
         "data": "synthetic (seeded stream-crash initial conditions)",
         "config": {"workload": "stream crash, 2 x 524,288-particle streams (1,048,576)" if n == 1_048_576 else
                    f"stream crash, {n} particles", "precision": args.precision, "kernel": "cubic, eta 1.2",
-                   "viscosity": "Monaghan alpha 1 beta 2 + Balsara", "cooling": "power law rho^2 T^0.5"},
+                   "viscosity": "Monaghan alpha 1 beta 2 + Balsara", "cooling": "power law rho^2 T^0.5",
+                   "driver": driver},
         "roofline": {"bound": bound, "kernel": dom, "achieved": achieved, "peak": peak, "unit": unit,
                      "frac": achieved / peak, "traffic": _traffic(dom, args.precision),
                      "avg_launch_ms": avg_s * 1e3,
@@ -633,15 +657,20 @@ def main():
         other = "fp64" if args.precision == "mixed" else "mixed"
         del st
         out["compare"] = {}
-        alongside = [other] + (["mixed"] if learned else [])  # and the mixed mode without the learned start
-        for name in alongside:
-            print(f"bench: {name} alongside", file=sys.stderr, flush=True)
-            w2, k2, _, _, info2 = run_gpu(ic, config(name), args.steps, args.warmup)
+        # the other precision (eager steps), the mixed mode without the learned start (same driver),
+        # and the headline's own configuration stepped eagerly (coupled.stable_dt + coupled.step)
+        alongside = [(other, other, False, "eager")]
+        if learned:
+            alongside += [("mixed", "mixed", False, driver), ("mixed+learned, eager steps", "mixed", True, "eager")]
+        for label, name, lrn, drv in alongside:
+            print(f"bench: {label} alongside", file=sys.stderr, flush=True)
+            w2, k2, _, _, info2 = run_gpu(ic, config(name), args.steps, args.warmup, learned=lrn, driver=drv)
             v2 = n * args.steps / w2
             gk = k2.get("k_gather_mixed", k2.get("k_gather_solve", (0, 0.0)))
-            out["compare"][name] = {"value": v2, "ms_per_step": w2 / args.steps * 1e3, "value_ratio": value / v2,
-                                    "newton_iters": info2["newton_iters"], "gather_launches_per_step": gk[0] / args.steps,
-                                    "kernels_ms_per_step": _per_step(k2, args.steps)}  # fmt: skip
+            out["compare"][label] = {"value": v2, "ms_per_step": w2 / args.steps * 1e3, "value_ratio": value / v2,
+                                     "driver": drv, "newton_iters": info2["newton_iters"],
+                                     "gather_launches_per_step": gk[0] / args.steps,
+                                     "kernels_ms_per_step": _per_step(k2, args.steps)}  # fmt: skip
     if args.drift_steps > 0:
         runs = {args.precision + ("+learned" if learned else ""): info["_cfg"] if learned else cfg}
         if not args.no_compare:

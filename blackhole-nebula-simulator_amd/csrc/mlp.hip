@@ -136,6 +136,15 @@ __global__ void k_mlp_pack(const float* __restrict__ P, short* __restrict__ wpac
 
 static constexpr int kInferTiles = 4;  // 16-particle tiles per wavefront
 
+// tanh from the hardware exponential and reciprocal, 1 - 2 / (e^{2x} + 1) (absolute error about
+// 1e-7, exact limits +-1): the inference rounds its activations to BF16 (8 significant bits), and
+// the libm form costs several times the instructions of the two MFMA layers it feeds.
+__device__ __forceinline__ float tanh_hw(float x)
+{
+    const float e = __builtin_amdgcn_exp2f(2.8853900817779268f * x);  // 2^(2 x / ln 2) = e^(2x)
+    return 1.f - 2.f * __builtin_amdgcn_rcpf(e + 1.f);
+}
+
 __global__ void __launch_bounds__(256)
 k_mlp_infer(const short* __restrict__ wpack, const float* __restrict__ P, const float* __restrict__ norm,
             const float* __restrict__ feat, int64_t n, float* __restrict__ out)
@@ -178,7 +187,7 @@ k_mlp_infer(const short* __restrict__ wpack, const float* __restrict__ P, const 
             f32x4 acc = {0.f, 0.f, 0.f, 0.f};
             acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1[mt], bx, acc, 0, 0, 0);
 #pragma unroll
-            for (int j = 0; j < 4; j++) h1[mt][j] = to_bf16(tanhf(acc[j] + b1[4 * mt + j]));
+            for (int j = 0; j < 4; j++) h1[mt][j] = to_bf16(tanh_hw(acc[j] + b1[4 * mt + j]));
         }
         float part = 0.f;
 #pragma unroll
@@ -187,7 +196,7 @@ k_mlp_infer(const short* __restrict__ wpack, const float* __restrict__ P, const 
 #pragma unroll
             for (int kt = 0; kt < 4; kt++) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[4 * mt + kt], h1[kt], acc, 0, 0, 0);
 #pragma unroll
-            for (int j = 0; j < 4; j++) part += tanhf(acc[j] + b2[4 * mt + j]) * w3[4 * mt + j];
+            for (int j = 0; j < 4; j++) part += tanh_hw(acc[j] + b2[4 * mt + j]) * w3[4 * mt + j];
         }
         // lanes l, l + 16, l + 32, l + 48 hold the four quarters of particle l % 16's hidden layer
         part += __shfl_xor(part, 16, 64);

@@ -526,17 +526,27 @@ def run_long(args):
         _, _, _, _, info = run_gpu(ic, cfg, 1, 1, learned=True)
         cfg = info["_cfg"]
     st = coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg)
+    sp = None
+    if args.precision == "mixed":  # the headline's driver
+        from nebula.sim.stepper import Stepper
+
+        sp = Stepper(st, cfg)
     windows = []
     done = 0
     while done < args.long_run:
         w = min(50, args.long_run - done)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        iters = 0.0
+        its = []
         for _ in range(w):
-            st = coupled.step(st, cfg, coupled.stable_dt(st, cfg))
-            iters += float(st.iterations.double().mean().item())
+            if sp is not None:
+                sp.step()
+                st = sp.state
+            else:
+                st = coupled.step(st, cfg, coupled.stable_dt(st, cfg))
+            its.append(st.iterations.double().mean())
         torch.cuda.synchronize()
+        iters = float(torch.stack(its).sum().item())
         done += w
         free, total = torch.cuda.mem_get_info()
         rec = {"steps": done, "t": round(st.t, 3), "ms_per_step": round((time.perf_counter() - t0) / w * 1e3, 3),
@@ -546,7 +556,8 @@ def run_long(args):
         windows.append(rec)
         print(f"bench: {json.dumps(rec)}", file=sys.stderr, flush=True)
     print(json.dumps({"workload": f"stream crash, {args.n} particles", "precision": args.precision,
-                      "learned_h": learned, "windows": windows}))
+                      "learned_h": learned, "driver": "stepper" if sp is not None else "eager",
+                      "fallbacks": None if sp is None else sp.fallbacks, "windows": windows}))
 
 
 def _per_step(d, steps):

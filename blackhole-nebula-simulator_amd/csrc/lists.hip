@@ -1056,19 +1056,18 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// f(q) and f'(q) of kernels_nb.py:12-43 in FP32, without branches: the cubic spline as
+// (2 - q)_+^3 / 4 - (1 - q)_+^3 (the same polynomial on each piece), so that the lanes of a
+// wavefront never split on q; both forms are exactly zero from q = 2 on.
 __device__ __forceinline__ void kernel_fdf32(float q, int kid, float& fq, float& dfq)
 {
     if (kid == 0) {
-        if (q < 1.f) {
-            fq = 1.f - 1.5f * q * q + 0.75f * q * q * q;
-            dfq = -3.f * q + 2.25f * q * q;
-        } else {
-            const float t = 2.f - q;
-            fq = 0.25f * t * t * t;
-            dfq = -0.75f * t * t;
-        }
+        const float t2 = fmaxf(2.f - q, 0.f), t1 = fmaxf(1.f - q, 0.f);
+        const float s2 = t2 * t2, s1 = t1 * t1;
+        fq = 0.25f * s2 * t2 - s1 * t1;
+        dfq = 3.f * s1 - 0.75f * s2;
     } else {
-        const float x = 1.f - 0.5f * q, x2 = x * x;
+        const float x = fmaxf(1.f - 0.5f * q, 0.f), x2 = x * x;
         fq = x2 * x2 * x2 * (35.f / 12.f * q * q + 3.f * q + 1.f);
         dfq = -7.f / 3.f * q * x2 * x2 * x * (5.f * q + 2.f);
     }
@@ -1085,9 +1084,8 @@ struct DensSums {
     float a1 = 0.f, c1 = 0.f, a2 = 0.f, c2 = 0.f;
     __device__ __forceinline__ void add(float q, float mk, int kid)
     {
-        if (!(q < 2.f)) return;
         float fq, dfq;
-        kernel_fdf32(q, kid, fq, dfq);
+        kernel_fdf32(q, kid, fq, dfq);  // zero terms past the support
         const float t1 = mk * fq - c1;
         const float u1 = a1 + t1;
         c1 = (u1 - a1) - t1;

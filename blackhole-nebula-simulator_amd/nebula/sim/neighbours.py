@@ -125,10 +125,9 @@ class BlockedLists:
             start = self.block_offset.cpu().numpy()
         off = np.zeros(n + 1, np.int64)
         np.cumsum(ln, out=off[1:])
-        out = np.empty(off[-1], np.int64)
-        for i in range(n):
-            b = start[i // 64] + i % 64
-            out[off[i]:off[i + 1]] = idx[b : b + 64 * ln[i] : 64]
+        row = np.repeat(np.arange(n, dtype=np.int64), ln)
+        k = np.arange(off[-1], dtype=np.int64) - off[row]  # entry k of its row
+        out = idx[start[row // 64] + row % 64 + 64 * k].astype(np.int64)
         return Neighbours(off, out, self.box)
 
 
@@ -167,9 +166,10 @@ class RowLists:
         idx = self.index.cpu().numpy()
         off = np.zeros(n + 1, np.int64)
         np.cumsum(ln, out=off[1:])
-        out = np.empty(off[-1], np.int64)
-        for i in range(n):
-            out[off[i]:off[i + 1]] = np.sort(idx[start[i]:start[i] + ln[i]])
+        row = np.repeat(np.arange(n, dtype=np.int64), ln)
+        k = np.arange(off[-1], dtype=np.int64) - off[row]
+        vals = idx[start[row] + k].astype(np.int64)
+        out = np.sort(row * max(n, 1) + vals) - row * max(n, 1)  # each row sorted in place
         return Neighbours(off, out, self.box)
 
 

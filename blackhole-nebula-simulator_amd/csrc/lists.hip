@@ -289,6 +289,37 @@ k_cell_id_seg(const double* __restrict__ pos, int64_t n, const GridSeg* __restri
     if (part) block_partial<false>(bad, part);
 }
 
+// The grid of the last order pass, reused by the density pass that follows it (positions and
+// search lengths the same, permuted into cell order): every particle must lie inside its
+// segment's grid (unclamped cell coordinates) and inside its cell's range of the sorted order
+// (start[c] <= i < start[c + 1]); that makes the ranges exactly this arrangement's cells.
+// part: the particles that fail.
+__global__ void __launch_bounds__(kBlock)
+k_check_grid_seg(const double* __restrict__ pos, int64_t n, const GridSeg* __restrict__ segs,
+                 const int32_t* __restrict__ cloud, const uint32_t* __restrict__ start, long long* __restrict__ part)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    long long bad = 0;
+    if (i < n) {
+        const GridSeg& G = segs[cloud[i]];
+        int32_t cc[3];
+        bool inside = true;
+        for (int a = 0; a < 3; a++) {
+            const double f = floor((pos[3 * i + a] - G.g.lo[a]) * G.g.inv[a]);
+            inside = inside && f >= 0.0 && f < (double)G.g.nc[a];
+            cc[a] = inside ? (int32_t)f : 0;
+        }
+        if (!inside) {
+            bad = 1;
+        } else {
+            const uint32_t c =
+                G.base + ((uint32_t)cc[0] * (uint32_t)G.g.nc[1] + (uint32_t)cc[1]) * (uint32_t)G.g.nc[2] + (uint32_t)cc[2];
+            bad = start[c] <= (uint32_t)i && (uint32_t)i < start[c + 1] ? 0 : 1;
+        }
+    }
+    block_partial<false>(bad, part);
+}
+
 // cell of each particle, particles per cell, and (part, one partial per workgroup) the number of
 // neighbouring pairs i, i + 1 out of cell order: zero when the particles are stored sorted.
 __global__ void __launch_bounds__(kBlock)
@@ -2035,6 +2066,7 @@ int build_grid(nb_ctx* ctx, hipStream_t s, const double* pos, const double* h, i
     // every list builder starts here and reuses the context's list buffers: lists of an
     // earlier build can no longer be emitted
     ctx->lists_ready = 0;
+    ctx->fx_grid_from_order = 0;
     ctx->hydro_ready = 0;
     ctx->hydro_mixed_ready = 0;
     ctx->order_ident = 0;
@@ -2168,6 +2200,7 @@ int build_grid_seg(nb_ctx* ctx, hipStream_t s, const double* pos, const double* 
     ctx->hydro_ready = 0;
     ctx->hydro_mixed_ready = 0;
     ctx->order_ident = 0;
+    ctx->fx_grid_from_order = 0;
     if (nseg < 1 || off[0] != 0 || off[nseg] != n) return ctx_fail(ctx, "segmented grid: bad segment offsets");
     for (int k = 0; k < nseg; k++)
         if (off[k + 1] <= off[k]) return ctx_fail(ctx, "segmented grid: empty segment");
@@ -2929,6 +2962,18 @@ int build_grid_seg_fixed(nb_ctx* ctx, hipStream_t s, const double* pos, const do
     ctx->lists_ready = 0;
     ctx->hydro_ready = 0;
     ctx->hydro_mixed_ready = 0;
+    if (!sort && ctx->fx_grid_from_order) {
+        // the order pass's grid, checked against the particles as they now lie (a failure is
+        // reported as not in cell order: the caller redoes the step eagerly)
+        ctx->fx_grid_from_order = 0;
+        hipLaunchKernelGGL(k_check_grid_seg, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, n, (const GridSeg*)ctx->segs,
+                           cloud, (const uint32_t*)ctx->cell_start, ctx->part);
+        finish_partials(ctx, s, grid_of(n), 1, 0u, 0);
+        hipLaunchKernelGGL(k_status_from, dim3(1), dim3(64), 0, s, (const long long*)ctx->scal, 0, status, kStUnsorted);
+        ctx->order_ident = 1;
+        return NB_OK;
+    }
+    ctx->fx_grid_from_order = 0;
     if (ctx->segpart_bytes < seg_part_bytes(n, nseg)) return ctx_fail(ctx, "fixed pass: box partials not reserved");
     launch_seg_bbox(ctx, s, pos, h, n, off_dev, nseg, ctx->red);
     hipLaunchKernelGGL(k_plan_seg, dim3(1), dim3(64), 0, s, ctx->red, off_dev, nseg, cell_factor, (double)ctx->fx_cells,
@@ -2942,6 +2987,7 @@ int build_grid_seg_fixed(nb_ctx* ctx, hipStream_t s, const double* pos, const do
         int rc = sort_fixed(ctx, s, n);
         if (rc) return rc;
         ctx->order_ident = 0;
+        ctx->fx_grid_from_order = 1;  // the density pass after the permutation may reuse this grid
     } else {
         finish_partials(ctx, s, grid_of(n), 1, 0u, 0);
         hipLaunchKernelGGL(k_status_from, dim3(1), dim3(64), 0, s, (const long long*)ctx->scal, 0, status, kStUnsorted);
@@ -2959,6 +3005,7 @@ extern "C" int32_t nb_fixed_reserve(nb_ctx* ctx, int64_t n, int32_t nseg, int64_
         cells_cap <= 0 || cells_cap >= 2147483647ll)
         return ctx_fail(ctx, "nb_fixed_reserve: bad arguments");
     ctx->fx_ready = 0;
+    ctx->fx_grid_from_order = 0;
     const size_t u64n = sizeof(unsigned long long) * (size_t)(n + 1);
     if (reserve_reductions(ctx, n) ||
         ctx_reserve(ctx, (void**)&ctx->red, &ctx->red_bytes, sizeof(double) * 9 * (size_t)nseg + 8 * (size_t)(nseg + 1) + 4096) ||

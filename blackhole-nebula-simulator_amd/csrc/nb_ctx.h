@@ -79,6 +79,7 @@ struct nb_ctx {
     int fx_ready = 0;
     int64_t fx_n = 0, fx_nseg = 0, fx_gather = 0, fx_extras = 0, fx_index = 0, fx_cells = 0;
     int fx_bits = 32;  // radix-sort key bits of fx_cells
+    int fx_grid_from_order = 0;  // the last fixed order pass's grid awaits its density pass
     void* segs = nullptr; size_t segs_bytes = 0;       // grid table of an ensemble's segments
     void* segpart = nullptr; size_t segpart_bytes = 0; // per-workgroup partials of the segment boxes
     std::vector<char> host_segs;                       // its host copy

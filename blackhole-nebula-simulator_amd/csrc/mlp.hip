@@ -48,9 +48,11 @@ __device__ __forceinline__ short to_bf16(float f)
 __global__ void k_mlp_features(const double* __restrict__ h, const double* __restrict__ rho,
                                const double* __restrict__ div_v, const double* __restrict__ curl_v,
                                const double* __restrict__ nneigh, const double* __restrict__ x,
-                               const double* __restrict__ u, int64_t rows, int64_t stride, double dt, double dt_prev,
-                               float* __restrict__ out)
+                               const double* __restrict__ u, int64_t rows, int64_t stride, double dt_host,
+                               double dt_prev_host, const double* __restrict__ dts_dev, float* __restrict__ out)
 {
+    // dts_dev: (dt, dt_prev) on the device (a recorded step), else the host values
+    const double dt = dts_dev ? dts_dev[0] : dt_host, dt_prev = dts_dev ? dts_dev[1] : dt_prev_host;
     const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (q >= rows) return;
     const int64_t i = q * stride;
@@ -403,8 +405,22 @@ extern "C" int32_t nb_mlp_features(nb_ctx* ctx, void* stream, const double* h_pr
     const int64_t rows = (n + stride - 1) / stride;
     if (rows == 0) return NB_OK;
     hipLaunchKernelGGL(k_mlp_features, dim3(blocks_of(rows, 256)), dim3(256), 0, (hipStream_t)stream, h_prev, rho_prev,
-                       div_v, curl_v, nneigh, x, u, rows, stride, dt, dt_prev, out);
+                       div_v, curl_v, nneigh, x, u, rows, stride, dt, dt_prev, (const double*)nullptr, out);
     return mlp_launched(ctx, "nb_mlp_features");
+}
+
+extern "C" int32_t nb_mlp_features_dev(nb_ctx* ctx, void* stream, const double* h_prev, const double* rho_prev,
+                                       const double* div_v, const double* curl_v, const double* nneigh,
+                                       const double* x, const double* u, int64_t n, int64_t stride,
+                                       const double* dts_dev, float* out)
+{
+    if (!ctx || n < 0 || stride < 1 || !h_prev || !rho_prev || !x || !u || !out || !dts_dev)
+        return ctx_fail(ctx, "nb_mlp_features_dev: bad arguments");
+    const int64_t rows = (n + stride - 1) / stride;
+    if (rows == 0) return NB_OK;
+    hipLaunchKernelGGL(k_mlp_features, dim3(blocks_of(rows, 256)), dim3(256), 0, (hipStream_t)stream, h_prev, rho_prev,
+                       div_v, curl_v, nneigh, x, u, rows, stride, 0.0, 0.0, dts_dev, out);
+    return mlp_launched(ctx, "nb_mlp_features_dev");
 }
 
 extern "C" int32_t nb_mlp_target(nb_ctx* ctx, void* stream, const double* h_prev, const double* h_final, int64_t n,

@@ -133,6 +133,16 @@ __global__ void k_axpy(const double* __restrict__ y, const double* __restrict__ 
     if (i < m) out[i] = y[i] + s * a[i];
 }
 
+// k_axpy with s = scale * s_dev[0] read on the device (a step whose dt the host writes between
+// replays of a recorded graph); the same arithmetic as k_axpy with s formed on the host
+__global__ void k_axpy_dev(const double* __restrict__ y, const double* __restrict__ a,
+                           const double* __restrict__ s_dev, double scale, int64_t m, double* __restrict__ out)
+{
+    const double s = scale * s_dev[0];
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < m) out[i] = y[i] + s * a[i];
+}
+
 // eos.py:21-28 with hydro.py:130-131: P = (gamma-1) rho u, c_s = sqrt(gamma (gamma-1) u);
 // counts particles with u <= 0.
 __global__ void k_eos_ideal(const double* __restrict__ rho, const double* __restrict__ u, double gm1,
@@ -169,9 +179,11 @@ __global__ void k_cool_beta(const double* __restrict__ u, const double* __restri
 // Optically thin power-law cooling du/dt = -k rho sqrt(u) above u_floor, integrated exactly at
 // fixed rho: sqrt(u) falls linearly by k rho dt / 2, stopping at the floor.
 __global__ void k_cool_powerlaw(const double* __restrict__ u, const double* __restrict__ rho,
-                                const double* __restrict__ mass, double u_floor, double k, double dt,
-                                int64_t n, double* __restrict__ u_out, double* __restrict__ e_part)
+                                const double* __restrict__ mass, double u_floor, double k, double dt_host,
+                                const double* __restrict__ dt_dev, int64_t n, double* __restrict__ u_out,
+                                double* __restrict__ e_part)
 {
+    const double dt = dt_dev ? dt_dev[0] : dt_host;
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     double ui = u[i];
@@ -375,6 +387,15 @@ extern "C" int32_t nb_axpy(nb_ctx* ctx, void* stream, const double* y, const dou
     return launch_check_s(ctx, "nb_axpy");
 }
 
+extern "C" int32_t nb_axpy_dev(nb_ctx* ctx, void* stream, const double* y, const double* a, const double* s_dev,
+                               double scale, int64_t m, double* out)
+{
+    if (!ctx || m < 0 || !s_dev) return ctx_fail(ctx, "nb_axpy_dev: bad arguments");
+    if (m == 0) return NB_OK;
+    hipLaunchKernelGGL(k_axpy_dev, dim3(grid_s(m)), dim3(kBlockS), 0, (hipStream_t)stream, y, a, s_dev, scale, m, out);
+    return launch_check_s(ctx, "nb_axpy_dev");
+}
+
 extern "C" int32_t nb_eos_ideal(nb_ctx* ctx, void* stream, const double* rho, const double* u, double gamma,
                                 int64_t n, double* pressure, double* cs, int32_t* nbad_dev)
 {
@@ -403,8 +424,19 @@ extern "C" int32_t nb_cool_powerlaw(nb_ctx* ctx, void* stream, const double* u, 
     if (!ctx || n < 0) return ctx_fail(ctx, "nb_cool_powerlaw: bad arguments");
     if (n == 0) return NB_OK;
     hipLaunchKernelGGL(k_cool_powerlaw, dim3(grid_s(n)), dim3(kBlockS), 0, (hipStream_t)stream, u, rho, mass,
-                       u_floor, k, dt, n, u_out, e_part);
+                       u_floor, k, dt, (const double*)nullptr, n, u_out, e_part);
     return launch_check_s(ctx, "nb_cool_powerlaw");
+}
+
+extern "C" int32_t nb_cool_powerlaw_dev(nb_ctx* ctx, void* stream, const double* u, const double* rho,
+                                        const double* mass, double u_floor, double k, const double* dt_dev,
+                                        int64_t n, double* u_out, double* e_part)
+{
+    if (!ctx || n < 0 || !dt_dev) return ctx_fail(ctx, "nb_cool_powerlaw_dev: bad arguments");
+    if (n == 0) return NB_OK;
+    hipLaunchKernelGGL(k_cool_powerlaw, dim3(grid_s(n)), dim3(kBlockS), 0, (hipStream_t)stream, u, rho, mass,
+                       u_floor, k, 0.0, dt_dev, n, u_out, e_part);
+    return launch_check_s(ctx, "nb_cool_powerlaw_dev");
 }
 
 static int32_t reduce_two_level(nb_ctx* ctx, hipStream_t s, int w, int op, int nparts, double* host_out)

@@ -84,9 +84,11 @@ SIGNATURES = {
     "nb_glf_step": (_i32, [_vp, _vp, _vp, _vp, _vp, _f64, _i64, _vp, _vp, _vp, _vp]),
     "nb_gn_v_from_p": (_i32, [_vp, _vp, _vp, _vp, _i64, _vp]),
     "nb_axpy": (_i32, [_vp, _vp, _vp, _vp, _f64, _i64, _vp]),
+    "nb_axpy_dev": (_i32, [_vp, _vp, _vp, _vp, _vp, _f64, _i64, _vp]),
     "nb_eos_ideal": (_i32, [_vp, _vp, _vp, _vp, _f64, _i64, _vp, _vp, _vp]),
     "nb_cool_beta": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _f64, _f64, _f64, _i64, _vp, _vp]),
     "nb_cool_powerlaw": (_i32, [_vp, _vp, _vp, _vp, _vp, _f64, _f64, _f64, _i64, _vp, _vp]),
+    "nb_cool_powerlaw_dev": (_i32, [_vp, _vp, _vp, _vp, _vp, _f64, _f64, _vp, _i64, _vp, _vp]),
     "nb_dt_minima": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _f64, _vp, _i64, _PD]),
     "nb_dt_minima_dev": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _f64, _vp, _i64, _vp]),
     "nb_ledger": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _PD]),
@@ -128,6 +130,7 @@ SIGNATURES = {
     "nb_expand_seg": (_i32, [_vp, _vp, _vp, _vp, _i64, _f64, _vp]),
     "nb_mlp_param_count": (_i32, []),
     "nb_mlp_features": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _f64, _f64, _vp]),
+    "nb_mlp_features_dev": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _vp, _vp]),
     "nb_mlp_target": (_i32, [_vp, _vp, _vp, _vp, _i64, _i64, _vp]),
     "nb_mlp_infer": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _vp]),
     "nb_mlp_h_start": (_i32, [_vp, _vp, _vp, _vp, _i64, ctypes.c_float, _vp, _vp]),

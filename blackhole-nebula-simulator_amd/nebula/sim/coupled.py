@@ -160,6 +160,18 @@ def _axpy(y, a, s, out=None):
     return out
 
 
+def _axpy_dev(y, a, s_dev, scale):
+    """_axpy with s = scale * s_dev[0] read on the device (a recorded step's dt)."""
+    out = torch.empty_like(y)
+    _hip.call("nb_axpy_dev", _hip.ptr(y), _hip.ptr(a), _hip.ptr(s_dev), float(scale), y.numel(), _hip.ptr(out))
+    return out
+
+
+def _step_dt(fixed):
+    """The device (dt, dt_prev) of a step recorded for replay (fixed.dts), or None."""
+    return None if fixed is None else getattr(fixed, "dts", None)
+
+
 def _full(n, v):
     t = _dev.empty(n)
     _hip.call("nb_fill", _hip.ptr(t), float(v), n)
@@ -229,16 +241,21 @@ def p_from_v(cfg: CoupledConfig, x: np.ndarray, v: np.ndarray) -> np.ndarray:
 def gravity_flow(cfg: CoupledConfig, x, p, dt, fixed=None):
     """coupled.py:131-142 (GN generalized leapfrog, or a straight drift without the hole). fixed:
     failures are counted in fixed.status[6] instead of raising (no host synchronisation)."""
+    dts = _step_dt(fixed)
     if _gn(cfg) and fixed is not None:
         n = int(x.shape[0])
         x1, p1 = _dev.empty((n, 3)), _dev.empty((n, 3))
         iters = _dev.empty(n, torch.int32)
-        _hip.call("nb_glf_step", _hip.ptr(x), _hip.ptr(p), None, float(dt), n, _hip.ptr(x1), _hip.ptr(p1),
+        dt_p = None
+        if dts is not None:  # the recorded step's dt, one value per particle
+            dt_p = _dev.empty(n)
+            _hip.call("nb_expand_seg", _hip.ptr(dts), _hip.ptr(fixed.cloud), n, 1.0, _hip.ptr(dt_p))
+        _hip.call("nb_glf_step", _hip.ptr(x), _hip.ptr(p), _hip.ptr(dt_p), float(dt), n, _hip.ptr(x1), _hip.ptr(p1),
                   _hip.ptr(iters), _hip.ptr(fixed.status[6:]))  # fmt: skip
     elif _gn(cfg):
         x1, p1, _ = glf_device(x, p, float(dt))
     else:
-        x1 = _axpy(x, p, dt)
+        x1 = _axpy(x, p, dt) if dts is None else _axpy_dev(x, p, dts, 1.0)
         p1 = p.clone()
     if cfg.hydro.box is not None:
         raise ValueError("periodic boxes are not supported by the coupled GPU step")
@@ -406,8 +423,15 @@ def step(st: GasState, cfg: CoupledConfig, dt: float, fixed=None) -> GasState:
     hc = cfg.hydro
     n = int(st.x.shape[0])
     half = 0.5 * dt
-    p_half = _axpy(st.p, st.a_hydro, half)
-    u_half = _axpy(st.u, st.dudt, half)
+    dts = _step_dt(fixed)
+    if dts is not None and (math.isfinite(cfg.cooling_beta) or (hc.viscosity and hc.switch)):
+        raise ValueError("a recorded step covers power-law cooling and the constant or Balsara viscosity")
+
+    def kick(y, a):
+        return _axpy(y, a, half) if dts is None else _axpy_dev(y, a, dts, 0.5)
+
+    p_half = kick(st.p, st.a_hydro)
+    u_half = kick(st.u, st.dudt)
     x1, p_flow = gravity_flow(cfg, st.x, p_half, dt, fixed)
     e_rad = st.e_radiated_dev.clone()
     if math.isfinite(cfg.cooling_beta) or cfg.cooling_powerlaw:
@@ -422,13 +446,17 @@ def step(st: GasState, cfg: CoupledConfig, dt: float, fixed=None) -> GasState:
             u_half = u_cool
         if cfg.cooling_powerlaw:
             u_cool = _dev.empty(n)
-            _hip.call("nb_cool_powerlaw", _hip.ptr(u_half), _hip.ptr(st.rho), _hip.ptr(st.mass), float(u_floor),
-                      float(units.powerlaw_cooling_coefficient(hc.gamma)), float(dt), n, _hip.ptr(u_cool),
-                      _hip.ptr(e_part))  # fmt: skip
+            k_cool = float(units.powerlaw_cooling_coefficient(hc.gamma))
+            if dts is None:
+                _hip.call("nb_cool_powerlaw", _hip.ptr(u_half), _hip.ptr(st.rho), _hip.ptr(st.mass), float(u_floor),
+                          k_cool, float(dt), n, _hip.ptr(u_cool), _hip.ptr(e_part))  # fmt: skip
+            else:
+                _hip.call("nb_cool_powerlaw_dev", _hip.ptr(u_half), _hip.ptr(st.rho), _hip.ptr(st.mass),
+                          float(u_floor), k_cool, _hip.ptr(dts), n, _hip.ptr(u_cool), _hip.ptr(e_part))  # fmt: skip
             _hip.call("nb_sum_into", _hip.ptr(e_part), n, _hip.ptr(e_rad))
             u_half = u_cool
-    p_pred = _axpy(p_flow, st.a_hydro, half)
-    u_pred = _axpy(u_half, st.dudt, half)
+    p_pred = kick(p_flow, st.a_hydro)
+    u_pred = kick(u_half, st.dudt)
     order = grid_permutation(cfg, x1, st.h, st, fixed)
     model = _learned(cfg)
     fields = [x1, p_flow, p_pred, u_half, u_pred, st.a_hydro, st.mass, st.h, st.alpha, st.perm]
@@ -450,7 +478,7 @@ def step(st: GasState, cfg: CoupledConfig, dt: float, fixed=None) -> GasState:
     if model is not None:
         it = iter(moved[len(fields):])
         rho_p, div_p, curl_p, nn_p = (None if t is None else next(it) for t in extra)
-        feat = model.features(h, rho_p, div_p, curl_p, nn_p, x1, u_pred, dt, st.dt_prev, stride=1)
+        feat = model.features(h, rho_p, div_p, curl_p, nn_p, x1, u_pred, dt, st.dt_prev, stride=1, dts=dts)
         if hc.learned_h:
             h_start = model.h_start(h, feat)
             fallbacks = model.last_fallbacks
@@ -459,7 +487,7 @@ def step(st: GasState, cfg: CoupledConfig, dt: float, fixed=None) -> GasState:
         fixed.pairs = (model, feat, h, new.h)
     else:
         log_pairs(cfg, model, feat, h, new.h)
-    return replace(new, p=_axpy(p_flow, new.a_hydro, half), u=_axpy(u_half, new.dudt, half), e_radiated_dev=e_rad,
+    return replace(new, p=kick(p_flow, new.a_hydro), u=kick(u_half, new.dudt), e_radiated_dev=e_rad,
                    dt_prev=float(dt), learned_fallbacks_dev=fallbacks)
 
 

@@ -120,13 +120,21 @@ class LearnedH:
                        record=False)  # fmt: skip
 
     # ------------------------------------------------------------------ data
-    def features(self, h_prev, rho_prev, div_v, curl_v, nneigh, x, u, dt, dt_prev, stride: int = 1) -> torch.Tensor:
+    def features(self, h_prev, rho_prev, div_v, curl_v, nneigh, x, u, dt, dt_prev, stride: int = 1,
+                 dts=None) -> torch.Tensor:  # fmt: skip
+        """Raw features of every stride-th particle. dts: (dt, dt_prev) in device memory (a step
+        recorded for replay), replacing the host values."""
         n = int(h_prev.shape[0])
         rows = (n + stride - 1) // stride
         out = _dev.empty((rows, N_FEATURES), torch.float32)
-        _hip.call("nb_mlp_features", _hip.ptr(h_prev), _hip.ptr(rho_prev), _hip.ptr(div_v), _hip.ptr(curl_v),
-                  _hip.ptr(nneigh), _hip.ptr(x), _hip.ptr(u), n, int(stride), float(dt), float(dt_prev),
-                  _hip.ptr(out))  # fmt: skip
+        if dts is None:
+            _hip.call("nb_mlp_features", _hip.ptr(h_prev), _hip.ptr(rho_prev), _hip.ptr(div_v), _hip.ptr(curl_v),
+                      _hip.ptr(nneigh), _hip.ptr(x), _hip.ptr(u), n, int(stride), float(dt), float(dt_prev),
+                      _hip.ptr(out))  # fmt: skip
+        else:
+            _hip.call("nb_mlp_features_dev", _hip.ptr(h_prev), _hip.ptr(rho_prev), _hip.ptr(div_v), _hip.ptr(curl_v),
+                      _hip.ptr(nneigh), _hip.ptr(x), _hip.ptr(u), n, int(stride), _hip.ptr(dts),
+                      _hip.ptr(out))  # fmt: skip
         return out
 
     @staticmethod

@@ -21,6 +21,7 @@ training included: its pairs are logged after the status check, in the eager ste
 
 from __future__ import annotations
 
+import ctypes
 from dataclasses import replace
 
 import numpy as np
@@ -28,7 +29,7 @@ import torch
 
 from .. import _hip
 from . import coupled
-from .captured import STATUS_NAMES, _Fixed
+from .captured import STATUS_NAMES, _Fixed, ctypes_void
 
 
 class Stepper:
@@ -110,3 +111,122 @@ class Stepper:
             self.dt = coupled.dt_from_minima(packed[16:20], cfg, new.nu is not None)
         self.steps += 1
         return dt
+
+
+# the state fields a recorded step carries from one replay to the next (those the state has)
+_FIELDS = ("x", "p", "u", "mass", "h", "alpha", "a_hydro", "dudt", "rho", "vsig_max", "c_s", "omega", "iterations",
+           "flags", "perm", "row_counts", "div_v", "curl_v", "nneigh", "e_radiated_dev")  # fmt: skip
+
+
+class RecordedStepper(Stepper):
+    """``Stepper`` whose step is recorded once as a hipGraph and replayed.
+
+    The recorded step is ``coupled.step`` through the fixed-capacity passes, with its dt (and the
+    previous dt the learned start's features use) read from device memory: the host writes them
+    before each replay. Each replay saves the state it starts from, runs the step, copies the new
+    state over the old and leaves the status word and the next dt's minima on the device; the
+    host reads them in one copy, as ``Stepper`` does. A replay with a nonzero status is undone
+    (the saved state restored), the step done by ``coupled.step`` and the graph recorded again.
+    The learned start's online training runs between replays, on the pairs of the replay.
+
+    Covers the physics a recorded step can take its dt from the device for: GN or Newtonian flow,
+    power-law cooling, constant or Balsara viscosity, the learned start (no beta cooling, no
+    Cullen-Dehnen switch, no disc viscosity). Replays are bit-identical to ``coupled.step``
+    (``tests/test_gpu_stepper.py``); the graph holds no memset node.
+    """
+
+    def __init__(self, st: coupled.GasState, cfg: coupled.CoupledConfig, headroom: float = 1.3):
+        hc = cfg.hydro
+        if hc.ns_nu > 0 or hc.ns_alpha > 0 or hc.disc_alpha_av > 0:
+            raise ValueError("the recorded step covers the cloud physics without disc viscosity")
+        super().__init__(st, cfg, headroom)
+        self.replays = 0
+        self._record()
+
+    def _record(self):
+        st = self.state
+        dev = st.x.device
+        self.fields = tuple(k for k in _FIELDS if getattr(st, k) is not None)
+        self.S = {k: getattr(st, k).clone() for k in self.fields}
+        self.backup = {k: torch.empty_like(v) for k, v in self.S.items()}
+        self.dts = torch.tensor([self.dt, st.dt_prev], dtype=torch.float64, device=dev)
+        self.fixed.dts = self.dts
+        self._template = st
+        self._host_dts = torch.empty(2, dtype=torch.float64).pin_memory()
+        # one pass of the recorded sequence outside the capture (first-call set-up), then the capture
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            self._body(commit=False)
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph(keep_graph=True)
+        with torch.cuda.graph(self.graph):
+            self._body(commit=True)
+        self.graph.instantiate()
+        torch.cuda.synchronize()
+        counts = np.zeros(16, np.int32)
+        _hip.check(_hip.lib().nb_graph_node_counts(ctypes_void(self.graph.raw_cuda_graph()),
+                                                   counts.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))),
+                   "nb_graph_node_counts")  # fmt: skip
+        self.node_counts = {"kernel": int(counts[0]), "memcpy": int(counts[1]), "memset": int(counts[2])}
+        if counts[2]:
+            raise RuntimeError("the recorded step holds memset nodes; it is not replayed")
+
+    def _body(self, commit: bool):
+        """The recorded step: S -> S (S saved in backup first), status and minima written."""
+        S = self.S
+        for k, v in S.items():
+            self.backup[k].copy_(v)
+        self.status.zero_()
+        self.fixed.pairs = None
+        st = replace(self._template, **{k: S[k] for k in self.fields})
+        new = coupled.step(st, self.cfg, self.dt, fixed=self.fixed)
+        _hip.call("nb_dt_minima_dev", _hip.ptr(new.h), _hip.ptr(new.vsig_max), _hip.ptr(new.a_hydro),
+                  _hip.ptr(new.nu), 0.25, _hip.ptr(new.x), self.n, _hip.ptr(self.minima))  # fmt: skip
+        self._pairs = self.fixed.pairs
+        self._fallbacks_dev = new.learned_fallbacks_dev
+        if commit:
+            for k in self.fields:
+                S[k].copy_(getattr(new, k))
+        else:
+            for k in self.fields:  # the untimed pass leaves the state as it was
+                S[k].copy_(self.backup[k])
+
+    def step(self) -> float:
+        """One replayed step of coupled.stable_dt(state); returns the dt taken."""
+        st, cfg, dt = self.state, self.cfg, self.dt
+        self._host_dts[0], self._host_dts[1] = dt, st.dt_prev
+        self.dts.copy_(self._host_dts, non_blocking=True)
+        self.graph.replay()
+        S = self.S
+        totals = torch.stack([S["row_counts"].sum(dtype=torch.int64), S["nneigh"].sum().to(torch.int64)]).double()
+        packed = torch.cat([self.status.double(), self.minima, totals]).cpu().numpy()  # the one synchronisation
+        status = packed[:16].astype(np.int64)
+        self.replays += 1
+        if np.any(status[:8] != 0):
+            self.fallbacks += 1
+            self.fallback_reasons.append({STATUS_NAMES[k]: int(status[k]) for k in range(8) if status[k]})
+            for k, v in self.backup.items():
+                S[k].copy_(v)
+            new = coupled.step(st, cfg, dt)
+            self.state = new
+            self.headroom = max(self.headroom, 1.3)
+            self._reserve()
+            self.dt = coupled.stable_dt(new, cfg)
+            self._record()
+        else:
+            model, feat, h_prev, h_final = self._pairs
+            coupled.log_pairs(cfg, model, feat, h_prev, h_final)
+            self.state = replace(self._template, t=st.t + dt, dt_prev=float(dt),
+                                 n_search_entries=int(packed[20]), n_tight_entries=int(packed[21]),
+                                 learned_fallbacks_dev=self._fallbacks_dev,
+                                 **{k: S[k] for k in self.fields})  # fmt: skip
+            self.dt = coupled.dt_from_minima(packed[16:20], cfg, st.nu is not None)
+        self.steps += 1
+        return dt
+
+    def sync_back(self) -> coupled.GasState:
+        """The replayed state as a GasState of its own (copies of the recorded step's buffers)."""
+        self.state = replace(self.state, **{k: v.clone() for k, v in self.S.items()})
+        return self.state

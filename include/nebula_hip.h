@@ -308,6 +308,10 @@ int32_t nb_gn_v_from_p(nb_ctx *ctx, void *stream, const double *x, const double 
 /* out = y + s a over m doubles (the half kicks of coupled.py:229-258). */
 int32_t nb_axpy(nb_ctx *ctx, void *stream, const double *y, const double *a, double s, int64_t m,
                 double *out);
+/* out = y + (scale s_dev[0]) a, the scalar read from device memory (no counterpart in the
+ * reference: the kicks of a step recorded for replay). */
+int32_t nb_axpy_dev(nb_ctx *ctx, void *stream, const double *y, const double *a, const double *s_dev, double scale,
+                    int64_t m, double *out);
 
 /* eos_state, ideal branch (hydro.py:121-139; eos.py:21-28); nbad_dev counts u <= 0. */
 int32_t nb_eos_ideal(nb_ctx *ctx, void *stream, const double *rho, const double *u, double gamma, int64_t n,
@@ -322,6 +326,11 @@ int32_t nb_cool_beta(nb_ctx *ctx, void *stream, const double *u, const double *x
  * (the reference has no such law; same place in the split as beta cooling). */
 int32_t nb_cool_powerlaw(nb_ctx *ctx, void *stream, const double *u, const double *rho, const double *mass,
                          double u_floor, double k, double dt, int64_t n, double *u_out, double *e_part);
+/* nb_cool_powerlaw with dt read from device memory (dt_dev[0]; no counterpart in the reference:
+ * a step recorded for replay, whose dt the host writes between replays). */
+int32_t nb_cool_powerlaw_dev(nb_ctx *ctx, void *stream, const double *u, const double *rho, const double *mass,
+                             double u_floor, double k, const double *dt_dev, int64_t n, double *u_out,
+                             double *e_part);
 
 /* stable_dt minima (coupled.py:213-226): out4 (host) = [min h/v_sig, min sqrt(h/|a|) over |a|>0,
  * min c_visc h^2/nu over nu>0 (nu may be NULL), min |x|]. */
@@ -392,6 +401,10 @@ int32_t nb_mlp_param_count(void);
 int32_t nb_mlp_features(nb_ctx *ctx, void *stream, const double *h_prev, const double *rho_prev, const double *div_v,
                         const double *curl_v, const double *nneigh, const double *x, const double *u, int64_t n,
                         int64_t stride, double dt, double dt_prev, float *out);
+/* nb_mlp_features with (dt, dt_prev) read from device memory (dts_dev[0], dts_dev[1]). */
+int32_t nb_mlp_features_dev(nb_ctx *ctx, void *stream, const double *h_prev, const double *rho_prev,
+                            const double *div_v, const double *curl_v, const double *nneigh, const double *x,
+                            const double *u, int64_t n, int64_t stride, const double *dts_dev, float *out);
 /* training target log h_final - log h_prev of the same rows */
 int32_t nb_mlp_target(nb_ctx *ctx, void *stream, const double *h_prev, const double *h_final, int64_t n,
                       int64_t stride, float *out);

@@ -5,6 +5,10 @@
   Balsara limiter, power-law cooling, GN flow) and the learned start learning online.
 - A step whose fixed capacities are too small reports it, is discarded and done again by
   coupled.step; the results are still coupled.step's.
+- RecordedStepper (the same step recorded once as a hipGraph, dt written to the device before
+  each replay): the same bits over ten replays, with and without the learned start; the graph
+  holds no memset node; an undersized recording falls back, records again and still gives
+  coupled.step's bits.
 """
 
 import copy
@@ -108,3 +112,56 @@ def test_stepper_overflow_falls_back_to_coupled_step():
     check("reported as a capacity overflow", True,
           bool(set(sp.fallback_reasons[0]) & {"gather rows", "extras", "tight index", "grid cells"}))
     _compare("stepper (with fallback) vs coupled.step", st, sp.state)
+
+
+@pytest.mark.parametrize("learned", [False, True])
+def test_recorded_stepper_equals_coupled_step(learned):
+    from nebula.ics import stream_crash
+    from nebula.sim import coupled
+    from nebula.sim.stepper import RecordedStepper
+
+    ic = stream_crash(16384, length=200.0, seed=20261014)
+    base = _config()
+    model_a = _trained_model(ic, base) if learned else None
+    model_b = copy.deepcopy(model_a) if learned else None
+    cfg_a, cfg_b = _config(model_a), _config(model_b)
+    print(f"recorded stepper: 16384-particle stream crash, learned start {learned}, 10 replays", flush=True)
+    st = coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg_a)
+    for _ in range(2):  # past the first steps, whose rows still grow from the initial conditions
+        st = coupled.step(st, cfg_a, coupled.stable_dt(st, cfg_a))
+    st_b = coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg_b)
+    for _ in range(2):
+        st_b = coupled.step(st_b, cfg_b, coupled.stable_dt(st_b, cfg_b))
+    rs = RecordedStepper(st_b, cfg_b)
+    print("recorded graph nodes:", rs.node_counts)
+    check("no memset node in the recorded step", 0, rs.node_counts["memset"])
+    dts, rdts = [], []
+    for _ in range(10):
+        dt = coupled.stable_dt(st, cfg_a)
+        dts.append(dt)
+        st = coupled.step(st, cfg_a, dt)
+        rdts.append(rs.step())
+    bit_equal("time steps", np.array(dts), np.array(rdts))
+    _compare("recorded stepper vs coupled.step", st, rs.state)
+    print("fallbacks:", rs.fallbacks, rs.fallback_reasons, "replays", rs.replays)
+    check("replays without a fallback", True, rs.fallbacks <= 2)
+    if learned:
+        bit_equal("online-trained weights", model_a.params.cpu().numpy(), model_b.params.cpu().numpy())
+
+
+def test_recorded_stepper_overflow_records_again():
+    from nebula.ics import stream_crash
+    from nebula.sim import coupled
+    from nebula.sim.stepper import RecordedStepper
+
+    ic = stream_crash(16384, length=200.0, seed=7)
+    cfg = _config()
+    print("recorded stepper: capacities a third of the need, 4 replays", flush=True)
+    st = coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg)
+    rs = RecordedStepper(coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg), cfg, headroom=0.3)
+    for _ in range(4):
+        st = coupled.step(st, cfg, coupled.stable_dt(st, cfg))
+        rs.step()
+    print("fallbacks:", rs.fallbacks, rs.fallback_reasons)
+    check("the undersized recording fell back", True, rs.fallbacks >= 1)
+    _compare("recorded stepper (with fallback) vs coupled.step", st, rs.state)

@@ -1720,34 +1720,31 @@ k_tight_emit_rm(const double* __restrict__ pos, const uint32_t* __restrict__ rid
     const float mu = (float)(F.m_uniform > 0.0 ? F.m_uniform : 1.0);
     DensSums sums;
     int32_t o = 0;
-    for (int64_t kb = 0;; kb += kG) {
-        const bool in_range = kb < len;
-        if (__ballot(in_range) == 0) break;
-        const int64_t k = kb + glane;
-        bool pred = false;
-        uint32_t j = 0;
-        if (in_range && k < len) {
-            float d2;
-            if (k < ng) {
-                j = ridx[g0 + k];
-                d2 = rd2[g0 + k];
-                if (fin) sums.add(sqrt32(d2) * inv_hf, F.rmass ? F.rmass[g0 + k] : mu, F.kid);
-            } else {
-                j = ebuf[e0 + (k - ng)];
-                d2 = ed2[e0 + (k - ng)];
-            }
-            const double rr = NB_R_KERN * pymax(hi, hf[j]);
-            const double rr2 = rr * rr;
-            const double df = (double)d2;
-            if (fabs(df - rr2) <= 2.5e-7 * rr2) {
-                const double dx = pos[3 * j] - pos[3 * ii];
-                const double dy = pos[3 * j + 1] - pos[3 * ii + 1];
-                const double dz = pos[3 * j + 2] - pos[3 * ii + 2];
-                pred = dx * dx + dy * dy + dz * dz < rr2;
-            } else {
-                pred = df < rr2;
-            }
+    // entry k of the merged row: its index j, stored FP32 d2 and (for the density sums) mass
+    auto load = [&](int64_t k, uint32_t& j, float& d2, float& mk) {
+        if (k < ng) {
+            j = ridx[g0 + k];
+            d2 = rd2[g0 + k];
+            mk = F.rmass ? F.rmass[g0 + k] : mu;
+        } else {
+            j = ebuf[e0 + (k - ng)];
+            d2 = ed2[e0 + (k - ng)];
         }
+    };
+    // kept at the final h: the stored d2 decides unless it lies within its rounding of the radius
+    auto keep = [&](uint32_t j, float d2, double hj) -> bool {
+        const double rr = NB_R_KERN * pymax(hi, hj);
+        const double rr2 = rr * rr;
+        const double df = (double)d2;
+        if (fabs(df - rr2) <= 2.5e-7 * rr2) {
+            const double dx = pos[3 * j] - pos[3 * ii];
+            const double dy = pos[3 * j + 1] - pos[3 * ii + 1];
+            const double dz = pos[3 * j + 2] - pos[3 * ii + 2];
+            return dx * dx + dy * dy + dz * dz < rr2;
+        }
+        return df < rr2;
+    };
+    auto emit = [&](bool pred, uint32_t j) {
         const unsigned long long bal = __ballot(pred);
         const unsigned gb = (unsigned)((bal >> gshift) & ((1ull << kG) - 1));
         if (pred) {
@@ -1756,6 +1753,24 @@ k_tight_emit_rm(const double* __restrict__ pos, const uint32_t* __restrict__ rid
             else if (status) status[kStIndex] = 1;
         }
         o += __popc(gb);
+    };
+    // two batches of kG entries per pass, their loads issued together (the neighbour's h depends
+    // on the index load): the same entries, sums and output order as one batch at a time
+    for (int64_t kb = 0;; kb += 2 * kG) {
+        if (__ballot(kb < len) == 0) break;
+        const int64_t ka = kb + glane, kc = kb + kG + glane;
+        const bool va = ka < len, vc = kc < len;
+        uint32_t ja = 0, jc = 0;
+        float da = 0.f, dc = 0.f, ma = mu, mc = mu;
+        if (va) load(ka, ja, da, ma);
+        if (vc) load(kc, jc, dc, mc);
+        const double ha = va ? hf[ja] : 0.0, hc = vc ? hf[jc] : 0.0;
+        if (fin && va && ka < ng) sums.add(sqrt32(da) * inv_hf, ma, F.kid);
+        if (fin && vc && kc < ng) sums.add(sqrt32(dc) * inv_hf, mc, F.kid);
+        const bool pa = va && keep(ja, da, ha);
+        const bool pc = vc && keep(jc, dc, hc);
+        emit(pa, ja);
+        emit(pc, jc);
     }
     double s1, s2;
     sums.total(s1, s2);

@@ -111,6 +111,19 @@ __device__ __forceinline__ void sep32(const double* pos, double pa0, double pa1,
     d2 = (float)min_image(pa2 - pos[3 * b + 2], box.l[2]);
 }
 
+// the neighbour's position and packed record, loaded before the pair's arithmetic
+template <class R>
+struct Nb {
+    R rec;
+    double x0, x1, x2;
+};
+
+template <class R>
+__device__ __forceinline__ Nb<R> load_nb(const double* __restrict__ pos, const R* __restrict__ recs, int64_t b)
+{
+    return Nb<R>{recs[b], pos[3 * b], pos[3 * b + 1], pos[3 * b + 2]};
+}
+
 // sph.py:525-652 without reconstruction: pressure force with grad-h terms, Monaghan viscosity on
 // approaching pairs, conductivity (pressure or velocity signal), P dV work, largest v_sig.
 template <class L, int W>
@@ -132,12 +145,11 @@ k_forces_mixed(const double* __restrict__ pos, const PF* __restrict__ pf, const 
     Kahan ax, ay, az, work, visc, cond;
     float vmax = 0.f;
     const int64_t m = lists.count(a);
-    for (int64_t k = glane; k < m; k += W) {
-        const int32_t b = lists.at(a, k);
-        if (b == a) continue;
-        const PF B = pf[b];
-        float dx0, dx1, dx2;
-        sep32(pos, pa0, pa1, pa2, b, box, dx0, dx1, dx2);
+    auto pair = [&](const Nb<PF>& N) {
+        const PF& B = N.rec;
+        const float dx0 = (float)min_image(pa0 - N.x0, box.l[0]);
+        const float dx1 = (float)min_image(pa1 - N.x1, box.l[1]);
+        const float dx2 = (float)min_image(pa2 - N.x2, box.l[2]);
         const float r2 = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
         const float inv_r = rsqrt32(r2);
         const float r = r2 * inv_r;
@@ -167,6 +179,15 @@ k_forces_mixed(const double* __restrict__ pos, const PF* __restrict__ pf, const 
         ay.add(-f * ry);
         az.add(-f * rz);
         work.add(B.c.x * vdotr * fa);
+    };
+    // two entries per pass, both neighbours' records loaded before either pair's arithmetic: the
+    // same pairs in the same order per lane as one entry at a time (self skipped)
+    for (int64_t k = glane; k < m; k += 2 * W) {
+        const int32_t b0 = lists.at(a, k);
+        const int32_t b1 = k + W < m ? lists.at(a, k + W) : (int32_t)a;
+        const Nb<PF> N0 = load_nb(pos, pf, b0), N1 = load_nb(pos, pf, b1);
+        if (b0 != a) pair(N0);
+        if (b1 != a) pair(N1);
     }
     const double sx = gsum<W>(ax.value()), sy = gsum<W>(ay.value()), sz = gsum<W>(az.value());
     const double sw = gsum<W>(work.value()), sv = gsum<W>(visc.value()), sc = gsum<W>(cond.value());
@@ -203,13 +224,13 @@ __device__ __forceinline__ void gradients_mixed_row(const double* __restrict__ p
         for (int c = 0; c < 3; c++) aa[c] = (float)accel[3 * a + c];
     int cnt = 0;
     const int64_t m = lists.count(a);
-    for (int64_t k = glane; k < m; k += W) {
-        const int32_t b = lists.at(a, k);
-        if (b == a) continue;
+    auto pair = [&](const Nb<PG>& N, int32_t b) {
         cnt++;
-        const PG B = pg[b];
+        const PG& B = N.rec;
         float dx[3];
-        sep32(pos, pa0, pa1, pa2, b, box, dx[0], dx[1], dx[2]);
+        dx[0] = (float)min_image(pa0 - N.x0, box.l[0]);
+        dx[1] = (float)min_image(pa1 - N.x1, box.l[1]);
+        dx[2] = (float)min_image(pa2 - N.x2, box.l[2]);
         const float r2 = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
         const float inv_r = rsqrt32(r2);
         const float g = A.b.x * df32(r2 * inv_r * A.a.w, kid) * inv_r;
@@ -232,6 +253,14 @@ __device__ __forceinline__ void gradients_mixed_row(const double* __restrict__ p
                 Gv[3 * i + j].add(dv[i] * gx[j]);
                 if (kAcc) Ga[kAcc ? 3 * i + j : 0].add(da[i] * gx[j]);
             }
+    };
+    // two entries per pass, both neighbours' records loaded first (same pairs and order per lane)
+    for (int64_t k = glane; k < m; k += 2 * W) {
+        const int32_t b0 = lists.at(a, k);
+        const int32_t b1 = k + W < m ? lists.at(a, k + W) : (int32_t)a;
+        const Nb<PG> N0 = load_nb(pos, pg, b0), N1 = load_nb(pos, pg, b1);
+        if (b0 != a) pair(N0, b0);
+        if (b1 != a) pair(N1, b1);
     }
     double rm[9], gvm[9], gam[9];
     {

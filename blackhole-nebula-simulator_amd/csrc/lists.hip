@@ -2923,9 +2923,28 @@ size_t fixed_tmp_bytes(int64_t n, int64_t cells)
 }
 
 // exclusive scan with kernels only (count entries of in into out; in may not alias out)
+// The fixed-capacity passes run either eagerly (the single-synchronisation driver) or while a
+// stream is being recorded into a graph. Only the recording needs the kernel-only sort and scan:
+// the library's record memsets, which have misbehaved on replays.
+static bool capturing(hipStream_t s)
+{
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
+}
+
 template <class T>
 int scan_fixed(nb_ctx* ctx, hipStream_t s, const T* in, T* out, int64_t count)
 {
+    if (!capturing(s)) {
+        // not being recorded: the library scan (fewer launches; its memsets are harmless here)
+        size_t bytes = 0;
+        if (hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)count, s) != hipSuccess)
+            return ctx_fail(ctx, "hipcub scan sizing failed");
+        if (ctx_reserve(ctx, (void**)&ctx->cub_tmp, &ctx->cub_tmp_bytes, bytes)) return NB_ERR_HIP;
+        if (hipcub::DeviceScan::ExclusiveSum(ctx->cub_tmp, bytes, in, out, (int)count, s) != hipSuccess)
+            return ctx_fail(ctx, "hipcub scan failed");
+        return NB_OK;
+    }
     const int64_t tiles = (count + kScanTile - 1) / kScanTile;
     if (sizeof(T) * (size_t)(tiles + 1) > ctx->fx_tmp_bytes) return ctx_fail(ctx, "fixed pass: scan scratch not reserved");
     T* sums = (T*)ctx->fx_tmp;
@@ -2939,6 +2958,18 @@ int scan_fixed(nb_ctx* ctx, hipStream_t s, const T* in, T* out, int64_t count)
 // ctx->order (keys and ping-pong buffers clobbered)
 int sort_fixed(nb_ctx* ctx, hipStream_t s, int64_t n)
 {
+    if (!capturing(s)) {
+        // not being recorded: the library's (also stable) radix sort over the same bits
+        size_t bytes = 0;
+        if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, ctx->cell_id, ctx->keys_out, ctx->iota, ctx->order,
+                                               (int)n, 0, ctx->fx_bits, s) != hipSuccess)
+            return ctx_fail(ctx, "hipcub sort sizing failed");
+        if (ctx_reserve(ctx, (void**)&ctx->cub_tmp, &ctx->cub_tmp_bytes, bytes)) return NB_ERR_HIP;
+        if (hipcub::DeviceRadixSort::SortPairs(ctx->cub_tmp, bytes, ctx->cell_id, ctx->keys_out, ctx->iota, ctx->order,
+                                               (int)n, 0, ctx->fx_bits, s) != hipSuccess)
+            return ctx_fail(ctx, "hipcub sort failed");
+        return NB_OK;
+    }
     const int64_t nblocks = (n + kSortThreads - 1) / kSortThreads;
     const int64_t hn = (int64_t)kRadix * nblocks;
     const int64_t tiles = (hn + kScanTile - 1) / kScanTile;

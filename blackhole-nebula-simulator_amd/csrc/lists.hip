@@ -1072,7 +1072,7 @@ k_tight_emit(const uint32_t* __restrict__ gbuf, const double* __restrict__ dbuf,
 // others are solved in full by k_solve_slow_mixed.
 static constexpr int kG = 16;
 static constexpr int kCols = 2 * kG;  // columns per chunk: two per lane
-static constexpr int kTab = 256;      // candidate slots per lane group in LDS (multiple of kG)
+static constexpr int kTab = 384;      // candidate slots per lane group in LDS (multiple of kG)
 
 // Status words (int32, device) of a step recorded for replay: what did not fit, or what the
 // caller must redo eagerly. Words 6 and 7 are written by nb_glf_step (failures) and

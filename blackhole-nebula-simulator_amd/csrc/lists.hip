@@ -1218,7 +1218,9 @@ __device__ __forceinline__ bool column_zrange(double ux, double uy, double zc, d
 // segments that outgrow their capacity keep counting (the pass is repeated with capacities from
 // the exact counts). kSeg: an ensemble of
 // independent systems; particle i searches only the cells of its own system cloud[i], whose grid
-// and first cell come from segs.
+// and first cell come from segs. kMasses (masses differ): rows hold indices and FP32 d2 only; row
+// i stores its particle's FP32 mass in rmass[i], which the later passes over the rows read by
+// index (4 bytes per particle instead of 4 per row entry).
 template <bool kMasses, bool kSeg>
 __global__ void __launch_bounds__(256, 5)
 k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const GridSeg* __restrict__ segs,
@@ -1256,7 +1258,6 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const Gr
     const int64_t cap = (int64_t)(goff[ii + 1] - gb0);
     uint32_t* row = ridx + gb0;
     float* drow = rd2 + gb0;
-    float* mrow = kMasses ? rmass + gb0 : nullptr;
     __shared__ uint32_t ctab[256 / kG][kTab];  // one candidate table per lane group
     uint32_t* tab = ctab[threadIdx.x / kG];
     // every loop below runs the same number of times on all lanes of the wave (ballot bounds),
@@ -1347,14 +1348,16 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const Gr
                 if (slot < cap) {
                     row[slot] = j;
                     drow[slot] = d2f;
-                    if (kMasses) mrow[slot] = mj;
                 }
                 sums.add(sqrt32(d2f) * inv_h0, kMasses ? mj : mu, A.kid);
             }
             m += __popc(gb);
         }
     }
-    if (live && glane == 0) rowcnt[i] = m;
+    if (live && glane == 0) {
+        rowcnt[i] = m;
+        if (kMasses) rmass[i] = (float)A.mass[i];  // the FP32 mass the sums of later passes take
+    }
     // the loop's first pass at h0 (sph.py:260-290), the Newton update and convergence test in FP64
     double S1, S2;
     sums.total(S1, S2);
@@ -1389,11 +1392,11 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const Gr
 }
 
 // the two density sums of row i at inverse length inv_h, from the stored row (group-wide)
-__device__ __forceinline__ void row_dens_sums(const float* drow, const float* mrow, int32_t m, int glane, float inv_h,
-                                              int kid, float mu, double& s1, double& s2)
+__device__ __forceinline__ void row_dens_sums(const float* drow, const uint32_t* irow, const float* mf, int32_t m,
+                                              int glane, float inv_h, int kid, float mu, double& s1, double& s2)
 {
     DensSums sums;
-    for (int32_t k = glane; k < m; k += kG) sums.add(sqrt32(drow[k]) * inv_h, mrow ? mrow[k] : mu, kid);
+    for (int32_t k = glane; k < m; k += kG) sums.add(sqrt32(drow[k]) * inv_h, mf ? mf[irow[k]] : mu, kid);
     sums.total(s1, s2);
 }
 
@@ -1401,8 +1404,8 @@ __device__ __forceinline__ void row_dens_sums(const float* drow, const float* mr
 // (sums FP32 compensated, Newton steps and tests FP64), rho and Omega at the final h.
 template <bool kMasses>
 __global__ void __launch_bounds__(256)
-k_solve_slow_mixed(const double* __restrict__ h_search, int64_t n, const float* __restrict__ rd2,
-                   const float* __restrict__ rmass, const unsigned long long* __restrict__ goff,
+k_solve_slow_mixed(const double* __restrict__ h_search, int64_t n, const uint32_t* __restrict__ ridx,
+                   const float* __restrict__ rd2, const float* __restrict__ rmass, const unsigned long long* __restrict__ goff,
                    const int32_t* __restrict__ rowcnt, MixArgs A)
 {
     const int glane = threadIdx.x & (kG - 1);
@@ -1411,7 +1414,8 @@ k_solve_slow_mixed(const double* __restrict__ h_search, int64_t n, const float* 
     // the stored row (shorter than rowcnt only after an overflow the caller is told about)
     const int32_t m = (int32_t)min((unsigned long long)rowcnt[i], goff[i + 1] - goff[i]);
     const float* drow = rd2 + goff[i];
-    const float* mrow = kMasses ? rmass + goff[i] : nullptr;
+    const uint32_t* irow = ridx + goff[i];
+    const float* mf = kMasses ? rmass : nullptr;
     const float mu = (float)(A.m_uniform > 0.0 ? A.m_uniform : 1.0);
     const double mscale = kMasses ? 1.0 : (A.m_uniform > 0.0 ? A.m_uniform / (double)mu : 1.0);
     const double m_i = A.mass[i];
@@ -1424,7 +1428,7 @@ k_solve_slow_mixed(const double* __restrict__ h_search, int64_t n, const float* 
     double s1, s2;
     while (it < 50) {
         it++;
-        row_dens_sums(drow, mrow, m, glane, (float)(1.0 / hh), A.kid, mu, s1, s2);
+        row_dens_sums(drow, irow, mf, m, glane, (float)(1.0 / hh), A.kid, mu, s1, s2);
         const double inv_h = 1.0 / hh;
         const double rs = s1 * mscale * A.cnorm * ipow3(inv_h);
         const double dws = -s2 * mscale * A.cnorm * ipow4(inv_h);
@@ -1442,14 +1446,14 @@ k_solve_slow_mixed(const double* __restrict__ h_search, int64_t n, const float* 
         hh = h_new;
     }
     if (hh >= r_lim * (1 - 1e-3)) {
-        row_dens_sums(drow, mrow, m, glane, (float)(1.0 / r_lim), A.kid, mu, s1, s2);
+        row_dens_sums(drow, irow, mf, m, glane, (float)(1.0 / r_lim), A.kid, mu, s1, s2);
         const double rs_lim = s1 * mscale * A.cnorm * ipow3(1.0 / r_lim);
         if (rs_lim - m_i * ipow3(A.eta / r_lim) < 0) {
             hh = r_lim;
             flag = r_lim >= A.h_max ? 1 : 2;
         }
     }
-    row_dens_sums(drow, mrow, m, glane, (float)(1.0 / hh), A.kid, mu, s1, s2);
+    row_dens_sums(drow, irow, mf, m, glane, (float)(1.0 / hh), A.kid, mu, s1, s2);
     const double inv_h = 1.0 / hh;
     const double rs = s1 * mscale * A.cnorm * ipow3(inv_h);
     const double dws = -s2 * mscale * A.cnorm * ipow4(inv_h);
@@ -1679,7 +1683,7 @@ static void sort_extras(nb_ctx* ctx, hipStream_t s, int64_t n, unsigned blocks, 
 
 // what the emission pass needs to finish the rows converged in one Newton step
 struct FinArgs {
-    const float* rmass;  // stored row masses (NULL: uniform masses)
+    const float* rmass;  // FP32 particle masses, by particle index (NULL: uniform masses)
     double m_uniform;
     int kid;
     double cnorm;
@@ -1725,7 +1729,7 @@ k_tight_emit_rm(const double* __restrict__ pos, const uint32_t* __restrict__ rid
         if (k < ng) {
             j = ridx[g0 + k];
             d2 = rd2[g0 + k];
-            mk = F.rmass ? F.rmass[g0 + k] : mu;
+            mk = F.rmass ? F.rmass[j] : mu;
         } else {
             j = ebuf[e0 + (k - ng)];
             d2 = ed2[e0 + (k - ng)];
@@ -2802,7 +2806,8 @@ extern "C" int32_t nb_hydro_density_mixed(nb_ctx* ctx, void* stream, const doubl
         if (rc) return rc;
         const size_t cells = (size_t)tot[0], ecells = (size_t)tot[1];
         if (ctx_reserve(ctx, (void**)&ctx->gbuf, &ctx->gbuf_bytes, sizeof(uint32_t) * cells) ||
-            ctx_reserve(ctx, (void**)&ctx->fscratch, &ctx->fscratch_bytes, sizeof(float) * cells * (masses ? 2 : 1)) ||
+            ctx_reserve(ctx, (void**)&ctx->fscratch, &ctx->fscratch_bytes,
+                        sizeof(float) * (cells + (masses ? (size_t)n : 0))) ||
             ctx_reserve(ctx, (void**)&ctx->ebuf, &ctx->ebuf_bytes, sizeof(uint32_t) * ecells) ||
             ctx_reserve(ctx, (void**)&ctx->edbuf, &ctx->edbuf_bytes, sizeof(float) * ecells))
             return NB_ERR_HIP;
@@ -2836,10 +2841,10 @@ extern "C" int32_t nb_hydro_density_mixed(nb_ctx* ctx, void* stream, const doubl
     if (slow_rows > 0) {
         if (masses)
             NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<true>, dim3(blocks), dim3(tile), 0, s, h_search, n,
-                     ctx->fscratch, rmass, ctx->rm_go, ctx->rowcnt, A);
+                     ctx->gbuf, ctx->fscratch, rmass, ctx->rm_go, ctx->rowcnt, A);
         else
             NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<false>, dim3(blocks), dim3(tile), 0, s, h_search,
-                     n, ctx->fscratch, rmass, ctx->rm_go, ctx->rowcnt, A);
+                     n, ctx->gbuf, ctx->fscratch, rmass, ctx->rm_go, ctx->rowcnt, A);
     }
     if (ctx_reserve(ctx, (void**)&ctx->ebuf_s, &ctx->ebuf_s_bytes, sizeof(uint32_t) * (size_t)tot[1]) ||
         ctx_reserve(ctx, (void**)&ctx->edbuf_s, &ctx->edbuf_s_bytes, sizeof(float) * (size_t)tot[1]))
@@ -3075,7 +3080,7 @@ extern "C" int32_t nb_fixed_reserve(nb_ctx* ctx, int64_t n, int32_t nseg, int64_
         ctx_reserve(ctx, (void**)&ctx->rm_to, &ctx->rm_to_bytes, u64n) ||
         ctx_reserve(ctx, (void**)&ctx->gbuf, &ctx->gbuf_bytes, sizeof(uint32_t) * (size_t)gather_cap) ||
         ctx_reserve(ctx, (void**)&ctx->fscratch, &ctx->fscratch_bytes,
-                    sizeof(float) * (size_t)gather_cap * (masses ? 2 : 1)) ||
+                    sizeof(float) * ((size_t)gather_cap + (masses ? (size_t)n : 0))) ||
         ctx_reserve(ctx, (void**)&ctx->ebuf, &ctx->ebuf_bytes, sizeof(uint32_t) * (size_t)extras_cap) ||
         ctx_reserve(ctx, (void**)&ctx->edbuf, &ctx->edbuf_bytes, sizeof(float) * (size_t)extras_cap) ||
         ctx_reserve(ctx, (void**)&ctx->ebuf_s, &ctx->ebuf_s_bytes, sizeof(uint32_t) * (size_t)extras_cap) ||
@@ -3138,8 +3143,8 @@ extern "C" int32_t nb_hydro_density_mixed_fixed(nb_ctx* ctx, void* stream, const
     int rc = fixed_check(ctx, n, nseg, "nb_hydro_density_mixed_fixed: capacities not reserved for this size");
     if (rc) return rc;
     const bool masses = !(m_uniform > 0.0);
-    if (masses && ctx->fscratch_bytes < sizeof(float) * 2 * (size_t)ctx->fx_gather)
-        return ctx_fail(ctx, "nb_hydro_density_mixed_fixed: mass rows not reserved");
+    if (masses && ctx->fscratch_bytes < sizeof(float) * ((size_t)ctx->fx_gather + (size_t)n))
+        return ctx_fail(ctx, "nb_hydro_density_mixed_fixed: particle masses not reserved");
     rc = build_grid_seg_fixed(ctx, s, pos, h_search, n, cloud, seg_off_dev, nseg, cell_factor, false, status);
     if (rc) return rc;
     NB_TIMED(ctx, s, "k_pack_posh", k_pack_posh, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, h_search, n,
@@ -3176,10 +3181,10 @@ extern "C" int32_t nb_hydro_density_mixed_fixed(nb_ctx* ctx, void* stream, const
     hipLaunchKernelGGL(k_status_from, dim3(1), dim3(64), 0, s, (const long long*)ctx->scal, 3, status, (int)kStGather);
     if (masses)
         NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<true>, dim3(blocks), dim3(tile), 0, s, h_search, n,
-                 ctx->fscratch, rmass, ctx->rm_go, ctx->rowcnt, A);
+                 ctx->gbuf, ctx->fscratch, rmass, ctx->rm_go, ctx->rowcnt, A);
     else
         NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<false>, dim3(blocks), dim3(tile), 0, s, h_search, n,
-                 ctx->fscratch, rmass, ctx->rm_go, ctx->rowcnt, A);
+                 ctx->gbuf, ctx->fscratch, rmass, ctx->rm_go, ctx->rowcnt, A);
     hipLaunchKernelGGL(k_status_grow, dim3(grid_of(n)), dim3(kBlock), 0, s, flags, n, status);
     sort_extras(ctx, s, n, (unsigned)blocks, tile);
     NB_TIMED(ctx, s, "k_row_total_rm", k_row_total_rm, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, ctx->rowcnt,

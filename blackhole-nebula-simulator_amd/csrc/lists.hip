@@ -1218,15 +1218,16 @@ __device__ __forceinline__ bool column_zrange(double ux, double uy, double zc, d
 // segments that outgrow their capacity keep counting (the pass is repeated with capacities from
 // the exact counts). kSeg: an ensemble of
 // independent systems; particle i searches only the cells of its own system cloud[i], whose grid
-// and first cell come from segs. kMasses (masses differ): rows hold indices and FP32 d2 only; row
-// i stores its particle's FP32 mass in rmass[i], which the later passes over the rows read by
-// index (4 bytes per particle instead of 4 per row entry).
+// and first cell come from segs. Rows hold neighbour indices only (4 bytes per entry): the passes
+// that read them form d2 again from the positions, in the same FP64 operations, so they see the
+// same d2 bits. kMasses (masses differ): row i stores its particle's FP32 mass in rmass[i], which
+// those passes read by index.
 template <bool kMasses, bool kSeg>
 __global__ void __launch_bounds__(256, 5)
 k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const GridSeg* __restrict__ segs,
                const int32_t* __restrict__ cloud, const uint32_t* __restrict__ start,
-               const uint32_t* __restrict__ order, uint32_t* __restrict__ ridx, float* __restrict__ rd2,
-               float* __restrict__ rmass, const unsigned long long* __restrict__ goff, int32_t* __restrict__ rowcnt,
+               const uint32_t* __restrict__ order, uint32_t* __restrict__ ridx, float* __restrict__ rmass,
+               const unsigned long long* __restrict__ goff, int32_t* __restrict__ rowcnt,
                unsigned int* __restrict__ efill, uint32_t* __restrict__ ebuf, float* __restrict__ ed2,
                const unsigned long long* __restrict__ eoff, MixArgs A)
 {
@@ -1257,7 +1258,6 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const Gr
     const unsigned long long gb0 = goff[ii];
     const int64_t cap = (int64_t)(goff[ii + 1] - gb0);
     uint32_t* row = ridx + gb0;
-    float* drow = rd2 + gb0;
     __shared__ uint32_t ctab[256 / kG][kTab];  // one candidate table per lane group
     uint32_t* tab = ctab[threadIdx.x / kG];
     // every loop below runs the same number of times on all lanes of the wave (ballot bounds),
@@ -1345,10 +1345,7 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const Gr
                     }
                     emax = es + 1 > emax ? es + 1 : emax;
                 }
-                if (slot < cap) {
-                    row[slot] = j;
-                    drow[slot] = d2f;
-                }
+                if (slot < cap) row[slot] = j;
                 sums.add(sqrt32(d2f) * inv_h0, kMasses ? mj : mu, A.kid);
             }
             m += __popc(gb);
@@ -1391,12 +1388,22 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const Gr
     }
 }
 
-// the two density sums of row i at inverse length inv_h, from the stored row (group-wide)
-__device__ __forceinline__ void row_dens_sums(const float* drow, const uint32_t* irow, const float* mf, int32_t m,
-                                              int glane, float inv_h, int kid, float mu, double& s1, double& s2)
+// the two density sums of row i at inverse length inv_h, over the stored row (group-wide); d2 as
+// the gather formed it
+__device__ __forceinline__ void row_dens_sums(const double4* posh, const double4& pi, const uint32_t* irow,
+                                              const float* mf, int32_t m, int glane, float inv_h, int kid, float mu,
+                                              double& s1, double& s2)
 {
     DensSums sums;
-    for (int32_t k = glane; k < m; k += kG) sums.add(sqrt32(drow[k]) * inv_h, mf ? mf[irow[k]] : mu, kid);
+    for (int32_t k = glane; k < m; k += kG) {
+        const uint32_t j = irow[k];
+        const double4 pj = posh[j];
+        const double dx = pj.x - pi.x;
+        const double dy = pj.y - pi.y;
+        const double dz = pj.z - pi.z;
+        const float d2f = (float)(dx * dx + dy * dy + dz * dz);
+        sums.add(sqrt32(d2f) * inv_h, mf ? mf[j] : mu, kid);
+    }
     sums.total(s1, s2);
 }
 
@@ -1404,8 +1411,8 @@ __device__ __forceinline__ void row_dens_sums(const float* drow, const uint32_t*
 // (sums FP32 compensated, Newton steps and tests FP64), rho and Omega at the final h.
 template <bool kMasses>
 __global__ void __launch_bounds__(256)
-k_solve_slow_mixed(const double* __restrict__ h_search, int64_t n, const uint32_t* __restrict__ ridx,
-                   const float* __restrict__ rd2, const float* __restrict__ rmass, const unsigned long long* __restrict__ goff,
+k_solve_slow_mixed(const double* __restrict__ h_search, int64_t n, const double4* __restrict__ posh,
+                   const uint32_t* __restrict__ ridx, const float* __restrict__ rmass, const unsigned long long* __restrict__ goff,
                    const int32_t* __restrict__ rowcnt, MixArgs A)
 {
     const int glane = threadIdx.x & (kG - 1);
@@ -1413,7 +1420,7 @@ k_solve_slow_mixed(const double* __restrict__ h_search, int64_t n, const uint32_
     if (i >= n || !A.slow[i]) return;  // whole groups leave together
     // the stored row (shorter than rowcnt only after an overflow the caller is told about)
     const int32_t m = (int32_t)min((unsigned long long)rowcnt[i], goff[i + 1] - goff[i]);
-    const float* drow = rd2 + goff[i];
+    const double4 pi = posh[i];
     const uint32_t* irow = ridx + goff[i];
     const float* mf = kMasses ? rmass : nullptr;
     const float mu = (float)(A.m_uniform > 0.0 ? A.m_uniform : 1.0);
@@ -1428,7 +1435,7 @@ k_solve_slow_mixed(const double* __restrict__ h_search, int64_t n, const uint32_
     double s1, s2;
     while (it < 50) {
         it++;
-        row_dens_sums(drow, irow, mf, m, glane, (float)(1.0 / hh), A.kid, mu, s1, s2);
+        row_dens_sums(posh, pi, irow, mf, m, glane, (float)(1.0 / hh), A.kid, mu, s1, s2);
         const double inv_h = 1.0 / hh;
         const double rs = s1 * mscale * A.cnorm * ipow3(inv_h);
         const double dws = -s2 * mscale * A.cnorm * ipow4(inv_h);
@@ -1446,14 +1453,14 @@ k_solve_slow_mixed(const double* __restrict__ h_search, int64_t n, const uint32_
         hh = h_new;
     }
     if (hh >= r_lim * (1 - 1e-3)) {
-        row_dens_sums(drow, irow, mf, m, glane, (float)(1.0 / r_lim), A.kid, mu, s1, s2);
+        row_dens_sums(posh, pi, irow, mf, m, glane, (float)(1.0 / r_lim), A.kid, mu, s1, s2);
         const double rs_lim = s1 * mscale * A.cnorm * ipow3(1.0 / r_lim);
         if (rs_lim - m_i * ipow3(A.eta / r_lim) < 0) {
             hh = r_lim;
             flag = r_lim >= A.h_max ? 1 : 2;
         }
     }
-    row_dens_sums(drow, irow, mf, m, glane, (float)(1.0 / hh), A.kid, mu, s1, s2);
+    row_dens_sums(posh, pi, irow, mf, m, glane, (float)(1.0 / hh), A.kid, mu, s1, s2);
     const double inv_h = 1.0 / hh;
     const double rs = s1 * mscale * A.cnorm * ipow3(inv_h);
     const double dws = -s2 * mscale * A.cnorm * ipow4(inv_h);
@@ -1693,18 +1700,16 @@ struct FinArgs {
 };
 
 // Tightened rows (row i at toff[i], capacity its merged length): gather row entries, then the
-// row's extras (sorted), kept
-// when d2 < (2 max(h_i, h_j))^2 at the final h. The stored FP32 d2 decides unless it lies within
-// its rounding error of the radius; then d2 is formed again in FP64 as the search formed it, so
-// the rows are exactly tighten() of the search lists at h_final. The same pass sums rho and
-// Omega at the final h of the rows the gather finished in one Newton step (extras never count:
-// their d2 >= (2 h_search_i)^2 >= (2 h_i)^2).
+// row's extras (sorted), kept when d2 < (2 max(h_i, h_j))^2 at the final h, with d2 formed in
+// FP64 from the positions as the search formed it, so the rows are exactly tighten() of the
+// search lists at h_final. The same pass sums rho and Omega at the final h of the rows the gather
+// finished in one Newton step, over FP32 d2 (extras never count: their d2 >= (2 h_search_i)^2 >=
+// (2 h_i)^2).
 __global__ void __launch_bounds__(256)
-k_tight_emit_rm(const double* __restrict__ pos, const uint32_t* __restrict__ ridx, const float* __restrict__ rd2,
+k_tight_emit_rm(const double4* __restrict__ posh, const uint32_t* __restrict__ ridx,
                 const unsigned long long* __restrict__ goff, const int32_t* __restrict__ rowcnt,
                 const unsigned int* __restrict__ efill, const unsigned long long* __restrict__ eoff,
-                const uint32_t* __restrict__ ebuf, const float* __restrict__ ed2, const double* __restrict__ hf,
-                int64_t n, const unsigned long long* __restrict__ toff, int64_t* __restrict__ out_off,
+                const uint32_t* __restrict__ ebuf, int64_t n, const unsigned long long* __restrict__ toff, int64_t* __restrict__ out_off,
                 int32_t* __restrict__ out_len, int32_t* __restrict__ out_idx, unsigned long long idx_cap,
                 int32_t* __restrict__ status, FinArgs F, long long* __restrict__ part)
 {
@@ -1713,7 +1718,8 @@ k_tight_emit_rm(const double* __restrict__ pos, const uint32_t* __restrict__ rid
     const int64_t i = (xcd_tile() * (int64_t)blockDim.x + threadIdx.x) / kG;
     const bool live = i < n;
     const int64_t ii = live ? i : n - 1;
-    const double hi = hf[ii];
+    const double4 pi = posh[ii];
+    const double hi = pi.w;
     const unsigned long long g0 = goff[ii], e0 = eoff[ii], t0 = toff[ii];
     // the stored entries (the counts exceed them only after an overflow the caller is told about)
     const int32_t ng = live ? (int32_t)min((unsigned long long)rowcnt[ii], goff[ii + 1] - g0) : 0;
@@ -1724,29 +1730,25 @@ k_tight_emit_rm(const double* __restrict__ pos, const uint32_t* __restrict__ rid
     const float mu = (float)(F.m_uniform > 0.0 ? F.m_uniform : 1.0);
     DensSums sums;
     int32_t o = 0;
-    // entry k of the merged row: its index j, stored FP32 d2 and (for the density sums) mass
-    auto load = [&](int64_t k, uint32_t& j, float& d2, float& mk) {
+    // entry k of the merged row: its index j and (for the density sums) mass
+    auto load = [&](int64_t k, uint32_t& j, float& mk) {
         if (k < ng) {
             j = ridx[g0 + k];
-            d2 = rd2[g0 + k];
             mk = F.rmass ? F.rmass[j] : mu;
         } else {
             j = ebuf[e0 + (k - ng)];
-            d2 = ed2[e0 + (k - ng)];
         }
     };
-    // kept at the final h: the stored d2 decides unless it lies within its rounding of the radius
-    auto keep = [&](uint32_t j, float d2, double hj) -> bool {
+    auto dist2 = [&](const double4& pj) -> double {
+        const double dx = pj.x - pi.x;
+        const double dy = pj.y - pi.y;
+        const double dz = pj.z - pi.z;
+        return dx * dx + dy * dy + dz * dz;
+    };
+    // kept at the final h
+    auto keep = [&](double d2, double hj) -> bool {
         const double rr = NB_R_KERN * pymax(hi, hj);
-        const double rr2 = rr * rr;
-        const double df = (double)d2;
-        if (fabs(df - rr2) <= 2.5e-7 * rr2) {
-            const double dx = pos[3 * j] - pos[3 * ii];
-            const double dy = pos[3 * j + 1] - pos[3 * ii + 1];
-            const double dz = pos[3 * j + 2] - pos[3 * ii + 2];
-            return dx * dx + dy * dy + dz * dz < rr2;
-        }
-        return df < rr2;
+        return d2 < rr * rr;
     };
     auto emit = [&](bool pred, uint32_t j) {
         const unsigned long long bal = __ballot(pred);
@@ -1765,14 +1767,16 @@ k_tight_emit_rm(const double* __restrict__ pos, const uint32_t* __restrict__ rid
         const int64_t ka = kb + glane, kc = kb + kG + glane;
         const bool va = ka < len, vc = kc < len;
         uint32_t ja = 0, jc = 0;
-        float da = 0.f, dc = 0.f, ma = mu, mc = mu;
-        if (va) load(ka, ja, da, ma);
-        if (vc) load(kc, jc, dc, mc);
-        const double ha = va ? hf[ja] : 0.0, hc = vc ? hf[jc] : 0.0;
-        if (fin && va && ka < ng) sums.add(sqrt32(da) * inv_hf, ma, F.kid);
-        if (fin && vc && kc < ng) sums.add(sqrt32(dc) * inv_hf, mc, F.kid);
-        const bool pa = va && keep(ja, da, ha);
-        const bool pc = vc && keep(jc, dc, hc);
+        float ma = mu, mc = mu;
+        if (va) load(ka, ja, ma);
+        if (vc) load(kc, jc, mc);
+        const double4 qa = posh[va ? ja : (uint32_t)ii], qc = posh[vc ? jc : (uint32_t)ii];
+        const double ha = qa.w, hc = qc.w;
+        const double da = va ? dist2(qa) : 0.0, dc = vc ? dist2(qc) : 0.0;
+        if (fin && va && ka < ng) sums.add(sqrt32((float)da) * inv_hf, ma, F.kid);
+        if (fin && vc && kc < ng) sums.add(sqrt32((float)dc) * inv_hf, mc, F.kid);
+        const bool pa = va && keep(da, ha);
+        const bool pc = vc && keep(dc, hc);
         emit(pa, ja);
         emit(pc, jc);
     }
@@ -2806,17 +2810,16 @@ extern "C" int32_t nb_hydro_density_mixed(nb_ctx* ctx, void* stream, const doubl
         if (rc) return rc;
         const size_t cells = (size_t)tot[0], ecells = (size_t)tot[1];
         if (ctx_reserve(ctx, (void**)&ctx->gbuf, &ctx->gbuf_bytes, sizeof(uint32_t) * cells) ||
-            ctx_reserve(ctx, (void**)&ctx->fscratch, &ctx->fscratch_bytes,
-                        sizeof(float) * (cells + (masses ? (size_t)n : 0))) ||
+            ctx_reserve(ctx, (void**)&ctx->fscratch, &ctx->fscratch_bytes, sizeof(float) * (masses ? (size_t)n : 1)) ||
             ctx_reserve(ctx, (void**)&ctx->ebuf, &ctx->ebuf_bytes, sizeof(uint32_t) * ecells) ||
             ctx_reserve(ctx, (void**)&ctx->edbuf, &ctx->edbuf_bytes, sizeof(float) * ecells))
             return NB_ERR_HIP;
         if (hipMemsetAsync(ctx->efill, 0, sizeof(unsigned int) * n, s) != hipSuccess) return ctx_fail(ctx, "memset");
-        float* rmass = masses ? ctx->fscratch + cells : nullptr;
+        float* rmass = masses ? ctx->fscratch : nullptr;
 #define NB_GATHER_MIXED(M, S)                                                                                        \
     NB_TIMED(ctx, s, "k_gather_mixed", (k_gather_mixed<M, S>), dim3(blocks), dim3(tile), 0, s,                      \
-             (const double4*)ctx->posh, n, g, segs, cloud, ctx->cell_start, order_arg(ctx), ctx->gbuf, ctx->fscratch, \
-             rmass, ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->ebuf, (float*)ctx->edbuf, ctx->rm_eo, A)
+             (const double4*)ctx->posh, n, g, segs, cloud, ctx->cell_start, order_arg(ctx), ctx->gbuf, rmass,       \
+             ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->ebuf, (float*)ctx->edbuf, ctx->rm_eo, A)
         if (masses && segmented) NB_GATHER_MIXED(true, true);
         else if (masses) NB_GATHER_MIXED(true, false);
         else if (segmented) NB_GATHER_MIXED(false, true);
@@ -2837,14 +2840,14 @@ extern "C" int32_t nb_hydro_density_mixed(nb_ctx* ctx, void* stream, const doubl
     ctx->rm_gtotal = (int64_t)tot[0];
     ctx->rm_etotal = (int64_t)tot[1];
     const long long slow_rows = sc[1];
-    float* rmass = masses ? ctx->fscratch + (size_t)tot[0] : nullptr;
+    float* rmass = masses ? ctx->fscratch : nullptr;
     if (slow_rows > 0) {
         if (masses)
             NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<true>, dim3(blocks), dim3(tile), 0, s, h_search, n,
-                     ctx->gbuf, ctx->fscratch, rmass, ctx->rm_go, ctx->rowcnt, A);
+                     (const double4*)ctx->posh, ctx->gbuf, rmass, ctx->rm_go, ctx->rowcnt, A);
         else
             NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<false>, dim3(blocks), dim3(tile), 0, s, h_search,
-                     n, ctx->gbuf, ctx->fscratch, rmass, ctx->rm_go, ctx->rowcnt, A);
+                     n, (const double4*)ctx->posh, ctx->gbuf, rmass, ctx->rm_go, ctx->rowcnt, A);
     }
     if (ctx_reserve(ctx, (void**)&ctx->ebuf_s, &ctx->ebuf_s_bytes, sizeof(uint32_t) * (size_t)tot[1]) ||
         ctx_reserve(ctx, (void**)&ctx->edbuf_s, &ctx->edbuf_s_bytes, sizeof(float) * (size_t)tot[1]))
@@ -2886,16 +2889,20 @@ extern "C" int32_t nb_hydro_tight_mixed(nb_ctx* ctx, void* stream, const double*
     if (index_cap < ctx->lists_total)
         return ctx_fail(ctx, "nb_hydro_tight_mixed: index capacity below the search-list entries");
     const int64_t n = ctx->lists_n;
+    if (ctx->posh_bytes < sizeof(double4) * (size_t)n) return ctx_fail(ctx, "nb_hydro_tight_mixed: records not reserved");
     const bool masses = !(ctx->mix_m_uniform > 0.0);
-    const FinArgs F{masses ? ctx->fscratch + (size_t)ctx->rm_gtotal : nullptr, ctx->mix_m_uniform, ctx->mix_kid,
+    const FinArgs F{masses ? ctx->fscratch : nullptr, ctx->mix_m_uniform, ctx->mix_kid,
                     ctx->mix_cnorm, ctx->pending, rho, omega};
     const int tile = ctx->tile;
     const unsigned blocks = (unsigned)((n * kG + tile - 1) / tile);
     if (ctx->part_bytes < sizeof(long long) * (size_t)blocks)
         return ctx_fail(ctx, "nb_hydro_tight_mixed: reduction buffer smaller than the launch");
-    NB_TIMED(ctx, s, "k_tight_emit_rm", k_tight_emit_rm, dim3(blocks), dim3(tile), 0, s, pos, ctx->gbuf, ctx->fscratch,
-             ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->rm_eo, ctx->ebuf_s, ctx->edbuf_s, h_final, n, ctx->rm_to,
-             out_off, out_len, out_index, (unsigned long long)index_cap, (int32_t*)nullptr, F, ctx->part);
+    // (x, y, z, h_final) records: the emission reads one per row entry
+    NB_TIMED(ctx, s, "k_pack_posh", k_pack_posh, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, h_final, n,
+             (double4*)ctx->posh);
+    NB_TIMED(ctx, s, "k_tight_emit_rm", k_tight_emit_rm, dim3(blocks), dim3(tile), 0, s, (const double4*)ctx->posh,
+             ctx->gbuf, ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->rm_eo, ctx->ebuf_s, n, ctx->rm_to, out_off, out_len,
+             out_index, (unsigned long long)index_cap, (int32_t*)nullptr, F, ctx->part);
     finish_partials(ctx, s, blocks, 1, 0u, 8);
     long long t = 0;
     int rc = sync_copy(ctx, s, &t, (long long*)ctx->scal + 8, sizeof(t), "nb_hydro_tight_mixed: total copy failed");
@@ -3080,7 +3087,7 @@ extern "C" int32_t nb_fixed_reserve(nb_ctx* ctx, int64_t n, int32_t nseg, int64_
         ctx_reserve(ctx, (void**)&ctx->rm_to, &ctx->rm_to_bytes, u64n) ||
         ctx_reserve(ctx, (void**)&ctx->gbuf, &ctx->gbuf_bytes, sizeof(uint32_t) * (size_t)gather_cap) ||
         ctx_reserve(ctx, (void**)&ctx->fscratch, &ctx->fscratch_bytes,
-                    sizeof(float) * ((size_t)gather_cap + (masses ? (size_t)n : 0))) ||
+                    sizeof(float) * (masses ? (size_t)n : 1)) ||
         ctx_reserve(ctx, (void**)&ctx->ebuf, &ctx->ebuf_bytes, sizeof(uint32_t) * (size_t)extras_cap) ||
         ctx_reserve(ctx, (void**)&ctx->edbuf, &ctx->edbuf_bytes, sizeof(float) * (size_t)extras_cap) ||
         ctx_reserve(ctx, (void**)&ctx->ebuf_s, &ctx->ebuf_s_bytes, sizeof(uint32_t) * (size_t)extras_cap) ||
@@ -3143,7 +3150,7 @@ extern "C" int32_t nb_hydro_density_mixed_fixed(nb_ctx* ctx, void* stream, const
     int rc = fixed_check(ctx, n, nseg, "nb_hydro_density_mixed_fixed: capacities not reserved for this size");
     if (rc) return rc;
     const bool masses = !(m_uniform > 0.0);
-    if (masses && ctx->fscratch_bytes < sizeof(float) * ((size_t)ctx->fx_gather + (size_t)n))
+    if (masses && ctx->fscratch_bytes < sizeof(float) * (size_t)n)
         return ctx_fail(ctx, "nb_hydro_density_mixed_fixed: particle masses not reserved");
     rc = build_grid_seg_fixed(ctx, s, pos, h_search, n, cloud, seg_off_dev, nseg, cell_factor, false, status);
     if (rc) return rc;
@@ -3165,26 +3172,26 @@ extern "C" int32_t nb_hydro_density_mixed_fixed(nb_ctx* ctx, void* stream, const
     const int tile = ctx->tile;
     const int64_t blocks = (n * kG + tile - 1) / tile;
     const GridSeg* segs = (const GridSeg*)ctx->segs;
-    float* rmass = masses ? ctx->fscratch + (size_t)ctx->fx_gather : nullptr;
+    float* rmass = masses ? ctx->fscratch : nullptr;
     const Grid g_none{};
     if (masses)
         NB_TIMED(ctx, s, "k_gather_mixed", (k_gather_mixed<true, true>), dim3(blocks), dim3(tile), 0, s,
                  (const double4*)ctx->posh, n, g_none, segs, cloud, ctx->cell_start, (const uint32_t*)nullptr, ctx->gbuf,
-                 ctx->fscratch, rmass, ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->ebuf, (float*)ctx->edbuf, ctx->rm_eo, A);
+                 rmass, ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->ebuf, (float*)ctx->edbuf, ctx->rm_eo, A);
     else
         NB_TIMED(ctx, s, "k_gather_mixed", (k_gather_mixed<false, true>), dim3(blocks), dim3(tile), 0, s,
                  (const double4*)ctx->posh, n, g_none, segs, cloud, ctx->cell_start, (const uint32_t*)nullptr, ctx->gbuf,
-                 ctx->fscratch, rmass, ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->ebuf, (float*)ctx->edbuf, ctx->rm_eo, A);
+                 rmass, ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->ebuf, (float*)ctx->edbuf, ctx->rm_eo, A);
     // [2] extras overflow, [3] rows over capacity
     finish_partials(ctx, s, blocks, 5, 1u | 4u | 16u, 0);
     hipLaunchKernelGGL(k_status_from, dim3(1), dim3(64), 0, s, (const long long*)ctx->scal, 2, status, (int)kStExtras);
     hipLaunchKernelGGL(k_status_from, dim3(1), dim3(64), 0, s, (const long long*)ctx->scal, 3, status, (int)kStGather);
     if (masses)
         NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<true>, dim3(blocks), dim3(tile), 0, s, h_search, n,
-                 ctx->gbuf, ctx->fscratch, rmass, ctx->rm_go, ctx->rowcnt, A);
+                 (const double4*)ctx->posh, ctx->gbuf, rmass, ctx->rm_go, ctx->rowcnt, A);
     else
         NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<false>, dim3(blocks), dim3(tile), 0, s, h_search, n,
-                 ctx->gbuf, ctx->fscratch, rmass, ctx->rm_go, ctx->rowcnt, A);
+                 (const double4*)ctx->posh, ctx->gbuf, rmass, ctx->rm_go, ctx->rowcnt, A);
     hipLaunchKernelGGL(k_status_grow, dim3(grid_of(n)), dim3(kBlock), 0, s, flags, n, status);
     sort_extras(ctx, s, n, (unsigned)blocks, tile);
     NB_TIMED(ctx, s, "k_row_total_rm", k_row_total_rm, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, ctx->rowcnt,
@@ -3213,14 +3220,18 @@ extern "C" int32_t nb_hydro_tight_mixed_fixed(nb_ctx* ctx, void* stream, const d
     if (!pos || !h_final || !rho || !omega || !out_off || !out_len || !out_index || !status)
         return ctx_fail(ctx, "nb_hydro_tight_mixed_fixed: bad arguments");
     const int64_t n = ctx->lists_n;
+    if (ctx->posh_bytes < sizeof(double4) * (size_t)n)
+        return ctx_fail(ctx, "nb_hydro_tight_mixed_fixed: records not reserved");
     const bool masses = !(ctx->mix_m_uniform > 0.0);
-    const FinArgs F{masses ? ctx->fscratch + (size_t)ctx->rm_gtotal : nullptr, ctx->mix_m_uniform, ctx->mix_kid,
+    const FinArgs F{masses ? ctx->fscratch : nullptr, ctx->mix_m_uniform, ctx->mix_kid,
                     ctx->mix_cnorm, ctx->pending, rho, omega};
     const int tile = ctx->tile;
     const unsigned blocks = (unsigned)((n * kG + tile - 1) / tile);
-    NB_TIMED(ctx, s, "k_tight_emit_rm", k_tight_emit_rm, dim3(blocks), dim3(tile), 0, s, pos, ctx->gbuf, ctx->fscratch,
-             ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->rm_eo, ctx->ebuf_s, ctx->edbuf_s, h_final, n, ctx->rm_to,
-             out_off, out_len, out_index, (unsigned long long)ctx->fx_index, status, F, ctx->part);
+    NB_TIMED(ctx, s, "k_pack_posh", k_pack_posh, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, h_final, n,
+             (double4*)ctx->posh);
+    NB_TIMED(ctx, s, "k_tight_emit_rm", k_tight_emit_rm, dim3(blocks), dim3(tile), 0, s, (const double4*)ctx->posh,
+             ctx->gbuf, ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->rm_eo, ctx->ebuf_s, n, ctx->rm_to, out_off, out_len,
+             out_index, (unsigned long long)ctx->fx_index, status, F, ctx->part);
     if (hipGetLastError() != hipSuccess) return ctx_fail(ctx, "nb_hydro_tight_mixed_fixed: kernel launch failed");
     return NB_OK;
 }

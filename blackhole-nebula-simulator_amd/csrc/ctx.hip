@@ -130,7 +130,7 @@ extern "C" void nb_ctx_destroy(nb_ctx* ctx)
     void* bufs[] = {ctx->cell_id, ctx->keys_out, ctx->iota, ctx->order, ctx->cell_count, ctx->cell_start,
                     ctx->gbuf, ctx->rowcnt, ctx->unsorted, ctx->pending, ctx->edbuf, ctx->ecount, ctx->eoff, ctx->efill, ctx->ebuf, ctx->tot,
                     ctx->offs, ctx->tcount, ctx->dscratch, ctx->fscratch, ctx->pack, ctx->part, ctx->posh, ctx->mlp_scratch, ctx->mlp_pack, ctx->mlp_part, ctx->segs, ctx->red, ctx->scal, ctx->cub_tmp,
-                    ctx->rm_gc, ctx->rm_go, ctx->rm_ec, ctx->rm_eo, ctx->rm_to, ctx->ebuf_s, ctx->edbuf_s, ctx->fx_tmp,
+                    ctx->rm_gc, ctx->rm_go, ctx->rm_ec, ctx->rm_eo, ctx->rm_to, ctx->ebuf_s, ctx->fx_tmp,
                     ctx->fb_sz, ctx->fb_go, ctx->fb_to, ctx->segpart, ctx->xlist};
     for (void* b : bufs)
         if (b) hipFree(b);
@@ -149,7 +149,7 @@ extern "C" int64_t nb_ctx_scratch_bytes(const nb_ctx* ctx)
                ctx->ecount_bytes + ctx->eoff_bytes + ctx->efill_bytes + ctx->ebuf_bytes + ctx->tot_bytes +
                ctx->offs_bytes + ctx->tcount_bytes + ctx->dscratch_bytes + ctx->fscratch_bytes + ctx->pack_bytes + ctx->part_bytes + ctx->posh_bytes + ctx->mlp_scratch_bytes + ctx->mlp_pack_bytes + ctx->mlp_part_bytes + ctx->segs_bytes +
                ctx->red_bytes + ctx->scal_bytes + ctx->cub_tmp_bytes + ctx->rm_gc_bytes + ctx->rm_go_bytes +
-               ctx->rm_ec_bytes + ctx->rm_eo_bytes + ctx->rm_to_bytes + ctx->ebuf_s_bytes + ctx->edbuf_s_bytes +
+               ctx->rm_ec_bytes + ctx->rm_eo_bytes + ctx->rm_to_bytes + ctx->ebuf_s_bytes +
                ctx->fx_tmp_bytes + ctx->fb_sz_bytes + ctx->fb_go_bytes + ctx->fb_to_bytes +
                ctx->segpart_bytes + ctx->xlist_bytes;
     return (int64_t)s;

@@ -1228,7 +1228,7 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const Gr
                const int32_t* __restrict__ cloud, const uint32_t* __restrict__ start,
                const uint32_t* __restrict__ order, uint32_t* __restrict__ ridx, float* __restrict__ rmass,
                const unsigned long long* __restrict__ goff, int32_t* __restrict__ rowcnt,
-               unsigned int* __restrict__ efill, uint32_t* __restrict__ ebuf, float* __restrict__ ed2,
+               unsigned int* __restrict__ efill, uint32_t* __restrict__ ebuf,
                const unsigned long long* __restrict__ eoff, MixArgs A)
 {
     const int glane = threadIdx.x & (kG - 1);
@@ -1337,12 +1337,8 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const Gr
                 if (flag) {
                     const unsigned int es = atomicAdd(&efill[j], 1u);
                     const unsigned long long eb = eoff[j];
-                    if (es < eoff[j + 1] - eb) {
-                        ebuf[eb + es] = (uint32_t)i;
-                        ed2[eb + es] = d2f;
-                    } else {
-                        eover = 1;
-                    }
+                    if (es < eoff[j + 1] - eb) ebuf[eb + es] = (uint32_t)i;
+                    else eover = 1;
                     emax = es + 1 > emax ? es + 1 : emax;
                 }
                 if (slot < cap) row[slot] = j;
@@ -1502,7 +1498,7 @@ k_row_total_rm(const int32_t* __restrict__ rowcnt, const unsigned int* __restric
 }
 
 // Extras arrive in atomic order: each row's segment is put in index order (so that the tightened
-// rows, and every sum over them, are the same from run to run), out of place into (es, eds). A
+// rows, and every sum over them, are the same from run to run), out of place into es. A
 // 16-lane group per row ranks up to 64 entries held in registers against each other (every entry
 // goes to its rank, the number of smaller entries; entries of a segment are distinct). Longer
 // segments, few and clustered where large search spheres meet a dense region, are sorted by the
@@ -1522,9 +1518,9 @@ __device__ __forceinline__ int64_t extras_of(const unsigned int* efill, const un
 // Short segments; longer ones go to the work lists of k_sort_extras_mid (rows up to kSortWave
 // entries, list[0 ..], count cnt[0]) and k_sort_extras_long (list[n ..], count cnt[1]).
 __global__ void __launch_bounds__(256)
-k_sort_extras(const uint32_t* __restrict__ ebuf, const float* __restrict__ ed2, const unsigned int* __restrict__ efill,
+k_sort_extras(const uint32_t* __restrict__ ebuf, const unsigned int* __restrict__ efill,
               const unsigned long long* __restrict__ eoff, int64_t n, uint32_t* __restrict__ es,
-              float* __restrict__ eds, uint32_t* __restrict__ list, unsigned int* __restrict__ cnt)
+              uint32_t* __restrict__ list, unsigned int* __restrict__ cnt)
 {
     const int glane = threadIdx.x & (kG - 1);
     const int64_t i = (xcd_tile() * (int64_t)blockDim.x + threadIdx.x) / kG;
@@ -1540,14 +1536,11 @@ k_sort_extras(const uint32_t* __restrict__ ebuf, const float* __restrict__ ed2, 
         return;
     }
     const uint32_t* e = ebuf + b0;
-    const float* d = ed2 + b0;
     uint32_t v[kSortRegs];
-    float dv[kSortRegs];
 #pragma unroll
     for (int t = 0; t < kSortRegs; t++) {
         const int64_t k = glane + kG * t;
         v[t] = k < ne ? e[k] : 0xffffffffu;
-        dv[t] = k < ne ? d[k] : 0.f;
     }
     int rank[kSortRegs] = {0, 0, 0, 0};
 #pragma unroll
@@ -1560,25 +1553,19 @@ k_sort_extras(const uint32_t* __restrict__ ebuf, const float* __restrict__ ed2, 
 #pragma unroll
     for (int t = 0; t < kSortRegs; t++) {
         const int64_t k = glane + kG * t;
-        if (k < ne) {
-            es[b0 + rank[t]] = v[t];
-            eds[b0 + rank[t]] = dv[t];
-        }
+        if (k < ne) es[b0 + rank[t]] = v[t];
     }
 }
 
-// Bitonic sort of (index, position) pairs in LDS (p2 a power of two >= m, pads last), then the
+// Bitonic sort of a segment's indices in LDS (p2 a power of two >= m, pads last), then the
 // segment written in index order. sync(): the barrier of the threads taking part (tid of nt).
 template <class Sync>
-__device__ __forceinline__ void bitonic_segment(const uint32_t* e, const float* d, int m, uint32_t* sk, uint32_t* sp,
-                                                int tid, int nt, uint32_t* eo, float* dout, Sync sync)
+__device__ __forceinline__ void bitonic_segment(const uint32_t* e, int m, uint32_t* sk, int tid, int nt, uint32_t* eo,
+                                                Sync sync)
 {
     int p2 = 1;
     while (p2 < m) p2 <<= 1;
-    for (int t = tid; t < p2; t += nt) {
-        sk[t] = t < m ? e[t] : 0xffffffffu;
-        sp[t] = (uint32_t)t;
-    }
+    for (int t = tid; t < p2; t += nt) sk[t] = t < m ? e[t] : 0xffffffffu;
     sync();
     for (int k = 2; k <= p2; k <<= 1)
         for (int j = k >> 1; j > 0; j >>= 1) {
@@ -1590,18 +1577,12 @@ __device__ __forceinline__ void bitonic_segment(const uint32_t* e, const float* 
                     if ((a > b) == up) {
                         sk[t] = b;
                         sk[u] = a;
-                        const uint32_t pa = sp[t];
-                        sp[t] = sp[u];
-                        sp[u] = pa;
                     }
                 }
             }
             sync();
         }
-    for (int t = tid; t < m; t += nt) {
-        eo[t] = sk[t];
-        dout[t] = d[sp[t]];
-    }
+    for (int t = tid; t < m; t += nt) eo[t] = sk[t];
     sync();  // the LDS slice is free again
 }
 
@@ -1609,13 +1590,11 @@ __device__ __forceinline__ void bitonic_segment(const uint32_t* e, const float* 
 // search lengths; few, and clustered where large search spheres meet a dense region): one
 // wavefront per row, the rows of the list spread over the whole launch.
 __global__ void __launch_bounds__(256)
-k_sort_extras_mid(const uint32_t* __restrict__ ebuf, const float* __restrict__ ed2,
-                  const unsigned int* __restrict__ efill, const unsigned long long* __restrict__ eoff,
-                  uint32_t* __restrict__ es, float* __restrict__ eds, const uint32_t* __restrict__ list,
-                  const unsigned int* __restrict__ cnt)
+k_sort_extras_mid(const uint32_t* __restrict__ ebuf, const unsigned int* __restrict__ efill,
+                  const unsigned long long* __restrict__ eoff, uint32_t* __restrict__ es,
+                  const uint32_t* __restrict__ list, const unsigned int* __restrict__ cnt)
 {
     __shared__ uint32_t sk[4][kSortWave];
-    __shared__ uint32_t sp[4][kSortWave];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const unsigned int count = cnt[0];
     const unsigned int stride = gridDim.x * (blockDim.x >> 6);
@@ -1623,20 +1602,18 @@ k_sort_extras_mid(const uint32_t* __restrict__ ebuf, const float* __restrict__ e
         const int64_t row = list[r];
         unsigned long long c0;
         const int m = (int)extras_of(efill, eoff, row, c0);
-        bitonic_segment(ebuf + c0, ed2 + c0, m, sk[wave], sp[wave], lane, 64, es + c0, eds + c0, [] { wave_sync(); });
+        bitonic_segment(ebuf + c0, m, sk[wave], lane, 64, es + c0, [] { wave_sync(); });
     }
 }
 
 // Segments longer than kSortWave: one workgroup per row (bitonic in LDS up to kSortLds entries,
 // else ranks counted against the segment read through LDS in tiles).
 __global__ void __launch_bounds__(256)
-k_sort_extras_long(const uint32_t* __restrict__ ebuf, const float* __restrict__ ed2,
-                   const unsigned int* __restrict__ efill, const unsigned long long* __restrict__ eoff, int64_t n,
-                   uint32_t* __restrict__ es, float* __restrict__ eds, const uint32_t* __restrict__ list,
-                   const unsigned int* __restrict__ cnt)
+k_sort_extras_long(const uint32_t* __restrict__ ebuf, const unsigned int* __restrict__ efill,
+                   const unsigned long long* __restrict__ eoff, int64_t n, uint32_t* __restrict__ es,
+                   const uint32_t* __restrict__ list, const unsigned int* __restrict__ cnt)
 {
     __shared__ uint32_t sk[kSortLds];
-    __shared__ uint32_t sp[kSortLds];
     const int tid = threadIdx.x, bs = blockDim.x;
     const unsigned int count = cnt[1];
     for (unsigned int r = blockIdx.x; r < count; r += gridDim.x) {
@@ -1644,9 +1621,8 @@ k_sort_extras_long(const uint32_t* __restrict__ ebuf, const float* __restrict__ 
         unsigned long long c0;
         const int64_t m = extras_of(efill, eoff, row, c0);
         const uint32_t* e = ebuf + c0;
-        const float* d = ed2 + c0;
         if (m <= kSortLds) {
-            bitonic_segment(e, d, (int)m, sk, sp, tid, bs, es + c0, eds + c0, [] { __syncthreads(); });
+            bitonic_segment(e, (int)m, sk, tid, bs, es + c0, [] { __syncthreads(); });
             continue;
         }
         for (int64_t kb = 0; kb < m; kb += bs) {
@@ -1661,10 +1637,7 @@ k_sort_extras_long(const uint32_t* __restrict__ ebuf, const float* __restrict__ 
                 if (k < m)
                     for (int q = 0; q < tl; q++) rank += sk[q] < v ? 1 : 0;
             }
-            if (k < m) {
-                es[c0 + rank] = v;
-                eds[c0 + rank] = d[k];
-            }
+            if (k < m) es[c0 + rank] = v;
         }
         __syncthreads();
     }
@@ -1678,14 +1651,12 @@ static void sort_extras(nb_ctx* ctx, hipStream_t s, int64_t n, unsigned blocks, 
     uint32_t* list = ctx->xlist;
     unsigned int* cnt = (unsigned int*)(ctx->xlist + 2 * n);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, (uint32_t*)cnt, 0u, (int64_t)2);
-    NB_TIMED(ctx, s, "k_sort_extras", k_sort_extras, dim3(blocks), dim3(tile), 0, s, ctx->ebuf,
-             (const float*)ctx->edbuf, ctx->efill, ctx->rm_eo, n, ctx->ebuf_s, ctx->edbuf_s, list, cnt);
-    NB_TIMED(ctx, s, "k_sort_extras_mid", k_sort_extras_mid, dim3(1024), dim3(256), 0, s, ctx->ebuf,
-             (const float*)ctx->edbuf, ctx->efill, ctx->rm_eo, ctx->ebuf_s, ctx->edbuf_s, (const uint32_t*)list,
-             (const unsigned int*)cnt);
-    NB_TIMED(ctx, s, "k_sort_extras_long", k_sort_extras_long, dim3(256), dim3(256), 0, s, ctx->ebuf,
-             (const float*)ctx->edbuf, ctx->efill, ctx->rm_eo, n, ctx->ebuf_s, ctx->edbuf_s, (const uint32_t*)list,
-             (const unsigned int*)cnt);
+    NB_TIMED(ctx, s, "k_sort_extras", k_sort_extras, dim3(blocks), dim3(tile), 0, s, ctx->ebuf, ctx->efill,
+             ctx->rm_eo, n, ctx->ebuf_s, list, cnt);
+    NB_TIMED(ctx, s, "k_sort_extras_mid", k_sort_extras_mid, dim3(1024), dim3(256), 0, s, ctx->ebuf, ctx->efill,
+             ctx->rm_eo, ctx->ebuf_s, (const uint32_t*)list, (const unsigned int*)cnt);
+    NB_TIMED(ctx, s, "k_sort_extras_long", k_sort_extras_long, dim3(256), dim3(256), 0, s, ctx->ebuf, ctx->efill,
+             ctx->rm_eo, n, ctx->ebuf_s, (const uint32_t*)list, (const unsigned int*)cnt);
 }
 
 // what the emission pass needs to finish the rows converged in one Newton step
@@ -2811,15 +2782,14 @@ extern "C" int32_t nb_hydro_density_mixed(nb_ctx* ctx, void* stream, const doubl
         const size_t cells = (size_t)tot[0], ecells = (size_t)tot[1];
         if (ctx_reserve(ctx, (void**)&ctx->gbuf, &ctx->gbuf_bytes, sizeof(uint32_t) * cells) ||
             ctx_reserve(ctx, (void**)&ctx->fscratch, &ctx->fscratch_bytes, sizeof(float) * (masses ? (size_t)n : 1)) ||
-            ctx_reserve(ctx, (void**)&ctx->ebuf, &ctx->ebuf_bytes, sizeof(uint32_t) * ecells) ||
-            ctx_reserve(ctx, (void**)&ctx->edbuf, &ctx->edbuf_bytes, sizeof(float) * ecells))
+            ctx_reserve(ctx, (void**)&ctx->ebuf, &ctx->ebuf_bytes, sizeof(uint32_t) * ecells))
             return NB_ERR_HIP;
         if (hipMemsetAsync(ctx->efill, 0, sizeof(unsigned int) * n, s) != hipSuccess) return ctx_fail(ctx, "memset");
         float* rmass = masses ? ctx->fscratch : nullptr;
 #define NB_GATHER_MIXED(M, S)                                                                                        \
     NB_TIMED(ctx, s, "k_gather_mixed", (k_gather_mixed<M, S>), dim3(blocks), dim3(tile), 0, s,                      \
              (const double4*)ctx->posh, n, g, segs, cloud, ctx->cell_start, order_arg(ctx), ctx->gbuf, rmass,       \
-             ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->ebuf, (float*)ctx->edbuf, ctx->rm_eo, A)
+             ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->ebuf, ctx->rm_eo, A)
         if (masses && segmented) NB_GATHER_MIXED(true, true);
         else if (masses) NB_GATHER_MIXED(true, false);
         else if (segmented) NB_GATHER_MIXED(false, true);
@@ -2849,8 +2819,7 @@ extern "C" int32_t nb_hydro_density_mixed(nb_ctx* ctx, void* stream, const doubl
             NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<false>, dim3(blocks), dim3(tile), 0, s, h_search,
                      n, (const double4*)ctx->posh, ctx->gbuf, rmass, ctx->rm_go, ctx->rowcnt, A);
     }
-    if (ctx_reserve(ctx, (void**)&ctx->ebuf_s, &ctx->ebuf_s_bytes, sizeof(uint32_t) * (size_t)tot[1]) ||
-        ctx_reserve(ctx, (void**)&ctx->edbuf_s, &ctx->edbuf_s_bytes, sizeof(float) * (size_t)tot[1]))
+    if (ctx_reserve(ctx, (void**)&ctx->ebuf_s, &ctx->ebuf_s_bytes, sizeof(uint32_t) * (size_t)tot[1]))
         return NB_ERR_HIP;
     sort_extras(ctx, s, n, (unsigned)blocks, tile);
     // merged row lengths (reusing the gather capacities' array), their scan: the tight rows' offsets
@@ -3089,9 +3058,7 @@ extern "C" int32_t nb_fixed_reserve(nb_ctx* ctx, int64_t n, int32_t nseg, int64_
         ctx_reserve(ctx, (void**)&ctx->fscratch, &ctx->fscratch_bytes,
                     sizeof(float) * (masses ? (size_t)n : 1)) ||
         ctx_reserve(ctx, (void**)&ctx->ebuf, &ctx->ebuf_bytes, sizeof(uint32_t) * (size_t)extras_cap) ||
-        ctx_reserve(ctx, (void**)&ctx->edbuf, &ctx->edbuf_bytes, sizeof(float) * (size_t)extras_cap) ||
         ctx_reserve(ctx, (void**)&ctx->ebuf_s, &ctx->ebuf_s_bytes, sizeof(uint32_t) * (size_t)extras_cap) ||
-        ctx_reserve(ctx, (void**)&ctx->edbuf_s, &ctx->edbuf_s_bytes, sizeof(float) * (size_t)extras_cap) ||
         ctx_reserve(ctx, (void**)&ctx->pack, &ctx->pack_bytes, 48 * (size_t)n))
         return NB_ERR_HIP;
     int bits = 1;
@@ -3177,11 +3144,11 @@ extern "C" int32_t nb_hydro_density_mixed_fixed(nb_ctx* ctx, void* stream, const
     if (masses)
         NB_TIMED(ctx, s, "k_gather_mixed", (k_gather_mixed<true, true>), dim3(blocks), dim3(tile), 0, s,
                  (const double4*)ctx->posh, n, g_none, segs, cloud, ctx->cell_start, (const uint32_t*)nullptr, ctx->gbuf,
-                 rmass, ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->ebuf, (float*)ctx->edbuf, ctx->rm_eo, A);
+                 rmass, ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->ebuf, ctx->rm_eo, A);
     else
         NB_TIMED(ctx, s, "k_gather_mixed", (k_gather_mixed<false, true>), dim3(blocks), dim3(tile), 0, s,
                  (const double4*)ctx->posh, n, g_none, segs, cloud, ctx->cell_start, (const uint32_t*)nullptr, ctx->gbuf,
-                 rmass, ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->ebuf, (float*)ctx->edbuf, ctx->rm_eo, A);
+                 rmass, ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->ebuf, ctx->rm_eo, A);
     // [2] extras overflow, [3] rows over capacity
     finish_partials(ctx, s, blocks, 5, 1u | 4u | 16u, 0);
     hipLaunchKernelGGL(k_status_from, dim3(1), dim3(64), 0, s, (const long long*)ctx->scal, 2, status, (int)kStExtras);

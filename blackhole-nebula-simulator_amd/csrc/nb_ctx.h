@@ -71,8 +71,7 @@ struct nb_ctx {
     int64_t fb_gtotal = 0, fb_ttotal = 0;
     uint32_t* ebuf_s = nullptr; size_t ebuf_s_bytes = 0;  // extras in index order
     uint32_t* xlist = nullptr; size_t xlist_bytes = 0;    // work lists of the longer extras segments
-    float* edbuf_s = nullptr; size_t edbuf_s_bytes = 0;
-    int64_t rm_gtotal = 0;  // gather-row entries of the last mixed pass (start of the mass rows)
+    int64_t rm_gtotal = 0;  // gather-row entries of the last mixed pass
     int64_t rm_etotal = 0;  // extras entries of the last mixed pass
     // fixed capacities of the passes that a captured step records (nb_fixed_reserve): no host
     // synchronisation or allocation inside them; what does not fit is reported in a status word

@@ -1701,14 +1701,9 @@ k_tight_emit_rm(const double4* __restrict__ posh, const uint32_t* __restrict__ r
     const float mu = (float)(F.m_uniform > 0.0 ? F.m_uniform : 1.0);
     DensSums sums;
     int32_t o = 0;
-    // entry k of the merged row: its index j and (for the density sums) mass
-    auto load = [&](int64_t k, uint32_t& j, float& mk) {
-        if (k < ng) {
-            j = ridx[g0 + k];
-            mk = F.rmass ? F.rmass[j] : mu;
-        } else {
-            j = ebuf[e0 + (k - ng)];
-        }
+    // index of entry k (< len) of the merged row: gather row entries, then the sorted extras
+    auto index_at = [&](int64_t k) -> uint32_t {
+        return k < ng ? ridx[g0 + k] : ebuf[e0 + (k - ng)];
     };
     auto dist2 = [&](const double4& pj) -> double {
         const double dx = pj.x - pi.x;
@@ -1731,23 +1726,26 @@ k_tight_emit_rm(const double4* __restrict__ posh, const uint32_t* __restrict__ r
         }
         o += __popc(gb);
     };
-    // two batches of kG entries per pass, their loads issued together (the neighbour's h depends
-    // on the index load): the same entries, sums and output order as one batch at a time
+    // Two batches of kG entries per pass. The indices of the next pass are loaded while this
+    // pass's records arrive (the records depend on the indices): the same entries, sums and output
+    // order as one batch at a time. Lanes past the row's end read the row's own record.
+    const uint32_t self = (uint32_t)ii;
+    uint32_t na = glane < len ? index_at(glane) : self;
+    uint32_t nc = kG + glane < len ? index_at(kG + glane) : self;
     for (int64_t kb = 0;; kb += 2 * kG) {
         if (__ballot(kb < len) == 0) break;
         const int64_t ka = kb + glane, kc = kb + kG + glane;
         const bool va = ka < len, vc = kc < len;
-        uint32_t ja = 0, jc = 0;
-        float ma = mu, mc = mu;
-        if (va) load(ka, ja, ma);
-        if (vc) load(kc, jc, mc);
-        const double4 qa = posh[va ? ja : (uint32_t)ii], qc = posh[vc ? jc : (uint32_t)ii];
-        const double ha = qa.w, hc = qc.w;
-        const double da = va ? dist2(qa) : 0.0, dc = vc ? dist2(qc) : 0.0;
+        const uint32_t ja = na, jc = nc;
+        const double4 qa = posh[ja], qc = posh[jc];
+        const float ma = F.rmass ? F.rmass[ja] : mu, mc = F.rmass ? F.rmass[jc] : mu;
+        na = ka + 2 * kG < len ? index_at(ka + 2 * kG) : self;
+        nc = kc + 2 * kG < len ? index_at(kc + 2 * kG) : self;
+        const double da = dist2(qa), dc = dist2(qc);
         if (fin && va && ka < ng) sums.add(sqrt32((float)da) * inv_hf, ma, F.kid);
         if (fin && vc && kc < ng) sums.add(sqrt32((float)dc) * inv_hf, mc, F.kid);
-        const bool pa = va && keep(da, ha);
-        const bool pc = vc && keep(dc, hc);
+        const bool pa = va && keep(da, qa.w);
+        const bool pc = vc && keep(dc, qc.w);
         emit(pa, ja);
         emit(pc, jc);
     }

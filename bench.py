@@ -610,17 +610,19 @@ def main():
     fp64 = args.precision == "fp64"
     # kernel -> (bound, work per launch, unit)
     models = {
-        # neighbour gather (+ the first Newton pass): write each search entry (index, d2, mass; FP64 d2
-        # in the bit-exact layout), read each particle's candidate record once
-        "k_gather_mixed": ("hbm", (4 + 4 + 4) * n_search * n + 40 * n, "GB/s"),
+        # neighbour gather (+ the first Newton pass): write each search entry's index (the list
+        # write of the neighbour build), read each particle's (x, y, z, h) record and mass once and
+        # write its FP32 mass; the bit-exact layout also writes each entry's FP64 d2 and mass
+        "k_gather_mixed": ("hbm", 4 * n_search * n + (32 + 8 + 4) * n, "GB/s"),
         "k_gather_solve": ("hbm", (4 + 8 + 8) * n_search * n + 32 * n, "GB/s"),
         "k_forces64": ("mfma", FLOPS_FORCE_PAIR * n_pairs * n, "TFLOP/s"),
         "k_forces_mixed": ("mfma", FLOPS_FORCE_PAIR * n_pairs * n, "TFLOP/s"),
         "k_gradients64": ("mfma", FLOPS_GRAD_PAIR * n_pairs * n, "TFLOP/s"),
         "k_gradients_mixed": ("mfma", FLOPS_GRAD_PAIR * n_pairs * n, "TFLOP/s"),
-        # tightening: read each search entry (index, d2) and the neighbour's h, write the kept index
+        # tightening: read each search entry (index; the bit-exact layout also FP64 d2 and the
+        # neighbour's h), each particle's (x, y, z, h) record and FP32 mass once, write the kept index
         "k_tight_emit": ("hbm", (4 + 8 + 8) * n_search * n + 4 * (n_pairs + 1) * n, "GB/s"),
-        "k_tight_emit_rm": ("hbm", (4 + 4 + 8) * n_search * n + 4 * (n_pairs + 1) * n, "GB/s"),
+        "k_tight_emit_rm": ("hbm", 4 * n_search * n + (32 + 4) * n + 4 * (n_pairs + 1) * n, "GB/s"),
     }
     timed = {k: v for k, v in kern.items() if k in models}
     dom = max(timed, key=lambda k: timed[k][1])

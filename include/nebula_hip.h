@@ -172,8 +172,8 @@ int32_t nb_hydro_tight(nb_ctx *ctx, void *stream, const double *h_final, int64_t
 /* ------------------------------------------------------------------ mixed-precision step passes */
 /* The density pass of coupled.step for the mixed-precision mode (open box): the same search
  * lists as nb_hydro_density (FP64 predicate, exact), gathered by 16-lane groups into rows of their
- * own capacity with FP32 d2; the h-rho solve sums in FP32 with compensated accumulation and takes
- * its Newton steps in FP64. cloud / seg_off / nseg: an ensemble of independent systems as in
+ * own capacity holding neighbour indices only; the h-rho solve sums in FP32 with compensated
+ * accumulation (d2 formed in FP64 and rounded) and takes its Newton steps in FP64. cloud / seg_off / nseg: an ensemble of independent systems as in
  * nb_grid_order_seg (cloud NULL: one system); cell_factor as in nb_grid_order.
  * Row capacities come from row_hint (device, 2 N: gather-row and extras counts of each particle
  * from a previous pass, e.g. row_counts of the last step carried through the particles'
@@ -194,8 +194,8 @@ int32_t nb_hydro_density_mixed(nb_ctx *ctx, void *stream, const double *pos, con
  * h_out), into rows of their own capacity: row i at out_off[i] (N + 1 entries written; row i's
  * capacity is its merged search row) with out_len[i] entries; out_index holds index_cap >= the
  * search-list entries (stats4[2]). pos as in the density pass. Rows are exactly tighten() of the
- * search lists (borderline FP32 d2 re-formed in FP64). Completes rho and omega (the same arrays
- * given to the density pass). */
+ * search lists (d2 formed in FP64 from pos as the search formed it). Completes rho and omega (the
+ * same arrays given to the density pass). */
 int32_t nb_hydro_tight_mixed(nb_ctx *ctx, void *stream, const double *pos, const double *h_final, double *rho,
                              double *omega, int64_t *out_off, int32_t *out_len, int32_t *out_index,
                              int64_t index_cap, int64_t *total_out);

@@ -1515,7 +1515,37 @@ __device__ __forceinline__ int64_t extras_of(const unsigned int* efill, const un
     return (int64_t)min((unsigned long long)efill[i], eoff[i + 1] - c0);
 }
 
-// Short segments; longer ones go to the work lists of k_sort_extras_mid (rows up to kSortWave
+// Ranks of one group's segment of at most kG * R entries, R per lane in registers: every entry
+// goes to its rank, the number of smaller entries (entries of a segment are distinct; pads
+// compare above all of them).
+template <int R>
+__device__ __forceinline__ void rank_segment(const uint32_t* __restrict__ e, int64_t ne, int glane,
+                                             uint32_t* __restrict__ out)
+{
+    uint32_t v[R];
+    int rank[R];
+#pragma unroll
+    for (int t = 0; t < R; t++) {
+        const int64_t k = glane + kG * t;
+        v[t] = k < ne ? e[k] : 0xffffffffu;
+        rank[t] = 0;
+    }
+#pragma unroll
+    for (int t2 = 0; t2 < R; t2++)
+        for (int l = 0; l < kG; l++) {
+            const uint32_t w = __shfl(v[t2], l, kG);
+#pragma unroll
+            for (int t = 0; t < R; t++) rank[t] += w < v[t] ? 1 : 0;
+        }
+#pragma unroll
+    for (int t = 0; t < R; t++) {
+        const int64_t k = glane + kG * t;
+        if (k < ne) out[rank[t]] = v[t];
+    }
+}
+
+// Short segments, ranked with as few registers per lane as they need (most hold at most kG
+// entries); longer ones go to the work lists of k_sort_extras_mid (rows up to kSortWave
 // entries, list[0 ..], count cnt[0]) and k_sort_extras_long (list[n ..], count cnt[1]).
 __global__ void __launch_bounds__(256)
 k_sort_extras(const uint32_t* __restrict__ ebuf, const unsigned int* __restrict__ efill,
@@ -1535,26 +1565,9 @@ k_sort_extras(const uint32_t* __restrict__ ebuf, const unsigned int* __restrict_
         }
         return;
     }
-    const uint32_t* e = ebuf + b0;
-    uint32_t v[kSortRegs];
-#pragma unroll
-    for (int t = 0; t < kSortRegs; t++) {
-        const int64_t k = glane + kG * t;
-        v[t] = k < ne ? e[k] : 0xffffffffu;
-    }
-    int rank[kSortRegs] = {0, 0, 0, 0};
-#pragma unroll
-    for (int t2 = 0; t2 < kSortRegs; t2++)
-        for (int l = 0; l < kG; l++) {
-            const uint32_t w = __shfl(v[t2], l, kG);
-#pragma unroll
-            for (int t = 0; t < kSortRegs; t++) rank[t] += w < v[t] ? 1 : 0;
-        }
-#pragma unroll
-    for (int t = 0; t < kSortRegs; t++) {
-        const int64_t k = glane + kG * t;
-        if (k < ne) es[b0 + rank[t]] = v[t];
-    }
+    if (ne <= kG) rank_segment<1>(ebuf + b0, ne, glane, es + b0);
+    else if (ne <= 2 * kG) rank_segment<2>(ebuf + b0, ne, glane, es + b0);
+    else rank_segment<kSortRegs>(ebuf + b0, ne, glane, es + b0);
 }
 
 // Bitonic sort of a segment's indices in LDS (p2 a power of two >= m, pads last), then the
@@ -1726,12 +1739,13 @@ k_tight_emit_rm(const double4* __restrict__ posh, const uint32_t* __restrict__ r
         }
         o += __popc(gb);
     };
-    // Two batches of kG entries per pass. The indices of the next pass are loaded while this
+    // Two batches of kG entries per pass. The indices of the pass after next are loaded while this
     // pass's records arrive (the records depend on the indices): the same entries, sums and output
     // order as one batch at a time. Lanes past the row's end read the row's own record.
     const uint32_t self = (uint32_t)ii;
-    uint32_t na = glane < len ? index_at(glane) : self;
-    uint32_t nc = kG + glane < len ? index_at(kG + glane) : self;
+    auto index_or_self = [&](int64_t k) -> uint32_t { return k < len ? index_at(k) : self; };
+    uint32_t na = index_or_self(glane), nc = index_or_self(kG + glane);
+    uint32_t na2 = index_or_self(2 * kG + glane), nc2 = index_or_self(3 * kG + glane);
     for (int64_t kb = 0;; kb += 2 * kG) {
         if (__ballot(kb < len) == 0) break;
         const int64_t ka = kb + glane, kc = kb + kG + glane;
@@ -1739,8 +1753,10 @@ k_tight_emit_rm(const double4* __restrict__ posh, const uint32_t* __restrict__ r
         const uint32_t ja = na, jc = nc;
         const double4 qa = posh[ja], qc = posh[jc];
         const float ma = F.rmass ? F.rmass[ja] : mu, mc = F.rmass ? F.rmass[jc] : mu;
-        na = ka + 2 * kG < len ? index_at(ka + 2 * kG) : self;
-        nc = kc + 2 * kG < len ? index_at(kc + 2 * kG) : self;
+        na = na2;
+        nc = nc2;
+        na2 = index_or_self(ka + 4 * kG);
+        nc2 = index_or_self(kc + 4 * kG);
         const double da = dist2(qa), dc = dist2(qc);
         if (fin && va && ka < ng) sums.add(sqrt32((float)da) * inv_hf, ma, F.kid);
         if (fin && vc && kc < ng) sums.add(sqrt32((float)dc) * inv_hf, mc, F.kid);

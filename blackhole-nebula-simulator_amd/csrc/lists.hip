@@ -1095,8 +1095,8 @@ __device__ __forceinline__ void kernel_fdf32(float q, int kid, float& fq, float&
     if (kid == 0) {
         const float t2 = fmaxf(2.f - q, 0.f), t1 = fmaxf(1.f - q, 0.f);
         const float s2 = t2 * t2, s1 = t1 * t1;
-        fq = 0.25f * s2 * t2 - s1 * t1;
-        dfq = 3.f * s1 - 0.75f * s2;
+        fq = __builtin_fmaf(0.25f * s2, t2, -(s1 * t1));
+        dfq = __builtin_fmaf(3.f, s1, -0.75f * s2);
     } else {
         const float x = fmaxf(1.f - 0.5f * q, 0.f), x2 = x * x;
         fq = x2 * x2 * x2 * (35.f / 12.f * q * q + 3.f * q + 1.f);
@@ -1117,11 +1117,11 @@ struct DensSums {
     {
         float fq, dfq;
         kernel_fdf32(q, kid, fq, dfq);  // zero terms past the support
-        const float t1 = mk * fq - c1;
+        const float t1 = __builtin_fmaf(mk, fq, -c1);  // the products fused with the compensation
         const float u1 = a1 + t1;
         c1 = (u1 - a1) - t1;
         a1 = u1;
-        const float t2 = mk * (3.f * fq + q * dfq) - c2;
+        const float t2 = __builtin_fmaf(mk, __builtin_fmaf(q, dfq, 3.f * fq), -c2);
         const float u2 = a2 + t2;
         c2 = (u2 - a2) - t2;
         a2 = u2;

@@ -84,6 +84,14 @@ struct Kahan {
         c = (u - s) - t;
         s = u;
     }
+    // adds the product a * b, formed in one fused multiply-add with the compensation
+    __device__ __forceinline__ void add_prod(float a, float b)
+    {
+        const float t = __builtin_fmaf(a, b, -c);
+        const float u = s + t;
+        c = (u - s) - t;
+        s = u;
+    }
     __device__ __forceinline__ double value() const { return (double)s - (double)c; }
 };
 
@@ -175,10 +183,10 @@ k_forces_mixed(const double* __restrict__ pos, const PF* __restrict__ pf, const 
             cond.add(B.c.x * alpha_u * vu * (A.c.y - B.c.y) * 0.5f * (fa * A.b.z + fb * B.b.z));
         }
         const float f = B.c.x * (na * fa + nbb * fb);
-        ax.add(-f * rx);
-        ay.add(-f * ry);
-        az.add(-f * rz);
-        work.add(B.c.x * vdotr * fa);
+        ax.add_prod(-f, rx);
+        ay.add_prod(-f, ry);
+        az.add_prod(-f, rz);
+        work.add_prod(B.c.x * vdotr, fa);
     };
     // two entries per pass, both neighbours' records loaded before either pair's arithmetic: the
     // same pairs in the same order per lane as one entry at a time (self skipped)
@@ -240,18 +248,18 @@ __device__ __forceinline__ void gradients_mixed_row(const double* __restrict__ p
             for (int c = 0; c < 3; c++) da[c] = (float)accel[3 * b + c] - aa[c];
         const float mg = B.b.y * g;
         const float gx[3] = {mg * dx[0], mg * dx[1], mg * dx[2]};
-        R[0].add(-(dx[0] * gx[0]));
-        R[1].add(-(dx[0] * gx[1]));
-        R[2].add(-(dx[0] * gx[2]));
-        R[3].add(-(dx[1] * gx[1]));
-        R[4].add(-(dx[1] * gx[2]));
-        R[5].add(-(dx[2] * gx[2]));
+        R[0].add_prod(-dx[0], gx[0]);
+        R[1].add_prod(-dx[0], gx[1]);
+        R[2].add_prod(-dx[0], gx[2]);
+        R[3].add_prod(-dx[1], gx[1]);
+        R[4].add_prod(-dx[1], gx[2]);
+        R[5].add_prod(-dx[2], gx[2]);
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
             for (int j = 0; j < 3; j++) {
-                Gv[3 * i + j].add(dv[i] * gx[j]);
-                if (kAcc) Ga[kAcc ? 3 * i + j : 0].add(da[i] * gx[j]);
+                Gv[3 * i + j].add_prod(dv[i], gx[j]);
+                if (kAcc) Ga[kAcc ? 3 * i + j : 0].add_prod(da[i], gx[j]);
             }
     };
     // two entries per pass, both neighbours' records loaded first (same pairs and order per lane)

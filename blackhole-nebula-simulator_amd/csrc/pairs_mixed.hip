@@ -16,10 +16,14 @@
 
 namespace nb {
 
-// Lanes per row of the gradient and force passes: 8 (rows of ~60 pairs give each lane 7-8; the
+// Lanes per row of the force and gradient passes: 8 (rows of ~60 pairs give each lane 7-8; the
 // per-row combination of the lane sums costs a quarter less than with 16 lanes, and the FP64
-// 3 x 3 solve of the gradients runs on 8 rows per wavefront instead of 4).
+// 3 x 3 solve of the gradients runs on 8 rows per wavefront instead of 4). The div/curl pass
+// takes 4: its per-row work (15 sums combined, the 3 x 3 solve on one lane per row) outweighs
+// its pair arithmetic, and 16 rows per wavefront share it (0.41 -> 0.36 ms at 1M, where the
+// force pass at 4 lanes went from 0.37 to 0.42 ms).
 static constexpr int kPairW = 8;
+static constexpr int kDivCurlW = 4;
 static constexpr int kBlockM = 256;
 
 struct __align__(16) PF {
@@ -352,10 +356,10 @@ static inline Box box_of(const double* box3)
         }                                                                                               \
     } while (0)
 
-#define NB_DISPATCH_M(L, KERNEL, NROWS, STREAM, ...)                                                    \
+#define NB_DISPATCH_M(L, KERNEL, W, NROWS, STREAM, ...)                                                 \
     do {                                                                                                \
         const int ts_ = ktime_begin(ctx, (STREAM), #KERNEL);                                            \
-        NB_DISPATCH_W(L, KERNEL, kPairW, NROWS, STREAM, __VA_ARGS__);                                   \
+        NB_DISPATCH_W(L, KERNEL, W, NROWS, STREAM, __VA_ARGS__);                                        \
         ktime_end(ctx, (STREAM), ts_);                                                                  \
     } while (0)
 
@@ -378,7 +382,7 @@ int32_t forces_mixed(nb_ctx* ctx, hipStream_t s, const double* pos, const double
     PF* pf = (PF*)ctx->pack;
     hipLaunchKernelGGL(k_pack_forces, dim3(grid_1(n)), dim3(kBlockM), 0, s, vel, mass, h, rho, omega, pressure,
                        P->use_cond ? u : nullptr, c_s, P->use_visc ? alpha : nullptr, P->cnorm, n, pf);
-    NB_DISPATCH_M(lists, k_forces_mixed, nrows, s, pos, pf, c_s, v_, box_of(box3), rows, nrows, *P, acc, dudt,
+    NB_DISPATCH_M(lists, k_forces_mixed, kPairW, nrows, s, pos, pf, c_s, v_, box_of(box3), rows, nrows, *P, acc, dudt,
                   vsig_max, vheat, cheat);
     return launched(ctx, "nb_forces (mixed)");
 }
@@ -402,10 +406,10 @@ extern "C" int32_t nb_gradients_mixed(nb_ctx* ctx, void* stream, const double* p
     PG* pg = (PG*)ctx->pack;
     hipLaunchKernelGGL(k_pack_grad, dim3(grid_1(n)), dim3(kBlockM), 0, s, vel, mass, h, cnorm, n, pg);
     if (acc)
-        NB_DISPATCH_M(lists, k_gradients_mixed, nrows, s, pos, acc, pg, v_, box_of(box3), rows, nrows, kid, div, curl,
-                      ddiv);
+        NB_DISPATCH_M(lists, k_gradients_mixed, kPairW, nrows, s, pos, acc, pg, v_, box_of(box3), rows, nrows, kid,
+                      div, curl, ddiv);
     else
-        NB_DISPATCH_M(lists, k_divcurl_mixed, nrows, s, pos, acc, pg, v_, box_of(box3), rows, nrows, kid, div, curl,
-                      ddiv);
+        NB_DISPATCH_M(lists, k_divcurl_mixed, kDivCurlW, nrows, s, pos, acc, pg, v_, box_of(box3), rows, nrows, kid,
+                      div, curl, ddiv);
     return launched(ctx, "nb_gradients_mixed");
 }

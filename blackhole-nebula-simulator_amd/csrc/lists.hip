@@ -1104,9 +1104,10 @@ __device__ __forceinline__ void kernel_fdf32(float q, int kid, float& fq, float&
     }
 }
 
+template <int W = kG>
 __device__ __forceinline__ double group_sum(double v)
 {
-    for (int o = kG / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kG);
+    for (int o = W / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, W);
     return v;
 }
 
@@ -1126,11 +1127,12 @@ struct DensSums {
         c2 = (u2 - a2) - t2;
         a2 = u2;
     }
-    // group totals (FP64), identical on every lane of the group
+    // group totals (FP64) over W lanes, identical on every lane of the group
+    template <int W = kG>
     __device__ __forceinline__ void total(double& s1, double& s2) const
     {
-        s1 = group_sum((double)a1 - (double)c1);
-        s2 = group_sum((double)a2 - (double)c2);
+        s1 = group_sum<W>((double)a1 - (double)c1);
+        s2 = group_sum<W>((double)a2 - (double)c2);
     }
 };
 
@@ -1689,6 +1691,12 @@ struct FinArgs {
 // search lists at h_final. The same pass sums rho and Omega at the final h of the rows the gather
 // finished in one Newton step, over FP32 d2 (extras never count: their d2 >= (2 h_search_i)^2 >=
 // (2 h_i)^2).
+// Lanes per row of the emission: 16. The pass waits on its loads (index, then the neighbour's
+// record), and more rows per wavefront with fewer lanes each issue fewer of them at once: at 1M,
+// 4 lanes took 1.17 ms, 8 took 0.80, 16 take 0.69 and 32 took 0.72.
+static constexpr int kEmitW = 16;
+
+template <int kE>
 __global__ void __launch_bounds__(256)
 k_tight_emit_rm(const double4* __restrict__ posh, const uint32_t* __restrict__ ridx,
                 const unsigned long long* __restrict__ goff, const int32_t* __restrict__ rowcnt,
@@ -1697,9 +1705,9 @@ k_tight_emit_rm(const double4* __restrict__ posh, const uint32_t* __restrict__ r
                 int32_t* __restrict__ out_len, int32_t* __restrict__ out_idx, unsigned long long idx_cap,
                 int32_t* __restrict__ status, FinArgs F, long long* __restrict__ part)
 {
-    const int glane = threadIdx.x & (kG - 1);
-    const int gshift = (threadIdx.x & 63) & ~(kG - 1);
-    const int64_t i = (xcd_tile() * (int64_t)blockDim.x + threadIdx.x) / kG;
+    const int glane = threadIdx.x & (kE - 1);
+    const int gshift = (threadIdx.x & 63) & ~(kE - 1);
+    const int64_t i = (xcd_tile() * (int64_t)blockDim.x + threadIdx.x) / kE;
     const bool live = i < n;
     const int64_t ii = live ? i : n - 1;
     const double4 pi = posh[ii];
@@ -1731,7 +1739,7 @@ k_tight_emit_rm(const double4* __restrict__ posh, const uint32_t* __restrict__ r
     };
     auto emit = [&](bool pred, uint32_t j) {
         const unsigned long long bal = __ballot(pred);
-        const unsigned gb = (unsigned)((bal >> gshift) & ((1ull << kG) - 1));
+        const unsigned gb = (unsigned)((bal >> gshift) & ((1ull << kE) - 1));
         if (pred) {
             const unsigned long long at = t0 + o + __popc(gb & ((1u << glane) - 1));
             if (at < idx_cap) out_idx[at] = (int32_t)j;
@@ -1739,24 +1747,24 @@ k_tight_emit_rm(const double4* __restrict__ posh, const uint32_t* __restrict__ r
         }
         o += __popc(gb);
     };
-    // Two batches of kG entries per pass. The indices of the pass after next are loaded while this
+    // Two batches of kE entries per pass. The indices of the pass after next are loaded while this
     // pass's records arrive (the records depend on the indices): the same entries, sums and output
     // order as one batch at a time. Lanes past the row's end read the row's own record.
     const uint32_t self = (uint32_t)ii;
     auto index_or_self = [&](int64_t k) -> uint32_t { return k < len ? index_at(k) : self; };
-    uint32_t na = index_or_self(glane), nc = index_or_self(kG + glane);
-    uint32_t na2 = index_or_self(2 * kG + glane), nc2 = index_or_self(3 * kG + glane);
-    for (int64_t kb = 0;; kb += 2 * kG) {
+    uint32_t na = index_or_self(glane), nc = index_or_self(kE + glane);
+    uint32_t na2 = index_or_self(2 * kE + glane), nc2 = index_or_self(3 * kE + glane);
+    for (int64_t kb = 0;; kb += 2 * kE) {
         if (__ballot(kb < len) == 0) break;
-        const int64_t ka = kb + glane, kc = kb + kG + glane;
+        const int64_t ka = kb + glane, kc = kb + kE + glane;
         const bool va = ka < len, vc = kc < len;
         const uint32_t ja = na, jc = nc;
         const double4 qa = posh[ja], qc = posh[jc];
         const float ma = F.rmass ? F.rmass[ja] : mu, mc = F.rmass ? F.rmass[jc] : mu;
         na = na2;
         nc = nc2;
-        na2 = index_or_self(ka + 4 * kG);
-        nc2 = index_or_self(kc + 4 * kG);
+        na2 = index_or_self(ka + 4 * kE);
+        nc2 = index_or_self(kc + 4 * kE);
         const double da = dist2(qa), dc = dist2(qc);
         if (fin && va && ka < ng) sums.add(sqrt32((float)da) * inv_hf, ma, F.kid);
         if (fin && vc && kc < ng) sums.add(sqrt32((float)dc) * inv_hf, mc, F.kid);
@@ -1766,7 +1774,7 @@ k_tight_emit_rm(const double4* __restrict__ posh, const uint32_t* __restrict__ r
         emit(pc, jc);
     }
     double s1, s2;
-    sums.total(s1, s2);
+    sums.total<kE>(s1, s2);
     if (live && glane == 0) {
         // a row cut by the index capacity (reported in status) keeps what was written
         out_len[i] = t0 + (unsigned long long)o <= idx_cap ? o : (t0 < idx_cap ? (int32_t)(idx_cap - t0) : 0);
@@ -2877,16 +2885,16 @@ extern "C" int32_t nb_hydro_tight_mixed(nb_ctx* ctx, void* stream, const double*
     const FinArgs F{masses ? ctx->fscratch : nullptr, ctx->mix_m_uniform, ctx->mix_kid,
                     ctx->mix_cnorm, ctx->pending, rho, omega};
     const int tile = ctx->tile;
-    const unsigned blocks = (unsigned)((n * kG + tile - 1) / tile);
-    if (ctx->part_bytes < sizeof(long long) * (size_t)blocks)
+    const unsigned eblocks = (unsigned)((n * kEmitW + tile - 1) / tile);
+    if (ctx->part_bytes < sizeof(long long) * (size_t)eblocks)
         return ctx_fail(ctx, "nb_hydro_tight_mixed: reduction buffer smaller than the launch");
     // (x, y, z, h_final) records: the emission reads one per row entry
     NB_TIMED(ctx, s, "k_pack_posh", k_pack_posh, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, h_final, n,
              (double4*)ctx->posh);
-    NB_TIMED(ctx, s, "k_tight_emit_rm", k_tight_emit_rm, dim3(blocks), dim3(tile), 0, s, (const double4*)ctx->posh,
-             ctx->gbuf, ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->rm_eo, ctx->ebuf_s, n, ctx->rm_to, out_off, out_len,
+    NB_TIMED(ctx, s, "k_tight_emit_rm", k_tight_emit_rm<kEmitW>, dim3(eblocks), dim3(tile), 0, s,
+             (const double4*)ctx->posh, ctx->gbuf, ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->rm_eo, ctx->ebuf_s, n, ctx->rm_to, out_off, out_len,
              out_index, (unsigned long long)index_cap, (int32_t*)nullptr, F, ctx->part);
-    finish_partials(ctx, s, blocks, 1, 0u, 8);
+    finish_partials(ctx, s, eblocks, 1, 0u, 8);
     long long t = 0;
     int rc = sync_copy(ctx, s, &t, (long long*)ctx->scal + 8, sizeof(t), "nb_hydro_tight_mixed: total copy failed");
     if (rc) return rc;
@@ -3207,11 +3215,11 @@ extern "C" int32_t nb_hydro_tight_mixed_fixed(nb_ctx* ctx, void* stream, const d
     const FinArgs F{masses ? ctx->fscratch : nullptr, ctx->mix_m_uniform, ctx->mix_kid,
                     ctx->mix_cnorm, ctx->pending, rho, omega};
     const int tile = ctx->tile;
-    const unsigned blocks = (unsigned)((n * kG + tile - 1) / tile);
+    const unsigned eblocks = (unsigned)((n * kEmitW + tile - 1) / tile);
     NB_TIMED(ctx, s, "k_pack_posh", k_pack_posh, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, h_final, n,
              (double4*)ctx->posh);
-    NB_TIMED(ctx, s, "k_tight_emit_rm", k_tight_emit_rm, dim3(blocks), dim3(tile), 0, s, (const double4*)ctx->posh,
-             ctx->gbuf, ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->rm_eo, ctx->ebuf_s, n, ctx->rm_to, out_off, out_len,
+    NB_TIMED(ctx, s, "k_tight_emit_rm", k_tight_emit_rm<kEmitW>, dim3(eblocks), dim3(tile), 0, s,
+             (const double4*)ctx->posh, ctx->gbuf, ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->rm_eo, ctx->ebuf_s, n, ctx->rm_to, out_off, out_len,
              out_index, (unsigned long long)ctx->fx_index, status, F, ctx->part);
     if (hipGetLastError() != hipSuccess) return ctx_fail(ctx, "nb_hydro_tight_mixed_fixed: kernel launch failed");
     return NB_OK;

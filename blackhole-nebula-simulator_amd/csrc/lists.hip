@@ -674,19 +674,22 @@ struct PermSet {
     int32_t total;  // sum of widths
 };
 
+// Work-item t writes element t of the concatenated destinations (array after array), so each
+// wavefront stores one contiguous run of one array; the reads follow the order, which is close to
+// the identity from one step to the next.
 __global__ void k_permute_many(PermSet P, const int32_t* __restrict__ order, int64_t n)
 {
-    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= n * P.total) return;
-    const int64_t i = t / P.total;
-    int c = (int)(t - i * P.total);
     int k = 0;
-    while (c >= P.width[k]) {
-        c -= P.width[k];
+    while (t >= n * P.width[k]) {
+        t -= n * P.width[k];
         k++;
     }
     const int w = P.width[k];
-    P.dst[k][i * w + c] = P.src[k][(int64_t)order[i] * w + c];
+    const int64_t i = w == 1 ? t : t / w;
+    const int c = (int)(t - i * w);
+    P.dst[k][t] = P.src[k][(int64_t)order[i] * w + c];
 }
 
 // ----------------------------------------------------------------- fused step passes

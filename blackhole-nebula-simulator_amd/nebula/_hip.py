@@ -167,6 +167,7 @@ _state = threading.local()
 _lock = threading.Lock()
 _LIB = None
 _CTX = None
+_DEV = None  # the torch device of the context (checked once: every step launches dozens of calls)
 
 
 def lib() -> ctypes.CDLL:
@@ -178,11 +179,16 @@ def lib() -> ctypes.CDLL:
 
 
 def torch_device():
-    import torch
+    """The GPU the library runs on (the current device when first asked; the context is bound
+    to it)."""
+    global _DEV
+    if _DEV is None:
+        import torch
 
-    if not torch.cuda.is_available():
-        raise RuntimeError("no GPU visible: the SPH step runs only through its HIP kernels")
-    return torch.device("cuda", torch.cuda.current_device())
+        if not torch.cuda.is_available():
+            raise RuntimeError("no GPU visible: the SPH step runs only through its HIP kernels")
+        _DEV = torch.device("cuda", torch.cuda.current_device())
+    return _DEV
 
 
 def ctx():
@@ -200,9 +206,10 @@ def ctx():
 
 
 def stream():
+    """torch's current stream on the library's device (the stream every entry point launches on)."""
     import torch
 
-    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    return ctypes.c_void_p(torch._C._cuda_getCurrentRawStream(torch_device().index))
 
 
 def check(rc: int, what: str = "") -> int:
@@ -276,8 +283,9 @@ def stop_kernel_timing() -> dict:
 
 
 def call(name: str, *args) -> int:
+    c = _CTX if _CTX is not None else ctx()
     if _TIMING is None:
-        return check(getattr(lib(), name)(ctx(), stream(), *args), name)
+        return check(getattr(_LIB, name)(c, stream(), *args), name)
     import torch
 
     a = torch.cuda.Event(enable_timing=True)

@@ -1374,11 +1374,9 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const Gr
     const bool fast = m <= cap && fabs(h_new - h0) < A.tol * h0 && !(h_new >= r_lim * (1 - 1e-3));
     // [0] longest gather row, [1] rows left slow, [2] an extras segment overflowed, [3] gather rows
     // over capacity, [4] longest extras segment
-    block_partial<true>(live ? m : 0, A.part);
-    block_partial<false>(live && glane == 0 && !fast ? 1 : 0, A.part + gridDim.x);
-    block_partial<true>(eover, A.part + 2 * gridDim.x);
-    block_partial<false>(live && glane == 0 && m > cap ? 1 : 0, A.part + 3 * gridDim.x);
-    block_partial<true>(emax, A.part + 4 * gridDim.x);
+    int32_t pv[5] = {live ? m : 0, live && glane == 0 && !fast ? 1 : 0, (int32_t)eover,
+                     live && glane == 0 && m > cap ? 1 : 0, (int32_t)emax};
+    block_partials<5>(pv, 0b10101u, A.part);
     if (live && glane == 0) {
         A.slow[i] = fast ? 0 : 1;
         if (fast) {

@@ -171,6 +171,39 @@ __device__ __forceinline__ void block_partial(long long v, long long* part)
     }
 }
 
+// K per-thread counts reduced over the workgroup in one pass (one pair of barriers for all K):
+// value k of workgroup b goes to part[k * gridDim.x + b], the maximum over the workgroup where
+// bit k of max_mask is set, else the sum (the sums must fit in int32). Every thread of the block
+// must call it; blockDim.x a multiple of 64, at most 1024.
+template <int K>
+__device__ __forceinline__ void block_partials(int32_t (&v)[K], unsigned max_mask, long long* part)
+{
+    __shared__ int32_t sh_parts[16][K];
+    for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int32_t t = __shfl_xor(v[k], o, 64);
+            v[k] = (max_mask >> k) & 1u ? (t > v[k] ? t : v[k]) : v[k] + t;
+        }
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) sh_parts[threadIdx.x >> 6][k] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < (unsigned)K) {
+        const int k = threadIdx.x;
+        const bool mx = (max_mask >> k) & 1u;
+        long long r = sh_parts[0][k];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+            const long long x = sh_parts[w][k];
+            r = mx ? (x > r ? x : r) : r + x;
+        }
+        part[(int64_t)k * gridDim.x + blockIdx.x] = r;
+    }
+}
+
 // Particle rows of a pass: rows[q] if a row list is given, else q.
 __device__ __forceinline__ int64_t row_of(const int64_t* rows, int64_t q)
 {

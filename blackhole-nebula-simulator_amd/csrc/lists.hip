@@ -469,8 +469,8 @@ k_gather(const double* __restrict__ pos, const double* __restrict__ h, int64_t n
         rowcnt[i] = m;
         unsorted[i] = sorted ? 0 : 1;
     }
-    block_partial<true>(m, part);
-    block_partial<false>(live && !sorted ? 1 : 0, part + gridDim.x);
+    int32_t pv[2] = {(int32_t)m, live && !sorted ? 1 : 0};
+    block_partials<2>(pv, 1u, part);
 }
 
 // Insertion sort of the rows marked unsorted (flag bit ignored in the key).
@@ -930,10 +930,8 @@ k_gather_solve(const double* __restrict__ pos, const double* __restrict__ h, int
         }
     }
     const bool done_here = sorted && m <= cap;
-    block_partial<true>(m, A.part);
-    block_partial<false>(live && !sorted ? 1 : 0, A.part + gridDim.x);
-    block_partial<false>(live && !done_here ? 1 : 0, A.part + 2 * gridDim.x);
-    block_partial<false>(live && m > cap ? 1 : 0, A.part + 3 * gridDim.x);
+    int32_t pv[4] = {(int32_t)m, live && !sorted ? 1 : 0, live && !done_here ? 1 : 0, live && m > cap ? 1 : 0};
+    block_partials<4>(pv, 1u, A.part);
     if (!live) return;
     rowcnt[i] = m;
     unsorted[i] = sorted ? 0 : 1;
@@ -1496,8 +1494,8 @@ k_row_total_rm(const int32_t* __restrict__ rowcnt, const unsigned int* __restric
     } else if (i == n) {
         mc[n] = 0;
     }
-    block_partial<true>(t, part);
-    block_partial<false>(t, part + gridDim.x);
+    long long pv[2] = {t, t};
+    block_partials<2, long long>(pv, 1u, part);
 }
 
 // Extras arrive in atomic order: each row's segment is put in index order (so that the tightened

@@ -173,16 +173,16 @@ __device__ __forceinline__ void block_partial(long long v, long long* part)
 
 // K per-thread counts reduced over the workgroup in one pass (one pair of barriers for all K):
 // value k of workgroup b goes to part[k * gridDim.x + b], the maximum over the workgroup where
-// bit k of max_mask is set, else the sum (the sums must fit in int32). Every thread of the block
-// must call it; blockDim.x a multiple of 64, at most 1024.
-template <int K>
-__device__ __forceinline__ void block_partials(int32_t (&v)[K], unsigned max_mask, long long* part)
+// bit k of max_mask is set, else the sum (the sums must fit in T). Every thread of the block must
+// call it; blockDim.x a multiple of 64, at most 1024.
+template <int K, class T = int32_t>
+__device__ __forceinline__ void block_partials(T (&v)[K], unsigned max_mask, long long* part)
 {
-    __shared__ int32_t sh_parts[16][K];
+    __shared__ T sh_parts[16][K];
     for (int o = 32; o > 0; o >>= 1) {
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            const int32_t t = __shfl_xor(v[k], o, 64);
+            const T t = __shfl_xor(v[k], o, 64);
             v[k] = (max_mask >> k) & 1u ? (t > v[k] ? t : v[k]) : v[k] + t;
         }
     }

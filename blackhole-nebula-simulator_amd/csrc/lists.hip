@@ -1032,19 +1032,37 @@ k_tight_emit(const uint32_t* __restrict__ gbuf, const double* __restrict__ dbuf,
         const double hi = hf[i];
         int32_t p = 0;
         int64_t q = 0;
+        // the head of each input held in registers; the one consumed is refilled while the
+        // neighbour's h is loaded (the same merge order and entries)
+        uint32_t gj = 0, ej = 0;
+        double gd = 0.0, edv = 0.0;
+        if (ng > 0) {
+            gj = gbuf[base] & kMask;
+            gd = dbuf[base];
+        }
+        if (ne > 0) {
+            ej = e[0];
+            edv = ed[0];
+        }
         while (p < ng || q < ne) {
-            uint32_t j;
-            double d2;
-            if (q >= ne || (p < ng && (gbuf[base + (int64_t)p * 64] & kMask) < e[q])) {
-                j = gbuf[base + (int64_t)p * 64] & kMask;
-                d2 = dbuf[base + (int64_t)p * 64];
+            const bool take_g = q >= ne || (p < ng && gj < ej);
+            const uint32_t j = take_g ? gj : ej;
+            const double d2 = take_g ? gd : edv;
+            const double hj = hf[j];
+            if (take_g) {
                 p++;
+                if (p < ng) {
+                    gj = gbuf[base + (int64_t)p * 64] & kMask;
+                    gd = dbuf[base + (int64_t)p * 64];
+                }
             } else {
-                j = e[q];
-                d2 = ed[q];
                 q++;
+                if (q < ne) {
+                    ej = e[q];
+                    edv = ed[q];
+                }
             }
-            const double rr = NB_R_KERN * pymax(hi, hf[j]);
+            const double rr = NB_R_KERN * pymax(hi, hj);
             if (d2 < rr * rr && o < cap_out) {
                 out_idx[obase + (int64_t)o * 64] = (int32_t)j;
                 o++;
@@ -1065,7 +1083,7 @@ k_tight_emit(const uint32_t* __restrict__ gbuf, const double* __restrict__ dbuf,
 // candidate on every pass whatever the column lengths. Candidate positions and search lengths
 // come in one 32-byte record. The FP64 list predicate keeps the lists exact; accepted entries
 // are compacted with a ballot (lane l's slot is the number of accepting lanes below it) into
-// rows of their own capacity (row i at goff[i] .. goff[i + 1]) with their FP32 d2 (and mass).
+// rows of their own capacity (row i at goff[i] .. goff[i + 1]) of neighbour indices.
 // The first evaluation of the h-rho Newton solve (sph.py:251-297, at the starting h) is summed
 // on the fly, in FP32 with compensated accumulation per lane, combined in FP64 by the group,
 // and the group takes the Newton step in FP64. Rows converged after that step (the common case)

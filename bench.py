@@ -567,8 +567,8 @@ def _per_step(d, steps):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
-                                   "described in DESIGN.md, not implemented)"}))  # fmt: skip
+        print(json.dumps({"error": "the benchmark runs on one GPU; --gpus must be 1 (sharding the particles "
+                                   "across GPUs is designed, not implemented)"}))  # fmt: skip
         sys.exit(2)
     from nebula import _hip
 

@@ -10,7 +10,7 @@ predictor d log h = div v dt / 3 (h rho^(1/3) fixed, d log rho / dt = -div v): d
 + MLP(features). An untrained or zero network then still starts from the physical predictor.
 
 The Newton iteration of the density pass then starts from h_prev exp(delta) instead of h_prev.
-It still runs to the same tolerance and stays the judge; a prediction that is not finite, or
+It still runs to the same tolerance and stays the arbiter; a prediction that is not finite, or
 that changes h by a factor 2 or more, falls back to h_prev (counted). The search lists are
 built from h_prev as before, so the lists are the plain method's.
 

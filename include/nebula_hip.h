@@ -391,7 +391,7 @@ int32_t nb_sum_into(nb_ctx *ctx, void *stream, const double *x, int64_t n, doubl
  * No counterpart in the reference: the learned-h mode of this implementation. A small MLP
  * (8 -> 64 -> 64 -> 1, tanh) predicts delta = log h_final - log h_prev from the previous step's
  * state; the Newton iteration starts from h_prev exp(delta) and still converges to the same
- * tolerance (it stays the judge). params: nb_mlp_param_count() floats, W1[8][64], b1[64],
+ * tolerance (it stays the arbiter). params: nb_mlp_param_count() floats, W1[8][64], b1[64],
  * W2[64][64], b2[64], w3[64], b3. norm: 18 floats, feature means[8], feature 1/std[8], target
  * mean, target scale. Features (raw, float32, 8 per row): log h, log rho, div v dt, |curl v| dt,
  * log(neighbour count), log r, log u, dt / dt_prev. */

@@ -986,15 +986,24 @@ __global__ void k_extras_fill_d2(const uint32_t* __restrict__ gbuf, const double
     if (i >= n) return;
     const int64_t base = (int64_t)boff[i >> 6] + (i & 63);
     const int32_t m = rowcnt[i];
-    for (int32_t a = 0; a < m; a++) {
-        const uint32_t v = gbuf[base + (int64_t)a * 64];
+    auto put = [&](uint32_t v, int32_t a) {
         if (v & kFlag) {
             const uint32_t j = v & kMask;
             const unsigned int s = atomicAdd(&efill[j], 1u);
             ebuf[eoff[j] + s] = (uint32_t)i;
             edbuf[eoff[j] + s] = dbuf[base + (int64_t)a * 64];
         }
+    };
+    // four entries' flags loaded before any is handled (same entries, same order per row)
+    int32_t a = 0;
+    for (; a + 4 <= m; a += 4) {
+        uint32_t v[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) v[t] = gbuf[base + (int64_t)(a + t) * 64];
+#pragma unroll
+        for (int t = 0; t < 4; t++) put(v[t], a + t);
     }
+    for (; a < m; a++) put(gbuf[base + (int64_t)a * 64], a);
 }
 
 // neighbours.py:284-303 on the merged row (gather row + sorted extras, both with d2), with the

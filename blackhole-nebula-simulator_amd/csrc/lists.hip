@@ -1356,8 +1356,10 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const Gr
                 const double dz = pj.z - zi;
                 d2 = dx * dx + dy * dy + dz * dz;
                 pred = d2 < r2;
+                // evaluated on every lane (no branch on pred), so the record's search length
+                // arrives with its position rather than in a load issued after the test
                 const double rj = NB_R_KERN * pj.w;
-                flag = pred && !(d2 < rj * rj);
+                flag = pred & !(d2 < rj * rj);
             }
             const unsigned long long bal = __ballot(pred);
             const unsigned gb = (unsigned)((bal >> gshift) & ((1ull << kG) - 1));

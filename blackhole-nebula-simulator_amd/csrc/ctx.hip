@@ -24,6 +24,7 @@ int ctx_reserve(nb_ctx* ctx, void** p, size_t* have, size_t bytes)
         hipFree(*p);
         *p = nullptr;
         *have = 0;
+        ctx->generation++;  // an address that a recording may hold is gone
     }
     size_t want = bytes + bytes / 8;  // headroom: sizes drift from step to step
     if (hipMalloc(p, want) != hipSuccess) {
@@ -122,35 +123,58 @@ extern "C" nb_ctx* nb_ctx_create(int32_t device)
     return ctx;
 }
 
+// every scratch buffer of the arena with its size, for release
+#define NB_CTX_SCRATCH(X)                                                                                  \
+    X(cell_id) X(keys_out) X(iota) X(order) X(cell_count) X(cell_start) X(gbuf) X(rowcnt) X(unsorted)     \
+    X(pending) X(edbuf) X(ecount) X(eoff) X(efill) X(ebuf) X(tot) X(offs) X(tcount) X(dscratch) X(fscratch) \
+    X(pack) X(part) X(posh) X(mlp_scratch) X(mlp_pack) X(mlp_part) X(segs) X(red) X(scal) X(cub_tmp) X(rm_gc) \
+    X(rm_go) X(rm_ec) X(rm_eo) X(rm_to) X(ebuf_s) X(fx_tmp) X(fb_sz) X(fb_go) X(fb_to) X(segpart) X(xlist)
+
+static void ctx_free_scratch(nb_ctx* ctx)
+{
+#define NB_FREE_ONE(F)          \
+    if (ctx->F) hipFree(ctx->F); \
+    ctx->F = nullptr;           \
+    ctx->F##_bytes = 0;
+    NB_CTX_SCRATCH(NB_FREE_ONE)
+#undef NB_FREE_ONE
+}
+
 extern "C" void nb_ctx_destroy(nb_ctx* ctx)
 {
     if (!ctx) return;
     ktime_flush(ctx);
     for (hipEvent_t e : ctx->ev_pool) hipEventDestroy(e);
-    void* bufs[] = {ctx->cell_id, ctx->keys_out, ctx->iota, ctx->order, ctx->cell_count, ctx->cell_start,
-                    ctx->gbuf, ctx->rowcnt, ctx->unsorted, ctx->pending, ctx->edbuf, ctx->ecount, ctx->eoff, ctx->efill, ctx->ebuf, ctx->tot,
-                    ctx->offs, ctx->tcount, ctx->dscratch, ctx->fscratch, ctx->pack, ctx->part, ctx->posh, ctx->mlp_scratch, ctx->mlp_pack, ctx->mlp_part, ctx->segs, ctx->red, ctx->scal, ctx->cub_tmp,
-                    ctx->rm_gc, ctx->rm_go, ctx->rm_ec, ctx->rm_eo, ctx->rm_to, ctx->ebuf_s, ctx->fx_tmp,
-                    ctx->fb_sz, ctx->fb_go, ctx->fb_to, ctx->segpart, ctx->xlist};
-    for (void* b : bufs)
-        if (b) hipFree(b);
+    ctx_free_scratch(ctx);
     delete ctx;
+}
+
+extern "C" int32_t nb_ctx_trim(nb_ctx* ctx)
+{
+    if (!ctx) return NB_ERR_ARG;
+    if (hipSetDevice(ctx->device) != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+        return ctx_fail_code(ctx, NB_ERR_HIP, "nb_ctx_trim: device synchronisation failed");
+    ctx_free_scratch(ctx);
+    // what lived in the arena is gone: passes that read an earlier pass's scratch must be redone
+    ctx->lists_ready = ctx->hydro_ready = ctx->hydro_mixed_ready = 0;
+    ctx->fx_ready = ctx->fx_grid_from_order = 0;
+    ctx->host_segs.clear();
+    ctx->generation++;
+    return NB_OK;
 }
 
 extern "C" const char* nb_last_error(const nb_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
 
 extern "C" int32_t nb_abi_version(void) { return 4; }
 
+extern "C" int64_t nb_ctx_generation(const nb_ctx* ctx) { return ctx ? (int64_t)ctx->generation : -1; }
+
 extern "C" int64_t nb_ctx_scratch_bytes(const nb_ctx* ctx)
 {
     if (!ctx) return 0;
-    size_t s = ctx->cell_id_bytes + ctx->keys_out_bytes + ctx->iota_bytes + ctx->order_bytes +
-               ctx->cell_count_bytes + ctx->cell_start_bytes + ctx->gbuf_bytes + ctx->rowcnt_bytes + ctx->unsorted_bytes + ctx->pending_bytes + ctx->edbuf_bytes +
-               ctx->ecount_bytes + ctx->eoff_bytes + ctx->efill_bytes + ctx->ebuf_bytes + ctx->tot_bytes +
-               ctx->offs_bytes + ctx->tcount_bytes + ctx->dscratch_bytes + ctx->fscratch_bytes + ctx->pack_bytes + ctx->part_bytes + ctx->posh_bytes + ctx->mlp_scratch_bytes + ctx->mlp_pack_bytes + ctx->mlp_part_bytes + ctx->segs_bytes +
-               ctx->red_bytes + ctx->scal_bytes + ctx->cub_tmp_bytes + ctx->rm_gc_bytes + ctx->rm_go_bytes +
-               ctx->rm_ec_bytes + ctx->rm_eo_bytes + ctx->rm_to_bytes + ctx->ebuf_s_bytes +
-               ctx->fx_tmp_bytes + ctx->fb_sz_bytes + ctx->fb_go_bytes + ctx->fb_to_bytes +
-               ctx->segpart_bytes + ctx->xlist_bytes;
+    size_t s = 0;
+#define NB_ADD_ONE(F) s += ctx->F##_bytes;
+    NB_CTX_SCRATCH(NB_ADD_ONE)
+#undef NB_ADD_ONE
     return (int64_t)s;
 }

@@ -28,6 +28,10 @@ struct nb_kernel_time {
 struct nb_ctx {
     int device = 0;
     std::string err;
+    // bumped whenever ctx_reserve moves a buffer it had handed out: a recorded graph holds raw
+    // pointers into the arena, so a caller replays one only while the generation it was
+    // recorded at is still current (nb_ctx_generation)
+    uint64_t generation = 0;
     // kernel timing (nb_timing)
     int timing = 0;
     std::vector<hipEvent_t> ev_pool;

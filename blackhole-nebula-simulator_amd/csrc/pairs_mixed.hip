@@ -3,7 +3,8 @@
 //
 // A group of 8 lanes serves one row: lane l takes entries l, l + 8, ... so that the group
 // reads a row-major or CSR row contiguously. Each lane accumulates its pairs in FP32 with
-// compensated (Kahan) sums and the group combines the lane partials in FP64. Separations are
+// compensated sums (Kahan's; TwoSum for the pair forces, whose terms must cancel exactly between
+// the two rows of a pair) and the group combines the lane partials in FP64. Separations are
 // formed in FP64 (positions stay FP64: at r ~ 2000 an FP32 position resolves only ~1e-4 of a
 // unit length) and then rounded to FP32; every other per-neighbour quantity comes from a packed
 // FP32 record prepared once per pass (48 bytes for forces, 32 for gradients: whole 16-byte loads).
@@ -29,8 +30,23 @@ static constexpr int kBlockM = 256;
 struct __align__(16) PF {
     float4 a;  // vx, vy, vz, 1/h
     float4 b;  // cnorm / h^4, P / (Omega rho^2), 1 / (Omega rho), alpha c_s
-    float4 c;  // m, u, P, rho
+    float4 c;  // m s (s = mass_scale), u, P, rho
 };
+
+// The force pass weighs each pair by the product of both FP32 masses, (m_a s)(m_b s): that
+// product is the same bits in row a and in row b, so the pair force m_a m_b S_ab r_ab of one row
+// is exactly minus that of the other, and the row's acceleration is that sum over the FP64 mass
+// m_a s^2. Weighing by the FP32 neighbour mass alone (a = sum m_b S r) rounds m_a fl(m_b S)
+// and m_b fl(m_a S) differently when masses differ, which leaks momentum and angular momentum
+// at 1e-7 relative per step. s is a power of two taken from the first particle's mass, so that
+// the FP32 products stay far from under- and overflow whatever the mass unit; scaling by it
+// changes no mantissa.
+__device__ __forceinline__ double mass_scale(const double* __restrict__ mass)
+{
+    int e = 0;
+    (void)frexp(mass[0], &e);
+    return ldexp(1.0, -e);
+}
 
 struct __align__(16) PG {
     float4 a;  // vx, vy, vz, 1/h
@@ -45,12 +61,12 @@ __global__ void k_pack_forces(const double* __restrict__ vel, const double* __re
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double hi = h[i], r = rho[i], om = omega[i], p = pressure[i];
+    const double hi = h[i], r = rho[i], om = omega[i], p = pressure[i], ms = mass_scale(mass);
     PF v;
     v.a = make_float4((float)vel[3 * i], (float)vel[3 * i + 1], (float)vel[3 * i + 2], (float)(1.0 / hi));
     v.b = make_float4((float)(cnorm / ipow4(hi)), (float)(p / (om * r * r)), (float)(1.0 / (om * r)),
                       (float)(alpha ? alpha[i] * c_s[i] : 0.0));
-    v.c = make_float4((float)mass[i], (float)(u ? u[i] : 0.0), (float)p, (float)r);
+    v.c = make_float4((float)(mass[i] * ms), (float)(u ? u[i] : 0.0), (float)p, (float)r);
     out[i] = v;
 }
 
@@ -99,6 +115,25 @@ struct Kahan {
     __device__ __forceinline__ double value() const { return (double)s - (double)c; }
 };
 
+// Compensated sum whose every term enters exactly (Knuth's TwoSum, the Kahan-Babuska form): the
+// rounding error of each addition is recovered exactly and gathered in c. The pair forces use it:
+// their terms are rounded products fl(f r_x), which row b forms with the opposite sign, so the
+// two rows hold exactly opposite terms and their sums are accurate to O(n u^2) of the terms.
+// Kahan's form (above) rounds y = x - c before the addition, an error of u |x| per term that
+// differs between row a and row b: momentum then leaks at about u / sqrt(N) per pass (2e-10 of
+// sum m|a| at 16k particles, against 1e-16 in FP64).
+struct TwoSumAcc {
+    float s = 0.f, c = 0.f;
+    __device__ __forceinline__ void add(float p)
+    {
+        const float t = s + p;
+        const float z = t - s;
+        c += (s - (t - z)) + (p - z);
+        s = t;
+    }
+    __device__ __forceinline__ double value() const { return (double)s + (double)c; }
+};
+
 // butterfly over the W lanes of a group: every lane ends with the same value
 template <int W>
 __device__ __forceinline__ double gsum(double v)
@@ -137,13 +172,15 @@ __device__ __forceinline__ Nb<R> load_nb(const double* __restrict__ pos, const R
 }
 
 // sph.py:525-652 without reconstruction: pressure force with grad-h terms, Monaghan viscosity on
-// approaching pairs, conductivity (pressure or velocity signal), P dV work, largest v_sig.
+// approaching pairs, conductivity (pressure or velocity signal), P dV work, largest v_sig. The
+// pair force is weighed by both masses (see PF) and the row's sum divided by m_a s^2 in FP64;
+// the energy sums carry the neighbour's scaled mass and are divided by s (exact).
 template <class L, int W>
 __global__ void __launch_bounds__(kBlockM)
-k_forces_mixed(const double* __restrict__ pos, const PF* __restrict__ pf, const double* __restrict__ c_s, L lists,
-               Box box, const int64_t* __restrict__ rows, int64_t nrows, nb_force_params P, double* __restrict__ acc,
-               double* __restrict__ dudt, double* __restrict__ vsig_max, double* __restrict__ vheat,
-               double* __restrict__ cheat)
+k_forces_mixed(const double* __restrict__ pos, const double* __restrict__ mass, const PF* __restrict__ pf,
+               const double* __restrict__ c_s, L lists, Box box, const int64_t* __restrict__ rows, int64_t nrows,
+               nb_force_params P, double* __restrict__ acc, double* __restrict__ dudt, double* __restrict__ vsig_max,
+               double* __restrict__ vheat, double* __restrict__ cheat)
 {
     const int glane = threadIdx.x & (W - 1);
     const int64_t q = (xcd_tile() * (int64_t)blockDim.x + threadIdx.x) / W;
@@ -151,17 +188,22 @@ k_forces_mixed(const double* __restrict__ pos, const PF* __restrict__ pf, const 
     const int64_t a = row_of(rows, q);
     const double pa0 = pos[3 * a], pa1 = pos[3 * a + 1], pa2 = pos[3 * a + 2];
     const PF A = pf[a];
+    // a massless row particle keeps the neighbour-mass weighting (its own mass would zero the sum)
+    const float wa = A.c.x > 0.f ? A.c.x : 1.f;
     const float beta = (float)P.beta, alpha_u = (float)P.alpha_u;
     const bool use_visc = P.use_visc, use_cond = P.use_cond;
     const int kid = P.kid;
-    Kahan ax, ay, az, work, visc, cond;
+    TwoSumAcc ax, ay, az;
+    float lx = 0.f, ly = 0.f, lz = 0.f;  // low parts of the pair forces
+    Kahan work, visc, cond;
     float vmax = 0.f;
     const int64_t m = lists.count(a);
     auto pair = [&](const Nb<PF>& N) {
         const PF& B = N.rec;
-        const float dx0 = (float)min_image(pa0 - N.x0, box.l[0]);
-        const float dx1 = (float)min_image(pa1 - N.x1, box.l[1]);
-        const float dx2 = (float)min_image(pa2 - N.x2, box.l[2]);
+        const double D0 = min_image(pa0 - N.x0, box.l[0]);
+        const double D1 = min_image(pa1 - N.x1, box.l[1]);
+        const double D2 = min_image(pa2 - N.x2, box.l[2]);
+        const float dx0 = (float)D0, dx1 = (float)D1, dx2 = (float)D2;
         const float r2 = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
         const float inv_r = rsqrt32(r2);
         const float r = r2 * inv_r;
@@ -186,10 +228,20 @@ k_forces_mixed(const double* __restrict__ pos, const PF* __restrict__ pf, const 
                 P.cond_signal == 0 ? sqrt32(fabsf(A.c.z - B.c.z) * rcp32(0.5f * (A.c.w + B.c.w))) : fabsf(vdotr);
             cond.add(B.c.x * alpha_u * vu * (A.c.y - B.c.y) * 0.5f * (fa * A.b.z + fb * B.b.z));
         }
-        const float f = B.c.x * (na * fa + nbb * fb);
-        ax.add_prod(-f, rx);
-        ay.add_prod(-f, ry);
-        az.add_prod(-f, rz);
+        // The pair force -f x_ab / r, split as hi + lo: hi = fl(g d) with g = f / r and d the FP32
+        // separation; lo = (g d - hi) + g (x_ab - d), the product's rounding error (exact, by
+        // FMA) and the separation's. Both are exactly opposite in row b, and hi + lo lies along
+        // the FP64 separation to O(u^2), so the pair exerts no torque beyond that (hi alone
+        // would tilt each force by u and leave a torque of u |x_ab| |F|).
+        const float f = (wa * B.c.x) * (na * fa + nbb * fb);
+        const float g = f * inv_r;
+        const float h0 = g * dx0, h1 = g * dx1, h2 = g * dx2;
+        ax.add(-h0);
+        ay.add(-h1);
+        az.add(-h2);
+        lx -= __builtin_fmaf(g, (float)(D0 - (double)dx0), __builtin_fmaf(g, dx0, -h0));
+        ly -= __builtin_fmaf(g, (float)(D1 - (double)dx1), __builtin_fmaf(g, dx1, -h1));
+        lz -= __builtin_fmaf(g, (float)(D2 - (double)dx2), __builtin_fmaf(g, dx2, -h2));
         work.add_prod(B.c.x * vdotr, fa);
     };
     // two entries per pass, both neighbours' records loaded before either pair's arithmetic: the
@@ -201,17 +253,19 @@ k_forces_mixed(const double* __restrict__ pos, const PF* __restrict__ pf, const 
         if (b0 != a) pair(N0);
         if (b1 != a) pair(N1);
     }
-    const double sx = gsum<W>(ax.value()), sy = gsum<W>(ay.value()), sz = gsum<W>(az.value());
+    const double sx = gsum<W>(ax.value() + lx), sy = gsum<W>(ay.value() + ly), sz = gsum<W>(az.value() + lz);
     const double sw = gsum<W>(work.value()), sv = gsum<W>(visc.value()), sc = gsum<W>(cond.value());
     const float vm = gmax<W>(vmax);
     if (glane == 0) {
-        acc[3 * q] = sx;
-        acc[3 * q + 1] = sy;
-        acc[3 * q + 2] = sz;
-        dudt[q] = (double)A.b.y * sw + sv + sc;
+        const double ms = mass_scale(mass);
+        const double da = A.c.x > 0.f ? (ms * ms) * mass[a] : ms;  // m_a s^2: acc = sum / (m_a s^2)
+        acc[3 * q] = sx / da;
+        acc[3 * q + 1] = sy / da;
+        acc[3 * q + 2] = sz / da;
+        dudt[q] = (double)A.b.y * (sw / ms) + sv / ms + sc / ms;
         vsig_max[q] = use_visc ? fmax(c_s[a], (double)vm) : 0.0;
-        if (vheat) vheat[q] = sv;
-        if (cheat) cheat[q] = sc;
+        if (vheat) vheat[q] = sv / ms;
+        if (cheat) cheat[q] = sc / ms;
     }
 }
 
@@ -382,8 +436,8 @@ int32_t forces_mixed(nb_ctx* ctx, hipStream_t s, const double* pos, const double
     PF* pf = (PF*)ctx->pack;
     hipLaunchKernelGGL(k_pack_forces, dim3(grid_1(n)), dim3(kBlockM), 0, s, vel, mass, h, rho, omega, pressure,
                        P->use_cond ? u : nullptr, c_s, P->use_visc ? alpha : nullptr, P->cnorm, n, pf);
-    NB_DISPATCH_M(lists, k_forces_mixed, kPairW, nrows, s, pos, pf, c_s, v_, box_of(box3), rows, nrows, *P, acc, dudt,
-                  vsig_max, vheat, cheat);
+    NB_DISPATCH_M(lists, k_forces_mixed, kPairW, nrows, s, pos, mass, pf, c_s, v_, box_of(box3), rows, nrows, *P, acc,
+                  dudt, vsig_max, vheat, cheat);
     return launched(ctx, "nb_forces (mixed)");
 }
 

@@ -200,7 +200,10 @@ __global__ void k_cool_powerlaw(const double* __restrict__ u, const double* __re
 // ----------------------------------------------------------------- reductions
 // Time-step minima (coupled.py:213-226): out[0] = min h/v_sig, out[1] = min sqrt(h/|a|) over
 // |a| > 0, out[2] = min viscous dt C h^2/nu over nu > 0 (if nu given), out[3] = min |x|.
-// Minima are exact whatever the reduction order.
+// Minima are exact whatever the reduction order. A NaN term makes its minimum NaN, as np.min
+// does (fmin would drop it): a particle that blew up then gives the reference's NaN dt instead
+// of a finite one taken from the others.
+__device__ __forceinline__ double nan_min(double a, double b) { return (a < b || a != a) ? a : b; }
 __global__ void k_dt_partial(const double* __restrict__ h, const double* __restrict__ vsig,
                              const double* __restrict__ acc, const double* __restrict__ nu, double c_visc,
                              const double* __restrict__ x, int64_t n, double* __restrict__ part)
@@ -208,20 +211,20 @@ __global__ void k_dt_partial(const double* __restrict__ h, const double* __restr
     double v[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         double hi = h[i];
-        v[0] = fmin(v[0], hi / vsig[i]);
+        v[0] = nan_min(v[0], hi / vsig[i]);
         double a0 = acc[3 * i], a1 = acc[3 * i + 1], a2 = acc[3 * i + 2];
         double am = sqrt(a0 * a0 + a1 * a1 + a2 * a2);
-        if (am > 0.0) v[1] = fmin(v[1], sqrt(hi / am));
-        if (nu && nu[i] > 0.0) v[2] = fmin(v[2], c_visc * (hi * hi) / nu[i]);
+        if (am > 0.0) v[1] = nan_min(v[1], sqrt(hi / am));
+        if (nu && nu[i] > 0.0) v[2] = nan_min(v[2], c_visc * (hi * hi) / nu[i]);
         double x0 = x[3 * i], x1 = x[3 * i + 1], x2 = x[3 * i + 2];
-        v[3] = fmin(v[3], sqrt(x0 * x0 + x1 * x1 + x2 * x2));
+        v[3] = nan_min(v[3], sqrt(x0 * x0 + x1 * x1 + x2 * x2));
     }
     __shared__ double sh[4][kBlockS];
     for (int k = 0; k < 4; k++) sh[k][threadIdx.x] = v[k];
     __syncthreads();
     for (int s = blockDim.x / 2; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s)
-            for (int k = 0; k < 4; k++) sh[k][threadIdx.x] = fmin(sh[k][threadIdx.x], sh[k][threadIdx.x + s]);
+            for (int k = 0; k < 4; k++) sh[k][threadIdx.x] = nan_min(sh[k][threadIdx.x], sh[k][threadIdx.x + s]);
         __syncthreads();
     }
     if (threadIdx.x == 0)
@@ -238,20 +241,20 @@ k_dt_seg(const double* __restrict__ h, const double* __restrict__ vsig, const do
     double v[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
     for (int64_t i = off[blockIdx.x] + threadIdx.x; i < off[blockIdx.x + 1]; i += blockDim.x) {
         const double hi = h[i];
-        v[0] = fmin(v[0], hi / vsig[i]);
+        v[0] = nan_min(v[0], hi / vsig[i]);
         const double a0 = acc[3 * i], a1 = acc[3 * i + 1], a2 = acc[3 * i + 2];
         const double am = sqrt(a0 * a0 + a1 * a1 + a2 * a2);
-        if (am > 0.0) v[1] = fmin(v[1], sqrt(hi / am));
-        if (nu && nu[i] > 0.0) v[2] = fmin(v[2], c_visc * (hi * hi) / nu[i]);
+        if (am > 0.0) v[1] = nan_min(v[1], sqrt(hi / am));
+        if (nu && nu[i] > 0.0) v[2] = nan_min(v[2], c_visc * (hi * hi) / nu[i]);
         const double x0 = x[3 * i], x1 = x[3 * i + 1], x2 = x[3 * i + 2];
-        v[3] = fmin(v[3], sqrt(x0 * x0 + x1 * x1 + x2 * x2));
+        v[3] = nan_min(v[3], sqrt(x0 * x0 + x1 * x1 + x2 * x2));
     }
     __shared__ double sh[4][kBlockS];
     for (int k = 0; k < 4; k++) sh[k][threadIdx.x] = v[k];
     __syncthreads();
     for (int st = blockDim.x / 2; st > 0; st >>= 1) {
         if ((int)threadIdx.x < st)
-            for (int k = 0; k < 4; k++) sh[k][threadIdx.x] = fmin(sh[k][threadIdx.x], sh[k][threadIdx.x + st]);
+            for (int k = 0; k < 4; k++) sh[k][threadIdx.x] = nan_min(sh[k][threadIdx.x], sh[k][threadIdx.x + st]);
         __syncthreads();
     }
     if (threadIdx.x == 0)
@@ -326,7 +329,7 @@ __global__ void k_reduce_final(const double* __restrict__ part, int nparts, int 
         double v = op == 0 ? INFINITY : 0.0;
         for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
             double y = part[w * b + k];
-            v = op == 0 ? fmin(v, y) : v + y;
+            v = op == 0 ? nan_min(v, y) : v + y;
         }
         sh[k][threadIdx.x] = v;
     }
@@ -334,7 +337,7 @@ __global__ void k_reduce_final(const double* __restrict__ part, int nparts, int 
     for (int s = blockDim.x / 2; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s)
             for (int k = 0; k < w; k++)
-                sh[k][threadIdx.x] = op == 0 ? fmin(sh[k][threadIdx.x], sh[k][threadIdx.x + s])
+                sh[k][threadIdx.x] = op == 0 ? nan_min(sh[k][threadIdx.x], sh[k][threadIdx.x + s])
                                              : sh[k][threadIdx.x] + sh[k][threadIdx.x + s];
         __syncthreads();
     }

@@ -71,6 +71,8 @@ SIGNATURES = {
     "nb_last_error": (ctypes.c_char_p, [_vp]),
     "nb_abi_version": (_i32, []),
     "nb_ctx_scratch_bytes": (_i64, [_vp]),
+    "nb_ctx_generation": (_i64, [_vp]),
+    "nb_ctx_trim": (_i32, [_vp]),
     "nb_lists_build": (_i32, [_vp, _vp, _vp, _vp, _i64, _PD, _vp, _vp, _i64, _P64]),
     "nb_lists_emit": (_i32, [_vp, _vp, _vp, _i64]),
     "nb_tighten": (_i32, [_vp, _vp, _vp, _vp, _i64, _PD, _vp, _vp, _vp, _i64, _vp, _vp, _i64, _P64]),
@@ -189,6 +191,17 @@ def torch_device():
             raise RuntimeError("no GPU visible: the SPH step runs only through its HIP kernels")
         _DEV = torch.device("cuda", torch.cuda.current_device())
     return _DEV
+
+
+def generation() -> int:
+    """The context's scratch generation (nb_ctx_generation): changes when the library frees a
+    scratch buffer to grow it, which invalidates a recorded graph."""
+    return int(lib().nb_ctx_generation(ctx()))
+
+
+def trim() -> None:
+    """Release the context's scratch arena (nb_ctx_trim); recorded steps record again."""
+    check(lib().nb_ctx_trim(ctx()), "nb_ctx_trim")
 
 
 def ctx():

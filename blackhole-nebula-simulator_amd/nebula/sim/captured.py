@@ -9,7 +9,9 @@ status word what did not fit. The host reads the status words and the per-cloud 
 after each replay (one small copy), computes each cloud's dt as ``ensemble.stable_dt_clouds``
 does, and writes it to the device for the next replay. If a replay reports anything, the state
 saved at its start is restored, the step is done eagerly, and the graph is recorded again with
-capacities from that step.
+capacities from that step. The graph also holds the addresses of the library's scratch buffers:
+when an eager call has grown (moved) one of them since the recording (``nb_ctx_generation``),
+the step is recorded again before the next replay.
 
 The graph's kernels and their arguments are those of the eager ensemble step, and the grids are
 planned on the device with the host plan's arithmetic, so a replayed step gives the eager step's
@@ -106,6 +108,7 @@ class CapturedEnsemble:
         self.cfg = ens.cfg
         self.headroom = float(headroom)
         self.replays = 0
+        self.rerecords = 0  # recordings redone because library scratch moved under them
         self.fallbacks = 0
         self.fallback_reasons = []
         self._record()
@@ -159,6 +162,7 @@ class CapturedEnsemble:
         if self.node_counts().get("memset", 0):
             # memset nodes have misbehaved on replays after the first: refuse such a recording
             raise RuntimeError("the recorded step holds memset nodes; it is not replayed")
+        self._generation = _hip.generation()
 
     def _record_body(self, commit: bool):
         """The recorded step: S -> S (with S saved in backup first), status and minima written."""
@@ -242,6 +246,12 @@ class CapturedEnsemble:
     def step(self) -> np.ndarray:
         """One replayed step of every cloud; returns the steps taken."""
         dt = self._dt
+        if _hip.generation() != self._generation:
+            # an eager call since the recording grew (moved) library scratch the graph points at
+            self.rerecords += 1
+            self.sync_back()
+            self._record()
+            self._set_dt()
         self.graph.replay()
         packed = torch.cat([self.status.double(), self.minima.reshape(-1)]).cpu().numpy()
         status = packed[:16].astype(np.int64)

@@ -45,8 +45,9 @@ class PairLog:
         self._target: list[torch.Tensor] = []
 
     def add(self, feat: torch.Tensor, target: torch.Tensor) -> None:
-        self._feat.append(feat)
-        self._target.append(target)
+        # copies: a recorded step hands in buffers of its graph, which every replay overwrites
+        self._feat.append(feat.clone())
+        self._target.append(target.clone())
 
     @property
     def size(self) -> int:

@@ -83,10 +83,13 @@ class Stepper:
                                       int(not (st.m_uniform > 0.0))), "nb_fixed_reserve")  # fmt: skip
         self.index = torch.empty(index, dtype=torch.int32, device=st.x.device)
         self.fixed = _Fixed(self.status, self.seg_off_dev, 1, self.index, cloud=self.cloud)
+        self._generation = _hip.generation()
 
     def step(self) -> float:
         """One step of coupled.stable_dt(state); returns the dt taken."""
         st, cfg, dt = self.state, self.cfg, self.dt
+        if _hip.generation() != self._generation:
+            self._reserve()  # the library's scratch was released or moved since the reservation
         self.status.zero_()
         self.fixed.pairs = None
         new = coupled.step(st, cfg, dt, fixed=self.fixed)
@@ -128,6 +131,9 @@ class RecordedStepper(Stepper):
     host reads them in one copy, as ``Stepper`` does. A replay with a nonzero status is undone
     (the saved state restored), the step done by ``coupled.step`` and the graph recorded again.
     The learned start's online training runs between replays, on the pairs of the replay.
+    The graph holds the addresses of the library's scratch buffers; when an eager call since the
+    recording has grown (moved) one of them (``nb_ctx_generation``), the step is recorded again
+    before the next replay.
 
     Covers the physics a recorded step can take its dt from the device for: GN or Newtonian flow,
     power-law cooling, constant or Balsara viscosity, the learned start (no beta cooling, no
@@ -141,6 +147,7 @@ class RecordedStepper(Stepper):
             raise ValueError("the recorded step covers the cloud physics without disc viscosity")
         super().__init__(st, cfg, headroom)
         self.replays = 0
+        self.rerecords = 0  # recordings redone because library scratch moved under them
         self._record()
 
     def _record(self):
@@ -172,6 +179,7 @@ class RecordedStepper(Stepper):
         self.node_counts = {"kernel": int(counts[0]), "memcpy": int(counts[1]), "memset": int(counts[2])}
         if counts[2]:
             raise RuntimeError("the recorded step holds memset nodes; it is not replayed")
+        self._generation = _hip.generation()
 
     def _body(self, commit: bool):
         """The recorded step: S -> S (S saved in backup first), status and minima written."""
@@ -196,6 +204,12 @@ class RecordedStepper(Stepper):
     def step(self) -> float:
         """One replayed step of coupled.stable_dt(state); returns the dt taken."""
         st, cfg, dt = self.state, self.cfg, self.dt
+        if _hip.generation() != self._generation:
+            # an eager call since the recording grew (moved) or released library scratch the graph
+            # points at: reserve the fixed passes' buffers again and record anew
+            self.rerecords += 1
+            self._reserve()
+            self._record()
         self._host_dts[0], self._host_dts[1] = dt, st.dt_prev
         self.dts.copy_(self._host_dts, non_blocking=True)
         self.graph.replay()

@@ -80,6 +80,17 @@ const char *nb_last_error(const nb_ctx *ctx);
 int32_t nb_abi_version(void);
 /* bytes of scratch the context currently holds */
 int64_t nb_ctx_scratch_bytes(const nb_ctx *ctx);
+/* Scratch generation: incremented whenever the context frees a scratch buffer to grow it.
+ * A hipGraph recorded over library calls holds the scratch addresses of its recording, so it
+ * may be replayed only while this number is what it was when the recording ended (any eager
+ * call in between may have grown a buffer). No counterpart in the reference (its loops are
+ * numba; nothing is recorded). */
+int64_t nb_ctx_generation(const nb_ctx *ctx);
+/* Release every scratch buffer of the context (after synchronising its device), for a caller
+ * that wants the memory back between phases of a run. Passes that read an earlier pass's
+ * scratch (lists, the fixed-capacity reservation) must be redone; the generation advances.
+ * No counterpart in the reference. */
+int32_t nb_ctx_trim(nb_ctx *ctx);
 /* Workgroup size (64, 128 or 256 threads = 4, 8 or 16 rows) of the mixed-precision lane-group
  * kernels of this context (no counterpart in the reference; set by the autotuner). */
 int32_t nb_set_tile(nb_ctx *ctx, int32_t tile);

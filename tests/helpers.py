@@ -47,3 +47,50 @@ def bit_equal(name, expected, actual):
     else:
         print(f"PASS {name} (bit-identical, {e.size} values)")
     assert bad.size == 0, name
+
+
+def lattice_sphere(n_target, radius, seed=1, jitter=0.1):
+    """The n_target points of a jittered cubic lattice nearest its centre, scaled to radius (the
+    reference cuts its Wendland glass instead; any near-uniform sphere serves the checks that
+    use it)."""
+    rng = np.random.default_rng(seed)
+    a = (4.0 / 3.0 * np.pi / n_target) ** (1.0 / 3.0)
+    k = int(np.ceil(1.0 / a)) + 2
+    g = np.arange(-k, k + 1) * a
+    x = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    x = x + rng.uniform(-jitter * a, jitter * a, x.shape)
+    x = x[np.argsort(np.linalg.norm(x, axis=1), kind="stable")[:n_target]]
+    return x * (radius / np.linalg.norm(x, axis=1).max())
+
+
+ELL_BLOB = 7.1951756691  # expected-values §2: the r_a = 500 -> r_p = 25 orbit (blob.py:23)
+U_BLOB = 4e-8  # blob.py:29
+T_END_BLOB = 18_000.0  # blob.py:34
+
+
+def hot_blob_ic(mass_noise=0.0, seed=2):
+    """blob.py:36-52: ~500 particles of radius 3 at apocentre x = (500, 0, 0), momentum
+    p = (0, ell / 500, 0), u = U_BLOB, each of mass 1/n (times 1 + mass_noise N(0, 1) when
+    mass_noise is set), h = 1.6 (m / rho)^(1/3). Returns x, v, u, mass, h."""
+    from oracle.integrator import gn_v_from_p
+
+    pos = lattice_sphere(500, 3.0) + np.array([500.0, 0.0, 0.0])
+    n = len(pos)
+    p0 = np.tile([0.0, ELL_BLOB / 500.0, 0.0], (n, 1))
+    rho = 1.0 / (4 / 3 * np.pi * 27.0)
+    mass = np.full(n, 1.0 / n)
+    if mass_noise:
+        mass = mass * (1.0 + mass_noise * np.random.default_rng(seed).standard_normal(n))
+    return pos, gn_v_from_p(pos, p0), np.full(n, U_BLOB), mass, np.full(n, 1.6 * (1 / n / rho) ** (1 / 3))
+
+
+def hot_blob_config(eta_orb, courant, **kw):
+    """blob.py:86-91: the reference's hydro defaults (Wendland C4, Cullen-Dehnen switch,
+    reconstruction, conductivity with the velocity signal) at the given step factors."""
+    from nebula.sim.coupled import CoupledConfig
+    from nebula.sim.hydro import HydroConfig
+    from nebula.sim.sph import COND_VELOCITY
+
+    hydro_kw = {k: kw.pop(k) for k in list(kw) if k in HydroConfig.__dataclass_fields__}
+    return CoupledConfig(hydro=HydroConfig(courant=courant, conductivity_signal=COND_VELOCITY, **hydro_kw),
+                         eta_orb=eta_orb, **kw)  # fmt: skip

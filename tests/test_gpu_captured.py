@@ -91,3 +91,33 @@ def test_overflow_falls_back_to_an_eager_step():
     cap.step()
     check("re-recorded with the eager step's sizes: no further fallback", 1, cap.fallbacks)
     _compare("next", eager, cap)
+
+
+def test_replay_records_again_when_scratch_moves():
+    """Releasing the library's scratch and an eager step of a larger ensemble between replays move
+    buffers the recorded graph points at: the next replay is recorded again first, with the eager
+    bits."""
+    from nebula import _hip
+    from nebula.ics import jittered_sphere_on_parabola
+    from nebula.sim import coupled
+    from nebula.sim.captured import CapturedEnsemble
+    from nebula.sim.ensemble import Ensemble
+
+    cfg = coupled.baseline_config(precision="mixed")
+    print("captured ensemble: a 40000-particle eager ensemble step between replays", flush=True)
+    eager = Ensemble(_clouds(), cfg)
+    eager.step()
+    other = Ensemble(_clouds(), cfg)
+    other.step()
+    cap = CapturedEnsemble(other)
+    for k in range(3):
+        bit_equal(f"replay {k}: dt of each cloud", eager.step(), cap.step())
+        _compare(k, eager, cap)
+        if k == 0:
+            # the arena is released first, so that the eager step below grows it whatever ran before
+            g0 = _hip.generation()
+            _hip.trim()
+            Ensemble([jittered_sphere_on_parabola(40000, radius=200.0, r_p=2500.0, seed=21)], cfg).step()
+            check("the scratch generation moved", True, _hip.generation() != g0)
+    check("recorded again after the scratch moved", True, cap.rerecords >= 1)
+    check("no replay fell back to an eager step", 0, cap.fallbacks)

@@ -70,6 +70,49 @@ def test_stream_crash_cooling_balsara():
     _compare(st, ost, cfg)
 
 
+def test_beta_cooling_step():
+    """The reference's in-step beta cooling (coupled.py:244-251): t_cool = beta r_mid^1.5 at the
+    mid-step radius, exact relaxation toward the 10^4 K floor, E_rad += sum m (u_half - u_cool).
+    The relaxation factor exp(-dt / t_cool) and r^-1.5 come from the device's libm, which may
+    differ from NumPy's in the last bit, so the state is compared at 1e-12 relative (the FP64
+    mode's bound) rather than bitwise; E_rad (a device sum against the host's pairwise sum) to
+    1e-12 of itself."""
+    from dataclasses import replace
+
+    from nebula.ics import jittered_sphere_on_parabola
+    from nebula.sim import coupled
+
+    ic = jittered_sphere_on_parabola(2048, r0=3000.0, r_p=500.0, seed=5)
+    ic = replace(ic, u=ic.u * 100.0)  # 10^6 K: well above the 10^4 K floor the cooling relaxes to
+    cfg = replace(coupled.baseline_config(), cooling_beta=10.0)
+    print("coupled: 2048-particle cloud, beta = 10 cooling, 4 steps", flush=True)
+    from nebula.sim import coupled as c2
+
+    from oracle import coupled as oc
+
+    st = c2.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg)
+    ost = oc.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg)
+    for _ in range(4):
+        dt = oc.stable_dt(ost, cfg)
+        check("stable_dt (1e-12 relative)", dt, c2.stable_dt(st, cfg), rtol=1e-12)
+        st = c2.step(st, cfg, dt)
+        ost = oc.step(ost, cfg, dt)
+    got = st.to_numpy()
+    for k in ("x", "p"):
+        check(f"beta cooling: {k} (1e-12 per vector)", ost.caller_order(k), got[k], rtol=1e-12, per_vector=True)
+    for k in ("u", "h", "rho", "vsig_max"):
+        check(f"beta cooling: {k} (1e-12 relative)", ost.caller_order(k), got[k], rtol=1e-12)
+    for k in ("a_hydro", "dudt"):  # sums that cancel: also 1e-12 of their rms
+        ref = ost.caller_order(k)
+        rms = float(np.sqrt(np.mean(ref**2) * (3 if ref.ndim > 1 else 1)))
+        check(f"beta cooling: {k} (1e-12 relative + 1e-12 rms)", ref, got[k], rtol=1e-12, atol=1e-12 * rms,
+              per_vector=ref.ndim > 1)  # fmt: skip
+    same = np.mean(ost.caller_order("u") == got["u"])
+    print(f"u bit-identical for {same:.4f} of the particles")
+    check("radiated energy (1e-12 relative)", ost.e_radiated, float(st.e_radiated), rtol=1e-12)
+    check("energy was radiated", True, ost.e_radiated > 0.0)
+
+
 def test_switch_conductivity_reconstruction():
     """The reference's default hole physics: Cullen-Dehnen switch, velocity-signal conductivity,
     velocity reconstruction (exp() in the closeness factor: compared at 1e-12)."""

@@ -165,3 +165,43 @@ def test_recorded_stepper_overflow_records_again():
     print("fallbacks:", rs.fallbacks, rs.fallback_reasons)
     check("the undersized recording fell back", True, rs.fallbacks >= 1)
     _compare("recorded stepper (with fallback) vs coupled.step", st, rs.state)
+
+
+def test_recorded_stepper_records_again_when_scratch_moves():
+    """Eager calls between replays that release or grow the library's scratch (nb_ctx_trim, the
+    reference's ledger pattern, a step of a larger cloud) move buffers the graph points at: the
+    next replay must be recorded again first, and the results stay coupled.step's."""
+    from nebula import _hip
+    from nebula.ics import stream_crash
+    from nebula.sim import coupled
+    from nebula.sim.stepper import RecordedStepper
+
+    ic = stream_crash(16384, length=200.0, seed=20261014)
+    big = stream_crash(65536, length=400.0, seed=3)
+    cfg = _config()
+    print("recorded stepper: ledger and a 65536-particle eager step between replays", flush=True)
+    st = coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg)
+    for _ in range(2):
+        st = coupled.step(st, cfg, coupled.stable_dt(st, cfg))
+    st_b = coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg)
+    for _ in range(2):
+        st_b = coupled.step(st_b, cfg, coupled.stable_dt(st_b, cfg))
+    rs = RecordedStepper(st_b, cfg)
+    dts, rdts = [], []
+    for k in range(6):
+        dt = coupled.stable_dt(st, cfg)
+        dts.append(dt)
+        st = coupled.step(st, cfg, dt)
+        rdts.append(rs.step())
+        if k == 1:
+            # the arena is released first, so that the eager calls below grow it whatever ran before
+            g0 = _hip.generation()
+            _hip.trim()
+            coupled.ledger(rs.state, cfg)
+            sb = coupled.initialise(big.x, big.v, big.u, big.mass, big.h, cfg)
+            coupled.step(sb, cfg, coupled.stable_dt(sb, cfg))
+            check("the scratch generation moved", True, _hip.generation() != g0)
+    print("re-recordings:", rs.rerecords, "fallbacks:", rs.fallbacks, rs.fallback_reasons)
+    check("the replay after the eager calls was recorded again", True, rs.rerecords >= 1)
+    bit_equal("time steps", np.array(dts), np.array(rdts))
+    _compare("recorded stepper (scratch moved) vs coupled.step", st, rs.state)

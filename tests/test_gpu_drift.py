@@ -4,13 +4,17 @@ then learning online from the run's own pairs).
 
 The workload is the plain-method cloud: a 4096-particle sphere of 0.1 M_sun and radius 150 on
 its parabolic orbit from r = 20000 with pericentre 2000, cubic spline, eta 1.2, Monaghan
-alpha 1, beta 2, at the default step factors. Over 1000 steps the ledger error of this run is set
-by the integrator's step (the reference's 1e-5 bound of tests/sim/test_coupled.py:44-88 holds at
-its halved and quartered step factors, not at these), so the FP64 GPU run, bit-identical to the
-FP64 oracle, sets the tolerance: the mixed run's maximum energy error may exceed the FP64 run's
-by at most 1e-3 of it plus 1e-8 of sum m|H|. Angular momentum: both runs within 1e-10 (the
-reference's bound). The learned start must also save Newton iterations over the whole run
-(plain: 1.6 per particle per step on this orbit).
+alpha 1, beta 2, at the default step factors (eta_orb 0.00125, Courant 0.3, force 0.25).
+
+The FP64 run is bit-identical to the FP64 oracle, whose largest energy error over these 1000
+steps is 1.405e-3 of sum m|H| (the cloud is on a parabolic orbit, H ~ 0, so sum m|H| is small).
+That error is the integrator's: with every step factor halved, over the same time, the oracle's
+is 3.509e-4 (4.0x smaller, second order). The test holds the FP64 run to 1.5e-3 and repeats the
+halving on the GPU (ratio 3-5, the reference's criterion of tests/sim/test_coupled.py:58); the
+reference's hot-blob bound itself (1e-5) is tested in test_gpu_ledger.py. The mixed run's
+maximum energy error may exceed the FP64 run's by at most 1e-3 of it plus 1e-8 of sum m|H|.
+Angular momentum: both runs within 1e-10 (the reference's bound). The learned start must also
+save Newton iterations over the whole run (plain: 1.6 per particle per step on this orbit).
 """
 
 import numpy as np
@@ -24,15 +28,18 @@ pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="n
 STEPS = 1000
 
 
-def _ledger_run(ic, cfg, steps):
+def _ledger_run(ic, cfg, steps, t_end=None):
     from nebula.sim import coupled
 
     st = coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg)
     l0 = coupled.ledger(st, cfg)
     e_max = l_max = 0.0
     iters = []
-    for _ in range(steps):
-        st = coupled.step(st, cfg, coupled.stable_dt(st, cfg))
+    k = 0
+    while (k < steps) if t_end is None else (st.t < t_end * (1 - 1e-14)):
+        dt = coupled.stable_dt(st, cfg)
+        st = coupled.step(st, cfg, dt if t_end is None else min(dt, t_end - st.t))
+        k += 1
         lg = coupled.ledger(st, cfg)
         e_max = max(e_max, abs(lg["energy"] - l0["energy"]) / l0["abs_h"])
         l_max = max(l_max, float(np.linalg.norm(lg["angmom"] - l0["angmom"]) / np.linalg.norm(l0["angmom"])))
@@ -69,7 +76,14 @@ def test_ledger_drift_1000_steps_mixed_learned_vs_fp64():
     print(f"max |dE| / sum m|H| over {STEPS} steps: FP64 {e64:.3g}, mixed+learned {eml:.3g}; "
           f"max |dL| / |L|: FP64 {l64:.3g}, mixed+learned {lml:.3g}; t_end {s64.t:.6g} / {sml.t:.6g}; "
           f"Newton iterations {it64:.3f} / {itml:.3f}")  # fmt: skip
-    check("FP64: energy error finite and below 1e-2 of sum m|H|", True, bool(e64 < 1e-2))
+    check("FP64: energy error (the oracle's 1.405e-3) below 1.5e-3 of sum m|H|", True, bool(e64 < 1.5e-3))
+    half = replace(cfg64, eta_orb=0.5 * cfg64.eta_orb,
+                   hydro=replace(cfg64.hydro, courant=0.5 * cfg64.hydro.courant, c_force=0.5 * cfg64.hydro.c_force))
+    print(f"drift: FP64 with every step factor halved, to t = {s64.t:.6g}", flush=True)
+    _, e_half, _, _ = _ledger_run(ic, half, 0, t_end=s64.t)
+    print(f"FP64 energy error, full / halved step factors: {e64:.4g} / {e_half:.4g} = {e64 / e_half:.3f}")
+    check("FP64: the energy error is the integrator's (falls 3-5x when the steps are halved)", 4.0, e64 / e_half,
+          atol=1.0)  # fmt: skip
     check("mixed+learned energy error within the FP64 run's (x 1.001 + 1e-8)", True, eml <= 1.001 * e64 + 1e-8)
     check("the learned start saves at least 0.3 Newton iterations per particle over the run", True, itml < it64 - 0.3)
     check("FP64: max angular-momentum error < 1e-10", True, l64 < 1e-10)

@@ -119,7 +119,11 @@ extern "C" nb_ctx* nb_ctx_create(int32_t device)
 {
     if (hipSetDevice(device) != hipSuccess) return nullptr;
     nb_ctx* ctx = new (std::nothrow) nb_ctx();
-    if (ctx) ctx->device = device;
+    if (!ctx) return nullptr;
+    ctx->device = device;
+    // the no-allocation entry point's device array; a failure only disables that entry point,
+    // which then reports it
+    if (static_scratch_init(device, &ctx->err)) ctx->err += " (nb_dt_minima_dev_static unavailable)";
     return ctx;
 }
 

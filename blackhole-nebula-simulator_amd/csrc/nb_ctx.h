@@ -111,6 +111,10 @@ struct nb_ctx {
     double last_cell = 0.0;
 };
 
+// Look up, once per device, the address of the fixed device array behind the no-allocation
+// entry point (step.hip); called by nb_ctx_create with that device current. Nonzero on failure.
+int static_scratch_init(int device, std::string* err);
+
 int32_t ctx_fail(nb_ctx* ctx, const char* msg);
 int32_t ctx_fail_code(nb_ctx* ctx, int32_t code, const char* msg);
 // grow *p to at least `bytes` (contents are not kept); nonzero on failure

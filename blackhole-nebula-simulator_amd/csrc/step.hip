@@ -5,6 +5,9 @@
 #include "nb_ctx.h"
 #include "nebula_hip.h"
 
+#include <mutex>
+#include <string>
+
 namespace nb {
 
 static constexpr int kBlockS = 256;
@@ -474,6 +477,62 @@ extern "C" int32_t nb_dt_minima_dev(nb_ctx* ctx, void* stream, const double* h, 
     hipLaunchKernelGGL(k_dt_partial, dim3(nparts), dim3(kBlockS), 0, s, h, vsig, acc, nu, c_visc, x, n, ctx->red);
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlockS), 0, s, ctx->red, nparts, 4, 0, out4_dev);
     return launch_check_s(ctx, "nb_dt_minima_dev");
+}
+
+// ------------------------------------------------------- nb_dt_minima_dev without allocation
+// For callers that cannot allocate (a stream being recorded into a hipGraph, a serving loop that
+// forbids hipMalloc): the partial minima live in `scratch`, a fixed device array of this file,
+// instead of the context's arena. k_dt_partial's grid is capped at kStaticParts workgroups, each
+// striding over the particles, so the array (kStaticParts + 1 rows of 4 doubles, 32.8 KB per
+// GPU, held until the process ends) fits every n. Every call on one GPU shares that GPU's
+// array: calls that overlap on one GPU (concurrent streams) race; calls in stream order do not.
+// The reduction reads only the rows its own launch wrote, so nothing needs clearing between
+// calls.
+static constexpr int kStaticParts = 1024;
+static constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[4 * (kStaticParts + 1)];
+static double* g_scratch_dev[kMaxDevices] = {};  // scratch's address on each device
+static std::mutex g_scratch_mu;
+
+// Called by nb_ctx_create with `device` current: looks the array's address up once per device.
+// In host code `scratch` names a host-side stand-in, never the device memory: kernels and copies
+// get the address found here.
+int static_scratch_init(int device, std::string* err)
+{
+    if (device < 0 || device >= kMaxDevices) {
+        if (err) *err = "static scratch: device index out of range";
+        return 1;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_dev[device]) return 0;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        if (err) *err = std::string("static scratch: hipGetSymbolAddress failed: ") + hipGetErrorString(e);
+        return 1;
+    }
+    g_scratch_dev[device] = (double*)ptr;
+    return 0;
+}
+
+extern "C" int32_t nb_dt_minima_dev_static(nb_ctx* ctx, void* stream, const double* h, const double* vsig,
+                                           const double* acc, const double* nu, double c_visc, const double* x,
+                                           int64_t n, double* out4_dev)
+{
+    if (!ctx || n <= 0 || !out4_dev || !h || !vsig || !acc || !x)
+        return ctx_fail(ctx, "nb_dt_minima_dev_static: bad arguments");
+    if (n > INT64_MAX / 3) return ctx_fail(ctx, "nb_dt_minima_dev_static: n beyond the largest (N,3) array indexable");
+    double* part = nullptr;
+    if (ctx->device >= 0 && ctx->device < kMaxDevices) {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        part = g_scratch_dev[ctx->device];
+    }
+    if (!part) return ctx_fail_code(ctx, NB_ERR_HIP, "nb_dt_minima_dev_static: no static scratch on this device");
+    const int nparts = (int)std::min<int64_t>(kStaticParts, (n + kBlockS - 1) / kBlockS);
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_dt_partial, dim3(nparts), dim3(kBlockS), 0, s, h, vsig, acc, nu, c_visc, x, n, part);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlockS), 0, s, part, nparts, 4, 0, out4_dev);
+    return launch_check_s(ctx, "nb_dt_minima_dev_static");
 }
 
 extern "C" int32_t nb_dt_minima_seg(nb_ctx* ctx, void* stream, const double* h, const double* vsig,

@@ -351,6 +351,16 @@ int32_t nb_dt_minima(nb_ctx *ctx, void *stream, const double *h, const double *v
  * with its other per-step words in one copy. */
 int32_t nb_dt_minima_dev(nb_ctx *ctx, void *stream, const double *h, const double *vsig, const double *acc,
                          const double *nu, double c_visc, const double *x, int64_t n, double *out4_dev);
+/* nb_dt_minima_dev that allocates nothing. No counterpart in the reference: it is for callers
+ * that cannot allocate (code under hipGraph capture, serving loops that forbid hipMalloc). Its
+ * partial minima live in a fixed device array of the library (1025 x 4 doubles = 32.8 KB per
+ * GPU, held until the process ends) instead of the context's scratch arena; a launch writes at
+ * most 1024 partial rows whatever n, so every n >= 1 fits (n above INT64_MAX / 3 is refused).
+ * Same results as nb_dt_minima_dev, bit for bit. Every call on one GPU shares its array: calls
+ * that overlap on one GPU (concurrent streams) race; calls in stream order do not. */
+int32_t nb_dt_minima_dev_static(nb_ctx *ctx, void *stream, const double *h, const double *vsig,
+                                const double *acc, const double *nu, double c_visc, const double *x, int64_t n,
+                                double *out4_dev);
 
 /* Ensemble helpers (no counterpart in the reference; the systems of nb_grid_order_seg):
  * nb_dt_minima_seg: nb_dt_minima per system, out (host) nseg x 4; nb_axpy_rows: out row i = y row i

@@ -572,9 +572,11 @@ __global__ void k_merge(const uint32_t* __restrict__ gbuf, int64_t cap, const in
 }
 
 // ----------------------------------------------------------------- tighten
-// neighbours.py:284-303: keep j with d2 < (2 max(h_i, h_j))^2 (positions as given, minimum image).
-// CSR in -> CSR out (count pass, then fill pass).
-template <bool kFill>
+// neighbours.py:284-303: keep j with d2 < (2 max(h_i, h_j))^2 (positions as given, minimum image);
+// kIncl keeps d2 <= (2 max(h_i, h_j))^2, the pair set of the reference's cKDTree twin
+// (neighbours.py:53-71: query_ball_point includes the boundary). CSR in -> CSR out (count pass,
+// then fill pass).
+template <bool kFill, bool kIncl>
 __global__ void k_tighten_csr(const double* __restrict__ pos, const double* __restrict__ h, Box box,
                               const int64_t* __restrict__ offset, const int32_t* __restrict__ index,
                               const int64_t* __restrict__ rows, int64_t nrows, unsigned long long* __restrict__ counts,
@@ -597,7 +599,7 @@ __global__ void k_tighten_csr(const double* __restrict__ pos, const double* __re
         d2 += dy * dy;
         d2 += dz * dz;
         const double rr = NB_R_KERN * pymax(hi, h[j]);
-        if (d2 < rr * rr) {
+        if (kIncl ? d2 <= rr * rr : d2 < rr * rr) {
             if (kFill) out_idx[o + m] = j;
             m++;
         }
@@ -2526,10 +2528,10 @@ extern "C" int32_t nb_lists_stats(const nb_ctx* ctx, int64_t* out4)
     return NB_OK;
 }
 
-extern "C" int32_t nb_tighten(nb_ctx* ctx, void* stream, const double* pos, const double* h, int64_t n,
-                              const double* box3, const int64_t* offset, const int32_t* index, const int64_t* rows,
-                              int64_t nrows, int64_t* out_offset, int32_t* out_index, int64_t capacity,
-                              int64_t* total_out)
+template <bool kIncl>
+static int32_t tighten_csr(nb_ctx* ctx, void* stream, const double* pos, const double* h, int64_t n,
+                           const double* box3, const int64_t* offset, const int32_t* index, const int64_t* rows,
+                           int64_t nrows, int64_t* out_offset, int32_t* out_index, int64_t capacity, int64_t* total_out)
 {
     hipStream_t s = (hipStream_t)stream;
     if (!ctx || n < 0 || nrows < 0 || !total_out) return ctx_fail(ctx, "nb_tighten: bad arguments");
@@ -2538,8 +2540,8 @@ extern "C" int32_t nb_tighten(nb_ctx* ctx, void* stream, const double* pos, cons
         return NB_ERR_HIP;
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, ctx->tcount, n + 1, 0ull);
     if (nrows > 0)
-        hipLaunchKernelGGL(k_tighten_csr<false>, dim3(grid_of(nrows)), dim3(kBlock), 0, s, pos, h, box, offset, index,
-                           rows, nrows, ctx->tcount, nullptr, nullptr);
+        hipLaunchKernelGGL((k_tighten_csr<false, kIncl>), dim3(grid_of(nrows)), dim3(kBlock), 0, s, pos, h, box, offset,
+                           index, rows, nrows, ctx->tcount, nullptr, nullptr);
     int rc = scan_u64(ctx, s, ctx->tcount, (unsigned long long*)out_offset, n + 1);
     if (rc) return rc;
     int64_t total = 0;
@@ -2548,10 +2550,28 @@ extern "C" int32_t nb_tighten(nb_ctx* ctx, void* stream, const double* pos, cons
     *total_out = total;
     if (total > capacity) return NB_RETRY_CAPACITY;
     if (nrows > 0)
-        hipLaunchKernelGGL(k_tighten_csr<true>, dim3(grid_of(nrows)), dim3(kBlock), 0, s, pos, h, box, offset, index,
-                           rows, nrows, nullptr, (const unsigned long long*)out_offset, out_index);
+        hipLaunchKernelGGL((k_tighten_csr<true, kIncl>), dim3(grid_of(nrows)), dim3(kBlock), 0, s, pos, h, box, offset,
+                           index, rows, nrows, nullptr, (const unsigned long long*)out_offset, out_index);
     if (hipGetLastError() != hipSuccess) return ctx_fail(ctx, "nb_tighten: kernel launch failed");
     return NB_OK;
+}
+
+extern "C" int32_t nb_tighten(nb_ctx* ctx, void* stream, const double* pos, const double* h, int64_t n,
+                              const double* box3, const int64_t* offset, const int32_t* index, const int64_t* rows,
+                              int64_t nrows, int64_t* out_offset, int32_t* out_index, int64_t capacity,
+                              int64_t* total_out)
+{
+    return tighten_csr<false>(ctx, stream, pos, h, n, box3, offset, index, rows, nrows, out_offset, out_index,
+                              capacity, total_out);
+}
+
+extern "C" int32_t nb_tighten_inclusive(nb_ctx* ctx, void* stream, const double* pos, const double* h, int64_t n,
+                                        const double* box3, const int64_t* offset, const int32_t* index,
+                                        const int64_t* rows, int64_t nrows, int64_t* out_offset, int32_t* out_index,
+                                        int64_t capacity, int64_t* total_out)
+{
+    return tighten_csr<true>(ctx, stream, pos, h, n, box3, offset, index, rows, nrows, out_offset, out_index,
+                             capacity, total_out);
 }
 
 extern "C" int32_t nb_tighten_blocked(nb_ctx* ctx, void* stream, const double* pos, const double* h, int64_t n,

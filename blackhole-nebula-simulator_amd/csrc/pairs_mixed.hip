@@ -4,10 +4,13 @@
 // A group of 8 lanes serves one row: lane l takes entries l, l + 8, ... so that the group
 // reads a row-major or CSR row contiguously. Each lane accumulates its pairs in FP32 with
 // compensated sums (Kahan's; TwoSum for the pair forces, whose terms must cancel exactly between
-// the two rows of a pair) and the group combines the lane partials in FP64. Separations are
-// formed in FP64 (positions stay FP64: at r ~ 2000 an FP32 position resolves only ~1e-4 of a
-// unit length) and then rounded to FP32; every other per-neighbour quantity comes from a packed
-// FP32 record prepared once per pass (48 bytes for forces, 32 for gradients: whole 16-byte loads).
+// the two rows of a pair) and the group combines the lane partials in FP64. Pair differences of
+// the FP64 state, separations and velocity differences, are formed in FP64 and then rounded to
+// FP32: positions and velocities carry the bulk orbital motion (r ~ 2000, v ~ 0.03 c), against
+// which an FP32 position resolves only ~1e-4 of a unit length and an FP32 velocity ~2e-9 c, as
+// large as the velocity differences inside a cloud moving as one (its du/dt and viscosity switch
+// then came out at percent-level error). Every other per-neighbour quantity comes from a packed
+// FP32 record prepared once per pass (32 bytes for forces, 16 for gradients: whole 16-byte loads).
 #include <string>
 
 #include "nb_common.h"
@@ -28,9 +31,8 @@ static constexpr int kDivCurlW = 4;
 static constexpr int kBlockM = 256;
 
 struct __align__(16) PF {
-    float4 a;  // vx, vy, vz, 1/h
-    float4 b;  // cnorm / h^4, P / (Omega rho^2), 1 / (Omega rho), alpha c_s
-    float4 c;  // m s (s = mass_scale), u, P, rho
+    float4 a;  // 1/h, P / (Omega rho^2), 1 / (Omega rho), alpha c_s
+    float4 b;  // m s (s = mass_scale), u, P, rho
 };
 
 // The force pass weighs each pair by the product of both FP32 masses, (m_a s)(m_b s): that
@@ -49,11 +51,10 @@ __device__ __forceinline__ double mass_scale(const double* __restrict__ mass)
 }
 
 struct __align__(16) PG {
-    float4 a;  // vx, vy, vz, 1/h
-    float4 b;  // cnorm / h^4, m, 0, 0
+    float4 a;  // 1/h, cnorm / h^4, m, 0
 };
 
-__global__ void k_pack_forces(const double* __restrict__ vel, const double* __restrict__ mass,
+__global__ void k_pack_forces(const double* __restrict__ mass,
                               const double* __restrict__ h, const double* __restrict__ rho,
                               const double* __restrict__ omega, const double* __restrict__ pressure,
                               const double* __restrict__ u, const double* __restrict__ c_s,
@@ -63,22 +64,20 @@ __global__ void k_pack_forces(const double* __restrict__ vel, const double* __re
     if (i >= n) return;
     const double hi = h[i], r = rho[i], om = omega[i], p = pressure[i], ms = mass_scale(mass);
     PF v;
-    v.a = make_float4((float)vel[3 * i], (float)vel[3 * i + 1], (float)vel[3 * i + 2], (float)(1.0 / hi));
-    v.b = make_float4((float)(cnorm / ipow4(hi)), (float)(p / (om * r * r)), (float)(1.0 / (om * r)),
+    v.a = make_float4((float)(1.0 / hi), (float)(p / (om * r * r)), (float)(1.0 / (om * r)),
                       (float)(alpha ? alpha[i] * c_s[i] : 0.0));
-    v.c = make_float4((float)(mass[i] * ms), (float)(u ? u[i] : 0.0), (float)p, (float)r);
+    v.b = make_float4((float)(mass[i] * ms), (float)(u ? u[i] : 0.0), (float)p, (float)r);
     out[i] = v;
 }
 
-__global__ void k_pack_grad(const double* __restrict__ vel, const double* __restrict__ mass,
-                            const double* __restrict__ h, double cnorm, int64_t n, PG* __restrict__ out)
+__global__ void k_pack_grad(const double* __restrict__ mass, const double* __restrict__ h, double cnorm, int64_t n,
+                            PG* __restrict__ out)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double hi = h[i];
     PG v;
-    v.a = make_float4((float)vel[3 * i], (float)vel[3 * i + 1], (float)vel[3 * i + 2], (float)(1.0 / hi));
-    v.b = make_float4((float)(cnorm / ipow4(hi)), (float)mass[i], 0.f, 0.f);
+    v.a = make_float4((float)(1.0 / hi), (float)(cnorm / ipow4(hi)), (float)mass[i], 0.f);
     out[i] = v;
 }
 
@@ -158,17 +157,18 @@ __device__ __forceinline__ void sep32(const double* pos, double pa0, double pa1,
     d2 = (float)min_image(pa2 - pos[3 * b + 2], box.l[2]);
 }
 
-// the neighbour's position and packed record, loaded before the pair's arithmetic
+// the neighbour's position, velocity and packed record, loaded before the pair's arithmetic
 template <class R>
 struct Nb {
     R rec;
-    double x0, x1, x2;
+    double x0, x1, x2, v0, v1, v2;
 };
 
 template <class R>
-__device__ __forceinline__ Nb<R> load_nb(const double* __restrict__ pos, const R* __restrict__ recs, int64_t b)
+__device__ __forceinline__ Nb<R> load_nb(const double* __restrict__ pos, const double* __restrict__ vel,
+                                         const R* __restrict__ recs, int64_t b)
 {
-    return Nb<R>{recs[b], pos[3 * b], pos[3 * b + 1], pos[3 * b + 2]};
+    return Nb<R>{recs[b], pos[3 * b], pos[3 * b + 1], pos[3 * b + 2], vel[3 * b], vel[3 * b + 1], vel[3 * b + 2]};
 }
 
 // sph.py:525-652 without reconstruction: pressure force with grad-h terms, Monaghan viscosity on
@@ -177,8 +177,9 @@ __device__ __forceinline__ Nb<R> load_nb(const double* __restrict__ pos, const R
 // the energy sums carry the neighbour's scaled mass and are divided by s (exact).
 template <class L, int W>
 __global__ void __launch_bounds__(kBlockM)
-k_forces_mixed(const double* __restrict__ pos, const double* __restrict__ mass, const PF* __restrict__ pf,
-               const double* __restrict__ c_s, L lists, Box box, const int64_t* __restrict__ rows, int64_t nrows,
+k_forces_mixed(const double* __restrict__ pos, const double* __restrict__ vel, const double* __restrict__ mass,
+               const PF* __restrict__ pf, const double* __restrict__ c_s, L lists, Box box,
+               const int64_t* __restrict__ rows, int64_t nrows,
                nb_force_params P, double* __restrict__ acc, double* __restrict__ dudt, double* __restrict__ vsig_max,
                double* __restrict__ vheat, double* __restrict__ cheat)
 {
@@ -187,9 +188,12 @@ k_forces_mixed(const double* __restrict__ pos, const double* __restrict__ mass, 
     if (q >= nrows) return;  // whole groups leave together
     const int64_t a = row_of(rows, q);
     const double pa0 = pos[3 * a], pa1 = pos[3 * a + 1], pa2 = pos[3 * a + 2];
+    const double va0 = vel[3 * a], va1 = vel[3 * a + 1], va2 = vel[3 * a + 2];
     const PF A = pf[a];
     // a massless row particle keeps the neighbour-mass weighting (its own mass would zero the sum)
-    const float wa = A.c.x > 0.f ? A.c.x : 1.f;
+    const float wa = A.b.x > 0.f ? A.b.x : 1.f;
+    const float cnorm = (float)P.cnorm;
+    const float iha2 = A.a.x * A.a.x, cn4a = cnorm * (iha2 * iha2);  // cnorm / h_a^4
     const float beta = (float)P.beta, alpha_u = (float)P.alpha_u;
     const bool use_visc = P.use_visc, use_cond = P.use_cond;
     const int kid = P.kid;
@@ -208,32 +212,34 @@ k_forces_mixed(const double* __restrict__ pos, const double* __restrict__ mass, 
         const float inv_r = rsqrt32(r2);
         const float r = r2 * inv_r;
         const float rx = dx0 * inv_r, ry = dx1 * inv_r, rz = dx2 * inv_r;
-        const float fa = A.b.x * df32(r * A.a.w, kid);
-        const float fb = B.b.x * df32(r * B.a.w, kid);
-        const float vdotr = (A.a.x - B.a.x) * rx + (A.a.y - B.a.y) * ry + (A.a.z - B.a.z) * rz;
-        float na = A.b.y, nbb = B.b.y;  // (P - rho v_sig v.r / 2) / (Omega rho^2) of either side
+        const float ihb2 = B.a.x * B.a.x;
+        const float fa = cn4a * df32(r * A.a.x, kid);
+        const float fb = (cnorm * (ihb2 * ihb2)) * df32(r * B.a.x, kid);
+        const float dv0 = (float)(va0 - N.v0), dv1 = (float)(va1 - N.v1), dv2 = (float)(va2 - N.v2);
+        const float vdotr = dv0 * rx + dv1 * ry + dv2 * rz;
+        float na = A.a.y, nbb = B.a.y;  // (P - rho v_sig v.r / 2) / (Omega rho^2) of either side
         if (use_visc) {
             const float bv = beta * fabsf(vdotr);
-            const float vsa = A.b.w + bv;
+            const float vsa = A.a.w + bv;
             vmax = fmaxf(vmax, vsa);
             if (vdotr < 0.f) {
-                const float vsb = B.b.w + bv;
-                na += -0.5f * vsa * vdotr * A.b.z;
-                nbb += -0.5f * vsb * vdotr * B.b.z;
-                visc.add(-(B.c.x * vsa * 0.5f * vdotr * vdotr * fa * A.b.z));
+                const float vsb = B.a.w + bv;
+                na += -0.5f * vsa * vdotr * A.a.z;
+                nbb += -0.5f * vsb * vdotr * B.a.z;
+                visc.add(-(B.b.x * vsa * 0.5f * vdotr * vdotr * fa * A.a.z));
             }
         }
         if (use_cond) {
             const float vu =
-                P.cond_signal == 0 ? sqrt32(fabsf(A.c.z - B.c.z) * rcp32(0.5f * (A.c.w + B.c.w))) : fabsf(vdotr);
-            cond.add(B.c.x * alpha_u * vu * (A.c.y - B.c.y) * 0.5f * (fa * A.b.z + fb * B.b.z));
+                P.cond_signal == 0 ? sqrt32(fabsf(A.b.z - B.b.z) * rcp32(0.5f * (A.b.w + B.b.w))) : fabsf(vdotr);
+            cond.add(B.b.x * alpha_u * vu * (A.b.y - B.b.y) * 0.5f * (fa * A.a.z + fb * B.a.z));
         }
         // The pair force -f x_ab / r, split as hi + lo: hi = fl(g d) with g = f / r and d the FP32
         // separation; lo = (g d - hi) + g (x_ab - d), the product's rounding error (exact, by
         // FMA) and the separation's. Both are exactly opposite in row b, and hi + lo lies along
         // the FP64 separation to O(u^2), so the pair exerts no torque beyond that (hi alone
         // would tilt each force by u and leave a torque of u |x_ab| |F|).
-        const float f = (wa * B.c.x) * (na * fa + nbb * fb);
+        const float f = (wa * B.b.x) * (na * fa + nbb * fb);
         const float g = f * inv_r;
         const float h0 = g * dx0, h1 = g * dx1, h2 = g * dx2;
         ax.add(-h0);
@@ -242,14 +248,14 @@ k_forces_mixed(const double* __restrict__ pos, const double* __restrict__ mass, 
         lx -= __builtin_fmaf(g, (float)(D0 - (double)dx0), __builtin_fmaf(g, dx0, -h0));
         ly -= __builtin_fmaf(g, (float)(D1 - (double)dx1), __builtin_fmaf(g, dx1, -h1));
         lz -= __builtin_fmaf(g, (float)(D2 - (double)dx2), __builtin_fmaf(g, dx2, -h2));
-        work.add_prod(B.c.x * vdotr, fa);
+        work.add_prod(B.b.x * vdotr, fa);
     };
     // two entries per pass, both neighbours' records loaded before either pair's arithmetic: the
     // same pairs in the same order per lane as one entry at a time (self skipped)
     for (int64_t k = glane; k < m; k += 2 * W) {
         const int32_t b0 = lists.at(a, k);
         const int32_t b1 = k + W < m ? lists.at(a, k + W) : (int32_t)a;
-        const Nb<PF> N0 = load_nb(pos, pf, b0), N1 = load_nb(pos, pf, b1);
+        const Nb<PF> N0 = load_nb(pos, vel, pf, b0), N1 = load_nb(pos, vel, pf, b1);
         if (b0 != a) pair(N0);
         if (b1 != a) pair(N1);
     }
@@ -258,11 +264,11 @@ k_forces_mixed(const double* __restrict__ pos, const double* __restrict__ mass, 
     const float vm = gmax<W>(vmax);
     if (glane == 0) {
         const double ms = mass_scale(mass);
-        const double da = A.c.x > 0.f ? (ms * ms) * mass[a] : ms;  // m_a s^2: acc = sum / (m_a s^2)
+        const double da = A.b.x > 0.f ? (ms * ms) * mass[a] : ms;  // m_a s^2: acc = sum / (m_a s^2)
         acc[3 * q] = sx / da;
         acc[3 * q + 1] = sy / da;
         acc[3 * q + 2] = sz / da;
-        dudt[q] = (double)A.b.y * (sw / ms) + sv / ms + sc / ms;
+        dudt[q] = (double)A.a.y * (sw / ms) + sv / ms + sc / ms;
         vsig_max[q] = use_visc ? fmax(c_s[a], (double)vm) : 0.0;
         if (vheat) vheat[q] = sv / ms;
         if (cheat) cheat[q] = sc / ms;
@@ -272,7 +278,8 @@ k_forces_mixed(const double* __restrict__ pos, const double* __restrict__ mass, 
 // sph.py:837-884: R, G_v and G_a of a row from FP32 pair terms, inverted in FP64 (kAcc false:
 // accelerations zero, G_a not summed).
 template <class L, bool kAcc, int W>
-__device__ __forceinline__ void gradients_mixed_row(const double* __restrict__ pos, const double* __restrict__ accel,
+__device__ __forceinline__ void gradients_mixed_row(const double* __restrict__ pos, const double* __restrict__ vel,
+                                                    const double* __restrict__ accel,
                                                     const PG* __restrict__ pg, const L& lists, const Box& box,
                                                     const int64_t* __restrict__ rows, int64_t nrows, int kid,
                                                     double* __restrict__ div, double* __restrict__ curl,
@@ -284,10 +291,11 @@ __device__ __forceinline__ void gradients_mixed_row(const double* __restrict__ p
     const int64_t a = row_of(rows, q);
     const double pa0 = pos[3 * a], pa1 = pos[3 * a + 1], pa2 = pos[3 * a + 2];
     const PG A = pg[a];
+    const double va0 = vel[3 * a], va1 = vel[3 * a + 1], va2 = vel[3 * a + 2];
     Kahan R[6], Gv[9], Ga[kAcc ? 9 : 1];  // R is symmetric: its upper triangle (00 01 02 11 12 22)
-    float aa[3] = {0.f, 0.f, 0.f};
+    double aa[3] = {0.0, 0.0, 0.0};
     if (kAcc)
-        for (int c = 0; c < 3; c++) aa[c] = (float)accel[3 * a + c];
+        for (int c = 0; c < 3; c++) aa[c] = accel[3 * a + c];
     int cnt = 0;
     const int64_t m = lists.count(a);
     auto pair = [&](const Nb<PG>& N, int32_t b) {
@@ -299,12 +307,12 @@ __device__ __forceinline__ void gradients_mixed_row(const double* __restrict__ p
         dx[2] = (float)min_image(pa2 - N.x2, box.l[2]);
         const float r2 = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
         const float inv_r = rsqrt32(r2);
-        const float g = A.b.x * df32(r2 * inv_r * A.a.w, kid) * inv_r;
-        const float dv[3] = {B.a.x - A.a.x, B.a.y - A.a.y, B.a.z - A.a.z};
+        const float g = A.a.y * df32(r2 * inv_r * A.a.x, kid) * inv_r;
+        const float dv[3] = {(float)(N.v0 - va0), (float)(N.v1 - va1), (float)(N.v2 - va2)};
         float da[3] = {0.f, 0.f, 0.f};
         if (kAcc)
-            for (int c = 0; c < 3; c++) da[c] = (float)accel[3 * b + c] - aa[c];
-        const float mg = B.b.y * g;
+            for (int c = 0; c < 3; c++) da[c] = (float)(accel[3 * b + c] - aa[c]);
+        const float mg = B.a.z * g;
         const float gx[3] = {mg * dx[0], mg * dx[1], mg * dx[2]};
         R[0].add_prod(-dx[0], gx[0]);
         R[1].add_prod(-dx[0], gx[1]);
@@ -324,7 +332,7 @@ __device__ __forceinline__ void gradients_mixed_row(const double* __restrict__ p
     for (int64_t k = glane; k < m; k += 2 * W) {
         const int32_t b0 = lists.at(a, k);
         const int32_t b1 = k + W < m ? lists.at(a, k + W) : (int32_t)a;
-        const Nb<PG> N0 = load_nb(pos, pg, b0), N1 = load_nb(pos, pg, b1);
+        const Nb<PG> N0 = load_nb(pos, vel, pg, b0), N1 = load_nb(pos, vel, pg, b1);
         if (b0 != a) pair(N0, b0);
         if (b1 != a) pair(N1, b1);
     }
@@ -368,20 +376,20 @@ __device__ __forceinline__ void gradients_mixed_row(const double* __restrict__ p
 
 template <class L, int W>
 __global__ void __launch_bounds__(kBlockM)
-k_gradients_mixed(const double* __restrict__ pos, const double* __restrict__ accel, const PG* __restrict__ pg, L lists,
-                  Box box, const int64_t* __restrict__ rows, int64_t nrows, int kid, double* __restrict__ div,
-                  double* __restrict__ curl, double* __restrict__ ddiv)
+k_gradients_mixed(const double* __restrict__ pos, const double* __restrict__ vel, const double* __restrict__ accel,
+                  const PG* __restrict__ pg, L lists, Box box, const int64_t* __restrict__ rows, int64_t nrows, int kid,
+                  double* __restrict__ div, double* __restrict__ curl, double* __restrict__ ddiv)
 {
-    gradients_mixed_row<L, true, W>(pos, accel, pg, lists, box, rows, nrows, kid, div, curl, ddiv);
+    gradients_mixed_row<L, true, W>(pos, vel, accel, pg, lists, box, rows, nrows, kid, div, curl, ddiv);
 }
 
 template <class L, int W>
 __global__ void __launch_bounds__(kBlockM)
-k_divcurl_mixed(const double* __restrict__ pos, const double* __restrict__ accel, const PG* __restrict__ pg, L lists,
-                Box box, const int64_t* __restrict__ rows, int64_t nrows, int kid, double* __restrict__ div,
-                double* __restrict__ curl, double* __restrict__ ddiv)
+k_divcurl_mixed(const double* __restrict__ pos, const double* __restrict__ vel, const double* __restrict__ accel,
+                const PG* __restrict__ pg, L lists, Box box, const int64_t* __restrict__ rows, int64_t nrows, int kid,
+                double* __restrict__ div, double* __restrict__ curl, double* __restrict__ ddiv)
 {
-    gradients_mixed_row<L, false, W>(pos, accel, pg, lists, box, rows, nrows, kid, div, curl, ddiv);
+    gradients_mixed_row<L, false, W>(pos, vel, accel, pg, lists, box, rows, nrows, kid, div, curl, ddiv);
 }
 
 static inline unsigned grid_m(int64_t nrows, int tile, int w) { return (unsigned)((nrows * w + tile - 1) / tile); }
@@ -434,10 +442,10 @@ int32_t forces_mixed(nb_ctx* ctx, hipStream_t s, const double* pos, const double
 {
     if (ctx_reserve(ctx, (void**)&ctx->pack, &ctx->pack_bytes, sizeof(PF) * (size_t)n)) return NB_ERR_HIP;
     PF* pf = (PF*)ctx->pack;
-    hipLaunchKernelGGL(k_pack_forces, dim3(grid_1(n)), dim3(kBlockM), 0, s, vel, mass, h, rho, omega, pressure,
+    hipLaunchKernelGGL(k_pack_forces, dim3(grid_1(n)), dim3(kBlockM), 0, s, mass, h, rho, omega, pressure,
                        P->use_cond ? u : nullptr, c_s, P->use_visc ? alpha : nullptr, P->cnorm, n, pf);
-    NB_DISPATCH_M(lists, k_forces_mixed, kPairW, nrows, s, pos, mass, pf, c_s, v_, box_of(box3), rows, nrows, *P, acc,
-                  dudt, vsig_max, vheat, cheat);
+    NB_DISPATCH_M(lists, k_forces_mixed, kPairW, nrows, s, pos, vel, mass, pf, c_s, v_, box_of(box3), rows, nrows, *P,
+                  acc, dudt, vsig_max, vheat, cheat);
     return launched(ctx, "nb_forces (mixed)");
 }
 
@@ -458,12 +466,12 @@ extern "C" int32_t nb_gradients_mixed(nb_ctx* ctx, void* stream, const double* p
     if (ctx_reserve(ctx, (void**)&ctx->pack, &ctx->pack_bytes, sizeof(PG) * (size_t)n)) return NB_ERR_HIP;
     hipStream_t s = (hipStream_t)stream;
     PG* pg = (PG*)ctx->pack;
-    hipLaunchKernelGGL(k_pack_grad, dim3(grid_1(n)), dim3(kBlockM), 0, s, vel, mass, h, cnorm, n, pg);
+    hipLaunchKernelGGL(k_pack_grad, dim3(grid_1(n)), dim3(kBlockM), 0, s, mass, h, cnorm, n, pg);
     if (acc)
-        NB_DISPATCH_M(lists, k_gradients_mixed, kPairW, nrows, s, pos, acc, pg, v_, box_of(box3), rows, nrows, kid,
-                      div, curl, ddiv);
+        NB_DISPATCH_M(lists, k_gradients_mixed, kPairW, nrows, s, pos, vel, acc, pg, v_, box_of(box3), rows, nrows,
+                      kid, div, curl, ddiv);
     else
-        NB_DISPATCH_M(lists, k_divcurl_mixed, kDivCurlW, nrows, s, pos, acc, pg, v_, box_of(box3), rows, nrows, kid,
-                      div, curl, ddiv);
+        NB_DISPATCH_M(lists, k_divcurl_mixed, kDivCurlW, nrows, s, pos, vel, acc, pg, v_, box_of(box3), rows, nrows,
+                      kid, div, curl, ddiv);
     return launched(ctx, "nb_gradients_mixed");
 }

@@ -76,6 +76,7 @@ SIGNATURES = {
     "nb_lists_build": (_i32, [_vp, _vp, _vp, _vp, _i64, _PD, _vp, _vp, _i64, _P64]),
     "nb_lists_emit": (_i32, [_vp, _vp, _vp, _i64]),
     "nb_tighten": (_i32, [_vp, _vp, _vp, _vp, _i64, _PD, _vp, _vp, _vp, _i64, _vp, _vp, _i64, _P64]),
+    "nb_tighten_inclusive": (_i32, [_vp, _vp, _vp, _vp, _i64, _PD, _vp, _vp, _vp, _i64, _vp, _vp, _i64, _P64]),
     "nb_density": (_i32, [_vp, _vp, _vp, _vp, _f64, _vp, _vp, _PL, _PD, _vp, _i64, _i32, _f64, _f64, _f64, _f64, _vp, _vp, _vp, _vp, _vp]),
     "nb_forces": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _PL, _PD, _vp, _i64, ctypes.POINTER(ForceParams), _vp, _vp, _vp, _vp, _vp]),
     "nb_gradients": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _PL, _PD, _vp, _i64, _i32, _f64, _vp, _vp, _vp]),

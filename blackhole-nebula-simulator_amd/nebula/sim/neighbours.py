@@ -42,7 +42,8 @@ class Neighbours:
     @property
     def index(self) -> np.ndarray:
         if self._index is None:
-            self._index = _dev.host(self._d_index)
+            # the reference's index array is int64 (neighbours.py:22-36); the device keeps int32
+            self._index = _dev.host(self._d_index).astype(np.int64)
         return self._index
 
     @property
@@ -318,13 +319,31 @@ def find_neighbours_numba(pos, h, box=None) -> Neighbours:
     return build_lists(_dev.f64(pos), _dev.f64(h), box_arr)
 
 
-# the reference's cKDTree twin gives identical lists (neighbours.py:53-71)
-find_neighbours = find_neighbours_numba
+# search lengths of find_neighbours' first pass: a few ulps above h, so that every pair at
+# exactly 2 max(h_i, h_j) is inside the strict search
+_INCLUSIVE_MARGIN = 1.0 + 8.0 * np.finfo(np.float64).eps
 
 
-def tighten(nb: Neighbours, pos, h, rows=None) -> Neighbours:
+def find_neighbours(pos, h, box=None, workers: int = -1) -> Neighbours:
+    """The reference's cKDTree twin (neighbours.py:53-71): symmetric lists of every j with
+    |x_i - x_j| <= 2 max(h_i, h_j), the boundary included as query_ball_point includes it
+    (find_neighbours_numba's grid search is strict). Lists are built at search lengths a few ulps
+    above h, then tightened with the inclusive test at h. ``workers`` is accepted for signature
+    compatibility (the GPU decides its own parallelism)."""
+    del workers
+    box_arr = _box3(box)
+    h_np = h.detach().cpu().numpy() if isinstance(h, torch.Tensor) else np.asarray(h, np.float64)
+    if box_arr is not None and len(h_np) and np.any(R_KERN * h_np.max() >= 0.5 * box_arr):
+        raise ValueError("kernel support exceeds half the periodic box: neighbours ambiguous")
+    pos_d, h_d = _dev.f64(pos), _dev.f64(h)
+    wide = build_lists(pos_d, h_d * _INCLUSIVE_MARGIN, box_arr)
+    return tighten(wide, pos_d, h_d, inclusive=True)
+
+
+def tighten(nb: Neighbours, pos, h, rows=None, inclusive: bool = False) -> Neighbours:
     """Keep only the pairs with r < 2 max(h_i, h_j) (neighbours.py:306-323). ``rows``: lists
-    only for these particles, the others' left empty."""
+    only for these particles, the others' left empty. ``inclusive`` keeps r <= 2 max(h_i, h_j)
+    (find_neighbours' pair set)."""
     pos_d = _dev.f64(pos)
     h_d = _dev.f64(h)
     n = int(pos_d.shape[0])
@@ -333,7 +352,8 @@ def tighten(nb: Neighbours, pos, h, rows=None) -> Neighbours:
     off = _dev.empty(n + 1, torch.int64)
     idx = _dev.empty(cap, torch.int32)
     total = np.zeros(1, np.int64)
-    _hip.call("nb_tighten", _hip.ptr(pos_d), _hip.ptr(h_d), n, _hip.box_arg(nb.box), _hip.ptr(nb.d_offset),
+    _hip.call("nb_tighten_inclusive" if inclusive else "nb_tighten", _hip.ptr(pos_d), _hip.ptr(h_d), n,
+              _hip.box_arg(nb.box), _hip.ptr(nb.d_offset),
               _hip.ptr(nb.d_index), _hip.ptr(rows_d), nrows, _hip.ptr(off), _hip.ptr(idx), cap,
               total.ctypes.data_as(_hip._P64))  # fmt: skip
     return Neighbours.from_device(off, idx[: int(total[0])], nb.box)

@@ -149,6 +149,13 @@ int32_t nb_tighten(nb_ctx *ctx, void *stream, const double *pos, const double *h
                    const double *box3, const int64_t *offset, const int32_t *index, const int64_t *rows,
                    int64_t nrows, int64_t *out_offset, int32_t *out_index, int64_t capacity,
                    int64_t *total_out);
+/* nb_tighten with the boundary included (|x_i - x_j| <= 2 max(h_i, h_j)): the pair set of the
+ * reference's cKDTree twin find_neighbours (neighbours.py:53-71; query_ball_point is inclusive),
+ * taken from lists built at a slightly larger search length. */
+int32_t nb_tighten_inclusive(nb_ctx *ctx, void *stream, const double *pos, const double *h, int64_t n,
+                             const double *box3, const int64_t *offset, const int32_t *index, const int64_t *rows,
+                             int64_t nrows, int64_t *out_offset, int32_t *out_index, int64_t capacity,
+                             int64_t *total_out);
 
 /* tighten of blocked rows (all particles): one pass into blocked rows of the same capacity;
  * *total_out = entries kept. */

@@ -61,6 +61,49 @@ def test_lists_bit_identical(rng, case):
     bit_equal(f"{case}: indices", ref.index, got.index)
 
 
+def _kdtree_lists(pos, h, box=None):
+    """The reference's cKDTree twin (neighbours.py:53-71) restated with scipy: query_ball_point
+    at r = 2 h (boundary included), made symmetric, rows sorted."""
+    import scipy.sparse as sp
+    from scipy.spatial import cKDTree
+
+    n = len(pos)
+    box_arr = None if box is None else np.broadcast_to(np.asarray(box, np.float64), (3,)).copy()
+    w = pos if box_arr is None else np.mod(pos, box_arr)
+    lists = cKDTree(w, boxsize=box_arr).query_ball_point(w, r=2.0 * h, return_sorted=False)
+    lengths = np.array([len(x) for x in lists], np.int64)
+    cols = np.fromiter((j for x in lists for j in x), np.int64, int(lengths.sum()))
+    g = sp.csr_matrix((np.ones(len(cols), np.int8), (np.repeat(np.arange(n), lengths), cols)), shape=(n, n))
+    sym = (g + g.T).tocsr()
+    sym.sort_indices()
+    return sym.indptr.astype(np.int64), sym.indices.astype(np.int64)
+
+
+@pytest.mark.parametrize("case", ["lattice_boundary", "open_uneven", "periodic_cube"])
+def test_find_neighbours_is_the_inclusive_kdtree_twin(rng, case):
+    """find_neighbours keeps pairs at exactly 2 max(h) (cKDTree's query_ball_point includes its
+    boundary); find_neighbours_numba's grid search does not. On a unit lattice with h = 0.5 every
+    nearest-neighbour pair sits on that boundary."""
+    from nebula.sim.neighbours import find_neighbours, find_neighbours_numba
+
+    if case == "lattice_boundary":
+        g = np.arange(6, dtype=np.float64)
+        pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+        h, box = np.full(len(pos), 0.5), None
+    else:
+        pos, h, box = _geometry(rng, case)
+    print(f"find_neighbours case={case} n={len(pos)}", flush=True)
+    off, idx = _kdtree_lists(pos, h, box)
+    got = find_neighbours(pos, h, box=box)
+    bit_equal(f"{case}: offsets = the cKDTree twin's", off, got.offset)
+    bit_equal(f"{case}: indices = the cKDTree twin's", idx, got.index)
+    check(f"{case}: host index is int64 as in the reference", np.dtype(np.int64), got.index.dtype)
+    if case == "lattice_boundary":
+        strict = find_neighbours_numba(pos, h)
+        check("boundary pairs: only self in the strict lists", len(pos), int(strict.offset[-1]))
+        check("boundary pairs: 6 lattice neighbours + self inside", 7, int(np.max(np.diff(got.offset))))
+
+
 def test_single_particle_and_empty():
     from nebula.sim.neighbours import find_neighbours_numba
 

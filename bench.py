@@ -36,6 +36,11 @@ FLOPS_FORCE_PAIR = 83  # force pair with Monaghan viscosity, no conductivity (sp
 FLOPS_GRAD_PAIR = 45  # gradient pair (sph.py:848-865)
 FLOPS_SEARCH_ENTRY = 10  # distance pass per search-list entry (sph.py:231-248)
 FLOPS_NEWTON_ENTRY = 20  # one kernel evaluation in the Newton sums (sph.py:215-228)
+FLOPS_GLF_ITER = 35  # one GLF fixed-point iteration (integrator.py:106-176)
+FLOPS_PER_PARTICLE = 40  # kicks, equation of state, cooling
+# VALU issue ceiling: 256 CUs x 4 SIMDs, a wave64 vector instruction every 4 cycles of a 16-lane SIMD,
+# at the 2.4 GHz peak engine clock (wavefront instructions per second)
+VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
 
 
 def parse():
@@ -48,7 +53,9 @@ def parse():
     ap.add_argument("--no-compare", action="store_true", help="skip timing the other precision alongside")
     ap.add_argument("--no-learned", action="store_true", help="mixed mode without the learned starting h")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--run-steps", type=int, default=300,
+                    help="steps of the whole-run mean from the initial state (the workload's 300 steps); 0: skip")
     ap.add_argument("--drift-steps", type=int, default=1000,
                     help="steps of the energy-ledger drift runs (both precisions, from the initial state); 0: skip")
     ap.add_argument("--long-run", type=int, default=0,
@@ -109,12 +116,13 @@ def run_gpu(ic, cfg, steps, warmup, learned=False, log_stride=8, train_steps=300
 
     from nebula import _hip
     from nebula.sim import coupled
-    from nebula.sim.learned_h import LearnedH, PairLog
+    from nebula.sim.learned_h import LearnedH, PairLog, StartGate
 
     st = coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg)
     info = {}
     if learned:
-        model = LearnedH(seed=1)
+        # the gate runs the network only while the plain start leaves many rows to the slow solve
+        model = LearnedH(seed=1, gate=StartGate())
         model.collect = PairLog(stride=log_stride)
         cfg_log = replace(cfg, hydro=replace(cfg.hydro, learned_model=model))
         plain_iters = []
@@ -178,6 +186,7 @@ def run_gpu(ic, cfg, steps, warmup, learned=False, log_stride=8, train_steps=300
         info["stepper_fallbacks"] = sp.fallbacks
     if learned:
         info["fallback_rate"] = fallbacks / (steps * len(ic.x))
+        info["gate"] = info["_cfg"].hydro.learned_model.gate.stats()
     return wall, kern, entry, st, info
 
 
@@ -339,7 +348,10 @@ def parity_report(st, cfg_learned):
     plain = replace(cfg_learned, hydro=replace(cfg_learned.hydro, learned_h=False, learned_model=None))
     dt = coupled.stable_dt(st, plain)
     p = coupled.step(st, plain, dt)
+    model = cfg_learned.hydro.learned_model
+    gate, model.gate = model.gate, None  # this comparison step runs the network whatever the gate says
     q = coupled.step(st, cfg_learned, dt)
+    model.gate = gate
     out["learned_vs_plain_start (one step, same state)"] = {
         "h": _err_stats(host(p.h), host(q.h), "rel"), "rho": _err_stats(host(p.rho), host(q.rho), "rel"),
         "newton_iters": [float(p.iterations.double().mean()), float(q.iterations.double().mean())]}
@@ -628,11 +640,20 @@ def main():
     dom = max(timed, key=lambda k: timed[k][1])
     launches, total_ms = timed[dom]
     avg_s = total_ms / launches / 1e3
+    fpeak = PEAK["fp64"] if fp64 else PEAK["fp32"]
+    flops = step_flops(st, n, n_search, n_pairs, k_it, value, kern, args.steps, fp64)
     bound, work, unit = models[dom]
+    if dom in flops["kernel_flop_per_particle"]:
+        # the gather and emission are priced as the compute they do (their PMC counts show vector
+        # issue, not HBM, bounding them); their HBM figure stays alongside
+        hbm_work = work
+        bound, work, unit = "mfma", flops["kernel_flop_per_particle"][dom] * n, "TFLOP/s"
+    else:
+        hbm_work = None
     if bound == "hbm":
         achieved, peak = work / avg_s / 1e9, HBM_PEAK
     else:
-        achieved, peak = work / avg_s / 1e12, PEAK["fp64"] if fp64 else PEAK["fp32"]
+        achieved, peak = work / avg_s / 1e12, fpeak
     out = {
         "metric": "SPH particle-steps/s at 1M particles (plain FP64 vs mixed+learned-h; % FP32 peak)",
         "value": value,
@@ -657,7 +678,9 @@ def main():
                              "MI355X's dense FP64 (78.6) or FP32 (157.3) TFLOP/s, which the vector and matrix "
                              "units share",
                      "work": f"{work / n:.0f} {'FLOP' if bound != 'hbm' else 'B'}/particle per launch "
-                             f"(n_s {n_search:.1f}, pairs {n_pairs:.1f}, Newton iterations {k_it:.2f})"},
+                             f"(n_s {n_search:.1f}, pairs {n_pairs:.1f}, Newton iterations {k_it:.2f})",
+                     **_issue_and_hbm(dom, args.precision, avg_s, hbm_work)},
+        "fp32_peak": flops["report"],
         "kernels_ms_per_step": _per_step(kern, args.steps),
         "gather_launches_per_step": kern["k_gather_mixed" if not fp64 else "k_gather_solve"][0] / args.steps,
         "learned_h": {k: v for k, v in info.items() if not k.startswith("_")} if learned else None,
@@ -693,10 +716,114 @@ def main():
             print(f"bench: {args.drift_steps}-step ledger run, {name}", file=sys.stderr, flush=True)
             out["drift"][name] = drift_run(ic, c, args.drift_steps)
         out["drift"]["tolerance"] = "max |dE| / sum m|H| < 1e-5 (the reference's coupled ledger tests)"
+    if args.run_steps > 0:
+        out["whole_run"] = whole_run(ic, info["_cfg"] if learned else cfg, args.run_steps, driver,
+                                     None if (args.no_compare or fp64) else config("fp64"))
     if not args.no_cpu:
         print("bench: CPU baseline (FP64 oracle, OpenMP)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(ic, cfg, args.cpu_steps)
     print(json.dumps(out))
+
+
+def step_flops(st, n, n_search, n_pairs, k_it, value, kern, steps, fp64):
+    """Algorithmic FLOPs of one particle-step, counted from the reference's source operations
+    (FMA = 2, div = sqrt = 1): 10 n_s (distance pass) + 20 (k + 1) n_d (Newton sums and the final
+    density) + 83 n_f (force pair) + 45 n_f (gradient pair: the Balsara limiter) + 35 (i_1 + i_2 + 2)
+    (GLF) + 40 (kicks, EOS, cooling). n_s, n_f and k come from this run's lists and counters;
+    n_d, the entries inside 2 h_search of the row, is taken as n_f + 1 (the tight rows); i_1 + i_2
+    is the GLF's own iteration count on the final state. Also the FLOPs of each pair kernel per
+    launch (its share of that sum) and its TFLOP/s from the kernel timing."""
+    from nebula.sim import coupled
+    from nebula.sim.integrator import glf_device
+
+    dt = coupled.stable_dt(st, config("fp64" if fp64 else "mixed"))
+    _, _, it = glf_device(st.x, st.p, dt, raise_on_failure=False)
+    i12 = float(it.clamp(min=0).double().mean().item())
+    n_d = n_pairs + 1.0
+    terms = {"distance": FLOPS_SEARCH_ENTRY * n_search, "newton": FLOPS_NEWTON_ENTRY * (k_it + 1.0) * n_d,
+             "forces": FLOPS_FORCE_PAIR * n_pairs, "gradients": FLOPS_GRAD_PAIR * n_pairs,
+             "glf": FLOPS_GLF_ITER * (i12 + 2.0), "per_particle": float(FLOPS_PER_PARTICLE)}  # fmt: skip
+    per = sum(terms.values())
+    peak = PEAK["fp64"] if fp64 else PEAK["fp32"]
+    # each kernel's share: the gather runs the distance pass and the first Newton evaluation, the
+    # emission the final density sums (and the tightening's distances again, not counted twice)
+    kernel_flop = {
+        "k_gather_mixed": FLOPS_SEARCH_ENTRY * n_search + FLOPS_NEWTON_ENTRY * k_it * n_d,
+        "k_tight_emit_rm": FLOPS_NEWTON_ENTRY * n_d,
+        "k_forces_mixed": FLOPS_FORCE_PAIR * n_pairs,
+        "k_divcurl_mixed": FLOPS_GRAD_PAIR * n_pairs,
+        "k_gather_solve": FLOPS_SEARCH_ENTRY * n_search + FLOPS_NEWTON_ENTRY * (k_it + 1.0) * n_d,
+        "k_forces64": FLOPS_FORCE_PAIR * n_pairs,
+        "k_gradients64": FLOPS_GRAD_PAIR * n_pairs,
+    }
+    kernels = {}
+    for name, f in kernel_flop.items():
+        if name in kern and kern[name][0]:
+            avg_s = kern[name][1] / kern[name][0] / 1e3
+            tf = f * n / avg_s / 1e12
+            kernels[name] = {"flop_per_particle": round(f, 1), "avg_launch_ms": round(avg_s * 1e3, 4),
+                             "tflops": round(tf, 3), "frac_of_peak": round(tf / peak, 4)}  # fmt: skip
+    step_tf = per * value / 1e12
+    report = {"peak_tflops": peak, "precision_peak": "fp64" if fp64 else "fp32",
+              "step": {"flop_per_particle_step": round(per, 1), "terms": {k: round(v, 1) for k, v in terms.items()},
+                       "tflops": round(step_tf, 3), "frac_of_peak": round(step_tf / peak, 4),
+                       "glf_iterations": round(i12, 3)},
+              "kernels": kernels}  # fmt: skip
+    return {"report": report, "kernel_flop_per_particle": {k: v for k, v in kernel_flop.items() if k in kernels}}
+
+
+def _issue_and_hbm(kernel, precision, avg_s, hbm_work):
+    """Second views of the dominant kernel: its vector-instruction issue (SQ_INSTS_VALU from the
+    committed PMC summary, at one wave64 instruction per 4 cycles per SIMD) as a fraction of its
+    launch, and its algorithmic HBM bytes over the launch against 8 TB/s."""
+    out = {}
+    f = REPO / "profiles" / "traffic.json"
+    rec = json.loads(f.read_text()).get(precision, {}).get(kernel, {}) if f.exists() else {}
+    insts = rec.get("valu_insts_per_launch")
+    if insts:
+        issue_s = insts / VALU_ISSUE_PEAK
+        out["valu_issue"] = {"insts_per_launch": insts, "issue_ms": round(issue_s * 1e3, 4),
+                             "frac_of_launch": round(issue_s / avg_s, 4)}  # fmt: skip
+    if hbm_work is not None:
+        gbs = hbm_work / avg_s / 1e9
+        out["hbm_view"] = {"algorithmic_bytes_per_launch": int(hbm_work), "achieved_gbs": round(gbs, 1),
+                           "frac_of_8tbs": round(gbs / HBM_PEAK, 4)}  # fmt: skip
+    return out
+
+
+def whole_run(ic, cfg, steps, driver, cfg64):
+    """The workload's whole run: ``steps`` steps from the initial state (the shocked layer grows
+    and the step gets dearer), timed as a whole: mean ms per step and particle-steps/s, for the
+    headline mode and, when given, the FP64 path (eager steps) over the same number of steps."""
+    import torch
+
+    from nebula.sim import coupled
+
+    out = {"steps": steps}
+    n = len(ic.x)
+    runs = [("headline", cfg, driver)] + ([("fp64", cfg64, "eager")] if cfg64 is not None else [])
+    for label, c, drv in runs:
+        print(f"bench: whole {steps}-step run, {label}", file=sys.stderr, flush=True)
+        st = coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, c)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if drv == "stepper":
+            from nebula.sim.stepper import Stepper
+
+            sp = Stepper(st, c)
+            for _ in range(steps):
+                sp.step()
+            extra = {"stepper_fallbacks": sp.fallbacks}
+        else:
+            for _ in range(steps):
+                st = coupled.step(st, c, coupled.stable_dt(st, c))
+            extra = {}
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        out[label] = {"ms_per_step": wall / steps * 1e3, "value": n * steps / wall, "driver": drv, **extra}
+    if "fp64" in out:
+        out["value_ratio"] = out["headline"]["value"] / out["fp64"]["value"]
+    return out
 
 
 def _traffic(kernel, precision):

@@ -130,6 +130,9 @@ struct TwoSumAcc {
         c += (s - (t - z)) + (p - z);
         s = t;
     }
+    // a term far below s (here the low part of a pair force, itself exactly opposite in the other
+    // row) goes straight into the compensation
+    __device__ __forceinline__ void add_low(float l) { c += l; }
     __device__ __forceinline__ double value() const { return (double)s + (double)c; }
 };
 
@@ -174,14 +177,17 @@ __device__ __forceinline__ Nb<R> load_nb(const double* __restrict__ pos, const d
 // sph.py:525-652 without reconstruction: pressure force with grad-h terms, Monaghan viscosity on
 // approaching pairs, conductivity (pressure or velocity signal), P dV work, largest v_sig. The
 // pair force is weighed by both masses (see PF) and the row's sum divided by m_a s^2 in FP64;
-// the energy sums carry the neighbour's scaled mass and are divided by s (exact).
-template <class L, int W>
-__global__ void __launch_bounds__(kBlockM)
-k_forces_mixed(const double* __restrict__ pos, const double* __restrict__ vel, const double* __restrict__ mass,
-               const PF* __restrict__ pf, const double* __restrict__ c_s, L lists, Box box,
-               const int64_t* __restrict__ rows, int64_t nrows,
-               nb_force_params P, double* __restrict__ acc, double* __restrict__ dudt, double* __restrict__ vsig_max,
-               double* __restrict__ vheat, double* __restrict__ cheat)
+// the energy sums carry the neighbour's scaled mass and are divided by s (exact). kMode: bit 0
+// viscosity, bit 1 conductivity (compile-time, so that a pass without them holds no register
+// for them).
+template <class L, int W, int kMode>
+__device__ __forceinline__ void forces_mixed_row(const double* __restrict__ pos, const double* __restrict__ vel,
+                                                 const double* __restrict__ mass, const PF* __restrict__ pf,
+                                                 const double* __restrict__ c_s, const L& lists, const Box& box,
+                                                 const int64_t* __restrict__ rows, int64_t nrows,
+                                                 const nb_force_params& P, double* __restrict__ acc,
+                                                 double* __restrict__ dudt, double* __restrict__ vsig_max,
+                                                 double* __restrict__ vheat, double* __restrict__ cheat)
 {
     const int glane = threadIdx.x & (W - 1);
     const int64_t q = (xcd_tile() * (int64_t)blockDim.x + threadIdx.x) / W;
@@ -195,10 +201,9 @@ k_forces_mixed(const double* __restrict__ pos, const double* __restrict__ vel, c
     const float cnorm = (float)P.cnorm;
     const float iha2 = A.a.x * A.a.x, cn4a = cnorm * (iha2 * iha2);  // cnorm / h_a^4
     const float beta = (float)P.beta, alpha_u = (float)P.alpha_u;
-    const bool use_visc = P.use_visc, use_cond = P.use_cond;
+    constexpr bool use_visc = kMode & 1, use_cond = kMode & 2;
     const int kid = P.kid;
     TwoSumAcc ax, ay, az;
-    float lx = 0.f, ly = 0.f, lz = 0.f;  // low parts of the pair forces
     Kahan work, visc, cond;
     float vmax = 0.f;
     const int64_t m = lists.count(a);
@@ -245,9 +250,9 @@ k_forces_mixed(const double* __restrict__ pos, const double* __restrict__ vel, c
         ax.add(-h0);
         ay.add(-h1);
         az.add(-h2);
-        lx -= __builtin_fmaf(g, (float)(D0 - (double)dx0), __builtin_fmaf(g, dx0, -h0));
-        ly -= __builtin_fmaf(g, (float)(D1 - (double)dx1), __builtin_fmaf(g, dx1, -h1));
-        lz -= __builtin_fmaf(g, (float)(D2 - (double)dx2), __builtin_fmaf(g, dx2, -h2));
+        ax.add_low(-__builtin_fmaf(g, (float)(D0 - (double)dx0), __builtin_fmaf(g, dx0, -h0)));
+        ay.add_low(-__builtin_fmaf(g, (float)(D1 - (double)dx1), __builtin_fmaf(g, dx1, -h1)));
+        az.add_low(-__builtin_fmaf(g, (float)(D2 - (double)dx2), __builtin_fmaf(g, dx2, -h2)));
         work.add_prod(B.b.x * vdotr, fa);
     };
     // two entries per pass, both neighbours' records loaded before either pair's arithmetic: the
@@ -259,7 +264,7 @@ k_forces_mixed(const double* __restrict__ pos, const double* __restrict__ vel, c
         if (b0 != a) pair(N0);
         if (b1 != a) pair(N1);
     }
-    const double sx = gsum<W>(ax.value() + lx), sy = gsum<W>(ay.value() + ly), sz = gsum<W>(az.value() + lz);
+    const double sx = gsum<W>(ax.value()), sy = gsum<W>(ay.value()), sz = gsum<W>(az.value());
     const double sw = gsum<W>(work.value()), sv = gsum<W>(visc.value()), sc = gsum<W>(cond.value());
     const float vm = gmax<W>(vmax);
     if (glane == 0) {
@@ -274,6 +279,23 @@ k_forces_mixed(const double* __restrict__ pos, const double* __restrict__ vel, c
         if (cheat) cheat[q] = sc / ms;
     }
 }
+
+#define NB_FORCES_KERNEL(MODE)                                                                                     \
+    template <class L, int W>                                                                                      \
+    __global__ void __launch_bounds__(kBlockM) k_forces_mixed_##MODE(                                              \
+        const double* __restrict__ pos, const double* __restrict__ vel, const double* __restrict__ mass,           \
+        const PF* __restrict__ pf, const double* __restrict__ c_s, L lists, Box box,                               \
+        const int64_t* __restrict__ rows, int64_t nrows, nb_force_params P, double* __restrict__ acc,              \
+        double* __restrict__ dudt, double* __restrict__ vsig_max, double* __restrict__ vheat,                      \
+        double* __restrict__ cheat)                                                                                \
+    {                                                                                                              \
+        forces_mixed_row<L, W, MODE>(pos, vel, mass, pf, c_s, lists, box, rows, nrows, P, acc, dudt, vsig_max,     \
+                                     vheat, cheat);                                                                \
+    }
+NB_FORCES_KERNEL(0)  // pressure only
+NB_FORCES_KERNEL(1)  // + viscosity
+NB_FORCES_KERNEL(3)  // + viscosity and conductivity
+#undef NB_FORCES_KERNEL
 
 // sph.py:837-884: R, G_v and G_a of a row from FP32 pair terms, inverted in FP64 (kAcc false:
 // accelerations zero, G_a not summed).
@@ -444,8 +466,18 @@ int32_t forces_mixed(nb_ctx* ctx, hipStream_t s, const double* pos, const double
     PF* pf = (PF*)ctx->pack;
     hipLaunchKernelGGL(k_pack_forces, dim3(grid_1(n)), dim3(kBlockM), 0, s, mass, h, rho, omega, pressure,
                        P->use_cond ? u : nullptr, c_s, P->use_visc ? alpha : nullptr, P->cnorm, n, pf);
-    NB_DISPATCH_M(lists, k_forces_mixed, kPairW, nrows, s, pos, vel, mass, pf, c_s, v_, box_of(box3), rows, nrows, *P,
-                  acc, dudt, vsig_max, vheat, cheat);
+    const int mode = (P->use_visc ? 1 : 0) | (P->use_visc && P->use_cond ? 2 : 0);
+    const int ts = ktime_begin(ctx, s, "k_forces_mixed");
+#define NB_FORCES_MIXED(MODE)                                                                                      \
+    NB_DISPATCH_W(lists, k_forces_mixed_##MODE, kPairW, nrows, s, pos, vel, mass, pf, c_s, v_, box_of(box3), rows, \
+                  nrows, *P, acc, dudt, vsig_max, vheat, cheat)
+    switch (mode) {
+        case 0: NB_FORCES_MIXED(0); break;
+        case 1: NB_FORCES_MIXED(1); break;
+        default: NB_FORCES_MIXED(3); break;
+    }
+#undef NB_FORCES_MIXED
+    ktime_end(ctx, s, ts);
     return launched(ctx, "nb_forces (mixed)");
 }
 

@@ -426,6 +426,12 @@ def step(st: GasState, cfg: CoupledConfig, dt: float, fixed=None) -> GasState:
     dts = _step_dt(fixed)
     if dts is not None and (math.isfinite(cfg.cooling_beta) or (hc.viscosity and hc.switch)):
         raise ValueError("a recorded step covers power-law cooling and the constant or Balsara viscosity")
+    model = _learned(cfg)
+    gate = getattr(model, "gate", None) if hc.learned_h else None
+    if gate is not None and dts is not None:
+        raise ValueError("a recorded step runs the learned start on every replay: set the model's gate to None")
+    # decided before any launch of this step (the previous step's count is ready by then)
+    use_net = hc.learned_h and (gate is None or gate.decide())
 
     def kick(y, a):
         return _axpy(y, a, half) if dts is None else _axpy_dev(y, a, dts, 0.5)
@@ -458,13 +464,12 @@ def step(st: GasState, cfg: CoupledConfig, dt: float, fixed=None) -> GasState:
     p_pred = kick(p_flow, st.a_hydro)
     u_pred = kick(u_half, st.dudt)
     order = grid_permutation(cfg, x1, st.h, st, fixed)
-    model = _learned(cfg)
     fields = [x1, p_flow, p_pred, u_half, u_pred, st.a_hydro, st.mass, st.h, st.alpha, st.perm]
     counts = st.row_counts
     if counts is not None:
         fields.append(counts.view(torch.int64).view(n))  # two int32 counts per particle move as one
     extra = []
-    if model is not None:
+    if model is not None and (use_net or model.collect is not None):
         # the learned start's inputs from the last hydro pass, in the new order
         extra = [t for t in (st.rho, st.div_v, st.curl_v, st.nneigh)]
     moved = permute_many(fields + [t for t in extra if t is not None], order)
@@ -475,14 +480,16 @@ def step(st: GasState, cfg: CoupledConfig, dt: float, fixed=None) -> GasState:
                    row_counts=counts)  # fmt: skip
     h_start = feat = None
     fallbacks = None
-    if model is not None:
+    if model is not None and (use_net or model.collect is not None):
         it = iter(moved[len(fields):])
         rho_p, div_p, curl_p, nn_p = (None if t is None else next(it) for t in extra)
         feat = model.features(h, rho_p, div_p, curl_p, nn_p, x1, u_pred, dt, st.dt_prev, stride=1, dts=dts)
-        if hc.learned_h:
+        if use_net:
             h_start = model.h_start(h, feat)
             fallbacks = model.last_fallbacks
     new = _hydro(pred, cfg, velocity(cfg, x1, p_pred), dt, a_prev, h_start=h_start, fixed=fixed)
+    if gate is not None:
+        gate.observe((new.iterations > 1).sum(), n, use_net)
     if fixed is not None:
         fixed.pairs = (model, feat, h, new.h)
     else:
@@ -494,7 +501,7 @@ def step(st: GasState, cfg: CoupledConfig, dt: float, fixed=None) -> GasState:
 def log_pairs(cfg: CoupledConfig, model, feat, h_prev, h_final) -> None:
     """The learned start's (features, converged h) pairs of a step: into the model's PairLog, and
     to its online training when the start is in use."""
-    if model is None:
+    if model is None or feat is None:
         return
     if model.collect is not None:
         s = model.collect.stride

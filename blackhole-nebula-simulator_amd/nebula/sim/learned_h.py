@@ -68,10 +68,81 @@ class OnlineSchedule:
         self.steps, self.batch, self.lr = int(steps), int(batch), float(lr)
 
 
-class LearnedH:
-    """The network, its normalisation and its Adam state (device tensors)."""
+class StartGate:
+    """When the learned start runs: only while plain Newton needs it.
 
-    def __init__(self, seed: int = 0, residual: bool = True):
+    The network and its online training cost a fixed time per step (inference, features,
+    training). They pay only when many rows would otherwise need more than one Newton step; where
+    the plain start (h_prev) already converges in one step for nearly every row, they cost time and
+    save nothing. The gate watches the fraction of rows left for the slow solve (more than one
+    Newton step) under the plain start:
+
+    - the first step, and every ``probe_every``-th step while the network runs, start from h_prev
+      (a probe) and measure that fraction;
+    - the network runs while the probed fraction is at least ``off_below``, and is switched on
+      again once a plain step's fraction exceeds ``on_above``.
+
+    Decisions depend only on the run's own iteration counts, so equal runs decide alike. A step
+    that starts from h_prev is exactly the plain method's step.
+    """
+
+    def __init__(self, on_above: float = 0.05, off_below: float = 0.03, probe_every: int = 32):
+        self.on_above, self.off_below, self.probe_every = float(on_above), float(off_below), int(probe_every)
+        self.on = None  # unknown until the first probe
+        self.since_probe = 0
+        self._pending = None  # (device count of slow rows, rows, network used) of the last step
+        self.last_plain_fraction = None
+        self.steps_network = 0
+        self.steps_plain = 0
+        self.probes = 0
+
+    def observe(self, slow_rows, rows: int, used_network: bool) -> None:
+        """The last step's count of rows left for the slow solve (a device scalar, read lazily)."""
+        self._pending = (slow_rows, int(rows), bool(used_network))
+
+    def _resolve(self) -> None:
+        if self._pending is None:
+            return
+        count, rows, used = self._pending
+        self._pending = None
+        if used:
+            return
+        frac = float(count.item() if hasattr(count, "item") else count) / max(rows, 1)
+        self.last_plain_fraction = frac
+        if self.on:  # a probe while the network runs
+            self.on = frac >= self.off_below
+        else:
+            self.on = frac > self.on_above
+
+    def decide(self) -> bool:
+        """True: the coming step starts from the network's prediction; False: from h_prev."""
+        self._resolve()
+        use = False
+        if self.on is None:
+            self.probes += 1
+        elif self.on:
+            if self.since_probe >= self.probe_every:
+                self.since_probe = 0
+                self.probes += 1
+            else:
+                self.since_probe += 1
+                use = True
+        if use:
+            self.steps_network += 1
+        else:
+            self.steps_plain += 1
+        return use
+
+    def stats(self) -> dict:
+        return {"steps_network": self.steps_network, "steps_plain": self.steps_plain, "probes": self.probes,
+                "last_plain_slow_fraction": self.last_plain_fraction, "on": self.on}  # fmt: skip
+
+
+class LearnedH:
+    """The network, its normalisation and its Adam state (device tensors). ``gate`` decides per
+    step whether the network runs (StartGate); None runs it on every step."""
+
+    def __init__(self, seed: int = 0, residual: bool = True, gate: StartGate | None = None):
         self.residual = bool(residual)
         n = param_count()
         rng = np.random.default_rng(seed)
@@ -90,6 +161,7 @@ class LearnedH:
         self.last_fallbacks = None  # device count of the last h_start call
         self.online: OnlineSchedule | None = None
         self._ring = None
+        self.gate = gate
 
     def start_online(self, **kw) -> None:
         """Keep training while the model is used (OnlineSchedule fields as keywords)."""

@@ -90,6 +90,8 @@ class Stepper:
         st, cfg, dt = self.state, self.cfg, self.dt
         if _hip.generation() != self._generation:
             self._reserve()  # the library's scratch was released or moved since the reservation
+        gate = _gate(cfg)
+        gate_before = None if gate is None else dict(gate.__dict__)
         self.status.zero_()
         self.fixed.pairs = None
         new = coupled.step(st, cfg, dt, fixed=self.fixed)
@@ -102,6 +104,8 @@ class Stepper:
         if np.any(status[:8] != 0):
             self.fallbacks += 1
             self.fallback_reasons.append({STATUS_NAMES[k]: int(status[k]) for k in range(8) if status[k]})
+            if gate is not None:  # the step is done again: the same decision, observed once
+                gate.__dict__.update(gate_before)
             new = coupled.step(st, cfg, dt)
             self.state = new
             self.headroom = max(self.headroom, 1.3)
@@ -114,6 +118,12 @@ class Stepper:
             self.dt = coupled.dt_from_minima(packed[16:20], cfg, new.nu is not None)
         self.steps += 1
         return dt
+
+
+def _gate(cfg):
+    """The learned start's gate of the configuration, or None."""
+    model = cfg.hydro.learned_model if cfg.hydro.learned_h else None
+    return getattr(model, "gate", None)
 
 
 # the state fields a recorded step carries from one replay to the next (those the state has)

@@ -44,11 +44,11 @@ def _compare(label, a, b):
     check(f"{label}: tight entries", a.n_tight_entries, b.n_tight_entries)
 
 
-def _trained_model(ic, cfg):
+def _trained_model(ic, cfg, gated=False):
     from dataclasses import replace
 
     from nebula.sim import coupled
-    from nebula.sim.learned_h import LearnedH, PairLog
+    from nebula.sim.learned_h import LearnedH, PairLog, StartGate
 
     model = LearnedH(seed=5)
     model.collect = PairLog(stride=2)
@@ -61,18 +61,23 @@ def _trained_model(ic, cfg):
     model.fit_normalisation(f, t)
     model.train(f, t, steps=60, batch=2048, record=False)
     model.start_online(stride=2, every=3, steps=2, batch=1024)
+    if gated:  # probes every 4 steps, so that 10 steps see the gate switch both ways
+        model.gate = StartGate(on_above=-1.0, off_below=0.0, probe_every=4)
     return model
 
 
-@pytest.mark.parametrize("learned", [False, True])
+@pytest.mark.parametrize("learned", [False, True, "gated"])
 def test_stepper_equals_coupled_step(learned):
+    """With a gated learned start (probes every 4 steps, on whenever a probe finds a slow row) the
+    gate's decisions are part of what must match: the stepper redoes a failed step with the
+    decision it had made, and observes it once."""
     from nebula.ics import stream_crash
     from nebula.sim import coupled
     from nebula.sim.stepper import Stepper
 
     ic = stream_crash(16384, length=200.0, seed=20261014)
     base = _config()
-    model_a = _trained_model(ic, base) if learned else None
+    model_a = _trained_model(ic, base, gated=learned == "gated") if learned else None
     model_b = copy.deepcopy(model_a) if learned else None
     cfg_a, cfg_b = _config(model_a), _config(model_b)
     print(f"stepper: 16384-particle stream crash, learned start {learned}, 10 steps", flush=True)
@@ -92,6 +97,11 @@ def test_stepper_equals_coupled_step(learned):
     check("most steps without a fallback", True, sp.fallbacks <= 3)
     if learned:
         bit_equal("online-trained weights", model_a.params.cpu().numpy(), model_b.params.cpu().numpy())
+    if learned == "gated":
+        print("gate:", model_a.gate.stats(), model_b.gate.stats())
+        check("the gates made the same decisions", model_a.gate.stats(), model_b.gate.stats())
+        check("the gate ran the network on some steps and probed on others", True,
+              model_a.gate.steps_network > 0 and model_a.gate.probes >= 2)  # fmt: skip
 
 
 def test_stepper_overflow_falls_back_to_coupled_step():

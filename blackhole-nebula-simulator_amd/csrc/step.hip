@@ -160,15 +160,92 @@ __global__ void k_eos_ideal(const double* __restrict__ rho, const double* __rest
     cs[i] = sqrt(ggm1 * ui);
 }
 
+// ----------------------------------------------------------------- gas + radiation EOS
+// fastphys.py:45-111 with eos.py:76-117 (local thermodynamic equilibrium, one temperature):
+//   u = 3 k T / (2 mu m_p) + a T^4 / rho,  P = rho k T / (mu m_p) + a T^4 / 3  (SI),
+//   beta = P_gas / P,  Gamma_1 = beta + (4 - 3 beta)^2 (gamma - 1) / (beta + 12 (gamma - 1)(1 - beta)),
+//   c_s = sqrt(Gamma_1 P / rho),
+// with T from (rho, u) by Newton on log T inside the bracket [log 1, log 1e13] K, which every
+// step narrows (a step leaving it, or not finite, bisects), from the smaller of the gas-only and
+// radiation-only temperatures, until a step in log T is below 1e-12. SI constants are those of
+// units.py (CODATA 2018).
+constexpr double kKB = 1.380649e-23, kMP = 1.67262192369e-27, kMuGas = 0.6, kCLight = 299792458.0;
+constexpr double kARad = 4.0 * 5.670374419e-8 / kCLight;
+constexpr double kTMin = 1.0, kTMax = 1.0e13, kEosTol = 1e-12;
+constexpr int kEosMaxIter = 200;
+
+__device__ __forceinline__ double gasrad_u_si(double rho_si, double t)
+{
+    const double t4 = (t * t) * (t * t);
+    return 1.5 * kKB * t / (kMuGas * kMP) + kARad * t4 / rho_si;
+}
+
+// P [code], c_s, T [K] of code rho and u (dens: kg/m^3 per code density). bad[0] counts u outside
+// u(1 K) .. u(1e13 K) (the reference's ValueError), bad[1] solves that did not converge.
+__global__ void k_eos_gasrad(const double* __restrict__ rho, const double* __restrict__ u, double dens, int64_t n,
+                             double* __restrict__ pressure, double* __restrict__ cs, double* __restrict__ t_out,
+                             int32_t* __restrict__ bad)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double r = rho[i] * dens, uu = u[i] * (kCLight * kCLight);
+    if (!(uu >= gasrad_u_si(r, kTMin) && uu <= gasrad_u_si(r, kTMax))) {
+        atomicAdd(&bad[0], 1);
+        pressure[i] = cs[i] = NAN;
+        if (t_out) t_out[i] = NAN;
+        return;
+    }
+    const double a1 = 1.5 * kKB / (kMuGas * kMP);
+    double lo = log(kTMin), hi = log(kTMax);
+    const double t0 = fmin(uu / a1, pow(uu * r / kARad, 0.25));
+    double lt = log(fmin(fmax(t0, kTMin), kTMax));
+    bool ok = false;
+    for (int it = 0; it < kEosMaxIter; it++) {
+        const double t = exp(lt);
+        const double t4 = (t * t) * (t * t);
+        const double f = a1 * t + kARad * t4 / r - uu;
+        const double dfdl = a1 * t + 4.0 * kARad * t4 / r;
+        if (f < 0.0) lo = lt;
+        if (f > 0.0) hi = lt;
+        double nw = lt - f / dfdl;
+        if (nw <= lo || nw >= hi || !isfinite(nw)) nw = 0.5 * (lo + hi);
+        const double step = fabs(nw - lt);
+        lt = nw;
+        if (step < kEosTol) {
+            ok = true;
+            break;
+        }
+    }
+    if (!ok) atomicAdd(&bad[1], 1);
+    const double t = exp(lt), t4 = (t * t) * (t * t);
+    const double p_gas = r * kKB * t / (kMuGas * kMP), p_rad = kARad * t4 / 3.0;
+    const double beta = p_gas / (p_gas + p_rad);
+    const double p_code = (p_gas + p_rad) / (dens * (kCLight * kCLight));
+    const double g = 5.0 / 3.0 - 1.0;
+    const double g1 = beta + (4.0 - 3.0 * beta) * (4.0 - 3.0 * beta) * g / (beta + 12.0 * g * (1.0 - beta));
+    pressure[i] = p_code;
+    cs[i] = sqrt(g1 * p_code / rho[i]);
+    if (t_out) t_out[i] = t;
+}
+
+// hydro.py:155-163 (gas + radiation): the code u of gas at temperature t_k at each density
+__global__ void k_gasrad_u(const double* __restrict__ rho, double dens, double t_k, int64_t n, double* __restrict__ out)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) out[i] = gasrad_u_si(rho[i] * dens, t_k) / (kCLight * kCLight);
+}
+
 // cooling.py:33-40 with coupled.py:244-251: u_floor + (u - u_floor) exp(-dt / t_cool),
-// t_cool = beta r_mid^1.5, r_mid = (|x_old| + |x_new|)/2. e_part[i] = m (u - u_new).
+// t_cool = beta r_mid^1.5, r_mid = (|x_old| + |x_new|)/2. e_part[i] = m (u - u_new). The floor is
+// one number (ideal gas) or one per particle (floors: gas + radiation, set by the density).
 __global__ void k_cool_beta(const double* __restrict__ u, const double* __restrict__ x_old,
-                            const double* __restrict__ x_new, const double* __restrict__ mass, double u_floor,
-                            double dt, double beta, int64_t n, double* __restrict__ u_out,
-                            double* __restrict__ e_part)
+                            const double* __restrict__ x_new, const double* __restrict__ mass, double u_floor_one,
+                            const double* __restrict__ floors, double dt, double beta, int64_t n,
+                            double* __restrict__ u_out, double* __restrict__ e_part)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const double u_floor = floors ? floors[i] : u_floor_one;
     double r0 = sqrt(x_old[3 * i] * x_old[3 * i] + x_old[3 * i + 1] * x_old[3 * i + 1] + x_old[3 * i + 2] * x_old[3 * i + 2]);
     double r1 = sqrt(x_new[3 * i] * x_new[3 * i] + x_new[3 * i + 1] * x_new[3 * i + 1] + x_new[3 * i + 2] * x_new[3 * i + 2]);
     double r_mid = 0.5 * (r0 + r1);
@@ -419,8 +496,39 @@ extern "C" int32_t nb_cool_beta(nb_ctx* ctx, void* stream, const double* u, cons
     if (!ctx || n < 0) return ctx_fail(ctx, "nb_cool_beta: bad arguments");
     if (n == 0) return NB_OK;
     hipLaunchKernelGGL(k_cool_beta, dim3(grid_s(n)), dim3(kBlockS), 0, (hipStream_t)stream, u, x_old, x_new, mass,
-                       u_floor, dt, beta, n, u_out, e_part);
+                       u_floor, (const double*)nullptr, dt, beta, n, u_out, e_part);
     return launch_check_s(ctx, "nb_cool_beta");
+}
+
+extern "C" int32_t nb_cool_beta_floors(nb_ctx* ctx, void* stream, const double* u, const double* x_old,
+                                       const double* x_new, const double* mass, const double* u_floor, double dt,
+                                       double beta, int64_t n, double* u_out, double* e_part)
+{
+    if (!ctx || n < 0 || (n > 0 && !u_floor)) return ctx_fail(ctx, "nb_cool_beta_floors: bad arguments");
+    if (n == 0) return NB_OK;
+    hipLaunchKernelGGL(k_cool_beta, dim3(grid_s(n)), dim3(kBlockS), 0, (hipStream_t)stream, u, x_old, x_new, mass,
+                       0.0, u_floor, dt, beta, n, u_out, e_part);
+    return launch_check_s(ctx, "nb_cool_beta_floors");
+}
+
+extern "C" int32_t nb_eos_gasrad(nb_ctx* ctx, void* stream, const double* rho, const double* u, double dens_kg_m3,
+                                 int64_t n, double* pressure, double* cs, double* t_out, int32_t* bad_dev)
+{
+    if (!ctx || n < 0 || !bad_dev || !(dens_kg_m3 > 0.0)) return ctx_fail(ctx, "nb_eos_gasrad: bad arguments");
+    if (n == 0) return NB_OK;
+    hipLaunchKernelGGL(k_eos_gasrad, dim3(grid_s(n)), dim3(kBlockS), 0, (hipStream_t)stream, rho, u, dens_kg_m3, n,
+                       pressure, cs, t_out, bad_dev);
+    return launch_check_s(ctx, "nb_eos_gasrad");
+}
+
+extern "C" int32_t nb_gasrad_u(nb_ctx* ctx, void* stream, const double* rho, double dens_kg_m3, double t_k, int64_t n,
+                               double* u_out)
+{
+    if (!ctx || n < 0 || !(dens_kg_m3 > 0.0)) return ctx_fail(ctx, "nb_gasrad_u: bad arguments");
+    if (n == 0) return NB_OK;
+    hipLaunchKernelGGL(k_gasrad_u, dim3(grid_s(n)), dim3(kBlockS), 0, (hipStream_t)stream, rho, dens_kg_m3, t_k, n,
+                       u_out);
+    return launch_check_s(ctx, "nb_gasrad_u");
 }
 
 extern "C" int32_t nb_cool_powerlaw(nb_ctx* ctx, void* stream, const double* u, const double* rho,

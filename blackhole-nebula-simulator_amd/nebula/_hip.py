@@ -90,6 +90,9 @@ SIGNATURES = {
     "nb_axpy_dev": (_i32, [_vp, _vp, _vp, _vp, _vp, _f64, _i64, _vp]),
     "nb_eos_ideal": (_i32, [_vp, _vp, _vp, _vp, _f64, _i64, _vp, _vp, _vp]),
     "nb_cool_beta": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _f64, _f64, _f64, _i64, _vp, _vp]),
+    "nb_cool_beta_floors": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _f64, _f64, _i64, _vp, _vp]),
+    "nb_eos_gasrad": (_i32, [_vp, _vp, _vp, _vp, _f64, _i64, _vp, _vp, _vp, _vp]),
+    "nb_gasrad_u": (_i32, [_vp, _vp, _vp, _f64, _f64, _i64, _vp]),
     "nb_cool_powerlaw": (_i32, [_vp, _vp, _vp, _vp, _vp, _f64, _f64, _f64, _i64, _vp, _vp]),
     "nb_cool_powerlaw_dev": (_i32, [_vp, _vp, _vp, _vp, _vp, _f64, _f64, _vp, _i64, _vp, _vp]),
     "nb_dt_minima": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _f64, _vp, _i64, _PD]),

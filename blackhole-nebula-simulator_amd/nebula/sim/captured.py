@@ -36,7 +36,7 @@ from .sph import C_NORM, KERNEL_IDS, DensityResult
 _FIELDS = ("x", "p", "u", "mass", "h", "alpha", "a_hydro", "dudt", "rho", "vsig_max", "c_s", "omega", "iterations",
            "flags", "perm", "row_counts")  # fmt: skip
 STATUS_NAMES = ("gather rows", "extras", "tight index", "grid cells", "h outgrew the search", "not in cell order",
-                "GN flow failures", "u <= 0")  # fmt: skip
+                "GN flow failures", "u <= 0 or outside the EOS bracket", "EOS did not converge")  # fmt: skip
 
 
 class _Fixed:
@@ -256,7 +256,7 @@ class CapturedEnsemble:
         packed = torch.cat([self.status.double(), self.minima.reshape(-1)]).cpu().numpy()
         status = packed[:16].astype(np.int64)
         self.replays += 1
-        if np.any(status[:8] != 0):
+        if np.any(status[:9] != 0):
             self._fallback(status)
         else:
             self._minima_host = packed[16:].reshape(self.nseg, 4)
@@ -267,7 +267,7 @@ class CapturedEnsemble:
     def _fallback(self, status):
         """Restore the state saved at the replay's start, do the step eagerly, record again."""
         self.fallbacks += 1
-        self.fallback_reasons.append({STATUS_NAMES[k]: int(status[k]) for k in range(8) if status[k]})
+        self.fallback_reasons.append({STATUS_NAMES[k]: int(status[k]) for k in range(9) if status[k]})
         for k, v in self.backup.items():
             self.S[k].copy_(v)
         self.sync_back()

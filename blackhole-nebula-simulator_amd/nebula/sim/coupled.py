@@ -33,7 +33,7 @@ import torch
 from .. import _dev, _hip
 from ..physics import units
 from .fastphys import gn_v_from_p_device
-from .hydro import HydroConfig
+from .hydro import HydroConfig, floor_energy, gasrad_state_device
 from .integrator import glf_device
 from .neighbours import IndexBuffer, grid_order, grid_order_seg, permute, permute_many
 from .sph import (
@@ -269,8 +269,8 @@ def _hydro(st: GasState, cfg: CoupledConfig, v, dt_alpha: float, a_prev, h_start
     checked after the fact by nebula.sim.stepper): one search round, no host synchronisation;
     what does not fit goes to fixed.status."""
     hc = cfg.hydro
-    if hc.isothermal_cs is not None or hc.eos != "ideal":
-        raise ValueError("the GPU step implements the ideal-gas equation of state")
+    if hc.isothermal_cs is not None or hc.eos not in ("ideal", "gasrad"):
+        raise ValueError("the GPU step implements the ideal-gas and gas + radiation equations of state")
     n = int(st.x.shape[0])
     buf = st.list_buffer if st.list_buffer is not None else IndexBuffer()
     counts = None
@@ -295,11 +295,20 @@ def _hydro(st: GasState, cfg: CoupledConfig, v, dt_alpha: float, a_prev, h_start
     p_gas = _dev.empty(n)
     cs = _dev.empty(n)
     nbad = torch.zeros(1, dtype=torch.int32, device=st.x.device) if fixed is None else fixed.status[7:]
-    _hip.call("nb_eos_ideal", _hip.ptr(d.rho), _hip.ptr(st.u), float(hc.gamma), n, _hip.ptr(p_gas), _hip.ptr(cs),
-              _hip.ptr(nbad))  # fmt: skip
-    bad = int(nbad.item()) if fixed is None else 0  # a recorded step reports it in its status
-    if bad:
-        raise FloatingPointError(f"non-positive internal energy at {bad} particles")
+    if hc.eos == "gasrad":
+        # u <= 0 lies below the solver bracket: counted with the bracket failures
+        _hip.call("nb_eos_ideal", _hip.ptr(d.rho), _hip.ptr(st.u), float(hc.gamma), n, _hip.ptr(p_gas),
+                  _hip.ptr(cs), _hip.ptr(nbad))  # fmt: skip
+        bad = int(nbad.item()) if fixed is None else 0
+        if bad:
+            raise FloatingPointError(f"non-positive internal energy at {bad} particles")
+        p_gas, cs, _ = gasrad_state_device(hc, d.rho, st.u, None if fixed is None else fixed.status)
+    else:
+        _hip.call("nb_eos_ideal", _hip.ptr(d.rho), _hip.ptr(st.u), float(hc.gamma), n, _hip.ptr(p_gas),
+                  _hip.ptr(cs), _hip.ptr(nbad))  # fmt: skip
+        bad = int(nbad.item()) if fixed is None else 0  # a recorded step reports it in its status
+        if bad:
+            raise FloatingPointError(f"non-positive internal energy at {bad} particles")
     nb = d.neighbours  # tightened at the final h
     alpha = st.alpha
     g = None
@@ -443,11 +452,18 @@ def step(st: GasState, cfg: CoupledConfig, dt: float, fixed=None) -> GasState:
     if math.isfinite(cfg.cooling_beta) or cfg.cooling_powerlaw:
         u_floor = units.ideal_u_floor(cfg.t_floor, hc.gamma)
         e_part = _dev.empty(n)
+        if cfg.cooling_powerlaw and hc.eos != "ideal":
+            raise ValueError("the power-law cooling law is defined for the ideal-gas equation of state")
         if math.isfinite(cfg.cooling_beta):
             u_cool = _dev.empty(n)
-            _hip.call("nb_cool_beta", _hip.ptr(u_half), _hip.ptr(st.x), _hip.ptr(x1), _hip.ptr(st.mass),
-                      float(u_floor), float(dt), float(cfg.cooling_beta), n, _hip.ptr(u_cool),
-                      _hip.ptr(e_part))  # fmt: skip
+            if hc.eos == "gasrad":  # the floor at each particle's density (hydro.py:155-163)
+                _hip.call("nb_cool_beta_floors", _hip.ptr(u_half), _hip.ptr(st.x), _hip.ptr(x1), _hip.ptr(st.mass),
+                          _hip.ptr(floor_energy(hc, st.rho, cfg.t_floor)), float(dt), float(cfg.cooling_beta), n,
+                          _hip.ptr(u_cool), _hip.ptr(e_part))  # fmt: skip
+            else:
+                _hip.call("nb_cool_beta", _hip.ptr(u_half), _hip.ptr(st.x), _hip.ptr(x1), _hip.ptr(st.mass),
+                          float(u_floor), float(dt), float(cfg.cooling_beta), n, _hip.ptr(u_cool),
+                          _hip.ptr(e_part))  # fmt: skip
             _hip.call("nb_sum_into", _hip.ptr(e_part), n, _hip.ptr(e_rad))
             u_half = u_cool
         if cfg.cooling_powerlaw:

@@ -101,9 +101,9 @@ class Stepper:
         totals = torch.stack([new.row_counts.sum(dtype=torch.int64), nb_len.sum().to(torch.int64)]).double()
         packed = torch.cat([self.status.double(), self.minima, totals]).cpu().numpy()  # the one synchronisation
         status = packed[:16].astype(np.int64)
-        if np.any(status[:8] != 0):
+        if np.any(status[:9] != 0):
             self.fallbacks += 1
-            self.fallback_reasons.append({STATUS_NAMES[k]: int(status[k]) for k in range(8) if status[k]})
+            self.fallback_reasons.append({STATUS_NAMES[k]: int(status[k]) for k in range(9) if status[k]})
             if gate is not None:  # the step is done again: the same decision, observed once
                 gate.__dict__.update(gate_before)
             new = coupled.step(st, cfg, dt)
@@ -228,9 +228,9 @@ class RecordedStepper(Stepper):
         packed = torch.cat([self.status.double(), self.minima, totals]).cpu().numpy()  # the one synchronisation
         status = packed[:16].astype(np.int64)
         self.replays += 1
-        if np.any(status[:8] != 0):
+        if np.any(status[:9] != 0):
             self.fallbacks += 1
-            self.fallback_reasons.append({STATUS_NAMES[k]: int(status[k]) for k in range(8) if status[k]})
+            self.fallback_reasons.append({STATUS_NAMES[k]: int(status[k]) for k in range(9) if status[k]})
             for k, v in self.backup.items():
                 S[k].copy_(v)
             new = coupled.step(st, cfg, dt)

@@ -339,6 +339,22 @@ int32_t nb_eos_ideal(nb_ctx *ctx, void *stream, const double *rho, const double 
 int32_t nb_cool_beta(nb_ctx *ctx, void *stream, const double *u, const double *x_old, const double *x_new,
                      const double *mass, double u_floor, double dt, double beta, int64_t n, double *u_out,
                      double *e_part);
+/* nb_cool_beta with a floor per particle (the gas + radiation floor depends on the density,
+ * hydro.py:155-163). */
+int32_t nb_cool_beta_floors(nb_ctx *ctx, void *stream, const double *u, const double *x_old, const double *x_new,
+                            const double *mass, const double *u_floor, double dt, double beta, int64_t n,
+                            double *u_out, double *e_part);
+/* Gas + radiation equation of state (fastphys.py:45-111, eos.py:76-148; hydro.py:133-135): P and
+ * c_s (code units) and T [K] (t_out may be NULL) from code rho and u, dens_kg_m3 being one code
+ * density unit (GasUnits.density_kg_m3). T by Newton on log T in the bracket [1, 1e13] K to
+ * 1e-12. bad_dev (two int32, zeroed by the caller) counts u outside the bracket (the reference's
+ * ValueError) and solves that did not converge (its RuntimeError). */
+int32_t nb_eos_gasrad(nb_ctx *ctx, void *stream, const double *rho, const double *u, double dens_kg_m3, int64_t n,
+                      double *pressure, double *cs, double *t_out, int32_t *bad_dev);
+/* The code u of gas + radiation at temperature t_k for each code density (hydro.py:155-163,
+ * floor_energy). */
+int32_t nb_gasrad_u(nb_ctx *ctx, void *stream, const double *rho, double dens_kg_m3, double t_k, int64_t n,
+                    double *u_out);
 
 /* optically thin power-law cooling du/dt = -k rho sqrt(u) above u_floor, exact at fixed rho
  * (the reference has no such law; same place in the split as beta cooling). */

@@ -104,9 +104,27 @@ def eos_state(hc, rho, u, x=None):
         raise FloatingPointError(f"non-positive internal energy at {int(np.sum(u <= 0))} particles")
     if hc.eos == "ideal":
         return ph.eos_ideal(rho, u, hc.gamma)
+    if hc.eos == "gasrad":
+        p, cs, _, _ = ph.gasrad_state(rho, u, _density_kg_m3(hc))
+        return p, cs
     if hc.eos == "radiation":
         return np.asarray(rho) * np.asarray(u) / 3.0, np.sqrt(4.0 * np.asarray(u) / 9.0)
     raise ValueError(f"unsupported equation of state {hc.eos!r}")
+
+
+def _density_kg_m3(hc):
+    """units.py:89-110: the code density unit of the configuration's gas mass scale."""
+    gu = hc.gas_units
+    return ph.gas_density_kg_m3(gu.cloud_mass_msun, gu.cloud_mass_code, gu.bh.mass_msun)
+
+
+def floor_energy(hc, rho, t_floor):
+    """hydro.py:155-163: the code u of the cooling floor at each particle's density."""
+    if hc.eos == "gasrad":
+        return ph.gasrad_u_floor(rho, t_floor, _density_kg_m3(hc))
+    if hc.eos == "ideal":
+        return np.full_like(np.asarray(rho, float), ph.ideal_u_floor(t_floor, hc.gamma))
+    raise ValueError(f"no temperature floor for equation of state {hc.eos!r}")
 
 
 def physical_viscosity(hc, x, c_s):
@@ -212,11 +230,13 @@ def step(st: State, cfg, dt: float) -> State:
     e_rad = st.e_radiated
     if math.isfinite(cfg.cooling_beta):
         r_mid = 0.5 * (np.linalg.norm(st.x, axis=1) + np.linalg.norm(x1, axis=1))
-        u_floor = np.full_like(st.rho, ph.ideal_u_floor(cfg.t_floor, hc.gamma))
+        u_floor = floor_energy(hc, st.rho, cfg.t_floor)
         u_cool = ph.beta_cool(u_half, u_floor, dt, ph.cooling_time(r_mid, cfg.cooling_beta))
         e_rad += float(np.sum(st.mass * (u_half - u_cool)))
         u_half = u_cool
     if cfg.cooling_powerlaw:
+        if hc.eos != "ideal":
+            raise ValueError("the power-law cooling law is defined for the ideal-gas equation of state")
         u_floor = ph.ideal_u_floor(cfg.t_floor, hc.gamma)
         u_cool = ph.powerlaw_cool(u_half, st.rho, dt, u_floor, ph.powerlaw_coefficient(hc.gamma))
         e_rad += float(np.sum(st.mass * (u_half - u_cool)))

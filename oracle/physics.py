@@ -5,6 +5,8 @@ laws the workloads need that the reference does not have (power-law cooling, Bal
 switch), defined here from their published forms.
 """
 
+import math
+
 import numpy as np
 
 # SI constants, CODATA 2018 (units.py:17-39 of the reference)
@@ -38,6 +40,107 @@ def ideal_u_floor(t_floor, gamma):
 def eos_ideal(rho, u, gamma):
     """eos.py:21-28: P = (gamma-1) rho u, c_s = sqrt(gamma (gamma-1) u)."""
     return (gamma - 1.0) * np.asarray(rho) * np.asarray(u), np.sqrt(gamma * (gamma - 1.0) * np.asarray(u))
+
+
+# ----------------------------------------------------------------- gas + radiation EOS
+SIGMA_SB = 5.670374419e-8  # W m^-2 K^-4 (units.py:26)
+A_RAD = 4.0 * SIGMA_SB / C  # radiation constant, J m^-3 K^-4 (units.py:27)
+T_MIN_SI, T_MAX_SI = 1.0, 1.0e13  # solver bracket in K (eos.py:82)
+EOS_TOL = 1e-12  # eos.py:83
+
+
+def gas_density_kg_m3(cloud_mass_msun=0.1, cloud_mass_code=1.0, bh_mass_msun=M_BH_MSUN):
+    """units.py:89-110 (GasUnits.density_kg_m3): one code density unit, (cloud_mass_msun M_sun /
+    cloud_mass_code) / (G M_bh / c^2)^3, in kg/m^3."""
+    length = G * bh_mass_msun * M_SUN / C**2
+    return cloud_mass_msun * M_SUN / cloud_mass_code / length**3
+
+
+def gasrad_u_si(rho_si, t_k):
+    """eos.py:86-92: u = 3 k T / (2 mu m_p) + a T^4 / rho  [J/kg]."""
+    t = np.asarray(t_k, float)
+    return 1.5 * K_B * t / (MU * M_P) + A_RAD * t**4 / np.asarray(rho_si)
+
+
+def gasrad_pressure_si(rho_si, t_k):
+    """eos.py:95-100: P = rho k T / (mu m_p) + a T^4 / 3  [Pa]."""
+    t = np.asarray(t_k, float)
+    return np.asarray(rho_si) * K_B * t / (MU * M_P) + A_RAD * t**4 / 3.0
+
+
+def gasrad_beta_si(rho_si, t_k):
+    """eos.py:103-109: beta_gas = P_gas / P."""
+    t = np.asarray(t_k, float)
+    p_gas = np.asarray(rho_si) * K_B * t / (MU * M_P)
+    return p_gas / (p_gas + A_RAD * t**4 / 3.0)
+
+
+def gamma1(beta, gamma=5.0 / 3.0):
+    """eos.py:112-117: Gamma_1 = beta + (4 - 3 beta)^2 (gamma - 1) / (beta + 12 (gamma - 1)(1 - beta))."""
+    b = np.asarray(beta, float)
+    g = gamma - 1.0
+    return b + (4.0 - 3.0 * b) ** 2 * g / (b + 12.0 * g * (1.0 - b))
+
+
+def gasrad_temperature_si(rho_si, u_si, tol=EOS_TOL, max_iter=200):
+    """fastphys.py:45-93 (the per-element form of eos.py:120-148): T [K] from rho [kg/m^3] and
+    u [J/kg] by Newton on log T, kept inside a bracket [log 1, log 1e13] narrowed at every step
+    (a Newton step leaving it, or not finite, bisects); starts from the smaller of the gas-only
+    and radiation-only temperatures; stops once a step in log T is below tol. ValueError when
+    u lies outside u(1 K) .. u(1e13 K), RuntimeError when an element does not converge."""
+    rho = np.ascontiguousarray(np.broadcast_to(np.asarray(rho_si, float), np.shape(u_si)), float).ravel()
+    u = np.ascontiguousarray(np.asarray(u_si, float)).ravel()
+    shape = np.shape(u_si)
+    if np.any(u < gasrad_u_si(rho, T_MIN_SI)) or np.any(u > gasrad_u_si(rho, T_MAX_SI)):
+        raise ValueError("u outside the EOS bracket [1, 1e13] K")
+    a1 = 1.5 * K_B / (MU * M_P)
+    lt_min, lt_max = np.log(T_MIN_SI), np.log(T_MAX_SI)
+    out = np.empty_like(u)
+    for i in range(len(u)):
+        r, uu = float(rho[i]), float(u[i])
+        lo, hi = lt_min, lt_max
+        t0 = min(uu / a1, (uu * r / A_RAD) ** 0.25)
+        lt = math.log(min(max(t0, T_MIN_SI), T_MAX_SI))
+        ok = False
+        for _ in range(max_iter):
+            t = math.exp(lt)
+            t4 = t**4
+            f = a1 * t + A_RAD * t4 / r - uu
+            dfdl = a1 * t + 4.0 * A_RAD * t4 / r
+            if f < 0:
+                lo = lt
+            if f > 0:
+                hi = lt
+            new = lt - f / dfdl
+            if new <= lo or new >= hi or not math.isfinite(new):
+                new = 0.5 * (lo + hi)
+            step = abs(new - lt)
+            lt = new
+            if step < tol:
+                ok = True
+                break
+        if not ok:
+            raise RuntimeError("gas + radiation EOS: temperature did not converge")
+        out[i] = math.exp(lt)
+    return out.reshape(shape)
+
+
+def gasrad_state(rho_code, u_code, density_kg_m3):
+    """fastphys.py:96-111: (P [code], c_s [c], T [K], beta_gas) from code rho and u, one code
+    density unit being density_kg_m3; c_s = sqrt(Gamma_1 P / rho)."""
+    rho_si = np.asarray(rho_code, float) * density_kg_m3
+    u_si = np.asarray(u_code, float) * C**2
+    t = gasrad_temperature_si(rho_si, u_si)
+    p_si = gasrad_pressure_si(rho_si, t)
+    beta = gasrad_beta_si(rho_si, t)
+    p_code = p_si / (density_kg_m3 * C**2)
+    cs = np.sqrt(gamma1(beta) * p_code / np.asarray(rho_code, float))
+    return p_code, cs, t, beta
+
+
+def gasrad_u_floor(rho_code, t_floor, density_kg_m3):
+    """hydro.py:155-163 (gas + radiation branch): the code u of gas at t_floor at each density."""
+    return gasrad_u_si(np.asarray(rho_code, float) * density_kg_m3, t_floor) / C**2
 
 
 # ----------------------------------------------------------------- cooling

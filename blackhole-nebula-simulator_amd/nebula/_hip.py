@@ -76,6 +76,9 @@ SIGNATURES = {
     "nb_lists_build": (_i32, [_vp, _vp, _vp, _vp, _i64, _PD, _vp, _vp, _i64, _P64]),
     "nb_lists_emit": (_i32, [_vp, _vp, _vp, _i64]),
     "nb_tighten": (_i32, [_vp, _vp, _vp, _vp, _i64, _PD, _vp, _vp, _vp, _i64, _vp, _vp, _i64, _P64]),
+    "nb_reuse_displacements": (_i32, [_vp, _vp, _vp, _vp, _PD, _i64, _vp]),
+    "nb_reuse_limits": (_i32, [_vp, _vp, _vp, _vp, _f64, _vp, _vp, _i64, _vp, _vp, _vp]),
+    "nb_reuse_fast_rows": (_i32, [_vp, _vp, _vp, _vp, _i64, _vp, _PD, _vp, _vp, _i64, _vp]),
     "nb_tighten_inclusive": (_i32, [_vp, _vp, _vp, _vp, _i64, _PD, _vp, _vp, _vp, _i64, _vp, _vp, _i64, _P64]),
     "nb_density": (_i32, [_vp, _vp, _vp, _vp, _f64, _vp, _vp, _PL, _PD, _vp, _i64, _i32, _f64, _f64, _f64, _f64, _vp, _vp, _vp, _vp, _vp]),
     "nb_forces": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _PL, _PD, _vp, _i64, ctypes.POINTER(ForceParams), _vp, _vp, _vp, _vp, _vp]),

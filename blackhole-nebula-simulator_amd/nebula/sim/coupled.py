@@ -127,6 +127,24 @@ class GasState:
     def e_radiated(self) -> float:
         return 0.0 if self.e_radiated_dev is None else float(self.e_radiated_dev.item())
 
+    # pickling (checkpoints, nebula.sim.runio): device tensors travel as host arrays, bit for bit,
+    # and come back on the current GPU; the tight-list buffer is scratch and is not kept
+    def __getstate__(self):
+        state = {}
+        for k, v in self.__dict__.items():
+            if k == "list_buffer":
+                v = None
+            elif isinstance(v, torch.Tensor):
+                v = ("__device_tensor__", v.detach().cpu().numpy())
+            state[k] = v
+        return state
+
+    def __setstate__(self, state):
+        for k, v in state.items():
+            if isinstance(v, tuple) and len(v) == 2 and v[0] == "__device_tensor__":
+                v = torch.from_numpy(np.ascontiguousarray(v[1])).to(_hip.torch_device())
+            self.__dict__[k] = v
+
     def to_numpy(self) -> dict:
         """Host copies in the caller's particle order."""
         out = {}

@@ -51,14 +51,92 @@ class DensityResult:
 
 @dataclass
 class NeighbourCache:
-    """Accepted for signature compatibility (sph.py:62-82). Lists are rebuilt at every density
-    pass on the GPU (a build costs about one pair pass), so no cache is kept."""
+    """Neighbour lists kept between density passes ("Verlet lists", sph.py:62-82), on the
+    device: built at positions ``x_ref`` with search lengths ``h_search``, so they held every pair
+    with r < 2 max(hs_i, hs_j) then. ``compute_density`` reuses them while they provably still
+    hold every interacting pair (``_reuse_limits``), and rebuilds them otherwise."""
 
     neighbours: Neighbours
-    x_ref: np.ndarray
-    h_search: np.ndarray
+    x_ref: object
+    h_search: object
     builds: int = 1
     reuses: int = 0
+
+
+FAST_SET_SIZES = (0, 1, 2, 4, 8, 16, 32, 64)  # sph.py:114
+
+
+def reuse_limits_device(cache: NeighbourCache, pos_d, h_guess_d, role, box_arr):
+    """Per-particle search limits h_eff (device) with which the cached lists are provably
+    complete, or None when they must be rebuilt (sph.py:117-147). ``role`` (uint8 device, N):
+    bit 0 marks the rows being solved, bit 1 the others (whose h stays h_guess).
+
+    The moves d since the build are measured on the device; fast sets F of the largest movers
+    are tried in growing sizes: h_eff = hs - (d + d_rest) / 2 with d_rest the largest move outside
+    F; rows need h_eff > h_guess / 2, the others h_guess <= h_eff; the rows of F are then checked
+    against every particle at the current positions (h_lim = h_eff for rows, h_guess for the
+    others). Moves are ordered by size, ties by index: a tie cannot change h_eff, which takes the
+    value of the move, only which of the equal movers is checked directly."""
+    n = int(pos_d.shape[0])
+    x_ref = _dev.f64(cache.x_ref)
+    hs = _dev.f64(cache.h_search)
+    disp = _dev.empty(n)
+    _hip.call("nb_reuse_displacements", _hip.ptr(pos_d), _hip.ptr(x_ref), _hip.box_arg(box_arr), n, _hip.ptr(disp))
+    top = min(n, FAST_SET_SIZES[-1] + 1)
+    vals, order = torch.topk(disp, top, largest=True, sorted=True)
+    vals_h, order_h = vals.cpu().numpy(), order.cpu().numpy()
+    perm = np.lexsort((order_h, -vals_h))
+    order_h, d_sorted = order_h[perm], vals_h[perm]
+    h_eff, h_lim = _dev.empty(n), _dev.empty(n)
+    bad = torch.zeros(2, dtype=torch.int32, device=pos_d.device)
+    missing = torch.zeros(1, dtype=torch.int32, device=pos_d.device)
+    nb = cache.neighbours
+    for nf in FAST_SET_SIZES:
+        if nf >= top:
+            break
+        bad.zero_()
+        _hip.call("nb_reuse_limits", _hip.ptr(hs), _hip.ptr(disp), float(d_sorted[nf]), _hip.ptr(h_guess_d),
+                  _hip.ptr(role), n, _hip.ptr(h_eff), _hip.ptr(h_lim), _hip.ptr(bad))  # fmt: skip
+        if int(bad.sum().item()):
+            continue
+        if nf == 0:
+            return h_eff
+        fast = _dev.i64(order_h[:nf])
+        missing.zero_()
+        _hip.call("nb_reuse_fast_rows", _hip.ptr(pos_d), _hip.ptr(fast), nf, _hip.ptr(h_lim), _hip.box_arg(box_arr),
+                  _hip.ptr(nb.d_offset), _hip.ptr(nb.d_index), n, _hip.ptr(missing))  # fmt: skip
+        if int(missing.item()) == 0:
+            return h_eff
+    return None
+
+
+def _roles(rows_d, others, n, device):
+    role = torch.zeros(n, dtype=torch.uint8, device=device)
+    if rows_d is None:
+        role.fill_(1)
+    else:
+        role[rows_d] = 1
+    if others is None:
+        role[role == 0] = 2
+    else:
+        o = torch.as_tensor(np.asarray(others, bool) if not isinstance(others, torch.Tensor) else others,
+                            device=device).to(torch.bool)  # fmt: skip
+        role[o] |= 2
+    return role
+
+
+def _reuse_limits(cache: NeighbourCache, pos, h_guess, rows, others, box_arr):
+    """sph.py:117-147 with the reference's arguments: ``rows`` the particles being solved,
+    ``others`` a boolean mask of those whose h is fixed (normally the complement). Returns h_eff
+    (host array for host positions, device tensor for device ones) or None."""
+    as_dev = _dev.is_dev(pos)
+    pos_d = _dev.f64(pos)
+    n = int(pos_d.shape[0])
+    rows_d, _ = _dev.rows_dev(rows, n)
+    role = _roles(rows_d, others, n, pos_d.device)
+    box = None if box_arr is None else np.broadcast_to(np.asarray(box_arr, np.float64), (3,)).copy()
+    h_eff = reuse_limits_device(cache, pos_d, _dev.f64(h_guess), role, box)
+    return None if h_eff is None else _out(h_eff, as_dev)
 
 
 def _box3(box):
@@ -97,15 +175,39 @@ def density_numba(pos, mass, h_start, nb: Neighbours, h_search, kernel_name=DEFA
 
 def compute_density(pos, mass, h_guess, box=None, kernel_name=DEFAULT_KERNEL, eta=DEFAULT_ETA, tol=1e-4,
                     h_max=np.inf, skin=1.4, cache=None, rows=None) -> DensityResult:  # fmt: skip
-    """Density pass with neighbour lists built at search radius 2 min(h_guess, h_max) x skin
-    (sph.py:329-387); particles whose h outgrows the search are redone with rebuilt lists,
-    doubling their search (up to 12 rounds). ``cache`` is accepted and ignored: lists are
-    rebuilt at every call."""
+    """Density pass with neighbour lists (sph.py:329-387): reused from ``cache`` while provably
+    still complete (``_reuse_limits``), else built at search radius 2 min(h_guess, h_max) x skin;
+    particles whose h outgrows the search are redone with rebuilt lists, doubling their search
+    (up to 12 rounds). The result carries the cache for the next call. ``rows``: solve only these
+    particles (results in that order); the lists stay complete for the others too."""
     as_dev = _dev.is_dev(pos)
     pos_d = _dev.f64(pos)
     mass_d = _dev.f64(mass)
-    res = compute_density_device(pos_d, mass_d, _dev.f64(h_guess, copy=True), _box3(box), kernel_name, eta,
-                                 tol, h_max, skin, rows)  # fmt: skip
+    h_guess_d = _dev.f64(h_guess, copy=True)
+    box_arr = _box3(box)
+    n = int(pos_d.shape[0])
+    res = None
+    if cache is not None and len(cache.x_ref) == n and n:
+        rows_d, nrows = _dev.rows_dev(rows, n)
+        h_eff = reuse_limits_device(cache, pos_d, h_guess_d, _roles(rows_d, None, n, pos_d.device), box_arr)
+        if h_eff is not None:
+            h, rho, omega, it, fl = density_device(pos_d, mass_d, h_guess_d, cache.neighbours, h_eff, kernel_name,
+                                                   eta, tol, h_max, rows_d, nrows)  # fmt: skip
+            if not bool((fl == FLAG_SEARCH_TOO_SMALL).any().item()):
+                cache.reuses += 1
+                res = DensityResult(h, rho, omega, it, fl, cache.neighbours, cache, cache.builds)
+            else:
+                # rows whose h outgrew the reuse limit: the rebuild starts from at least their h
+                if rows_d is None:
+                    h_guess_d = torch.maximum(h_guess_d, h)
+                else:
+                    h_guess_d[rows_d] = torch.maximum(h_guess_d[rows_d], h)
+    if res is None:
+        res = compute_density_device(pos_d, mass_d, h_guess_d, box_arr, kernel_name, eta, tol, h_max, skin, rows,
+                                     keep_search=True)  # fmt: skip
+        prior = (0, 0) if cache is None else (cache.builds, cache.reuses)
+        res.cache = NeighbourCache(res.neighbours, pos_d.clone(), res.cache, prior[0] + res.builds, prior[1])
+        res.builds = res.cache.builds
     if not as_dev:
         for k in ("h", "rho", "omega", "iterations", "flags"):
             setattr(res, k, _dev.host(getattr(res, k)))
@@ -113,9 +215,11 @@ def compute_density(pos, mass, h_guess, box=None, kernel_name=DEFAULT_KERNEL, et
 
 
 def compute_density_device(pos_d, mass_d, h_guess_d, box_arr, kernel_name, eta, tol, h_max, skin, rows=None,
-                           m_uniform=None, capacity_hint=0, blocked=False) -> DensityResult:  # fmt: skip
+                           m_uniform=None, capacity_hint=0, blocked=False, keep_search=False) -> DensityResult:  # fmt: skip
     """compute_density on device tensors (h_guess_d is overwritten by the rebuild rounds).
-    ``blocked``: build the lists in 64-row blocks (the step's own layout) instead of CSR."""
+    ``blocked``: build the lists in 64-row blocks (the step's own layout) instead of CSR.
+    ``keep_search``: the result's ``cache`` field holds the final search lengths (a
+    NeighbourCache's h_search)."""
     n = int(pos_d.shape[0])
     rows_d, nrows = _dev.rows_dev(rows, n)
     box_cap = np.inf if box_arr is None else 0.2499 * float(np.min(box_arr))
@@ -137,7 +241,7 @@ def compute_density_device(pos_d, mass_d, h_guess_d, box_arr, kernel_name, eta, 
         _hip.call("nb_grow_search", _hip.ptr(fl), _hip.ptr(h), _hip.ptr(rows_d), nrows, float(box_cap),
                   _hip.ptr(h_search), _hip.ptr(h_guess_d), grown.ctypes.data_as(_hip._P64))  # fmt: skip
         if grown[0] == 0:
-            return DensityResult(h, rho, omega, it, fl, nb, None, builds)
+            return DensityResult(h, rho, omega, it, fl, nb, h_search if keep_search else None, builds)
         capacity_hint = max(capacity_hint, nb.row_cap if blocked else nb.n_entries)
     raise RuntimeError("smoothing lengths kept outgrowing the neighbour search")
 

@@ -143,6 +143,26 @@ int32_t nb_permute_i64(nb_ctx *ctx, void *stream, const int64_t *src, const int3
 int32_t nb_permute_many(nb_ctx *ctx, void *stream, int32_t count, const void *const *src, void *const *dst,
                         const int32_t *width, const int32_t *order, int64_t n);
 
+/* Reuse of cached neighbour lists (sph.py:62-147, NeighbourCache / _reuse_limits /
+ * _fast_rows_complete): the exact test that lists built at x_ref with search lengths h_search
+ * still hold every interacting pair.
+ * nb_reuse_displacements: disp[i] = |minimum_image(pos_i - x_ref_i)| (box3 NULL: open).
+ * nb_reuse_limits: h_eff = h_search - (disp + d_rest) / 2; bad_dev (two int32, zeroed by the
+ *   caller) counts solved rows (role bit 0) with h_eff <= h_guess / 2 and the other particles
+ *   (role bit 1) with h_guess > h_eff (sph.py:134-137); h_lim = h_guess for role bit 1, else
+ *   h_eff (sph.py:142-143).
+ * nb_reuse_fast_rows: for each particle j of fast (int64, nfast) and each k with
+ *   r_jk <= 2 max(h_lim_j, h_lim_k) now, k must be in j's sorted CSR row; *missing_dev (int32,
+ *   zeroed by the caller) counts the pairs that are not (sph.py:86-111). */
+int32_t nb_reuse_displacements(nb_ctx *ctx, void *stream, const double *pos, const double *x_ref, const double *box3,
+                               int64_t n, double *disp);
+int32_t nb_reuse_limits(nb_ctx *ctx, void *stream, const double *h_search, const double *disp, double d_rest,
+                        const double *h_guess, const uint8_t *role, int64_t n, double *h_eff, double *h_lim,
+                        int32_t *bad_dev);
+int32_t nb_reuse_fast_rows(nb_ctx *ctx, void *stream, const double *pos, const int64_t *fast, int64_t nfast,
+                           const double *h_lim, const double *box3, const int64_t *offset, const int32_t *index,
+                           int64_t n, int32_t *missing_dev);
+
 /* tighten (neighbours.py:306-323): keep pairs with |x_i - x_j| < 2 max(h_i, h_j); rows not in
  * ``rows`` are left empty. Same capacity protocol (the input list total always suffices). */
 int32_t nb_tighten(nb_ctx *ctx, void *stream, const double *pos, const double *h, int64_t n,

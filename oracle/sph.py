@@ -10,7 +10,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import lib
-from .neighbours import Neighbours, find_neighbours_grid
+from .neighbours import Neighbours, find_neighbours_grid, minimum_image
 
 KERNEL_IDS = {"cubic": 0, "wendland_c4": 1}  # kernels_nb.py:12
 C_NORM = np.array([1.0 / np.pi, 495.0 / (256.0 * np.pi)])  # kernels.py:94-95
@@ -26,6 +26,59 @@ class DensityResult:
     flags: np.ndarray
     neighbours: Neighbours
     builds: int = 1
+    cache: object = None
+
+
+@dataclass
+class NeighbourCache:
+    """sph.py:62-82: lists built at x_ref with search lengths h_search."""
+
+    neighbours: Neighbours
+    x_ref: np.ndarray
+    h_search: np.ndarray
+    builds: int = 1
+    reuses: int = 0
+
+
+FAST_SET_SIZES = (0, 1, 2, 4, 8, 16, 32, 64)  # sph.py:114
+
+
+def fast_rows_complete(pos, fast, h_lim, box_arr, nb: Neighbours) -> bool:
+    """sph.py:86-111: for every j in fast, every k with r_jk <= 2 max(h_lim_j, h_lim_k) at pos is
+    in j's list (brute force over all k; per-component squares summed in order x, y, z)."""
+    pos = np.asarray(pos, float)
+    for j in fast:
+        d = minimum_image(pos - pos[j], None if box_arr is None else np.asarray(box_arr, float))
+        d2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+        rr = 2.0 * np.maximum(h_lim[j], h_lim)
+        need = np.nonzero(d2 <= rr * rr)[0]
+        if not np.all(np.isin(need, nb.of(int(j)))):
+            return False
+    return True
+
+
+def reuse_limits(cache: NeighbourCache, pos, h_guess, rows, others, box_arr):
+    """sph.py:117-147: h_eff with which the cached lists are provably complete, or None. Moves
+    are ordered by size, ties by index (the reference leaves the order of ties to argpartition;
+    a tie changes no h_eff, only which of the equal movers is checked directly)."""
+    d = minimum_image(np.asarray(pos, float) - cache.x_ref, None if box_arr is None else np.asarray(box_arr, float))
+    disp = np.sqrt((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2])
+    n = len(disp)
+    top = min(n, FAST_SET_SIZES[-1] + 1)
+    order = np.lexsort((np.arange(n), -disp))[:top]
+    for nf in FAST_SET_SIZES:
+        if nf >= len(order):
+            break
+        h_eff = cache.h_search - 0.5 * (disp + disp[order[nf]])
+        if not (np.all(h_eff[rows] > 0.5 * h_guess[rows]) and np.all(h_guess[others] <= h_eff[others])):
+            continue
+        if nf == 0:
+            return h_eff
+        h_lim = h_eff.copy()
+        h_lim[others] = h_guess[others]
+        if fast_rows_complete(pos, order[:nf], h_lim, box_arr, cache.neighbours):
+            return h_eff
+    return None
 
 
 @dataclass
@@ -79,17 +132,30 @@ def density_numba(pos, mass, h_start, nb: Neighbours, h_search, kernel_name="cub
 
 
 def compute_density(pos, mass, h_guess, box=None, kernel_name="cubic", eta=1.2, tol=1e-4,
-                    h_max=np.inf, skin=1.4, rows=None) -> DensityResult:  # fmt: skip
-    """sph.py:329-387 with no neighbour cache (lists rebuilt at every call, the policy the
-    GPU path follows too): search 2 min(h_guess, h_max) skin (capped at a quarter periodic
-    box), redo flagged rows with doubled search, up to 12 rounds."""
+                    h_max=np.inf, skin=1.4, rows=None, cache=None) -> DensityResult:  # fmt: skip
+    """sph.py:329-387: lists reused from ``cache`` while reuse_limits proves them complete,
+    else built at search 2 min(h_guess, h_max) skin (capped at a quarter periodic box); flagged
+    rows are redone with doubled search, up to 12 rounds. The result carries the cache
+    (``builds`` counts the builds since the first, as the reference's NeighbourCache does)."""
     h_guess = np.asarray(h_guess, float)
     n_all = len(pos)
     rows = _rows(rows, n_all)
+    others = np.ones(n_all, bool)
+    others[rows] = False
     box_arr = None if box is None else np.broadcast_to(np.asarray(box, float), (3,))
     box_cap = np.inf if box_arr is None else 0.2499 * float(np.min(box_arr))
+    if cache is not None and len(cache.x_ref) == n_all and n_all:
+        h_eff = reuse_limits(cache, pos, h_guess, rows, others, box_arr)
+        if h_eff is not None:
+            res = density_numba(pos, mass, h_guess, cache.neighbours, h_eff, kernel_name, eta, tol, h_max, rows)
+            if not np.any(res.flags == FLAG_SEARCH_TOO_SMALL):
+                cache.reuses += 1
+                res.cache, res.builds = cache, cache.builds
+                return res
+            h_guess = h_guess.copy()
+            h_guess[rows] = np.maximum(h_guess[rows], res.h)
     h_search = np.minimum(np.minimum(h_guess, h_max) * skin, box_cap)
-    builds = 0
+    builds = 0 if cache is None else cache.builds
     for _ in range(12):
         nb = find_neighbours_grid(pos, h_search, box=box_arr)
         builds += 1
@@ -97,6 +163,8 @@ def compute_density(pos, mass, h_guess, box=None, kernel_name="cubic", eta=1.2, 
         grow = res.flags == FLAG_SEARCH_TOO_SMALL
         if not np.any(grow):
             res.builds = builds
+            reuses = 0 if cache is None else cache.reuses
+            res.cache = NeighbourCache(nb, np.array(pos, float), h_search.copy(), builds, reuses)
             return res
         h_search = h_search.copy()
         h_search[rows[grow]] = np.minimum(h_search[rows[grow]] * 2.0, box_cap)

@@ -1224,25 +1224,57 @@ __global__ void k_pack_posh(const double* __restrict__ pos, const double* __rest
     if (i < n) out[i] = make_double4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], h[i]);
 }
 
-// The z-cells [z0, z1] that a sphere (centre at cell coordinates ux, uy and height zc, radius^2
-// r2) reaches in grid column (cx, cy) of an open grid; false when it misses the column. The x-y
-// distance to the column is shortened by 1e-9 of a cell and the z half-extent lengthened by as
-// much, so rounding can only widen the range (extra candidates fail the exact predicate).
-__device__ __forceinline__ bool column_zrange(double ux, double uy, double zc, double r2, int64_t cx, int64_t cy,
-                                              const Grid& g, int64_t& z0, int64_t& z1)
+// The z cells [z0, z1] that a search sphere reaches in grid column (cx, cy), in FP32 cell units
+// (centre ux, uy, uz; radius rc in x-y cells; zr z cells per x-y cell); false when it misses the
+// column. The caller widens the radius to rc (1 + 1e-5) + 1e-3 and the x-y distances here are
+// shortened by 1e-3 cells, far beyond the FP32 rounding of these values (below 1e-3 cells for
+// grids of up to 2048 cells a side, relative 1e-7 in rc^2), so the range only grows: extra
+// candidates fail the exact FP64 test.
+__device__ __forceinline__ bool column_zrange32(float ux, float uy, float uz, float rc, float zr, int32_t cx, int32_t cy,
+                                                int32_t nz, int32_t& z0, int32_t& z1)
 {
-    const double cell = 1.0 / g.inv[0];
-    const double ex = fmax(fmax((double)cx - ux, ux - (double)(cx + 1)) - 1e-9, 0.0) * cell;
-    const double ey = fmax(fmax((double)cy - uy, uy - (double)(cy + 1)) - 1e-9, 0.0) * cell;
-    const double rest = r2 - (ex * ex + ey * ey);
-    if (!(rest > 0.0)) return false;
-    const double hz = sqrt(rest) + 1e-9 * cell;
-    const double fa = fmax(floor((zc - hz - g.lo[2]) * g.inv[2]), 0.0);
-    const double fb = fmin(floor((zc + hz - g.lo[2]) * g.inv[2]), (double)(g.nc[2] - 1));
+    const float ex = fmaxf(fmaxf((float)cx - ux, ux - (float)(cx + 1)) - 1e-3f, 0.f);
+    const float ey = fmaxf(fmaxf((float)cy - uy, uy - (float)(cy + 1)) - 1e-3f, 0.f);
+    const float rest = rc * rc - (ex * ex + ey * ey);
+    if (!(rest > 0.f)) return false;
+    const float hz = (sqrt32(rest) + 2e-3f) * zr + 4e-3f;
+    const float fa = fmaxf(floorf(uz - hz), 0.f);
+    const float fb = fminf(floorf(uz + hz), (float)(nz - 1));
     if (fb < fa) return false;
-    z0 = (int64_t)fa;
-    z1 = (int64_t)fb;
+    z0 = (int32_t)fa;
+    z1 = (int32_t)fb;
     return true;
+}
+
+// Column c of an nx x ny block of columns as (c / ny, c % ny), for c < 2^22 (the float quotient
+// is off by at most one, which the remainder corrects)
+__device__ __forceinline__ void column_of(uint32_t c, uint32_t ny, float inv_ny, int32_t& qx, int32_t& qy)
+{
+    qx = (int32_t)((float)c * inv_ny);
+    qy = (int32_t)c - qx * (int32_t)ny;
+    if (qy < 0) {
+        qx--;
+        qy += (int32_t)ny;
+    } else if (qy >= (int32_t)ny) {
+        qx++;
+        qy -= (int32_t)ny;
+    }
+}
+
+// f(q) and f'(q) of one kernel (kKid 0 cubic spline, 1 Wendland C4), as kernel_fdf32
+template <int kKid>
+__device__ __forceinline__ void kernel_fdf32_k(float q, float& fq, float& dfq)
+{
+    if (kKid == 0) {
+        const float t2 = fmaxf(2.f - q, 0.f), t1 = fmaxf(1.f - q, 0.f);
+        const float s2 = t2 * t2, s1 = t1 * t1;
+        fq = __builtin_fmaf(0.25f * s2, t2, -(s1 * t1));
+        dfq = __builtin_fmaf(3.f, s1, -0.75f * s2);
+    } else {
+        const float x = fmaxf(1.f - 0.5f * q, 0.f), x2 = x * x;
+        fq = x2 * x2 * x2 * (35.f / 12.f * q * q + 3.f * q + 1.f);
+        dfq = -7.f / 3.f * q * x2 * x2 * x * (5.f * q + 2.f);
+    }
 }
 
 // Extras (pairs seen only from the other side, d2 >= (2 h_search_j)^2) go straight to row j's
@@ -1253,8 +1285,9 @@ __device__ __forceinline__ bool column_zrange(double ux, double uy, double zc, d
 // and first cell come from segs. Rows hold neighbour indices only (4 bytes per entry): the passes
 // that read them form d2 again from the positions, in the same FP64 operations, so they see the
 // same d2 bits. kMasses (masses differ): row i stores its particle's FP32 mass in rmass[i], which
-// those passes read by index.
-template <bool kMasses, bool kSeg>
+// those passes read by index. The first-step sums take every candidate's FP32 term (zero off the
+// row) into FP64 per lane, branch-free.
+template <bool kMasses, bool kSeg, int kKid>
 __global__ void __launch_bounds__(256, 5)
 k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const GridSeg* __restrict__ segs,
                const int32_t* __restrict__ cloud, const uint32_t* __restrict__ start,
@@ -1277,14 +1310,19 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const Gr
     int64_t ax, nx, ay, ny;
     cell_range(xi, r, g, 0, false, ax, nx);
     cell_range(yi, r, g, 1, false, ay, ny);
-    const int64_t ncol = live ? nx * ny : 0;
-    const double ux = (xi - g.lo[0]) * g.inv[0], uy = (yi - g.lo[1]) * g.inv[1];
+    const uint32_t ncol = live ? (uint32_t)(nx * ny) : 0u;
+    const uint32_t ny32 = (uint32_t)ny;
+    const float inv_ny = 1.f / (float)(ny > 0 ? ny : 1);
+    const float ux = (float)((xi - g.lo[0]) * g.inv[0]), uy = (float)((yi - g.lo[1]) * g.inv[1]);
+    const float uz = (float)((zi - g.lo[2]) * g.inv[2]);
+    const float rc = (float)(r * g.inv[0]) * (1.f + 1e-5f) + 1e-3f;
+    const float zr = (float)(g.inv[2] / g.inv[0]);
     const uint32_t nc1 = (uint32_t)g.nc[1], nc2 = (uint32_t)g.nc[2];
     const double r_lim = pymin(hs, A.h_max);
     const double h0 = pymin(A.h_start[ii], r_lim);
     const float inv_h0 = (float)(1.0 / h0);
     const float mu = (float)(A.m_uniform > 0.0 ? A.m_uniform : 1.0);
-    DensSums sums;
+    double s1 = 0.0, s2 = 0.0;  // per-lane first-step sums m f(q) and m (3 f + q f')
     int32_t m = 0;
     unsigned int emax = 0, eover = 0;
     const unsigned long long gb0 = goff[ii];
@@ -1294,20 +1332,28 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const Gr
     uint32_t* tab = ctab[threadIdx.x / kG];
     // every loop below runs the same number of times on all lanes of the wave (ballot bounds),
     // so the shuffles always see their whole group active
-    for (int64_t c0 = 0;; c0 += kCols) {
+    for (uint32_t c0 = 0;; c0 += kCols) {
         if (__ballot(c0 < ncol) == 0) break;
         uint32_t k0a = 0, lena = 0, k0b = 0, lenb = 0;
-        int64_t z0, z1;
-        const int64_t ca = c0 + glane, cb = c0 + kG + glane;
-        if (ca < ncol && column_zrange(ux, uy, zi, r2, ax + ca / ny, ay + ca % ny, g, z0, z1)) {
-            const uint32_t base = cbase + ((uint32_t)(ax + ca / ny) * nc1 + (uint32_t)(ay + ca % ny)) * nc2;
-            k0a = start[base + (uint32_t)z0];
-            lena = start[base + (uint32_t)z1 + 1u] - k0a;
+        int32_t z0, z1, qx, qy;
+        const uint32_t ca = c0 + glane, cb = c0 + kG + glane;
+        if (ca < ncol) {
+            column_of(ca, ny32, inv_ny, qx, qy);
+            const int32_t cx = (int32_t)ax + qx, cy = (int32_t)ay + qy;
+            if (column_zrange32(ux, uy, uz, rc, zr, cx, cy, (int32_t)nc2, z0, z1)) {
+                const uint32_t base = cbase + ((uint32_t)cx * nc1 + (uint32_t)cy) * nc2;
+                k0a = start[base + (uint32_t)z0];
+                lena = start[base + (uint32_t)z1 + 1u] - k0a;
+            }
         }
-        if (cb < ncol && column_zrange(ux, uy, zi, r2, ax + cb / ny, ay + cb % ny, g, z0, z1)) {
-            const uint32_t base = cbase + ((uint32_t)(ax + cb / ny) * nc1 + (uint32_t)(ay + cb % ny)) * nc2;
-            k0b = start[base + (uint32_t)z0];
-            lenb = start[base + (uint32_t)z1 + 1u] - k0b;
+        if (cb < ncol) {
+            column_of(cb, ny32, inv_ny, qx, qy);
+            const int32_t cx = (int32_t)ax + qx, cy = (int32_t)ay + qy;
+            if (column_zrange32(ux, uy, uz, rc, zr, cx, cy, (int32_t)nc2, z0, z1)) {
+                const uint32_t base = cbase + ((uint32_t)cx * nc1 + (uint32_t)cy) * nc2;
+                k0b = start[base + (uint32_t)z0];
+                lenb = start[base + (uint32_t)z1 + 1u] - k0b;
+            }
         }
         uint32_t ia = lena, ib = lenb;  // inclusive prefix sums over the group
         for (int o = 1; o < kG; o <<= 1) {
@@ -1347,12 +1393,12 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const Gr
             bool pred = false, flag = false;
             uint32_t j = 0;
             double d2 = 0.0;
-            float mj = 0.f;
+            double mjd = 0.0;
             if (v < total) {
                 const uint32_t k = kc;
                 j = order ? order[k] : k;
                 const double4 pj = posh[j];
-                if (kMasses) mj = (float)A.mass[j];
+                if (kMasses) mjd = A.mass[j];
                 const double dx = pj.x - xi;
                 const double dy = pj.y - yi;
                 const double dz = pj.z - zi;
@@ -1367,7 +1413,6 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const Gr
             const unsigned gb = (unsigned)((bal >> gshift) & ((1ull << kG) - 1));
             if (pred) {
                 const int slot = m + __popc(gb & ((1u << glane) - 1));
-                const float d2f = (float)d2;
                 if (flag) {
                     const unsigned int es = atomicAdd(&efill[j], 1u);
                     const unsigned long long eb = eoff[j];
@@ -1376,8 +1421,14 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const Gr
                     emax = es + 1 > emax ? es + 1 : emax;
                 }
                 if (slot < cap) row[slot] = j;
-                sums.add(sqrt32(d2f) * inv_h0, kMasses ? mj : mu, A.kid);
             }
+            // the first-step sums on every lane: terms off the row add an exact zero
+            const float q = sqrt32((float)d2) * inv_h0;
+            float fq, dfq;
+            kernel_fdf32_k<kKid>(q, fq, dfq);
+            const float mj = kMasses ? (float)mjd : mu;
+            s1 += pred ? (double)(mj * fq) : 0.0;
+            s2 += pred ? (double)(mj * __builtin_fmaf(q, dfq, 3.f * fq)) : 0.0;
             m += __popc(gb);
         }
     }
@@ -1386,8 +1437,7 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const Gr
         if (kMasses) rmass[i] = (float)A.mass[i];  // the FP32 mass the sums of later passes take
     }
     // the loop's first pass at h0 (sph.py:260-290), the Newton update and convergence test in FP64
-    double S1, S2;
-    sums.total(S1, S2);
+    const double S1 = group_sum<kG>(s1), S2 = group_sum<kG>(s2);
     const double m_i = A.mass[ii];
     const double mscale = kMasses ? 1.0 : (A.m_uniform > 0.0 ? A.m_uniform / (double)mu : 1.0);
     const double inv_h = 1.0 / h0;
@@ -2856,14 +2906,21 @@ extern "C" int32_t nb_hydro_density_mixed(nb_ctx* ctx, void* stream, const doubl
             return NB_ERR_HIP;
         if (hipMemsetAsync(ctx->efill, 0, sizeof(unsigned int) * n, s) != hipSuccess) return ctx_fail(ctx, "memset");
         float* rmass = masses ? ctx->fscratch : nullptr;
-#define NB_GATHER_MIXED(M, S)                                                                                        \
-    NB_TIMED(ctx, s, "k_gather_mixed", (k_gather_mixed<M, S>), dim3(blocks), dim3(tile), 0, s,                      \
+#define NB_GATHER_MIXED(M, S, K)                                                                                     \
+    NB_TIMED(ctx, s, "k_gather_mixed", (k_gather_mixed<M, S, K>), dim3(blocks), dim3(tile), 0, s,                   \
              (const double4*)ctx->posh, n, g, segs, cloud, ctx->cell_start, order_arg(ctx), ctx->gbuf, rmass,       \
              ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->ebuf, ctx->rm_eo, A)
-        if (masses && segmented) NB_GATHER_MIXED(true, true);
-        else if (masses) NB_GATHER_MIXED(true, false);
-        else if (segmented) NB_GATHER_MIXED(false, true);
-        else NB_GATHER_MIXED(false, false);
+        if (kid == 0) {
+            if (masses && segmented) NB_GATHER_MIXED(true, true, 0);
+            else if (masses) NB_GATHER_MIXED(true, false, 0);
+            else if (segmented) NB_GATHER_MIXED(false, true, 0);
+            else NB_GATHER_MIXED(false, false, 0);
+        } else {
+            if (masses && segmented) NB_GATHER_MIXED(true, true, 1);
+            else if (masses) NB_GATHER_MIXED(true, false, 1);
+            else if (segmented) NB_GATHER_MIXED(false, true, 1);
+            else NB_GATHER_MIXED(false, false, 1);
+        }
 #undef NB_GATHER_MIXED
         // [0] longest gather row, [1] rows left slow, [2] extras overflow, [3] rows over capacity,
         // [4] longest extras segment
@@ -3211,14 +3268,15 @@ extern "C" int32_t nb_hydro_density_mixed_fixed(nb_ctx* ctx, void* stream, const
     const GridSeg* segs = (const GridSeg*)ctx->segs;
     float* rmass = masses ? ctx->fscratch : nullptr;
     const Grid g_none{};
-    if (masses)
-        NB_TIMED(ctx, s, "k_gather_mixed", (k_gather_mixed<true, true>), dim3(blocks), dim3(tile), 0, s,
-                 (const double4*)ctx->posh, n, g_none, segs, cloud, ctx->cell_start, (const uint32_t*)nullptr, ctx->gbuf,
-                 rmass, ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->ebuf, ctx->rm_eo, A);
-    else
-        NB_TIMED(ctx, s, "k_gather_mixed", (k_gather_mixed<false, true>), dim3(blocks), dim3(tile), 0, s,
-                 (const double4*)ctx->posh, n, g_none, segs, cloud, ctx->cell_start, (const uint32_t*)nullptr, ctx->gbuf,
-                 rmass, ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->ebuf, ctx->rm_eo, A);
+#define NB_GATHER_FIXED(M, K)                                                                                        \
+    NB_TIMED(ctx, s, "k_gather_mixed", (k_gather_mixed<M, true, K>), dim3(blocks), dim3(tile), 0, s,                \
+             (const double4*)ctx->posh, n, g_none, segs, cloud, ctx->cell_start, (const uint32_t*)nullptr, ctx->gbuf, \
+             rmass, ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->ebuf, ctx->rm_eo, A)
+    if (masses && kid == 0) NB_GATHER_FIXED(true, 0);
+    else if (masses) NB_GATHER_FIXED(true, 1);
+    else if (kid == 0) NB_GATHER_FIXED(false, 0);
+    else NB_GATHER_FIXED(false, 1);
+#undef NB_GATHER_FIXED
     // [2] extras overflow, [3] rows over capacity
     finish_partials(ctx, s, blocks, 5, 1u | 4u | 16u, 0);
     hipLaunchKernelGGL(k_status_from, dim3(1), dim3(64), 0, s, (const long long*)ctx->scal, 2, status, (int)kStExtras);

@@ -57,14 +57,17 @@ __global__ void k_alpha_balsara(const double* __restrict__ div, const double* __
     alpha[i] = alpha_const * (a / (a + fabs(curl[i]) + eps * cs[i] / h[i]));
 }
 
-// Cullen-Dehnen switch (viscosity.py:82-106): xi, alpha_loc, decay time, backward-Euler decay
+// Cullen-Dehnen switch (viscosity.py:82-106): xi, alpha_loc, decay time, backward-Euler decay;
+// the step dt, or dt_arr[i] when given (block time steps: each row its own step)
 __global__ void k_alpha_cd(const double* __restrict__ div, const double* __restrict__ curl,
                            const double* __restrict__ ddiv, const double* __restrict__ cs,
-                           const double* __restrict__ h, const double* __restrict__ alpha_in, double dt,
-                           double alpha_max, double sigma, int64_t n, double* __restrict__ alpha_out)
+                           const double* __restrict__ h, const double* __restrict__ alpha_in, double dt_one,
+                           const double* __restrict__ dt_arr, double alpha_max, double sigma, int64_t n,
+                           double* __restrict__ alpha_out)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const double dt = dt_arr ? dt_arr[i] : dt_one;
     double d2 = div[i] * div[i];
     double c2 = curl[i] * curl[i];
     double xi = d2 + c2 > 0 ? d2 / (d2 + c2) : 0.0;
@@ -229,8 +232,19 @@ extern "C" int32_t nb_alpha_switch(nb_ctx* ctx, void* stream, const double* div,
     if (!ctx || n < 0) return ctx_fail(ctx, "nb_alpha_switch: bad arguments");
     if (n == 0) return NB_OK;
     hipLaunchKernelGGL(k_alpha_cd, dim3(grid_g(n)), dim3(kBlockG), 0, (hipStream_t)stream, div, curl, ddiv, cs, h,
-                       alpha_in, dt, alpha_max, sigma, n, alpha_out);
+                       alpha_in, dt, (const double*)nullptr, alpha_max, sigma, n, alpha_out);
     return launched(ctx, "nb_alpha_switch");
+}
+
+extern "C" int32_t nb_alpha_switch_dt(nb_ctx* ctx, void* stream, const double* div, const double* curl,
+                                      const double* ddiv, const double* cs, const double* h, const double* alpha_in,
+                                      const double* dt, double alpha_max, double sigma, int64_t n, double* alpha_out)
+{
+    if (!ctx || n < 0 || (n > 0 && !dt)) return ctx_fail(ctx, "nb_alpha_switch_dt: bad arguments");
+    if (n == 0) return NB_OK;
+    hipLaunchKernelGGL(k_alpha_cd, dim3(grid_g(n)), dim3(kBlockG), 0, (hipStream_t)stream, div, curl, ddiv, cs, h,
+                       alpha_in, 0.0, dt, alpha_max, sigma, n, alpha_out);
+    return launched(ctx, "nb_alpha_switch_dt");
 }
 
 extern "C" int32_t nb_ns_stress(nb_ctx* ctx, void* stream, const double* g, const double* rho, const double* cs,

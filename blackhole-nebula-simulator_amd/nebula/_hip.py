@@ -76,6 +76,18 @@ SIGNATURES = {
     "nb_lists_build": (_i32, [_vp, _vp, _vp, _vp, _i64, _PD, _vp, _vp, _i64, _P64]),
     "nb_lists_emit": (_i32, [_vp, _vp, _vp, _i64]),
     "nb_tighten": (_i32, [_vp, _vp, _vp, _vp, _i64, _PD, _vp, _vp, _vp, _i64, _vp, _vp, _i64, _P64]),
+    "nb_alpha_switch_dt": (_i32, [_vp] * 9 + [_f64, _f64, _i64, _vp]),
+    "nb_block_elapsed": (_i32, [_vp, _vp, _vp, _vp, _i64, _f64, _i64, _vp, _vp]),
+    "nb_block_predict": (_i32, [_vp, _vp, _vp, _vp, _i64, _f64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f64, _vp, _f64,
+                                _i64, _vp, _vp, _vp, _vp, _vp]),
+    "nb_block_close": (_i32, [_vp, _vp, _vp, _i64, _vp, _vp, _f64] + [_vp] * 13),
+    "nb_block_levels": (_i32, [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _f64, _f64, _f64, _i32, _f64, _vp, _i64,
+                               _vp, _vp]),
+    "nb_block_limit": (_i32, [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _vp, _vp]),
+    "nb_block_land": (_i32, [_vp, _vp, _vp, _i64, _i64, _i64, _vp, _vp]),
+    "nb_block_wake": (_i32, [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _i64, _i64, _f64] + [_vp] * 15),
+    "nb_block_open": (_i32, [_vp, _vp, _vp, _i64, _i64, _f64] + [_vp] * 18),
+    "nb_block_mark": (_i32, [_vp, _vp, _vp, _i64, _vp, _vp, _vp]),
     "nb_reuse_displacements": (_i32, [_vp, _vp, _vp, _vp, _PD, _i64, _vp]),
     "nb_reuse_limits": (_i32, [_vp, _vp, _vp, _vp, _f64, _vp, _vp, _i64, _vp, _vp, _vp]),
     "nb_reuse_fast_rows": (_i32, [_vp, _vp, _vp, _vp, _i64, _vp, _PD, _vp, _vp, _i64, _vp]),

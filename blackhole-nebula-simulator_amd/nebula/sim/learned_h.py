@@ -264,8 +264,9 @@ class LearnedH:
         target = self.net_target(feat, target)
         g = torch.empty_like(self.params)
         loss = _dev.empty(1, torch.float32)
-        _hip.call("nb_mlp_grad", _hip.ptr(self.params), _hip.ptr(self.norm), _hip.ptr(feat.contiguous()),
-                  _hip.ptr(target.contiguous()), int(target.numel()), _hip.ptr(g), _hip.ptr(loss))  # fmt: skip
+        feat, target = feat.contiguous(), target.contiguous()  # referenced until the call returns
+        _hip.call("nb_mlp_grad", _hip.ptr(self.params), _hip.ptr(self.norm), _hip.ptr(feat), _hip.ptr(target),
+                  int(target.numel()), _hip.ptr(g), _hip.ptr(loss))  # fmt: skip
         return float(loss.item()), g
 
     def train(self, feat: torch.Tensor, target: torch.Tensor, steps: int = 400, batch: int = 4096,

@@ -163,6 +163,60 @@ int32_t nb_reuse_fast_rows(nb_ctx *ctx, void *stream, const double *pos, const i
                            const double *h_lim, const double *box3, const int64_t *offset, const int32_t *index,
                            int64_t n, int32_t *missing_dev);
 
+/* ------------------------------------------------------------------ block time steps
+ * Individual power-of-two steps on an int64 timeline (blocks.py, timesteps.py:21-88; levels
+ * k <= 40, a step of 2^(40 - k) ticks). Rows are int64 particle indices, levels int64.
+ * nb_block_elapsed: s = (n_next - n_beg) tick and L/2 = 0.5 (n_end - n_beg) tick per particle.
+ * nb_block_predict: the predictions of a system step at tick n_next (blocks.py:502-530): p_out =
+ *   p_flow + (s - L/2) a, u_out = u_half + (s - L/2) du/dt, with beta cooling (beta finite; floor
+ *   u_floor, or u_floors per particle) of u_half over s at the mid radius; the active particles
+ *   (n_end = n_next) keep their cooled u_half and get e_part = m du (0 elsewhere), luminosity
+ *   and heat_cool.
+ * nb_block_close: closing kicks of the rows (blocks.py:543-546) and their heat budget.
+ * nb_block_levels: new levels of the rows (blocks.py:318-335, 367-372): step criteria (hydro
+ *   0: orbit only), level_for_dt, forced offset, allowed_level; nu may be NULL; eta_orb <= 0
+ *   without the hole. bad_dev[0] counts criteria not positive, bad_dev[1] levels beyond 40.
+ * nb_block_limit: one sweep of the neighbour limiter over the rows (blocks.py:338-353) on CSR
+ *   lists; *changed_dev set when a level rose (repeat until it stays 0).
+ * nb_block_land: levels raised until steps opened at n_now end by n_stop (blocks.py:383-393);
+ *   *bad_dev counts levels beyond 40.
+ * nb_block_wake: wake-up (blocks.py:357-365, 396-431): need (N, filled with -1 by the caller)
+ *   gets max over the rows' list entries of level - gap; running steps longer than need allows
+ *   are cut, their opening kicks corrected; *woken_dev counts the cuts.
+ * nb_block_open: opening kicks of the rows at n_now (blocks.py:443-452).
+ * nb_block_mark: mark[i] = 1 for the rows and every entry of their lists (blocks.py:202-214). */
+int32_t nb_block_elapsed(nb_ctx *ctx, void *stream, const int64_t *n_beg, const int64_t *n_end, int64_t n_next,
+                         double tick, int64_t n, double *s_out, double *half_out);
+int32_t nb_block_predict(nb_ctx *ctx, void *stream, const int64_t *n_beg, const int64_t *n_end, int64_t n_next,
+                         double tick, const double *p_flow, const double *a, double *u_half, const double *dudt,
+                         const double *x_beg, const double *x, const double *mass, double u_floor,
+                         const double *u_floors, double beta, int64_t n, double *p_out, double *u_out,
+                         double *e_part, double *lum, double *heat_cool);
+int32_t nb_block_close(nb_ctx *ctx, void *stream, const int64_t *rows, int64_t nrows, const int64_t *n_beg,
+                       const int64_t *n_end, double tick, const double *p_flow, const double *a, const double *u_half,
+                       const double *dudt, const double *visc, const double *cond, const double *disk, double *p,
+                       double *u, double *heat_visc, double *heat_cond, double *heat_disk, double *heat_pdv);
+int32_t nb_block_levels(nb_ctx *ctx, void *stream, const int64_t *rows, int64_t nrows, const double *x,
+                        const double *h, const double *vsig, const double *a, const double *nu, double courant,
+                        double c_force, double eta_orb, int32_t hydro, double dt_max, const int64_t *level_offset,
+                        int64_t n_now, int64_t *level, int32_t *bad_dev);
+int32_t nb_block_limit(nb_ctx *ctx, void *stream, const int64_t *rows, int64_t nrows, const int64_t *offset,
+                       const int32_t *index, int64_t gap, int64_t *level, int32_t *changed_dev);
+int32_t nb_block_land(nb_ctx *ctx, void *stream, const int64_t *rows, int64_t nrows, int64_t n_now, int64_t n_stop,
+                      int64_t *level, int32_t *bad_dev);
+int32_t nb_block_wake(nb_ctx *ctx, void *stream, const int64_t *rows, int64_t nrows, const int64_t *offset,
+                      const int32_t *index, int64_t gap, int64_t n, int64_t n_now, double tick, int64_t *need,
+                      int64_t *level, int64_t *n_end, const double *a, const double *dudt, const double *visc,
+                      const double *cond, const double *disk, double *p_half, double *u_half, double *heat_visc,
+                      double *heat_cond, double *heat_disk, double *heat_pdv, int32_t *woken_dev);
+int32_t nb_block_open(nb_ctx *ctx, void *stream, const int64_t *rows, int64_t nrows, int64_t n_now, double tick,
+                      const int64_t *level, int64_t *n_beg, int64_t *n_end, const double *x, const double *p,
+                      const double *u, const double *a, const double *dudt, const double *visc, const double *cond,
+                      const double *disk, double *x_beg, double *p_half, double *u_half, double *heat_visc,
+                      double *heat_cond, double *heat_disk, double *heat_pdv);
+int32_t nb_block_mark(nb_ctx *ctx, void *stream, const int64_t *rows, int64_t nrows, const int64_t *offset,
+                      const int32_t *index, uint8_t *mark);
+
 /* tighten (neighbours.py:306-323): keep pairs with |x_i - x_j| < 2 max(h_i, h_j); rows not in
  * ``rows`` are left empty. Same capacity protocol (the input list total always suffices). */
 int32_t nb_tighten(nb_ctx *ctx, void *stream, const double *pos, const double *h, int64_t n,
@@ -200,6 +254,60 @@ int32_t nb_hydro_density(nb_ctx *ctx, void *stream, const double *pos, const dou
                          int32_t *row_counts, int32_t kid, double cnorm, double eta, double tol, double h_max,
                          int32_t precision, double *h_out, double *rho, double *omega, int32_t *iters,
                          int32_t *flags, int64_t *stats4);
+/* ------------------------------------------------------------------ block time steps
+ * Individual power-of-two steps on an int64 timeline (blocks.py, timesteps.py:21-88; levels
+ * k <= 40, a step of 2^(40 - k) ticks). Rows are int64 particle indices, levels int64.
+ * nb_block_elapsed: s = (n_next - n_beg) tick and L/2 = 0.5 (n_end - n_beg) tick per particle.
+ * nb_block_predict: the predictions of a system step at tick n_next (blocks.py:502-530): p_out =
+ *   p_flow + (s - L/2) a, u_out = u_half + (s - L/2) du/dt, with beta cooling (beta finite; floor
+ *   u_floor, or u_floors per particle) of u_half over s at the mid radius; the active particles
+ *   (n_end = n_next) keep their cooled u_half and get e_part = m du (0 elsewhere), luminosity
+ *   and heat_cool.
+ * nb_block_close: closing kicks of the rows (blocks.py:543-546) and their heat budget.
+ * nb_block_levels: new levels of the rows (blocks.py:318-335, 367-372): step criteria (hydro
+ *   0: orbit only), level_for_dt, forced offset, allowed_level; nu may be NULL; eta_orb <= 0
+ *   without the hole. bad_dev[0] counts criteria not positive, bad_dev[1] levels beyond 40.
+ * nb_block_limit: one sweep of the neighbour limiter over the rows (blocks.py:338-353) on CSR
+ *   lists; *changed_dev set when a level rose (repeat until it stays 0).
+ * nb_block_land: levels raised until steps opened at n_now end by n_stop (blocks.py:383-393);
+ *   *bad_dev counts levels beyond 40.
+ * nb_block_wake: wake-up (blocks.py:357-365, 396-431): need (N, filled with -1 by the caller)
+ *   gets max over the rows' list entries of level - gap; running steps longer than need allows
+ *   are cut, their opening kicks corrected; *woken_dev counts the cuts.
+ * nb_block_open: opening kicks of the rows at n_now (blocks.py:443-452).
+ * nb_block_mark: mark[i] = 1 for the rows and every entry of their lists (blocks.py:202-214). */
+int32_t nb_block_elapsed(nb_ctx *ctx, void *stream, const int64_t *n_beg, const int64_t *n_end, int64_t n_next,
+                         double tick, int64_t n, double *s_out, double *half_out);
+int32_t nb_block_predict(nb_ctx *ctx, void *stream, const int64_t *n_beg, const int64_t *n_end, int64_t n_next,
+                         double tick, const double *p_flow, const double *a, double *u_half, const double *dudt,
+                         const double *x_beg, const double *x, const double *mass, double u_floor,
+                         const double *u_floors, double beta, int64_t n, double *p_out, double *u_out,
+                         double *e_part, double *lum, double *heat_cool);
+int32_t nb_block_close(nb_ctx *ctx, void *stream, const int64_t *rows, int64_t nrows, const int64_t *n_beg,
+                       const int64_t *n_end, double tick, const double *p_flow, const double *a, const double *u_half,
+                       const double *dudt, const double *visc, const double *cond, const double *disk, double *p,
+                       double *u, double *heat_visc, double *heat_cond, double *heat_disk, double *heat_pdv);
+int32_t nb_block_levels(nb_ctx *ctx, void *stream, const int64_t *rows, int64_t nrows, const double *x,
+                        const double *h, const double *vsig, const double *a, const double *nu, double courant,
+                        double c_force, double eta_orb, int32_t hydro, double dt_max, const int64_t *level_offset,
+                        int64_t n_now, int64_t *level, int32_t *bad_dev);
+int32_t nb_block_limit(nb_ctx *ctx, void *stream, const int64_t *rows, int64_t nrows, const int64_t *offset,
+                       const int32_t *index, int64_t gap, int64_t *level, int32_t *changed_dev);
+int32_t nb_block_land(nb_ctx *ctx, void *stream, const int64_t *rows, int64_t nrows, int64_t n_now, int64_t n_stop,
+                      int64_t *level, int32_t *bad_dev);
+int32_t nb_block_wake(nb_ctx *ctx, void *stream, const int64_t *rows, int64_t nrows, const int64_t *offset,
+                      const int32_t *index, int64_t gap, int64_t n, int64_t n_now, double tick, int64_t *need,
+                      int64_t *level, int64_t *n_end, const double *a, const double *dudt, const double *visc,
+                      const double *cond, const double *disk, double *p_half, double *u_half, double *heat_visc,
+                      double *heat_cond, double *heat_disk, double *heat_pdv, int32_t *woken_dev);
+int32_t nb_block_open(nb_ctx *ctx, void *stream, const int64_t *rows, int64_t nrows, int64_t n_now, double tick,
+                      const int64_t *level, int64_t *n_beg, int64_t *n_end, const double *x, const double *p,
+                      const double *u, const double *a, const double *dudt, const double *visc, const double *cond,
+                      const double *disk, double *x_beg, double *p_half, double *u_half, double *heat_visc,
+                      double *heat_cond, double *heat_disk, double *heat_pdv);
+int32_t nb_block_mark(nb_ctx *ctx, void *stream, const int64_t *rows, int64_t nrows, const int64_t *offset,
+                      const int32_t *index, uint8_t *mark);
+
 /* tighten (neighbours.py:306-323) of the last nb_hydro_density lists at the final smoothing
  * lengths, straight into 64-row blocks with capacities of their own: out_off (N / 64 + 1 int64,
  * device) receives the blocks' starts (the nb_lists offset of that layout), out_index holds
@@ -228,6 +336,60 @@ int32_t nb_hydro_density_mixed(nb_ctx *ctx, void *stream, const double *pos, con
                                int32_t *row_counts, int32_t kid, double cnorm, double eta, double tol, double h_max,
                                double *h_out, double *rho, double *omega, int32_t *iters, int32_t *flags,
                                int64_t *stats4);
+/* ------------------------------------------------------------------ block time steps
+ * Individual power-of-two steps on an int64 timeline (blocks.py, timesteps.py:21-88; levels
+ * k <= 40, a step of 2^(40 - k) ticks). Rows are int64 particle indices, levels int64.
+ * nb_block_elapsed: s = (n_next - n_beg) tick and L/2 = 0.5 (n_end - n_beg) tick per particle.
+ * nb_block_predict: the predictions of a system step at tick n_next (blocks.py:502-530): p_out =
+ *   p_flow + (s - L/2) a, u_out = u_half + (s - L/2) du/dt, with beta cooling (beta finite; floor
+ *   u_floor, or u_floors per particle) of u_half over s at the mid radius; the active particles
+ *   (n_end = n_next) keep their cooled u_half and get e_part = m du (0 elsewhere), luminosity
+ *   and heat_cool.
+ * nb_block_close: closing kicks of the rows (blocks.py:543-546) and their heat budget.
+ * nb_block_levels: new levels of the rows (blocks.py:318-335, 367-372): step criteria (hydro
+ *   0: orbit only), level_for_dt, forced offset, allowed_level; nu may be NULL; eta_orb <= 0
+ *   without the hole. bad_dev[0] counts criteria not positive, bad_dev[1] levels beyond 40.
+ * nb_block_limit: one sweep of the neighbour limiter over the rows (blocks.py:338-353) on CSR
+ *   lists; *changed_dev set when a level rose (repeat until it stays 0).
+ * nb_block_land: levels raised until steps opened at n_now end by n_stop (blocks.py:383-393);
+ *   *bad_dev counts levels beyond 40.
+ * nb_block_wake: wake-up (blocks.py:357-365, 396-431): need (N, filled with -1 by the caller)
+ *   gets max over the rows' list entries of level - gap; running steps longer than need allows
+ *   are cut, their opening kicks corrected; *woken_dev counts the cuts.
+ * nb_block_open: opening kicks of the rows at n_now (blocks.py:443-452).
+ * nb_block_mark: mark[i] = 1 for the rows and every entry of their lists (blocks.py:202-214). */
+int32_t nb_block_elapsed(nb_ctx *ctx, void *stream, const int64_t *n_beg, const int64_t *n_end, int64_t n_next,
+                         double tick, int64_t n, double *s_out, double *half_out);
+int32_t nb_block_predict(nb_ctx *ctx, void *stream, const int64_t *n_beg, const int64_t *n_end, int64_t n_next,
+                         double tick, const double *p_flow, const double *a, double *u_half, const double *dudt,
+                         const double *x_beg, const double *x, const double *mass, double u_floor,
+                         const double *u_floors, double beta, int64_t n, double *p_out, double *u_out,
+                         double *e_part, double *lum, double *heat_cool);
+int32_t nb_block_close(nb_ctx *ctx, void *stream, const int64_t *rows, int64_t nrows, const int64_t *n_beg,
+                       const int64_t *n_end, double tick, const double *p_flow, const double *a, const double *u_half,
+                       const double *dudt, const double *visc, const double *cond, const double *disk, double *p,
+                       double *u, double *heat_visc, double *heat_cond, double *heat_disk, double *heat_pdv);
+int32_t nb_block_levels(nb_ctx *ctx, void *stream, const int64_t *rows, int64_t nrows, const double *x,
+                        const double *h, const double *vsig, const double *a, const double *nu, double courant,
+                        double c_force, double eta_orb, int32_t hydro, double dt_max, const int64_t *level_offset,
+                        int64_t n_now, int64_t *level, int32_t *bad_dev);
+int32_t nb_block_limit(nb_ctx *ctx, void *stream, const int64_t *rows, int64_t nrows, const int64_t *offset,
+                       const int32_t *index, int64_t gap, int64_t *level, int32_t *changed_dev);
+int32_t nb_block_land(nb_ctx *ctx, void *stream, const int64_t *rows, int64_t nrows, int64_t n_now, int64_t n_stop,
+                      int64_t *level, int32_t *bad_dev);
+int32_t nb_block_wake(nb_ctx *ctx, void *stream, const int64_t *rows, int64_t nrows, const int64_t *offset,
+                      const int32_t *index, int64_t gap, int64_t n, int64_t n_now, double tick, int64_t *need,
+                      int64_t *level, int64_t *n_end, const double *a, const double *dudt, const double *visc,
+                      const double *cond, const double *disk, double *p_half, double *u_half, double *heat_visc,
+                      double *heat_cond, double *heat_disk, double *heat_pdv, int32_t *woken_dev);
+int32_t nb_block_open(nb_ctx *ctx, void *stream, const int64_t *rows, int64_t nrows, int64_t n_now, double tick,
+                      const int64_t *level, int64_t *n_beg, int64_t *n_end, const double *x, const double *p,
+                      const double *u, const double *a, const double *dudt, const double *visc, const double *cond,
+                      const double *disk, double *x_beg, double *p_half, double *u_half, double *heat_visc,
+                      double *heat_cond, double *heat_disk, double *heat_pdv);
+int32_t nb_block_mark(nb_ctx *ctx, void *stream, const int64_t *rows, int64_t nrows, const int64_t *offset,
+                      const int32_t *index, uint8_t *mark);
+
 /* tighten (neighbours.py:306-323) of the last nb_hydro_density_mixed lists at h_final (its
  * h_out), into rows of their own capacity: row i at out_off[i] (N + 1 entries written; row i's
  * capacity is its merged search row) with out_len[i] entries; out_index holds index_cap >= the
@@ -439,6 +601,10 @@ int32_t nb_alpha_balsara(nb_ctx *ctx, void *stream, const double *div, const dou
 int32_t nb_alpha_switch(nb_ctx *ctx, void *stream, const double *div, const double *curl, const double *ddiv,
                         const double *cs, const double *h, const double *alpha_in, double dt, double alpha_max,
                         double sigma, int64_t n, double *alpha_out);
+/* nb_alpha_switch with a step per element (blocks.py:266-270: each active row its own step). */
+int32_t nb_alpha_switch_dt(nb_ctx *ctx, void *stream, const double *div, const double *curl, const double *ddiv,
+                           const double *cs, const double *h, const double *alpha_in, const double *dt,
+                           double alpha_max, double sigma, int64_t n, double *alpha_out);
 /* Navier-Stokes glue (navier_stokes.py:29-50 and hydro.py:221-230): nu, viscous stress (N,3,3),
  * viscous heating (N). alpha_ss > 0: nu = alpha_ss c_s^2 |x|^1.5, else nu = nu_const. */
 int32_t nb_ns_stress(nb_ctx *ctx, void *stream, const double *g, const double *rho, const double *cs,

@@ -27,6 +27,28 @@ def test_library_exports_every_declared_symbol():
     check("ABI version", 4, lib.nb_abi_version())
 
 
+def declared_arities():
+    """Parameter count of every prototype in the header (comments stripped)."""
+    text = (REPO / "include" / "nebula_hip.h").read_text()
+    text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+    out = {}
+    for m in re.finditer(r"\b(nb_[a-z0-9_]+)\s*\(([^;{]*?)\)\s*;", text):
+        params = m.group(2).strip()
+        out[m.group(1)] = 0 if params in ("", "void") else params.count(",") + 1
+    return out
+
+
+def test_binding_arities_match_the_header():
+    """Each ctypes signature takes as many arguments as the C prototype (a missing pointer would
+    shift every later argument)."""
+    from nebula import _hip
+
+    ar = declared_arities()
+    wrong = {n: (len(_hip.SIGNATURES[n][1]), k) for n, k in ar.items() if len(_hip.SIGNATURES[n][1]) != k}
+    check("prototypes parsed", True, len(ar) > 20)
+    check("ctypes argument counts = header parameter counts", {}, wrong)
+
+
 def test_product_never_imports_the_oracle():
     pkg = REPO / "blackhole-nebula-simulator_amd"
     offenders = [str(p) for p in pkg.rglob("*.py") if re.search(r"^\s*(from|import)\s+oracle", p.read_text(), re.M)]

@@ -109,7 +109,7 @@ def run_gpu(ic, cfg, steps, warmup, learned=False, log_stride=8, train_steps=300
     its stream) and the entry-point timing. With ``learned``, the warm-up steps log (features,
     converged h) pairs (every ``log_stride``-th particle), the network is trained on them with the
     FP32 training kernels, and the timed steps start each Newton solve from its BF16 MFMA
-    prediction. driver: "eager" (coupled.stable_dt + coupled.step) or "stepper" (the same steps,
+    prediction. driver: "eager" (coupled.stable_dt + coupled.step), "stepper" (the same steps,
     bit for bit, through nebula.sim.stepper.Stepper: one host synchronisation per step).
     Returns (wall s, kernel times, entry-point times, state, info)."""
     import torch
@@ -147,10 +147,10 @@ def run_gpu(ic, cfg, steps, warmup, learned=False, log_stride=8, train_steps=300
     else:
         for _ in range(warmup):
             st = coupled.step(st, cfg, coupled.stable_dt(st, cfg))
-    if driver == "stepper":
-        from nebula.sim.stepper import Stepper
+    if driver in ("stepper", "pipelined"):
+        from nebula.sim.stepper import PipelinedStepper, Stepper
 
-        sp = Stepper(st, cfg)
+        sp = (PipelinedStepper if driver == "pipelined" else Stepper)(st, cfg)
         for _ in range(2):  # warm-up of the fixed-capacity passes
             sp.step()
 
@@ -182,8 +182,10 @@ def run_gpu(ic, cfg, steps, warmup, learned=False, log_stride=8, train_steps=300
     kern = _hip.stop_kernel_timing()
     info["newton_iters"] = float(sum(float(x) for x in iters) / len(iters))
     info["driver"] = driver
-    if driver == "stepper":
+    if driver in ("stepper", "pipelined"):
         info["stepper_fallbacks"] = sp.fallbacks
+    if driver == "pipelined":
+        info["overlapped"], info["discarded"] = sp.overlapped, sp.discarded
     if learned:
         info["fallback_rate"] = fallbacks / (steps * len(ic.x))
         info["gate"] = info["_cfg"].hydro.learned_model.gate.stats()
@@ -540,9 +542,9 @@ def run_long(args):
     st = coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg)
     sp = None
     if args.precision == "mixed":  # the headline's driver
-        from nebula.sim.stepper import Stepper
+        from nebula.sim.stepper import PipelinedStepper
 
-        sp = Stepper(st, cfg)
+        sp = PipelinedStepper(st, cfg)
     windows = []
     done = 0
     while done < args.long_run:
@@ -568,7 +570,7 @@ def run_long(args):
         windows.append(rec)
         print(f"bench: {json.dumps(rec)}", file=sys.stderr, flush=True)
     print(json.dumps({"workload": f"stream crash, {args.n} particles", "precision": args.precision,
-                      "learned_h": learned, "driver": "stepper" if sp is not None else "eager",
+                      "learned_h": learned, "driver": "pipelined" if sp is not None else "eager",
                       "fallbacks": None if sp is None else sp.fallbacks, "windows": windows}))
 
 
@@ -610,7 +612,7 @@ def main():
     print(f"bench: {n} particles, {args.precision}, warmup {args.warmup}, steps {args.steps}", file=sys.stderr,
           flush=True)  # fmt: skip
     learned = args.precision == "mixed" and not args.no_learned
-    driver = "stepper" if args.precision == "mixed" else "eager"
+    driver = "pipelined" if args.precision == "mixed" else "eager"
     wall, kern, entry, st, info = run_gpu(ic, cfg, args.steps, args.warmup, learned=learned, driver=driver)
     value = n * args.steps / wall
     print(f"bench: {value:.4g} particle-steps/s", file=sys.stderr, flush=True)
@@ -697,7 +699,8 @@ def main():
         # and the headline's own configuration stepped eagerly (coupled.stable_dt + coupled.step)
         alongside = [(other, other, False, "eager")]
         if learned:
-            alongside += [("mixed", "mixed", False, driver), ("mixed+learned, eager steps", "mixed", True, "eager")]
+            alongside += [("mixed", "mixed", False, driver), ("mixed+learned, stepper", "mixed", True, "stepper"),
+                          ("mixed+learned, eager steps", "mixed", True, "eager")]  # fmt: skip
         for label, name, lrn, drv in alongside:
             print(f"bench: {label} alongside", file=sys.stderr, flush=True)
             w2, k2, _, _, info2 = run_gpu(ic, config(name), args.steps, args.warmup, learned=lrn, driver=drv)
@@ -807,10 +810,10 @@ def whole_run(ic, cfg, steps, driver, cfg64):
         st = coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, c)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        if drv == "stepper":
-            from nebula.sim.stepper import Stepper
+        if drv in ("stepper", "pipelined"):
+            from nebula.sim.stepper import PipelinedStepper, Stepper
 
-            sp = Stepper(st, c)
+            sp = (PipelinedStepper if drv == "pipelined" else Stepper)(st, c)
             for _ in range(steps):
                 sp.step()
             extra = {"stepper_fallbacks": sp.fallbacks}

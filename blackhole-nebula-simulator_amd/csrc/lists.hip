@@ -1103,6 +1103,7 @@ k_tight_emit(const uint32_t* __restrict__ gbuf, const double* __restrict__ dbuf,
 static constexpr int kG = 16;
 static constexpr int kCols = 2 * kG;  // columns per chunk: two per lane
 static constexpr int kTab = 384;      // candidate slots per lane group in LDS (multiple of kG)
+static constexpr int kQueue = 32;     // extras queued per row in LDS (more go out at once)
 
 // Status words (int32, device) of a step recorded for replay: what did not fit, or what the
 // caller must redo eagerly. Words 6 and 7 are written by nb_glf_step (failures) and
@@ -1328,12 +1329,17 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const Gr
     const unsigned long long gb0 = goff[ii];
     const int64_t cap = (int64_t)(goff[ii + 1] - gb0);
     uint32_t* row = ridx + gb0;
-    __shared__ uint32_t ctab[256 / kG][kTab];  // one candidate table per lane group
+    __shared__ uint32_t ctab[256 / kG][kTab];     // one candidate table per lane group
+    __shared__ uint32_t cqueue[256 / kG][kQueue];  // its row's extras, handed out after the walk
     uint32_t* tab = ctab[threadIdx.x / kG];
+    uint32_t* queue = cqueue[threadIdx.x / kG];
+    const unsigned below = (1u << glane) - 1u;
+    int qm = 0;
     // every loop below runs the same number of times on all lanes of the wave (ballot bounds),
     // so the shuffles always see their whole group active
     for (uint32_t c0 = 0;; c0 += kCols) {
         if (__ballot(c0 < ncol) == 0) break;
+        const int32_t m0 = m;  // the row's entries before this chunk
         uint32_t k0a = 0, lena = 0, k0b = 0, lenb = 0;
         int32_t z0, z1, qx, qy;
         const uint32_t ca = c0 + glane, cb = c0 + kG + glane;
@@ -1412,15 +1418,29 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const Gr
             const unsigned long long bal = __ballot(pred);
             const unsigned gb = (unsigned)((bal >> gshift) & ((1ull << kG) - 1));
             if (pred) {
-                const int slot = m + __popc(gb & ((1u << glane) - 1));
+                // the chunk's entries go to the table slots already read (slot - m0 < vb + kG),
+                // and out to the row after the walk: no store inside the walk for its loads to
+                // wait behind
+                const int slot = m + __popc(gb & below);
+                if (slot - m0 < kTab) tab[slot - m0] = j;
+                else if (slot < cap) row[slot] = j;
+            }
+            const unsigned long long fbal = __ballot(flag);
+            if (fbal) {
+                const unsigned fg = (unsigned)((fbal >> gshift) & ((1ull << kG) - 1));
                 if (flag) {
-                    const unsigned int es = atomicAdd(&efill[j], 1u);
-                    const unsigned long long eb = eoff[j];
-                    if (es < eoff[j + 1] - eb) ebuf[eb + es] = (uint32_t)i;
-                    else eover = 1;
-                    emax = es + 1 > emax ? es + 1 : emax;
+                    const int qs = qm + __popc(fg & below);
+                    if (qs < kQueue) {
+                        queue[qs] = j;
+                    } else {
+                        const unsigned int es = atomicAdd(&efill[j], 1u);
+                        const unsigned long long eb = eoff[j];
+                        if (es < eoff[j + 1] - eb) ebuf[eb + es] = (uint32_t)i;
+                        else eover = 1;
+                        emax = es + 1 > emax ? es + 1 : emax;
+                    }
                 }
-                if (slot < cap) row[slot] = j;
+                qm += __popc(fg);
             }
             // the first-step sums on every lane: terms off the row add an exact zero
             const float q = sqrt32((float)d2) * inv_h0;
@@ -1431,6 +1451,22 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const Gr
             s2 += pred ? (double)(mj * __builtin_fmaf(q, dfq, 3.f * fq)) : 0.0;
             m += __popc(gb);
         }
+        // the chunk's entries, from the table to the row (contiguous per group)
+        wave_sync();
+        const int32_t mc = m - m0 < kTab ? m - m0 : kTab;
+        for (int32_t t = glane; t < mc; t += kG)
+            if (m0 + t < cap) row[m0 + t] = tab[t];
+        wave_sync();  // read before the next chunk's candidates overwrite the table
+    }
+    // the queued extras to their segments
+    const int qn = qm < kQueue ? qm : kQueue;
+    for (int t = glane; t < qn; t += kG) {
+        const uint32_t j = queue[t];
+        const unsigned int es = atomicAdd(&efill[j], 1u);
+        const unsigned long long eb = eoff[j];
+        if (es < eoff[j + 1] - eb) ebuf[eb + es] = (uint32_t)i;
+        else eover = 1;
+        emax = es + 1 > emax ? es + 1 : emax;
     }
     if (live && glane == 0) {
         rowcnt[i] = m;

@@ -587,6 +587,33 @@ extern "C" int32_t nb_dt_minima_dev(nb_ctx* ctx, void* stream, const double* h, 
     return launch_check_s(ctx, "nb_dt_minima_dev");
 }
 
+// The next step's dt from the four minima of nb_dt_minima_dev, on the device: coupled.py:213-226
+// (stable_dt) as the host takes it from the same minima (the Courant, force, viscous and orbit
+// criteria in that order, each kept only where strictly smaller, as Python's min(dt, c)), with
+// r^1.5 from pow15. dts = (dt, dt_prev) becomes (the new dt, the old dt). For a driver that
+// launches the next step before reading this one back (it compares the value with the host's
+// when it does).
+__global__ void k_dt_advance(const double* __restrict__ m, double courant, double c_force, int viscous, int hole,
+                             double eta_orb, double* __restrict__ dts)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double dt = courant * m[0];
+    if (isfinite(m[1])) dt = pymin(dt, c_force * m[1]);
+    if (viscous) dt = pymin(dt, m[2]);
+    if (hole) dt = pymin(dt, eta_orb * pow15(m[3]));
+    dts[1] = dts[0];
+    dts[0] = dt;
+}
+
+extern "C" int32_t nb_dt_advance(nb_ctx* ctx, void* stream, const double* minima4_dev, double courant,
+                                 double c_force, int32_t viscous, int32_t hole, double eta_orb, double* dts2_dev)
+{
+    if (!ctx || !minima4_dev || !dts2_dev) return ctx_fail(ctx, "nb_dt_advance: bad arguments");
+    hipLaunchKernelGGL(k_dt_advance, dim3(1), dim3(64), 0, (hipStream_t)stream, minima4_dev, courant, c_force,
+                       (int)(viscous != 0), (int)(hole != 0), eta_orb, dts2_dev);
+    return launch_check_s(ctx, "nb_dt_advance");
+}
+
 // ------------------------------------------------------- nb_dt_minima_dev without allocation
 // For callers that cannot allocate (a stream being recorded into a hipGraph, a serving loop that
 // forbids hipMalloc): the partial minima live in `scratch`, a fixed device array of this file,

@@ -455,10 +455,11 @@ def step(st: GasState, cfg: CoupledConfig, dt: float, fixed=None) -> GasState:
         raise ValueError("a recorded step covers power-law cooling and the constant or Balsara viscosity")
     model = _learned(cfg)
     gate = getattr(model, "gate", None) if hc.learned_h else None
-    if gate is not None and dts is not None:
+    decided = getattr(fixed, "use_net", None)  # a pipelined driver consults the gate itself
+    if gate is not None and dts is not None and decided is None:
         raise ValueError("a recorded step runs the learned start on every replay: set the model's gate to None")
-    # decided before any launch of this step (the previous step's count is ready by then)
-    use_net = hc.learned_h and (gate is None or gate.decide())
+    # decided before any launch of this step (from the counts of the steps before the last)
+    use_net = hc.learned_h and (bool(decided) if decided is not None else (gate is None or gate.decide()))
 
     def kick(y, a):
         return _axpy(y, a, half) if dts is None else _axpy_dev(y, a, dts, 0.5)

@@ -83,36 +83,56 @@ class StartGate:
       again once a plain step's fraction exceeds ``on_above``.
 
     Decisions depend only on the run's own iteration counts, so equal runs decide alike. A step
-    that starts from h_prev is exactly the plain method's step.
+    that starts from h_prev is exactly the plain method's step. A decision uses the counts of every
+    step but the last one: the last step may still be running on the device when the next one is
+    launched (a pipelined driver), and reading its count would wait for it.
     """
 
     def __init__(self, on_above: float = 0.05, off_below: float = 0.03, probe_every: int = 32):
         self.on_above, self.off_below, self.probe_every = float(on_above), float(off_below), int(probe_every)
         self.on = None  # unknown until the first probe
         self.since_probe = 0
-        self._pending = None  # (device count of slow rows, rows, network used) of the last step
+        self._pending = []  # (device count of slow rows, rows, network used) of the steps not yet read
         self.last_plain_fraction = None
         self.steps_network = 0
         self.steps_plain = 0
         self.probes = 0
 
     def observe(self, slow_rows, rows: int, used_network: bool) -> None:
-        """The last step's count of rows left for the slow solve (a device scalar, read lazily)."""
-        self._pending = (slow_rows, int(rows), bool(used_network))
+        """A step's count of rows left for the slow solve: a number, or a device scalar, copied to
+        pinned host memory behind the step's work and read only when a later decision needs it."""
+        if isinstance(slow_rows, torch.Tensor) and slow_rows.is_cuda:
+            host = torch.empty((), dtype=slow_rows.dtype).pin_memory()
+            host.copy_(slow_rows, non_blocking=True)
+            ready = torch.cuda.Event()
+            ready.record()
+            slow_rows = (host, ready)
+        self._pending.append((slow_rows, int(rows), bool(used_network)))
 
     def _resolve(self) -> None:
-        if self._pending is None:
-            return
-        count, rows, used = self._pending
-        self._pending = None
-        if used:
-            return
-        frac = float(count.item() if hasattr(count, "item") else count) / max(rows, 1)
-        self.last_plain_fraction = frac
-        if self.on:  # a probe while the network runs
-            self.on = frac >= self.off_below
-        else:
-            self.on = frac > self.on_above
+        while len(self._pending) > 1:  # all but the newest count
+            count, rows, used = self._pending.pop(0)
+            if used:
+                continue
+            if isinstance(count, tuple):  # (pinned copy, the event after it): waits for that step only
+                count[1].synchronize()
+                count = count[0]
+            frac = float(count.item() if hasattr(count, "item") else count) / max(rows, 1)
+            self.last_plain_fraction = frac
+            if self.on:  # a probe while the network runs
+                self.on = frac >= self.off_below
+            else:
+                self.on = frac > self.on_above
+
+    def snapshot(self) -> dict:
+        """The gate's state, for restore() when a step is done again."""
+        d = dict(self.__dict__)
+        d["_pending"] = list(self._pending)
+        return d
+
+    def restore(self, snap: dict) -> None:
+        self.__dict__.update(snap)
+        self._pending = list(snap["_pending"])
 
     def decide(self) -> bool:
         """True: the coming step starts from the network's prediction; False: from h_prev."""

@@ -22,6 +22,7 @@ training included: its pairs are logged after the status check, in the eager ste
 from __future__ import annotations
 
 import ctypes
+import math
 from dataclasses import replace
 
 import numpy as np
@@ -91,7 +92,7 @@ class Stepper:
         if _hip.generation() != self._generation:
             self._reserve()  # the library's scratch was released or moved since the reservation
         gate = _gate(cfg)
-        gate_before = None if gate is None else dict(gate.__dict__)
+        gate_before = None if gate is None else gate.snapshot()
         self.status.zero_()
         self.fixed.pairs = None
         new = coupled.step(st, cfg, dt, fixed=self.fixed)
@@ -105,7 +106,7 @@ class Stepper:
             self.fallbacks += 1
             self.fallback_reasons.append({STATUS_NAMES[k]: int(status[k]) for k in range(9) if status[k]})
             if gate is not None:  # the step is done again: the same decision, observed once
-                gate.__dict__.update(gate_before)
+                gate.restore(gate_before)
             new = coupled.step(st, cfg, dt)
             self.state = new
             self.headroom = max(self.headroom, 1.3)
@@ -116,6 +117,137 @@ class Stepper:
             coupled.log_pairs(cfg, model, feat, h_prev, h_final)
             self.state = replace(new, n_search_entries=int(packed[20]), n_tight_entries=int(packed[21]))
             self.dt = coupled.dt_from_minima(packed[16:20], cfg, new.nu is not None)
+        self.steps += 1
+        return dt
+
+
+class PipelinedStepper(Stepper):
+    """``Stepper`` that launches step k + 1 before it reads step k back.
+
+    ``Stepper`` ends each step with its one host synchronisation, and the GPU then waits while the
+    host launches the next step's first, short kernels one by one. Here the next step is launched
+    first, so the host prepares step k + 1 while the GPU still runs step k:
+
+    - each step's dt is taken on the device from the previous step's minima (nb_dt_advance) and
+      read from device memory by the step's kernels, as in a recorded step;
+    - the host then reads step k's status, minima and totals (copied to pinned memory when step k
+      ended), computes dt from the minima as ``coupled.dt_from_minima`` does, and keeps step k + 1
+      only when step k was good and the device took the same dt bit for bit; otherwise step k + 1
+      is discarded (and, for a bad step k, step k done again by ``coupled.step``, as ``Stepper``
+      does);
+    - the learned start's gate decides from the counts of every step but the last, so its
+      decision for step k + 1 needs nothing of step k; a step that runs the network waits for
+      step k's pairs (its online training comes first), so it is launched after the read-back.
+
+    States are those of ``coupled.step`` with ``coupled.stable_dt``, bit for bit
+    (``tests/test_gpu_stepper.py``). ``state`` is the last step read back; one more may be running.
+    Covers the physics whose dt a step can take from the device: power-law cooling, constant or
+    Balsara viscosity, GN or Newtonian flow, with or without the learned start.
+    """
+
+    def __init__(self, st: coupled.GasState, cfg: coupled.CoupledConfig, headroom: float = 1.3):
+        hc = cfg.hydro
+        if math.isfinite(cfg.cooling_beta) or (hc.viscosity and hc.switch):
+            raise ValueError("the pipelined step takes its dt from the device: power-law cooling and the constant "
+                             "or Balsara viscosity only")  # fmt: skip
+        if hc.ns_nu > 0 or hc.ns_alpha > 0 or hc.disc_alpha_av > 0:
+            raise ValueError("the pipelined step covers the cloud physics without disc viscosity")
+        self.dts = torch.zeros(2, dtype=torch.float64, device=st.x.device)
+        super().__init__(st, cfg, headroom)
+        self._pinned = [torch.empty(24, dtype=torch.float64).pin_memory() for _ in range(2)]
+        self._slot = 0
+        self._flight = None  # the launched step not yet read back
+        self.overlapped = 0  # steps launched before their predecessor was read back
+        self.discarded = 0  # launched steps thrown away (their predecessor failed, or its dt differed)
+
+    def _reserve(self):
+        super()._reserve()
+        self.fixed.dts = self.dts
+
+    def _launch(self, st, host_dt=None):
+        """Enqueue one step from st. host_dt: its dt from the host (written to the device first);
+        None: the dt the previous launch left on the device."""
+        cfg = self.cfg
+        gate = _gate(cfg)
+        snap = None if gate is None else gate.snapshot()
+        self.fixed.use_net = bool(cfg.hydro.learned_h and (gate is None or gate.decide()))
+        if host_dt is not None:
+            self.dts[0].fill_(float(host_dt))
+            self.dts[1].fill_(float(st.dt_prev))
+        self.status.zero_()
+        self.fixed.pairs = None
+        # coupled.step takes the dt from self.dts; the host value only dates the state (set again
+        # once the step is read back)
+        new = coupled.step(st, cfg, self.dt if host_dt is None else host_dt, fixed=self.fixed)
+        _hip.call("nb_dt_minima_dev", _hip.ptr(new.h), _hip.ptr(new.vsig_max), _hip.ptr(new.a_hydro),
+                  _hip.ptr(new.nu), 0.25, _hip.ptr(new.x), self.n, _hip.ptr(self.minima))  # fmt: skip
+        hc = cfg.hydro
+        _hip.call("nb_dt_advance", _hip.ptr(self.minima), float(hc.courant), float(hc.c_force),
+                  int(new.nu is not None), int(bool(cfg.hole)), float(cfg.eta_orb), _hip.ptr(self.dts))  # fmt: skip
+        totals = torch.stack([new.row_counts.sum(dtype=torch.int64), new.nneigh.sum().to(torch.int64)]).double()
+        buf = self._pinned[self._slot]
+        self._slot ^= 1
+        buf.copy_(torch.cat([self.status.double(), self.minima, totals, self.dts]), non_blocking=True)
+        done = torch.cuda.Event()
+        done.record()
+        return {"state": new, "buf": buf, "done": done, "snap": snap, "pairs": self.fixed.pairs}
+
+    def _next_uses_net(self) -> bool:
+        cfg = self.cfg
+        if not cfg.hydro.learned_h:
+            return False
+        gate = _gate(cfg)
+        if gate is None:
+            return True
+        snap = gate.snapshot()
+        use = gate.decide()
+        gate.restore(snap)
+        return use
+
+    def step(self) -> float:
+        """One step of coupled.stable_dt(state) (read back); returns the dt taken."""
+        cfg, dt = self.cfg, self.dt
+        if self._flight is None:
+            if _hip.generation() != self._generation:
+                self._reserve()  # the library's scratch was released or moved since the reservation
+            self._flight = self._launch(self.state, host_dt=dt)
+        cur, nxt = self._flight, None
+        if not self._next_uses_net():
+            nxt = self._launch(cur["state"])
+            self.overlapped += 1
+        cur["done"].synchronize()
+        packed = cur["buf"].numpy().copy()
+        status = packed[:16].astype(np.int64)
+        gate = _gate(cfg)
+        if np.any(status[:9] != 0):
+            self.fallbacks += 1
+            self.fallback_reasons.append({STATUS_NAMES[k]: int(status[k]) for k in range(9) if status[k]})
+            if gate is not None:  # undoes both launches' decisions; the step is decided again
+                gate.restore(cur["snap"])
+            if nxt is not None:
+                self.discarded += 1
+            torch.cuda.synchronize()  # the discarded launch has finished with the shared buffers
+            new = coupled.step(self.state, cfg, dt)
+            self.state = new
+            self.headroom = max(self.headroom, 1.3)
+            self._reserve()
+            self.dt = coupled.stable_dt(new, cfg)
+            self._flight = None
+            self.steps += 1
+            return dt
+        model, feat, h_prev, h_final = cur["pairs"]
+        coupled.log_pairs(cfg, model, feat, h_prev, h_final)
+        new = cur["state"]
+        self.state = replace(new, t=self.state.t + dt, dt_prev=float(dt), n_search_entries=int(packed[20]),
+                             n_tight_entries=int(packed[21]))  # fmt: skip
+        self.dt = coupled.dt_from_minima(packed[16:20], cfg, new.nu is not None)
+        if nxt is not None and packed[22] != self.dt:  # the device's dt differs from the host's
+            if gate is not None:
+                gate.restore(nxt["snap"])
+            nxt = None
+            self.discarded += 1
+            torch.cuda.synchronize()
+        self._flight = nxt
         self.steps += 1
         return dt
 

@@ -566,6 +566,13 @@ int32_t nb_dt_minima_dev(nb_ctx *ctx, void *stream, const double *h, const doubl
 int32_t nb_dt_minima_dev_static(nb_ctx *ctx, void *stream, const double *h, const double *vsig,
                                 const double *acc, const double *nu, double c_visc, const double *x, int64_t n,
                                 double *out4_dev);
+/* The next step's dt from the minima of nb_dt_minima_dev, on the device (replaces the host half
+ * of coupled.py:213-226 stable_dt, as coupled.dt_from_minima computes it: courant min_0, then
+ * c_force min_1 when finite, min_2 when viscous, eta_orb min_3^1.5 with the hole, each kept only
+ * where strictly smaller): dts2_dev = (dt, dt_prev) becomes (new dt, old dt). For a step driver
+ * that launches the next step before it reads this one back. */
+int32_t nb_dt_advance(nb_ctx *ctx, void *stream, const double *minima4_dev, double courant, double c_force,
+                      int32_t viscous, int32_t hole, double eta_orb, double *dts2_dev);
 
 /* Ensemble helpers (no counterpart in the reference; the systems of nb_grid_order_seg):
  * nb_dt_minima_seg: nb_dt_minima per system, out (host) nseg x 4; nb_axpy_rows: out row i = y row i

@@ -5,6 +5,10 @@
   Balsara limiter, power-law cooling, GN flow) and the learned start learning online.
 - A step whose fixed capacities are too small reports it, is discarded and done again by
   coupled.step; the results are still coupled.step's.
+- PipelinedStepper (step k + 1 launched before step k is read back, dt taken on the device): the
+  same bits, with and without the learned start (whose network steps wait for the read-back) and
+  with the gate switching the network on and off; a failed step discards the step launched after
+  it and is done again by coupled.step; the launches overlap where the network does not run.
 - RecordedStepper (the same step recorded once as a hipGraph, dt written to the device before
   each replay): the same bits over ten replays, with and without the learned start; the graph
   holds no memset node; an undersized recording falls back, records again and still gives
@@ -66,23 +70,25 @@ def _trained_model(ic, cfg, gated=False):
     return model
 
 
+@pytest.mark.parametrize("driver", ["stepper", "pipelined"])
 @pytest.mark.parametrize("learned", [False, True, "gated"])
-def test_stepper_equals_coupled_step(learned):
+def test_stepper_equals_coupled_step(learned, driver):
     """With a gated learned start (probes every 4 steps, on whenever a probe finds a slow row) the
     gate's decisions are part of what must match: the stepper redoes a failed step with the
     decision it had made, and observes it once."""
     from nebula.ics import stream_crash
     from nebula.sim import coupled
-    from nebula.sim.stepper import Stepper
+    from nebula.sim.stepper import PipelinedStepper, Stepper
 
     ic = stream_crash(16384, length=200.0, seed=20261014)
     base = _config()
     model_a = _trained_model(ic, base, gated=learned == "gated") if learned else None
     model_b = copy.deepcopy(model_a) if learned else None
     cfg_a, cfg_b = _config(model_a), _config(model_b)
-    print(f"stepper: 16384-particle stream crash, learned start {learned}, 10 steps", flush=True)
+    print(f"{driver}: 16384-particle stream crash, learned start {learned}, 10 steps", flush=True)
     st = coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg_a)
-    sp = Stepper(coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg_b), cfg_b)
+    cls = PipelinedStepper if driver == "pipelined" else Stepper
+    sp = cls(coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg_b), cfg_b)
     dts, sdts = [], []
     for _ in range(10):
         dt = coupled.stable_dt(st, cfg_a)
@@ -95,6 +101,12 @@ def test_stepper_equals_coupled_step(learned):
     # step then regathers or rebuilds): those steps are done again by coupled.step
     print("fallbacks:", sp.fallbacks, sp.fallback_reasons)
     check("most steps without a fallback", True, sp.fallbacks <= 3)
+    if driver == "pipelined":
+        print("overlapped launches:", sp.overlapped, "discarded:", sp.discarded)
+        if learned is False:
+            check("most launches overlapped their predecessor", True, sp.overlapped >= 5)
+        elif learned == "gated":  # the probes (plain starts) overlap; network steps wait
+            check("the plain-start launches overlapped", True, sp.overlapped >= 1)
     if learned:
         bit_equal("online-trained weights", model_a.params.cpu().numpy(), model_b.params.cpu().numpy())
     if learned == "gated":
@@ -104,16 +116,18 @@ def test_stepper_equals_coupled_step(learned):
               model_a.gate.steps_network > 0 and model_a.gate.probes >= 2)  # fmt: skip
 
 
-def test_stepper_overflow_falls_back_to_coupled_step():
+@pytest.mark.parametrize("driver", ["stepper", "pipelined"])
+def test_stepper_overflow_falls_back_to_coupled_step(driver):
     from nebula.ics import stream_crash
     from nebula.sim import coupled
-    from nebula.sim.stepper import Stepper
+    from nebula.sim.stepper import PipelinedStepper, Stepper
 
     ic = stream_crash(16384, length=200.0, seed=7)
     cfg = _config()
-    print("stepper: capacities a third of the need, 4 steps", flush=True)
+    print(f"{driver}: capacities a third of the need, 4 steps", flush=True)
     st = coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg)
-    sp = Stepper(coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg), cfg, headroom=0.3)
+    cls = PipelinedStepper if driver == "pipelined" else Stepper
+    sp = cls(coupled.initialise(ic.x, ic.v, ic.u, ic.mass, ic.h, cfg), cfg, headroom=0.3)
     for _ in range(4):
         st = coupled.step(st, cfg, coupled.stable_dt(st, cfg))
         sp.step()

@@ -1811,6 +1811,7 @@ struct FinArgs {
 // record), and more rows per wavefront with fewer lanes each issue fewer of them at once: at 1M,
 // 4 lanes took 1.17 ms, 8 took 0.80, 16 take 0.69 and 32 took 0.72.
 static constexpr int kEmitW = 16;
+static constexpr int kEmitBuf = 128;  // kept entries per row held in LDS until the walk ends
 
 template <int kE>
 __global__ void __launch_bounds__(256)
@@ -1853,13 +1854,22 @@ k_tight_emit_rm(const double4* __restrict__ posh, const uint32_t* __restrict__ r
         const double rr = NB_R_KERN * pymax(hi, hj);
         return d2 < rr * rr;
     };
+    // the row's first kEmitBuf kept entries wait in LDS and go out after the walk (contiguous per
+    // group), so that no store sits in the walk for its loads to wait behind; later ones go at once
+    __shared__ int32_t obuf[256 / kE][kEmitBuf];
+    int32_t* ob = obuf[threadIdx.x / kE];
+    auto put = [&](int32_t k, uint32_t j) {
+        const unsigned long long at = t0 + (unsigned long long)k;
+        if (at < idx_cap) out_idx[at] = (int32_t)j;
+        else if (status) status[kStIndex] = 1;
+    };
     auto emit = [&](bool pred, uint32_t j) {
         const unsigned long long bal = __ballot(pred);
         const unsigned gb = (unsigned)((bal >> gshift) & ((1ull << kE) - 1));
         if (pred) {
-            const unsigned long long at = t0 + o + __popc(gb & ((1u << glane) - 1));
-            if (at < idx_cap) out_idx[at] = (int32_t)j;
-            else if (status) status[kStIndex] = 1;
+            const int32_t k = o + __popc(gb & ((1u << glane) - 1));
+            if (k < kEmitBuf) ob[k] = (int32_t)j;
+            else put(k, j);
         }
         o += __popc(gb);
     };
@@ -1889,6 +1899,8 @@ k_tight_emit_rm(const double4* __restrict__ posh, const uint32_t* __restrict__ r
         emit(pa, ja);
         emit(pc, jc);
     }
+    wave_sync();
+    for (int32_t k = glane; k < (o < kEmitBuf ? o : kEmitBuf); k += kE) put(k, (uint32_t)ob[k]);
     double s1, s2;
     sums.total<kE>(s1, s2);
     if (live && glane == 0) {

@@ -22,7 +22,7 @@
 // order (nb_grid_order), every gathered row comes out sorted and the sort pass does no work.
 //
 // Pipeline (one stream): k_bbox / k_bbox_final / k_hsum (grid shape), k_cell_id, radix sort
-// (hipCUB: by cell, ascending index inside a cell), k_count via atomics + scan (cell starts),
+// (rocPRIM radix sort: by cell, ascending index inside a cell), k_count via atomics + scan (cell starts),
 // k_gather (one work-item per particle; pairs that j's own gather will not see, d2 >= (2 h_j)^2,
 // are flagged and counted for j), k_sort_rows (only rows that came out unsorted), k_extras_fill
 // (the flagged pairs transposed into per-row extra lists), k_merge_* (sorted gather row merged
@@ -31,7 +31,8 @@
 #include "nb_ctx.h"
 #include "nebula_hip.h"
 
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -2142,6 +2143,21 @@ __global__ void k_status_grow(const int32_t* __restrict__ flags, int64_t n, int3
 }  // namespace nb
 
 // =================================================================== host side
+// rocPRIM's device-wide primitives, called directly: the stable LSD radix sort of (key, value)
+// pairs by the key's low `bits` bits, and the exclusive prefix sum. Each is called twice, first
+// with temp = nullptr for its scratch size, then to run.
+template <class T>
+static hipError_t excl_sum(void* temp, size_t& bytes, const T* in, T* out, int64_t count, hipStream_t s)
+{
+    return rocprim::exclusive_scan(temp, bytes, in, out, T(0), (size_t)count, rocprim::plus<T>(), s);
+}
+
+static hipError_t sort_pairs(void* temp, size_t& bytes, const uint32_t* kin, uint32_t* kout, const uint32_t* vin,
+                             uint32_t* vout, int64_t n, int bits, hipStream_t s)
+{
+    return rocprim::radix_sort_pairs(temp, bytes, kin, kout, vin, vout, (size_t)n, 0u, (unsigned)bits, s);
+}
+
 using namespace nb;
 
 namespace {
@@ -2149,11 +2165,11 @@ namespace {
 int scan_u64(nb_ctx* ctx, hipStream_t s, const unsigned long long* in, unsigned long long* out, int64_t count)
 {
     size_t bytes = 0;
-    if (hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)count, s) != hipSuccess)
-        return ctx_fail(ctx, "hipcub scan sizing failed");
+    if (excl_sum(nullptr, bytes, in, out, (int64_t)(count), s) != hipSuccess)
+        return ctx_fail(ctx, "scan sizing failed");
     if (ctx_reserve(ctx, (void**)&ctx->cub_tmp, &ctx->cub_tmp_bytes, bytes)) return NB_ERR_HIP;
-    if (hipcub::DeviceScan::ExclusiveSum(ctx->cub_tmp, bytes, in, out, (int)count, s) != hipSuccess)
-        return ctx_fail(ctx, "hipcub scan failed");
+    if (excl_sum(ctx->cub_tmp, bytes, in, out, (int64_t)(count), s) != hipSuccess)
+        return ctx_fail(ctx, "scan failed");
     return NB_OK;
 }
 
@@ -2284,23 +2300,21 @@ int build_grid(nb_ctx* ctx, hipStream_t s, const double* pos, const double* h, i
         int end_bit = 1;
         while ((int64_t(1) << end_bit) < ncells && end_bit < 32) end_bit++;
         size_t bytes = 0;
-        if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, ctx->cell_id, ctx->keys_out, ctx->iota, ctx->order,
-                                               (int)n, 0, end_bit, s) != hipSuccess)
-            return ctx_fail(ctx, "hipcub sort sizing failed");
+        if (sort_pairs(nullptr, bytes, ctx->cell_id, ctx->keys_out, ctx->iota, ctx->order, n, end_bit, s) != hipSuccess)
+            return ctx_fail(ctx, "sort sizing failed");
         if (ctx_reserve(ctx, (void**)&ctx->cub_tmp, &ctx->cub_tmp_bytes, bytes)) return NB_ERR_HIP;
-        if (hipcub::DeviceRadixSort::SortPairs(ctx->cub_tmp, bytes, ctx->cell_id, ctx->keys_out, ctx->iota, ctx->order,
-                                               (int)n, 0, end_bit, s) != hipSuccess)
-            return ctx_fail(ctx, "hipcub sort failed");
+        if (sort_pairs(ctx->cub_tmp, bytes, ctx->cell_id, ctx->keys_out, ctx->iota, ctx->order, n, end_bit, s) != hipSuccess)
+            return ctx_fail(ctx, "sort failed");
     }
     ctx->order_ident = sorted ? 1 : 0;
     size_t bytes = 0;
-    if (hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, ctx->cell_count, ctx->cell_start, (int)(ncells + 1), s) !=
+    if (excl_sum(nullptr, bytes, ctx->cell_count, ctx->cell_start, (int64_t)(ncells + 1), s) !=
         hipSuccess)
-        return ctx_fail(ctx, "hipcub scan sizing failed");
+        return ctx_fail(ctx, "scan sizing failed");
     if (ctx_reserve(ctx, (void**)&ctx->cub_tmp, &ctx->cub_tmp_bytes, bytes)) return NB_ERR_HIP;
-    if (hipcub::DeviceScan::ExclusiveSum(ctx->cub_tmp, bytes, ctx->cell_count, ctx->cell_start, (int)(ncells + 1), s) !=
+    if (excl_sum(ctx->cub_tmp, bytes, ctx->cell_count, ctx->cell_start, (int64_t)(ncells + 1), s) !=
         hipSuccess)
-        return ctx_fail(ctx, "hipcub scan failed");
+        return ctx_fail(ctx, "scan failed");
     return NB_OK;
 }
 
@@ -2424,23 +2438,21 @@ int build_grid_seg(nb_ctx* ctx, hipStream_t s, const double* pos, const double* 
         int end_bit = 1;
         while ((int64_t(1) << end_bit) < ncells && end_bit < 32) end_bit++;
         size_t bytes = 0;
-        if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, ctx->cell_id, ctx->keys_out, ctx->iota, ctx->order,
-                                               (int)n, 0, end_bit, s) != hipSuccess)
-            return ctx_fail(ctx, "hipcub sort sizing failed");
+        if (sort_pairs(nullptr, bytes, ctx->cell_id, ctx->keys_out, ctx->iota, ctx->order, n, end_bit, s) != hipSuccess)
+            return ctx_fail(ctx, "sort sizing failed");
         if (ctx_reserve(ctx, (void**)&ctx->cub_tmp, &ctx->cub_tmp_bytes, bytes)) return NB_ERR_HIP;
-        if (hipcub::DeviceRadixSort::SortPairs(ctx->cub_tmp, bytes, ctx->cell_id, ctx->keys_out, ctx->iota, ctx->order,
-                                               (int)n, 0, end_bit, s) != hipSuccess)
-            return ctx_fail(ctx, "hipcub sort failed");
+        if (sort_pairs(ctx->cub_tmp, bytes, ctx->cell_id, ctx->keys_out, ctx->iota, ctx->order, n, end_bit, s) != hipSuccess)
+            return ctx_fail(ctx, "sort failed");
     }
     ctx->order_ident = sorted ? 1 : 0;
     size_t bytes = 0;
-    if (hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, ctx->cell_count, ctx->cell_start, (int)(ncells + 1), s) !=
+    if (excl_sum(nullptr, bytes, ctx->cell_count, ctx->cell_start, (int64_t)(ncells + 1), s) !=
         hipSuccess)
-        return ctx_fail(ctx, "hipcub scan sizing failed");
+        return ctx_fail(ctx, "scan sizing failed");
     if (ctx_reserve(ctx, (void**)&ctx->cub_tmp, &ctx->cub_tmp_bytes, bytes)) return NB_ERR_HIP;
-    if (hipcub::DeviceScan::ExclusiveSum(ctx->cub_tmp, bytes, ctx->cell_count, ctx->cell_start, (int)(ncells + 1), s) !=
+    if (excl_sum(ctx->cub_tmp, bytes, ctx->cell_count, ctx->cell_start, (int64_t)(ncells + 1), s) !=
         hipSuccess)
-        return ctx_fail(ctx, "hipcub scan failed");
+        return ctx_fail(ctx, "scan failed");
     return NB_OK;
 }
 
@@ -3094,11 +3106,11 @@ int scan_fixed(nb_ctx* ctx, hipStream_t s, const T* in, T* out, int64_t count)
     if (!capturing(s)) {
         // not being recorded: the library scan (fewer launches; its memsets are harmless here)
         size_t bytes = 0;
-        if (hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)count, s) != hipSuccess)
-            return ctx_fail(ctx, "hipcub scan sizing failed");
+        if (excl_sum(nullptr, bytes, in, out, (int64_t)(count), s) != hipSuccess)
+            return ctx_fail(ctx, "scan sizing failed");
         if (ctx_reserve(ctx, (void**)&ctx->cub_tmp, &ctx->cub_tmp_bytes, bytes)) return NB_ERR_HIP;
-        if (hipcub::DeviceScan::ExclusiveSum(ctx->cub_tmp, bytes, in, out, (int)count, s) != hipSuccess)
-            return ctx_fail(ctx, "hipcub scan failed");
+        if (excl_sum(ctx->cub_tmp, bytes, in, out, (int64_t)(count), s) != hipSuccess)
+            return ctx_fail(ctx, "scan failed");
         return NB_OK;
     }
     const int64_t tiles = (count + kScanTile - 1) / kScanTile;
@@ -3117,13 +3129,11 @@ int sort_fixed(nb_ctx* ctx, hipStream_t s, int64_t n)
     if (!capturing(s)) {
         // not being recorded: the library's (also stable) radix sort over the same bits
         size_t bytes = 0;
-        if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, ctx->cell_id, ctx->keys_out, ctx->iota, ctx->order,
-                                               (int)n, 0, ctx->fx_bits, s) != hipSuccess)
-            return ctx_fail(ctx, "hipcub sort sizing failed");
+        if (sort_pairs(nullptr, bytes, ctx->cell_id, ctx->keys_out, ctx->iota, ctx->order, n, ctx->fx_bits, s) != hipSuccess)
+            return ctx_fail(ctx, "sort sizing failed");
         if (ctx_reserve(ctx, (void**)&ctx->cub_tmp, &ctx->cub_tmp_bytes, bytes)) return NB_ERR_HIP;
-        if (hipcub::DeviceRadixSort::SortPairs(ctx->cub_tmp, bytes, ctx->cell_id, ctx->keys_out, ctx->iota, ctx->order,
-                                               (int)n, 0, ctx->fx_bits, s) != hipSuccess)
-            return ctx_fail(ctx, "hipcub sort failed");
+        if (sort_pairs(ctx->cub_tmp, bytes, ctx->cell_id, ctx->keys_out, ctx->iota, ctx->order, n, ctx->fx_bits, s) != hipSuccess)
+            return ctx_fail(ctx, "sort failed");
         return NB_OK;
     }
     const int64_t nblocks = (n + kSortThreads - 1) / kSortThreads;

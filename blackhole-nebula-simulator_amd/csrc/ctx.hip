@@ -160,7 +160,7 @@ extern "C" int32_t nb_ctx_trim(nb_ctx* ctx)
         return ctx_fail_code(ctx, NB_ERR_HIP, "nb_ctx_trim: device synchronisation failed");
     ctx_free_scratch(ctx);
     // what lived in the arena is gone: passes that read an earlier pass's scratch must be redone
-    ctx->lists_ready = ctx->hydro_ready = ctx->hydro_mixed_ready = 0;
+    ctx->lists_ready = ctx->hydro_ready = ctx->hydro_mixed_ready = ctx->hydro64_ready = 0;
     ctx->fx_ready = ctx->fx_grid_from_order = 0;
     ctx->host_segs.clear();
     ctx->generation++;

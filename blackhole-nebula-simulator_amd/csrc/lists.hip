@@ -1921,6 +1921,256 @@ k_tight_emit_rm(const double4* __restrict__ posh, const uint32_t* __restrict__ r
     block_partial<false>(live && glane == 0 ? o : 0, part);
 }
 
+// ----------------------------------------------------------------- FP64 step on the lane-group lists
+// The FP64 step's density pass on the lists of k_gather_mixed (rows of neighbour indices in the
+// search order, which for particles stored in grid order is ascending index order, and sorted
+// extras segments), with the reference's FP64 arithmetic bit for bit.
+
+// sph.py:215-228 for one row, 16 lanes: each lane forms one term of 16 consecutive entries (d2 from
+// the (x, y, z) records as the gather formed it; the terms of entries outside 2 min(h_search,
+// h_max) or past q = 2 are exact zeros), the group writes them to LDS, and every lane then adds
+// them in row order. The sums are therefore the reference's left-to-right sums, bit for bit
+// (adding +0.0 changes no sum here: neither sum can be -0.0).
+__device__ __forceinline__ void row_sums_group(const double4* __restrict__ posh, const double4& pi,
+                                               const uint32_t* __restrict__ row, int32_t m, const double* mass,
+                                               double m_uniform, double r_max2, double h, int kid, double cnorm,
+                                               int glane, double2* terms, double& rs_out, double& dws_out)
+{
+    double rs = 0.0, dws = 0.0;
+    const double inv_h = 1.0 / h;
+    const double ih3 = ipow3(inv_h);
+    const double ncih4 = -cnorm * ipow4(inv_h);
+    for (int32_t k0 = 0; k0 < m; k0 += kG) {
+        const int32_t k = k0 + glane;
+        double trs = 0.0, tdws = 0.0;
+        if (k < m) {
+            const uint32_t j = row[k];
+            const double4 pj = posh[j];
+            const double dx = pj.x - pi.x;
+            const double dy = pj.y - pi.y;
+            const double dz = pj.z - pi.z;
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 < r_max2) {
+                const double q = sqrt(d2) * inv_h;
+                if (q < 2.0) {
+                    const double mk = mass ? mass[j] : m_uniform;
+                    const double fq = kernel_f(q, kid);
+                    trs = mk * cnorm * ih3 * fq;
+                    tdws = mk * (ncih4 * (3.0 * fq + q * kernel_df(q, kid)));
+                }
+            }
+        }
+        terms[glane] = make_double2(trs, tdws);
+        wave_sync();
+        const int32_t cnt = m - k0 < kG ? m - k0 : kG;
+        for (int32_t l = 0; l < cnt; l++) {
+            const double2 t = terms[l];
+            rs += t.x;
+            dws += t.y;
+        }
+        wave_sync();  // every lane has read the terms before the next batch overwrites them
+    }
+    rs_out = rs;
+    dws_out = dws;
+}
+
+// sph.py:251-297 for every row (as solve_row over a blocked row), 16 lanes per row.
+__global__ void __launch_bounds__(256)
+k_solve64(const double4* __restrict__ posh, const uint32_t* __restrict__ ridx,
+          const unsigned long long* __restrict__ goff, const int32_t* __restrict__ rowcnt, int64_t n, DensArgs A)
+{
+    __shared__ double2 terms[256 / kG][kG];
+    const int glane = threadIdx.x & (kG - 1);
+    const int64_t i = (xcd_tile() * (int64_t)blockDim.x + threadIdx.x) / kG;
+    if (i >= n) return;  // whole groups leave together
+    const double4 pi = posh[i];
+    const double h_search_i = pi.w;
+    const int32_t m = (int32_t)min((unsigned long long)rowcnt[i], goff[i + 1] - goff[i]);
+    const uint32_t* row = ridx + goff[i];
+    double2* tg = terms[threadIdx.x / kG];
+    const double* mass = A.m_uniform > 0.0 ? nullptr : A.mass;
+    const double m_i = A.mass[i];
+    const double r_max = 2.0 * pymin(h_search_i, A.h_max);
+    const double r_max2 = r_max * r_max;
+    double hi = pymin(h_search_i, A.h_max);
+    double lo = 0.0;
+    double h = pymin(A.h_start[i], hi);
+    const double h0 = h;
+    int flag = 3;
+    int it = 0;
+    const double eta3 = ipow3(A.eta);
+    double rs, dws;
+    while (it < 50) {
+        it += 1;
+        row_sums_group(posh, pi, row, m, mass, A.m_uniform, r_max2, h, A.kid, A.cnorm, glane, tg, rs, dws);
+        const double f = rs - m_i * ipow3(A.eta / h);
+        const double df = dws + 3.0 * m_i * eta3 / ipow4(h);
+        if (f > 0) hi = h;
+        else lo = h;
+        double h_new = df > 0 ? h - f / df : 0.5 * (lo + hi);
+        if (!(lo <= h_new && h_new <= hi)) h_new = 0.5 * (lo + hi);
+        if (fabs(h_new - h) < A.tol * h0) {
+            h = h_new;
+            flag = 0;
+            break;
+        }
+        h = h_new;
+    }
+    const double h_lim = pymin(h_search_i, A.h_max);
+    if (h >= h_lim * (1 - 1e-3)) {
+        double rs_lim, dum;
+        row_sums_group(posh, pi, row, m, mass, A.m_uniform, r_max2, h_lim, A.kid, A.cnorm, glane, tg, rs_lim, dum);
+        if (rs_lim - m_i * ipow3(A.eta / h_lim) < 0) {
+            h = h_lim;
+            flag = h_lim >= A.h_max ? 1 : 2;
+        }
+    }
+    row_sums_group(posh, pi, row, m, mass, A.m_uniform, r_max2, h, A.kid, A.cnorm, glane, tg, rs, dws);
+    if (glane == 0) {
+        A.h_out[i] = h;
+        A.rho[i] = rs;
+        A.omega[i] = flag == 1 ? 1.0 : 1.0 + h / (3.0 * rs) * dws;
+        A.iters[i] = it;
+        A.flags[i] = flag;
+    }
+}
+
+// Tight-row blocks of the FP64 step: capacity of each 64-row block the longest merged row (gather
+// row + extras) of its rows, times 64 (bsize[nblk] = 0, for the exclusive scan).
+__global__ void k_block_caps64(const int32_t* __restrict__ rowcnt, const unsigned int* __restrict__ efill,
+                               const unsigned long long* __restrict__ goff, const unsigned long long* __restrict__ eoff,
+                               int64_t n, unsigned long long* __restrict__ bsize, int64_t nblk)
+{
+    const int64_t blk = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);  // one wavefront per block
+    if (blk > nblk) return;
+    const int64_t i = blk * 64 + (threadIdx.x & 63);
+    long long need = 0;
+    if (blk < nblk && i < n)
+        need = (long long)min((unsigned long long)rowcnt[i], goff[i + 1] - goff[i]) +
+               (long long)min((unsigned long long)efill[i], eoff[i + 1] - eoff[i]);
+    need = need < 1 ? 1 : need;
+    for (int o = 32; o > 0; o >>= 1) {
+        const long long t = __shfl_xor(need, o, 64);
+        need = t > need ? t : need;
+    }
+    if ((threadIdx.x & 63) == 0) bsize[blk] = blk < nblk ? (unsigned long long)need * 64ull : 0ull;
+}
+
+// The FP64 step's tight rows (neighbours.py:284-323 on the search lists at the final h) in 64-row
+// blocks (entry k of row i at boff[i / 64] + 64 k + i % 64): the gather row (ascending) merged
+// with the row's sorted extras, each entry kept when d2 < (2 max(h_i, h_j))^2, so that each row
+// comes out in ascending order as tighten() leaves it. 16 lanes per row: the kept extras (up to
+// kMergeLds) are compacted into LDS first; the gather row is then tightened 16 entries at a time,
+// each kept entry going to its merged position (the kept entries before it plus the kept extras
+// below it, by binary search), and each kept extra counts the kept gather entries below it (a
+// binary search over the batch); the extras go out last. Rows with more extras merge on one lane.
+static constexpr int kMergeLds = 128;
+
+__device__ __forceinline__ int lower_bound_u32(const uint32_t* a, int n, uint32_t v)
+{
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(256)
+k_emit64(const double4* __restrict__ posh, const uint32_t* __restrict__ ridx,
+         const unsigned long long* __restrict__ goff, const int32_t* __restrict__ rowcnt,
+         const unsigned int* __restrict__ efill, const unsigned long long* __restrict__ eoff,
+         const uint32_t* __restrict__ es, int64_t n, const unsigned long long* __restrict__ boff,
+         int32_t* __restrict__ out_len, int32_t* __restrict__ out_idx, long long* __restrict__ part)
+{
+    __shared__ uint32_t ek[256 / kG][kMergeLds];  // kept extras, ascending
+    __shared__ int32_t ecnt[256 / kG][kMergeLds];  // kept gather entries below each
+    __shared__ uint32_t jb[256 / kG][kG];         // the batch's indices (ascending)
+    const int glane = threadIdx.x & (kG - 1);
+    const int grp = threadIdx.x / kG;
+    const int gshift = (threadIdx.x & 63) & ~(kG - 1);
+    const unsigned below = (1u << glane) - 1u;
+    const int64_t i = (xcd_tile() * (int64_t)blockDim.x + threadIdx.x) / kG;
+    const bool live = i < n;
+    const int64_t ii = live ? i : n - 1;
+    const double4 pi = posh[ii];
+    const double hi = pi.w;
+    const int32_t ng = live ? (int32_t)min((unsigned long long)rowcnt[ii], goff[ii + 1] - goff[ii]) : 0;
+    const unsigned long long e0 = eoff[ii];
+    const int32_t ne = live ? (int32_t)min((unsigned long long)efill[ii], eoff[ii + 1] - e0) : 0;
+    const uint32_t* row = ridx + goff[ii];
+    const uint32_t* ex = es + e0;
+    int32_t* orow = out_idx + (boff[ii >> 6] + (ii & 63));
+    auto keep = [&](uint32_t j, double& d2) -> bool {
+        const double4 pj = posh[j];
+        const double dx = pj.x - pi.x;
+        const double dy = pj.y - pi.y;
+        const double dz = pj.z - pi.z;
+        d2 = dx * dx + dy * dy + dz * dz;
+        const double rr = NB_R_KERN * pymax(hi, pj.w);
+        return d2 < rr * rr;
+    };
+    int32_t total = 0;
+    if (ne > kMergeLds) {
+        // many extras: one lane merges the two ascending lists
+        if (glane == 0) {
+            int32_t a = 0, b = 0, o = 0;
+            while (a < ng || b < ne) {
+                const bool take_g = b >= ne || (a < ng && row[a] < ex[b]);
+                const uint32_t j = take_g ? row[a++] : ex[b++];
+                double d2;
+                if (keep(j, d2)) orow[(int64_t)(o++) * 64] = (int32_t)j;
+            }
+            total = o;
+        }
+    } else {
+        // the kept extras, compacted into LDS (order kept)
+        int32_t nek = 0;
+        for (int32_t t0 = 0; t0 < ne; t0 += kG) {
+            const int32_t t = t0 + glane;
+            double d2 = 0.0;
+            const uint32_t e = t < ne ? ex[t] : 0u;
+            const bool kp = t < ne && keep(e, d2);
+            const unsigned kb = (unsigned)((__ballot(kp) >> gshift) & 0xffffull);
+            if (kp) {
+                const int32_t q = nek + __popc(kb & below);
+                ek[grp][q] = e;
+                ecnt[grp][q] = 0;
+            }
+            nek += __popc(kb);
+        }
+        wave_sync();
+        int32_t g = 0;  // kept gather entries so far
+        for (int32_t k0 = 0; k0 < ng; k0 += kG) {
+            const int32_t k = k0 + glane;
+            double d2 = 0.0;
+            const uint32_t j = k < ng ? row[k] : 0xffffffffu;
+            const bool kp = k < ng && keep(j, d2);
+            const unsigned kb = (unsigned)((__ballot(kp) >> gshift) & 0xffffull);
+            if (kp) {
+                const int32_t pos = g + __popc(kb & below) + lower_bound_u32(ek[grp], nek, j);
+                orow[(int64_t)pos * 64] = (int32_t)j;
+            }
+            if (nek > 0) {
+                jb[grp][glane] = j;  // ascending; past the row's end above every index
+                wave_sync();
+                for (int32_t q = glane; q < nek; q += kG) {
+                    // the kept gather entries of this batch below extra q
+                    const int p = lower_bound_u32(jb[grp], kG, ek[grp][q]);
+                    ecnt[grp][q] += __popc(kb & ((1u << p) - 1u));
+                }
+                wave_sync();
+            }
+            g += __popc(kb);
+        }
+        for (int32_t q = glane; q < nek; q += kG) orow[(int64_t)(q + ecnt[grp][q]) * 64] = (int32_t)ek[grp][q];
+        total = g + nek;
+    }
+    if (live && glane == 0) out_len[i] = total;
+    block_partial<false>(live && glane == 0 ? total : 0, part);
+}
+
 static inline unsigned grid_of(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 // ------------------------------------------------------------------ fixed-capacity passes
@@ -2229,6 +2479,7 @@ int build_grid(nb_ctx* ctx, hipStream_t s, const double* pos, const double* h, i
     ctx->fx_grid_from_order = 0;
     ctx->hydro_ready = 0;
     ctx->hydro_mixed_ready = 0;
+    ctx->hydro64_ready = 0;
     ctx->order_ident = 0;
     int nparts = (int)std::min<int64_t>(1024, (n + kBlock - 1) / kBlock);
     if (ctx_reserve(ctx, (void**)&ctx->red, &ctx->red_bytes, sizeof(double) * 8 * (nparts + 2))) return NB_ERR_HIP;
@@ -2357,6 +2608,7 @@ int build_grid_seg(nb_ctx* ctx, hipStream_t s, const double* pos, const double* 
     ctx->lists_ready = 0;
     ctx->hydro_ready = 0;
     ctx->hydro_mixed_ready = 0;
+    ctx->hydro64_ready = 0;
     ctx->order_ident = 0;
     ctx->fx_grid_from_order = 0;
     if (nseg < 1 || off[0] != 0 || off[nseg] != n) return ctx_fail(ctx, "segmented grid: bad segment offsets");
@@ -3067,6 +3319,155 @@ extern "C" int32_t nb_hydro_tight_mixed(nb_ctx* ctx, void* stream, const double*
     return NB_OK;
 }
 
+// The FP64 step's density pass on the lane-group lists (k_gather_mixed over the FP64 grid,
+// extras sorted, k_solve64 on every row), with nb_hydro_density's outputs, statistics and tight
+// blocks (nb_hydro_tight64). The rows come out in ascending index order only when the particles
+// are stored in this grid's order (the step's own permutation puts them so); otherwise nothing is
+// computed and NB_NOT_GRID_ORDER is returned, for the caller to take nb_hydro_density.
+extern "C" int32_t nb_hydro_density64(nb_ctx* ctx, void* stream, const double* pos, const double* mass,
+                                      double m_uniform, const double* h_start, const double* h_search, int64_t n,
+                                      const int32_t* row_hint, int32_t* row_counts, int32_t kid, double cnorm,
+                                      double eta, double tol, double h_max, double* h_out, double* rho,
+                                      double* omega, int32_t* iters, int32_t* flags, int64_t* stats4)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (!ctx || n <= 0 || !stats4 || n >= (int64_t)kMask) return ctx_fail(ctx, "nb_hydro_density64: bad arguments");
+    const Box open{{0.0, 0.0, 0.0}};
+    Grid g{};
+    int64_t ncells = 0;
+    int rc = build_grid(ctx, s, pos, h_search, n, open, g, ncells, true);
+    if (rc) return rc;
+    if (!ctx->order_ident) return NB_NOT_GRID_ORDER;
+    const bool masses = !(m_uniform > 0.0);
+    const int64_t nblk = (n + 63) / 64;
+    const size_t u64n = sizeof(unsigned long long) * (size_t)(n + 1);
+    if (ctx_reserve(ctx, (void**)&ctx->rowcnt, &ctx->rowcnt_bytes, sizeof(int32_t) * n) ||
+        ctx_reserve(ctx, (void**)&ctx->pending, &ctx->pending_bytes, n) ||
+        ctx_reserve(ctx, (void**)&ctx->posh, &ctx->posh_bytes, sizeof(double4) * n) ||
+        ctx_reserve(ctx, (void**)&ctx->efill, &ctx->efill_bytes, sizeof(unsigned int) * n) ||
+        ctx_reserve(ctx, (void**)&ctx->xlist, &ctx->xlist_bytes, sizeof(uint32_t) * (size_t)(2 * n + 2)) ||
+        ctx_reserve(ctx, (void**)&ctx->rm_gc, &ctx->rm_gc_bytes, u64n) ||
+        ctx_reserve(ctx, (void**)&ctx->rm_go, &ctx->rm_go_bytes, u64n) ||
+        ctx_reserve(ctx, (void**)&ctx->rm_ec, &ctx->rm_ec_bytes, u64n) ||
+        ctx_reserve(ctx, (void**)&ctx->rm_eo, &ctx->rm_eo_bytes, u64n) ||
+        ctx_reserve(ctx, (void**)&ctx->fb_sz, &ctx->fb_sz_bytes, sizeof(unsigned long long) * (nblk + 1)) ||
+        ctx_reserve(ctx, (void**)&ctx->fb_to, &ctx->fb_to_bytes, sizeof(unsigned long long) * (nblk + 1)))
+        return NB_ERR_HIP;
+    NB_TIMED(ctx, s, "k_pack_posh", k_pack_posh, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, h_search, n,
+             (double4*)ctx->posh);
+    const int64_t cap_u = ctx->row_cap_rm > 0 ? ctx->row_cap_rm : 192;
+    const int64_t ecap_u = ctx->ecap_rm > 0 ? ctx->ecap_rm : 32;
+    long long sc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // the gather's own first Newton step is discarded: k_solve64 solves every row
+    MixArgs M{mass, m_uniform, h_start, h_max, kid, cnorm, eta, tol, h_out, rho, omega, iters, flags, ctx->pending,
+              ctx->part};
+    const int tile = 256;
+    const int64_t blocks = (n * kG + tile - 1) / tile;
+    int mode = row_hint ? 1 : 0;
+    unsigned long long tot[2] = {0, 0};
+    for (int attempt = 0;; attempt++) {
+        hipLaunchKernelGGL(k_caps_mixed, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, row_hint, ctx->rowcnt, ctx->efill, n,
+                           mode, cap_u, ecap_u, ctx->rm_gc, ctx->rm_ec);
+        rc = scan_u64(ctx, s, ctx->rm_gc, ctx->rm_go, n + 1);
+        if (rc) return rc;
+        rc = scan_u64(ctx, s, ctx->rm_ec, ctx->rm_eo, n + 1);
+        if (rc) return rc;
+        if (hipMemcpyAsync(&tot[0], ctx->rm_go + n, sizeof(unsigned long long), hipMemcpyDeviceToHost, s) != hipSuccess)
+            return ctx_fail(ctx, "nb_hydro_density64: capacity copy failed");
+        rc = sync_copy(ctx, s, &tot[1], ctx->rm_eo + n, sizeof(unsigned long long),
+                       "nb_hydro_density64: capacity copy failed");
+        if (rc) return rc;
+        if (ctx_reserve(ctx, (void**)&ctx->gbuf, &ctx->gbuf_bytes, sizeof(uint32_t) * (size_t)tot[0]) ||
+            ctx_reserve(ctx, (void**)&ctx->fscratch, &ctx->fscratch_bytes, sizeof(float) * (masses ? (size_t)n : 1)) ||
+            ctx_reserve(ctx, (void**)&ctx->ebuf, &ctx->ebuf_bytes, sizeof(uint32_t) * (size_t)tot[1]))
+            return NB_ERR_HIP;
+        if (hipMemsetAsync(ctx->efill, 0, sizeof(unsigned int) * n, s) != hipSuccess) return ctx_fail(ctx, "memset");
+        float* rmass = masses ? ctx->fscratch : nullptr;
+#define NB_GATHER64(M_, K_)                                                                                        \
+    NB_TIMED(ctx, s, "k_gather_mixed", (k_gather_mixed<M_, false, K_>), dim3(blocks), dim3(tile), 0, s,           \
+             (const double4*)ctx->posh, n, g, (const GridSeg*)nullptr, (const int32_t*)nullptr, ctx->cell_start, \
+             (const uint32_t*)nullptr, ctx->gbuf, rmass, ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->ebuf,           \
+             ctx->rm_eo, M)
+        if (masses && kid == 0) NB_GATHER64(true, 0);
+        else if (masses) NB_GATHER64(true, 1);
+        else if (kid == 0) NB_GATHER64(false, 0);
+        else NB_GATHER64(false, 1);
+#undef NB_GATHER64
+        finish_partials(ctx, s, blocks, 5, 1u | 4u | 16u, 0);
+        rc = sync_copy(ctx, s, sc, ctx->scal, sizeof(long long) * 5, "nb_hydro_density64: gather failed");
+        if (rc) return rc;
+        if (sc[2] == 0 && sc[3] == 0) break;
+        if (attempt == 2) return ctx_fail(ctx, "nb_hydro_density64: row capacity did not settle");
+        mode = 2;
+    }
+    ctx->row_cap_rm = (int32_t)(sc[0] + sc[0] / 8 + 8);
+    ctx->ecap_rm = sc[4] + sc[4] / 4 + 4;
+    const long long longest = sc[0];
+    if (ctx_reserve(ctx, (void**)&ctx->ebuf_s, &ctx->ebuf_s_bytes, sizeof(uint32_t) * (size_t)tot[1]))
+        return NB_ERR_HIP;
+    sort_extras(ctx, s, n, (unsigned)blocks, tile);
+    DensArgs A{mass, m_uniform, h_start, h_max, kid, cnorm, eta, tol, h_out, rho, omega, iters, flags, ctx->pending,
+               ctx->part};
+    NB_TIMED(ctx, s, "k_solve64", k_solve64, dim3((unsigned)blocks), dim3(tile), 0, s, (const double4*)ctx->posh,
+             ctx->gbuf, ctx->rm_go, ctx->rowcnt, n, A);
+    // merged row lengths (the search-list entries) and the tight blocks' capacities
+    NB_TIMED(ctx, s, "k_row_total_rm", k_row_total_rm, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, ctx->rowcnt,
+             ctx->efill, ctx->rm_go, ctx->rm_eo, ctx->rm_gc, row_counts, ctx->part, n);
+    finish_partials(ctx, s, grid_of(n + 1), 2, 1u, 5);  // [5] longest merged row, [6] search-list entries
+    const int64_t wg = (nblk + 1 + 3) / 4;
+    hipLaunchKernelGGL(k_block_caps64, dim3((unsigned)wg), dim3(kBlock), 0, s, ctx->rowcnt, ctx->efill, ctx->rm_go,
+                       ctx->rm_eo, n, ctx->fb_sz, nblk);
+    rc = scan_u64(ctx, s, ctx->fb_sz, ctx->fb_to, nblk + 1);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_collect, dim3(1), dim3(64), 0, s, (long long*)ctx->scal + 7, ctx->fb_to + nblk,
+                       ctx->fb_to + nblk);
+    rc = sync_copy(ctx, s, sc + 5, (long long*)ctx->scal + 5, sizeof(long long) * 3,
+                   "nb_hydro_density64: totals copy failed");
+    if (rc) return rc;
+    if (hipGetLastError() != hipSuccess) return ctx_fail(ctx, "nb_hydro_density64: kernel launch failed");
+    ctx->lists_n = n;
+    ctx->lists_maxrow = sc[5];
+    ctx->lists_total = sc[6];
+    ctx->fb_ttotal = sc[7];
+    ctx->hydro64_ready = 1;
+    stats4[0] = longest;  // longest gather row
+    stats4[1] = 0;        // rows gathered unsorted (none: the particles are in grid order)
+    stats4[2] = sc[6];    // entries of the search lists (incl. self)
+    stats4[3] = sc[7];    // entries of the tight rows' blocks (each block as long as its longest merged row)
+    return NB_OK;
+}
+
+// The tight rows of the last nb_hydro_density64 pass at the final h (nb_hydro_tight's outputs:
+// rows in 64-row blocks, block starts in out_off, lengths in out_len), from the positions.
+extern "C" int32_t nb_hydro_tight64(nb_ctx* ctx, void* stream, const double* pos, const double* h_final,
+                                    int64_t* out_off, int32_t* out_len, int32_t* out_index, int64_t index_cap,
+                                    int64_t* total_out)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (!ctx || !ctx->hydro64_ready || !total_out)
+        return ctx_fail_code(ctx, NB_ERR_NOT_BUILT, "nb_hydro_tight64: no nb_hydro_density64 pass");
+    if (index_cap < ctx->fb_ttotal) return ctx_fail(ctx, "nb_hydro_tight64: index capacity below the tight rows' blocks");
+    if (!pos || !h_final || !out_off || !out_len || !out_index) return ctx_fail(ctx, "nb_hydro_tight64: bad arguments");
+    const int64_t n = ctx->lists_n;
+    const int64_t nblk = (n + 63) / 64;
+    if (hipMemcpyAsync(out_off, ctx->fb_to, sizeof(int64_t) * (nblk + 1), hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return ctx_fail(ctx, "nb_hydro_tight64: offset copy failed");
+    NB_TIMED(ctx, s, "k_pack_posh", k_pack_posh, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, h_final, n,
+             (double4*)ctx->posh);
+    const unsigned eblocks = (unsigned)((n * kG + 255) / 256);
+    if (ctx->part_bytes < sizeof(long long) * (size_t)eblocks)
+        return ctx_fail(ctx, "nb_hydro_tight64: reduction buffer smaller than the launch");
+    NB_TIMED(ctx, s, "k_emit64", k_emit64, dim3(eblocks), dim3(256), 0, s, (const double4*)ctx->posh, ctx->gbuf,
+             ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->rm_eo, ctx->ebuf_s, n, ctx->fb_to, out_len, out_index,
+             ctx->part);
+    finish_partials(ctx, s, eblocks, 1, 0u, 8);
+    long long t = 0;
+    int rc = sync_copy(ctx, s, &t, (long long*)ctx->scal + 8, sizeof(t), "nb_hydro_tight64: total copy failed");
+    if (rc) return rc;
+    *total_out = (int64_t)t;
+    return NB_OK;
+}
+
 // ===================================================================== fixed-capacity passes
 // The mixed step's grid and density passes with capacities fixed beforehand (nb_fixed_reserve):
 // no host synchronisation, no allocation and no memset, so that a caller can record a whole step
@@ -3174,6 +3575,7 @@ int build_grid_seg_fixed(nb_ctx* ctx, hipStream_t s, const double* pos, const do
     ctx->lists_ready = 0;
     ctx->hydro_ready = 0;
     ctx->hydro_mixed_ready = 0;
+    ctx->hydro64_ready = 0;
     if (!sort && ctx->fx_grid_from_order) {
         // the order pass's grid, checked against the particles as they now lie (a failure is
         // reported as not in cell order: the caller redoes the step eagerly)

@@ -215,6 +215,7 @@ __device__ __forceinline__ int64_t row_of(const int64_t* rows, int64_t q)
 // status codes of the C ABI (include/nebula_hip.h)
 #define NB_OK 0
 #define NB_RETRY_CAPACITY 1
+#define NB_NOT_GRID_ORDER 2  // nb_hydro_density64: particles not stored in grid order (nothing computed)
 #define NB_ERR_ARG -1
 #define NB_ERR_HIP -2
 #define NB_ERR_BOX -3

@@ -102,6 +102,7 @@ struct nb_ctx {
     int lists_ready = 0;
     int hydro_ready = 0;
     int hydro_mixed_ready = 0;
+    int hydro64_ready = 0;  // nb_hydro_density64's lists await nb_hydro_tight64
     int order_ident = 0;  // last grid: particles already stored in cell order (no order array)
     int tile = 256;       // workgroup size of the mixed-precision lane-group kernels (nb_set_tile)
     // settings of the last mixed density pass, for its emission pass

@@ -138,6 +138,9 @@ SIGNATURES = {
     "nb_hydro_density_mixed": (_i32, [_vp, _vp, _vp, _vp, _f64, _vp, _vp, _i64, _vp, _P64, _i32, _f64, _vp, _vp, _i32,
                                       _f64, _f64, _f64, _f64, _vp, _vp, _vp, _vp, _vp, _P64]),
     "nb_hydro_tight_mixed": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _P64]),
+    "nb_hydro_density64": (_i32, [_vp, _vp, _vp, _vp, _f64, _vp, _vp, _i64, _vp, _vp, _i32, _f64, _f64, _f64, _f64,
+                                  _vp, _vp, _vp, _vp, _vp, _P64]),
+    "nb_hydro_tight64": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _P64]),
     "nb_timing": (_i32, [_vp, _i32]),
     "nb_set_tile": (_i32, [_vp, _i32]),
     "nb_fixed_reserve": (_i32, [_vp, _i64, _i32, _i64, _i64, _i64, _i64, _i32]),

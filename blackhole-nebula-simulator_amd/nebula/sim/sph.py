@@ -31,6 +31,10 @@ COND_VELOCITY = 1
 GATE_BOTH, GATE_PUBLISHED, GATE_MINMOD = 0, 1, 2
 
 PRECISIONS = {"fp64": 0, "mixed": 1}
+# FP64 density passes of particles stored in grid order take the lane-group gather
+# (nb_hydro_density64; same bits as nb_hydro_density, which the parity tests run beside it)
+FP64_LANE_LISTS = True
+NB_NOT_GRID_ORDER = 2
 
 
 def _out(t, as_dev: bool):
@@ -253,8 +257,10 @@ def hydro_density_device(pos_d, mass_d, h_guess_d, kernel_name, eta, tol, h_max,
     (sph.py:329-387) on the fused gather + h-rho solve kernel, then the tightened lists at the
     final h, in 64-row blocks with capacities of their own. h_guess_d is overwritten.
     row_hint / row_counts: (N, 2) int32 device tensors, the previous pass's row counts of the
-    same particles (sizes the blocks) and this pass's (filled). Returns (DensityResult with the
-    tight lists as ``neighbours``, search-list entries)."""
+    same particles (sizes the blocks) and this pass's (filled). In FP64, particles stored in grid
+    order (the coupled step keeps them so) take the lane-group gather (nb_hydro_density64: the
+    same rows, sums and bits), others the per-thread one. Returns (DensityResult with the tight
+    lists as ``neighbours``, search-list entries)."""
     from .neighbours import BlockedLists
 
     n = int(pos_d.shape[0])
@@ -267,13 +273,19 @@ def hydro_density_device(pos_d, mass_d, h_guess_d, kernel_name, eta, tol, h_max,
     flags = _dev.empty(n, torch.int32)
     stats = np.zeros(4, np.int64)
     grown = np.zeros(1, np.int64)
+    lanes = False
     for builds in range(1, 13):
         start = h_start_d if (h_start_d is not None and builds == 1) else h_guess_d
-        _hip.call("nb_hydro_density", _hip.ptr(pos_d), _hip.ptr(mass_d), float(m_uniform), _hip.ptr(start),
-                  _hip.ptr(h_search), n, _hip.ptr(row_hint if builds == 1 or row_counts is None else row_counts),
-                  _hip.ptr(row_counts), kid, float(C_NORM[kid]), float(eta), float(tol), float(h_max),
-                  PRECISIONS[precision], _hip.ptr(h), _hip.ptr(rho), _hip.ptr(omega), _hip.ptr(iters), _hip.ptr(flags),
-                  stats.ctypes.data_as(_hip._P64))  # fmt: skip
+        hint = row_hint if builds == 1 or row_counts is None else row_counts
+        common = (n, _hip.ptr(hint), _hip.ptr(row_counts), kid, float(C_NORM[kid]), float(eta), float(tol),
+                  float(h_max))  # fmt: skip
+        outs = (_hip.ptr(h), _hip.ptr(rho), _hip.ptr(omega), _hip.ptr(iters), _hip.ptr(flags),
+                stats.ctypes.data_as(_hip._P64))  # fmt: skip
+        args = (_hip.ptr(pos_d), _hip.ptr(mass_d), float(m_uniform), _hip.ptr(start), _hip.ptr(h_search))
+        lanes = (precision == "fp64" and FP64_LANE_LISTS
+                 and _hip.call("nb_hydro_density64", *args, *common, *outs) != NB_NOT_GRID_ORDER)  # fmt: skip
+        if not lanes:
+            _hip.call("nb_hydro_density", *args, *common, PRECISIONS[precision], *outs)
         _hip.call("nb_grow_search", _hip.ptr(flags), _hip.ptr(h), None, n, float(np.inf), _hip.ptr(h_search),
                   _hip.ptr(h_guess_d), grown.ctypes.data_as(_hip._P64))  # fmt: skip
         if grown[0] == 0:
@@ -285,8 +297,12 @@ def hydro_density_device(pos_d, mass_d, h_guess_d, kernel_name, eta, tol, h_max,
     boff = _dev.empty((n + 63) // 64 + 1, torch.int64)
     idx = _dev.empty(entries, torch.int32) if index_buffer is None else index_buffer.take(entries)
     total = np.zeros(1, np.int64)
-    _hip.call("nb_hydro_tight", _hip.ptr(h), _hip.ptr(boff), _hip.ptr(ln), _hip.ptr(idx), entries,
-              total.ctypes.data_as(_hip._P64))  # fmt: skip
+    if lanes:
+        _hip.call("nb_hydro_tight64", _hip.ptr(pos_d), _hip.ptr(h), _hip.ptr(boff), _hip.ptr(ln), _hip.ptr(idx),
+                  entries, total.ctypes.data_as(_hip._P64))  # fmt: skip
+    else:
+        _hip.call("nb_hydro_tight", _hip.ptr(h), _hip.ptr(boff), _hip.ptr(ln), _hip.ptr(idx), entries,
+                  total.ctypes.data_as(_hip._P64))  # fmt: skip
     tight = BlockedLists(ln, idx, 0, int(total[0]), None, block_offset=boff)
     return DensityResult(h, rho, omega, iters, flags, tight, None, builds), int(stats[2])
 

@@ -8,7 +8,8 @@
  * synchronize the stream before returning.
  *
  * Return codes: 0 = ok; 1 = NB_RETRY_CAPACITY (output buffer too small, required size
- * reported, state kept for a follow-up call); negative = error, message from
+ * reported, state kept for a follow-up call); 2 = NB_NOT_GRID_ORDER (nb_hydro_density64 only);
+ * negative = error, message from
  * nb_last_error(ctx): -1 bad argument / failure, -2 HIP allocation, -3 kernel support larger
  * than half a periodic box (the reference's ValueError), -4 nothing built.
  *
@@ -399,6 +400,22 @@ int32_t nb_block_mark(nb_ctx *ctx, void *stream, const int64_t *rows, int64_t nr
 int32_t nb_hydro_tight_mixed(nb_ctx *ctx, void *stream, const double *pos, const double *h_final, double *rho,
                              double *omega, int64_t *out_off, int32_t *out_len, int32_t *out_index,
                              int64_t index_cap, int64_t *total_out);
+/* nb_hydro_density with its outputs, statistics and bits (sph.py:251-297, compute_density at
+ * sph.py:329-387 per build), gathered by 16-lane groups over trimmed column runs into rows of
+ * their own capacity (the lists of nb_hydro_density_mixed) and solved in FP64 with every sum in
+ * the reference's row order. Needs the particles stored in grid order (the coupled step's own
+ * permutation): otherwise nothing is computed and NB_NOT_GRID_ORDER (2) is returned, for the
+ * caller to take nb_hydro_density. stats4 as nb_hydro_density's (stats4[1] = 0). */
+int32_t nb_hydro_density64(nb_ctx *ctx, void *stream, const double *pos, const double *mass, double m_uniform,
+                           const double *h_start, const double *h_search, int64_t n, const int32_t *row_hint,
+                           int32_t *row_counts, int32_t kid, double cnorm, double eta, double tol, double h_max,
+                           double *h_out, double *rho, double *omega, int32_t *iters, int32_t *flags,
+                           int64_t *stats4);
+/* nb_hydro_tight after nb_hydro_density64 (neighbours.py:284-323 at h_final): the same rows in
+ * the same 64-row blocks (out_off from the density pass's block capacities, index_cap at least
+ * stats4[3]), each tight row ascending. pos as given to the density pass. */
+int32_t nb_hydro_tight64(nb_ctx *ctx, void *stream, const double *pos, const double *h_final, int64_t *out_off,
+                         int32_t *out_len, int32_t *out_index, int64_t index_cap, int64_t *total_out);
 /* Fixed-capacity passes for steps recorded in a hipGraph (no counterpart in the reference; for
  * callers that replay a captured step). nb_fixed_reserve sizes every buffer of the passes below
  * once (N particles, nseg systems, gather-row, extras and tight-index entries, grid cells;

@@ -56,17 +56,32 @@ def test_cloud_on_parabola_plain_fp64():
     _compare(st, ost, cfg)
 
 
-def test_stream_crash_cooling_balsara():
+def test_stream_crash_cooling_balsara(monkeypatch):
+    """Past the first step (whose smoothing lengths grow from the initial conditions and rebuild
+    the grid) the FP64 step's density pass is the lane-group one (nb_hydro_density64): the
+    comparison with the oracle covers it."""
     from dataclasses import replace
 
+    from nebula import _hip
     from nebula.ics import stream_crash
     from nebula.sim import coupled
 
+    called = []
+    real = _hip.call
+
+    def spy(name, *a):
+        called.append(name)
+        return real(name, *a)
+
+    monkeypatch.setattr(_hip, "call", spy)
     ic = stream_crash(8192, length=200.0, seed=20261014)
     base = coupled.baseline_config(balsara=True)
     cfg = replace(base, cooling_powerlaw=True)
     print("coupled: 8192-particle stream crash, power-law cooling + Balsara, 4 steps", flush=True)
     st, ost = _run(ic, cfg, 4)
+    lane_passes = called.count("nb_hydro_tight64")
+    print("density passes on the lane-group lists:", lane_passes, "of", called.count("nb_hydro_tight") + lane_passes)
+    check("steps after the first took the lane-group density pass", True, lane_passes >= 3)
     _compare(st, ost, cfg)
 
 
@@ -226,6 +241,88 @@ def test_fp64_step_lists_in_blocks_of_their_own_capacity(masses):
     check("every tight row fits its block", True, bool(np.all(ln <= np.repeat(cap, 64)[:n])))
     check("merged rows (gather + extras) hold the tight rows", True, bool(np.all(c0.cpu().numpy().sum(1) >= ln)))
     print(f"tight entries allocated {boff[-1]} vs uniform blocks {((n + 63) // 64) * 64 * int(cap.max())}")
+
+
+@pytest.mark.parametrize("case", ["cubic, uniform masses", "wendland, per-particle masses", "eta 2, light particles"])
+def test_fp64_lane_lists_equal_per_thread_pass(case, monkeypatch):
+    """nb_hydro_density64 / nb_hydro_tight64 (16-lane gather over trimmed column runs, FP64 terms
+    added in row order, tight rows merged from the gather row and the sorted extras) against
+    nb_hydro_density / nb_hydro_tight on particles stored in grid order: h, rho, omega,
+    iterations, flags, row counts, the blocks' offsets, the rows' lengths and every tight row
+    bit-identical, and the tight rows the reference's lists at the final h. The third case (eta
+    2, Wendland: about 270 neighbours a row; 64 particles of 1e-4 the mass, whose smoothing
+    lengths come out a fraction of their neighbours') has rows of more than 128 extras, which one
+    lane merges."""
+    import torch
+
+    from nebula import _dev, _hip
+    from nebula.ics import stream_crash
+    from nebula.sim import coupled, sph
+    from nebula.sim.neighbours import grid_order
+
+    from oracle import neighbours as on
+
+    light = case.startswith("eta 2")
+    kernel = "cubic" if case.startswith("cubic") else "wendland_c4"
+    eta = 2.0 if light else (1.2 if kernel == "cubic" else 1.6)
+    ic = stream_crash(20000, length=400.0, eta=eta, seed=11)
+    n = len(ic.h)
+    rng = np.random.default_rng(4)
+    mass, h0 = ic.mass.copy(), ic.h.copy()
+    if case.startswith("wendland"):
+        mass = mass * rng.uniform(0.5, 1.5, n)
+    if light:
+        few = rng.choice(n, 64, replace=False)
+        mass[few] *= 1e-4
+    hc = coupled.baseline_config().hydro
+    x_d, m_d, h_d = _dev.f64(ic.x), _dev.f64(mass), _dev.f64(h0)
+    mu = _dev.uniform_mass(m_d)
+    # start from solved smoothing lengths (as the step does after its first): one build, whose
+    # grid is the one the particles are ordered by
+    h_d = sph.hydro_density_device(x_d, m_d, h_d, kernel, eta, 1e-4, hc.h_max, hc.skin, mu, "fp64")[0].h
+    hs = _dev.empty(n)
+    _hip.call("nb_search_lengths", _hip.ptr(h_d), float(hc.h_max), float(hc.skin), float(np.inf), n, _hip.ptr(hs))
+    order = grid_order(x_d, hs).long()
+    x_d, m_d, h_d = x_d[order].contiguous(), m_d[order].contiguous(), h_d[order].contiguous()
+
+    called = []
+    real = _hip.call
+
+    def spy(name, *a):
+        called.append(name)
+        return real(name, *a)
+
+    monkeypatch.setattr(_hip, "call", spy)
+
+    def run(lanes):
+        monkeypatch.setattr(sph, "FP64_LANE_LISTS", lanes)
+        called.clear()
+        counts = _dev.empty((n, 2), torch.int32)
+        d, n_search = sph.hydro_density_device(x_d, m_d, h_d.clone(), kernel, eta, 1e-4, hc.h_max, hc.skin, mu,
+                                               "fp64", row_counts=counts)  # fmt: skip
+        return d, counts, n_search, list(called)
+
+    print(f"fp64 lane-group lists ({case}, {n} particles in grid order)", flush=True)
+    d0, c0, s0, calls0 = run(False)
+    d1, c1, s1, calls1 = run(True)
+    print("builds:", d0.builds, d1.builds, "longest extras segment:", int(c1[:, 1].max()), "calls:", calls1)
+    check("the per-thread pass ran without the lane-group entries", False, "nb_hydro_density64" in calls0)
+    check("the final lists came from the lane-group pass", True, "nb_hydro_tight64" in calls1)
+    for k in ("h", "rho", "omega", "iterations", "flags"):
+        bit_equal(f"{case}: {k}", getattr(d0, k).cpu().numpy(), getattr(d1, k).cpu().numpy())
+    bit_equal(f"{case}: row counts", c0.cpu().numpy(), c1.cpu().numpy())
+    check(f"{case}: search-list entries", s0, s1)
+    bit_equal(f"{case}: block offsets", d0.neighbours.block_offset.cpu().numpy(),
+              d1.neighbours.block_offset.cpu().numpy())  # fmt: skip
+    bit_equal(f"{case}: row lengths", d0.neighbours.length.cpu().numpy(), d1.neighbours.length.cpu().numpy())
+    t0, t1 = d0.neighbours.to_csr(), d1.neighbours.to_csr()
+    bit_equal(f"{case}: tight offsets", t0.offset, t1.offset)
+    bit_equal(f"{case}: tight indices", t0.index, t1.index)
+    ref = on.find_neighbours_grid(x_d.cpu().numpy(), d1.h.cpu().numpy())
+    bit_equal(f"{case}: tight rows = reference lists at the final h: offsets", ref.offset, t1.offset)
+    bit_equal(f"{case}: tight rows = reference lists at the final h: indices", ref.index, t1.index)
+    if light:
+        check("rows of more than 128 extras were merged", True, int(c1[:, 1].max()) > 128)
 
 
 def test_torus_shakura_sunyaev_cooling():

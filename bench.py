@@ -695,6 +695,11 @@ def main():
         # neighbour's h), each particle's (x, y, z, h) record and FP32 mass once, write the kept index
         "k_tight_emit": ("hbm", (4 + 8 + 8) * n_search * n + 4 * (n_pairs + 1) * n, "GB/s"),
         "k_tight_emit_rm": ("hbm", 4 * n_search * n + (32 + 4) * n + 4 * (n_pairs + 1) * n, "GB/s"),
+        # the FP64 step on the lane-group lists: the Newton solve over each stored row (its index
+        # read once per evaluation, the records from cache), and the tightening of the row merged
+        # with its extras into the 64-row blocks
+        "k_solve64": ("mfma", FLOPS_NEWTON_ENTRY * (k_it + 1) * (n_pairs + 1) * n, "TFLOP/s"),
+        "k_emit64": ("hbm", 4 * n_search * n + 32 * n + 4 * (n_pairs + 1) * n, "GB/s"),
     }
     timed = {k: v for k, v in kern.items() if k in models}
     dom = max(timed, key=lambda k: timed[k][1])
@@ -742,7 +747,7 @@ def main():
                      **_issue_and_hbm(dom, args.precision, avg_s, hbm_work)},
         "fp32_peak": flops["report"],
         "kernels_ms_per_step": _per_step(kern, args.steps),
-        "gather_launches_per_step": kern["k_gather_mixed" if not fp64 else "k_gather_solve"][0] / args.steps,
+        "gather_launches_per_step": kern.get("k_gather_mixed", kern.get("k_gather_solve", (0, 0.0)))[0] / args.steps,
         "learned_h": {k: v for k, v in info.items() if not k.startswith("_")} if learned else None,
         "newton_iters": info["newton_iters"],
     }
@@ -809,7 +814,9 @@ def step_flops(st, n, n_search, n_pairs, k_it, value, kern, steps, fp64):
     # each kernel's share: the gather runs the distance pass and the first Newton evaluation, the
     # emission the final density sums (and the tightening's distances again, not counted twice)
     kernel_flop = {
-        "k_gather_mixed": FLOPS_SEARCH_ENTRY * n_search + FLOPS_NEWTON_ENTRY * k_it * n_d,
+        # mixed: the first Newton evaluations run in the gather; FP64: all of them in k_solve64
+        "k_gather_mixed": FLOPS_SEARCH_ENTRY * n_search + (0.0 if fp64 else FLOPS_NEWTON_ENTRY * k_it * n_d),
+        "k_solve64": FLOPS_NEWTON_ENTRY * (k_it + 1.0) * n_d,
         "k_tight_emit_rm": FLOPS_NEWTON_ENTRY * n_d,
         "k_forces_mixed": FLOPS_FORCE_PAIR * n_pairs,
         "k_divcurl_mixed": FLOPS_GRAD_PAIR * n_pairs,

@@ -1926,21 +1926,135 @@ k_tight_emit_rm(const double4* __restrict__ posh, const uint32_t* __restrict__ r
 // search order, which for particles stored in grid order is ascending index order, and sorted
 // extras segments), with the reference's FP64 arithmetic bit for bit.
 
-// sph.py:215-228 for one row, 16 lanes: each lane forms one term of 16 consecutive entries (d2 from
-// the (x, y, z) records as the gather formed it; the terms of entries outside 2 min(h_search,
-// h_max) or past q = 2 are exact zeros), the group writes them to LDS, and every lane then adds
-// them in row order. The sums are therefore the reference's left-to-right sums, bit for bit
-// (adding +0.0 changes no sum here: neither sum can be -0.0).
+// sph.py:215-228 for one row, 16 lanes. Each lane forms the terms of entries glane, glane + 16,
+// ... (the terms of entries outside 2 min(h_search, h_max) or past q = 2 exact zeros); the
+// group writes each batch of 16 to LDS, and lane 0 adds the density terms, lane 1 the dW/dh
+// terms, in row order: the reference's left-to-right sums, bit for bit (adding +0.0 changes
+// neither sum: neither can be -0.0). cache_row forms r = sqrt(d2) (+inf outside 2 min(h_search,
+// h_max)) and takes the mass of each entry once: the first kRowCache batches into registers, the
+// next kRowLds batches into the group's LDS (each lane reads back only what it wrote), so the
+// Newton evaluations of rows up to 256 entries make no global load (q = r / h is the same
+// product either way); entries past them are formed from the records each evaluation.
+static constexpr int kRowCache = 8;  // batches of 16 entries held in each lane's registers
+static constexpr int kRowLds = 8;    // batches of 16 entries held in LDS (16 B each)
+
+template <bool kMasses>
+__device__ __forceinline__ void cache_row(const double4* __restrict__ posh, const double4& pi,
+                                          const uint32_t* __restrict__ row, int32_t m, const double* mass,
+                                          double r_max2, int glane, double (&rc)[kRowCache],
+                                          double (&mc)[kRowCache], double2* __restrict__ tc)
+{
+    auto entry = [&](int32_t k, double& r, double& mk) {
+        r = __builtin_inf();
+        mk = 0.0;
+        if (k < m) {
+            const uint32_t j = row[k];
+            const double4 pj = posh[j];
+            const double dx = pj.x - pi.x;
+            const double dy = pj.y - pi.y;
+            const double dz = pj.z - pi.z;
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 < r_max2) r = sqrt(d2);
+            if (kMasses) mk = mass[j];
+        }
+    };
+#pragma unroll
+    for (int b = 0; b < kRowCache; b++) {
+        double r, mk;
+        entry(b * kG + glane, r, mk);
+        rc[b] = r;
+        if (kMasses) mc[b] = mk;
+    }
+    for (int b = 0; b < kRowLds && (kRowCache + b) * kG < m; b++) {
+        double r, mk;
+        entry((kRowCache + b) * kG + glane, r, mk);
+        tc[b * kG + glane] = make_double2(r, mk);
+    }
+}
+
+// adds the batch's terms (cnt of them) in order: lane 0 of the group the density sum, lane 1
+// dW/dh (the other lanes skip the reads)
+__device__ __forceinline__ void add_batch(double2* terms, int glane, double trs, double tdws, int32_t cnt, double& acc)
+{
+    terms[glane] = make_double2(trs, tdws);
+    wave_sync();
+    if (glane < 2) {
+        const double* t = reinterpret_cast<const double*>(terms) + glane;
+        if (cnt == kG) {
+#pragma unroll
+            for (int l = 0; l < kG; l++) acc += t[2 * l];
+        } else {
+            for (int32_t l = 0; l < cnt; l++) acc += t[2 * l];
+        }
+    }
+    wave_sync();  // every lane has read the terms before the next batch overwrites them
+    // the next batch's terms are formed after this one is added (scheduling them earlier only
+    // holds more registers: the adds are a dependent chain either way)
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+// kernel_f and kernel_df of nb_common.h without branches: both pieces of the cubic spline are
+// formed by the same expressions and one is selected (the same bits for q < 2; the caller
+// discards q >= 2)
+template <int kKid>
+__device__ __forceinline__ void kernel_fdf(double q, double& f, double& df)
+{
+    if (kKid == 0) {
+        const double f_in = 1.0 - 1.5 * q * q + 0.75 * q * q * q;
+        const double df_in = -3.0 * q + 2.25 * q * q;
+        const double t = 2.0 - q;
+        const double f_out = 0.25 * t * t * t;
+        const double df_out = -0.75 * t * t;
+        const bool in = q < 1.0;
+        f = in ? f_in : f_out;
+        df = in ? df_in : df_out;
+    } else {
+        const double x = 1.0 - 0.5 * q;
+        const double x2 = x * x;
+        f = x2 * x2 * x2 * (35.0 / 12.0 * q * q + 3.0 * q + 1.0);
+        df = -7.0 / 3.0 * q * x2 * x2 * x * (5.0 * q + 2.0);
+    }
+}
+
+template <bool kMasses, int kKid>
 __device__ __forceinline__ void row_sums_group(const double4* __restrict__ posh, const double4& pi,
                                                const uint32_t* __restrict__ row, int32_t m, const double* mass,
-                                               double m_uniform, double r_max2, double h, int kid, double cnorm,
-                                               int glane, double2* terms, double& rs_out, double& dws_out)
+                                               double m_uniform, double r_max2, double h, double cnorm, int glane,
+                                               const double (&rc)[kRowCache], const double (&mc)[kRowCache],
+                                               const double2* __restrict__ tc, double2* terms, double& rs_out,
+                                               double& dws_out)
 {
-    double rs = 0.0, dws = 0.0;
+    double acc = 0.0;
     const double inv_h = 1.0 / h;
     const double ih3 = ipow3(inv_h);
     const double ncih4 = -cnorm * ipow4(inv_h);
-    for (int32_t k0 = 0; k0 < m; k0 += kG) {
+#pragma unroll
+    for (int b = 0; b < kRowCache; b++) {
+        const int32_t k0 = b * kG;
+        if (k0 >= m) break;
+        const double q = rc[b] * inv_h;
+        const double mk = kMasses ? mc[b] : m_uniform;
+        double fq, dfq;
+        kernel_fdf<kKid>(q, fq, dfq);
+        const double trs_q = mk * cnorm * ih3 * fq;
+        const double tdws_q = mk * (ncih4 * (3.0 * fq + q * dfq));
+        const bool inside = q < 2.0;  // also false for entries past the row or the search (r = inf)
+        add_batch(terms, glane, inside ? trs_q : 0.0, inside ? tdws_q : 0.0, m - k0 < kG ? m - k0 : kG, acc);
+    }
+    for (int b = 0; b < kRowLds; b++) {
+        const int32_t k0 = (kRowCache + b) * kG;
+        if (k0 >= m) break;
+        const double2 e = tc[b * kG + glane];
+        const double q = e.x * inv_h;
+        const double mk = kMasses ? e.y : m_uniform;
+        double fq, dfq;
+        kernel_fdf<kKid>(q, fq, dfq);
+        const double trs_q = mk * cnorm * ih3 * fq;
+        const double tdws_q = mk * (ncih4 * (3.0 * fq + q * dfq));
+        const bool inside = q < 2.0;
+        add_batch(terms, glane, inside ? trs_q : 0.0, inside ? tdws_q : 0.0, m - k0 < kG ? m - k0 : kG, acc);
+    }
+    for (int32_t k0 = (kRowCache + kRowLds) * kG; k0 < m; k0 += kG) {
         const int32_t k = k0 + glane;
         double trs = 0.0, tdws = 0.0;
         if (k < m) {
@@ -1953,33 +2067,28 @@ __device__ __forceinline__ void row_sums_group(const double4* __restrict__ posh,
             if (d2 < r_max2) {
                 const double q = sqrt(d2) * inv_h;
                 if (q < 2.0) {
-                    const double mk = mass ? mass[j] : m_uniform;
-                    const double fq = kernel_f(q, kid);
+                    const double mk = kMasses ? mass[j] : m_uniform;
+                    const double fq = kernel_f(q, kKid);
                     trs = mk * cnorm * ih3 * fq;
-                    tdws = mk * (ncih4 * (3.0 * fq + q * kernel_df(q, kid)));
+                    tdws = mk * (ncih4 * (3.0 * fq + q * kernel_df(q, kKid)));
                 }
             }
         }
-        terms[glane] = make_double2(trs, tdws);
-        wave_sync();
-        const int32_t cnt = m - k0 < kG ? m - k0 : kG;
-        for (int32_t l = 0; l < cnt; l++) {
-            const double2 t = terms[l];
-            rs += t.x;
-            dws += t.y;
-        }
-        wave_sync();  // every lane has read the terms before the next batch overwrites them
+        add_batch(terms, glane, trs, tdws, m - k0 < kG ? m - k0 : kG, acc);
     }
-    rs_out = rs;
-    dws_out = dws;
+    // lane 0 of the group holds the density sum, lane 1 dW/dh
+    rs_out = __shfl(acc, 0, kG);
+    dws_out = __shfl(acc, 1, kG);
 }
 
 // sph.py:251-297 for every row (as solve_row over a blocked row), 16 lanes per row.
-__global__ void __launch_bounds__(256)
+template <bool kMasses, int kKid>
+__global__ void __launch_bounds__(256, 4)
 k_solve64(const double4* __restrict__ posh, const uint32_t* __restrict__ ridx,
           const unsigned long long* __restrict__ goff, const int32_t* __restrict__ rowcnt, int64_t n, DensArgs A)
 {
     __shared__ double2 terms[256 / kG][kG];
+    __shared__ double2 tcache[256 / kG][kRowLds * kG];  // (r, mass) of row entries 128 .. 255
     const int glane = threadIdx.x & (kG - 1);
     const int64_t i = (xcd_tile() * (int64_t)blockDim.x + threadIdx.x) / kG;
     if (i >= n) return;  // whole groups leave together
@@ -1988,44 +2097,62 @@ k_solve64(const double4* __restrict__ posh, const uint32_t* __restrict__ ridx,
     const int32_t m = (int32_t)min((unsigned long long)rowcnt[i], goff[i + 1] - goff[i]);
     const uint32_t* row = ridx + goff[i];
     double2* tg = terms[threadIdx.x / kG];
-    const double* mass = A.m_uniform > 0.0 ? nullptr : A.mass;
+    const double* mass = A.mass;
     const double m_i = A.mass[i];
     const double r_max = 2.0 * pymin(h_search_i, A.h_max);
     const double r_max2 = r_max * r_max;
+    double rc[kRowCache], mc[kRowCache];
+    double2* tc = tcache[threadIdx.x / kG];
+    cache_row<kMasses>(posh, pi, row, m, mass, r_max2, glane, rc, mc, tc);
     double hi = pymin(h_search_i, A.h_max);
     double lo = 0.0;
     double h = pymin(A.h_start[i], hi);
     const double h0 = h;
+    const double h_lim = hi;
     int flag = 3;
     int it = 0;
     const double eta3 = ipow3(A.eta);
-    double rs, dws;
-    while (it < 50) {
-        it += 1;
-        row_sums_group(posh, pi, row, m, mass, A.m_uniform, r_max2, h, A.kid, A.cnorm, glane, tg, rs, dws);
-        const double f = rs - m_i * ipow3(A.eta / h);
-        const double df = dws + 3.0 * m_i * eta3 / ipow4(h);
-        if (f > 0) hi = h;
-        else lo = h;
-        double h_new = df > 0 ? h - f / df : 0.5 * (lo + hi);
-        if (!(lo <= h_new && h_new <= hi)) h_new = 0.5 * (lo + hi);
-        if (fabs(h_new - h) < A.tol * h0) {
-            h = h_new;
-            flag = 0;
+    double rs = 0.0, dws = 0.0;
+    // solve_row's evaluations from one call site: Newton steps (phase 0), the check at h_lim
+    // (phase 1, when h ends within 1e-3 of it), the sums at the final h (phase 2)
+    int phase = 0;
+    double h_eval = h;
+    for (;;) {
+        double s1, s2;
+        row_sums_group<kMasses, kKid>(posh, pi, row, m, mass, A.m_uniform, r_max2, h_eval, A.cnorm, glane, rc, mc, tc,
+                                      tg, s1, s2);
+        if (phase == 2) {
+            rs = s1;
+            dws = s2;
             break;
         }
-        h = h_new;
-    }
-    const double h_lim = pymin(h_search_i, A.h_max);
-    if (h >= h_lim * (1 - 1e-3)) {
-        double rs_lim, dum;
-        row_sums_group(posh, pi, row, m, mass, A.m_uniform, r_max2, h_lim, A.kid, A.cnorm, glane, tg, rs_lim, dum);
-        if (rs_lim - m_i * ipow3(A.eta / h_lim) < 0) {
+        if (phase == 0) {
+            it += 1;
+            const double f = s1 - m_i * ipow3(A.eta / h);
+            const double df = s2 + 3.0 * m_i * eta3 / ipow4(h);
+            if (f > 0) hi = h;
+            else lo = h;
+            double h_new = df > 0 ? h - f / df : 0.5 * (lo + hi);
+            if (!(lo <= h_new && h_new <= hi)) h_new = 0.5 * (lo + hi);
+            const bool done = fabs(h_new - h) < A.tol * h0;
+            h = h_new;
+            if (done) flag = 0;
+            if (!done && it < 50) {
+                h_eval = h;
+                continue;
+            }
+            if (h >= h_lim * (1 - 1e-3)) {
+                phase = 1;
+                h_eval = h_lim;
+                continue;
+            }
+        } else if (s1 - m_i * ipow3(A.eta / h_lim) < 0) {  // phase 1
             h = h_lim;
             flag = h_lim >= A.h_max ? 1 : 2;
         }
+        phase = 2;
+        h_eval = h;
     }
-    row_sums_group(posh, pi, row, m, mass, A.m_uniform, r_max2, h, A.kid, A.cnorm, glane, tg, rs, dws);
     if (glane == 0) {
         A.h_out[i] = h;
         A.rho[i] = rs;
@@ -3408,8 +3535,14 @@ extern "C" int32_t nb_hydro_density64(nb_ctx* ctx, void* stream, const double* p
     sort_extras(ctx, s, n, (unsigned)blocks, tile);
     DensArgs A{mass, m_uniform, h_start, h_max, kid, cnorm, eta, tol, h_out, rho, omega, iters, flags, ctx->pending,
                ctx->part};
-    NB_TIMED(ctx, s, "k_solve64", k_solve64, dim3((unsigned)blocks), dim3(tile), 0, s, (const double4*)ctx->posh,
-             ctx->gbuf, ctx->rm_go, ctx->rowcnt, n, A);
+#define NB_SOLVE64(M_, K_)                                                                                      \
+    NB_TIMED(ctx, s, "k_solve64", (k_solve64<M_, K_>), dim3((unsigned)blocks), dim3(tile), 0, s,                   \
+             (const double4*)ctx->posh, ctx->gbuf, ctx->rm_go, ctx->rowcnt, n, A)
+    if (masses && kid == 0) NB_SOLVE64(true, 0);
+    else if (masses) NB_SOLVE64(true, 1);
+    else if (kid == 0) NB_SOLVE64(false, 0);
+    else NB_SOLVE64(false, 1);
+#undef NB_SOLVE64
     // merged row lengths (the search-list entries) and the tight blocks' capacities
     NB_TIMED(ctx, s, "k_row_total_rm", k_row_total_rm, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, ctx->rowcnt,
              ctx->efill, ctx->rm_go, ctx->rm_eo, ctx->rm_gc, row_counts, ctx->part, n);

@@ -1288,8 +1288,9 @@ __device__ __forceinline__ void kernel_fdf32_k(float q, float& fq, float& dfq)
 // that read them form d2 again from the positions, in the same FP64 operations, so they see the
 // same d2 bits. kMasses (masses differ): row i stores its particle's FP32 mass in rmass[i], which
 // those passes read by index. The first-step sums take every candidate's FP32 term (zero off the
-// row) into FP64 per lane, branch-free.
-template <bool kMasses, bool kSeg, int kKid>
+// row) into FP64 per lane, branch-free. kSums false (the FP64 pass, whose solve sums every row
+// itself): the lists and extras only, no first step, masses or solve outputs.
+template <bool kMasses, bool kSeg, int kKid, bool kSums = true>
 __global__ void __launch_bounds__(256, 5)
 k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const GridSeg* __restrict__ segs,
                const int32_t* __restrict__ cloud, const uint32_t* __restrict__ start,
@@ -1405,7 +1406,7 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const Gr
                 const uint32_t k = kc;
                 j = order ? order[k] : k;
                 const double4 pj = posh[j];
-                if (kMasses) mjd = A.mass[j];
+                if (kSums && kMasses) mjd = A.mass[j];
                 const double dx = pj.x - xi;
                 const double dy = pj.y - yi;
                 const double dz = pj.z - zi;
@@ -1443,13 +1444,15 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const Gr
                 }
                 qm += __popc(fg);
             }
-            // the first-step sums on every lane: terms off the row add an exact zero
-            const float q = sqrt32((float)d2) * inv_h0;
-            float fq, dfq;
-            kernel_fdf32_k<kKid>(q, fq, dfq);
-            const float mj = kMasses ? (float)mjd : mu;
-            s1 += pred ? (double)(mj * fq) : 0.0;
-            s2 += pred ? (double)(mj * __builtin_fmaf(q, dfq, 3.f * fq)) : 0.0;
+            if (kSums) {
+                // the first-step sums on every lane: terms off the row add an exact zero
+                const float q = sqrt32((float)d2) * inv_h0;
+                float fq, dfq;
+                kernel_fdf32_k<kKid>(q, fq, dfq);
+                const float mj = kMasses ? (float)mjd : mu;
+                s1 += pred ? (double)(mj * fq) : 0.0;
+                s2 += pred ? (double)(mj * __builtin_fmaf(q, dfq, 3.f * fq)) : 0.0;
+            }
             m += __popc(gb);
         }
         // the chunk's entries, from the table to the row (contiguous per group)
@@ -1471,7 +1474,14 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const Gr
     }
     if (live && glane == 0) {
         rowcnt[i] = m;
-        if (kMasses) rmass[i] = (float)A.mass[i];  // the FP32 mass the sums of later passes take
+        if (kSums && kMasses) rmass[i] = (float)A.mass[i];  // the FP32 mass the sums of later passes take
+    }
+    if (!kSums) {
+        // [0] longest gather row, [2] an extras segment overflowed, [3] gather rows over
+        // capacity, [4] longest extras segment ([1] no rows left slow: none solved here)
+        int32_t pv[5] = {live ? m : 0, 0, (int32_t)eover, live && glane == 0 && m > cap ? 1 : 0, (int32_t)emax};
+        block_partials<5>(pv, 0b10101u, A.part);
+        return;
     }
     // the loop's first pass at h0 (sph.py:260-290), the Newton update and convergence test in FP64
     const double S1 = group_sum<kG>(s1), S2 = group_sum<kG>(s2);
@@ -1964,6 +1974,9 @@ __device__ __forceinline__ void cache_row(const double4* __restrict__ posh, cons
         entry(b * kG + glane, r, mk);
         rc[b] = r;
         if (kMasses) mc[b] = mk;
+        // four batches' records in flight at a time (all eight would hold registers the
+        // evaluations need)
+        if ((b & 3) == 3) __builtin_amdgcn_sched_barrier(0);
     }
     for (int b = 0; b < kRowLds && (kRowCache + b) * kG < m; b++) {
         double r, mk;
@@ -2028,6 +2041,10 @@ __device__ __forceinline__ void row_sums_group(const double4* __restrict__ posh,
     const double inv_h = 1.0 / h;
     const double ih3 = ipow3(inv_h);
     const double ncih4 = -cnorm * ipow4(inv_h);
+    // the row length as if redefined here (an empty statement): the batch tests and counts
+    // derived from it are formed in each evaluation instead of being hoisted out of the Newton
+    // loop, where they would hold registers across it
+    asm volatile("" : "+v"(m));
 #pragma unroll
     for (int b = 0; b < kRowCache; b++) {
         const int32_t k0 = b * kG;
@@ -3485,7 +3502,8 @@ extern "C" int32_t nb_hydro_density64(nb_ctx* ctx, void* stream, const double* p
     const int64_t cap_u = ctx->row_cap_rm > 0 ? ctx->row_cap_rm : 192;
     const int64_t ecap_u = ctx->ecap_rm > 0 ? ctx->ecap_rm : 32;
     long long sc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    // the gather's own first Newton step is discarded: k_solve64 solves every row
+    // the gather builds the lists only (k_solve64 solves every row); its arguments carry the
+    // partials and the solve's inputs
     MixArgs M{mass, m_uniform, h_start, h_max, kid, cnorm, eta, tol, h_out, rho, omega, iters, flags, ctx->pending,
               ctx->part};
     const int tile = 256;
@@ -3505,21 +3523,13 @@ extern "C" int32_t nb_hydro_density64(nb_ctx* ctx, void* stream, const double* p
                        "nb_hydro_density64: capacity copy failed");
         if (rc) return rc;
         if (ctx_reserve(ctx, (void**)&ctx->gbuf, &ctx->gbuf_bytes, sizeof(uint32_t) * (size_t)tot[0]) ||
-            ctx_reserve(ctx, (void**)&ctx->fscratch, &ctx->fscratch_bytes, sizeof(float) * (masses ? (size_t)n : 1)) ||
             ctx_reserve(ctx, (void**)&ctx->ebuf, &ctx->ebuf_bytes, sizeof(uint32_t) * (size_t)tot[1]))
             return NB_ERR_HIP;
         if (hipMemsetAsync(ctx->efill, 0, sizeof(unsigned int) * n, s) != hipSuccess) return ctx_fail(ctx, "memset");
-        float* rmass = masses ? ctx->fscratch : nullptr;
-#define NB_GATHER64(M_, K_)                                                                                        \
-    NB_TIMED(ctx, s, "k_gather_mixed", (k_gather_mixed<M_, false, K_>), dim3(blocks), dim3(tile), 0, s,           \
-             (const double4*)ctx->posh, n, g, (const GridSeg*)nullptr, (const int32_t*)nullptr, ctx->cell_start, \
-             (const uint32_t*)nullptr, ctx->gbuf, rmass, ctx->rm_go, ctx->rowcnt, ctx->efill, ctx->ebuf,           \
-             ctx->rm_eo, M)
-        if (masses && kid == 0) NB_GATHER64(true, 0);
-        else if (masses) NB_GATHER64(true, 1);
-        else if (kid == 0) NB_GATHER64(false, 0);
-        else NB_GATHER64(false, 1);
-#undef NB_GATHER64
+        NB_TIMED(ctx, s, "k_gather_mixed", (k_gather_mixed<false, false, 0, false>), dim3(blocks), dim3(tile), 0, s,
+                 (const double4*)ctx->posh, n, g, (const GridSeg*)nullptr, (const int32_t*)nullptr, ctx->cell_start,
+                 (const uint32_t*)nullptr, ctx->gbuf, (float*)nullptr, ctx->rm_go, ctx->rowcnt, ctx->efill,
+                 ctx->ebuf, ctx->rm_eo, M);
         finish_partials(ctx, s, blocks, 5, 1u | 4u | 16u, 0);
         rc = sync_copy(ctx, s, sc, ctx->scal, sizeof(long long) * 5, "nb_hydro_density64: gather failed");
         if (rc) return rc;

@@ -2546,10 +2546,16 @@ static hipError_t excl_sum(void* temp, size_t& bytes, const T* in, T* out, int64
     return rocprim::exclusive_scan(temp, bytes, in, out, T(0), (size_t)count, rocprim::plus<T>(), s);
 }
 
+// rocPRIM's default takes its block-sort + merge path up to 2^20 items (21 launches, 125 us for
+// 2^20 cell ids of 24 bits on MI355X); above 512K items its onesweep radix passes are faster
+// (107 us at 2^20, the same stable order), below it the merge path is (80 against 94 us at 256K)
+using SortConfig = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                              rocprim::default_config, 512 * 1024>;
+
 static hipError_t sort_pairs(void* temp, size_t& bytes, const uint32_t* kin, uint32_t* kout, const uint32_t* vin,
                              uint32_t* vout, int64_t n, int bits, hipStream_t s)
 {
-    return rocprim::radix_sort_pairs(temp, bytes, kin, kout, vin, vout, (size_t)n, 0u, (unsigned)bits, s);
+    return rocprim::radix_sort_pairs<SortConfig>(temp, bytes, kin, kout, vin, vout, (size_t)n, 0u, (unsigned)bits, s);
 }
 
 using namespace nb;

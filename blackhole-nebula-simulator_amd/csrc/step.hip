@@ -146,6 +146,19 @@ __global__ void k_axpy_dev(const double* __restrict__ y, const double* __restric
     if (i < m) out[i] = y[i] + s * a[i];
 }
 
+// k_axpy_dev over two pairs in one launch: out3 = y3 + s a3 (3 n values), out1 = y1 + s a1 (n):
+// a momentum and an energy kick of the same dt
+__global__ void k_kick2_dev(const double* __restrict__ y3, const double* __restrict__ a3,
+                            const double* __restrict__ y1, const double* __restrict__ a1,
+                            const double* __restrict__ s_dev, double scale, int64_t n, double* __restrict__ out3,
+                            double* __restrict__ out1)
+{
+    const double s = scale * s_dev[0];
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < 3 * n) out3[i] = y3[i] + s * a3[i];
+    else if (i < 4 * n) out1[i - 3 * n] = y1[i - 3 * n] + s * a1[i - 3 * n];
+}
+
 // eos.py:21-28 with hydro.py:130-131: P = (gamma-1) rho u, c_s = sqrt(gamma (gamma-1) u);
 // counts particles with u <= 0.
 __global__ void k_eos_ideal(const double* __restrict__ rho, const double* __restrict__ u, double gm1,
@@ -479,6 +492,18 @@ extern "C" int32_t nb_axpy_dev(nb_ctx* ctx, void* stream, const double* y, const
     return launch_check_s(ctx, "nb_axpy_dev");
 }
 
+extern "C" int32_t nb_kick2_dev(nb_ctx* ctx, void* stream, const double* y3, const double* a3, const double* y1,
+                                const double* a1, const double* s_dev, double scale, int64_t n, double* out3,
+                                double* out1)
+{
+    if (!ctx || n < 0 || !s_dev || !y3 || !a3 || !y1 || !a1 || !out3 || !out1)
+        return ctx_fail(ctx, "nb_kick2_dev: bad arguments");
+    if (n == 0) return NB_OK;
+    hipLaunchKernelGGL(k_kick2_dev, dim3(grid_s(4 * n)), dim3(kBlockS), 0, (hipStream_t)stream, y3, a3, y1, a1, s_dev,
+                       scale, n, out3, out1);
+    return launch_check_s(ctx, "nb_kick2_dev");
+}
+
 extern "C" int32_t nb_eos_ideal(nb_ctx* ctx, void* stream, const double* rho, const double* u, double gamma,
                                 int64_t n, double* pressure, double* cs, int32_t* nbad_dev)
 {
@@ -612,6 +637,81 @@ extern "C" int32_t nb_dt_advance(nb_ctx* ctx, void* stream, const double* minima
     hipLaunchKernelGGL(k_dt_advance, dim3(1), dim3(64), 0, (hipStream_t)stream, minima4_dev, courant, c_force,
                        (int)(viscous != 0), (int)(hole != 0), eta_orb, dts2_dev);
     return launch_check_s(ctx, "nb_dt_advance");
+}
+
+// ------------------------------------------------------- read-back of a step launched ahead
+// The counts a step driver reads back, reduced on the device: acc[0] += the per-row gather and
+// extras counts (the search-list entries), acc[1] += the tight-row lengths (integral doubles),
+// acc[2] += the rows that took more than one Newton step.
+__global__ void __launch_bounds__(kBlockS)
+k_step_sums(const int32_t* __restrict__ row_counts, const double* __restrict__ nneigh,
+            const int32_t* __restrict__ iters, int64_t n, unsigned long long* __restrict__ acc)
+{
+    long long a = 0, b = 0, c = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        a += (long long)row_counts[2 * i] + (long long)row_counts[2 * i + 1];
+        b += (long long)nneigh[i];
+        c += iters[i] > 1 ? 1 : 0;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
+        c += __shfl_xor(c, o, 64);
+    }
+    __shared__ long long sh[3][kBlockS / 64];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][w] = a;
+        sh[1][w] = b;
+        sh[2][w] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        long long t = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); k++) t += sh[threadIdx.x][k];
+        atomicAdd(&acc[threadIdx.x], (unsigned long long)t);
+    }
+}
+
+// out = [status words (as doubles), minima (4), search entries, tight entries, dts (2), slow
+// rows]; the status words and acc are left at zero for the next step
+__global__ void k_step_pack(int32_t* __restrict__ status, int nstatus, const double* __restrict__ minima,
+                            const double* __restrict__ dts, unsigned long long* __restrict__ acc,
+                            double* __restrict__ out)
+{
+    const int t = threadIdx.x;
+    if (t < nstatus) {
+        out[t] = (double)status[t];
+        status[t] = 0;
+    }
+    if (t < 4) out[nstatus + t] = minima[t];
+    if (t == 0) {
+        out[nstatus + 4] = (double)acc[0];
+        out[nstatus + 5] = (double)acc[1];
+        out[nstatus + 6] = dts[0];
+        out[nstatus + 7] = dts[1];
+        out[nstatus + 8] = (double)acc[2];
+        acc[0] = 0ull;
+        acc[1] = 0ull;
+        acc[2] = 0ull;
+    }
+}
+
+extern "C" int32_t nb_step_readback(nb_ctx* ctx, void* stream, int32_t* status, int32_t nstatus,
+                                    const double* minima4, const int32_t* row_counts, const double* nneigh,
+                                    const int32_t* iters, int64_t n, const double* dts2, unsigned long long* acc3,
+                                    double* out)
+{
+    if (!ctx || !status || !minima4 || !row_counts || !nneigh || !iters || !dts2 || !acc3 || !out || n < 0 ||
+        nstatus < 0 || nstatus > 64)
+        return ctx_fail(ctx, "nb_step_readback: bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t blocks = std::min<int64_t>(1024, (n + kBlockS - 1) / kBlockS);
+    if (blocks > 0)
+        hipLaunchKernelGGL(k_step_sums, dim3((unsigned)blocks), dim3(kBlockS), 0, s, row_counts, nneigh, iters, n,
+                           acc3);
+    hipLaunchKernelGGL(k_step_pack, dim3(1), dim3(64), 0, s, status, (int)nstatus, minima4, dts2, acc3, out);
+    return launch_check_s(ctx, "nb_step_readback");
 }
 
 // ------------------------------------------------------- nb_dt_minima_dev without allocation

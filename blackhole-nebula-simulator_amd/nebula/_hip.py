@@ -114,6 +114,8 @@ SIGNATURES = {
     "nb_dt_minima_dev": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _f64, _vp, _i64, _vp]),
     "nb_dt_minima_dev_static": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _f64, _vp, _i64, _vp]),
     "nb_dt_advance": (_i32, [_vp, _vp, _vp, _f64, _f64, _i32, _i32, _f64, _vp]),
+    "nb_step_readback": (_i32, [_vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp]),
+    "nb_kick2_dev": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _f64, _i64, _vp, _vp]),
     "nb_ledger": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _PD]),
     "nb_search_lengths": (_i32, [_vp, _vp, _vp, _f64, _f64, _f64, _i64, _vp]),
     "nb_grow_search": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _f64, _vp, _vp, _P64]),

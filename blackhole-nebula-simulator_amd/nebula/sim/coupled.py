@@ -185,6 +185,18 @@ def _axpy_dev(y, a, s_dev, scale):
     return out
 
 
+def _kick2(p, a, u, dudt, dts, scale):
+    """(p + s a, u + s dudt) with s = scale * dts[0] read on the device, in one launch."""
+    n = int(u.numel())
+    if p.numel() != 3 * n or a.numel() != 3 * n or dudt.numel() != n:
+        raise ValueError("a kick pairs (N, 3) momenta with N energies")
+    p_out = torch.empty_like(p)
+    u_out = torch.empty_like(u)
+    _hip.call("nb_kick2_dev", _hip.ptr(p), _hip.ptr(a), _hip.ptr(u), _hip.ptr(dudt), _hip.ptr(dts), float(scale),
+              n, _hip.ptr(p_out), _hip.ptr(u_out))  # fmt: skip
+    return p_out, u_out
+
+
 def _step_dt(fixed):
     """The device (dt, dt_prev) of a step recorded for replay (fixed.dts), or None."""
     return None if fixed is None else getattr(fixed, "dts", None)
@@ -464,8 +476,12 @@ def step(st: GasState, cfg: CoupledConfig, dt: float, fixed=None) -> GasState:
     def kick(y, a):
         return _axpy(y, a, half) if dts is None else _axpy_dev(y, a, dts, 0.5)
 
-    p_half = kick(st.p, st.a_hydro)
-    u_half = kick(st.u, st.dudt)
+    def kick2(p, a, u, dudt):  # a momentum and an energy kick of the same half step
+        if dts is None:
+            return kick(p, a), kick(u, dudt)
+        return _kick2(p, a, u, dudt, dts, 0.5)
+
+    p_half, u_half = kick2(st.p, st.a_hydro, st.u, st.dudt)
     x1, p_flow = gravity_flow(cfg, st.x, p_half, dt, fixed)
     e_rad = st.e_radiated_dev.clone()
     if math.isfinite(cfg.cooling_beta) or cfg.cooling_powerlaw:
@@ -496,8 +512,7 @@ def step(st: GasState, cfg: CoupledConfig, dt: float, fixed=None) -> GasState:
                           float(u_floor), k_cool, _hip.ptr(dts), n, _hip.ptr(u_cool), _hip.ptr(e_part))  # fmt: skip
             _hip.call("nb_sum_into", _hip.ptr(e_part), n, _hip.ptr(e_rad))
             u_half = u_cool
-    p_pred = kick(p_flow, st.a_hydro)
-    u_pred = kick(u_half, st.dudt)
+    p_pred, u_pred = kick2(p_flow, st.a_hydro, u_half, st.dudt)
     order = grid_permutation(cfg, x1, st.h, st, fixed)
     fields = [x1, p_flow, p_pred, u_half, u_pred, st.a_hydro, st.mass, st.h, st.alpha, st.perm]
     counts = st.row_counts
@@ -523,13 +538,14 @@ def step(st: GasState, cfg: CoupledConfig, dt: float, fixed=None) -> GasState:
             h_start = model.h_start(h, feat)
             fallbacks = model.last_fallbacks
     new = _hydro(pred, cfg, velocity(cfg, x1, p_pred), dt, a_prev, h_start=h_start, fixed=fixed)
-    if gate is not None:
+    if gate is not None and not getattr(fixed, "reads_back", False):  # else the driver observes it
         gate.observe((new.iterations > 1).sum(), n, use_net)
     if fixed is not None:
         fixed.pairs = (model, feat, h, new.h)
     else:
         log_pairs(cfg, model, feat, h, new.h)
-    return replace(new, p=kick(p_flow, new.a_hydro), u=kick(u_half, new.dudt), e_radiated_dev=e_rad,
+    p1, u1 = kick2(p_flow, new.a_hydro, u_half, new.dudt)
+    return replace(new, p=p1, u=u1, e_radiated_dev=e_rad,
                    dt_prev=float(dt), learned_fallbacks_dev=fallbacks)
 
 

@@ -155,6 +155,10 @@ class PipelinedStepper(Stepper):
         self.dts = torch.zeros(2, dtype=torch.float64, device=st.x.device)
         super().__init__(st, cfg, headroom)
         self._pinned = [torch.empty(24, dtype=torch.float64).pin_memory() for _ in range(2)]
+        # nb_step_readback's output (the 24 values read back, then the gate's slow-row count) and
+        # its three counters (zero between calls)
+        self._rb_out = torch.zeros(25, dtype=torch.float64, device=st.x.device)
+        self._rb_acc = torch.zeros(3, dtype=torch.int64, device=st.x.device)
         self._slot = 0
         self._flight = None  # the launched step not yet read back
         self.overlapped = 0  # steps launched before their predecessor was read back
@@ -163,6 +167,7 @@ class PipelinedStepper(Stepper):
     def _reserve(self):
         super()._reserve()
         self.fixed.dts = self.dts
+        self.fixed.reads_back = True  # the gate's count comes from nb_step_readback
 
     def _launch(self, st, host_dt=None):
         """Enqueue one step from st. host_dt: its dt from the host (written to the device first);
@@ -174,7 +179,7 @@ class PipelinedStepper(Stepper):
         if host_dt is not None:
             self.dts[0].fill_(float(host_dt))
             self.dts[1].fill_(float(st.dt_prev))
-        self.status.zero_()
+            self.status.zero_()  # later launches find it cleared by nb_step_readback
         self.fixed.pairs = None
         # coupled.step takes the dt from self.dts; the host value only dates the state (set again
         # once the step is read back)
@@ -184,10 +189,16 @@ class PipelinedStepper(Stepper):
         hc = cfg.hydro
         _hip.call("nb_dt_advance", _hip.ptr(self.minima), float(hc.courant), float(hc.c_force),
                   int(new.nu is not None), int(bool(cfg.hole)), float(cfg.eta_orb), _hip.ptr(self.dts))  # fmt: skip
-        totals = torch.stack([new.row_counts.sum(dtype=torch.int64), new.nneigh.sum().to(torch.int64)]).double()
+        if new.iterations.dtype != torch.int32 or new.row_counts.dtype != torch.int32:
+            raise TypeError("nb_step_readback takes int32 iteration and row counts")
+        _hip.call("nb_step_readback", _hip.ptr(self.status), int(self.status.numel()), _hip.ptr(self.minima),
+                  _hip.ptr(new.row_counts), _hip.ptr(new.nneigh), _hip.ptr(new.iterations), self.n,
+                  _hip.ptr(self.dts), _hip.ptr(self._rb_acc), _hip.ptr(self._rb_out))  # fmt: skip
+        if gate is not None:
+            gate.observe(self._rb_out[24], self.n, self.fixed.use_net)
         buf = self._pinned[self._slot]
         self._slot ^= 1
-        buf.copy_(torch.cat([self.status.double(), self.minima, totals, self.dts]), non_blocking=True)
+        buf.copy_(self._rb_out[:24], non_blocking=True)
         done = torch.cuda.Event()
         done.record()
         return {"state": new, "buf": buf, "done": done, "snap": snap, "pairs": self.fixed.pairs}

@@ -529,6 +529,12 @@ int32_t nb_axpy(nb_ctx *ctx, void *stream, const double *y, const double *a, dou
  * reference: the kicks of a step recorded for replay). */
 int32_t nb_axpy_dev(nb_ctx *ctx, void *stream, const double *y, const double *a, const double *s_dev, double scale,
                     int64_t m, double *out);
+/* nb_axpy_dev on a momentum (n x 3) and an energy (n) pair with the same dt, in one launch
+ * (coupled.py:231-233, 262-263 and the predictor kick; no separate counterpart in the reference):
+ * out3 = y3 + scale s_dev[0] a3, out1 = y1 + scale s_dev[0] a1, the same values as two nb_axpy_dev
+ * calls. */
+int32_t nb_kick2_dev(nb_ctx *ctx, void *stream, const double *y3, const double *a3, const double *y1, const double *a1,
+                     const double *s_dev, double scale, int64_t n, double *out3, double *out1);
 
 /* eos_state, ideal branch (hydro.py:121-139; eos.py:21-28); nbad_dev counts u <= 0. */
 int32_t nb_eos_ideal(nb_ctx *ctx, void *stream, const double *rho, const double *u, double gamma, int64_t n,
@@ -590,6 +596,16 @@ int32_t nb_dt_minima_dev_static(nb_ctx *ctx, void *stream, const double *h, cons
  * that launches the next step before it reads this one back. */
 int32_t nb_dt_advance(nb_ctx *ctx, void *stream, const double *minima4_dev, double courant, double c_force,
                       int32_t viscous, int32_t hole, double eta_orb, double *dts2_dev);
+
+/* What a step driver that launches the next step before reading this one back reads back, in
+ * one device buffer (no counterpart in the reference: the sums coupled.step's caller forms on
+ * the host, GasState.n_search_entries and n_tight_entries, and the gate's count of rows that
+ * took more than one Newton step): out (device, nstatus + 9 doubles) = [status words, minima4,
+ * sum of row_counts (2 int32 per particle), sum of nneigh, dts2, count of iters > 1]. The status
+ * words are cleared for the next step; acc3 (device, 3 uint64, zero on entry) is left zero. */
+int32_t nb_step_readback(nb_ctx *ctx, void *stream, int32_t *status, int32_t nstatus, const double *minima4,
+                         const int32_t *row_counts, const double *nneigh, const int32_t *iters, int64_t n,
+                         const double *dts2, unsigned long long *acc3, double *out);
 
 /* Ensemble helpers (no counterpart in the reference; the systems of nb_grid_order_seg):
  * nb_dt_minima_seg: nb_dt_minima per system, out (host) nseg x 4; nb_axpy_rows: out row i = y row i

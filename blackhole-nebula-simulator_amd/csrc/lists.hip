@@ -284,7 +284,7 @@ k_cell_id_seg(const double* __restrict__ pos, int64_t n, const GridSeg* __restri
         const uint32_t c = seg_key(pos, i, segs, cloud);
         cell_id[i] = c;
         iota[i] = (uint32_t)i;
-        atomicAdd(&cell_count[c], 1u);
+        if (cell_count) atomicAdd(&cell_count[c], 1u);  // NULL: the starts come from the sorted ids
         if (part && i + 1 < n && seg_key(pos, i + 1, segs, cloud) < c) bad = 1;
     }
     if (part) block_partial<false>(bad, part);
@@ -3673,8 +3673,8 @@ int scan_fixed(nb_ctx* ctx, hipStream_t s, const T* in, T* out, int64_t count)
 }
 
 // stable LSD radix sort of (ctx->cell_id, ctx->iota) by the key's low fx_bits bits, values into
-// ctx->order (keys and ping-pong buffers clobbered)
-int sort_fixed(nb_ctx* ctx, hipStream_t s, int64_t n)
+// ctx->order (keys and ping-pong buffers clobbered); *sorted_keys: where the sorted keys are
+int sort_fixed(nb_ctx* ctx, hipStream_t s, int64_t n, const uint32_t** sorted_keys)
 {
     if (!capturing(s)) {
         // not being recorded: the library's (also stable) radix sort over the same bits
@@ -3684,6 +3684,7 @@ int sort_fixed(nb_ctx* ctx, hipStream_t s, int64_t n)
         if (ctx_reserve(ctx, (void**)&ctx->cub_tmp, &ctx->cub_tmp_bytes, bytes)) return NB_ERR_HIP;
         if (sort_pairs(ctx->cub_tmp, bytes, ctx->cell_id, ctx->keys_out, ctx->iota, ctx->order, n, ctx->fx_bits, s) != hipSuccess)
             return ctx_fail(ctx, "sort failed");
+        *sorted_keys = ctx->keys_out;
         return NB_OK;
     }
     const int64_t nblocks = (n + kSortThreads - 1) / kSortThreads;
@@ -3711,6 +3712,109 @@ int sort_fixed(nb_ctx* ctx, hipStream_t s, int64_t n)
     }
     if (va != ctx->order)  // an even number of passes ends in the iota buffer
         hipLaunchKernelGGL(k_copy_u32, dim3(grid_of(n)), dim3(kBlock), 0, s, va, ctx->order, n);
+    *sorted_keys = ka;
+    return NB_OK;
+}
+
+// Cell starts from the sorted cell ids of a stable sort: start[c] = the first sorted position
+// whose id is >= c (the exclusive scan of the cells' counts), for c = 0 .. the grid's cell count
+// (no further: later entries of the ncap + 1 are never read). Thread k of 0 .. n owns the run of
+// cells keys[k - 1] < c <= keys[k] (thread n: up to the cell count) and k is written
+// to each: runs of up to kStartLane cells by their thread, up to kStartRun by the whole
+// wavefront (one run at a time), longer ones as chunks of kStartChunk cells through a work list
+// that k_starts_fill hands out to whole workgroups. work holds 3 words per chunk; a run of
+// L > kStartRun cells makes ceil(L / kStartChunk) chunks, so ncap + 1 cells make at most
+// (ncap + 1) (1 / kStartChunk + 1 / (kStartRun + 1)) of them.
+static constexpr int kStartLane = 8;
+static constexpr int kStartRun = 256;
+static constexpr int kStartChunk = 4096;
+
+__global__ void __launch_bounds__(kBlock)
+k_starts_from_keys(const uint32_t* __restrict__ keys, int64_t n, int64_t ncap, const GridSeg* __restrict__ segs,
+                   int nseg, uint32_t* __restrict__ start, uint32_t* __restrict__ work,
+                   unsigned int* __restrict__ nwork, int64_t work_cap)
+{
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t lo = 0, hi = -1;  // the thread's run (empty past n)
+    if (k <= n) {
+        lo = k == 0 ? 0 : (int64_t)keys[k - 1] + 1;
+        int64_t top = ncap;
+        if (k == n) {  // the last run ends at the grid's cell count
+            const GridSeg& last = segs[nseg - 1];
+            const int64_t cells = (int64_t)last.base + (int64_t)last.g.nc[0] * last.g.nc[1] * last.g.nc[2];
+            top = cells < ncap ? cells : ncap;
+        }
+        const int64_t hk = k == n ? top : (int64_t)keys[k];
+        hi = hk < ncap ? hk : ncap;  // inclusive (ids are below ncap + 1)
+    }
+    const int64_t len = hi - lo + 1;
+    if (len > 0 && len <= kStartLane)
+        for (int64_t c = lo; c <= hi; c++) start[c] = (uint32_t)k;
+    // every lane takes part (no early exit above): the loop bound is the same on all of them
+    unsigned long long mask = __ballot(len > kStartLane && len <= kStartRun);
+    const int lane = threadIdx.x & 63;
+    while (mask) {
+        const int src = __ffsll((unsigned long long)mask) - 1;
+        mask &= mask - 1ull;
+        const uint32_t a = __shfl((uint32_t)lo, src, 64), b = __shfl((uint32_t)hi, src, 64);
+        const uint32_t v = __shfl((uint32_t)k, src, 64);
+        for (int64_t c = (int64_t)a + lane; c <= (int64_t)b; c += 64) start[c] = v;
+    }
+    // longer runs: their chunks' slots in the work list, one atomic per wavefront
+    const int chunks = len > kStartRun ? (int)((len + kStartChunk - 1) / kStartChunk) : 0;
+    int incl = chunks;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    const int wave_total = __shfl(incl, 63, 64);
+    unsigned int base = 0;
+    if (lane == 63 && wave_total > 0) base = atomicAdd(nwork, (unsigned int)wave_total);
+    base = __shfl(base, 63, 64);
+    if (chunks > 0) {
+        const int64_t w = (int64_t)base + (incl - chunks);
+        for (int64_t q = 0; q < chunks; q++) {
+            const int64_t idx = w + q;
+            if (idx >= work_cap) break;  // cannot happen for ids <= ncap (the bound above)
+            const int64_t c0 = lo + q * kStartChunk;
+            work[3 * idx] = (uint32_t)c0;
+            work[3 * idx + 1] = (uint32_t)(c0 + kStartChunk - 1 < hi ? c0 + kStartChunk - 1 : hi);
+            work[3 * idx + 2] = (uint32_t)k;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_starts_fill(uint32_t* __restrict__ start, const uint32_t* __restrict__ work, const unsigned int* __restrict__ nwork,
+              int64_t work_cap)
+{
+    const int64_t nw = (int64_t)*nwork < work_cap ? (int64_t)*nwork : work_cap;
+    for (int64_t item = blockIdx.x; item < nw; item += gridDim.x) {
+        const int64_t a = work[3 * item], b = work[3 * item + 1];
+        const uint32_t v = work[3 * item + 2];
+        for (int64_t c = a + threadIdx.x; c <= b; c += blockDim.x) start[c] = v;
+    }
+}
+
+// whether the work list of the starts of cells 0 .. ncap fits in ncap + 1 words
+inline bool starts_fit(int64_t ncap)
+{
+    return (ncap + 1) / kStartChunk + (ncap + 1) / (kStartRun + 1) + 1 <= ncap / 3;
+}
+
+// the starts of cells 0 .. the grid's cell count (<= ncap) from n sorted ids; scratch: ncap + 1 words (the
+// cells' count buffer, not needed by this path) for the work list and its counter
+int starts_from_keys(nb_ctx* ctx, hipStream_t s, const uint32_t* keys, int64_t n, int64_t ncap, const GridSeg* segs,
+                     int nseg, uint32_t* start, uint32_t* scratch)
+{
+    unsigned int* nwork = (unsigned int*)(scratch + ncap);  // the last word
+    const int64_t work_cap = ncap / 3;
+    if (!starts_fit(ncap)) return ctx_fail(ctx, "cell starts: work list does not fit");
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, (uint32_t*)nwork, 0u, (int64_t)1);
+    hipLaunchKernelGGL(k_starts_from_keys, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, keys, n, ncap, segs, nseg, start,
+                       scratch, nwork, work_cap);
+    hipLaunchKernelGGL(k_starts_fill, dim3(1024), dim3(kBlock), 0, s, start, (const uint32_t*)scratch,
+                       (const unsigned int*)nwork, work_cap);
     return NB_OK;
 }
 
@@ -3742,15 +3846,22 @@ int build_grid_seg_fixed(nb_ctx* ctx, hipStream_t s, const double* pos, const do
     hipLaunchKernelGGL(k_plan_seg, dim3(1), dim3(64), 0, s, ctx->red, off_dev, nseg, cell_factor, (double)ctx->fx_cells,
                        (double)kMaxAxisCells, kMaxTotalCells, (GridSeg*)ctx->segs, status);
     const int64_t nc1 = ctx->fx_cells + 1;
-    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)std::min<int64_t>(4096, (nc1 + kBlock - 1) / kBlock)), dim3(kBlock), 0,
-                       s, ctx->cell_count, 0u, nc1);
+    // sorting: the starts from the sorted ids (no histogram, no scan over the cells)
+    const bool from_keys = sort && starts_fit(nc1 - 1);
+    if (!from_keys)
+        hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)std::min<int64_t>(4096, (nc1 + kBlock - 1) / kBlock)),
+                           dim3(kBlock), 0, s, ctx->cell_count, 0u, nc1);
     NB_TIMED(ctx, s, "k_cell_id", k_cell_id_seg, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, n, (const GridSeg*)ctx->segs,
-             cloud, ctx->cell_id, ctx->iota, ctx->cell_count, sort ? nullptr : ctx->part);
+             cloud, ctx->cell_id, ctx->iota, from_keys ? nullptr : ctx->cell_count, sort ? nullptr : ctx->part);
     if (sort) {
-        int rc = sort_fixed(ctx, s, n);
+        const uint32_t* skeys = nullptr;
+        int rc = sort_fixed(ctx, s, n, &skeys);
         if (rc) return rc;
         ctx->order_ident = 0;
         ctx->fx_grid_from_order = 1;  // the density pass after the permutation may reuse this grid
+        if (from_keys)
+            return starts_from_keys(ctx, s, skeys, n, nc1 - 1, (const GridSeg*)ctx->segs, nseg, ctx->cell_start,
+                                    ctx->cell_count);
     } else {
         finish_partials(ctx, s, grid_of(n), 1, 0u, 0);
         hipLaunchKernelGGL(k_status_from, dim3(1), dim3(64), 0, s, (const long long*)ctx->scal, 0, status, kStUnsorted);

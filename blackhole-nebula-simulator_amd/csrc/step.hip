@@ -706,7 +706,8 @@ extern "C" int32_t nb_step_readback(nb_ctx* ctx, void* stream, int32_t* status, 
         nstatus < 0 || nstatus > 64)
         return ctx_fail(ctx, "nb_step_readback: bad arguments");
     hipStream_t s = (hipStream_t)stream;
-    const int64_t blocks = std::min<int64_t>(1024, (n + kBlockS - 1) / kBlockS);
+    // 256 workgroups (a few particles per thread): three atomics each on three shared counters
+    const int64_t blocks = std::min<int64_t>(256, (n + kBlockS - 1) / kBlockS);
     if (blocks > 0)
         hipLaunchKernelGGL(k_step_sums, dim3((unsigned)blocks), dim3(kBlockS), 0, s, row_counts, nneigh, iters, n,
                            acc3);

@@ -2405,6 +2405,70 @@ k_scan_apply(const T* __restrict__ in, int64_t count, const T* __restrict__ sums
     }
 }
 
+// Two exclusive scans of count values each in one set of launches (blockIdx.y or, for the tile
+// sums, blockIdx.x picks the array), each result clamped to its capacity: out_a[i] = min(scan,
+// cap_a), and status[word_a] = 1 when the total (entry count - 1) exceeds cap_a (rows past the
+// capacity then get none: the pass overflows and says so).
+struct Scan2 {
+    const unsigned long long* in[2];
+    unsigned long long* out[2];
+    unsigned long long cap[2];
+    int word[2];
+};
+
+__global__ void __launch_bounds__(kScanThreads) k_scan2_tiles(Scan2 S, int64_t count, unsigned long long* __restrict__ sums,
+                                                              int64_t ntiles)
+{
+    const unsigned long long* in = S.in[blockIdx.y];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    unsigned long long v = 0;
+    for (int k = 0; k < kScanItems; k++)
+        if (base + k < count) v += in[base + k];
+    unsigned long long total;
+    block_exclusive(v, total);
+    if (threadIdx.x == 0) sums[blockIdx.y * ntiles + blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan2_sums(unsigned long long* __restrict__ sums, int64_t ntiles)
+{
+    unsigned long long* sm = sums + blockIdx.x * ntiles;
+    unsigned long long carry = 0;
+    for (int64_t c = 0; c < ntiles; c += kScanThreads) {
+        const int64_t i = c + threadIdx.x;
+        const unsigned long long v = i < ntiles ? sm[i] : 0ull;
+        unsigned long long total;
+        const unsigned long long ex = block_exclusive(v, total);
+        if (i < ntiles) sm[i] = carry + ex;
+        carry += total;
+    }
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan2_apply(Scan2 S, int64_t count,
+                                                              const unsigned long long* __restrict__ sums,
+                                                              int64_t ntiles, int32_t* __restrict__ status)
+{
+    const int a = blockIdx.y;
+    const unsigned long long* in = S.in[a];
+    unsigned long long* out = S.out[a];
+    const unsigned long long cap = S.cap[a];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    unsigned long long v[kScanItems];
+    unsigned long long local = 0;
+    for (int k = 0; k < kScanItems; k++) {
+        v[k] = base + k < count ? in[base + k] : 0ull;
+        local += v[k];
+    }
+    unsigned long long total;
+    unsigned long long run = sums[a * ntiles + blockIdx.x] + block_exclusive(local, total);
+    for (int k = 0; k < kScanItems; k++) {
+        if (base + k < count) {
+            out[base + k] = run < cap ? run : cap;
+            if (base + k == count - 1 && run > cap) status[S.word[a]] = 1;
+        }
+        run += v[k];
+    }
+}
+
 static constexpr int kSortThreads = 256;
 static constexpr int kRadixBits = 4;
 static constexpr int kRadix = 1 << kRadixBits;
@@ -2505,20 +2569,6 @@ __global__ void k_plan_seg(const double* __restrict__ bb, const int64_t* __restr
         grow *= 1.25;
     }
     status[kStCells] = 1;
-}
-
-// offsets (exclusive scan, n + 1 entries) clamped to the fixed capacity; status[word] set when
-// the total did not fit (rows past the capacity then get none: the pass overflows and says so)
-__global__ void k_clamp_off(unsigned long long* __restrict__ off, int64_t n1, unsigned long long cap,
-                            int32_t* __restrict__ status, int word)
-{
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n1) return;
-    const unsigned long long v = off[i];
-    if (v > cap) {
-        off[i] = cap;
-        if (i == n1 - 1) status[word] = 1;
-    }
 }
 
 // status[word] = 1 when the collected scalar is nonzero
@@ -3672,6 +3722,21 @@ int scan_fixed(nb_ctx* ctx, hipStream_t s, const T* in, T* out, int64_t count)
     return NB_OK;
 }
 
+// the gather and extras offsets of the fixed density pass: both scans and their clamps in three
+// launches (against two library scans and two clamp passes)
+int scan2_fixed(nb_ctx* ctx, hipStream_t s, const Scan2& S, int64_t count, int32_t* status)
+{
+    const int64_t tiles = (count + kScanTile - 1) / kScanTile;
+    if (sizeof(unsigned long long) * (size_t)(2 * tiles) > ctx->fx_tmp_bytes)
+        return ctx_fail(ctx, "fixed pass: scan scratch not reserved");
+    unsigned long long* sums = (unsigned long long*)ctx->fx_tmp;
+    hipLaunchKernelGGL(k_scan2_tiles, dim3((unsigned)tiles, 2), dim3(kScanThreads), 0, s, S, count, sums, tiles);
+    hipLaunchKernelGGL(k_scan2_sums, dim3(2), dim3(kScanThreads), 0, s, sums, tiles);
+    hipLaunchKernelGGL(k_scan2_apply, dim3((unsigned)tiles, 2), dim3(kScanThreads), 0, s, S, count,
+                       (const unsigned long long*)sums, tiles, status);
+    return NB_OK;
+}
+
 // stable LSD radix sort of (ctx->cell_id, ctx->iota) by the key's low fx_bits bits, values into
 // ctx->order (keys and ping-pong buffers clobbered); *sorted_keys: where the sorted keys are
 int sort_fixed(nb_ctx* ctx, hipStream_t s, int64_t n, const uint32_t** sorted_keys)
@@ -3972,14 +4037,14 @@ extern "C" int32_t nb_hydro_density_mixed_fixed(nb_ctx* ctx, void* stream, const
              (double4*)ctx->posh);
     hipLaunchKernelGGL(k_caps_mixed, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, row_hint, ctx->rowcnt, ctx->efill, n, 1,
                        (int64_t)0, (int64_t)0, ctx->rm_gc, ctx->rm_ec);
-    rc = scan_fixed(ctx, s, (const unsigned long long*)ctx->rm_gc, ctx->rm_go, n + 1);
-    if (rc) return rc;
-    rc = scan_fixed(ctx, s, (const unsigned long long*)ctx->rm_ec, ctx->rm_eo, n + 1);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_clamp_off, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, ctx->rm_go, n + 1,
-                       (unsigned long long)ctx->fx_gather, status, (int)kStGather);
-    hipLaunchKernelGGL(k_clamp_off, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, ctx->rm_eo, n + 1,
-                       (unsigned long long)ctx->fx_extras, status, (int)kStExtras);
+    {
+        const Scan2 S{{(const unsigned long long*)ctx->rm_gc, (const unsigned long long*)ctx->rm_ec},
+                      {ctx->rm_go, ctx->rm_eo},
+                      {(unsigned long long)ctx->fx_gather, (unsigned long long)ctx->fx_extras},
+                      {(int)kStGather, (int)kStExtras}};
+        rc = scan2_fixed(ctx, s, S, n + 1, status);
+        if (rc) return rc;
+    }
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_of(n)), dim3(kBlock), 0, s, (uint32_t*)ctx->efill, 0u, n);
     MixArgs A{mass, m_uniform, h_start, h_max, kid, cnorm, eta, tol, h_out, rho, omega, iters, flags, ctx->pending,
               ctx->part};

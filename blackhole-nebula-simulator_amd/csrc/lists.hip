@@ -1182,6 +1182,8 @@ struct MixArgs {
     int32_t* flags;
     uint8_t* slow;    // 1: row solved by k_solve_slow_mixed (not converged in one Newton step)
     long long* part;  // per-workgroup partials: longest row, slow rows
+    uint32_t* slow_list = nullptr;        // the slow rows (in no particular order), for k_solve_slow_mixed
+    unsigned int* slow_count = nullptr;   // their number (zeroed by k_caps_mixed before the gather)
 };
 
 // Row capacities of the mixed lists (gather row gc, extras segment ec): uniform (mode 0), from
@@ -1192,9 +1194,11 @@ struct MixArgs {
 // million outgrowing it, which repeats the whole pass, stays rare.
 __global__ void k_caps_mixed(const int32_t* __restrict__ hint, const int32_t* __restrict__ rowcnt,
                              const unsigned int* __restrict__ efill, int64_t n, int mode, int64_t cap_u,
-                             int64_t ecap_u, unsigned long long* __restrict__ gc, unsigned long long* __restrict__ ec)
+                             int64_t ecap_u, unsigned long long* __restrict__ gc, unsigned long long* __restrict__ ec,
+                             unsigned int* __restrict__ zero1)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i == 0 && zero1) *zero1 = 0u;  // the gather's slow-row counter
     if (i > n) return;
     unsigned long long g = 0, e = 0;
     if (i < n) {
@@ -1511,6 +1515,18 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const Gr
             A.flags[i] = 0;
         }
     }
+    if (A.slow_list) {
+        // the slow rows to the work list of k_solve_slow_mixed, one atomic per wavefront
+        const bool listed = live && glane == 0 && !fast;
+        const unsigned long long lb = __ballot(listed);
+        if (lb) {
+            const int lane = threadIdx.x & 63;
+            unsigned int base = 0;
+            if (lane == 0) base = atomicAdd(A.slow_count, (unsigned int)__popcll(lb));
+            base = __shfl(base, 0, 64);
+            if (listed) A.slow_list[base + (unsigned int)__popcll(lb & ((1ull << lane) - 1ull))] = (uint32_t)i;
+        }
+    }
 }
 
 // the two density sums of row i at inverse length inv_h, over the stored row (group-wide); d2 as
@@ -1541,60 +1557,65 @@ k_solve_slow_mixed(const double* __restrict__ h_search, int64_t n, const double4
                    const int32_t* __restrict__ rowcnt, MixArgs A)
 {
     const int glane = threadIdx.x & (kG - 1);
-    const int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kG;
-    if (i >= n || !A.slow[i]) return;  // whole groups leave together
-    // the stored row (shorter than rowcnt only after an overflow the caller is told about)
-    const int32_t m = (int32_t)min((unsigned long long)rowcnt[i], goff[i + 1] - goff[i]);
-    const double4 pi = posh[i];
-    const uint32_t* irow = ridx + goff[i];
-    const float* mf = kMasses ? rmass : nullptr;
-    const float mu = (float)(A.m_uniform > 0.0 ? A.m_uniform : 1.0);
-    const double mscale = kMasses ? 1.0 : (A.m_uniform > 0.0 ? A.m_uniform / (double)mu : 1.0);
-    const double m_i = A.mass[i];
-    const double r_lim = pymin(h_search[i], A.h_max);
-    double hi = r_lim, lo = 0.0;
-    double hh = pymin(A.h_start[i], hi);
-    const double h0 = hh;
-    int flag = 3, it = 0;
-    const double eta3 = ipow3(A.eta);
-    double s1, s2;
-    while (it < 50) {
-        it++;
+    // the slow rows listed by the gather, spread over the launch's lane groups (whole groups
+    // take the same rows, so the group shuffles see all their lanes)
+    const int64_t groups = (int64_t)gridDim.x * blockDim.x / kG;
+    const int64_t cnt = (int64_t)*A.slow_count < n ? (int64_t)*A.slow_count : n;
+    for (int64_t t = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kG; t < cnt; t += groups) {
+        const int64_t i = A.slow_list[t];
+        // the stored row (shorter than rowcnt only after an overflow the caller is told about)
+        const int32_t m = (int32_t)min((unsigned long long)rowcnt[i], goff[i + 1] - goff[i]);
+        const double4 pi = posh[i];
+        const uint32_t* irow = ridx + goff[i];
+        const float* mf = kMasses ? rmass : nullptr;
+        const float mu = (float)(A.m_uniform > 0.0 ? A.m_uniform : 1.0);
+        const double mscale = kMasses ? 1.0 : (A.m_uniform > 0.0 ? A.m_uniform / (double)mu : 1.0);
+        const double m_i = A.mass[i];
+        const double r_lim = pymin(h_search[i], A.h_max);
+        double hi = r_lim, lo = 0.0;
+        double hh = pymin(A.h_start[i], hi);
+        const double h0 = hh;
+        int flag = 3, it = 0;
+        const double eta3 = ipow3(A.eta);
+        double s1, s2;
+        while (it < 50) {
+            it++;
+            row_dens_sums(posh, pi, irow, mf, m, glane, (float)(1.0 / hh), A.kid, mu, s1, s2);
+            const double inv_h = 1.0 / hh;
+            const double rs = s1 * mscale * A.cnorm * ipow3(inv_h);
+            const double dws = -s2 * mscale * A.cnorm * ipow4(inv_h);
+            const double f = rs - m_i * ipow3(A.eta / hh);
+            const double df = dws + 3.0 * m_i * eta3 / ipow4(hh);
+            if (f > 0) hi = hh;
+            else lo = hh;
+            double h_new = df > 0 ? hh - f / df : 0.5 * (lo + hi);
+            if (!(lo <= h_new && h_new <= hi)) h_new = 0.5 * (lo + hi);
+            if (fabs(h_new - hh) < A.tol * h0) {
+                hh = h_new;
+                flag = 0;
+                break;
+            }
+            hh = h_new;
+        }
+        if (hh >= r_lim * (1 - 1e-3)) {
+            row_dens_sums(posh, pi, irow, mf, m, glane, (float)(1.0 / r_lim), A.kid, mu, s1, s2);
+            const double rs_lim = s1 * mscale * A.cnorm * ipow3(1.0 / r_lim);
+            if (rs_lim - m_i * ipow3(A.eta / r_lim) < 0) {
+                hh = r_lim;
+                flag = r_lim >= A.h_max ? 1 : 2;
+            }
+        }
         row_dens_sums(posh, pi, irow, mf, m, glane, (float)(1.0 / hh), A.kid, mu, s1, s2);
         const double inv_h = 1.0 / hh;
         const double rs = s1 * mscale * A.cnorm * ipow3(inv_h);
         const double dws = -s2 * mscale * A.cnorm * ipow4(inv_h);
-        const double f = rs - m_i * ipow3(A.eta / hh);
-        const double df = dws + 3.0 * m_i * eta3 / ipow4(hh);
-        if (f > 0) hi = hh;
-        else lo = hh;
-        double h_new = df > 0 ? hh - f / df : 0.5 * (lo + hi);
-        if (!(lo <= h_new && h_new <= hi)) h_new = 0.5 * (lo + hi);
-        if (fabs(h_new - hh) < A.tol * h0) {
-            hh = h_new;
-            flag = 0;
-            break;
+        if (glane == 0) {
+            A.h_out[i] = hh;
+            A.rho[i] = rs;
+            A.omega[i] = flag == 1 ? 1.0 : 1.0 + hh / (3.0 * rs) * dws;
+            A.iters[i] = it;
+            A.flags[i] = flag;
         }
-        hh = h_new;
-    }
-    if (hh >= r_lim * (1 - 1e-3)) {
-        row_dens_sums(posh, pi, irow, mf, m, glane, (float)(1.0 / r_lim), A.kid, mu, s1, s2);
-        const double rs_lim = s1 * mscale * A.cnorm * ipow3(1.0 / r_lim);
-        if (rs_lim - m_i * ipow3(A.eta / r_lim) < 0) {
-            hh = r_lim;
-            flag = r_lim >= A.h_max ? 1 : 2;
-        }
-    }
-    row_dens_sums(posh, pi, irow, mf, m, glane, (float)(1.0 / hh), A.kid, mu, s1, s2);
-    const double inv_h = 1.0 / hh;
-    const double rs = s1 * mscale * A.cnorm * ipow3(inv_h);
-    const double dws = -s2 * mscale * A.cnorm * ipow4(inv_h);
-    if (glane == 0) {
-        A.h_out[i] = hh;
-        A.rho[i] = rs;
-        A.omega[i] = flag == 1 ? 1.0 : 1.0 + hh / (3.0 * rs) * dws;
-        A.iters[i] = it;
-        A.flags[i] = flag;
     }
 }
 
@@ -2316,6 +2337,10 @@ k_emit64(const double4* __restrict__ posh, const uint32_t* __restrict__ ridx,
 }
 
 static inline unsigned grid_of(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+// workgroups of k_solve_slow_mixed: it walks the gather's list of slow rows, so a few hundred
+// workgroups cover any count (the launch over every row cost 33 us at 1M for 0.5 % of them)
+static inline unsigned slow_grid(int64_t blocks) { return (unsigned)(blocks < 512 ? blocks : 512); }
 
 // ------------------------------------------------------------------ fixed-capacity passes
 __global__ void k_fill_i64(long long* __restrict__ p, long long v, int64_t n)
@@ -3392,6 +3417,9 @@ extern "C" int32_t nb_hydro_density_mixed(nb_ctx* ctx, void* stream, const doubl
     long long sc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     MixArgs A{mass, m_uniform, h_start, h_max, kid, cnorm, eta, tol, h_out, rho, omega, iters, flags, ctx->pending,
               ctx->part};
+    // the slow rows' work list: the extras sort's list buffer, free until that sort
+    A.slow_list = ctx->xlist;
+    A.slow_count = (unsigned int*)(ctx->xlist + 2 * n + 1);
     const int tile = ctx->tile;  // workgroup size of the lane-group kernels (autotuned: 64, 128 or 256)
     const int64_t blocks = (n * kG + tile - 1) / tile;
     const GridSeg* segs = (const GridSeg*)ctx->segs;
@@ -3401,7 +3429,7 @@ extern "C" int32_t nb_hydro_density_mixed(nb_ctx* ctx, void* stream, const doubl
     unsigned long long tot[2] = {0, 0};
     for (int attempt = 0;; attempt++) {
         hipLaunchKernelGGL(k_caps_mixed, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, row_hint, ctx->rowcnt, ctx->efill, n,
-                           mode, cap_u, ecap_u, ctx->rm_gc, ctx->rm_ec);
+                           mode, cap_u, ecap_u, ctx->rm_gc, ctx->rm_ec, A.slow_count);
         rc = scan_u64(ctx, s, ctx->rm_gc, ctx->rm_go, n + 1);
         if (rc) return rc;
         rc = scan_u64(ctx, s, ctx->rm_ec, ctx->rm_eo, n + 1);
@@ -3452,10 +3480,10 @@ extern "C" int32_t nb_hydro_density_mixed(nb_ctx* ctx, void* stream, const doubl
     float* rmass = masses ? ctx->fscratch : nullptr;
     if (slow_rows > 0) {
         if (masses)
-            NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<true>, dim3(blocks), dim3(tile), 0, s, h_search, n,
+            NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<true>, dim3(slow_grid(blocks)), dim3(tile), 0, s, h_search, n,
                      (const double4*)ctx->posh, ctx->gbuf, rmass, ctx->rm_go, ctx->rowcnt, A);
         else
-            NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<false>, dim3(blocks), dim3(tile), 0, s, h_search,
+            NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<false>, dim3(slow_grid(blocks)), dim3(tile), 0, s, h_search,
                      n, (const double4*)ctx->posh, ctx->gbuf, rmass, ctx->rm_go, ctx->rowcnt, A);
     }
     if (ctx_reserve(ctx, (void**)&ctx->ebuf_s, &ctx->ebuf_s_bytes, sizeof(uint32_t) * (size_t)tot[1]))
@@ -3568,7 +3596,7 @@ extern "C" int32_t nb_hydro_density64(nb_ctx* ctx, void* stream, const double* p
     unsigned long long tot[2] = {0, 0};
     for (int attempt = 0;; attempt++) {
         hipLaunchKernelGGL(k_caps_mixed, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, row_hint, ctx->rowcnt, ctx->efill, n,
-                           mode, cap_u, ecap_u, ctx->rm_gc, ctx->rm_ec);
+                           mode, cap_u, ecap_u, ctx->rm_gc, ctx->rm_ec, (unsigned int*)nullptr);
         rc = scan_u64(ctx, s, ctx->rm_gc, ctx->rm_go, n + 1);
         if (rc) return rc;
         rc = scan_u64(ctx, s, ctx->rm_ec, ctx->rm_eo, n + 1);
@@ -4035,8 +4063,11 @@ extern "C" int32_t nb_hydro_density_mixed_fixed(nb_ctx* ctx, void* stream, const
     if (rc) return rc;
     NB_TIMED(ctx, s, "k_pack_posh", k_pack_posh, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, h_search, n,
              (double4*)ctx->posh);
+    // the slow rows' work list: the extras sort's list buffer, free until that sort
+    uint32_t* slow_list = ctx->xlist;
+    unsigned int* slow_count = (unsigned int*)(ctx->xlist + 2 * n + 1);
     hipLaunchKernelGGL(k_caps_mixed, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, row_hint, ctx->rowcnt, ctx->efill, n, 1,
-                       (int64_t)0, (int64_t)0, ctx->rm_gc, ctx->rm_ec);
+                       (int64_t)0, (int64_t)0, ctx->rm_gc, ctx->rm_ec, slow_count);
     {
         const Scan2 S{{(const unsigned long long*)ctx->rm_gc, (const unsigned long long*)ctx->rm_ec},
                       {ctx->rm_go, ctx->rm_eo},
@@ -4048,6 +4079,8 @@ extern "C" int32_t nb_hydro_density_mixed_fixed(nb_ctx* ctx, void* stream, const
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_of(n)), dim3(kBlock), 0, s, (uint32_t*)ctx->efill, 0u, n);
     MixArgs A{mass, m_uniform, h_start, h_max, kid, cnorm, eta, tol, h_out, rho, omega, iters, flags, ctx->pending,
               ctx->part};
+    A.slow_list = slow_list;
+    A.slow_count = slow_count;
     const int tile = ctx->tile;
     const int64_t blocks = (n * kG + tile - 1) / tile;
     const GridSeg* segs = (const GridSeg*)ctx->segs;
@@ -4067,10 +4100,10 @@ extern "C" int32_t nb_hydro_density_mixed_fixed(nb_ctx* ctx, void* stream, const
     hipLaunchKernelGGL(k_status_from, dim3(1), dim3(64), 0, s, (const long long*)ctx->scal, 2, status, (int)kStExtras);
     hipLaunchKernelGGL(k_status_from, dim3(1), dim3(64), 0, s, (const long long*)ctx->scal, 3, status, (int)kStGather);
     if (masses)
-        NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<true>, dim3(blocks), dim3(tile), 0, s, h_search, n,
+        NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<true>, dim3(slow_grid(blocks)), dim3(tile), 0, s, h_search, n,
                  (const double4*)ctx->posh, ctx->gbuf, rmass, ctx->rm_go, ctx->rowcnt, A);
     else
-        NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<false>, dim3(blocks), dim3(tile), 0, s, h_search, n,
+        NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<false>, dim3(slow_grid(blocks)), dim3(tile), 0, s, h_search, n,
                  (const double4*)ctx->posh, ctx->gbuf, rmass, ctx->rm_go, ctx->rowcnt, A);
     hipLaunchKernelGGL(k_status_grow, dim3(grid_of(n)), dim3(kBlock), 0, s, flags, n, status);
     sort_extras(ctx, s, n, (unsigned)blocks, tile);

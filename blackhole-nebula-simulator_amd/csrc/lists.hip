@@ -1558,10 +1558,14 @@ k_solve_slow_mixed(const double* __restrict__ h_search, int64_t n, const double4
 {
     const int glane = threadIdx.x & (kG - 1);
     // the slow rows listed by the gather, spread over the launch's lane groups (whole groups
-    // take the same rows, so the group shuffles see all their lanes)
+    // take the same rows, so the group shuffles see all their lanes). Neighbouring list entries
+    // (neighbouring particles, often slow together) go to different workgroups first, and to
+    // the groups of one wavefront last, which would wait for the slowest of them.
     const int64_t groups = (int64_t)gridDim.x * blockDim.x / kG;
     const int64_t cnt = (int64_t)*A.slow_count < n ? (int64_t)*A.slow_count : n;
-    for (int64_t t = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kG; t < cnt; t += groups) {
+    const int64_t wpb = blockDim.x / 64;                   // wavefronts per workgroup
+    const int64_t slot = (threadIdx.x >> 6) + wpb * ((threadIdx.x & 63) / kG);  // 0 .. groups per workgroup
+    for (int64_t t = blockIdx.x + (int64_t)gridDim.x * slot; t < cnt; t += groups) {
         const int64_t i = A.slow_list[t];
         // the stored row (shorter than rowcnt only after an overflow the caller is told about)
         const int32_t m = (int32_t)min((unsigned long long)rowcnt[i], goff[i + 1] - goff[i]);

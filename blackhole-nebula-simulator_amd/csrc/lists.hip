@@ -671,10 +671,13 @@ __global__ void k_permute_i64(const int64_t* __restrict__ src, const int32_t* __
 static constexpr int kMaxPermute = 16;
 struct PermSet {
     const uint64_t* src[kMaxPermute];
+    const uint64_t* src2[kMaxPermute];  // NULL: a copy; else a kick, dst = src + s src2 (doubles)
     uint64_t* dst[kMaxPermute];
     int32_t width[kMaxPermute];
     int32_t count;
-    int32_t total;  // sum of widths
+    int32_t total;       // sum of widths
+    const double* s_dev;  // the kicks' s = scale * s_dev[0] (a device dt)
+    double scale;
 };
 
 // Work-item t writes element t of the concatenated destinations (array after array), so each
@@ -692,7 +695,15 @@ __global__ void k_permute_many(PermSet P, const int32_t* __restrict__ order, int
     const int w = P.width[k];
     const int64_t i = w == 1 ? t : t / w;
     const int c = (int)(t - i * w);
-    P.dst[k][t] = P.src[k][(int64_t)order[i] * w + c];
+    const int64_t from = (int64_t)order[i] * w + c;
+    if (!P.src2[k]) {
+        P.dst[k][t] = P.src[k][from];
+    } else {  // the kick k_axpy_dev forms (y + s a, s = scale s_dev[0]) of the moved element
+        const double s = P.scale * P.s_dev[0];
+        const double y = __longlong_as_double((long long)P.src[k][from]);
+        const double a = __longlong_as_double((long long)P.src2[k][from]);
+        P.dst[k][t] = (uint64_t)__double_as_longlong(y + s * a);
+    }
 }
 
 // ----------------------------------------------------------------- fused step passes
@@ -3213,6 +3224,35 @@ extern "C" int32_t nb_permute_many(nb_ctx* ctx, void* stream, int32_t count, con
     if (n == 0 || count == 0) return NB_OK;
     hipLaunchKernelGGL(k_permute_many, dim3(grid_of(n * P.total)), dim3(kBlock), 0, (hipStream_t)stream, P, order, n);
     if (hipGetLastError() != hipSuccess) return ctx_fail(ctx, "nb_permute_many: kernel launch failed");
+    return NB_OK;
+}
+
+// nb_permute_many where some destinations are kicks of moved elements: src2[k] non-NULL makes
+// dst[k] = src[k] + s src2[k] (doubles, s = scale * s_dev[0]), in the new order
+extern "C" int32_t nb_permute_kick_many(nb_ctx* ctx, void* stream, int32_t count, const void* const* src,
+                                        const void* const* src2, void* const* dst, const int32_t* width,
+                                        const double* s_dev, double scale, const int32_t* order, int64_t n)
+{
+    if (!ctx || count < 0 || count > kMaxPermute || n < 0 || !order || !s_dev ||
+        (count && (!src || !src2 || !dst || !width)))
+        return ctx_fail(ctx, "nb_permute_kick_many: bad arguments");
+    PermSet P{};
+    P.count = count;
+    P.total = 0;
+    P.s_dev = s_dev;
+    P.scale = scale;
+    for (int k = 0; k < count; k++) {
+        if (!src[k] || !dst[k] || src[k] == dst[k] || src2[k] == dst[k] || width[k] <= 0)
+            return ctx_fail(ctx, "nb_permute_kick_many: bad array");
+        P.src[k] = (const uint64_t*)src[k];
+        P.src2[k] = (const uint64_t*)src2[k];
+        P.dst[k] = (uint64_t*)dst[k];
+        P.width[k] = width[k];
+        P.total += width[k];
+    }
+    if (n == 0 || count == 0) return NB_OK;
+    hipLaunchKernelGGL(k_permute_many, dim3(grid_of(n * P.total)), dim3(kBlock), 0, (hipStream_t)stream, P, order, n);
+    if (hipGetLastError() != hipSuccess) return ctx_fail(ctx, "nb_permute_kick_many: kernel launch failed");
     return NB_OK;
 }
 

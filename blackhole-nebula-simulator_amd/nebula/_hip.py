@@ -165,6 +165,7 @@ SIGNATURES = {
     "nb_mlp_grad": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp]),
     "nb_mlp_train_step": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _i64, ctypes.c_float, _vp]),
     "nb_permute_many": (_i32, [_vp, _vp, _i32, _vp, _vp, _vp, _vp, _i64]),
+    "nb_permute_kick_many": (_i32, [_vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _f64, _vp, _i64]),
     "nb_timing_report": (_i64, [_vp, ctypes.c_char_p, _i64]),
     "nb_gradients_mixed": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _PL, _PD, _vp, _i64, _i32, _f64, _vp, _vp,
                                   _vp]),

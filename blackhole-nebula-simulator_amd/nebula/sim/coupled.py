@@ -512,8 +512,13 @@ def step(st: GasState, cfg: CoupledConfig, dt: float, fixed=None) -> GasState:
                           float(u_floor), k_cool, _hip.ptr(dts), n, _hip.ptr(u_cool), _hip.ptr(e_part))  # fmt: skip
             _hip.call("nb_sum_into", _hip.ptr(e_part), n, _hip.ptr(e_rad))
             u_half = u_cool
-    p_pred, u_pred = kick2(p_flow, st.a_hydro, u_half, st.dudt)
     order = grid_permutation(cfg, x1, st.h, st, fixed)
+    if dts is None:
+        p_pred, u_pred = kick2(p_flow, st.a_hydro, u_half, st.dudt)
+        kicks = None
+    else:  # the predictor kick of a device dt is formed by the permutation, of the moved values
+        p_pred = u_pred = None
+        kicks = {2: (p_flow, st.a_hydro), 4: (u_half, st.dudt)}
     fields = [x1, p_flow, p_pred, u_half, u_pred, st.a_hydro, st.mass, st.h, st.alpha, st.perm]
     counts = st.row_counts
     if counts is not None:
@@ -522,7 +527,7 @@ def step(st: GasState, cfg: CoupledConfig, dt: float, fixed=None) -> GasState:
     if model is not None and (use_net or model.collect is not None):
         # the learned start's inputs from the last hydro pass, in the new order
         extra = [t for t in (st.rho, st.div_v, st.curl_v, st.nneigh)]
-    moved = permute_many(fields + [t for t in extra if t is not None], order)
+    moved = permute_many(fields + [t for t in extra if t is not None], order, kicks=kicks, s_dev=dts, scale=0.5)
     x1, p_flow, p_pred, u_half, u_pred, a_prev, mass, h, alpha, perm = moved[:10]
     if counts is not None:
         counts = moved[10].view(torch.int32).view(n, 2)

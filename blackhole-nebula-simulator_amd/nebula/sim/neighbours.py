@@ -257,20 +257,33 @@ def permute(t: torch.Tensor, order: torch.Tensor) -> torch.Tensor:
     return out
 
 
-def permute_many(tensors, order: torch.Tensor) -> list:
-    """permute() of several float64 / int64 tensors with one kernel launch."""
+def permute_many(tensors, order: torch.Tensor, kicks=None, s_dev=None, scale: float = 0.5) -> list:
+    """permute() of several float64 / int64 tensors with one kernel launch. kicks: {position:
+    (y, a)} of float64 pairs whose moved kick y + s a (s = scale * s_dev[0], read on the device)
+    takes that position of the result (its entry in ``tensors`` is None)."""
     import ctypes
 
     n = int(order.numel())
-    outs = [torch.empty_like(t) for t in tensors]
-    k = len(tensors)
-    src = (ctypes.c_void_p * k)(*[t.data_ptr() for t in tensors])
-    dst = (ctypes.c_void_p * k)(*[o.data_ptr() for o in outs])
-    width = (ctypes.c_int32 * k)(*[int(t.numel() // max(n, 1)) for t in tensors])
-    for t in tensors:
+    kicks = kicks or {}
+    srcs = [kicks[k][0] if k in kicks else t for k, t in enumerate(tensors)]
+    for t in srcs:
         if t.dtype not in (torch.float64, torch.int64) or not t.is_contiguous() or t.shape[0] != n:
             raise ValueError("permute_many takes contiguous float64 / int64 tensors of one length")
-    _hip.call("nb_permute_many", k, src, dst, width, _hip.ptr(order), n)
+    for y, a in kicks.values():
+        if y.dtype != torch.float64 or a.dtype != torch.float64 or a.shape != y.shape or not a.is_contiguous():
+            raise ValueError("a kick pairs two contiguous float64 tensors of one shape")
+    outs = [torch.empty_like(t) for t in srcs]
+    k = len(srcs)
+    src = (ctypes.c_void_p * k)(*[t.data_ptr() for t in srcs])
+    dst = (ctypes.c_void_p * k)(*[o.data_ptr() for o in outs])
+    width = (ctypes.c_int32 * k)(*[int(t.numel() // max(n, 1)) for t in srcs])
+    if not kicks:
+        _hip.call("nb_permute_many", k, src, dst, width, _hip.ptr(order), n)
+        return outs
+    if s_dev is None:
+        raise ValueError("kicks in the permutation take their step from the device (s_dev)")
+    src2 = (ctypes.c_void_p * k)(*[kicks[j][1].data_ptr() if j in kicks else None for j in range(k)])
+    _hip.call("nb_permute_kick_many", k, src, src2, dst, width, _hip.ptr(s_dev), float(scale), _hip.ptr(order), n)
     return outs
 
 

@@ -143,6 +143,13 @@ int32_t nb_permute_i64(nb_ctx *ctx, void *stream, const int64_t *src, const int3
  * = src[k] row order[i], rows of width[k] values (src, dst, width: host arrays of count). */
 int32_t nb_permute_many(nb_ctx *ctx, void *stream, int32_t count, const void *const *src, void *const *dst,
                         const int32_t *width, const int32_t *order, int64_t n);
+/* nb_permute_many where some destinations are kicks of the moved elements (no counterpart in
+ * the reference: coupled.py:231-233's predictor kick fused into the step's reordering):
+ * src2[k] (device, may be NULL per array) makes dst[k] = src[k] + s src2[k] as doubles, with
+ * s = scale * s_dev[0] read on the device; the values of nb_axpy_dev followed by the move. */
+int32_t nb_permute_kick_many(nb_ctx *ctx, void *stream, int32_t count, const void *const *src, const void *const *src2,
+                             void *const *dst, const int32_t *width, const double *s_dev, double scale,
+                             const int32_t *order, int64_t n);
 
 /* Reuse of cached neighbour lists (sph.py:62-147, NeighbourCache / _reuse_limits /
  * _fast_rows_complete): the exact test that lists built at x_ref with search lengths h_search

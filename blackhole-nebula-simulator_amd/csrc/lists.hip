@@ -342,8 +342,14 @@ k_cell_id(const double* __restrict__ pos, int64_t n, Box box, Grid g, uint32_t* 
 // Sum (bit s of maxmask clear) or max (set) of the per-workgroup partials of nslots slots
 // (slot s at part[s * nb + b]) into out[s] (zeroed beforehand): each workgroup reduces a strided
 // share in LDS and adds its result with one atomic per slot (a few hundred at most).
+// status words that a reduced slot's nonzero total sets (-1: none), for the fixed passes
+struct SlotStatus {
+    int word[5];
+};
+
 __global__ void __launch_bounds__(kBlock)
-k_partials_final(const long long* __restrict__ part, int64_t nb, int nslots, unsigned maxmask, long long* out)
+k_partials_final(const long long* __restrict__ part, int64_t nb, int nslots, unsigned maxmask, long long* out,
+                 int32_t* __restrict__ status, SlotStatus ss)
 {
     __shared__ long long sh[kBlock];
     for (int sl = 0; sl < nslots; sl++) {
@@ -365,6 +371,9 @@ k_partials_final(const long long* __restrict__ part, int64_t nb, int nslots, uns
         if (threadIdx.x == 0) {
             if (mx) atomicMax(&out[sl], sh[0]);
             else atomicAdd((unsigned long long*)&out[sl], (unsigned long long)sh[0]);
+            // the slots hold counts and maxima (never negative): the total is nonzero exactly
+            // when some workgroup's part is, and every such workgroup stores the same 1
+            if (status && sl < 5 && ss.word[sl] >= 0 && sh[0] != 0) status[ss.word[sl]] = 1;
         }
         __syncthreads();
     }
@@ -2611,12 +2620,6 @@ __global__ void k_plan_seg(const double* __restrict__ bb, const int64_t* __restr
     status[kStCells] = 1;
 }
 
-// status[word] = 1 when the collected scalar is nonzero
-__global__ void k_status_from(const long long* __restrict__ scal, int at, int32_t* __restrict__ status, int word)
-{
-    if (threadIdx.x == 0 && scal[at] != 0) status[word] = 1;
-}
-
 // rows whose h-rho root lies beyond their search (flag 2): the caller redoes the step eagerly
 __global__ void k_status_grow(const int32_t* __restrict__ flags, int64_t n, int32_t* __restrict__ status)
 {
@@ -2694,13 +2697,14 @@ int reserve_reductions(nb_ctx* ctx, int64_t n)
     return NB_OK;
 }
 
-inline void finish_partials(nb_ctx* ctx, hipStream_t s, int64_t nb, int nslots, unsigned maxmask, int at)
+inline void finish_partials(nb_ctx* ctx, hipStream_t s, int64_t nb, int nslots, unsigned maxmask, int at,
+                            int32_t* status = nullptr, SlotStatus ss = SlotStatus{{-1, -1, -1, -1, -1}})
 {
     long long* out = (long long*)ctx->scal + at;
     hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(64), 0, s, out, 0ll, (int64_t)nslots);  // a kernel: graph-safe
     const int64_t blocks = std::min<int64_t>(256, (nb + 4 * kBlock - 1) / (4 * kBlock));
     hipLaunchKernelGGL(k_partials_final, dim3((unsigned)blocks), dim3(kBlock), 0, s, ctx->part, nb, nslots, maxmask,
-                       out);
+                       out, status, ss);
 }
 
 // The particle order of the last grid for the gather kernels: NULL when the particles were stored
@@ -3810,8 +3814,9 @@ int scan2_fixed(nb_ctx* ctx, hipStream_t s, const Scan2& S, int64_t count, int32
 }
 
 // stable LSD radix sort of (ctx->cell_id, ctx->iota) by the key's low fx_bits bits, values into
-// ctx->order (keys and ping-pong buffers clobbered); *sorted_keys: where the sorted keys are
-int sort_fixed(nb_ctx* ctx, hipStream_t s, int64_t n, const uint32_t** sorted_keys)
+// vout (ctx->order, or the caller's order array; keys and ping-pong buffers clobbered);
+// *sorted_keys: where the sorted keys are
+int sort_fixed(nb_ctx* ctx, hipStream_t s, int64_t n, const uint32_t** sorted_keys, uint32_t* vout)
 {
     if (!capturing(s)) {
         // not being recorded: the library's (also stable) radix sort over the same bits
@@ -3819,7 +3824,7 @@ int sort_fixed(nb_ctx* ctx, hipStream_t s, int64_t n, const uint32_t** sorted_ke
         if (sort_pairs(nullptr, bytes, ctx->cell_id, ctx->keys_out, ctx->iota, ctx->order, n, ctx->fx_bits, s) != hipSuccess)
             return ctx_fail(ctx, "sort sizing failed");
         if (ctx_reserve(ctx, (void**)&ctx->cub_tmp, &ctx->cub_tmp_bytes, bytes)) return NB_ERR_HIP;
-        if (sort_pairs(ctx->cub_tmp, bytes, ctx->cell_id, ctx->keys_out, ctx->iota, ctx->order, n, ctx->fx_bits, s) != hipSuccess)
+        if (sort_pairs(ctx->cub_tmp, bytes, ctx->cell_id, ctx->keys_out, ctx->iota, vout, n, ctx->fx_bits, s) != hipSuccess)
             return ctx_fail(ctx, "sort failed");
         *sorted_keys = ctx->keys_out;
         return NB_OK;
@@ -3847,8 +3852,8 @@ int sort_fixed(nb_ctx* ctx, hipStream_t s, int64_t n, const uint32_t** sorted_ke
         std::swap(ka, kb);
         std::swap(va, vb);
     }
-    if (va != ctx->order)  // an even number of passes ends in the iota buffer
-        hipLaunchKernelGGL(k_copy_u32, dim3(grid_of(n)), dim3(kBlock), 0, s, va, ctx->order, n);
+    if (va != vout)  // the passes end in the iota or the order buffer
+        hipLaunchKernelGGL(k_copy_u32, dim3(grid_of(n)), dim3(kBlock), 0, s, va, vout, n);
     *sorted_keys = ka;
     return NB_OK;
 }
@@ -3960,7 +3965,7 @@ int starts_from_keys(nb_ctx* ctx, hipStream_t s, const uint32_t* keys, int64_t n
 // permutation), which is checked (status[kStUnsorted]).
 int build_grid_seg_fixed(nb_ctx* ctx, hipStream_t s, const double* pos, const double* h, int64_t n,
                          const int32_t* cloud, const int64_t* off_dev, int32_t nseg, double cell_factor, bool sort,
-                         int32_t* status)
+                         int32_t* status, uint32_t* order_out = nullptr)
 {
     ctx->lists_ready = 0;
     ctx->hydro_ready = 0;
@@ -3972,8 +3977,7 @@ int build_grid_seg_fixed(nb_ctx* ctx, hipStream_t s, const double* pos, const do
         ctx->fx_grid_from_order = 0;
         hipLaunchKernelGGL(k_check_grid_seg, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, n, (const GridSeg*)ctx->segs,
                            cloud, (const uint32_t*)ctx->cell_start, ctx->part);
-        finish_partials(ctx, s, grid_of(n), 1, 0u, 0);
-        hipLaunchKernelGGL(k_status_from, dim3(1), dim3(64), 0, s, (const long long*)ctx->scal, 0, status, kStUnsorted);
+        finish_partials(ctx, s, grid_of(n), 1, 0u, 0, status, SlotStatus{{(int)kStUnsorted, -1, -1, -1, -1}});
         ctx->order_ident = 1;
         return NB_OK;
     }
@@ -3992,7 +3996,7 @@ int build_grid_seg_fixed(nb_ctx* ctx, hipStream_t s, const double* pos, const do
              cloud, ctx->cell_id, ctx->iota, from_keys ? nullptr : ctx->cell_count, sort ? nullptr : ctx->part);
     if (sort) {
         const uint32_t* skeys = nullptr;
-        int rc = sort_fixed(ctx, s, n, &skeys);
+        int rc = sort_fixed(ctx, s, n, &skeys, order_out ? order_out : ctx->order);
         if (rc) return rc;
         ctx->order_ident = 0;
         ctx->fx_grid_from_order = 1;  // the density pass after the permutation may reuse this grid
@@ -4000,8 +4004,7 @@ int build_grid_seg_fixed(nb_ctx* ctx, hipStream_t s, const double* pos, const do
             return starts_from_keys(ctx, s, skeys, n, nc1 - 1, (const GridSeg*)ctx->segs, nseg, ctx->cell_start,
                                     ctx->cell_count);
     } else {
-        finish_partials(ctx, s, grid_of(n), 1, 0u, 0);
-        hipLaunchKernelGGL(k_status_from, dim3(1), dim3(64), 0, s, (const long long*)ctx->scal, 0, status, kStUnsorted);
+        finish_partials(ctx, s, grid_of(n), 1, 0u, 0, status, SlotStatus{{(int)kStUnsorted, -1, -1, -1, -1}});
         ctx->order_ident = 1;
     }
     return scan_fixed(ctx, s, (const uint32_t*)ctx->cell_count, ctx->cell_start, nc1);
@@ -4080,11 +4083,9 @@ extern "C" int32_t nb_grid_order_seg_fixed(nb_ctx* ctx, void* stream, const doub
         return ctx_fail(ctx, "nb_grid_order_seg_fixed: bad arguments");
     int rc = fixed_check(ctx, n, nseg, "nb_grid_order_seg_fixed: capacities not reserved for this size");
     if (rc) return rc;
-    rc = build_grid_seg_fixed(ctx, s, pos, h, n, cloud, seg_off_dev, nseg, cell_factor, true, status);
-    if (rc) return rc;
-    if (hipMemcpyAsync(order_out, ctx->order, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s) != hipSuccess)
-        return ctx_fail(ctx, "nb_grid_order_seg_fixed: copy failed");
-    return NB_OK;
+    // the sort writes the order straight into the caller's array
+    return build_grid_seg_fixed(ctx, s, pos, h, n, cloud, seg_off_dev, nseg, cell_factor, true, status,
+                                (uint32_t*)order_out);
 }
 
 extern "C" int32_t nb_hydro_density_mixed_fixed(nb_ctx* ctx, void* stream, const double* pos, const double* mass,
@@ -4140,9 +4141,8 @@ extern "C" int32_t nb_hydro_density_mixed_fixed(nb_ctx* ctx, void* stream, const
     else NB_GATHER_FIXED(false, 1);
 #undef NB_GATHER_FIXED
     // [2] extras overflow, [3] rows over capacity
-    finish_partials(ctx, s, blocks, 5, 1u | 4u | 16u, 0);
-    hipLaunchKernelGGL(k_status_from, dim3(1), dim3(64), 0, s, (const long long*)ctx->scal, 2, status, (int)kStExtras);
-    hipLaunchKernelGGL(k_status_from, dim3(1), dim3(64), 0, s, (const long long*)ctx->scal, 3, status, (int)kStGather);
+    finish_partials(ctx, s, blocks, 5, 1u | 4u | 16u, 0, status,
+                    SlotStatus{{-1, -1, (int)kStExtras, (int)kStGather, -1}});
     if (masses)
         NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<true>, dim3(slow_grid(blocks)), dim3(tile), 0, s, h_search, n,
                  (const double4*)ctx->posh, ctx->gbuf, rmass, ctx->rm_go, ctx->rowcnt, A);

@@ -1202,8 +1202,6 @@ struct MixArgs {
     int32_t* flags;
     uint8_t* slow;    // 1: row solved by k_solve_slow_mixed (not converged in one Newton step)
     long long* part;  // per-workgroup partials: longest row, slow rows
-    uint32_t* slow_list = nullptr;        // the slow rows (in no particular order), for k_solve_slow_mixed
-    unsigned int* slow_count = nullptr;   // their number (zeroed by k_caps_mixed before the gather)
 };
 
 // Row capacities of the mixed lists (gather row gc, extras segment ec): uniform (mode 0), from
@@ -1214,11 +1212,9 @@ struct MixArgs {
 // million outgrowing it, which repeats the whole pass, stays rare.
 __global__ void k_caps_mixed(const int32_t* __restrict__ hint, const int32_t* __restrict__ rowcnt,
                              const unsigned int* __restrict__ efill, int64_t n, int mode, int64_t cap_u,
-                             int64_t ecap_u, unsigned long long* __restrict__ gc, unsigned long long* __restrict__ ec,
-                             unsigned int* __restrict__ zero1)
+                             int64_t ecap_u, unsigned long long* __restrict__ gc, unsigned long long* __restrict__ ec)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i == 0 && zero1) *zero1 = 0u;  // the gather's slow-row counter
     if (i > n) return;
     unsigned long long g = 0, e = 0;
     if (i < n) {
@@ -1535,18 +1531,6 @@ k_gather_mixed(const double4* __restrict__ posh, int64_t n, Grid g_one, const Gr
             A.flags[i] = 0;
         }
     }
-    if (A.slow_list) {
-        // the slow rows to the work list of k_solve_slow_mixed, one atomic per wavefront
-        const bool listed = live && glane == 0 && !fast;
-        const unsigned long long lb = __ballot(listed);
-        if (lb) {
-            const int lane = threadIdx.x & 63;
-            unsigned int base = 0;
-            if (lane == 0) base = atomicAdd(A.slow_count, (unsigned int)__popcll(lb));
-            base = __shfl(base, 0, 64);
-            if (listed) A.slow_list[base + (unsigned int)__popcll(lb & ((1ull << lane) - 1ull))] = (uint32_t)i;
-        }
-    }
 }
 
 // the two density sums of row i at inverse length inv_h, over the stored row (group-wide); d2 as
@@ -1568,25 +1552,54 @@ __device__ __forceinline__ void row_dens_sums(const double4* posh, const double4
     sums.total(s1, s2);
 }
 
+// position of the t-th set bit (t counted from 0) of a mask with more than t bits set: a binary
+// search on the popcounts of halves
+__device__ __forceinline__ int nth_set_bit(unsigned long long mask, int t)
+{
+    int pos = 0;
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) {
+        const unsigned long long lo = mask & ((1ull << w) - 1ull);
+        const int c = __popcll(lo);
+        if (t >= c) {
+            t -= c;
+            mask >>= w;
+            pos += w;
+        } else {
+            mask = lo;
+        }
+    }
+    return pos;
+}
+
 // Rows not converged in one Newton step: the whole solve of sph.py:251-297 over the stored row
 // (sums FP32 compensated, Newton steps and tests FP64), rho and Omega at the final h.
 template <bool kMasses>
 __global__ void __launch_bounds__(256)
 k_solve_slow_mixed(const double* __restrict__ h_search, int64_t n, const double4* __restrict__ posh,
                    const uint32_t* __restrict__ ridx, const float* __restrict__ rmass, const unsigned long long* __restrict__ goff,
-                   const int32_t* __restrict__ rowcnt, MixArgs A)
+                   const int32_t* __restrict__ rowcnt, MixArgs A, const long long* __restrict__ slow_total)
 {
     const int glane = threadIdx.x & (kG - 1);
-    // the slow rows listed by the gather, spread over the launch's lane groups (whole groups
-    // take the same rows, so the group shuffles see all their lanes). Neighbouring list entries
-    // (neighbouring particles, often slow together) go to different workgroups first, and to
-    // the groups of one wavefront last, which would wait for the slowest of them.
-    const int64_t groups = (int64_t)gridDim.x * blockDim.x / kG;
-    const int64_t cnt = (int64_t)*A.slow_count < n ? (int64_t)*A.slow_count : n;
-    const int64_t wpb = blockDim.x / 64;                   // wavefronts per workgroup
-    const int64_t slot = (threadIdx.x >> 6) + wpb * ((threadIdx.x & 63) / kG);  // 0 .. groups per workgroup
-    for (int64_t t = blockIdx.x + (int64_t)gridDim.x * slot; t < cnt; t += groups) {
-        const int64_t i = A.slow_list[t];
+    // Each wavefront takes 64 rows (one a lane), reads their slow flags into one ballot mask, and
+    // its 64 / kG lane groups walk the set bits in order, one row per group at a time (whole
+    // groups take the same row, so the group shuffles see all their lanes). No counters or
+    // atomics; a wavefront with no slow row exits. Many slow rows (a start far from the root):
+    // 64 consecutive rows a wavefront, in grid order, so the rows in flight at once share their
+    // neighbour records in L2. Few (slow rows come in clusters of neighbours): rows strided by
+    // the launch's wavefront count, so a cluster spreads over many wavefronts instead of
+    // queueing in one.
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const bool spread = *slow_total * 16 < n;
+    const int64_t w = spread ? (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6 : 0;
+    const int64_t r0 = spread ? w : xcd_tile() * (int64_t)blockDim.x + (threadIdx.x & ~63);
+    const int64_t step = spread ? waves : 1;
+    const int64_t mine = r0 + lane * step;
+    const unsigned long long mask = __ballot(mine < n && A.slow[mine] != 0);
+    const int nslow = __popcll(mask);
+    for (int t = lane / kG; t < nslow; t += 64 / kG) {
+        const int64_t i = r0 + nth_set_bit(mask, t) * step;
         // the stored row (shorter than rowcnt only after an overflow the caller is told about)
         const int32_t m = (int32_t)min((unsigned long long)rowcnt[i], goff[i + 1] - goff[i]);
         const double4 pi = posh[i];
@@ -2362,9 +2375,8 @@ k_emit64(const double4* __restrict__ posh, const uint32_t* __restrict__ ridx,
 
 static inline unsigned grid_of(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
-// workgroups of k_solve_slow_mixed: it walks the gather's list of slow rows, so a few hundred
-// workgroups cover any count (the launch over every row cost 33 us at 1M for 0.5 % of them)
-static inline unsigned slow_grid(int64_t blocks) { return (unsigned)(blocks < 512 ? blocks : 512); }
+// workgroups of k_solve_slow_mixed: one lane per row
+static inline unsigned slow_grid(int64_t n, int tile) { return (unsigned)((n + tile - 1) / tile); }
 
 // ------------------------------------------------------------------ fixed-capacity passes
 __global__ void k_fill_i64(long long* __restrict__ p, long long v, int64_t n)
@@ -3466,8 +3478,6 @@ extern "C" int32_t nb_hydro_density_mixed(nb_ctx* ctx, void* stream, const doubl
     MixArgs A{mass, m_uniform, h_start, h_max, kid, cnorm, eta, tol, h_out, rho, omega, iters, flags, ctx->pending,
               ctx->part};
     // the slow rows' work list: the extras sort's list buffer, free until that sort
-    A.slow_list = ctx->xlist;
-    A.slow_count = (unsigned int*)(ctx->xlist + 2 * n + 1);
     const int tile = ctx->tile;  // workgroup size of the lane-group kernels (autotuned: 64, 128 or 256)
     const int64_t blocks = (n * kG + tile - 1) / tile;
     const GridSeg* segs = (const GridSeg*)ctx->segs;
@@ -3477,7 +3487,7 @@ extern "C" int32_t nb_hydro_density_mixed(nb_ctx* ctx, void* stream, const doubl
     unsigned long long tot[2] = {0, 0};
     for (int attempt = 0;; attempt++) {
         hipLaunchKernelGGL(k_caps_mixed, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, row_hint, ctx->rowcnt, ctx->efill, n,
-                           mode, cap_u, ecap_u, ctx->rm_gc, ctx->rm_ec, A.slow_count);
+                           mode, cap_u, ecap_u, ctx->rm_gc, ctx->rm_ec);
         rc = scan_u64(ctx, s, ctx->rm_gc, ctx->rm_go, n + 1);
         if (rc) return rc;
         rc = scan_u64(ctx, s, ctx->rm_ec, ctx->rm_eo, n + 1);
@@ -3528,11 +3538,13 @@ extern "C" int32_t nb_hydro_density_mixed(nb_ctx* ctx, void* stream, const doubl
     float* rmass = masses ? ctx->fscratch : nullptr;
     if (slow_rows > 0) {
         if (masses)
-            NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<true>, dim3(slow_grid(blocks)), dim3(tile), 0, s, h_search, n,
-                     (const double4*)ctx->posh, ctx->gbuf, rmass, ctx->rm_go, ctx->rowcnt, A);
+            NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<true>, dim3(slow_grid(n, tile)), dim3(tile), 0, s, h_search, n,
+                     (const double4*)ctx->posh, ctx->gbuf, rmass, ctx->rm_go, ctx->rowcnt, A,
+                     (const long long*)ctx->scal + 1);  // the partials' slot 1: rows left slow
         else
-            NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<false>, dim3(slow_grid(blocks)), dim3(tile), 0, s, h_search,
-                     n, (const double4*)ctx->posh, ctx->gbuf, rmass, ctx->rm_go, ctx->rowcnt, A);
+            NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<false>, dim3(slow_grid(n, tile)), dim3(tile), 0, s, h_search,
+                     n, (const double4*)ctx->posh, ctx->gbuf, rmass, ctx->rm_go, ctx->rowcnt, A,
+                     (const long long*)ctx->scal + 1);  // the partials' slot 1: rows left slow
     }
     if (ctx_reserve(ctx, (void**)&ctx->ebuf_s, &ctx->ebuf_s_bytes, sizeof(uint32_t) * (size_t)tot[1]))
         return NB_ERR_HIP;
@@ -3644,7 +3656,7 @@ extern "C" int32_t nb_hydro_density64(nb_ctx* ctx, void* stream, const double* p
     unsigned long long tot[2] = {0, 0};
     for (int attempt = 0;; attempt++) {
         hipLaunchKernelGGL(k_caps_mixed, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, row_hint, ctx->rowcnt, ctx->efill, n,
-                           mode, cap_u, ecap_u, ctx->rm_gc, ctx->rm_ec, (unsigned int*)nullptr);
+                           mode, cap_u, ecap_u, ctx->rm_gc, ctx->rm_ec);
         rc = scan_u64(ctx, s, ctx->rm_gc, ctx->rm_go, n + 1);
         if (rc) return rc;
         rc = scan_u64(ctx, s, ctx->rm_ec, ctx->rm_eo, n + 1);
@@ -4108,11 +4120,8 @@ extern "C" int32_t nb_hydro_density_mixed_fixed(nb_ctx* ctx, void* stream, const
     if (rc) return rc;
     NB_TIMED(ctx, s, "k_pack_posh", k_pack_posh, dim3(grid_of(n)), dim3(kBlock), 0, s, pos, h_search, n,
              (double4*)ctx->posh);
-    // the slow rows' work list: the extras sort's list buffer, free until that sort
-    uint32_t* slow_list = ctx->xlist;
-    unsigned int* slow_count = (unsigned int*)(ctx->xlist + 2 * n + 1);
     hipLaunchKernelGGL(k_caps_mixed, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, row_hint, ctx->rowcnt, ctx->efill, n, 1,
-                       (int64_t)0, (int64_t)0, ctx->rm_gc, ctx->rm_ec, slow_count);
+                       (int64_t)0, (int64_t)0, ctx->rm_gc, ctx->rm_ec);
     {
         const Scan2 S{{(const unsigned long long*)ctx->rm_gc, (const unsigned long long*)ctx->rm_ec},
                       {ctx->rm_go, ctx->rm_eo},
@@ -4124,8 +4133,6 @@ extern "C" int32_t nb_hydro_density_mixed_fixed(nb_ctx* ctx, void* stream, const
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_of(n)), dim3(kBlock), 0, s, (uint32_t*)ctx->efill, 0u, n);
     MixArgs A{mass, m_uniform, h_start, h_max, kid, cnorm, eta, tol, h_out, rho, omega, iters, flags, ctx->pending,
               ctx->part};
-    A.slow_list = slow_list;
-    A.slow_count = slow_count;
     const int tile = ctx->tile;
     const int64_t blocks = (n * kG + tile - 1) / tile;
     const GridSeg* segs = (const GridSeg*)ctx->segs;
@@ -4144,11 +4151,13 @@ extern "C" int32_t nb_hydro_density_mixed_fixed(nb_ctx* ctx, void* stream, const
     finish_partials(ctx, s, blocks, 5, 1u | 4u | 16u, 0, status,
                     SlotStatus{{-1, -1, (int)kStExtras, (int)kStGather, -1}});
     if (masses)
-        NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<true>, dim3(slow_grid(blocks)), dim3(tile), 0, s, h_search, n,
-                 (const double4*)ctx->posh, ctx->gbuf, rmass, ctx->rm_go, ctx->rowcnt, A);
+        NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<true>, dim3(slow_grid(n, tile)), dim3(tile), 0, s, h_search, n,
+                 (const double4*)ctx->posh, ctx->gbuf, rmass, ctx->rm_go, ctx->rowcnt, A,
+                 (const long long*)ctx->scal + 1);  // the partials' slot 1: rows left slow
     else
-        NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<false>, dim3(slow_grid(blocks)), dim3(tile), 0, s, h_search, n,
-                 (const double4*)ctx->posh, ctx->gbuf, rmass, ctx->rm_go, ctx->rowcnt, A);
+        NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<false>, dim3(slow_grid(n, tile)), dim3(tile), 0, s, h_search, n,
+                 (const double4*)ctx->posh, ctx->gbuf, rmass, ctx->rm_go, ctx->rowcnt, A,
+                 (const long long*)ctx->scal + 1);  // the partials' slot 1: rows left slow
     hipLaunchKernelGGL(k_status_grow, dim3(grid_of(n)), dim3(kBlock), 0, s, flags, n, status);
     sort_extras(ctx, s, n, (unsigned)blocks, tile);
     NB_TIMED(ctx, s, "k_row_total_rm", k_row_total_rm, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, ctx->rowcnt,

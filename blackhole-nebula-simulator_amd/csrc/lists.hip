@@ -1578,7 +1578,8 @@ template <bool kMasses>
 __global__ void __launch_bounds__(256)
 k_solve_slow_mixed(const double* __restrict__ h_search, int64_t n, const double4* __restrict__ posh,
                    const uint32_t* __restrict__ ridx, const float* __restrict__ rmass, const unsigned long long* __restrict__ goff,
-                   const int32_t* __restrict__ rowcnt, MixArgs A, const long long* __restrict__ slow_total)
+                   const int32_t* __restrict__ rowcnt, MixArgs A, const long long* __restrict__ slow_total,
+                   int per_group)
 {
     const int glane = threadIdx.x & (kG - 1);
     // Each wavefront takes 64 rows (one a lane), reads their slow flags into one ballot mask, and
@@ -1588,18 +1589,23 @@ k_solve_slow_mixed(const double* __restrict__ h_search, int64_t n, const double4
     // 64 consecutive rows a wavefront, in grid order, so the rows in flight at once share their
     // neighbour records in L2. Few (slow rows come in clusters of neighbours): rows strided by
     // the launch's wavefront count, so a cluster spreads over many wavefronts instead of
-    // queueing in one.
+    // queueing in one. per_group (a launch of kG lanes per row, chosen by a host that knows
+    // the count is large): 64 / kG consecutive rows a wavefront, the first lane of each group
+    // voting, which keeps small systems with many slow rows parallel.
+    static_assert(kG == 16, "per-group rows: one vote each 16 lanes");
     const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
     const int lane = threadIdx.x & 63;
-    const bool spread = *slow_total * 16 < n;
+    const bool spread = !per_group && *slow_total * 16 < n;
+    const int sh = per_group ? 4 : 0;  // lanes per row: 1 << sh
     const int64_t w = spread ? (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6 : 0;
-    const int64_t r0 = spread ? w : xcd_tile() * (int64_t)blockDim.x + (threadIdx.x & ~63);
+    const int64_t r0 = spread ? w : (xcd_tile() * (int64_t)blockDim.x + (threadIdx.x & ~63)) >> sh;
     const int64_t step = spread ? waves : 1;
-    const int64_t mine = r0 + lane * step;
-    const unsigned long long mask = __ballot(mine < n && A.slow[mine] != 0);
+    const int64_t mine = r0 + (int64_t)(lane >> sh) * step;
+    const bool vote = (lane & ((1 << sh) - 1)) == 0 && mine < n && A.slow[mine] != 0;
+    const unsigned long long mask = __ballot(vote);
     const int nslow = __popcll(mask);
     for (int t = lane / kG; t < nslow; t += 64 / kG) {
-        const int64_t i = r0 + nth_set_bit(mask, t) * step;
+        const int64_t i = r0 + (int64_t)(nth_set_bit(mask, t) >> sh) * step;
         // the stored row (shorter than rowcnt only after an overflow the caller is told about)
         const int32_t m = (int32_t)min((unsigned long long)rowcnt[i], goff[i + 1] - goff[i]);
         const double4 pi = posh[i];
@@ -3535,16 +3541,19 @@ extern "C" int32_t nb_hydro_density_mixed(nb_ctx* ctx, void* stream, const doubl
     ctx->rm_gtotal = (int64_t)tot[0];
     ctx->rm_etotal = (int64_t)tot[1];
     const long long slow_rows = sc[1];
+    // many slow rows: one lane group per row (the kernel's per_group rows), else one lane per row
+    const int per_group = slow_rows * 16 >= n ? 1 : 0;
+    const unsigned slow_wgs = per_group ? (unsigned)blocks : slow_grid(n, tile);
     float* rmass = masses ? ctx->fscratch : nullptr;
     if (slow_rows > 0) {
         if (masses)
-            NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<true>, dim3(slow_grid(n, tile)), dim3(tile), 0, s, h_search, n,
+            NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<true>, dim3(slow_wgs), dim3(tile), 0, s, h_search, n,
                      (const double4*)ctx->posh, ctx->gbuf, rmass, ctx->rm_go, ctx->rowcnt, A,
-                     (const long long*)ctx->scal + 1);  // the partials' slot 1: rows left slow
+                     (const long long*)ctx->scal + 1, per_group);  // scal[1]: the partials' rows left slow
         else
-            NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<false>, dim3(slow_grid(n, tile)), dim3(tile), 0, s, h_search,
+            NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<false>, dim3(slow_wgs), dim3(tile), 0, s, h_search,
                      n, (const double4*)ctx->posh, ctx->gbuf, rmass, ctx->rm_go, ctx->rowcnt, A,
-                     (const long long*)ctx->scal + 1);  // the partials' slot 1: rows left slow
+                     (const long long*)ctx->scal + 1, per_group);  // scal[1]: the partials' rows left slow
     }
     if (ctx_reserve(ctx, (void**)&ctx->ebuf_s, &ctx->ebuf_s_bytes, sizeof(uint32_t) * (size_t)tot[1]))
         return NB_ERR_HIP;
@@ -4153,11 +4162,11 @@ extern "C" int32_t nb_hydro_density_mixed_fixed(nb_ctx* ctx, void* stream, const
     if (masses)
         NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<true>, dim3(slow_grid(n, tile)), dim3(tile), 0, s, h_search, n,
                  (const double4*)ctx->posh, ctx->gbuf, rmass, ctx->rm_go, ctx->rowcnt, A,
-                 (const long long*)ctx->scal + 1);  // the partials' slot 1: rows left slow
+                 (const long long*)ctx->scal + 1, 0);  // scal[1]: the partials' rows left slow
     else
         NB_TIMED(ctx, s, "k_solve_slow_mixed", k_solve_slow_mixed<false>, dim3(slow_grid(n, tile)), dim3(tile), 0, s, h_search, n,
                  (const double4*)ctx->posh, ctx->gbuf, rmass, ctx->rm_go, ctx->rowcnt, A,
-                 (const long long*)ctx->scal + 1);  // the partials' slot 1: rows left slow
+                 (const long long*)ctx->scal + 1, 0);  // scal[1]: the partials' rows left slow
     hipLaunchKernelGGL(k_status_grow, dim3(grid_of(n)), dim3(kBlock), 0, s, flags, n, status);
     sort_extras(ctx, s, n, (unsigned)blocks, tile);
     NB_TIMED(ctx, s, "k_row_total_rm", k_row_total_rm, dim3(grid_of(n + 1)), dim3(kBlock), 0, s, ctx->rowcnt,
